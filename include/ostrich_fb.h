@@ -1,0 +1,219 @@
+/*
+ * ostrich_fb.h -- C ABI of the MI355X (gfx950) Fisher-Burmeister Newton contact solver.
+ *
+ * This is the drop-in boundary for Ostrich's solver plugin
+ * (`OstrichEngine(SolverBase)`, src/ostrich/core/engine.py:14, constructed by
+ * `OstrichEngineConfig.create_engine`, src/ostrich/core/engine_config.py:527-557).
+ * The Python class `ostrich_amd.OstrichEngine` binds these entry points with ctypes;
+ * every pointer named "device" is a HIP device pointer owned by the caller and valid
+ * for the duration of the call.  Scratch memory is owned by the solver handle and is
+ * allocated in ofb_create only, so ofb_step allocates nothing and can be captured in a
+ * hipGraph.
+ *
+ * Conventions (identical to Newton's, as read by the reference):
+ *   transform   = px py pz qx qy qz qw            (7 x f32, quaternion xyzw)
+ *   spatial     = linear(3) angular(3)            (6 x f32, world frame;
+ *                 src/ostrich/constraints/dynamics_constraint.py:40)
+ *   body arrays are world-major: index = world * bodies_per_world + body
+ *   joint types: core/model.py:88-99; dof modes: core/types.py:4-18
+ *
+ * Return codes: 0 = ok, negative = invalid argument / shape / launch failure; the
+ * message is available from ofb_last_error().  Non-convergence is not an error: it is
+ * reported by ofb_get_stats (src/ostrich/core/base_engine.py:42-80 has no error path).
+ */
+#ifndef OSTRICH_FB_H
+#define OSTRICH_FB_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  OFB_JOINT_PRISMATIC = 0,
+  OFB_JOINT_REVOLUTE = 1,
+  OFB_JOINT_BALL = 2,
+  OFB_JOINT_FIXED = 3,
+  OFB_JOINT_FREE = 4,
+  OFB_JOINT_DISTANCE = 5,
+  OFB_JOINT_D6 = 6,
+  OFB_JOINT_CABLE = 7
+};
+
+enum { OFB_DOF_NONE = 0, OFB_DOF_TARGET_POSITION = 1, OFB_DOF_TARGET_VELOCITY = 2 };
+
+/* linear.preconditioner_type, core/engine_config.py:137 */
+enum { OFB_PRECOND_JACOBI = 0, OFB_PRECOND_PER_BODY_PAIR = 1 };
+
+/* relaxation.friction / relaxation.contact, core/engine_config.py:243-244 */
+enum { OFB_FRICTION_CONE = 0, OFB_FRICTION_BILATERAL = 1, OFB_FRICTION_BILATERAL_PATCH = 2 };
+enum { OFB_CONTACT_COMPLEMENTARITY = 0, OFB_CONTACT_EQUALITY = 1 };
+
+/* Termination of the Newton and PCR loops.
+ * GLOBAL:    the reference's batch-global stop flag (base_engine.py:42-80,
+ *            pcr_solver.py:84-117): every world iterates until all worlds converge.
+ * PER_WORLD: each world stops on its own criterion and is retired (fused path). */
+enum { OFB_TERM_GLOBAL = 0, OFB_TERM_PER_WORLD = 1 };
+
+/* Execution path. FUSED: one persistent launch, one wavefront per world, rows held in
+ * registers.  PLAIN: one kernel per reference kernel family, data in HBM. */
+enum { OFB_PATH_FUSED = 0, OFB_PATH_PLAIN = 1 };
+
+/* Model description; mirrors OstrichModel (src/ostrich/core/model.py:171-342).
+ * All worlds must share one topology (model.py:200-203). */
+typedef struct ofb_model {
+  int32_t num_worlds;
+  int32_t bodies_per_world;
+  int32_t joints_per_world;
+  int32_t dofs_per_world;
+  int32_t shapes_per_world;   /* S: shapes owned by each world */
+  int32_t num_global_shapes;  /* G: shapes shared by every world (ground, heightfield) */
+  float gravity[3];
+  /* bodies [W*B] */
+  const float* body_mass;
+  const float* body_inv_mass;
+  const float* body_inertia;     /* [W*B*9] row-major, body frame */
+  const float* body_inv_inertia; /* [W*B*9] */
+  const float* body_com;         /* [W*B*3] body frame */
+  /* joints [W*J] */
+  const int32_t* joint_type;
+  const int32_t* joint_parent; /* world-local body index, -1 = world */
+  const int32_t* joint_child;
+  const int32_t* joint_qd_start; /* world-local dof index */
+  const int32_t* joint_enabled;
+  const float* joint_X_p;        /* [W*J*7] */
+  const float* joint_X_c;        /* [W*J*7] */
+  const float* joint_compliance; /* [W*J], < 0 selects config.compliance_joint */
+  /* dofs [W*D] */
+  const float* joint_axis; /* [W*D*3] */
+  const int32_t* joint_dof_mode;
+  const float* joint_target_ke;
+  const float* joint_target_kd;
+  /* shapes [W*(S+G)]: per-world columns [0,S) then the globals [S,S+G) */
+  const int32_t* shape_body; /* world-local body, -1 = static */
+  const float* shape_mu;
+  const float* shape_friction_axis; /* [W*(S+G)*3] */
+  const float* shape_mu_perp;
+  const float* shape_mu_stiction_scale;
+  const float* shape_v_stribeck;
+  const float* shape_stribeck_lateral_only;
+  /* Newton flat shape index -> world, [G + W*S]; globals hold -1 */
+  const int32_t* shape_world;
+} ofb_model;
+
+/* Solver configuration; mirrors OstrichEngineConfig (core/engine_config.py:103-577). */
+typedef struct ofb_config {
+  int32_t nr_max_iters;          /* nr.max_iters */
+  int32_t nr_backtrack_min_iter; /* nr.backtrack_min_iter */
+  float nr_atol;                 /* nr.atol on ||h||_2 */
+  int32_t lin_max_iters;         /* linear.max_iters */
+  float lin_tol;                 /* linear.tol */
+  float lin_atol;                /* linear.atol */
+  int32_t preconditioner;        /* OFB_PRECOND_* */
+  float regularization;          /* linear.regularization */
+  float compliance_joint;
+  float compliance_contact;
+  float compliance_friction;
+  float contact_fb_eps_sq;       /* compliance.contact_fb_smooth_eps_sq */
+  float contact_fb_alpha;        /* CONTACT_FB_ALPHA, contact_constraint.py:9 */
+  float gyro_scale;              /* relaxation.gyro -> 1 or 0 */
+  float friction_load_gate;      /* relaxation.friction_load_gate */
+  int32_t friction_mode;         /* OFB_FRICTION_* */
+  int32_t contact_mode;          /* OFB_CONTACT_* */
+  int32_t max_contacts_per_world;/* contacts.max_per_world */
+  int32_t termination;           /* OFB_TERM_* */
+  int32_t seed_iterate;          /* warm_start.seed_iterate: start NR from the lambda seed */
+  int32_t rigid_contact_max;     /* capacity of the flat contact arrays (Newton
+                                    Contacts.rigid_contact_max); sizes the packing scratch */
+} ofb_config;
+
+typedef struct ofb_state_in {
+  const float* body_q;  /* [W*B*7] */
+  const float* body_qd; /* [W*B*6] */
+  const float* body_f;  /* [W*B*6] */
+} ofb_state_in;
+
+typedef struct ofb_state_out {
+  float* body_q;  /* [W*B*7] */
+  float* body_qd; /* [W*B*6] */
+} ofb_state_out;
+
+typedef struct ofb_control {
+  const float* joint_target_pos; /* [W*D] */
+  const float* joint_target_vel; /* [W*D] */
+} ofb_control;
+
+/* Newton rigid contacts (flat, unbatched) as read by core/contacts.py:25-99.
+ * normal points from shape0 to shape1 (Newton A-to-B); point0/point1 are in the frame
+ * of the owning body (world frame for static shapes); margins become thicknesses. */
+typedef struct ofb_contacts {
+  const int32_t* count; /* [1] device */
+  int32_t max_contacts;
+  const float* point0;  /* [max*3] */
+  const float* point1;  /* [max*3] */
+  const float* normal;  /* [max*3] */
+  const int32_t* shape0;
+  const int32_t* shape1;
+  const float* margin0;
+  const float* margin1;
+} ofb_contacts;
+
+/* Opaque solver handle (engine state + scratch), one per OstrichEngine instance. */
+typedef struct ofb_solver ofb_solver;
+
+/* Replaces OstrichEngineBase.__init__ (core/base_engine.py:112-237). Copies the model
+ * into solver-owned device memory and allocates every scratch buffer. `path` selects
+ * OFB_PATH_FUSED or OFB_PATH_PLAIN. Model pointers are device pointers. */
+int ofb_create(const ofb_model* model, const ofb_config* config, int32_t path,
+               ofb_solver** out_solver);
+
+/* Releases everything ofb_create allocated. */
+int ofb_destroy(ofb_solver* solver);
+
+/* Number of constraint rows per world, N_c = N_j + N_ctrl + 3*C_max
+ * (core/engine_dims.py:56-61), and the block offsets of the row layout
+ * [joints | control | normals | friction] (engine_dims.py:68-86). */
+int ofb_dims(const ofb_solver* solver, int32_t* n_c, int32_t* offset_ctrl, int32_t* offset_n,
+             int32_t* offset_f);
+
+/* Replaces OstrichEngine.step (core/engine.py:25-71): load_data (base_engine.py:289-335),
+ * the Newton loop (_solve, base_engine.py:416-526), best-iterate backtracking
+ * (core/backtracking.py:12-95) and the copy-out of body_q/body_qd. All pointers are
+ * device pointers; `stream` is a hipStream_t (NULL = default stream). No host
+ * synchronisation happens inside. */
+int ofb_step(ofb_solver* solver, const ofb_state_in* state_in, const ofb_state_out* state_out,
+             const ofb_control* control, const ofb_contacts* contacts, float dt, void* stream);
+
+/* Per-world statistics of the last step, copied into caller device buffers (any may be
+ * NULL): Newton iterations, summed PCR iterations, the picked merit ||h||^2, the picked
+ * candidate index and a flag word (bit0 NR cap hit, bit1 contact overflow, bit2 block
+ * factor failure, bit3 pair member overflow). Replaces the HDF5 history
+ * (core/base_engine.py:239-247). */
+int ofb_get_stats(const ofb_solver* solver, int32_t* nr_iters, int32_t* pcr_iters,
+                  float* merit, int32_t* best_iter, int32_t* flags, void* stream);
+
+/* Copies the picked iterate's constraint impulses lambda [W*N_c] (data._constr_force
+ * after perform_backtracking) into a caller device buffer. */
+int ofb_get_lambda(const ofb_solver* solver, float* lambda_out, void* stream);
+
+/* Copies the packed per-world contact set of the last step: count [W] and, for each of
+ * the W*C_max slots, shape0/shape1 (world-local columns), point0/point1/normal (after the
+ * normal flip, core/contacts.py:80-83) and thickness0/thickness1. Any pointer may be NULL. */
+int ofb_get_packed_contacts(const ofb_solver* solver, int32_t* count, int32_t* shape0,
+                            int32_t* shape1, float* point0, float* point1, float* normal,
+                            float* thickness0, float* thickness1, void* stream);
+
+/* Sets the lambda [W*N_c] that the next ofb_step starts Newton from when
+ * config.seed_iterate is on (core/engine.py:55-59); also the injection point of the
+ * learned warm start. */
+int ofb_set_lambda_seed(ofb_solver* solver, const float* lambda_seed, void* stream);
+
+/* Last error message of the calling thread ("" if none). */
+const char* ofb_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OSTRICH_FB_H */

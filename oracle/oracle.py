@@ -1,0 +1,151 @@
+"""ctypes binding of the CPU oracle (oracle/fb_oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and by the
+cpu_baseline leg of bench.py, as the checker or the timed CPU baseline.  The product
+path (ostrich_amd) never imports it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from ostrich_amd import _abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libfb_oracle.so")
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        _lib.oracle_step.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, C.c_float, C.c_int, vp,
+                                     vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+        _lib.oracle_step.restype = C.c_int
+        _lib.oracle_dims.argtypes = [vp, vp, vp, vp, vp]
+        _lib.oracle_fb_diff.argtypes = [C.c_double] * 5 + [vp]
+        _lib.oracle_integrate_pose.argtypes = [vp, vp, vp, C.c_double, vp]
+        _lib.oracle_linear_system.argtypes = [vp] * 8 + [C.c_float] + [vp] * 6
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+class HostModel:
+    """Model arrays pinned in host memory with an OfbModel pointing at them."""
+
+    def __init__(self, model):
+        self.model = model
+        self.arrays = {k: np.ascontiguousarray(v) for k, v in model.arrays.items()}
+        ptrs = {k: self.arrays[k].ctypes.data for k in self.arrays}
+        self.abi = model.to_abi(ptrs)
+
+
+class HostContacts:
+    def __init__(self, contacts):
+        c = {k: (v.detach().cpu().numpy() if hasattr(v, "detach") else v) for k, v in contacts.items()}
+        self.arrays = dict(
+            count=np.ascontiguousarray(np.asarray(c["count"], np.int32).reshape(1)),
+            point0=np.ascontiguousarray(np.asarray(c["point0"], np.float32)),
+            point1=np.ascontiguousarray(np.asarray(c["point1"], np.float32)),
+            normal=np.ascontiguousarray(np.asarray(c["normal"], np.float32)),
+            shape0=np.ascontiguousarray(np.asarray(c["shape0"], np.int32)),
+            shape1=np.ascontiguousarray(np.asarray(c["shape1"], np.int32)),
+            margin0=np.ascontiguousarray(np.asarray(c["margin0"], np.float32)),
+            margin1=np.ascontiguousarray(np.asarray(c["margin1"], np.float32)),
+        )
+        a = self.arrays
+        self.abi = _abi.OfbContacts()
+        self.abi.count = a["count"].ctypes.data
+        self.abi.max_contacts = int(c["max_contacts"])
+        for k in ("point0", "point1", "normal", "shape0", "shape1", "margin0", "margin1"):
+            setattr(self.abi, k, a[k].ctypes.data)
+
+
+def dims(model, config):
+    hm = HostModel(model)
+    cfg = config.to_abi()
+    nc, nj, nctrl = C.c_int32(), C.c_int32(), C.c_int32()
+    lib().oracle_dims(C.byref(hm.abi), C.byref(cfg), C.byref(nc), C.byref(nj), C.byref(nctrl))
+    return nc.value, nj.value, nctrl.value
+
+
+def step(model, config, state, control, contacts, dt, precision=32, lam_seed=None, host_model=None):
+    """One solver step on the CPU.  Returns a dict with body_q, body_qd (float64), lambda,
+    nr_iters, pcr_iters, merit, best_iter, flags, contact_count."""
+    hm = host_model or HostModel(model)
+    hc = contacts if isinstance(contacts, HostContacts) else HostContacts(contacts)
+    cfg = config.to_abi()
+    W = model.num_worlds
+    WB = W * model.bodies_per_world
+    nc, _, _ = dims(model, config)
+    q_in = np.ascontiguousarray(np.asarray(state["body_q"], np.float64).reshape(WB, 7))
+    qd_in = np.ascontiguousarray(np.asarray(state["body_qd"], np.float64).reshape(WB, 6))
+    f_in = np.ascontiguousarray(np.asarray(state["body_f"], np.float32).reshape(WB, 6))
+    tp = np.ascontiguousarray(np.asarray(control["joint_target_pos"], np.float32))
+    tv = np.ascontiguousarray(np.asarray(control["joint_target_vel"], np.float32))
+    seed = None if lam_seed is None else np.ascontiguousarray(np.asarray(lam_seed, np.float64))
+    out = dict(
+        body_q=np.zeros((WB, 7)), body_qd=np.zeros((WB, 6)), lam=np.zeros((W, nc)),
+        nr_iters=np.zeros(W, np.int32), pcr_iters=np.zeros(W, np.int32), merit=np.zeros(W),
+        best_iter=np.zeros(W, np.int32), flags=np.zeros(W, np.int32), contact_count=np.zeros(W, np.int32),
+        merit_hist=np.full((W, config.nr.max_iters), np.nan),
+        residual=np.zeros((W, 6 * model.bodies_per_world + nc)),
+    )
+    rc = lib().oracle_step(C.byref(hm.abi), C.byref(cfg), _p(q_in), _p(qd_in), _p(f_in), _p(tp), _p(tv),
+                           C.byref(hc.abi), C.c_float(dt), precision, _p(seed), _p(out["body_q"]),
+                           _p(out["body_qd"]), _p(out["lam"]), _p(out["nr_iters"]), _p(out["pcr_iters"]),
+                           _p(out["merit"]), _p(out["best_iter"]), _p(out["flags"]), _p(out["contact_count"]),
+                           _p(out["merit_hist"]), _p(out["residual"]))
+    if rc != 0:
+        raise RuntimeError(f"oracle_step failed ({rc})")
+    return out
+
+
+def linear_system(model, config, state, control, contacts, dt, x=None):
+    """FP64 dense Schur system of the first Newton iteration (for brute-force checks)."""
+    hm = HostModel(model)
+    hc = HostContacts(contacts)
+    cfg = config.to_abi()
+    W = model.num_worlds
+    WB = W * model.bodies_per_world
+    nc, _, _ = dims(model, config)
+    q_in = np.ascontiguousarray(np.asarray(state["body_q"], np.float64).reshape(WB, 7))
+    qd_in = np.ascontiguousarray(np.asarray(state["body_qd"], np.float64).reshape(WB, 6))
+    f_in = np.ascontiguousarray(np.asarray(state["body_f"], np.float32).reshape(WB, 6))
+    tp = np.ascontiguousarray(np.asarray(control["joint_target_pos"], np.float32))
+    tv = np.ascontiguousarray(np.asarray(control["joint_target_vel"], np.float32))
+    xv = np.zeros((W, nc)) if x is None else np.ascontiguousarray(np.asarray(x, np.float64))
+    A = np.zeros((W, nc, nc)); rhs = np.zeros((W, nc)); pinv = np.zeros((W, nc)); Ax = np.zeros((W, nc))
+    act = np.zeros((W, nc))
+    lib().oracle_linear_system(C.byref(hm.abi), C.byref(cfg), _p(q_in), _p(qd_in), _p(f_in), _p(tp), _p(tv),
+                               C.byref(hc.abi), C.c_float(dt), _p(xv), _p(A), _p(rhs), _p(pinv), _p(Ax), _p(act))
+    return dict(A=A, rhs=rhs, pinv=pinv, Ax=Ax, active=act)
+
+
+def fb_diff(a, b, alpha, beta, eps_sq):
+    out = np.zeros(3)
+    lib().oracle_fb_diff(a, b, alpha, beta, eps_sq, _p(out))
+    return out
+
+
+def integrate_pose(vel, pose, com, dt):
+    v = np.ascontiguousarray(np.asarray(vel, np.float64))
+    p = np.ascontiguousarray(np.asarray(pose, np.float64))
+    c = np.ascontiguousarray(np.asarray(com, np.float64))
+    out = np.zeros(7)
+    lib().oracle_integrate_pose(_p(v), _p(p), _p(c), C.c_double(dt), _p(out))
+    return out

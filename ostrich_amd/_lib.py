@@ -1,0 +1,46 @@
+"""Loads the gfx950 solver library (ostrich_amd/_lib/libostrich_fb.so) with ctypes.
+
+There is no fallback: if the library is missing or cannot be loaded the import fails
+with the reason, so nothing silently runs on another path.
+"""
+import ctypes as C
+import os
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "libostrich_fb.so")
+
+EXPORTS = (
+    "ofb_create", "ofb_destroy", "ofb_dims", "ofb_step", "ofb_get_stats", "ofb_get_lambda",
+    "ofb_get_packed_contacts", "ofb_set_lambda_seed", "ofb_last_error",
+)
+
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built; run `python -m ostrich_amd.build`")
+    lib = C.CDLL(LIB_PATH)
+    vp, i32 = C.c_void_p, C.c_int32
+    lib.ofb_create.argtypes = [vp, vp, i32, C.POINTER(vp)]
+    lib.ofb_destroy.argtypes = [vp]
+    lib.ofb_dims.argtypes = [vp, vp, vp, vp, vp]
+    lib.ofb_step.argtypes = [vp, vp, vp, vp, vp, C.c_float, vp]
+    lib.ofb_get_stats.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+    lib.ofb_get_lambda.argtypes = [vp, vp, vp]
+    lib.ofb_get_packed_contacts.argtypes = [vp] * 9 + [vp]
+    lib.ofb_set_lambda_seed.argtypes = [vp, vp, vp]
+    lib.ofb_last_error.restype = C.c_char_p
+    for name in EXPORTS:
+        if name != "ofb_last_error":
+            getattr(lib, name).restype = C.c_int
+    _lib = lib
+    return lib
+
+
+def check(rc, what):
+    if rc != 0:
+        msg = load().ofb_last_error().decode(errors="replace")
+        raise RuntimeError(f"{what} failed ({rc}): {msg}")

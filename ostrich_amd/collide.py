@@ -1,0 +1,239 @@
+"""Primitive collider for the benchmark scenes (plane / box / cylinder / sphere).
+
+It produces Newton-format rigid contacts (flat arrays; normal from shape0 to shape1;
+point0/point1 in the frame of the owning body, world frame for static shapes; margins),
+the input format of the solver boundary (core/contacts.py:25-99).  Newton's own
+`collide` is not available here, so contact geometry is this module's approximation:
+sample points of the moving shape are tested against planes and boxes.  The routine is
+written with torch tensor ops so the same code runs on host tensors (tests) and on the
+GPU (benchmark loop); the solver never depends on it.
+
+Candidate pairs are fixed per model; each pair contributes a fixed block of candidate
+points, and valid candidates are emitted in (world, pair, point) order, which makes the
+flat contact order deterministic.
+"""
+import math
+
+import numpy as np
+import torch
+
+from .model import GEO_BOX, GEO_CYLINDER, GEO_PLANE, GEO_SPHERE
+
+
+def _quat_rotate(q, v):
+    """q [...,4] xyzw (unit), v [...,3]."""
+    u = q[..., :3]
+    w = q[..., 3:4]
+    c = 2.0 * w * w - 1.0
+    d = 2.0 * (u * v).sum(-1, keepdim=True)
+    return v * c + u * d + torch.cross(u, v, dim=-1) * (2.0 * w)
+
+
+def _quat_rotate_inv(q, v):
+    qc = torch.cat([-q[..., :3], q[..., 3:4]], dim=-1)
+    return _quat_rotate(qc, v)
+
+
+def _quat_mul(a, b):
+    ax, ay, az, aw = a.unbind(-1)
+    bx, by, bz, bw = b.unbind(-1)
+    return torch.stack([aw * bx + bw * ax + ay * bz - az * by,
+                        aw * by + bw * ay + az * bx - ax * bz,
+                        aw * bz + bw * az + ax * by - ay * bx,
+                        aw * bw - ax * bx - ay * by - az * bz], dim=-1)
+
+
+def _box_corners(h):
+    s = torch.tensor([[sx, sy, sz] for sx in (-1, 1) for sy in (-1, 1) for sz in (-1, 1)],
+                     dtype=h.dtype, device=h.device)
+    return s * h[..., None, :]  # [..., 8, 3]
+
+
+def _cylinder_samples(r, hh, k):
+    ang = torch.arange(k, dtype=r.dtype, device=r.device) * (2.0 * math.pi / k)
+    pts = []
+    for y in (-1.0, 0.0, 1.0):
+        pts.append(torch.stack([torch.cos(ang) * r[..., None], torch.full_like(ang.expand(r.shape + (k,)), 0.0) + y * hh[..., None],
+                                torch.sin(ang) * r[..., None]], dim=-1))
+    return torch.cat(pts, dim=-2)  # [..., 3k, 3], axis = local y
+
+
+class PrimitiveCollider:
+    def __init__(self, model, device="cpu", gap=0.02, cylinder_samples=16, lateral_slack=0.06,
+                 max_contacts_per_world=128):
+        self.model = model
+        self.device = torch.device(device)
+        self.gap = gap
+        self.k = cylinder_samples
+        self.slack = lateral_slack
+        self.max_per_world = max_contacts_per_world
+        dt = torch.float32
+        self.geo = torch.as_tensor(model.shape_geo, device=self.device)           # [W, SC]
+        self.xform = torch.as_tensor(model.shape_xform, dtype=dt, device=self.device)  # [W, SC, 7]
+        self.size = torch.as_tensor(model.shape_size, dtype=dt, device=self.device)    # [W, SC, 3]
+        self.sbody = torch.as_tensor(model.shape_body_2d, device=self.device)    # [W, SC]
+        # candidate pairs from world 0 (uniform topology): (col_a, col_b) with a dynamic
+        g0 = model.shape_geo[0]
+        b0 = model.shape_body_2d[0]
+        SC = g0.shape[0]
+        # bodies joined by a joint do not collide with each other (Newton's default
+        # parent/child collision filter)
+        J = model.joints_per_world
+        jp = model.arrays["joint_parent"][:J]
+        jc = model.arrays["joint_child"][:J]
+        joined = {(int(min(p, c)), int(max(p, c))) for p, c in zip(jp, jc) if p >= 0 and c >= 0}
+        pairs = []
+        for a in range(SC):
+            if b0[a] < 0 or g0[a] == GEO_PLANE:
+                continue
+            for b in range(SC):
+                if b == a or g0[b] not in (GEO_PLANE, GEO_BOX):
+                    continue
+                if b0[b] >= 0 and b0[b] == b0[a]:
+                    continue
+                if b0[b] >= 0 and (int(min(b0[a], b0[b])), int(max(b0[a], b0[b]))) in joined:
+                    continue
+                if b0[b] >= 0:
+                    # dynamic-dynamic: test only the shape with the larger column against
+                    # the lower one (one direction per pair, as in a stack)
+                    if not (a > b):
+                        continue
+                pairs.append((a, b))
+        self.pairs = pairs
+
+    def _shape_pose(self, body_q, col):
+        """World pose [W,7] of shape column `col` for every world."""
+        W = self.model.num_worlds
+        B = self.model.bodies_per_world
+        X = self.xform[:, col]
+        bidx = self.sbody[:, col]
+        if int(bidx[0]) < 0:
+            return X
+        bq = body_q.view(W, B, 7)[torch.arange(W, device=self.device), bidx.long()]
+        p = bq[:, :3] + _quat_rotate(bq[:, 3:], X[:, :3])
+        q = _quat_mul(bq[:, 3:], X[:, 3:])
+        return torch.cat([p, q], dim=-1)
+
+    def _body_pose(self, body_q, col):
+        W, B = self.model.num_worlds, self.model.bodies_per_world
+        bidx = self.sbody[:, col]
+        if int(bidx[0]) < 0:
+            out = torch.zeros(W, 7, dtype=body_q.dtype, device=self.device)
+            out[:, 6] = 1.0
+            return out
+        return body_q.view(W, B, 7)[torch.arange(W, device=self.device), bidx.long()]
+
+    def _samples(self, col):
+        """Sample points of shape `col` in its shape frame, [W, P, 3]."""
+        geo = int(self.model.shape_geo[0, col])
+        size = self.size[:, col]
+        if geo == GEO_BOX:
+            return _box_corners(size)
+        if geo == GEO_CYLINDER:
+            return _cylinder_samples(size[:, 0], size[:, 1], self.k)
+        if geo == GEO_SPHERE:
+            return torch.zeros(size.shape[0], 1, 3, dtype=size.dtype, device=self.device)
+        raise ValueError(f"no samples for geometry {geo}")
+
+    def collide(self, body_q):
+        """body_q: [W*B, 7] tensor on the collider device.  Returns a dict of flat
+        Newton-format contact tensors (count[1], point0, point1, normal, shape0, shape1,
+        margin0, margin1) with max_contacts = W * max_contacts_per_world."""
+        body_q = body_q.to(self.device, torch.float32)
+        W = self.model.num_worlds
+        out_p0, out_p1, out_n, out_s0, out_s1, out_valid = [], [], [], [], [], []
+        for (a, b) in self.pairs:
+            geo_a = int(self.model.shape_geo[0, a])
+            geo_b = int(self.model.shape_geo[0, b])
+            Xa = self._shape_pose(body_q, a)
+            Ba = self._body_pose(body_q, a)
+            Xb = self._shape_pose(body_q, b)
+            Bb = self._body_pose(body_q, b)
+            if geo_b == GEO_PLANE:
+                nrm = _quat_rotate(Xb[:, 3:], torch.tensor([0.0, 0.0, 1.0], device=self.device).expand(W, 3))
+                if geo_a == GEO_CYLINDER:
+                    # lowest point of the middle circle
+                    ax = _quat_rotate(Xa[:, 3:], torch.tensor([0.0, 1.0, 0.0], device=self.device).expand(W, 3))
+                    d = -(nrm - (nrm * ax).sum(-1, keepdim=True) * ax)
+                    d = d / d.norm(dim=-1, keepdim=True).clamp_min(1e-9)
+                    pw = (Xa[:, :3] + d * self.size[:, a, 0:1])[:, None, :]
+                elif geo_a == GEO_SPHERE:
+                    pw = (Xa[:, :3] - nrm * self.size[:, a, 0:1])[:, None, :]
+                else:
+                    loc = self._samples(a)
+                    pw = Xa[:, None, :3] + _quat_rotate(Xa[:, None, 3:], loc)
+                dist = ((pw - Xb[:, None, :3]) * nrm[:, None, :]).sum(-1)          # [W,P]
+                p1w = pw - dist[..., None] * nrm[:, None, :]
+                n_out = nrm[:, None, :].expand_as(pw)
+                valid = dist < self.gap
+            else:  # box
+                hb = self.size[:, b]                                                  # [W,3]
+                if geo_a == GEO_SPHERE:
+                    loc = self._samples(a)
+                else:
+                    loc = self._samples(a)
+                pw = Xa[:, None, :3] + _quat_rotate(Xa[:, None, 3:], loc)
+                q = _quat_rotate_inv(Xb[:, None, 3:], pw - Xb[:, None, :3])         # [W,P,3]
+                excess = q.abs() - hb[:, None, :]
+                if int(self.model.shape_body_2d[0, b]) >= 0:
+                    # dynamic box below: face chosen from the relative centre position
+                    dc = _quat_rotate_inv(Xb[:, 3:], Xa[:, :3] - Xb[:, :3])
+                    ha = self.size[:, a]
+                    score = dc.abs() / (hb + ha.norm(dim=-1, keepdim=True))
+                    axis = score.argmax(-1)[:, None].expand(q.shape[:-1])
+                else:
+                    axis = excess.argmax(-1)
+                onehot = torch.nn.functional.one_hot(axis, 3).to(q.dtype)
+                sgn = torch.where(q >= 0, 1.0, -1.0)
+                ex_n = (excess * onehot).sum(-1)
+                lat_ok = ((excess <= self.slack) | (onehot > 0)).all(-1)
+                qp = torch.maximum(torch.minimum(q, hb[:, None, :]), -hb[:, None, :])
+                qp = qp * (1 - onehot) + onehot * sgn * hb[:, None, :]
+                n_loc = onehot * sgn
+                if geo_a == GEO_SPHERE:
+                    r = self.size[:, a, 0][:, None]
+                    ex_n = ex_n - r
+                    pw = pw - _quat_rotate(Xb[:, None, 3:], n_loc) * r[..., None]
+                dist = ex_n
+                n_out = _quat_rotate(Xb[:, None, 3:], n_loc)
+                p1w = Xb[:, None, :3] + _quat_rotate(Xb[:, None, 3:], qp)
+                valid = (dist < self.gap) & lat_ok
+            # frames of the owning bodies
+            p0 = _quat_rotate_inv(Ba[:, None, 3:], pw - Ba[:, None, :3])
+            p1 = _quat_rotate_inv(Bb[:, None, 3:], p1w - Bb[:, None, :3])
+            P = pw.shape[1]
+            out_p0.append(p0)
+            out_p1.append(p1)
+            out_n.append(-n_out)  # Newton: from shape0 (a) to shape1 (b)
+            fa = torch.as_tensor(self.model.flat_shape_id(np.arange(W), np.full(W, a)), device=self.device)
+            fb = torch.as_tensor(self.model.flat_shape_id(np.arange(W), np.full(W, b)), device=self.device)
+            out_s0.append(fa[:, None].expand(W, P))
+            out_s1.append(fb[:, None].expand(W, P))
+            out_valid.append(valid)
+        p0 = torch.cat(out_p0, 1)
+        p1 = torch.cat(out_p1, 1)
+        nn = torch.cat(out_n, 1)
+        s0 = torch.cat(out_s0, 1)
+        s1 = torch.cat(out_s1, 1)
+        valid = torch.cat(out_valid, 1)
+        idx = valid.reshape(-1).nonzero().squeeze(-1)
+        n = int(idx.numel())
+        cap = W * self.max_per_world
+        n_keep = min(n, cap)
+        idx = idx[:n_keep]
+        def take(t, width):
+            flat = t.reshape(-1, width) if width > 1 else t.reshape(-1)
+            buf = torch.zeros((cap, width) if width > 1 else (cap,), dtype=flat.dtype, device=self.device)
+            buf[:n_keep] = flat[idx]
+            return buf
+        return dict(
+            count=torch.tensor([n_keep], dtype=torch.int32, device=self.device),
+            max_contacts=cap,
+            point0=take(p0, 3).float().contiguous(),
+            point1=take(p1, 3).float().contiguous(),
+            normal=take(nn, 3).float().contiguous(),
+            shape0=take(s0.to(torch.int32), 1).contiguous(),
+            shape1=take(s1.to(torch.int32), 1).contiguous(),
+            margin0=torch.zeros(cap, dtype=torch.float32, device=self.device),
+            margin1=torch.zeros(cap, dtype=torch.float32, device=self.device),
+        )

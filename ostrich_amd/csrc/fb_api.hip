@@ -1,0 +1,387 @@
+// fb_api.hip -- C ABI entry points (include/ostrich_fb.h): solver lifetime, contact
+// packing and step dispatch.
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include <hip/hip_runtime.h>
+#include <rocprim/device/device_radix_sort.hpp>
+
+#include "fb_solver.h"
+
+namespace ofb {
+
+static thread_local char g_err[512] = "";
+
+void set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+#define OFB_CHECK(expr)                                                              \
+  do {                                                                               \
+    hipError_t e_ = (expr);                                                          \
+    if (e_ != hipSuccess) {                                                          \
+      set_error("%s failed: %s", #expr, hipGetErrorString(e_));                     \
+      return -4;                                                                     \
+    }                                                                                \
+  } while (0)
+
+static int joint_row_count(int type) {  // core/model.py:88-99
+  static const int m[8] = {5, 5, 3, 6, 0, 1, 6, 0};
+  return (type >= 0 && type < 8) ? m[type] : 0;
+}
+
+// ---------------------------------------------------------------------------------
+// Contact packing (core/contacts.py:25-99).  The reference assigns slots with a
+// per-world atomic counter, which orders a world's contacts by thread id on Warp's CPU
+// device and arbitrarily on a GPU.  Here the order is the input order, on every device:
+// contacts are keyed by world, stably partitioned with a radix sort (keys only span
+// ceil(log2(W+1)) bits), and each contact's slot is its position inside its world's
+// segment.
+// ---------------------------------------------------------------------------------
+__global__ void pack_keys_kernel(int n_max, const int32_t* __restrict__ count, const int32_t* __restrict__ s0,
+                                 const int32_t* __restrict__ s1, const int32_t* __restrict__ shape_world, int W,
+                                 int32_t* __restrict__ keys, int32_t* __restrict__ vals) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_max) return;
+  int key = W;
+  int a = s0[i], b = s1[i];
+  if (i < count[0] && a != b) {
+    int w0 = a >= 0 ? shape_world[a] : -1;
+    int w1 = b >= 0 ? shape_world[b] : -1;
+    int w = w0 >= 0 ? w0 : w1;
+    if (w >= 0) key = w;
+  }
+  keys[i] = key;
+  vals[i] = i;
+}
+
+__global__ void pack_reset_kernel(int W, int32_t* __restrict__ ccount, int32_t* __restrict__ seg_start,
+                                  int32_t* __restrict__ flags) {
+  int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= W) return;
+  ccount[w] = 0;
+  seg_start[w] = 0;
+  flags[w] = 0;
+}
+
+__global__ void pack_segments_kernel(int n_max, int W, const int32_t* __restrict__ keys,
+                                     int32_t* __restrict__ seg_start, int32_t* __restrict__ ccount) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n_max) return;
+  int k = keys[j];
+  if (k >= W) return;
+  if (j == 0 || keys[j - 1] != k) seg_start[k] = j;
+  if (j == n_max - 1 || keys[j + 1] != k) ccount[k] = j + 1;  // end, fixed up below
+}
+
+__global__ void pack_scatter_kernel(int n_max, int W, int G, int S, int K, const int32_t* __restrict__ keys,
+                                    const int32_t* __restrict__ vals, const int32_t* __restrict__ seg_start,
+                                    const float* __restrict__ p0, const float* __restrict__ p1,
+                                    const float* __restrict__ nrm, const int32_t* __restrict__ s0,
+                                    const int32_t* __restrict__ s1, const float* __restrict__ m0,
+                                    const float* __restrict__ m1, float* __restrict__ cf, int32_t* __restrict__ ci,
+                                    int32_t* __restrict__ flags) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n_max) return;
+  int w = keys[j];
+  if (w >= W) return;
+  int slot = j - seg_start[w];
+  if (slot >= K) {
+    if (slot == K) flags[w] |= FLAG_CONTACT_OVERFLOW;
+    return;
+  }
+  int i = vals[j];
+  float* f = cf + (size_t)w * CF * K + slot;
+  f[CF_P0X * K] = p0[i * 3 + 0]; f[CF_P0Y * K] = p0[i * 3 + 1]; f[CF_P0Z * K] = p0[i * 3 + 2];
+  f[CF_P1X * K] = p1[i * 3 + 0]; f[CF_P1Y * K] = p1[i * 3 + 1]; f[CF_P1Z * K] = p1[i * 3 + 2];
+  // Newton emits the normal from shape0 to shape1; the solver uses the opposite sign
+  // (core/contacts.py:80-83)
+  f[CF_NX * K] = -nrm[i * 3 + 0]; f[CF_NY * K] = -nrm[i * 3 + 1]; f[CF_NZ * K] = -nrm[i * 3 + 2];
+  f[CF_TH0 * K] = m0[i];
+  f[CF_TH1 * K] = m1[i];
+  int a = s0[i], b = s1[i];
+  auto local = [&](int s) { return s < G ? S + s : (s - G) % S; };  // contacts.py:10-22
+  ci[(size_t)w * 2 * K + slot] = a >= 0 ? local(a) : a;
+  ci[(size_t)w * 2 * K + K + slot] = b >= 0 ? local(b) : b;
+}
+
+__global__ void pack_counts_kernel(int W, const int32_t* __restrict__ seg_start, int32_t* __restrict__ ccount) {
+  int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= W) return;
+  int end = ccount[w];
+  ccount[w] = end > 0 ? end - seg_start[w] : 0;
+}
+
+static int pack_contacts(ofb_solver* s, const ofb_contacts* c, hipStream_t stream) {
+  const int W = s->m.W, n = c->max_contacts;
+  if (n > s->rigid_contact_cap) {
+    set_error("contacts.max_contacts=%d exceeds config.rigid_contact_max=%d", n, s->rigid_contact_cap);
+    return -2;
+  }
+  const int T = 256;
+  hipLaunchKernelGGL(pack_reset_kernel, dim3((W + T - 1) / T), dim3(T), 0, stream, W, s->ccount, s->pk_seg_start,
+                     s->st_flags);
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(pack_keys_kernel, dim3((n + T - 1) / T), dim3(T), 0, stream, n, c->count, c->shape0, c->shape1,
+                     s->m.shape_world, W, s->pk_keys, s->pk_vals);
+  unsigned end_bit = 1;
+  while ((1u << end_bit) <= (unsigned)W) ++end_bit;
+  size_t tb = s->pk_temp_bytes;
+  OFB_CHECK(rocprim::radix_sort_pairs(s->pk_temp, tb, s->pk_keys, s->pk_keys_out, s->pk_vals, s->pk_vals_out,
+                                      (unsigned)n, 0, end_bit, stream));
+  hipLaunchKernelGGL(pack_segments_kernel, dim3((n + T - 1) / T), dim3(T), 0, stream, n, W, s->pk_keys_out,
+                     s->pk_seg_start, s->ccount);
+  hipLaunchKernelGGL(pack_counts_kernel, dim3((W + T - 1) / T), dim3(T), 0, stream, W, s->pk_seg_start, s->ccount);
+  hipLaunchKernelGGL(pack_scatter_kernel, dim3((n + T - 1) / T), dim3(T), 0, stream, n, W, s->m.G, s->m.S,
+                     s->L.K, s->pk_keys_out, s->pk_vals_out, s->pk_seg_start, c->point0, c->point1, c->normal,
+                     c->shape0, c->shape1, c->margin0, c->margin1, s->cf, s->ci, s->st_flags);
+  OFB_CHECK(hipGetLastError());
+  return 0;
+}
+
+template <class T>
+static int dev_copy(T** dst, const T* src, size_t n) {
+  *dst = nullptr;
+  if (n == 0) return 0;
+  if (!src) { set_error("model array missing"); return -1; }
+  OFB_CHECK(hipMalloc((void**)dst, n * sizeof(T)));
+  OFB_CHECK(hipMemcpy(*dst, src, n * sizeof(T), hipMemcpyDefault));
+  return 0;
+}
+
+template <class T>
+static int dev_alloc(T** dst, size_t n) {
+  *dst = nullptr;
+  if (n == 0) n = 1;
+  OFB_CHECK(hipMalloc((void**)dst, n * sizeof(T)));
+  OFB_CHECK(hipMemset(*dst, 0, n * sizeof(T)));
+  return 0;
+}
+
+}  // namespace ofb
+
+using namespace ofb;
+
+extern "C" {
+
+const char* ofb_last_error(void) { return g_err; }
+
+int ofb_destroy(ofb_solver* s) {
+  if (!s) return 0;
+  DeviceModel& m = s->m;
+  void* ptrs[] = {m.body_mass, m.body_inv_mass, m.body_inertia, m.body_inv_inertia, m.body_com, m.joint_type,
+                  m.joint_parent, m.joint_child, m.joint_qd_start, m.joint_enabled, m.joint_X_p, m.joint_X_c,
+                  m.joint_compliance, m.joint_axis, m.joint_dof_mode, m.joint_target_ke, m.joint_target_kd,
+                  m.shape_body, m.shape_mu, m.shape_friction_axis, m.shape_mu_perp, m.shape_mu_stiction_scale,
+                  m.shape_v_stribeck, m.shape_stribeck_lateral_only, m.shape_world, s->ccount, s->cf, s->ci,
+                  s->pk_keys, s->pk_keys_out, s->pk_vals, s->pk_vals_out, s->pk_seg_start, s->pk_temp, s->topo,
+                  s->lambda, s->lambda_seed, s->st_nr, s->st_pcr, s->st_best, s->st_flags, s->st_merit};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  if (s->plain) plain_destroy(s);
+  delete s;
+  return 0;
+}
+
+int ofb_create(const ofb_model* model, const ofb_config* config, int32_t path, ofb_solver** out_solver) {
+  if (!model || !config || !out_solver) { set_error("null argument"); return -1; }
+  *out_solver = nullptr;
+  const int W = model->num_worlds, B = model->bodies_per_world, J = model->joints_per_world,
+            D = model->dofs_per_world, S = model->shapes_per_world, G = model->num_global_shapes;
+  if (W < 1 || B < 1 || J < 0 || D < 0 || S < 0 || G < 0) { set_error("invalid model dimensions"); return -2; }
+  if (J > kMaxJoints) { set_error("at most %d joints per world are supported (got %d)", kMaxJoints, J); return -2; }
+  if (config->nr_max_iters < 1 || config->nr_backtrack_min_iter < 0 ||
+      config->nr_backtrack_min_iter >= config->nr_max_iters) {
+    set_error("invalid Newton iteration limits (engine_config.py:110-123)");
+    return -2;
+  }
+  if (config->lin_max_iters < 1) { set_error("linear.max_iters must be >= 1"); return -2; }
+  if (config->max_contacts_per_world < 1) { set_error("contacts.max_per_world must be >= 1"); return -2; }
+  if (config->preconditioner != OFB_PRECOND_JACOBI && config->preconditioner != OFB_PRECOND_PER_BODY_PAIR) {
+    set_error("unknown preconditioner"); return -2;
+  }
+  if (path != OFB_PATH_FUSED && path != OFB_PATH_PLAIN) { set_error("unknown execution path"); return -2; }
+  if (path == OFB_PATH_FUSED && config->termination != OFB_TERM_PER_WORLD) {
+    set_error("the fused path retires worlds independently: use termination=per_world (global needs the plain path)");
+    return -2;
+  }
+  if (path == OFB_PATH_FUSED && config->preconditioner != OFB_PRECOND_JACOBI) {
+    set_error("the fused path implements the Jacobi preconditioner; per_body_pair runs on the plain path");
+    return -2;
+  }
+  if (path == OFB_PATH_FUSED && B > 64) { set_error("the fused path supports at most 64 bodies per world"); return -2; }
+  ofb_solver* s = new ofb_solver;
+  std::memset(s, 0, sizeof(*s));
+  s->cfg = *config;
+  s->path = path;
+  DeviceModel& m = s->m;
+  m.W = W; m.B = B; m.J = J; m.D = D; m.S = S; m.G = G; m.SC = S + G;
+  for (int i = 0; i < 3; ++i) m.gravity[i] = model->gravity[i];
+  m.flat_shapes = G + W * S;
+  const size_t WB = (size_t)W * B, WJ = (size_t)W * J, WD = (size_t)W * D, WS = (size_t)W * (S + G);
+  // host copies of the topology of world 0 (identical worlds, core/model.py:200-203)
+  std::vector<int32_t> jt(J), jq(J), dm(D > 0 ? D : 1);
+  int rc = 0;
+  if (J > 0) {
+    if (hipMemcpy(jt.data(), model->joint_type, J * sizeof(int32_t), hipMemcpyDefault) != hipSuccess ||
+        hipMemcpy(jq.data(), model->joint_qd_start, J * sizeof(int32_t), hipMemcpyDefault) != hipSuccess) {
+      set_error("cannot read the joint topology");
+      delete s;
+      return -4;
+    }
+  }
+  if (D > 0 && hipMemcpy(dm.data(), model->joint_dof_mode, D * sizeof(int32_t), hipMemcpyDefault) != hipSuccess) {
+    set_error("cannot read the dof modes");
+    delete s;
+    return -4;
+  }
+  RowLayout& L = s->L;
+  L.Nj = 0; L.Nctrl = 0;
+  std::vector<int32_t> topo_j, topo_k, topo_c;
+  for (int j = 0; j < J; ++j) {
+    L.joint_off[j] = L.Nj;
+    int rcn = joint_row_count(jt[j]);
+    for (int k = 0; k < rcn; ++k) { topo_j.push_back(j); topo_k.push_back(k); }
+    L.Nj += rcn;
+  }
+  for (int j = 0; j < J; ++j) {
+    L.ctrl_off[j] = -1;
+    if ((jt[j] == 0 || jt[j] == 1) && jq[j] >= 0 && jq[j] < D && dm[jq[j]] != 0) {
+      L.ctrl_off[j] = L.Nctrl++;
+      topo_c.push_back(j);
+    }
+  }
+  L.K = config->max_contacts_per_world;
+  L.Nc = L.Nj + L.Nctrl + 3 * L.K;
+  L.off_ctrl = L.Nj;
+  L.off_n = L.Nj + L.Nctrl;
+  L.off_f = L.off_n + L.K;
+  std::vector<int32_t> topo(topo_j);
+  topo.insert(topo.end(), topo_k.begin(), topo_k.end());
+  topo.insert(topo.end(), topo_c.begin(), topo_c.end());
+  if (topo.empty()) topo.push_back(0);
+#define COPY(field, n) if ((rc = dev_copy(&m.field, model->field, n)) != 0) { ofb_destroy(s); return rc; }
+  COPY(body_mass, WB); COPY(body_inv_mass, WB); COPY(body_inertia, WB * 9); COPY(body_inv_inertia, WB * 9);
+  COPY(body_com, WB * 3);
+  COPY(joint_type, WJ); COPY(joint_parent, WJ); COPY(joint_child, WJ); COPY(joint_qd_start, WJ);
+  COPY(joint_enabled, WJ); COPY(joint_X_p, WJ * 7); COPY(joint_X_c, WJ * 7); COPY(joint_compliance, WJ);
+  COPY(joint_axis, WD * 3); COPY(joint_dof_mode, WD); COPY(joint_target_ke, WD); COPY(joint_target_kd, WD);
+  COPY(shape_body, WS); COPY(shape_mu, WS); COPY(shape_friction_axis, WS * 3); COPY(shape_mu_perp, WS);
+  COPY(shape_mu_stiction_scale, WS); COPY(shape_v_stribeck, WS); COPY(shape_stribeck_lateral_only, WS);
+  COPY(shape_world, (size_t)m.flat_shapes);
+#undef COPY
+#define ALLOC(ptr, n) if ((rc = dev_alloc(&s->ptr, n)) != 0) { ofb_destroy(s); return rc; }
+  const size_t K = L.K;
+  ALLOC(ccount, W); ALLOC(cf, (size_t)W * CF * K); ALLOC(ci, (size_t)W * 2 * K);
+  ALLOC(lambda, (size_t)W * L.Nc); ALLOC(lambda_seed, (size_t)W * L.Nc);
+  ALLOC(st_nr, W); ALLOC(st_pcr, W); ALLOC(st_best, W); ALLOC(st_flags, W); ALLOC(st_merit, W);
+  ALLOC(topo, topo.size());
+  int cap = config->rigid_contact_max > 0 ? config->rigid_contact_max : W * L.K;
+  s->rigid_contact_cap = cap;
+  ALLOC(pk_keys, cap); ALLOC(pk_keys_out, cap); ALLOC(pk_vals, cap); ALLOC(pk_vals_out, cap);
+  ALLOC(pk_seg_start, W);
+#undef ALLOC
+  if (hipMemcpy(s->topo, topo.data(), topo.size() * sizeof(int32_t), hipMemcpyHostToDevice) != hipSuccess) {
+    set_error("topology upload failed");
+    ofb_destroy(s);
+    return -4;
+  }
+  size_t tb = 0;
+  if (rocprim::radix_sort_pairs((void*)nullptr, tb, s->pk_keys, s->pk_keys_out, s->pk_vals, s->pk_vals_out,
+                                (unsigned)cap, 0, 31) != hipSuccess) {
+    set_error("radix sort sizing failed");
+    ofb_destroy(s);
+    return -4;
+  }
+  s->pk_temp_bytes = tb;
+  if (hipMalloc(&s->pk_temp, tb > 0 ? tb : 1) != hipSuccess) { set_error("hipMalloc failed"); ofb_destroy(s); return -4; }
+  s->fused_worlds_per_block = 4;
+  if (path == OFB_PATH_PLAIN) {
+    if ((rc = plain_create(s)) != 0) { ofb_destroy(s); return rc; }
+  }
+  *out_solver = s;
+  return 0;
+}
+
+int ofb_dims(const ofb_solver* s, int32_t* n_c, int32_t* offset_ctrl, int32_t* offset_n, int32_t* offset_f) {
+  if (!s) { set_error("null solver"); return -1; }
+  if (n_c) *n_c = s->L.Nc;
+  if (offset_ctrl) *offset_ctrl = s->L.off_ctrl;
+  if (offset_n) *offset_n = s->L.off_n;
+  if (offset_f) *offset_f = s->L.off_f;
+  return 0;
+}
+
+int ofb_step(ofb_solver* s, const ofb_state_in* in, const ofb_state_out* out, const ofb_control* ctl,
+             const ofb_contacts* contacts, float dt, void* stream) {
+  if (!s || !in || !out || !ctl || !contacts) { set_error("null argument"); return -1; }
+  if (!(dt > 0.f)) { set_error("dt must be positive"); return -2; }
+  if (!in->body_q || !in->body_qd || !in->body_f || !out->body_q || !out->body_qd) {
+    set_error("state pointers must be set");
+    return -1;
+  }
+  if (s->m.D > 0 && (!ctl->joint_target_pos || !ctl->joint_target_vel)) { set_error("control pointers must be set"); return -1; }
+  hipStream_t st = (hipStream_t)stream;
+  int rc = pack_contacts(s, contacts, st);
+  if (rc) return rc;
+  if (s->path == OFB_PATH_FUSED) return fused_step(s, in, out, ctl, dt, st);
+  return plain_step(s, in, out, ctl, dt, st);
+}
+
+int ofb_get_stats(const ofb_solver* s, int32_t* nr_iters, int32_t* pcr_iters, float* merit, int32_t* best_iter,
+                  int32_t* flags, void* stream) {
+  if (!s) { set_error("null solver"); return -1; }
+  hipStream_t st = (hipStream_t)stream;
+  const size_t W = s->m.W;
+  if (nr_iters) OFB_CHECK(hipMemcpyAsync(nr_iters, s->st_nr, W * 4, hipMemcpyDeviceToDevice, st));
+  if (pcr_iters) OFB_CHECK(hipMemcpyAsync(pcr_iters, s->st_pcr, W * 4, hipMemcpyDeviceToDevice, st));
+  if (merit) OFB_CHECK(hipMemcpyAsync(merit, s->st_merit, W * 4, hipMemcpyDeviceToDevice, st));
+  if (best_iter) OFB_CHECK(hipMemcpyAsync(best_iter, s->st_best, W * 4, hipMemcpyDeviceToDevice, st));
+  if (flags) OFB_CHECK(hipMemcpyAsync(flags, s->st_flags, W * 4, hipMemcpyDeviceToDevice, st));
+  return 0;
+}
+
+int ofb_get_lambda(const ofb_solver* s, float* lambda_out, void* stream) {
+  if (!s || !lambda_out) { set_error("null argument"); return -1; }
+  OFB_CHECK(hipMemcpyAsync(lambda_out, s->lambda, (size_t)s->m.W * s->L.Nc * 4, hipMemcpyDeviceToDevice,
+                           (hipStream_t)stream));
+  return 0;
+}
+
+int ofb_set_lambda_seed(ofb_solver* s, const float* seed, void* stream) {
+  if (!s || !seed) { set_error("null argument"); return -1; }
+  OFB_CHECK(hipMemcpyAsync(s->lambda_seed, seed, (size_t)s->m.W * s->L.Nc * 4, hipMemcpyDeviceToDevice,
+                           (hipStream_t)stream));
+  return 0;
+}
+
+int ofb_get_packed_contacts(const ofb_solver* s, int32_t* count, int32_t* shape0, int32_t* shape1, float* point0,
+                            float* point1, float* normal, float* thickness0, float* thickness1, void* stream) {
+  if (!s) { set_error("null solver"); return -1; }
+  hipStream_t st = (hipStream_t)stream;
+  const int W = s->m.W, K = s->L.K;
+  if (count) OFB_CHECK(hipMemcpyAsync(count, s->ccount, (size_t)W * 4, hipMemcpyDeviceToDevice, st));
+  // component-major blocks -> [W*K*3] / [W*K] rows
+  for (int w = 0; w < W; ++w) {
+    const float* f = s->cf + (size_t)w * CF * K;
+    const int32_t* ii = s->ci + (size_t)w * 2 * K;
+    if (shape0) OFB_CHECK(hipMemcpyAsync(shape0 + (size_t)w * K, ii, K * 4, hipMemcpyDeviceToDevice, st));
+    if (shape1) OFB_CHECK(hipMemcpyAsync(shape1 + (size_t)w * K, ii + K, K * 4, hipMemcpyDeviceToDevice, st));
+    if (thickness0) OFB_CHECK(hipMemcpyAsync(thickness0 + (size_t)w * K, f + CF_TH0 * K, K * 4, hipMemcpyDeviceToDevice, st));
+    if (thickness1) OFB_CHECK(hipMemcpyAsync(thickness1 + (size_t)w * K, f + CF_TH1 * K, K * 4, hipMemcpyDeviceToDevice, st));
+    for (int d = 0; d < 3; ++d) {
+      if (point0) OFB_CHECK(hipMemcpy2DAsync(point0 + (size_t)w * K * 3 + d, 12, f + (CF_P0X + d) * K, 4, 4, K, hipMemcpyDeviceToDevice, st));
+      if (point1) OFB_CHECK(hipMemcpy2DAsync(point1 + (size_t)w * K * 3 + d, 12, f + (CF_P1X + d) * K, 4, 4, K, hipMemcpyDeviceToDevice, st));
+      if (normal) OFB_CHECK(hipMemcpy2DAsync(normal + (size_t)w * K * 3 + d, 12, f + (CF_NX + d) * K, 4, 4, K, hipMemcpyDeviceToDevice, st));
+    }
+  }
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,93 @@
+// fb_solver.h -- solver handle shared by the fused and the plain execution paths.
+//
+// HBM layout (all world-major; W worlds, B bodies, J joints, D dofs, SC shape columns,
+// K = max contacts per world, R = N_c constraint rows per world):
+//   model      : copies of the ofb_model arrays (read-only during a step)
+//   contacts   : packed per world in component-major blocks so that lanes that walk
+//                consecutive contacts issue coalesced loads:
+//                  cf[(w*CF + k)*K + c]   k in {p0x,p0y,p0z,p1x,p1y,p1z,nx,ny,nz,th0,th1}
+//                  ci[(w*2 + k)*K + c]    k in {shape0, shape1} (world-local columns)
+//   lambda     : [W*R] picked iterate (reference row layout, core/engine_dims.py:68-107)
+//   stats      : [W] Newton iterations, PCR iterations, merit, picked index, flags
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/ostrich_fb.h"
+
+namespace ofb {
+
+enum { CF_P0X = 0, CF_P0Y, CF_P0Z, CF_P1X, CF_P1Y, CF_P1Z, CF_NX, CF_NY, CF_NZ, CF_TH0, CF_TH1, CF };
+
+enum {
+  FLAG_NR_CAP = 1,
+  FLAG_CONTACT_OVERFLOW = 2,
+  FLAG_FACTOR_FAILURE = 4,
+  FLAG_PAIR_OVERFLOW = 8,
+};
+
+constexpr int kMaxJoints = 64;      // joints per world supported by the fused path
+constexpr int kMaxPairMembers = 64; // per_body_pair_preconditioner.py:59
+
+struct DeviceModel {
+  int W, B, J, D, S, G, SC;
+  float gravity[3];
+  float *body_mass, *body_inv_mass, *body_inertia, *body_inv_inertia, *body_com;
+  int32_t *joint_type, *joint_parent, *joint_child, *joint_qd_start, *joint_enabled;
+  float *joint_X_p, *joint_X_c, *joint_compliance;
+  float *joint_axis;
+  int32_t* joint_dof_mode;
+  float *joint_target_ke, *joint_target_kd;
+  int32_t* shape_body;
+  float *shape_mu, *shape_friction_axis, *shape_mu_perp, *shape_mu_stiction_scale, *shape_v_stribeck,
+      *shape_stribeck_lateral_only;
+  int32_t* shape_world;
+  int flat_shapes;  // G + W*S
+};
+
+// Row layout derived from world 0 (core/model.py:83-168)
+struct RowLayout {
+  int Nj, Nctrl, K, Nc, off_ctrl, off_n, off_f;
+  int32_t joint_off[kMaxJoints];  // first joint row of each joint
+  int32_t ctrl_off[kMaxJoints];   // control row of each joint (-1 = none)
+};
+
+struct PlainBuffers;  // fb_plain.hip
+
+}  // namespace ofb
+
+struct ofb_solver {
+  ofb_config cfg;
+  int32_t path;
+  ofb::DeviceModel m;
+  ofb::RowLayout L;
+  // packed contacts
+  int32_t* ccount;   // [W] raw per-world count (may exceed K, core/contacts.py:73-76)
+  float* cf;         // [W*CF*K]
+  int32_t* ci;       // [W*2*K]
+  // contact packing scratch (stable partition by world)
+  int32_t rigid_contact_cap;
+  int32_t *pk_keys, *pk_keys_out, *pk_vals, *pk_vals_out, *pk_seg_start;
+  void* pk_temp;
+  size_t pk_temp_bytes;
+  // outputs
+  int32_t* topo;     // joint of each joint row [Nj], row within joint [Nj], joint of each control row [Nctrl]
+  float* lambda;     // [W*Nc]
+  float* lambda_seed;// [W*Nc]
+  int32_t *st_nr, *st_pcr, *st_best, *st_flags;
+  float* st_merit;
+  ofb::PlainBuffers* plain;
+  // launch shape chosen by the autotuner for the fused path
+  int fused_worlds_per_block;
+};
+
+namespace ofb {
+int fused_step(ofb_solver* s, const ofb_state_in* in, const ofb_state_out* out, const ofb_control* ctl, float dt,
+               hipStream_t stream);
+int plain_create(ofb_solver* s);
+void plain_destroy(ofb_solver* s);
+int plain_step(ofb_solver* s, const ofb_state_in* in, const ofb_state_out* out, const ofb_control* ctl, float dt,
+               hipStream_t stream);
+void set_error(const char* fmt, ...);
+}  // namespace ofb

@@ -1,0 +1,148 @@
+"""OstrichEngine: the solver plugin, with the reference's entry points.
+
+Mirrors `OstrichEngine` (src/ostrich/core/engine.py:14-71) and the parts of
+`OstrichEngineBase` (core/base_engine.py:111-751) a caller uses:
+`step(state_in, state_out, control, contacts, dt)`, `reset_timestep_counter()` and the
+per-step diagnostics.  All arithmetic runs in the gfx950 library behind the C ABI
+(include/ostrich_fb.h); this class only moves pointers.
+"""
+import ctypes as C
+
+import torch
+
+from . import _abi, _lib
+from .config import OstrichEngineConfig
+from .containers import Contacts, Control, State
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else None
+
+
+class OstrichEngine:
+    def __init__(self, model, sim_steps=None, config: OstrichEngineConfig = OstrichEngineConfig(),
+                 logging_config=None, differentiable_simulation=False, device="cuda:0",
+                 rigid_contact_max=None):
+        if not torch.cuda.is_available():
+            raise RuntimeError("OstrichEngine needs a HIP device (torch.cuda.is_available() is False)")
+        self.lib = _lib.load()
+        self.model = model
+        self.config = config
+        self.sim_steps = sim_steps
+        self.differentiable_simulation = differentiable_simulation
+        self.device = torch.device(device)
+        self.timestep = 0
+        W = model.num_worlds
+        # model arrays live on the device for the duration of ofb_create (the library copies them)
+        dev = {}
+        for name, dtype, _, _ in _abi.MODEL_ARRAYS:
+            arr = model.arrays[name]
+            td = torch.int32 if dtype == "i4" else torch.float32
+            dev[name] = torch.as_tensor(arr, dtype=td, device=self.device).contiguous()
+        ptrs = {k: v.data_ptr() for k, v in dev.items()}
+        abi_model = model.to_abi(ptrs)
+        cfg = config.to_abi()
+        cap = rigid_contact_max if rigid_contact_max is not None else W * config.contacts.max_per_world
+        cfg.rigid_contact_max = int(cap)
+        self.rigid_contact_max = int(cap)
+        h = C.c_void_p()
+        torch.cuda.synchronize(self.device)
+        _lib.check(self.lib.ofb_create(C.byref(abi_model), C.byref(cfg), config.path_id, C.byref(h)), "ofb_create")
+        self._h = h
+        del dev
+        nc, oc, on, of = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+        _lib.check(self.lib.ofb_dims(h, C.byref(nc), C.byref(oc), C.byref(on), C.byref(of)), "ofb_dims")
+        self.num_constraints = nc.value
+        self.offset_ctrl, self.offset_n, self.offset_f = oc.value, on.value, of.value
+        self.num_worlds = W
+        # keep the argument structs alive between calls
+        self._in = _abi.OfbStateIn()
+        self._out = _abi.OfbStateOut()
+        self._ctl = _abi.OfbControl()
+        self._con = _abi.OfbContacts()
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                torch.cuda.synchronize(self.device)
+                self.lib.ofb_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    def _stream(self):
+        return C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+
+    def step(self, state_in: State, state_out: State, control: Control, contacts: Contacts, dt: float):
+        """core/engine.py:25-71."""
+        for t in (state_in.body_q, state_in.body_qd, state_in.body_f, state_out.body_q, state_out.body_qd):
+            if t.dtype != torch.float32 or not t.is_contiguous() or t.device != self.device:
+                raise ValueError("state tensors must be contiguous float32 on the solver device")
+        if contacts.rigid_contact_max > self.rigid_contact_max:
+            raise ValueError(f"contacts hold {contacts.rigid_contact_max} slots, more than "
+                             f"rigid_contact_max={self.rigid_contact_max} given at construction")
+        self._in.body_q, self._in.body_qd, self._in.body_f = (
+            state_in.body_q.data_ptr(), state_in.body_qd.data_ptr(), state_in.body_f.data_ptr())
+        self._out.body_q, self._out.body_qd = state_out.body_q.data_ptr(), state_out.body_qd.data_ptr()
+        self._ctl.joint_target_pos = control.joint_target_pos.data_ptr()
+        self._ctl.joint_target_vel = control.joint_target_vel.data_ptr()
+        c = self._con
+        c.count = contacts.rigid_contact_count.data_ptr()
+        c.max_contacts = contacts.rigid_contact_max
+        c.point0 = contacts.rigid_contact_point0.data_ptr()
+        c.point1 = contacts.rigid_contact_point1.data_ptr()
+        c.normal = contacts.rigid_contact_normal.data_ptr()
+        c.shape0 = contacts.rigid_contact_shape0.data_ptr()
+        c.shape1 = contacts.rigid_contact_shape1.data_ptr()
+        c.margin0 = contacts.rigid_contact_margin0.data_ptr()
+        c.margin1 = contacts.rigid_contact_margin1.data_ptr()
+        _lib.check(self.lib.ofb_step(self._h, C.byref(self._in), C.byref(self._out), C.byref(self._ctl),
+                                     C.byref(c), C.c_float(dt), self._stream()), "ofb_step")
+        self.timestep += 1
+
+    def reset_timestep_counter(self):
+        """core/base_engine.py:278-279."""
+        self.timestep = 0
+
+    def stats(self):
+        """Per-world diagnostics of the last step (tensors on the device)."""
+        W = self.num_worlds
+        out = {k: torch.zeros(W, dtype=torch.int32, device=self.device)
+               for k in ("nr_iters", "pcr_iters", "best_iter", "flags")}
+        out["merit"] = torch.zeros(W, dtype=torch.float32, device=self.device)
+        _lib.check(self.lib.ofb_get_stats(self._h, _ptr(out["nr_iters"]), _ptr(out["pcr_iters"]),
+                                          _ptr(out["merit"]), _ptr(out["best_iter"]), _ptr(out["flags"]),
+                                          self._stream()), "ofb_get_stats")
+        return out
+
+    def constraint_impulses(self):
+        """lambda [W, N_c] of the picked iterate (reference row layout)."""
+        lam = torch.zeros(self.num_worlds, self.num_constraints, dtype=torch.float32, device=self.device)
+        _lib.check(self.lib.ofb_get_lambda(self._h, _ptr(lam), self._stream()), "ofb_get_lambda")
+        return lam
+
+    def set_lambda_seed(self, seed):
+        seed = seed.to(self.device, torch.float32).contiguous()
+        _lib.check(self.lib.ofb_set_lambda_seed(self._h, _ptr(seed), self._stream()), "ofb_set_lambda_seed")
+        self._seed_keepalive = seed
+
+    def packed_contacts(self):
+        W, K = self.num_worlds, self.config.contacts.max_per_world
+        d = self.device
+        out = dict(count=torch.zeros(W, dtype=torch.int32, device=d),
+                   shape0=torch.zeros(W * K, dtype=torch.int32, device=d),
+                   shape1=torch.zeros(W * K, dtype=torch.int32, device=d),
+                   point0=torch.zeros(W * K * 3, dtype=torch.float32, device=d),
+                   point1=torch.zeros(W * K * 3, dtype=torch.float32, device=d),
+                   normal=torch.zeros(W * K * 3, dtype=torch.float32, device=d),
+                   thickness0=torch.zeros(W * K, dtype=torch.float32, device=d),
+                   thickness1=torch.zeros(W * K, dtype=torch.float32, device=d))
+        _lib.check(self.lib.ofb_get_packed_contacts(
+            self._h, *(_ptr(out[k]) for k in ("count", "shape0", "shape1", "point0", "point1", "normal",
+                                               "thickness0", "thickness1")), self._stream()),
+            "ofb_get_packed_contacts")
+        return out
+
+    def step_backward(self):
+        raise NotImplementedError("the adjoint (step_backward) is not built on this solver yet")
