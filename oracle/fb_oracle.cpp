@@ -1238,7 +1238,7 @@ void oracle_integrate_pose(const double* vel6, const double* pose7, const double
 int oracle_linear_system(const ofb_model* m, const ofb_config* cfg, const double* q_in, const double* qd_in,
                          const float* f_in, const float* tpos, const float* tvel, const ofb_contacts* contacts,
                          float dt, const double* x, double* A_dense, double* rhs, double* pinv, double* Ax,
-                         double* active) {
+                         double* active, double* pcr_x, int32_t* pcr_iters) {
   Batch<double> Bt(*m, *cfg);
   Bt.dt = dt;
   pack_contacts(Bt, *contacts);
@@ -1279,6 +1279,19 @@ int oracle_linear_system(const ofb_model* m, const ofb_config* cfg, const double
         }
         A_dense[(size_t(w) * Nc + i) * Nc + j] = val;
       }
+    }
+  }
+  if (pcr_x) {  // PCR on the assembled systems (config preconditioner and limits)
+    std::vector<int> all(W);
+    for (int w = 0; w < W; ++w) {
+      all[w] = w;
+      if (cfg->preconditioner == OFB_PRECOND_PER_BODY_PAIR) pair_update(Bt, w);
+      for (int i = 0; i < Nc; ++i) Bt.dlam[Bt.ri(w, i)] = 0;
+    }
+    pcr_solve(Bt, all, cfg->termination == OFB_TERM_GLOBAL);
+    for (int w = 0; w < W; ++w) {
+      for (int i = 0; i < Nc; ++i) pcr_x[Bt.ri(w, i)] = Bt.dlam[Bt.ri(w, i)];
+      if (pcr_iters) pcr_iters[w] = Bt.pcr_iters[w];
     }
   }
   return 0;

@@ -36,7 +36,7 @@ def lib():
         _lib.oracle_dims.argtypes = [vp, vp, vp, vp, vp]
         _lib.oracle_fb_diff.argtypes = [C.c_double] * 5 + [vp]
         _lib.oracle_integrate_pose.argtypes = [vp, vp, vp, C.c_double, vp]
-        _lib.oracle_linear_system.argtypes = [vp] * 8 + [C.c_float] + [vp] * 6
+        _lib.oracle_linear_system.argtypes = [vp] * 8 + [C.c_float] + [vp] * 8
     return _lib
 
 
@@ -131,9 +131,12 @@ def linear_system(model, config, state, control, contacts, dt, x=None):
     xv = np.zeros((W, nc)) if x is None else np.ascontiguousarray(np.asarray(x, np.float64))
     A = np.zeros((W, nc, nc)); rhs = np.zeros((W, nc)); pinv = np.zeros((W, nc)); Ax = np.zeros((W, nc))
     act = np.zeros((W, nc))
+    px = np.zeros((W, nc))
+    pit = np.zeros(W, np.int32)
     lib().oracle_linear_system(C.byref(hm.abi), C.byref(cfg), _p(q_in), _p(qd_in), _p(f_in), _p(tp), _p(tv),
-                               C.byref(hc.abi), C.c_float(dt), _p(xv), _p(A), _p(rhs), _p(pinv), _p(Ax), _p(act))
-    return dict(A=A, rhs=rhs, pinv=pinv, Ax=Ax, active=act)
+                               C.byref(hc.abi), C.c_float(dt), _p(xv), _p(A), _p(rhs), _p(pinv), _p(Ax), _p(act),
+                               _p(px), _p(pit))
+    return dict(A=A, rhs=rhs, pinv=pinv, Ax=Ax, active=act, pcr_x=px, pcr_iters=pit)
 
 
 def fb_diff(a, b, alpha, beta, eps_sq):

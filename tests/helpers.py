@@ -1,0 +1,54 @@
+"""Scene and config helpers shared by the tests."""
+import numpy as np
+import torch
+
+from ostrich_amd import collide
+from ostrich_amd.config import (ComplianceConfig, ContactsConfig, ExecutionConfig, LinearSolverConfig,
+                                NewtonRaphsonConfig, OstrichEngineConfig)
+from ostrich_amd.model import (DOF_TARGET_POSITION, DOF_TARGET_VELOCITY, GEO_BOX, GEO_PLANE, IDENTITY7, Model,
+                               Shape, WorldBuilder, box_inertia, cylinder_inertia_y)
+
+
+def config(nr_iters=20, atol=1e-5, pcr_iters=16, tol=1e-5, patol=1e-5, K=16, termination="per_world",
+           precond="jacobi", **kw):
+    return OstrichEngineConfig(
+        nr=NewtonRaphsonConfig(max_iters=nr_iters, atol=atol),
+        linear=LinearSolverConfig(max_iters=pcr_iters, tol=tol, atol=patol, preconditioner_type=precond),
+        contacts=ContactsConfig(max_per_world=K),
+        execution=ExecutionConfig(termination=termination, path=kw.pop("path", "fused")),
+        **kw)
+
+
+def zero_control(model):
+    n = model.num_worlds * model.dofs_per_world
+    return dict(joint_target_pos=np.zeros(n, np.float32), joint_target_vel=np.zeros(n, np.float32))
+
+
+def no_contacts(model, cap=1):
+    return dict(count=np.zeros(1, np.int32), max_contacts=cap, point0=np.zeros((cap, 3), np.float32),
+                point1=np.zeros((cap, 3), np.float32), normal=np.zeros((cap, 3), np.float32),
+                shape0=np.zeros(cap, np.int32), shape1=np.zeros(cap, np.int32),
+                margin0=np.zeros(cap, np.float32), margin1=np.zeros(cap, np.float32))
+
+
+def single_box(mass=1.0, h=0.25, mu=0.5, ground_mu=None, pose=(0, 0, 0.25, 0, 0, 0, 1), **shape_kw):
+    wb = WorldBuilder()
+    b = wb.add_body(mass, box_inertia(mass, h, h, h))
+    wb.add_shape(Shape(GEO_BOX, b, IDENTITY7, (h, h, h), mu=mu, **shape_kw))
+    wb.add_joint_free(b)
+    gmu = mu if ground_mu is None else ground_mu
+    model = Model.from_worlds([wb], [Shape(GEO_PLANE, -1, IDENTITY7, (0, 0, 0), mu=gmu)])
+    state = dict(body_q=np.array([pose], np.float64), body_qd=np.zeros((1, 6)), body_f=np.zeros((1, 6)))
+    return model, state
+
+
+def rollout(oracle, model, cfg, state, control, steps, dt, gap=0.02, K=16, precision=64):
+    col = collide.PrimitiveCollider(model, gap=gap, max_contacts_per_world=K)
+    st = {k: np.asarray(v, np.float64).copy() for k, v in state.items()}
+    outs = []
+    for _ in range(steps):
+        c = col.collide(torch.as_tensor(st["body_q"], dtype=torch.float32))
+        o = oracle.step(model, cfg, st, control, c, dt, precision=precision)
+        st["body_q"], st["body_qd"] = o["body_q"], o["body_qd"]
+        outs.append(o)
+    return st, outs
