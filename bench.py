@@ -1,0 +1,196 @@
+#!/usr/bin/env python
+"""World-steps/s of the Fisher-Burmeister Newton contact solver on one MI355X.
+
+Workload: the uneven-batch skid-steer scene - 4096 worlds, each a 10 kg chassis with three
+velocity-driven wheels, 0-24 static boxes, mu U(0.4,1.0), dt 0.05 s, Newton <= 20
+iterations to ||h|| <= 1e-5, PCR <= 16 iterations (Jacobi).  A step is one full solver
+step (contact packing + Newton loop + backtracking) over all worlds; contacts are
+generated between steps by the primitive collider on the GPU and are not timed, as the
+reference times collision separately (profiling/engine_profiler.py:46-53).
+
+Prints ONE JSON line.  `value` is whole-job world-steps/s of the default path over the
+timed steps, computed from HIP events around each step on the solver's stream.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+import torch
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from ostrich_amd import collide, scenes  # noqa: E402
+from ostrich_amd.config import (ContactsConfig, ExecutionConfig, LinearSolverConfig,  # noqa: E402
+                                NewtonRaphsonConfig, OstrichEngineConfig)
+from ostrich_amd.containers import Contacts, Control, State  # noqa: E402
+
+DT = 0.05
+K_CONTACTS = 128
+HBM_PEAK_GBS = 8000.0
+
+
+def solver_config(path, termination="per_world"):
+    return OstrichEngineConfig(
+        nr=NewtonRaphsonConfig(max_iters=20, atol=1e-5),
+        linear=LinearSolverConfig(max_iters=16),
+        contacts=ContactsConfig(max_per_world=K_CONTACTS),
+        execution=ExecutionConfig(path=path, termination=termination),
+    )
+
+
+def compulsory_bytes_per_world(model, n_c, n_rows):
+    """Bytes the fused kernel must move per world-step at minimum: bodies read
+    (pose 28, velocity 24, wrench 24, masses 8, inertias 72, com 12) and written (52),
+    joints 80 each, dofs 32 each (axis, mode, gains, targets), shapes 36 each, packed
+    contacts 52 each, lambda out 4 per row of the reference layout, stats 20."""
+    B, J, D = model.bodies_per_world, model.joints_per_world, model.dofs_per_world
+    S = model.shapes_per_world + model.num_global_shapes
+    return 220 * B + 80 * J + 32 * D + 36 * S + 52 * n_c + 4 * n_rows + 20
+
+
+def operand_bytes_per_world(n_rows_active, nr_iters, pcr_iters):
+    """Operand bytes the algorithm touches per world-step (read from on-chip memory in
+    the fused kernel): 400 B per active row per Newton iteration (assembly, RHS, update,
+    residual operands) and 304 B per active row per PCR iteration (matvec 224, Jacobi 12,
+    three dots 20... see the design notes for the count)."""
+    return n_rows_active * (400.0 * nr_iters + 304.0 * pcr_iters)
+
+
+def run_path(path, model, state, control, steps, warmup, device, collider):
+    from ostrich_amd import OstrichEngine
+    cfg = solver_config(path)
+    eng = OstrichEngine(model, config=cfg, device=device)
+    s_a = State.from_numpy(state, device)
+    s_b = s_a.clone()
+    ctl = Control.from_numpy(control, device)
+    stream = torch.cuda.current_stream(device)
+    W = model.num_worlds
+    step_ms, solve_ms, nr, pcr, conv, ncon = [], [], [], [], [], []
+    for k in range(warmup + steps):
+        con = Contacts.from_dict(collider.collide(s_a.body_q), device)
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        eng.step(s_a, s_b, ctl, con, DT)
+        e1.record(stream)
+        e1.synchronize()
+        if k >= warmup:
+            step_ms.append(e0.elapsed_time(e1))
+            solve_ms.append(eng.last_solve_ms())
+            st = eng.stats()
+            nr.append(st["nr_iters"].float().mean().item())
+            pcr.append(st["pcr_iters"].float().mean().item())
+            conv.append((st["merit"] <= cfg.nr.atol ** 2).float().mean().item())
+            ncon.append(eng.packed_contacts()["count"].clamp(max=K_CONTACTS).float().mean().item())
+        s_a, s_b = s_b, s_a
+    torch.cuda.synchronize()
+    return dict(step_ms=float(np.sum(step_ms)), solve_ms=float(np.sum(solve_ms)), steps=steps,
+                world_steps_per_s=W * steps / (np.sum(step_ms) / 1e3),
+                ms_per_step=float(np.mean(step_ms)), kernel_ms=float(np.mean(solve_ms)),
+                nr_mean=float(np.mean(nr)), pcr_per_step_mean=float(np.mean(pcr)),
+                converged_fraction=float(np.mean(conv)), contacts_mean=float(np.mean(ncon)))
+
+
+def cpu_baseline(model_fn, sample_worlds, sample_steps):
+    """The oracle (FP32 CPU restatement, one thread) on the first `sample_worlds` worlds
+    of the same seeded scene; only the solver calls are timed."""
+    from oracle import oracle
+    model, state, control = model_fn(sample_worlds)
+    cfg = solver_config("fused")
+    col = collide.PrimitiveCollider(model, device="cpu", gap=0.02, max_contacts_per_world=K_CONTACTS)
+    hm = oracle.HostModel(model)
+    st = {k: np.asarray(v, np.float64) for k, v in state.items()}
+    t = 0.0
+    for _ in range(sample_steps):
+        c = col.collide(torch.as_tensor(st["body_q"], dtype=torch.float32))
+        hc = oracle.HostContacts(c)
+        t0 = time.perf_counter()
+        o = oracle.step(model, cfg, st, control, hc, DT, precision=32, host_model=hm)
+        t += time.perf_counter() - t0
+        st["body_q"], st["body_qd"] = o["body_q"], o["body_qd"]
+    return dict(value=sample_worlds * sample_steps / t, unit="world-steps/s", cores=1, kind="port",
+                sample=f"oracle FP32 (C++ CPU restatement, 1 thread), first {sample_worlds} worlds of the "
+                       f"same seeded scene, {sample_steps} closed-loop steps, solver calls only ({t:.1f} s)")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--worlds", type=int, default=4096)
+    ap.add_argument("--paths", default="fused,plain")
+    ap.add_argument("--cpu-sample-worlds", type=int, default=64)
+    ap.add_argument("--cpu-sample-steps", type=int, default=40)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: worlds are independent, so the solver shards by world range "
+              "with no exchange; multi-GPU runs are not part of this build", file=sys.stderr)
+        sys.exit(2)
+    device = "cuda:0"
+    W = args.worlds
+    model, state, control = scenes.skid_steer(W, seed=1)
+    collider = collide.PrimitiveCollider(model, device=device, gap=0.02, max_contacts_per_world=K_CONTACTS)
+    paths = [p for p in args.paths.split(",") if p]
+    results = {}
+    for p in paths:
+        print(f"[bench] path={p} worlds={W} warmup={args.warmup} steps={args.steps}", file=sys.stderr, flush=True)
+        try:
+            results[p] = run_path(p, model, state, control, args.steps, args.warmup, device, collider)
+        except RuntimeError as e:
+            results[p] = dict(error=str(e))
+        print(f"[bench] {p}: {results[p]}", file=sys.stderr, flush=True)
+    main_path = paths[0]
+    r = results[main_path]
+    n_c = r["contacts_mean"]
+    from ostrich_amd.engine import OstrichEngine  # noqa: F401  (library loaded above)
+    nj, nctrl = 15, 3
+    n_rows_ref = nj + nctrl + 3 * K_CONTACTS
+    n_rows_active = nj + nctrl + 3 * n_c
+    bws = compulsory_bytes_per_world(model, n_c, n_rows_ref)
+    aws = operand_bytes_per_world(n_rows_active, r["nr_mean"], r["pcr_per_step_mean"])
+    achieved = W * bws / (r["kernel_ms"] / 1e3) / 1e9
+    lds_peak = 256 * 128 * 2.4e9 / 1e9  # 256 CUs x 128 B/clk x 2.4 GHz
+    out = {
+        "metric": "world-steps/s at FB residual<=1e-5 (fused persistent solver)",
+        "value": r["world_steps_per_s"],
+        "unit": "world-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded skid-steer scene, primitive collider)",
+        "config": {"workload": "uneven-batch skid-steer, 4096 worlds, dt 0.05, NR<=20 to ||h||<=1e-5, "
+                               "PCR<=16 Jacobi, 128 contact slots", "worlds": W, "dt": DT,
+                   "path": main_path, "termination": "per_world"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "fused_step_kernel", "kernel_ms": r["kernel_ms"],
+                     "bytes_per_world_step": bws,
+                     "note": "compulsory HBM bytes; the fused kernel is latency/LDS bound",
+                     "operand_bytes_per_world_step": aws,
+                     "operand_GBs": W * aws / (r["kernel_ms"] / 1e3) / 1e9, "lds_peak_GBs": lds_peak},
+        "solver": {"nr_mean": r["nr_mean"], "pcr_per_step_mean": r["pcr_per_step_mean"],
+                   "converged_fraction": r["converged_fraction"], "contacts_mean": n_c},
+        "paths": {p: results[p] for p in paths},
+    }
+    if "plain" in results and "world_steps_per_s" in results.get("plain", {}):
+        out["fused_over_plain"] = r["world_steps_per_s"] / results["plain"]["world_steps_per_s"]
+    if not args.no_cpu_baseline:
+        print("[bench] cpu baseline", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(lambda n: scenes.skid_steer(n, seed=1), args.cpu_sample_worlds,
+                                           args.cpu_sample_steps)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -209,6 +209,11 @@ int ofb_get_packed_contacts(const ofb_solver* solver, int32_t* count, int32_t* s
  * learned warm start. */
 int ofb_set_lambda_seed(ofb_solver* solver, const float* lambda_seed, void* stream);
 
+/* Duration in milliseconds of the last step's solve (the fused kernel, or the plain
+ * path's Newton loop), measured with HIP events recorded on the step's stream around
+ * those launches; contact packing is excluded. Blocks until that work has finished. */
+int ofb_last_solve_ms(const ofb_solver* solver, float* ms);
+
 /* Last error message of the calling thread ("" if none). */
 const char* ofb_last_error(void);
 

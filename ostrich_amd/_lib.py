@@ -10,7 +10,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "lib
 
 EXPORTS = (
     "ofb_create", "ofb_destroy", "ofb_dims", "ofb_step", "ofb_get_stats", "ofb_get_lambda",
-    "ofb_get_packed_contacts", "ofb_set_lambda_seed", "ofb_last_error",
+    "ofb_get_packed_contacts", "ofb_set_lambda_seed", "ofb_last_solve_ms", "ofb_last_error",
 )
 
 _lib = None
@@ -32,6 +32,7 @@ def load():
     lib.ofb_get_lambda.argtypes = [vp, vp, vp]
     lib.ofb_get_packed_contacts.argtypes = [vp] * 9 + [vp]
     lib.ofb_set_lambda_seed.argtypes = [vp, vp, vp]
+    lib.ofb_last_solve_ms.argtypes = [vp, C.POINTER(C.c_float)]
     lib.ofb_last_error.restype = C.c_char_p
     for name in EXPORTS:
         if name != "ofb_last_error":

@@ -145,6 +145,25 @@ static int pack_contacts(ofb_solver* s, const ofb_contacts* c, hipStream_t strea
   return 0;
 }
 
+// component-major packed contacts -> row-major [W*K(*3)] arrays (diagnostics)
+__global__ void unpack_contacts_kernel(int W, int K, const float* __restrict__ cf, const int32_t* __restrict__ ci,
+                                       int32_t* shape0, int32_t* shape1, float* point0, float* point1, float* normal,
+                                       float* th0, float* th1) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= W * K) return;
+  int w = t / K, c = t % K;
+  const float* f = cf + (size_t)w * CF * K + c;
+  if (shape0) shape0[t] = ci[(size_t)w * 2 * K + c];
+  if (shape1) shape1[t] = ci[(size_t)w * 2 * K + K + c];
+  if (th0) th0[t] = f[CF_TH0 * K];
+  if (th1) th1[t] = f[CF_TH1 * K];
+  for (int d = 0; d < 3; ++d) {
+    if (point0) point0[t * 3 + d] = f[(CF_P0X + d) * K];
+    if (point1) point1[t * 3 + d] = f[(CF_P1X + d) * K];
+    if (normal) normal[t * 3 + d] = f[(CF_NX + d) * K];
+  }
+}
+
 template <class T>
 static int dev_copy(T** dst, const T* src, size_t n) {
   *dst = nullptr;
@@ -185,6 +204,7 @@ int ofb_destroy(ofb_solver* s) {
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (s->plain) plain_destroy(s);
+  if (s->events_ready) { (void)hipEventDestroy(s->ev_solve0); (void)hipEventDestroy(s->ev_solve1); }
   delete s;
   return 0;
 }
@@ -302,6 +322,12 @@ int ofb_create(const ofb_model* model, const ofb_config* config, int32_t path, o
   s->pk_temp_bytes = tb;
   if (hipMalloc(&s->pk_temp, tb > 0 ? tb : 1) != hipSuccess) { set_error("hipMalloc failed"); ofb_destroy(s); return -4; }
   s->fused_worlds_per_block = 4;
+  if (hipEventCreate(&s->ev_solve0) != hipSuccess || hipEventCreate(&s->ev_solve1) != hipSuccess) {
+    set_error("hipEventCreate failed");
+    ofb_destroy(s);
+    return -4;
+  }
+  s->events_ready = 1;
   if (path == OFB_PATH_PLAIN) {
     if ((rc = plain_create(s)) != 0) { ofb_destroy(s); return rc; }
   }
@@ -330,8 +356,18 @@ int ofb_step(ofb_solver* s, const ofb_state_in* in, const ofb_state_out* out, co
   hipStream_t st = (hipStream_t)stream;
   int rc = pack_contacts(s, contacts, st);
   if (rc) return rc;
-  if (s->path == OFB_PATH_FUSED) return fused_step(s, in, out, ctl, dt, st);
-  return plain_step(s, in, out, ctl, dt, st);
+  OFB_CHECK(hipEventRecord(s->ev_solve0, st));
+  rc = s->path == OFB_PATH_FUSED ? fused_step(s, in, out, ctl, dt, st) : plain_step(s, in, out, ctl, dt, st);
+  if (rc) return rc;
+  OFB_CHECK(hipEventRecord(s->ev_solve1, st));
+  return 0;
+}
+
+int ofb_last_solve_ms(const ofb_solver* s, float* ms) {
+  if (!s || !ms) { set_error("null argument"); return -1; }
+  OFB_CHECK(hipEventSynchronize(s->ev_solve1));
+  OFB_CHECK(hipEventElapsedTime(ms, s->ev_solve0, s->ev_solve1));
+  return 0;
 }
 
 int ofb_get_stats(const ofb_solver* s, int32_t* nr_iters, int32_t* pcr_iters, float* merit, int32_t* best_iter,
@@ -367,20 +403,10 @@ int ofb_get_packed_contacts(const ofb_solver* s, int32_t* count, int32_t* shape0
   hipStream_t st = (hipStream_t)stream;
   const int W = s->m.W, K = s->L.K;
   if (count) OFB_CHECK(hipMemcpyAsync(count, s->ccount, (size_t)W * 4, hipMemcpyDeviceToDevice, st));
-  // component-major blocks -> [W*K*3] / [W*K] rows
-  for (int w = 0; w < W; ++w) {
-    const float* f = s->cf + (size_t)w * CF * K;
-    const int32_t* ii = s->ci + (size_t)w * 2 * K;
-    if (shape0) OFB_CHECK(hipMemcpyAsync(shape0 + (size_t)w * K, ii, K * 4, hipMemcpyDeviceToDevice, st));
-    if (shape1) OFB_CHECK(hipMemcpyAsync(shape1 + (size_t)w * K, ii + K, K * 4, hipMemcpyDeviceToDevice, st));
-    if (thickness0) OFB_CHECK(hipMemcpyAsync(thickness0 + (size_t)w * K, f + CF_TH0 * K, K * 4, hipMemcpyDeviceToDevice, st));
-    if (thickness1) OFB_CHECK(hipMemcpyAsync(thickness1 + (size_t)w * K, f + CF_TH1 * K, K * 4, hipMemcpyDeviceToDevice, st));
-    for (int d = 0; d < 3; ++d) {
-      if (point0) OFB_CHECK(hipMemcpy2DAsync(point0 + (size_t)w * K * 3 + d, 12, f + (CF_P0X + d) * K, 4, 4, K, hipMemcpyDeviceToDevice, st));
-      if (point1) OFB_CHECK(hipMemcpy2DAsync(point1 + (size_t)w * K * 3 + d, 12, f + (CF_P1X + d) * K, 4, 4, K, hipMemcpyDeviceToDevice, st));
-      if (normal) OFB_CHECK(hipMemcpy2DAsync(normal + (size_t)w * K * 3 + d, 12, f + (CF_NX + d) * K, 4, 4, K, hipMemcpyDeviceToDevice, st));
-    }
-  }
+  const int n = W * K, T = 256;
+  hipLaunchKernelGGL(unpack_contacts_kernel, dim3((n + T - 1) / T), dim3(T), 0, st, W, K, s->cf, s->ci, shape0,
+                     shape1, point0, point1, normal, thickness0, thickness1);
+  OFB_CHECK(hipGetLastError());
   return 0;
 }
 

@@ -26,34 +26,14 @@
 #include <hip/hip_runtime.h>
 
 #include "fb_math.h"
+#include "fb_rows.h"
 #include "fb_solver.h"
 
 namespace ofb {
 
+using namespace rows;
+
 namespace {
-
-constexpr int WAVE = 64;
-constexpr int BS = 97;  // LDS floats per body (odd stride spreads bodies over banks)
-
-// body block offsets (floats)
-enum {
-  BQ = 0,       // pose (7)
-  BU = 7,       // velocity (6)
-  BQP = 13,     // pose at step start (7)
-  BUP = 20,     // velocity at step start (6)
-  BF = 26,      // external wrench (6)
-  BHD = 32,     // dynamics residual h_d (6)
-  BIW = 38,     // world inverse inertia (9)
-  BACC = 47,    // J^T x accumulator (6)
-  BVV = 53,     // M^-1 acc (6)
-  BBQ = 59,     // best pose (7)
-  BBU = 66,     // best velocity (6)
-  BM = 72,      // mass
-  BMINV = 73,   // inverse mass
-  BCOM = 74,    // com (3)
-  BIB = 77,     // body inertia (9)
-  BIIB = 86,    // body inverse inertia (9)
-};
 
 struct FusedParams {
   DeviceModel m;
@@ -77,498 +57,6 @@ struct FusedParams {
   float* st_merit;
   int lds_per_wave;  // floats
 };
-
-__device__ __forceinline__ void wsync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// butterfly sum: every lane ends with the bitwise-identical total (each step adds the
-// same two operands in both partner lanes, and fp addition is commutative)
-__device__ __forceinline__ float wave_sum(float v) {
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, WAVE);
-  return v;
-}
-
-__device__ __forceinline__ v3 ld3(const float* p) { return mk(p[0], p[1], p[2]); }
-__device__ __forceinline__ tf ldtf(const float* p) { return tf{mk(p[0], p[1], p[2]), qt{p[3], p[4], p[5], p[6]}}; }
-__device__ __forceinline__ sv ldsv(const float* p) { return sv{mk(p[0], p[1], p[2]), mk(p[3], p[4], p[5])}; }
-__device__ __forceinline__ m3 ldm3(const float* p) {
-  m3 r;
-#pragma unroll
-  for (int i = 0; i < 9; ++i) r.m[i] = p[i];
-  return r;
-}
-__device__ __forceinline__ void sttf(float* p, const tf& x) {
-  p[0] = x.p.x; p[1] = x.p.y; p[2] = x.p.z; p[3] = x.q.x; p[4] = x.q.y; p[5] = x.q.z; p[6] = x.q.w;
-}
-__device__ __forceinline__ void stsv(float* p, const sv& x) {
-  p[0] = x.a.x; p[1] = x.a.y; p[2] = x.a.z; p[3] = x.b.x; p[4] = x.b.y; p[5] = x.b.z;
-}
-__device__ __forceinline__ void atomic_add_sv(float* p, const sv& x) {
-  atomicAdd(p + 0, x.a.x); atomicAdd(p + 1, x.a.y); atomicAdd(p + 2, x.a.z);
-  atomicAdd(p + 3, x.b.x); atomicAdd(p + 4, x.b.y); atomicAdd(p + 5, x.b.z);
-}
-// M^-1 v with M^-1 = diag(m^-1 I3, I_w^-1)
-__device__ __forceinline__ sv apply_minv(const float* body, const sv& v) {
-  m3 Iw = ldm3(body + BIW);
-  return sv{v.a * body[BMINV], mv(Iw, v.b)};
-}
-
-// ---------------------------------------------------------------------------------
-// Per-row computations.  Each returns the linearisation (Jhat pair, C, active) and the
-// residual h_c, and scatters -Jhat^T lambda dt into the dynamics residual.
-// ---------------------------------------------------------------------------------
-struct RowOut {
-  sv J0, J1;
-  int b0, b1;
-  float hc, C;
-  bool active;
-};
-
-// joint rows (constraints/joint_constraint.py:62-162, joint_kinematics.py:5-261)
-__device__ void joint_frames(const FusedParams& P, const float* sm, int w, int j, tf& Xwp, tf& Xwc, v3& r_p,
-                             v3& r_c, v3& pos_p, v3& pos_c, v3& com_p) {
-  const int J = P.m.J;
-  int p_ = P.m.joint_parent[w * J + j], c_ = P.m.joint_child[w * J + j];
-  const float* bc = sm + c_ * BS;
-  tf Xc_l = ldtf(P.m.joint_X_c + (size_t)(w * J + j) * 7);
-  tf bq = ldtf(bc + BQ);
-  Xwc = tmul(bq, Xc_l);
-  pos_c = Xwc.p;
-  r_c = pos_c - tpoint(bq, ld3(bc + BCOM));
-  tf Xp_l = ldtf(P.m.joint_X_p + (size_t)(w * J + j) * 7);
-  tf bp = tident();
-  com_p = mk(0.f, 0.f, 0.f);
-  if (p_ >= 0) { bp = ldtf(sm + p_ * BS + BQ); com_p = ld3(sm + p_ * BS + BCOM); }
-  Xwp = tmul(bp, Xp_l);
-  pos_p = Xwp.p;
-  r_p = pos_p - tpoint(bp, com_p);
-}
-
-__device__ RowOut joint_row(const FusedParams& P, float* sm, int w, int row, float lam, bool full) {
-  RowOut o;
-  o.J0 = svzero(); o.J1 = svzero(); o.hc = 0.f; o.C = 0.f; o.active = false;
-  const int J = P.m.J;
-  int j = P.topo[row];
-  int k = P.topo[P.L.Nj + row];
-  o.b0 = P.m.joint_parent[w * J + j];
-  o.b1 = P.m.joint_child[w * J + j];
-  if (P.m.joint_enabled[w * J + j] == 0 || o.b1 < 0) return o;
-  int type = P.m.joint_type[w * J + j];
-  if (type < 0 || type > 3) return o;
-  float comp_ = P.m.joint_compliance[w * J + j];
-  if (comp_ < 0.f) comp_ = P.cfg.compliance_joint;
-  tf Xwp, Xwc; v3 r_p, r_c, pos_p, pos_c, com_p;
-  joint_frames(P, sm, w, j, Xwp, Xwc, r_p, r_c, pos_p, pos_c, com_p);
-  v3 ax = mk(0.f, 0.f, 0.f);
-  if (type == 0 || type == 1) ax = ld3(P.m.joint_axis + (size_t)(w * P.m.D + P.m.joint_qd_start[w * J + j]) * 3);
-  sv Jp = svzero(), Jc = svzero();
-  float err = 0.f;
-  bool act = false;
-  // compute_joint_row (joint_kinematics.py:191-261)
-  int mode = -1, idx = 0;  // 0 linear, 1 angular, 2 revolute-ortho, 3 prismatic-ortho
-  if (type == 0) { if (k < 2) { mode = 3; idx = k; } else if (k < 5) { mode = 1; idx = k - 2; } }
-  else if (type == 1) { if (k < 3) { mode = 0; idx = k; } else if (k < 5) { mode = 2; idx = k - 3; } }
-  else if (type == 2) { if (k < 3) { mode = 0; idx = k; } }
-  else if (type == 3) { if (k < 3) { mode = 0; idx = k; } else if (k < 6) { mode = 1; idx = k - 3; } }
-  if (mode == 0) {  // get_linear_component :43-74
-    v3 a = unit(idx);
-    Jc = sv{a, cross(r_c, a)};
-    Jp = sv{-a, -cross(r_p, a)};
-    err = comp(pos_c - pos_p, idx);
-    act = true;
-  } else if (mode == 1) {  // get_angular_component :77-113
-    v3 aw = qrot(Xwp.q, unit(idx));
-    Jc = sv{mk(0.f, 0.f, 0.f), aw};
-    Jp = sv{mk(0.f, 0.f, 0.f), -aw};
-    qt qr = qmul(qconj(Xwp.q), Xwc.q);
-    float e = 2.f * (idx == 0 ? qr.x : (idx == 1 ? qr.y : qr.z));
-    err = qr.w < 0.f ? -e : e;
-    act = true;
-  } else if (mode == 2) {  // get_revolute_angular_component :116-147
-    v3 b1, b2;
-    orthogonal_basis(ax, b1, b2);
-    v3 axc = qrot(Xwc.q, ax);
-    v3 tgt = qrot(Xwp.q, idx == 0 ? b1 : b2);
-    err = dot(axc, tgt);
-    v3 ra = cross(axc, tgt);
-    Jc = sv{mk(0.f, 0.f, 0.f), ra};
-    Jp = sv{mk(0.f, 0.f, 0.f), -ra};
-    act = true;
-  } else if (mode == 3) {  // get_prismatic_linear_component :150-183
-    v3 b1, b2;
-    orthogonal_basis(ax, b1, b2);
-    v3 nv = qrot(Xwp.q, idx == 0 ? b1 : b2);
-    err = dot(pos_c - pos_p, nv);
-    Jc = sv{nv, cross(r_c, nv)};
-    Jp = sv{-nv, cross(nv, pos_c - com_p)};  // com_p is the body-frame com, as the reference passes it
-    act = true;
-  }
-  if (!act) return o;
-  o.active = true;
-  float dt = P.dt;
-  if (o.b0 >= 0) atomic_add_sv(sm + o.b0 * BS + BHD, Jp * (-lam * dt));
-  atomic_add_sv(sm + o.b1 * BS + BHD, Jc * (-lam * dt));
-  o.hc = (err + comp_ * lam) / dt;
-  o.J0 = Jp; o.J1 = Jc;
-  o.C = comp_ / (dt * dt);
-  return o;
-}
-
-// control rows (constraints/control_constraint.py:117-338)
-__device__ RowOut control_row(const FusedParams& P, float* sm, int w, int crow, float lam) {
-  RowOut o;
-  o.J0 = svzero(); o.J1 = svzero(); o.hc = 0.f; o.C = 0.f; o.active = false;
-  const int J = P.m.J, D = P.m.D;
-  int j = P.topo[2 * P.L.Nj + crow];
-  o.b0 = P.m.joint_parent[w * J + j];
-  o.b1 = P.m.joint_child[w * J + j];
-  if (P.m.joint_enabled[w * J + j] == 0) return o;
-  int type = P.m.joint_type[w * J + j];
-  if (type != 0 && type != 1) return o;
-  int qs = P.m.joint_qd_start[w * J + j];
-  int mode = P.m.joint_dof_mode[w * D + qs];
-  if (mode == 0 || o.b1 < 0) return o;
-  tf Xwp, Xwc; v3 r_p, r_c, pos_p, pos_c, com_p;
-  joint_frames(P, sm, w, j, Xwp, Xwc, r_p, r_c, pos_p, pos_c, com_p);
-  v3 ax = ld3(P.m.joint_axis + (size_t)(w * D + qs) * 3);
-  sv up = svzero();
-  if (o.b0 >= 0) up = ldsv(sm + o.b0 * BS + BU);
-  sv uc = ldsv(sm + o.b1 * BS + BU);
-  float target = 0.f;
-  if (mode == 1) target = P.target_pos[w * D + qs];
-  else if (mode == 2) target = P.target_vel[w * D + qs];
-  float ke = P.m.joint_target_ke[w * D + qs], kd = P.m.joint_target_kd[w * D + qs];
-  float dt = P.dt;
-  float qv, qdv;
-  sv Jp, Jc;
-  bool angular;
-  if (type == 1) {
-    qt qr = qmul(qconj(Xwp.q), Xwc.q);
-    float sh = qr.x * ax.x + qr.y * ax.y + qr.z * ax.z;
-    qv = 2.f * atan2f(sh, qr.w);
-    v3 aw = qrot(Xwp.q, ax);
-    qdv = dot(uc.b - up.b, aw);
-    Jc = sv{mk(0.f, 0.f, 0.f), aw};
-    Jp = sv{mk(0.f, 0.f, 0.f), -aw};
-    angular = true;
-  } else {
-    v3 aw = qrot(Xwp.q, ax);
-    v3 delta = pos_c - pos_p;
-    qv = dot(delta, aw);
-    v3 vc = uc.a + cross(uc.b, r_c);
-    v3 vp = up.a + cross(up.b, r_p);
-    qdv = dot(vc - vp, aw) + dot(delta, cross(up.b, aw));
-    Jc = sv{aw, cross(r_c, aw)};
-    Jp = sv{-aw, -cross(pos_c - com_p, aw)};
-    angular = false;
-  }
-  float err = 0.f, alpha = 0.f;
-  if (mode == 1) {
-    float raw = qv - target;
-    if (angular) {
-      const float PI = 3.141592653589793f, TWO_PI = 6.283185307179586f;
-      float s = raw + PI;
-      err = (s - TWO_PI * floorf(s / TWO_PI)) - PI;
-    } else {
-      err = raw;
-    }
-    err = err / dt;
-    float den = dt * dt * ke + dt * kd;
-    alpha = den > 1e-6f ? 1.f / den : 1.0e8f;
-  } else if (mode == 2) {
-    err = qdv - target;
-    float den = dt * ke;
-    alpha = den > 1e-6f ? 1.f / den : 1.0e8f;
-  }
-  o.active = true;
-  if (o.b0 >= 0) atomic_add_sv(sm + o.b0 * BS + BHD, Jp * (-lam * dt));
-  atomic_add_sv(sm + o.b1 * BS + BHD, Jc * (-lam * dt));
-  o.hc = err + alpha * lam * dt;
-  o.J0 = Jp; o.J1 = Jc;
-  o.C = alpha;
-  return o;
-}
-
-struct ContactGeo {
-  v3 p0, p1, n;
-  float th0, th1;
-  int s0, s1, b0, b1;
-};
-
-__device__ __forceinline__ ContactGeo load_contact(const FusedParams& P, int w, int c) {
-  const int K = P.L.K;
-  const float* f = P.cf + (size_t)w * CF * K + c;
-  ContactGeo g;
-  g.p0 = mk(f[CF_P0X * K], f[CF_P0Y * K], f[CF_P0Z * K]);
-  g.p1 = mk(f[CF_P1X * K], f[CF_P1Y * K], f[CF_P1Z * K]);
-  g.n = mk(f[CF_NX * K], f[CF_NY * K], f[CF_NZ * K]);
-  g.th0 = f[CF_TH0 * K];
-  g.th1 = f[CF_TH1 * K];
-  const int32_t* ii = P.ci + (size_t)w * 2 * K + c;
-  g.s0 = ii[0];
-  g.s1 = ii[K];
-  const int SC = P.m.SC;
-  g.b0 = g.s0 >= 0 ? P.m.shape_body[w * SC + g.s0] : -1;
-  g.b1 = g.s1 >= 0 ? P.m.shape_body[w * SC + g.s1] : -1;
-  return g;
-}
-
-// effective mass J M^-1 J^T with the inverse inertia rotated by the step-start pose
-// (constraints/utils.py:6-33)
-__device__ __forceinline__ float eff_mass_prev(const float* sm, int b0, int b1, const sv& J0, const sv& J1) {
-  float val = 0.f;
-  if (b0 >= 0) {
-    const float* bb = sm + b0 * BS;
-    m3 Iw = rot_inertia(qt{bb[BQP + 3], bb[BQP + 4], bb[BQP + 5], bb[BQP + 6]}, ldm3(bb + BIIB));
-    val += dot(J0.a, J0.a * bb[BMINV]) + dot(J0.b, mv(Iw, J0.b));
-  }
-  if (b1 >= 0) {
-    const float* bb = sm + b1 * BS;
-    m3 Iw = rot_inertia(qt{bb[BQP + 3], bb[BQP + 4], bb[BQP + 5], bb[BQP + 6]}, ldm3(bb + BIIB));
-    val += dot(J1.a, J1.a * bb[BMINV]) + dot(J1.b, mv(Iw, J1.b));
-  }
-  return val;
-}
-
-// contact normal row (constraints/contact_constraint.py:48-374)
-__device__ RowOut normal_row(const FusedParams& P, float* sm, int w, int c, float lam) {
-  RowOut o;
-  o.J0 = svzero(); o.J1 = svzero(); o.hc = 0.f; o.C = 0.f; o.active = false; o.b0 = -1; o.b1 = -1;
-  ContactGeo g = load_contact(P, w, c);
-  if (g.s0 == g.s1) return o;
-  o.b0 = g.b0; o.b1 = g.b1;
-  tf q0 = tident(), q0p = tident(), q1 = tident(), q1p = tident();
-  v3 com0 = mk(0.f, 0.f, 0.f), com1 = com0;
-  if (g.b0 >= 0) { const float* bb = sm + g.b0 * BS; q0 = ldtf(bb + BQ); q0p = ldtf(bb + BQP); com0 = ld3(bb + BCOM); }
-  if (g.b1 >= 0) { const float* bb = sm + g.b1 * BS; q1 = ldtf(bb + BQ); q1p = ldtf(bb + BQP); com1 = ld3(bb + BCOM); }
-  v3 p0_adj_prev = tpoint(q0p, g.p0) - g.n * g.th0;
-  v3 p1_adj_prev = tpoint(q1p, g.p1) + g.n * g.th1;
-  v3 p0_adj = tpoint(q0, g.p0) - g.n * g.th0;
-  v3 p1_adj = tpoint(q1, g.p1) + g.n * g.th1;
-  sv Ja = svzero(), Jb = svzero();
-  if (g.b0 >= 0) Ja = sv{g.n, cross(p0_adj_prev - tpoint(q0p, com0), g.n)};
-  if (g.b1 >= 0) Jb = sv{-g.n, cross(p1_adj_prev - tpoint(q1p, com1), -g.n)};
-  float gap = dot(g.n, p0_adj - p1_adj);
-  float dt = P.dt;
-  float precond = dt * dt * eff_mass_prev(sm, g.b0, g.b1, Ja, Jb);
-  float phi, dphi_dg, dphi_dl;
-  if (P.cfg.contact_mode == OFB_CONTACT_EQUALITY) {
-    phi = gap; dphi_dg = 1.f; dphi_dl = 0.f;
-  } else {  // scaled_fisher_burmeister_diff (mechanics/complementarity.py:22-47)
-    float sa = P.cfg.contact_fb_alpha * gap, sb = precond * lam;
-    float nrm = sqrtf(sa * sa + sb * sb + P.cfg.contact_fb_eps_sq);
-    phi = sa + sb - nrm;
-    dphi_dg = P.cfg.contact_fb_alpha * (1.f - sa / nrm);
-    dphi_dl = precond * (1.f - sb / nrm);
-  }
-  o.J0 = Ja * dphi_dg;
-  o.J1 = Jb * dphi_dg;
-  o.active = true;
-  if (g.b0 >= 0) atomic_add_sv(sm + g.b0 * BS + BHD, o.J0 * (-lam * dt));
-  if (g.b1 >= 0) atomic_add_sv(sm + g.b1 * BS + BHD, o.J1 * (-lam * dt));
-  o.hc = phi / dt;
-  o.C = (dphi_dl + P.cfg.compliance_contact) / (dt * dt);
-  return o;
-}
-
-// friction row t in {0,1} of contact c (constraints/friction_constraint.py:79-908)
-__device__ RowOut friction_row(const FusedParams& P, float* sm, const float* lamprev, int w, int c, int t,
-                               int n_c, float lam) {
-  RowOut o;
-  o.J0 = svzero(); o.J1 = svzero(); o.hc = 0.f; o.C = 0.f; o.active = false; o.b0 = -1; o.b1 = -1;
-  ContactGeo g = load_contact(P, w, c);
-  bool skip = g.s0 == g.s1;
-  if (P.cfg.friction_mode == OFB_FRICTION_BILATERAL_PATCH && !skip) {  // bilateral_row_is_skipped :39-76
-    for (int jj = 0; jj < c; ++jj) {
-      const int32_t* ii = P.ci + (size_t)w * 2 * P.L.K + jj;
-      if (ii[0] == g.s0 && ii[P.L.K] == g.s1) { skip = true; break; }
-    }
-  }
-  if (skip) return o;
-  const int SC = P.m.SC;
-  float mu0 = P.m.shape_mu[w * SC + g.s0], mu1 = P.m.shape_mu[w * SC + g.s1];
-  // lambda_prev of this contact: normal row then the two friction rows (compacted index)
-  const int base_n = P.L.Nj + P.L.Nctrl;
-  float fn_prev = lamprev[base_n + c];
-  if (fmaxf(mu0, mu1) * fn_prev <= P.cfg.friction_load_gate) return o;
-  o.b0 = g.b0; o.b1 = g.b1;
-  float l1p = lamprev[base_n + n_c + 2 * c], l2p = lamprev[base_n + n_c + 2 * c + 1];
-  tf q0p = tident(), q1p = tident();
-  sv vel0 = svzero(), vel1 = svzero();
-  v3 com0 = mk(0.f, 0.f, 0.f), com1 = com0;
-  if (g.b0 >= 0) { const float* bb = sm + g.b0 * BS; q0p = ldtf(bb + BQP); vel0 = ldsv(bb + BU); com0 = ld3(bb + BCOM); }
-  if (g.b1 >= 0) { const float* bb = sm + g.b1 * BS; q1p = ldtf(bb + BQP); vel1 = ldsv(bb + BU); com1 = ld3(bb + BCOM); }
-  // resolve_friction_frame :79-156
-  v3 ax0 = ld3(P.m.shape_friction_axis + (size_t)(w * SC + g.s0) * 3);
-  v3 ax1 = ld3(P.m.shape_friction_axis + (size_t)(w * SC + g.s1) * 3);
-  float mp0 = P.m.shape_mu_perp[w * SC + g.s0], mp1 = P.m.shape_mu_perp[w * SC + g.s1];
-  v3 t1, t2;
-  float mux, muy;
-  bool an0 = dot(ax0, ax0) > 1e-6f && g.b0 >= 0;
-  bool an1 = dot(ax1, ax1) > 1e-6f && g.b1 >= 0;
-  if (!an0 && !an1) {
-    orthogonal_basis(g.n, t1, t2);
-    mux = muy = (mu0 + mu1) * 0.5f;
-  } else {
-    v3 aw;
-    float mal, mpa, mot;
-    if (an0) { aw = qrot(q0p.q, ax0); mal = mu0; mpa = mp0 >= 0.f ? mp0 : mu0; mot = mu1; }
-    else { aw = qrot(q1p.q, ax1); mal = mu1; mpa = mp1 >= 0.f ? mp1 : mu1; mot = mu0; }
-    v3 at = aw - g.n * dot(aw, g.n);
-    float tl2 = dot(at, at);
-    if (tl2 < 1e-8f) {
-      orthogonal_basis(g.n, t1, t2);
-      mux = muy = (mal + mot) * 0.5f;
-    } else {
-      t1 = at * (1.f / sqrtf(tl2));
-      t2 = cross(g.n, t1);
-      mux = (mal + mot) * 0.5f;
-      muy = (mpa + mot) * 0.5f;
-    }
-  }
-  // resolve_stribeck_params :159-176 / resolve_stribeck_lateral_only :179-194
-  float ss0 = P.m.shape_mu_stiction_scale[w * SC + g.s0], vs0 = P.m.shape_v_stribeck[w * SC + g.s0];
-  float ss1 = P.m.shape_mu_stiction_scale[w * SC + g.s1], vs1 = P.m.shape_v_stribeck[w * SC + g.s1];
-  float mss = -1.f, vst = -1.f;
-  if (ss0 > 0.f && vs0 > 0.f) { mss = ss0; vst = vs0; }
-  else if (ss1 > 0.f && vs1 > 0.f) { mss = ss1; vst = vs1; }
-  float lo0 = P.m.shape_stribeck_lateral_only[w * SC + g.s0], lo1 = P.m.shape_stribeck_lateral_only[w * SC + g.s1];
-  float lat = lo0 > 0.f ? lo0 : (lo1 > 0.f ? lo1 : -1.f);
-  // compute_friction_core :379-469
-  sv Jt1a = svzero(), Jt2a = svzero(), Jt1b = svzero(), Jt2b = svzero();
-  if (g.b0 >= 0) {
-    v3 r_ = (tpoint(q0p, g.p0) - g.n * g.th0) - tpoint(q0p, com0);
-    Jt1a = sv{t1, cross(r_, t1)};
-    Jt2a = sv{t2, cross(r_, t2)};
-  }
-  if (g.b1 >= 0) {
-    v3 r_ = (tpoint(q1p, g.p1) + g.n * g.th1) - tpoint(q1p, com1);
-    Jt1b = sv{-t1, -cross(r_, t1)};
-    Jt2b = sv{-t2, -cross(r_, t2)};
-  }
-  float wt1 = eff_mass_prev(sm, g.b0, g.b1, Jt1a, Jt1b);
-  float wt2 = eff_mass_prev(sm, g.b0, g.b1, Jt2a, Jt2b);
-  float rr = (wt1 + wt2) * 0.5f;
-  // compute_friction_model :197-371
-  float vt0 = sdot(Jt1a, vel0) + sdot(Jt1b, vel1);
-  float vt1 = sdot(Jt2a, vel0) + sdot(Jt2b, vel1);
-  if (mss > 0.f && vst > 0.f) {
-    float s = sqrtf(vt0 * vt0 + vt1 * vt1);
-    float factor = 1.f + (mss - 1.f) * expf(-s / vst);
-    if (lat > 0.f && mux != muy) { mux = mux * factor; }
-    else { mux = mux * factor; muy = muy * factor; }
-  }
-  float dt = P.dt;
-  float wx = 0.f, wy = 0.f;
-  if (P.cfg.friction_mode == OFB_FRICTION_CONE) {
-    const float eps = 1e-8f;
-    float denom_eps = 1e-6f * dt;
-    float d0 = vt0 * dt, d1 = vt1 * dt;
-    float i0 = l1p * dt, i1 = l2p * dt;
-    if (mux == muy) {
-      float dn = sqrtf(d0 * d0 + d1 * d1 + eps);
-      float raw = sqrtf(i0 * i0 + i1 * i1);
-      float lim = mux * (fn_prev * dt);
-      float gap = lim - fminf(raw, lim);
-      float sa = dn, sb = rr * gap;
-      float phi = sa + sb - sqrtf(sa * sa + sb * sb);
-      float wv = rr * ((dn - phi) / (rr * raw + phi + denom_eps));
-      wv = fminf(fmaxf(wv, 0.f), 1e5f);
-      wx = wv; wy = wv;
-    } else {
-      float mxs = fmaxf(mux, 1e-6f), mys = fmaxf(muy, 1e-6f);
-      float dxt = mxs * d0, dyt = mys * d1;
-      float dn = sqrtf(dxt * dxt + dyt * dyt + eps);
-      float imx = 1.f / mxs, imy = 1.f / mys;
-      float fxt = i0 * imx, fyt = i1 * imy;
-      float raw = sqrtf(fxt * fxt + fyt * fyt);
-      float lim = fn_prev * dt;
-      float gap = lim - fminf(raw, lim);
-      float sa = dn, sb = rr * gap;
-      float phi = sa + sb - sqrtf(sa * sa + sb * sb);
-      float wt = rr * ((dn - phi) / (rr * raw + phi + denom_eps));
-      wt = fminf(fmaxf(wt, 0.f), 1e5f);
-      wx = wt * imx * imx;
-      wy = wt * imy * imy;
-    }
-  }
-  o.active = true;
-  sv Ja = t == 0 ? Jt1a : Jt2a;
-  sv Jb = t == 0 ? Jt1b : Jt2b;
-  float wgt = t == 0 ? wx : wy;
-  float vt = t == 0 ? vt0 : vt1;
-  if (g.b0 >= 0) atomic_add_sv(sm + g.b0 * BS + BHD, Ja * (-dt * lam));
-  if (g.b1 >= 0) atomic_add_sv(sm + g.b1 * BS + BHD, Jb * (-dt * lam));
-  o.hc = vt + wgt * lam;
-  o.J0 = Ja; o.J1 = Jb;
-  o.C = wgt / dt + P.cfg.compliance_friction;
-  return o;
-}
-
-// dispatch on the compacted row index
-__device__ __forceinline__ RowOut assemble_row(const FusedParams& P, float* sm, const float* lamprev, int w, int i,
-                                               int n_c, float lam) {
-  const int Nj = P.L.Nj, Nct = P.L.Nctrl;
-  if (i < Nj) return joint_row(P, sm, w, i, lam, true);
-  if (i < Nj + Nct) return control_row(P, sm, w, i - Nj, lam);
-  int c = i - Nj - Nct;
-  if (c < n_c) return normal_row(P, sm, w, c, lam);
-  int f = c - n_c;
-  return friction_row(P, sm, lamprev, w, f >> 1, f & 1, n_c, lam);
-}
-
-__device__ __forceinline__ int original_row(const RowLayout& L, int i, int n_c) {
-  if (i < L.Nj + L.Nctrl) return i;
-  int c = i - L.Nj - L.Nctrl;
-  if (c < n_c) return L.off_n + c;
-  int f = c - n_c;
-  return L.off_f + f;
-}
-
-// dynamics residual of every body (constraints/dynamics_constraint.py:6-50); lanes over bodies
-__device__ __forceinline__ void dynamics_bodies(const FusedParams& P, float* sm, int lane, bool with_inv) {
-  const int B = P.m.B;
-  const v3 g = mk(P.m.gravity[0], P.m.gravity[1], P.m.gravity[2]);
-  for (int b = lane; b < B; b += WAVE) {
-    float* bb = sm + b * BS;
-    qt q = qt{bb[BQ + 3], bb[BQ + 4], bb[BQ + 5], bb[BQ + 6]};
-    if (with_inv) {
-      m3 Iwi = rot_inertia(q, ldm3(bb + BIIB));
-#pragma unroll
-      for (int k = 0; k < 9; ++k) bb[BIW + k] = Iwi.m[k];
-    }
-    m3 Iw = rot_inertia(q, ldm3(bb + BIB));
-    sv u = ldsv(bb + BU), up = ldsv(bb + BUP), f = ldsv(bb + BF);
-    float m = bb[BM];
-    v3 wv = u.b;
-    v3 gyro = cross(wv, mv(Iw, wv)) * P.cfg.gyro_scale;
-    v3 lin = (u.a - up.a) * m - (f.a + g * m) * P.dt;
-    v3 ang = mv(Iw, u.b - up.b) - (f.b + gyro) * P.dt;
-    bb[BHD + 0] = lin.x; bb[BHD + 1] = lin.y; bb[BHD + 2] = lin.z;
-    bb[BHD + 3] = ang.x; bb[BHD + 4] = ang.y; bb[BHD + 5] = ang.z;
-  }
-}
-
-// integrate_body_pose (mechanics/pose_integration.py:6-38)
-__device__ __forceinline__ tf integrate(const sv& vel, const tf& pp, v3 com, float dt) {
-  v3 pcom = pp.p + qrot(pp.q, com);
-  v3 op = pcom + vel.a * dt;
-  float s = dt * 0.5f;
-  float t2 = pp.q.x, t3 = pp.q.y, t4 = pp.q.z, t1 = pp.q.w;
-  v3 wv = vel.b;
-  qt rq;
-  rq.x = pp.q.x + (t1 * wv.x + t4 * wv.y - t3 * wv.z) * s;
-  rq.y = pp.q.y + (-t4 * wv.x + t1 * wv.y + t2 * wv.z) * s;
-  rq.z = pp.q.z + (t3 * wv.x - t2 * wv.y + t1 * wv.z) * s;
-  rq.w = pp.q.w + (-t2 * wv.x - t3 * wv.y - t4 * wv.z) * s;
-  float l = sqrtf(rq.x * rq.x + rq.y * rq.y + rq.z * rq.z + rq.w * rq.w);
-  qt rn = l > 0.f ? qt{rq.x / l, rq.y / l, rq.z / l, rq.w / l} : qt{0.f, 0.f, 0.f, 0.f};
-  return tf{op - qrot(rn, com), rn};
-}
 
 // out = (J M^-1 J^T + C + reg) v over the rows of one world (optim/system_operator.py:36-124):
 // scatter J^T v into the LDS body accumulator, apply M^-1 per body, gather.

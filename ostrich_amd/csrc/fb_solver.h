@@ -80,6 +80,9 @@ struct ofb_solver {
   ofb::PlainBuffers* plain;
   // launch shape chosen by the autotuner for the fused path
   int fused_worlds_per_block;
+  // events around the solve of the last step
+  hipEvent_t ev_solve0, ev_solve1;
+  int events_ready;
 };
 
 namespace ofb {

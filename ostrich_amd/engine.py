@@ -144,5 +144,11 @@ class OstrichEngine:
             "ofb_get_packed_contacts")
         return out
 
+    def last_solve_ms(self):
+        """Duration of the last step's solve (HIP events on the step's stream)."""
+        ms = C.c_float()
+        _lib.check(self.lib.ofb_last_solve_ms(self._h, C.byref(ms)), "ofb_last_solve_ms")
+        return ms.value
+
     def step_backward(self):
         raise NotImplementedError("the adjoint (step_backward) is not built on this solver yet")

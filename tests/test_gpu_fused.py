@@ -1,0 +1,196 @@
+"""Parity of the gfx950 fused solver (through the C ABI) with the CPU oracle.
+
+Every comparison starts both solvers from the same state and the same contact set.
+Tolerances: in worlds whose Newton loop converged (||h||^2 <= atol^2) in the GPU, FP32
+and FP64 solves, poses and velocities must agree with the FP32 oracle within 1e-4 of the
+quantity's scale (1 m, 1 rad for poses; g*dt for velocities).  In worlds stopped at the
+iteration cap the iterate is not a solution and any rounding change moves it, so the
+GPU must stay within four times the FP32-vs-FP64 spread of that world.  Normal impulses
+of statically indeterminate contact patches are compared through their per-world sums.
+"""
+import numpy as np
+import pytest
+import torch
+
+from ostrich_amd import collide, scenes
+from ostrich_amd.containers import Contacts, Control, State
+from tests.helpers import config, no_contacts, single_box, zero_control
+
+pytestmark = pytest.mark.gpu
+
+G = 9.81
+
+
+def _engine(model, cfg, device):
+    from ostrich_amd import OstrichEngine
+    return OstrichEngine(model, config=cfg, device=device)
+
+
+def _gpu_step(eng, state, control, contacts, dt, device):
+    s_in = State.from_numpy({k: np.asarray(v, np.float32) for k, v in state.items()}, device)
+    s_out = s_in.clone()
+    eng.step(s_in, s_out, Control.from_numpy(control, device), Contacts.from_dict(contacts, device), dt)
+    st = {k: v.cpu().numpy() for k, v in eng.stats().items()}
+    return dict(body_q=s_out.body_q.cpu().numpy().astype(np.float64),
+                body_qd=s_out.body_qd.cpu().numpy().astype(np.float64),
+                lam=eng.constraint_impulses().cpu().numpy().astype(np.float64), **st)
+
+
+def _assert_close(g, o32, o64, key, scale, atol_sq, rel=1e-4):
+    """Worlds that converged in all three solves: every entry within rel*scale of the FP32
+    oracle.  Worlds stopped at the Newton cap: the world's worst GPU-vs-FP32 error within
+    four times its FP32-vs-FP64 spread (plus rel*scale)."""
+    W = g["merit"].shape[0]
+    eg = np.abs(g[key] - o32[key]).reshape(W, -1).max(1)
+    ep = np.abs(o32[key] - o64[key]).reshape(W, -1).max(1)
+    conv = (g["merit"] <= atol_sq) & (o32["merit"] <= atol_sq) & (o64["merit"] <= atol_sq)
+    bad_c = conv & (eg > rel * scale)
+    bad_u = (~conv) & (eg > 4.0 * ep + rel * scale)
+    assert not bad_c.any(), f"{key}: converged worlds off by {eg[bad_c].max():.3e} (scale {scale})"
+    assert not bad_u.any(), (f"{key}: capped worlds {np.nonzero(bad_u)[0][:8]} gpu-vs-f32 {eg[bad_u].max():.3e}, "
+                             f"f32-vs-f64 {ep[bad_u].max():.3e}")
+    return conv
+
+
+def _run_compare(oracle_mod, model, cfg, state, control, steps, dt, device, K, gap=0.02):
+    eng = _engine(model, cfg, device)
+    col = collide.PrimitiveCollider(model, gap=gap, max_contacts_per_world=K)
+    st = {k: np.asarray(v, np.float32).astype(np.float64) for k, v in state.items()}
+    nr_diffs, converged = [], []
+    for _ in range(steps):
+        c = col.collide(torch.as_tensor(st["body_q"], dtype=torch.float32))
+        c = {k: (v.numpy() if hasattr(v, "numpy") else v) for k, v in c.items()}
+        st32 = {k: v.astype(np.float32) for k, v in st.items()}
+        g = _gpu_step(eng, st32, control, c, dt, device)
+        o32 = oracle_mod.step(model, cfg, st32, control, c, dt, precision=32)
+        o64 = oracle_mod.step(model, cfg, st32, control, c, dt, precision=64)
+        atol_sq = cfg.nr.atol ** 2
+        conv = _assert_close(g, o32, o64, "body_q", 1.0, atol_sq)
+        _assert_close(g, o32, o64, "body_qd", G * dt, atol_sq)
+        converged.append(conv)
+        nc, nj, nctrl = oracle_mod.dims(model, cfg)
+        ln_g = g["lam"][:, nj + nctrl: nj + nctrl + K].sum(1)
+        ln_o = o32["lam"][:, nj + nctrl: nj + nctrl + K].sum(1)
+        ln_p = o64["lam"][:, nj + nctrl: nj + nctrl + K].sum(1)
+        scale = np.maximum(np.abs(ln_o), 1.0)
+        ok = np.abs(ln_g - ln_o) <= np.where(conv, 1e-4 * scale, 4 * np.abs(ln_o - ln_p) + 1e-4 * scale)
+        assert ok.all(), np.abs(ln_g - ln_o)[~ok]
+        nr_diffs.append(np.abs(g["nr_iters"] - o32["nr_iters"])[conv])
+        st = dict(st, body_q=o32["body_q"], body_qd=o32["body_qd"])
+    return np.concatenate(nr_diffs), np.concatenate(converged)
+
+
+def test_box_stack_parity(oracle_mod, gpu_device):
+    model, state, control = scenes.box_stack(16, 3, seed=0)
+    cfg = config(K=16)
+    nr_diff, conv = _run_compare(oracle_mod, model, cfg, state, control, steps=5, dt=0.05, device=gpu_device, K=16)
+    assert conv.mean() > 0.9
+    assert np.mean(nr_diff <= 1) >= 0.9 and nr_diff.max() <= 3
+
+
+@pytest.mark.parametrize("nr_iters", [1, 2])
+def test_skid_steer_fixed_newton_iterations(oracle_mod, gpu_device, nr_iters):
+    """Skid-steer worlds after 1-2 Newton iterations from identical inputs (friction enters
+    at the second).  These iterates are well-conditioned functions of the inputs, so all
+    worlds are held to 1e-4 of scale.  From the third iteration on, the friction weight
+    w = r (|d| - phi) / (r |lambda_prev| + phi + eps) (friction_constraint.py:299-324)
+    cancels |d| - phi at small slip and the FP32 oracle itself drifts from FP64; those
+    iterates are covered by test_skid_steer_rollout_parity."""
+    from ostrich_amd.config import NewtonRaphsonConfig
+    model, state, control = scenes.skid_steer(96, seed=1)
+    cfg = config(K=128)
+    cfg = type(cfg)(nr=NewtonRaphsonConfig(max_iters=nr_iters, backtrack_min_iter=0, atol=1e-5),
+                    linear=cfg.linear, contacts=cfg.contacts, execution=cfg.execution)
+    col = collide.PrimitiveCollider(model, gap=0.02, max_contacts_per_world=128)
+    rng = np.random.default_rng(nr_iters)
+    st = {k: np.asarray(v, np.float32).copy() for k, v in state.items()}
+    st["body_qd"] = rng.normal(scale=0.3, size=st["body_qd"].shape).astype(np.float32)
+    c = col.collide(torch.as_tensor(st["body_q"], dtype=torch.float32))
+    c = {k: (v.numpy() if hasattr(v, "numpy") else v) for k, v in c.items()}
+    eng = _engine(model, cfg, gpu_device)
+    g = _gpu_step(eng, st, control, c, 0.05, gpu_device)
+    o = oracle_mod.step(model, cfg, st, control, c, 0.05, precision=32)
+    np.testing.assert_allclose(g["body_q"], o["body_q"], atol=1e-4)
+    np.testing.assert_allclose(g["body_qd"], o["body_qd"], atol=1e-4 * G * 0.05)
+    lscale = np.maximum(np.abs(o["lam"]).max(1, keepdims=True), 1.0)
+    assert np.all(np.abs(g["lam"] - o["lam"]) <= 1e-4 * lscale)
+    assert np.array_equal(g["nr_iters"], o["nr_iters"])
+    assert np.array_equal(g["pcr_iters"], o["pcr_iters"]) or np.mean(g["pcr_iters"] == o["pcr_iters"]) > 0.95
+
+
+def test_skid_steer_rollout_parity(oracle_mod, gpu_device):
+    """Full Newton loop (up to 20 iterations) on skid-steer worlds.  The GPU result must
+    be another FP32 rounding of the same algorithm: its per-world deviation from the FP32
+    oracle is distributed like the FP32 oracle's deviation from the FP64 oracle (median
+    within 3x, maximum within 4x, plus 1e-4 of scale)."""
+    model, state, control = scenes.skid_steer(64, seed=1)
+    W = model.num_worlds
+    eng = _engine(model, config(K=128), gpu_device)
+    col = collide.PrimitiveCollider(model, gap=0.02, max_contacts_per_world=128)
+    st = {k: np.asarray(v, np.float32).astype(np.float64) for k, v in state.items()}
+    for _ in range(3):
+        c = col.collide(torch.as_tensor(st["body_q"], dtype=torch.float32))
+        c = {k: (v.numpy() if hasattr(v, "numpy") else v) for k, v in c.items()}
+        st32 = {k: v.astype(np.float32) for k, v in st.items()}
+        g = _gpu_step(eng, st32, control, c, 0.05, gpu_device)
+        o32 = oracle_mod.step(model, config(K=128), st32, control, c, 0.05, precision=32)
+        o64 = oracle_mod.step(model, config(K=128), st32, control, c, 0.05, precision=64)
+        for key, scale in (("body_q", 1.0), ("body_qd", G * 0.05)):
+            eg = np.abs(g[key] - o32[key]).reshape(W, -1).max(1)
+            ep = np.abs(o32[key] - o64[key]).reshape(W, -1).max(1)
+            assert np.median(eg) <= 3.0 * np.median(ep) + 1e-4 * scale, (key, np.median(eg), np.median(ep))
+            assert eg.max() <= 4.0 * ep.max() + 1e-4 * scale, (key, eg.max(), ep.max())
+        st = dict(st, body_q=o32["body_q"], body_qd=o32["body_qd"])
+
+
+def test_free_fall_closed_form_gpu(gpu_device):
+    model, state = single_box(mass=2.0, pose=(0, 0, 3.0, 0, 0, 0, 1))
+    state["body_qd"][0] = [0.5, -0.2, 0.1, 0, 0, 0]
+    eng = _engine(model, config(nr_iters=4, atol=1e-9, K=4), gpu_device)
+    g = _gpu_step(eng, state, zero_control(model), no_contacts(model), 0.05, gpu_device)
+    u = np.array([0.5, -0.2, 0.1 - 0.05 * G])
+    np.testing.assert_allclose(g["body_qd"][0, :3], u, atol=1e-6)
+    np.testing.assert_allclose(g["body_q"][0, :3], np.array([0, 0, 3.0]) + 0.05 * u, atol=1e-6)
+
+
+def test_packing_follows_input_order_bit_exact(gpu_device):
+    """core/contacts.py:25-99 restated: per-world slots in input order, normal negated,
+    Newton flat shape ids mapped to world-local columns; contacts arrive shuffled across
+    worlds, so the stable partition is exercised."""
+    model, state, control = scenes.box_stack(5, 3, seed=4)
+    col = collide.PrimitiveCollider(model, gap=0.05, max_contacts_per_world=16)
+    c = col.collide(torch.as_tensor(state["body_q"], dtype=torch.float32))
+    n = int(c["count"][0])
+    perm = np.random.default_rng(0).permutation(n)
+    cs = {k: (v.numpy().copy() if hasattr(v, "numpy") else v) for k, v in c.items()}
+    for k in ("point0", "point1", "normal", "shape0", "shape1", "margin0", "margin1"):
+        cs[k][:n] = cs[k][:n][perm]
+    cs["margin0"][:n] = np.linspace(0.0, 0.01, n, dtype=np.float32)
+    K = 16
+    eng = _engine(model, config(K=K), gpu_device)
+    _gpu_step(eng, state, control, cs, 0.05, gpu_device)
+    pk = {k: v.cpu().numpy() for k, v in eng.packed_contacts().items()}
+    W, S, G_ = model.num_worlds, model.shapes_per_world, model.num_global_shapes
+    sw = model.arrays["shape_world"]
+    expect_count = np.zeros(W, np.int32)
+    for i in range(n):
+        a, b = int(cs["shape0"][i]), int(cs["shape1"][i])
+        w = sw[a] if sw[a] >= 0 else sw[b]
+        slot = expect_count[w]
+        expect_count[w] += 1
+        if slot >= K:
+            continue
+        j = w * K + slot
+        local = lambda s: S + s if s < G_ else (s - G_) % S
+        assert pk["shape0"][j] == local(a) and pk["shape1"][j] == local(b)
+        assert np.array_equal(pk["point0"][3 * j: 3 * j + 3], cs["point0"][i])
+        assert np.array_equal(pk["normal"][3 * j: 3 * j + 3], -cs["normal"][i])
+        assert pk["thickness0"][j] == cs["margin0"][i]
+    assert np.array_equal(pk["count"], expect_count)
+
+
+def test_invalid_arguments_raise(gpu_device):
+    model, state = single_box()
+    eng = _engine(model, config(K=4), gpu_device)
+    with pytest.raises(RuntimeError, match="dt must be positive"):
+        _gpu_step(eng, state, zero_control(model), no_contacts(model), 0.0, gpu_device)
