@@ -205,6 +205,8 @@ int ofb_destroy(ofb_solver* s) {
     if (p) (void)hipFree(p);
   if (s->plain) plain_destroy(s);
   if (s->events_ready) { (void)hipEventDestroy(s->ev_solve0); (void)hipEventDestroy(s->ev_solve1); }
+  if (s->cls_counts) (void)hipFree(s->cls_counts);
+  if (s->cls_lists) (void)hipFree(s->cls_lists);
   delete s;
   return 0;
 }
@@ -330,6 +332,8 @@ int ofb_create(const ofb_model* model, const ofb_config* config, int32_t path, o
   s->events_ready = 1;
   if (path == OFB_PATH_PLAIN) {
     if ((rc = plain_create(s)) != 0) { ofb_destroy(s); return rc; }
+  } else {
+    if ((rc = fused_classes_alloc(s)) != 0) { ofb_destroy(s); return rc; }
   }
   *out_solver = s;
   return 0;

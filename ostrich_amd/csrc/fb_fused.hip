@@ -13,7 +13,7 @@
 //     row i -> lane i % 64, slot i / 64; every per-row quantity (Jacobian pair, C,
 //     Jacobi diagonal, the seven PCR vectors, lambda) lives in VGPRs, so PCR vector
 //     updates are pure VALU and dot products are in-register partial sums followed by
-//     one cross-lane butterfly;
+//     one DPP cross-lane reduction;
 //   * body state (pose, velocity, previous step, world inverse inertia, the J^T x
 //     accumulator) lives in LDS, 96 floats per body, and couples the rows: the Schur
 //     matvec J M^-1 J^T x is a scatter of J^T x into LDS (ds_add_f32 from one wave, so
@@ -32,6 +32,18 @@
 namespace ofb {
 
 using namespace rows;
+
+#ifdef OFB_PHASE_TIMING
+// Profiling build only: per-phase cycle counts of the Newton loop, summed over worlds.
+__device__ unsigned long long g_phase_cycles[8];
+#define OFB_PHASE_INIT unsigned long long ph_[8] = {0, 0, 0, 0, 0, 0, 0, 0}; unsigned long long ph_t = clock64();
+#define OFB_PHASE(k) { unsigned long long t_ = clock64(); ph_[k] += t_ - ph_t; ph_t = t_; }
+#define OFB_PHASE_FLUSH if (lane == 0) for (int k_ = 0; k_ < 8; ++k_) atomicAdd(&g_phase_cycles[k_], ph_[k_]);
+#else
+#define OFB_PHASE_INIT
+#define OFB_PHASE(k)
+#define OFB_PHASE_FLUSH
+#endif
 
 namespace {
 
@@ -56,6 +68,8 @@ struct FusedParams {
   int32_t *st_nr, *st_pcr, *st_best, *st_flags;
   float* st_merit;
   int lds_per_wave;  // floats
+  const int32_t* wlist;   // worlds of this launch's row class
+  const int32_t* wcount;  // number of them
 };
 
 // out = (J M^-1 J^T + C + reg) v over the rows of one world (optim/system_operator.py:36-124):
@@ -102,8 +116,9 @@ __global__ void __launch_bounds__(256) fused_step_kernel(FusedParams P) {
   extern __shared__ float lds[];
   const int lane = threadIdx.x & (WAVE - 1);
   const int wave = threadIdx.x / WAVE;
-  const int w = blockIdx.x * (blockDim.x / WAVE) + wave;
-  if (w >= P.m.W) return;
+  const int idx = blockIdx.x * (blockDim.x / WAVE) + wave;
+  if (idx >= *P.wcount) return;
+  const int w = P.wlist[idx];
   float* sm = lds + wave * P.lds_per_wave;
   const int B = P.m.B;
   const RowLayout& L = P.L;
@@ -154,6 +169,7 @@ __global__ void __launch_bounds__(256) fused_step_kernel(FusedParams P) {
   int n_it = 0, pcr_total = 0, best_idx = -1;
   float best_merit = 0.f, merit = 0.f;
 
+  OFB_PHASE_INIT
   while (true) {
     // ---- friction-lag snapshot (base_engine.py:441) ----
 #pragma unroll
@@ -163,6 +179,7 @@ __global__ void __launch_bounds__(256) fused_step_kernel(FusedParams P) {
     }
     dynamics_bodies(P, sm, lane, true);
     wsync();
+    OFB_PHASE(0)
     // ---- assembly (linear_system.py:140-322) ----
     float hc[R];
 #pragma unroll
@@ -182,6 +199,7 @@ __global__ void __launch_bounds__(256) fused_step_kernel(FusedParams P) {
       }
     }
     wsync();
+    OFB_PHASE(1)
     // M^-1 h_d per body
     for (int b = lane; b < B; b += WAVE) {
       float* bb = sm + b * BS;
@@ -212,6 +230,7 @@ __global__ void __launch_bounds__(256) fused_step_kernel(FusedParams P) {
     }
     bsq = wave_sum(bsq);
 
+    OFB_PHASE(2)
     // ---- PCR (optim/pcr_solver.py:170-295), x0 = 0 => r = b ----
 #pragma unroll
     for (int k = 0; k < R; ++k) { x[k] = 0.f; r[k] = rhs[k]; z[k] = Pi[k] * r[k]; }
@@ -247,6 +266,7 @@ __global__ void __launch_bounds__(256) fused_step_kernel(FusedParams P) {
       if (!(rsq > target) || it >= P.cfg.lin_max_iters) break;
     }
     pcr_total += it;
+    OFB_PHASE(3)
 
     // ---- velocity recovery + Newton update (linear_system.py:344-377, newton_step.py:22-54) ----
     for (int e = lane; e < 6 * B; e += WAVE) sm[(e / 6) * BS + BACC + (e % 6)] = 0.f;
@@ -271,6 +291,7 @@ __global__ void __launch_bounds__(256) fused_step_kernel(FusedParams P) {
     }
     wsync();
 
+    OFB_PHASE(4)
     // ---- residual at the new iterate (core/residual.py:18-173) ----
     dynamics_bodies(P, sm, lane, false);
     wsync();
@@ -289,6 +310,7 @@ __global__ void __launch_bounds__(256) fused_step_kernel(FusedParams P) {
       msum += v * v;
     }
     merit = wave_sum(msum);
+    OFB_PHASE(5)
 
     // ---- candidate bookkeeping (history_group.py + backtracking.py:12-95) ----
     int kidx = n_it;
@@ -310,10 +332,12 @@ __global__ void __launch_bounds__(256) fused_step_kernel(FusedParams P) {
     bool keep = merit > atol_sq;
     if (n_it < P.cfg.nr_backtrack_min_iter) keep = true;
     if (n_it >= P.cfg.nr_max_iters) keep = false;
+    OFB_PHASE(6)
     if (!keep) break;
     wsync();
   }
   wsync();
+  OFB_PHASE_FLUSH
   // ---- pick the candidate ----
   int picked = n_it - 1;
   float picked_merit = merit;
@@ -360,10 +384,56 @@ __global__ void __launch_bounds__(256) fused_step_kernel(FusedParams P) {
   }
 }
 
+// Row classes: a world with n_r active rows runs the instance with R = the smallest
+// class >= ceil(n_r / 64) slots per lane, so small worlds do not pay for the padded
+// row count's registers.  Worlds are bucketed on the device every step.
+constexpr int kNumClasses = 11;
+__constant__ int kClassR[kNumClasses] = {1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 16};
+static const int kClassRHost[kNumClasses] = {1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 16};
+
+__global__ void reset_classes_kernel(int32_t* counts) {
+  if (threadIdx.x < kNumClasses) counts[threadIdx.x] = 0;
+}
+
+__global__ void classify_worlds_kernel(int W, const int32_t* __restrict__ ccount, int K, int base_rows,
+                                       int32_t* __restrict__ counts, int32_t* __restrict__ lists) {
+  int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= W) return;
+  int n_r = base_rows + 3 * min(ccount[w], K);
+  int r = max(1, (n_r + WAVE - 1) / WAVE);
+  int c = 0;
+  while (c < kNumClasses - 1 && kClassR[c] < r) ++c;
+  int slot = atomicAdd(counts + c, 1);
+  lists[(size_t)c * W + slot] = w;
+}
+
 template <int R>
-static void launch_fused(const FusedParams& P, int wpb, size_t lds_bytes, hipStream_t stream) {
+static void launch_fused(FusedParams P, int cls, int32_t* counts, int32_t* lists, int wpb, size_t lds_bytes,
+                         hipStream_t stream) {
+  P.wlist = lists + (size_t)cls * P.m.W;
+  P.wcount = counts + cls;
   int blocks = (P.m.W + wpb - 1) / wpb;
   hipLaunchKernelGGL(fused_step_kernel<R>, dim3(blocks), dim3(wpb * WAVE), lds_bytes, stream, P);
+}
+
+#ifdef OFB_PHASE_TIMING
+extern "C" int ofb_debug_phase_cycles(unsigned long long* out, int reset) {
+  if (hipMemcpyFromSymbol(out, HIP_SYMBOL(g_phase_cycles), sizeof(unsigned long long) * 8) != hipSuccess) return -4;
+  if (reset) {
+    unsigned long long z[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (hipMemcpyToSymbol(HIP_SYMBOL(g_phase_cycles), z, sizeof(z)) != hipSuccess) return -4;
+  }
+  return 0;
+}
+#endif
+
+int fused_classes_alloc(ofb_solver* s) {
+  if (hipMalloc((void**)&s->cls_counts, kNumClasses * sizeof(int32_t)) != hipSuccess ||
+      hipMalloc((void**)&s->cls_lists, (size_t)kNumClasses * s->m.W * sizeof(int32_t)) != hipSuccess) {
+    set_error("hipMalloc failed (world classes)");
+    return -4;
+  }
+  return 0;
 }
 
 int fused_step(ofb_solver* s, const ofb_state_in* in, const ofb_state_out* out, const ofb_control* ctl, float dt,
@@ -382,24 +452,25 @@ int fused_step(ofb_solver* s, const ofb_state_in* in, const ofb_state_out* out, 
   P.lambda_out = s->lambda;
   P.st_nr = s->st_nr; P.st_pcr = s->st_pcr; P.st_best = s->st_best; P.st_flags = s->st_flags; P.st_merit = s->st_merit;
   P.lds_per_wave = s->m.B * BS + s->L.Nc + 1;
-  int wpb = s->fused_worlds_per_block;
-  size_t lds_bytes = sizeof(float) * (size_t)P.lds_per_wave * wpb;
-  int R = (s->L.Nc + WAVE - 1) / WAVE;
-  switch (R) {
-    case 1: launch_fused<1>(P, wpb, lds_bytes, stream); break;
-    case 2: launch_fused<2>(P, wpb, lds_bytes, stream); break;
-    case 3: launch_fused<3>(P, wpb, lds_bytes, stream); break;
-    case 4: launch_fused<4>(P, wpb, lds_bytes, stream); break;
-    case 5: launch_fused<5>(P, wpb, lds_bytes, stream); break;
-    case 6: launch_fused<6>(P, wpb, lds_bytes, stream); break;
-    case 7: launch_fused<7>(P, wpb, lds_bytes, stream); break;
-    case 8: launch_fused<8>(P, wpb, lds_bytes, stream); break;
-    case 9: case 10: launch_fused<10>(P, wpb, lds_bytes, stream); break;
-    case 11: case 12: launch_fused<12>(P, wpb, lds_bytes, stream); break;
-    case 13: case 14: case 15: case 16: launch_fused<16>(P, wpb, lds_bytes, stream); break;
-    default:
-      set_error("fused path supports at most 1024 constraint rows per world (got %d)", s->L.Nc);
-      return -3;
+  const int wpb = s->fused_worlds_per_block;
+  const size_t lds_bytes = sizeof(float) * (size_t)P.lds_per_wave * wpb;
+  const int W = s->m.W;
+  const int r_max = (s->L.Nc + WAVE - 1) / WAVE;
+  if (r_max > kClassRHost[kNumClasses - 1]) {
+    set_error("fused path supports at most %d constraint rows per world (got %d)",
+              WAVE * kClassRHost[kNumClasses - 1], s->L.Nc);
+    return -3;
+  }
+  hipLaunchKernelGGL(reset_classes_kernel, dim3(1), dim3(64), 0, stream, s->cls_counts);
+  hipLaunchKernelGGL(classify_worlds_kernel, dim3((W + 255) / 256), dim3(256), 0, stream, W, s->ccount, s->L.K,
+                     s->L.Nj + s->L.Nctrl, s->cls_counts, s->cls_lists);
+  for (int c = 0; c < kNumClasses; ++c) {
+    if (c > 0 && kClassRHost[c - 1] >= r_max) break;  // no world can need this class
+    switch (kClassRHost[c]) {
+#define CASE(RR) case RR: launch_fused<RR>(P, c, s->cls_counts, s->cls_lists, wpb, lds_bytes, stream); break;
+      CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(10) CASE(12) CASE(16)
+#undef CASE
+    }
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) { set_error("fused kernel launch failed: %s", hipGetErrorString(e)); return -4; }

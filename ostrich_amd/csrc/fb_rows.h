@@ -47,12 +47,21 @@ __device__ __forceinline__ void wsync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// butterfly sum: every lane ends with the bitwise-identical total (each step adds the
-// same two operands in both partner lanes, and fp addition is commutative)
+#define OFB_DPP(v, ctrl, rmask)                                                                  \
+  __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), (ctrl), (rmask), 0xF, false))
+
+// Sum over the 64 lanes with DPP: pairwise and quad swaps, row rotations by 4 and 8 (every
+// lane of a 16-lane row then holds the row sum), row broadcasts 15 and 31 (lane 63 holds
+// the wave sum), read back from lane 63 so every lane returns the same value.  All lanes
+// of the wave must be active.
 __device__ __forceinline__ float wave_sum(float v) {
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, WAVE);
-  return v;
+  v = v + OFB_DPP(v, 0xB1, 0xF);   // quad_perm [1,0,3,2]
+  v = v + OFB_DPP(v, 0x4E, 0xF);   // quad_perm [2,3,0,1]
+  v = v + OFB_DPP(v, 0x124, 0xF);  // row_ror:4
+  v = v + OFB_DPP(v, 0x128, 0xF);  // row_ror:8
+  v = v + OFB_DPP(v, 0x142, 0xA);  // row_bcast:15 into rows 1 and 3
+  v = v + OFB_DPP(v, 0x143, 0xC);  // row_bcast:31 into rows 2 and 3
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
 }
 
 __device__ __forceinline__ v3 ld3(const float* p) { return mk(p[0], p[1], p[2]); }

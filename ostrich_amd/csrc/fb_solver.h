@@ -80,6 +80,9 @@ struct ofb_solver {
   ofb::PlainBuffers* plain;
   // launch shape chosen by the autotuner for the fused path
   int fused_worlds_per_block;
+  // per-step bucketing of worlds by active-row class (fused path)
+  int32_t* cls_counts;
+  int32_t* cls_lists;
   // events around the solve of the last step
   hipEvent_t ev_solve0, ev_solve1;
   int events_ready;
@@ -88,6 +91,7 @@ struct ofb_solver {
 namespace ofb {
 int fused_step(ofb_solver* s, const ofb_state_in* in, const ofb_state_out* out, const ofb_control* ctl, float dt,
                hipStream_t stream);
+int fused_classes_alloc(ofb_solver* s);
 int plain_create(ofb_solver* s);
 void plain_destroy(ofb_solver* s);
 int plain_step(ofb_solver* s, const ofb_state_in* in, const ofb_state_out* out, const ofb_control* ctl, float dt,
