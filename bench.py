@@ -33,10 +33,10 @@ K_CONTACTS = 128
 HBM_PEAK_GBS = 8000.0
 
 
-def solver_config(path, termination="per_world"):
+def solver_config(path, termination="per_world", precond="jacobi"):
     return OstrichEngineConfig(
         nr=NewtonRaphsonConfig(max_iters=20, atol=1e-5),
-        linear=LinearSolverConfig(max_iters=16),
+        linear=LinearSolverConfig(max_iters=16, preconditioner_type=precond),
         contacts=ContactsConfig(max_per_world=K_CONTACTS),
         execution=ExecutionConfig(path=path, termination=termination),
     )
@@ -60,9 +60,10 @@ def operand_bytes_per_world(n_rows_active, nr_iters, pcr_iters):
     return n_rows_active * (400.0 * nr_iters + 304.0 * pcr_iters)
 
 
-def run_path(path, model, state, control, steps, warmup, device, collider):
+def run_path(variant, model, state, control, steps, warmup, device, collider):
     from ostrich_amd import OstrichEngine
-    cfg = solver_config(path)
+    path, precond = variant.split(":")
+    cfg = solver_config(path, precond=precond)
     eng = OstrichEngine(model, config=cfg, device=device)
     s_a = State.from_numpy(state, device)
     s_b = s_a.clone()
@@ -95,12 +96,12 @@ def run_path(path, model, state, control, steps, warmup, device, collider):
                 converged_fraction=float(np.mean(conv)), contacts_mean=float(np.mean(ncon)))
 
 
-def cpu_baseline(model_fn, sample_worlds, sample_steps):
+def cpu_baseline(model_fn, sample_worlds, sample_steps, precond="jacobi"):
     """The oracle (FP32 CPU restatement, one thread) on the first `sample_worlds` worlds
     of the same seeded scene; only the solver calls are timed."""
     from oracle import oracle
     model, state, control = model_fn(sample_worlds)
-    cfg = solver_config("fused")
+    cfg = solver_config("fused", precond=precond)
     col = collide.PrimitiveCollider(model, device="cpu", gap=0.02, max_contacts_per_world=K_CONTACTS)
     hm = oracle.HostModel(model)
     st = {k: np.asarray(v, np.float64) for k, v in state.items()}
@@ -123,7 +124,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--worlds", type=int, default=4096)
-    ap.add_argument("--paths", default="fused,plain")
+    ap.add_argument("--paths", default="fused:jacobi,plain:jacobi,fused:per_body_pair",
+                    help="comma-separated path:preconditioner variants; the first is the headline")
     ap.add_argument("--cpu-sample-worlds", type=int, default=64)
     ap.add_argument("--cpu-sample-steps", type=int, default=40)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -170,8 +172,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded skid-steer scene, primitive collider)",
         "config": {"workload": "uneven-batch skid-steer, 4096 worlds, dt 0.05, NR<=20 to ||h||<=1e-5, "
-                               "PCR<=16 Jacobi, 128 contact slots", "worlds": W, "dt": DT,
-                   "path": main_path, "termination": "per_world"},
+                               "PCR<=16 (Jacobi, the library default), 128 contact slots", "worlds": W,
+                   "dt": DT, "variant": main_path, "termination": "per_world"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "fused_step_kernel", "kernel_ms": r["kernel_ms"],
@@ -183,12 +185,13 @@ def main():
                    "converged_fraction": r["converged_fraction"], "contacts_mean": n_c},
         "paths": {p: results[p] for p in paths},
     }
-    if "plain" in results and "world_steps_per_s" in results.get("plain", {}):
-        out["fused_over_plain"] = r["world_steps_per_s"] / results["plain"]["world_steps_per_s"]
+    fj, pj = results.get("fused:jacobi", {}), results.get("plain:jacobi", {})
+    if "world_steps_per_s" in fj and "world_steps_per_s" in pj:
+        out["fused_over_plain_jacobi"] = fj["world_steps_per_s"] / pj["world_steps_per_s"]
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(lambda n: scenes.skid_steer(n, seed=1), args.cpu_sample_worlds,
-                                           args.cpu_sample_steps)
+                                           args.cpu_sample_steps, precond=main_path.split(":")[1])
     print(json.dumps(out))
 
 

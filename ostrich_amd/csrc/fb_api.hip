@@ -233,11 +233,11 @@ int ofb_create(const ofb_model* model, const ofb_config* config, int32_t path, o
     set_error("the fused path retires worlds independently: use termination=per_world (global needs the plain path)");
     return -2;
   }
-  if (path == OFB_PATH_FUSED && config->preconditioner != OFB_PRECOND_JACOBI) {
-    set_error("the fused path implements the Jacobi preconditioner; per_body_pair runs on the plain path");
+  if (path == OFB_PATH_FUSED && B > 64) { set_error("the fused path supports at most 64 bodies per world"); return -2; }
+  if (path == OFB_PATH_PLAIN && config->preconditioner == OFB_PRECOND_PER_BODY_PAIR) {
+    set_error("the plain path preconditions with Jacobi only: use the fused path for per_body_pair");
     return -2;
   }
-  if (path == OFB_PATH_FUSED && B > 64) { set_error("the fused path supports at most 64 bodies per world"); return -2; }
   ofb_solver* s = new ofb_solver;
   std::memset(s, 0, sizeof(*s));
   s->cfg = *config;

@@ -70,7 +70,222 @@ struct FusedParams {
   int lds_per_wave;  // floats
   const int32_t* wlist;   // worlds of this launch's row class
   const int32_t* wcount;  // number of them
+  int bj;                 // per-body-pair block-Jacobi preconditioner
+  int rows_cap;           // 64 R: row capacity of this launch's class
 };
+
+// ---------------------------------------------------------------------------------
+// Per-body-pair block-Jacobi inside one wavefront (optim/per_body_pair_preconditioner.py:
+// 62-410).  Rows that couple the same two bodies (static side = body B) form a pair;
+// the pair's lowest row is its leader.  Once per Newton iteration each leader builds
+// its dense block A_pair = sum_b J_a,b M_b^-1 J_c,b^T (+ C + reg on the diagonal) in
+// LDS and factors it in place (column Cholesky, packed lower triangle); in every PCR
+// iteration the leader runs both triangular solves for its members.  A pair that fails
+// to factor uses the Jacobi diagonal for its rows, and rows ranked past 64 in a pair
+// use the Jacobi diagonal too (the reference's fixed 64-member blocks,
+// per_body_pair_preconditioner.py:59).
+// LDS per wave after the bodies and lambda_prev (cap = 64 R rows):
+//   pid[cap] i32, lsz[cap] i32, mlist[cap] i32, wbuf[cap], vbuf[cap],
+//   rowd[16 cap] (J0 6, J1 6, b0, b1, C+reg, Jacobi 1/diag), Lbuf[R * 2080]
+// ---------------------------------------------------------------------------------
+constexpr int kPairCap = 64;
+constexpr int kPairL = kPairCap * (kPairCap + 1) / 2;
+
+__host__ __device__ inline int bj_words(int cap) { return 21 * cap + (cap / 64) * kPairL; }
+
+struct BJView {
+  int* pid;
+  int* lsz;
+  int* mlist;
+  float* wbuf;
+  float* vbuf;
+  float* rowd;
+  float* Lbuf;
+  int cap;
+};
+
+__device__ __forceinline__ BJView bj_view(float* base, int cap) {
+  BJView v;
+  v.pid = reinterpret_cast<int*>(base);
+  v.lsz = v.pid + cap;
+  v.mlist = v.lsz + cap;
+  v.wbuf = reinterpret_cast<float*>(v.mlist + cap);
+  v.vbuf = v.wbuf + cap;
+  v.rowd = v.vbuf + cap;
+  v.Lbuf = v.rowd + 16 * cap;
+  return v;
+}
+
+// J of row `row` on body `tb` (_jacobian_for_body, per_body_pair_preconditioner.py:146-184)
+__device__ __forceinline__ sv bj_jac(const BJView& V, int row, int tb, int B) {
+  const float* d = V.rowd + 16 * row;
+  if (tb >= B) return svzero();
+  if (__float_as_int(d[12]) == tb) return sv{mk(d[0], d[1], d[2]), mk(d[3], d[4], d[5])};
+  if (__float_as_int(d[13]) == tb) return sv{mk(d[6], d[7], d[8]), mk(d[9], d[10], d[11])};
+  return svzero();
+}
+
+// Per-row-slot block-Jacobi bookkeeping kept in registers.
+template <int R>
+struct BJRegs {
+  int rank[R];   // rank of the row inside its pair
+  int moff[R];   // offset of the pair's member list / solve workspace
+  int lead[R];   // member count if the row leads its pair, else 0
+  int loff[R];   // offset of the pair's factor (leaders)
+  int fail[R];   // factorisation failed (leaders)
+};
+
+template <int R>
+__device__ void bj_setup(const BJView& V, const float* sm, int lane, int n_r, int B, const bool (&act)[R],
+                         const int (&b0)[R], const int (&b1)[R], const sv (&J0)[R], const sv (&J1)[R],
+                         const float (&Cr)[R], const float (&Pi)[R], BJRegs<R>& G, int& flags) {
+  int mypid[R];
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    int i = lane + WAVE * k;
+    mypid[k] = -1;
+    if (i < n_r) {
+      if (act[k]) {  // _compute_pair_ids_kernel :62-109
+        int e1 = b0[k] < 0 ? B : b0[k], e2 = b1[k] < 0 ? B : b1[k];
+        int lo = min(e1, e2), hi = max(e1, e2);
+        mypid[k] = lo * (B + 1) + hi;
+      }
+      V.pid[i] = mypid[k];
+      float* d = V.rowd + 16 * i;
+      d[0] = J0[k].a.x; d[1] = J0[k].a.y; d[2] = J0[k].a.z; d[3] = J0[k].b.x; d[4] = J0[k].b.y; d[5] = J0[k].b.z;
+      d[6] = J1[k].a.x; d[7] = J1[k].a.y; d[8] = J1[k].a.z; d[9] = J1[k].b.x; d[10] = J1[k].b.y; d[11] = J1[k].b.z;
+      d[12] = __int_as_float(b0[k]); d[13] = __int_as_float(b1[k]);
+      d[14] = Cr[k]; d[15] = Pi[k];
+    }
+  }
+  wsync();
+  // rank and size of every row's pair; leaders publish their size
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    int i = lane + WAVE * k;
+    int rank = 0, size = 0;
+    if (i < n_r && mypid[k] >= 0) {
+      for (int j = 0; j < n_r; ++j) {
+        int pj = V.pid[j];
+        if (pj == mypid[k]) { size += 1; if (j < i) rank += 1; }
+      }
+    }
+    G.rank[k] = rank;
+    G.lead[k] = (i < n_r && mypid[k] >= 0 && rank == 0) ? size : 0;
+    if (i < n_r) V.lsz[i] = G.lead[k];
+    if (G.lead[k] > kPairCap) flags |= FLAG_PAIR_OVERFLOW;
+  }
+  wsync();
+  // offsets of member lists and factors; leaders write their member lists
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    int i = lane + WAVE * k;
+    int mo = 0, lo = 0;
+    if (i < n_r && mypid[k] >= 0) {
+      // the pair's leader is the first row with this pair id
+      int leader = i;
+      for (int j = 0; j < i; ++j) if (V.pid[j] == mypid[k]) { leader = j; break; }
+      for (int j = 0; j < leader; ++j) {
+        int s = V.lsz[j];
+        int sc = min(s, kPairCap);
+        mo += sc;
+        lo += sc * (sc + 1) / 2;
+      }
+    }
+    G.moff[k] = mo;
+    G.loff[k] = lo;
+    if (G.lead[k] > 0) {
+      int cnt = 0;
+      for (int j = i; j < n_r && cnt < kPairCap; ++j)
+        if (V.pid[j] == mypid[k]) V.mlist[mo + cnt++] = j;
+    }
+  }
+  wsync();
+  // leaders: dense block and in-place column Cholesky (:187-321)
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    G.fail[k] = 0;
+    int m = min(G.lead[k], kPairCap);
+    if (m > 0) {
+      int i = lane + WAVE * k;
+      int e1 = b0[k] < 0 ? B : b0[k], e2 = b1[k] < 0 ? B : b1[k];
+      int blo = min(e1, e2), bhi = max(e1, e2);
+      (void)i;
+      float* Lp = V.Lbuf + G.loff[k];
+      const int* mem = V.mlist + G.moff[k];
+      for (int a = 0; a < m; ++a) {
+        int ra = mem[a];
+        for (int c = 0; c <= a; ++c) {
+          int rc = mem[c];
+          float val = 0.f;
+          if (blo < B) val += sdot(bj_jac(V, ra, blo, B), apply_minv(sm + blo * BS, bj_jac(V, rc, blo, B)));
+          if (bhi < B) val += sdot(bj_jac(V, ra, bhi, B), apply_minv(sm + bhi * BS, bj_jac(V, rc, bhi, B)));
+          if (a == c) val += V.rowd[16 * ra + 14];
+          Lp[a * (a + 1) / 2 + c] = val;
+        }
+      }
+      for (int kk = 0; kk < m; ++kk) {
+        float ds = 0.f;
+        for (int q = 0; q < kk; ++q) { float l = Lp[kk * (kk + 1) / 2 + q]; ds += l * l; }
+        float dv = Lp[kk * (kk + 1) / 2 + kk] - ds;
+        if (dv <= 0.f) { G.fail[k] = 1; flags |= FLAG_FACTOR_FAILURE; break; }
+        float Lkk = sqrtf(dv);
+        Lp[kk * (kk + 1) / 2 + kk] = Lkk;
+        float inv = 1.f / Lkk;
+        for (int a = kk + 1; a < m; ++a) {
+          float rs = 0.f;
+          for (int q = 0; q < kk; ++q) rs += Lp[a * (a + 1) / 2 + q] * Lp[kk * (kk + 1) / 2 + q];
+          Lp[a * (a + 1) / 2 + kk] = (Lp[a * (a + 1) / 2 + kk] - rs) * inv;
+        }
+      }
+    }
+  }
+  wsync();
+}
+
+// out = P v (apply of per_body_pair_preconditioner.py:324-410)
+template <int R>
+__device__ void bj_apply(const BJView& V, int lane, int n_r, const bool (&act)[R], const float (&Pi)[R],
+                         const BJRegs<R>& G, const float (&v)[R], float (&out)[R]) {
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    int i = lane + WAVE * k;
+    if (i < n_r) V.vbuf[i] = v[k];
+  }
+  wsync();
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    int m = min(G.lead[k], kPairCap);
+    if (m > 0) {
+      const int* mem = V.mlist + G.moff[k];
+      float* ws = V.wbuf + G.moff[k];
+      if (G.fail[k]) {
+        for (int a = 0; a < m; ++a) ws[a] = V.rowd[16 * mem[a] + 15] * V.vbuf[mem[a]];
+      } else {
+        const float* Lp = V.Lbuf + G.loff[k];
+        for (int a = 0; a < m; ++a) {
+          float s = V.vbuf[mem[a]];
+          for (int c = 0; c < a; ++c) s -= Lp[a * (a + 1) / 2 + c] * ws[c];
+          ws[a] = s / Lp[a * (a + 1) / 2 + a];
+        }
+        for (int aa = 0; aa < m; ++aa) {
+          int a = m - 1 - aa;
+          float s = ws[a];
+          for (int c = a + 1; c < m; ++c) s -= Lp[c * (c + 1) / 2 + a] * ws[c];
+          ws[a] = s / Lp[a * (a + 1) / 2 + a];
+        }
+      }
+    }
+  }
+  wsync();
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    float o = 0.f;
+    if (act[k]) o = G.rank[k] < kPairCap ? V.wbuf[G.moff[k] + G.rank[k]] : Pi[k] * v[k];
+    out[k] = o;
+  }
+  wsync();
+}
 
 // out = (J M^-1 J^T + C + reg) v over the rows of one world (optim/system_operator.py:36-124):
 // scatter J^T v into the LDS body accumulator, apply M^-1 per body, gather.
@@ -124,6 +339,9 @@ __global__ void __launch_bounds__(256) fused_step_kernel(FusedParams P) {
   const RowLayout& L = P.L;
   float* lamprev = sm + B * BS;  // compacted row index
   const float dt = P.dt;
+  const BJView V = bj_view(lamprev + L.Nc + 1, P.rows_cap);
+  BJRegs<R> G;
+  int bj_flags = 0;
 
   // ---- load bodies (load_data, base_engine.py:303-308; engine.py:48-50) ----
   for (int b = lane; b < B; b += WAVE) {
@@ -229,11 +447,13 @@ __global__ void __launch_bounds__(256) fused_step_kernel(FusedParams P) {
       bsq += rhs[k] * rhs[k];
     }
     bsq = wave_sum(bsq);
+    if (P.bj) bj_setup<R>(V, sm, lane, n_r, B, act, b0, b1, J0, J1, Cr, Pi, G, bj_flags);
 
     OFB_PHASE(2)
     // ---- PCR (optim/pcr_solver.py:170-295), x0 = 0 => r = b ----
 #pragma unroll
     for (int k = 0; k < R; ++k) { x[k] = 0.f; r[k] = rhs[k]; z[k] = Pi[k] * r[k]; }
+    if (P.bj) bj_apply<R>(V, lane, n_r, act, Pi, G, r, z);
     schur_matvec<R>(sm, lane, B, act, b0, b1, J0, J1, Cr, z, Az);
     float zAz = 0.f, rsq = 0.f;
 #pragma unroll
@@ -247,7 +467,10 @@ __global__ void __launch_bounds__(256) fused_step_kernel(FusedParams P) {
       float zAz_old = zAz;
       float yAp = 0.f;
 #pragma unroll
-      for (int k = 0; k < R; ++k) { y[k] = Pi[k] * Ap[k]; yAp += y[k] * Ap[k]; }
+      for (int k = 0; k < R; ++k) y[k] = Pi[k] * Ap[k];
+      if (P.bj) bj_apply<R>(V, lane, n_r, act, Pi, G, Ap, y);
+#pragma unroll
+      for (int k = 0; k < R; ++k) yAp += y[k] * Ap[k];
       yAp = wave_sum(yAp);
       float a = yAp > 0.f ? zAz_old / yAp : 0.f;
 #pragma unroll
@@ -373,12 +596,15 @@ __global__ void __launch_bounds__(256) fused_step_kernel(FusedParams P) {
     P.lambda_out[(size_t)w * L.Nc + L.off_f + 2 * c] = 0.f;
     P.lambda_out[(size_t)w * L.Nc + L.off_f + 2 * c + 1] = 0.f;
   }
+  int bj_any = 0;
+  if (__any(bj_flags & FLAG_FACTOR_FAILURE)) bj_any |= FLAG_FACTOR_FAILURE;
+  if (__any(bj_flags & FLAG_PAIR_OVERFLOW)) bj_any |= FLAG_PAIR_OVERFLOW;
   if (lane == 0) {
     P.st_nr[w] = n_it;
     P.st_pcr[w] = pcr_total;
     P.st_merit[w] = picked_merit;
     P.st_best[w] = picked;
-    int fl = P.st_flags[w] & FLAG_CONTACT_OVERFLOW;
+    int fl = (P.st_flags[w] & FLAG_CONTACT_OVERFLOW) | bj_any;
     if (n_it >= P.cfg.nr_max_iters && merit > atol_sq) fl |= FLAG_NR_CAP;
     P.st_flags[w] = fl;
   }
@@ -407,13 +633,37 @@ __global__ void classify_worlds_kernel(int W, const int32_t* __restrict__ ccount
   lists[(size_t)c * W + slot] = w;
 }
 
+// LDS floats per wave of the class with R row slots per lane
+static int lds_words(const ofb_solver* s, int R) {
+  int base = s->m.B * BS + s->L.Nc + 1;
+  return base + (s->cfg.preconditioner == OFB_PRECOND_PER_BODY_PAIR ? bj_words(WAVE * R) : 0);
+}
+
 template <int R>
-static void launch_fused(FusedParams P, int cls, int32_t* counts, int32_t* lists, int wpb, size_t lds_bytes,
-                         hipStream_t stream) {
-  P.wlist = lists + (size_t)cls * P.m.W;
-  P.wcount = counts + cls;
+static void launch_fused(FusedParams P, const ofb_solver* s, int cls, hipStream_t stream) {
+  P.wlist = s->cls_lists + (size_t)cls * P.m.W;
+  P.wcount = s->cls_counts + cls;
+  P.rows_cap = WAVE * R;
+  P.lds_per_wave = lds_words(s, R);
+  // worlds (waves) per workgroup: as many as fit in 64 KB of LDS, at most the tuned value
+  int wpb = s->fused_worlds_per_block;
+  while (wpb > 1 && (size_t)wpb * P.lds_per_wave * sizeof(float) > 65536) --wpb;
+  size_t lds_bytes = (size_t)wpb * P.lds_per_wave * sizeof(float);
   int blocks = (P.m.W + wpb - 1) / wpb;
   hipLaunchKernelGGL(fused_step_kernel<R>, dim3(blocks), dim3(wpb * WAVE), lds_bytes, stream, P);
+}
+
+// lets a class kernel use more than 64 KB of LDS when one world needs it
+template <int R>
+static int allow_lds(const ofb_solver* s) {
+  size_t bytes = (size_t)lds_words(s, R) * sizeof(float);
+  if (bytes <= 65536) return 0;
+  if (hipFuncSetAttribute((const void*)fused_step_kernel<R>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)bytes) != hipSuccess) {
+    set_error("fused kernel needs %zu bytes of LDS per world", bytes);
+    return -4;
+  }
+  return 0;
 }
 
 #ifdef OFB_PHASE_TIMING
@@ -428,6 +678,17 @@ extern "C" int ofb_debug_phase_cycles(unsigned long long* out, int reset) {
 #endif
 
 int fused_classes_alloc(ofb_solver* s) {
+  int rc = 0;
+  const int r_max = (s->L.Nc + WAVE - 1) / WAVE;
+  for (int c = 0; c < kNumClasses; ++c) {
+    if (c > 0 && kClassRHost[c - 1] >= r_max) break;
+    switch (kClassRHost[c]) {
+#define CASE(RR) case RR: rc = allow_lds<RR>(s); break;
+      CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(10) CASE(12) CASE(16)
+#undef CASE
+    }
+    if (rc) return rc;
+  }
   if (hipMalloc((void**)&s->cls_counts, kNumClasses * sizeof(int32_t)) != hipSuccess ||
       hipMalloc((void**)&s->cls_lists, (size_t)kNumClasses * s->m.W * sizeof(int32_t)) != hipSuccess) {
     set_error("hipMalloc failed (world classes)");
@@ -451,9 +712,7 @@ int fused_step(ofb_solver* s, const ofb_state_in* in, const ofb_state_out* out, 
   P.lambda_seed = s->lambda_seed;
   P.lambda_out = s->lambda;
   P.st_nr = s->st_nr; P.st_pcr = s->st_pcr; P.st_best = s->st_best; P.st_flags = s->st_flags; P.st_merit = s->st_merit;
-  P.lds_per_wave = s->m.B * BS + s->L.Nc + 1;
-  const int wpb = s->fused_worlds_per_block;
-  const size_t lds_bytes = sizeof(float) * (size_t)P.lds_per_wave * wpb;
+  P.bj = s->cfg.preconditioner == OFB_PRECOND_PER_BODY_PAIR;
   const int W = s->m.W;
   const int r_max = (s->L.Nc + WAVE - 1) / WAVE;
   if (r_max > kClassRHost[kNumClasses - 1]) {
@@ -467,7 +726,7 @@ int fused_step(ofb_solver* s, const ofb_state_in* in, const ofb_state_out* out, 
   for (int c = 0; c < kNumClasses; ++c) {
     if (c > 0 && kClassRHost[c - 1] >= r_max) break;  // no world can need this class
     switch (kClassRHost[c]) {
-#define CASE(RR) case RR: launch_fused<RR>(P, c, s->cls_counts, s->cls_lists, wpb, lds_bytes, stream); break;
+#define CASE(RR) case RR: launch_fused<RR>(P, s, c, stream); break;
       CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(10) CASE(12) CASE(16)
 #undef CASE
     }
