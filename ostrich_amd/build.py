@@ -1,17 +1,32 @@
 """Builds the gfx950 solver library ostrich_amd/_lib/libostrich_fb.so with hipcc.
 
-Packed-FP32 instructions are disabled for every device pass
-(-Xclang -target-feature -Xclang -packed-fp32-ops).
+Each translation unit compiles to an object under ostrich_amd/_build/ in parallel; the
+fused kernel's instances are split over four units (fb_fused_classes.hip compiled with
+OFB_BJ x OFB_RSET), then hipcc links the shared library.  Packed-FP32 instructions are
+disabled for every device pass (-Xclang -target-feature -Xclang -packed-fp32-ops).
+Extra hipcc flags may be given in OFB_HIPCC_FLAGS (e.g. -DOFB_PHASE_TIMING).
 """
 import os
+import shlex
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_lib")
+OBJ_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT_DIR, "libostrich_fb.so")
-SOURCES = ["fb_api.hip", "fb_fused.hip", "fb_plain.hip"]
+# (object name, source, extra defines)
+UNITS = [
+    ("fb_api", "fb_api.hip", []),
+    ("fb_fused", "fb_fused.hip", []),
+    ("fb_plain", "fb_plain.hip", []),
+    ("fb_fused_j0", "fb_fused_classes.hip", ["-DOFB_BJ=0", "-DOFB_RSET=0"]),
+    ("fb_fused_j1", "fb_fused_classes.hip", ["-DOFB_BJ=0", "-DOFB_RSET=1"]),
+    ("fb_fused_b0", "fb_fused_classes.hip", ["-DOFB_BJ=1", "-DOFB_RSET=0"]),
+    ("fb_fused_b1", "fb_fused_classes.hip", ["-DOFB_BJ=1", "-DOFB_RSET=1"]),
+]
 ARCH = os.environ.get("OFB_OFFLOAD_ARCH", "gfx950")
 
 
@@ -22,12 +37,35 @@ def hipcc():
     return "hipcc"
 
 
-def build(verbose=False):
-    os.makedirs(OUT_DIR, exist_ok=True)
-    cmd = [hipcc(), f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-shared", "-fPIC",
+def _compile(unit, verbose):
+    name, src, defs = unit
+    obj = os.path.join(OBJ_DIR, name + ".o")
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-c",
            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-           "-Wno-unused-result", "-I", os.path.join(HERE, "..", "include"),
-           "-o", LIB + ".tmp"] + [os.path.join(CSRC, s) for s in SOURCES]
+           "-Wno-unused-result", "-I", os.path.join(HERE, "..", "include")] + defs
+    cmd += shlex.split(os.environ.get("OFB_HIPCC_FLAGS", ""))
+    cmd += ["-o", obj, os.path.join(CSRC, src)]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    return obj, r
+
+
+def build(verbose=False, jobs=None):
+    os.makedirs(OUT_DIR, exist_ok=True)
+    os.makedirs(OBJ_DIR, exist_ok=True)
+    jobs = jobs or min(len(UNITS), max(1, min(16, os.cpu_count() or 1)))
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        results = list(ex.map(lambda u: _compile(u, verbose), UNITS))
+    failed = [(o, r) for o, r in results if r.returncode != 0]
+    for o, r in results:
+        msg = "\n".join(l for l in (r.stdout + r.stderr).splitlines() if "packed-fp32-ops" not in l
+                        and "is not a recognized feature" not in l and l.strip())
+        if msg and (verbose or r.returncode != 0):
+            print(msg, file=sys.stderr)
+    if failed:
+        raise RuntimeError("hipcc failed for " + ", ".join(os.path.basename(o) for o, _ in failed))
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", LIB + ".tmp"] + [o for o, _ in results]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

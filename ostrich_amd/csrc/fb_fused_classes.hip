@@ -1,0 +1,45 @@
+// fb_fused_classes.hip -- instances of the fused kernel for one preconditioner and one
+// set of row classes.  Compiled once per (OFB_BJ, OFB_RSET) pair so that the instances
+// build in parallel; OFB_BJ = 1 selects per-body-pair block-Jacobi, OFB_RSET = 0 the
+// classes R = 1..5 and 1 the classes R = 6..16.
+
+#include "fb_fused_kernel.h"
+
+#if !defined(OFB_BJ) || !defined(OFB_RSET)
+#error "compile with -DOFB_BJ=0|1 -DOFB_RSET=0|1"
+#endif
+
+#define OFB_CAT_(a, b, c, d) a##b##c##d
+#define OFB_CAT(a, b, c, d) OFB_CAT_(a, b, c, d)
+
+namespace ofb {
+
+int OFB_CAT(fused_class_op_bj, OFB_BJ, _set, OFB_RSET)(int R, int op, const FusedParams& P, int blocks, int threads,
+                                                     size_t lds, hipStream_t st) {
+  switch (R) {
+#define CASE(RR) \
+  case RR: return fused_class_op<RR, OFB_BJ != 0>(op, P, blocks, threads, lds, st);
+#if OFB_RSET == 0
+    CASE(1) CASE(2) CASE(3) CASE(4) CASE(5)
+#else
+    CASE(6) CASE(7) CASE(8) CASE(10) CASE(12) CASE(16)
+#endif
+#undef CASE
+  }
+  return -1;
+}
+
+// Profiling build only: the phase counters of the Jacobi instances R = 1..5 (each
+// translation unit holds its own copy of the counters).
+#if defined(OFB_PHASE_TIMING) && OFB_BJ == 0 && OFB_RSET == 0
+extern "C" int ofb_debug_phase_cycles(unsigned long long* out, int reset) {
+  if (hipMemcpyFromSymbol(out, HIP_SYMBOL(g_phase_cycles), sizeof(unsigned long long) * 8) != hipSuccess) return -4;
+  if (reset) {
+    unsigned long long z[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (hipMemcpyToSymbol(HIP_SYMBOL(g_phase_cycles), z, sizeof(z)) != hipSuccess) return -4;
+  }
+  return 0;
+}
+#endif
+
+}  // namespace ofb

@@ -1,0 +1,653 @@
+// fb_fused_kernel.h -- fused persistent Fisher-Burmeister Newton solver for gfx950.
+//
+// One wavefront (64 lanes) owns one world for the whole step: the Newton loop
+// (src/ostrich/core/base_engine.py:416-526), the linear-system assembly
+// (core/linear_system.py:140-341), the Jacobi-preconditioned conjugate-residual solve
+// (optim/pcr_solver.py:170-295), the Newton update (core/newton_step.py:22-54), the
+// merit evaluation (core/residual.py:18-173) and best-iterate backtracking
+// (core/backtracking.py:12-95) all run without leaving the kernel.
+//
+// Layout on chip:
+//   * constraint rows of the world are compacted to
+//     [joints | control | normals(n_c) | friction(2 n_c)] and dealt to lanes,
+//     row i -> lane i % 64, slot i / 64; every per-row quantity (Jacobian pair, C,
+//     Jacobi diagonal, the seven PCR vectors, lambda) lives in VGPRs, so PCR vector
+//     updates are pure VALU and dot products are in-register partial sums followed by
+//     one DPP cross-lane reduction;
+//   * body state (pose, velocity, previous step, world inverse inertia, the J^T x
+//     accumulator) lives in LDS, 96 floats per body, and couples the rows: the Schur
+//     matvec J M^-1 J^T x is a scatter of J^T x into LDS (ds_add_f32 from one wave, so
+//     the accumulation order is fixed), a per-body M^-1 and a gather;
+//   * a world leaves the Newton loop on its own criterion (per-world termination), and
+//     its wave retires, freeing the SIMD for the next world.
+// Waves of one workgroup never synchronise with each other; every LDS exchange is
+// wave-local.
+
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "fb_math.h"
+#include "fb_rows.h"
+#include "fb_solver.h"
+
+namespace ofb {
+
+using namespace rows;
+
+#ifdef OFB_PHASE_TIMING
+// Profiling build only: per-phase cycle counts of the Newton loop, summed over worlds.
+__device__ unsigned long long g_phase_cycles[8];
+#define OFB_PHASE_INIT unsigned long long ph_[8] = {0, 0, 0, 0, 0, 0, 0, 0}; unsigned long long ph_t = clock64();
+#define OFB_PHASE(k) { unsigned long long t_ = clock64(); ph_[k] += t_ - ph_t; ph_t = t_; }
+#define OFB_PHASE_FLUSH if (lane == 0) for (int k_ = 0; k_ < 8; ++k_) atomicAdd(&g_phase_cycles[k_], ph_[k_]);
+#else
+#define OFB_PHASE_INIT
+#define OFB_PHASE(k)
+#define OFB_PHASE_FLUSH
+#endif
+
+struct FusedParams {
+  DeviceModel m;
+  RowLayout L;
+  ofb_config cfg;
+  float dt;
+  const float* body_q;
+  const float* body_qd;
+  const float* body_f;
+  const float* target_pos;
+  const float* target_vel;
+  float* out_q;
+  float* out_qd;
+  const int32_t* ccount;
+  const float* cf;
+  const int32_t* ci;
+  const int32_t* topo;  // [Nj] joint of each joint row, [Nj] row within joint, [Nctrl] joint of each control row
+  const float* lambda_seed;
+  float* lambda_out;
+  int32_t *st_nr, *st_pcr, *st_best, *st_flags;
+  float* st_merit;
+  int lds_per_wave;  // floats
+  const int32_t* wlist;   // worlds of this launch's row class
+  const int32_t* wcount;  // number of them
+  int rows_cap;           // 64 R: row capacity of this launch's class
+};
+
+// ---------------------------------------------------------------------------------
+// Per-body-pair block-Jacobi inside one wavefront (optim/per_body_pair_preconditioner.py:
+// 62-410).  Rows that couple the same two bodies (static side = body B) form a pair; a
+// row's rank is its position among the pair's rows in row order, and rows ranked 64 or
+// more fall back to the Jacobi diagonal (the reference's fixed 64-member blocks, :59).
+//
+// Work is spread over the pair's member lanes instead of one thread per pair:
+//  * structure (once per step: row activity and bodies do not change inside a step):
+//    pair ids, ranks, sizes, the pair's slot range and packed-block offset;
+//  * factor (once per Newton iteration): lane a builds row a of the packed lower
+//    triangle of A_pair = sum_b J_a,b M_b^-1 J_c,b^T (+ C + reg on the diagonal), then
+//    the wave sweeps the block in place, pivot by pivot (X_kk <- -1/X_kk, X_ik <-
+//    X_ik/X_kk, X_ij <- X_ij - X_ik X_kj / X_kk), which leaves -A_pair^-1.  The k-th
+//    pivot equals the k-th Cholesky pivot of the reference (A_kk - sum L_kq^2), so a
+//    pivot <= 0 is exactly its factorisation failure (:305-311); such a pair uses the
+//    Jacobi diagonal (:374-380);
+//  * apply (twice per PCR iteration): lane a forms -sum_c X_ac v_c over its pair, a
+//    dense row of at most 64 LDS reads with no serial dependence, where the reference
+//    runs two triangular solves (:384-410) - the same operator A_pair^-1.
+// LDS per wave after the bodies and lambda_prev (cap = 64 R rows):
+//   pid[cap] i32, lsz[cap] i32, colbuf[cap], vbuf[cap], rowd[16 cap] (J0 6, J1 6,
+//   b0 bits, b1 bits, C + reg, pad) indexed by member slot, X[R * 2080] packed blocks.
+// ---------------------------------------------------------------------------------
+constexpr int kPairCap = 64;
+constexpr int kPairL = kPairCap * (kPairCap + 1) / 2;
+
+__host__ __device__ inline int bj_words(int cap) { return 20 * cap + (cap / 64) * kPairL; }
+
+struct BJView {
+  int* pid;
+  int* lsz;
+  float* colbuf;
+  float* vbuf;
+  float* rowd;
+  float* X;
+};
+
+__device__ __forceinline__ BJView bj_view(float* base, int cap) {
+  BJView v;
+  v.pid = reinterpret_cast<int*>(base);
+  v.lsz = v.pid + cap;
+  v.colbuf = reinterpret_cast<float*>(v.lsz + cap);
+  v.vbuf = v.colbuf + cap;
+  v.rowd = v.vbuf + cap;
+  v.X = v.rowd + 16 * cap;
+  return v;
+}
+
+// Per-row-slot block-Jacobi bookkeeping kept in registers.
+template <int R>
+struct BJRegs {
+  int slot[R];  // member slot (pair offset + rank), -1 if the row is in no block
+  int rank[R];  // rank of the row inside its pair
+  int m[R];     // rows in the pair's block (<= 64)
+  int poff[R];  // offset of the pair's packed block in X
+  int fail[R];  // the pair's block has a pivot <= 0 this Newton iteration
+  int mmax;     // largest block of the wave
+};
+
+__device__ __forceinline__ int wave_max_int(int v) {
+  for (int off = 32; off >= 1; off >>= 1) v = max(v, __shfl_xor(v, off, WAVE));
+  return v;
+}
+
+// pair ids (_compute_pair_ids_kernel :62-109), member ranks and sizes (:112-144)
+template <int R>
+__device__ void bj_structure(const BJView& V, int lane, int n_r, int B, const bool (&act)[R], const int (&b0)[R],
+                             const int (&b1)[R], BJRegs<R>& G, int& flags) {
+  int mypid[R], leader[R];
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    int i = lane + WAVE * k;
+    mypid[k] = -1;
+    if (i < n_r) {
+      if (act[k]) {
+        int e1 = b0[k] < 0 ? B : b0[k], e2 = b1[k] < 0 ? B : b1[k];
+        mypid[k] = min(e1, e2) * (B + 1) + max(e1, e2);
+      }
+      V.pid[i] = mypid[k];
+    }
+  }
+  wsync();
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    int i = lane + WAVE * k;
+    int rank = 0, size = 0, lead = -1;
+    if (i < n_r && mypid[k] >= 0) {
+      for (int j = 0; j < n_r; ++j) {
+        if (V.pid[j] == mypid[k]) {
+          if (lead < 0) lead = j;
+          size += 1;
+          if (j < i) rank += 1;
+        }
+      }
+    }
+    G.rank[k] = rank;
+    G.m[k] = min(size, kPairCap);
+    leader[k] = lead;
+    if (i < n_r) V.lsz[i] = (lead == i) ? min(size, kPairCap) : 0;
+    if (lead == i && size > kPairCap) flags |= FLAG_PAIR_OVERFLOW;
+  }
+  wsync();
+  int mm = 0;
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    int mo = 0, po = 0;
+    for (int j = 0; j < leader[k]; ++j) {
+      int s = V.lsz[j];
+      mo += s;
+      po += s * (s + 1) / 2;
+    }
+    G.slot[k] = (leader[k] >= 0 && G.rank[k] < kPairCap) ? mo + G.rank[k] : -1;
+    G.poff[k] = po;
+    if (G.slot[k] >= 0) mm = max(mm, G.m[k]);
+  }
+  G.mmax = wave_max_int(mm);
+  wsync();
+}
+
+// dense blocks (_extract_pair_blocks_kernel :187-264) and their inverses
+template <int R>
+__device__ void bj_factor(const BJView& V, const float* sm, const int (&b0)[R], const int (&b1)[R],
+                          const sv (&J0)[R], const sv (&J1)[R], const float (&Cr)[R], BJRegs<R>& G, int& flags) {
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    G.fail[k] = 0;
+    if (G.slot[k] >= 0) {
+      float* d = V.rowd + 16 * G.slot[k];
+      d[0] = J0[k].a.x; d[1] = J0[k].a.y; d[2] = J0[k].a.z; d[3] = J0[k].b.x; d[4] = J0[k].b.y; d[5] = J0[k].b.z;
+      d[6] = J1[k].a.x; d[7] = J1[k].a.y; d[8] = J1[k].a.z; d[9] = J1[k].b.x; d[10] = J1[k].b.y; d[11] = J1[k].b.z;
+      d[12] = __int_as_float(b0[k]); d[13] = __int_as_float(b1[k]); d[14] = Cr[k];
+    }
+  }
+  wsync();
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    if (G.slot[k] < 0) continue;
+    sv m0 = svzero(), m1 = svzero();
+    if (b0[k] >= 0) m0 = apply_minv(sm + b0[k] * BS, J0[k]);
+    if (b1[k] >= 0) m1 = apply_minv(sm + b1[k] * BS, J1[k]);
+    const int rank = G.rank[k], base = G.slot[k] - rank;
+    float* Xr = V.X + G.poff[k] + rank * (rank + 1) / 2;
+    for (int c = 0; c <= rank; ++c) {
+      const float* dc = V.rowd + 16 * (base + c);
+      sv Jc0{mk(dc[0], dc[1], dc[2]), mk(dc[3], dc[4], dc[5])};
+      sv Jc1{mk(dc[6], dc[7], dc[8]), mk(dc[9], dc[10], dc[11])};
+      int c0 = __float_as_int(dc[12]), c1 = __float_as_int(dc[13]);
+      float val = 0.f;
+      if (b0[k] >= 0) { if (c0 == b0[k]) val += sdot(Jc0, m0); else if (c1 == b0[k]) val += sdot(Jc1, m0); }
+      if (b1[k] >= 0) { if (c0 == b1[k]) val += sdot(Jc0, m1); else if (c1 == b1[k]) val += sdot(Jc1, m1); }
+      if (c == rank) val += Cr[k];
+      Xr[c] = val;
+    }
+  }
+  wsync();
+  for (int piv = 0; piv < G.mmax; ++piv) {
+    float colv[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      colv[k] = 0.f;
+      if (G.slot[k] >= 0 && piv < G.m[k] && !G.fail[k]) {
+        const int rank = G.rank[k];
+        int e = rank >= piv ? rank * (rank + 1) / 2 + piv : piv * (piv + 1) / 2 + rank;
+        colv[k] = V.X[G.poff[k] + e];
+        V.colbuf[G.slot[k]] = colv[k];
+      }
+    }
+    wsync();
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      if (G.slot[k] < 0 || piv >= G.m[k] || G.fail[k]) continue;
+      const int rank = G.rank[k], base = G.slot[k] - rank;
+      const float dpiv = V.colbuf[base + piv];
+      if (dpiv <= 0.f) {
+        G.fail[k] = 1;
+        flags |= FLAG_FACTOR_FAILURE;
+        continue;
+      }
+      const float inv = 1.f / dpiv;
+      float* Xr = V.X + G.poff[k] + rank * (rank + 1) / 2;
+      if (rank == piv) {
+        for (int c = 0; c < rank; ++c) Xr[c] = V.colbuf[base + c] * inv;
+        Xr[rank] = -inv;
+      } else {
+        const float f = colv[k] * inv;
+        for (int c = 0; c <= rank; ++c) {
+          if (c == piv) Xr[c] = f;
+          else Xr[c] -= f * V.colbuf[base + c];
+        }
+      }
+    }
+    wsync();
+  }
+}
+
+// out = P v (apply of per_body_pair_preconditioner.py:324-410)
+template <int R>
+__device__ void bj_apply(const BJView& V, const bool (&act)[R], const float (&Pi)[R], const BJRegs<R>& G,
+                         const float (&v)[R], float (&out)[R]) {
+#pragma unroll
+  for (int k = 0; k < R; ++k)
+    if (G.slot[k] >= 0 && !G.fail[k]) V.vbuf[G.slot[k]] = v[k];
+  wsync();
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    float o = 0.f;
+    if (act[k]) {
+      if (G.slot[k] >= 0 && !G.fail[k]) {
+        const int rank = G.rank[k], base = G.slot[k] - rank;
+        const float* Xp = V.X + G.poff[k];
+        const int rowb = rank * (rank + 1) / 2;
+        float s = 0.f;
+        for (int c = 0; c < G.m[k]; ++c) {
+          int e = c <= rank ? rowb + c : c * (c + 1) / 2 + rank;
+          s += Xp[e] * V.vbuf[base + c];
+        }
+        o = -s;
+      } else {
+        o = Pi[k] * v[k];
+      }
+    }
+    out[k] = o;
+  }
+  wsync();
+}
+
+// out = (J M^-1 J^T + C + reg) v over the rows of one world (optim/system_operator.py:36-124):
+// scatter J^T v into the LDS body accumulator, apply M^-1 per body, gather.
+template <int R>
+__device__ __forceinline__ void schur_matvec(float* sm, int lane, int B, const bool (&act)[R], const int (&b0)[R],
+                                             const int (&b1)[R], const sv (&J0)[R], const sv (&J1)[R],
+                                             const float (&Cr)[R], const float (&v)[R], float (&out)[R]) {
+  for (int e = lane; e < 6 * B; e += WAVE) sm[(e / 6) * BS + BACC + (e % 6)] = 0.f;
+  wsync();
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    if (act[k]) {
+      if (b0[k] >= 0) atomic_add_sv(sm + b0[k] * BS + BACC, J0[k] * v[k]);
+      if (b1[k] >= 0) atomic_add_sv(sm + b1[k] * BS + BACC, J1[k] * v[k]);
+    }
+  }
+  wsync();
+  for (int b = lane; b < B; b += WAVE) {
+    float* bb = sm + b * BS;
+    stsv(bb + BVV, apply_minv(bb, ldsv(bb + BACC)));
+  }
+  wsync();
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    float o = 0.f;
+    if (act[k]) {
+      if (b0[k] >= 0) o += sdot(J0[k], ldsv(sm + b0[k] * BS + BVV));
+      if (b1[k] >= 0) o += sdot(J1[k], ldsv(sm + b1[k] * BS + BVV));
+      o += Cr[k] * v[k];
+    }
+    out[k] = o;
+  }
+  wsync();
+}
+
+// ------------------------------------------------------------------------------------
+// The kernel.  R = row slots per lane (rows per world <= 64 R); BJ = per-body-pair
+// block-Jacobi preconditioner (else Jacobi), a template flag so that the Jacobi kernel
+// carries none of the block code's registers.  The R = 1 instance (most worlds) is
+// held to 128 VGPRs so that four waves share a SIMD; the few spilled values cost less
+// than the latency hiding gained (3.68 -> 3.47 ms per 4096-world skid-steer step).
+// ------------------------------------------------------------------------------------
+#ifndef OFB_R1_WAVES_PER_EU
+#define OFB_R1_WAVES_PER_EU 4
+#endif
+template <int R, bool BJ>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(R == 1 ? OFB_R1_WAVES_PER_EU : 1)))
+fused_step_kernel(FusedParams P) {
+  extern __shared__ float lds[];
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int wave = threadIdx.x / WAVE;
+  const int idx = blockIdx.x * (blockDim.x / WAVE) + wave;
+  if (idx >= *P.wcount) return;
+  const int w = P.wlist[idx];
+  float* sm = lds + wave * P.lds_per_wave;
+  const int B = P.m.B;
+  const RowLayout& L = P.L;
+  float* lamprev = sm + B * BS;  // compacted row index
+  const float dt = P.dt;
+  const BJView V = bj_view(lamprev + L.Nc + 1, P.rows_cap);
+  BJRegs<R> G;
+  int bj_flags = 0;
+
+  // ---- load bodies (load_data, base_engine.py:303-308; engine.py:48-50) ----
+  for (int b = lane; b < B; b += WAVE) {
+    float* bb = sm + b * BS;
+    size_t k = (size_t)w * B + b;
+#pragma unroll
+    for (int i = 0; i < 7; ++i) { float v = P.body_q[k * 7 + i]; bb[BQ + i] = v; bb[BQP + i] = v; }
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      float v = P.body_qd[k * 6 + i];
+      bb[BU + i] = v; bb[BUP + i] = v;
+      bb[BF + i] = P.body_f[k * 6 + i];
+    }
+    bb[BM] = P.m.body_mass[k];
+    bb[BMINV] = P.m.body_inv_mass[k];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) bb[BCOM + i] = P.m.body_com[k * 3 + i];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) { bb[BIB + i] = P.m.body_inertia[k * 9 + i]; bb[BIIB + i] = P.m.body_inv_inertia[k * 9 + i]; }
+  }
+  const int n_c = min(P.ccount[w], L.K);
+  const int n_r = L.Nj + L.Nctrl + 3 * n_c;
+
+  // ---- per-row registers ----
+  float lam[R], lamb[R], x[R], r[R], z[R], p[R], Ap[R], y[R], Az[R], Cr[R], Pi[R], rhs[R];
+  sv J0[R], J1[R];
+  int b0[R], b1[R];
+  bool act[R];
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    int i = lane + WAVE * k;
+    float l0 = 0.f;
+    if (P.cfg.seed_iterate && i < n_r) l0 = P.lambda_seed[(size_t)w * L.Nc + original_row(L, i, n_c)];
+    lam[k] = l0; lamb[k] = l0;
+    act[k] = false; b0[k] = -1; b1[k] = -1;
+  }
+  wsync();
+
+  const float atol_sq = P.cfg.nr_atol * P.cfg.nr_atol;
+  const float tol_sq = P.cfg.lin_tol >= 0.f ? P.cfg.lin_tol * P.cfg.lin_tol : -1.f;
+  const float latol_sq = P.cfg.lin_atol >= 0.f ? P.cfg.lin_atol * P.cfg.lin_atol : -1.f;
+  const float reg = P.cfg.regularization;
+  int n_it = 0, pcr_total = 0, best_idx = -1;
+  float best_merit = 0.f, merit = 0.f;
+
+  OFB_PHASE_INIT
+  while (true) {
+    // ---- friction-lag snapshot (base_engine.py:441) ----
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      int i = lane + WAVE * k;
+      if (i < n_r) lamprev[i] = lam[k];
+    }
+    dynamics_bodies(P, sm, lane, true);
+    wsync();
+    OFB_PHASE(0)
+    // ---- assembly (linear_system.py:140-322) ----
+    float hc[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      int i = lane + WAVE * k;
+      hc[k] = 0.f;
+      if (i < n_r) {
+        RowOut o = assemble_row(P, sm, lamprev, w, i, n_c, lam[k]);
+        J0[k] = o.J0; J1[k] = o.J1; b0[k] = o.b0; b1[k] = o.b1; act[k] = o.active;
+        hc[k] = o.hc;
+        Cr[k] = o.C + reg;
+        // inactive contact / friction rows force lambda := 0 (contact_constraint.py:289-296,
+        // friction_constraint.py:713-787)
+        if (!o.active && i >= L.Nj + L.Nctrl) lam[k] = 0.f;
+      } else {
+        act[k] = false;
+      }
+    }
+    wsync();
+    OFB_PHASE(1)
+    // M^-1 h_d per body
+    for (int b = lane; b < B; b += WAVE) {
+      float* bb = sm + b * BS;
+      stsv(bb + BVV, apply_minv(bb, ldsv(bb + BHD)));
+    }
+    wsync();
+    // Schur RHS (linear_system.py:17-65) and Jacobi diagonal (preconditioner.py:9-50)
+    float bsq = 0.f;
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      rhs[k] = 0.f; Pi[k] = 0.f;
+      if (act[k]) {
+        float s = 0.f, d = 0.f;
+        if (b0[k] >= 0) {
+          const float* bb = sm + b0[k] * BS;
+          s += sdot(J0[k], ldsv(bb + BVV));
+          d += sdot(J0[k], apply_minv(bb, J0[k]));
+        }
+        if (b1[k] >= 0) {
+          const float* bb = sm + b1[k] * BS;
+          s += sdot(J1[k], ldsv(bb + BVV));
+          d += sdot(J1[k], apply_minv(bb, J1[k]));
+        }
+        rhs[k] = (s - hc[k]) / dt;
+        Pi[k] = 1.f / (d + Cr[k] + 1e-6f);
+      }
+      bsq += rhs[k] * rhs[k];
+    }
+    bsq = wave_sum(bsq);
+    if (BJ) {
+      if (n_it == 0) bj_structure<R>(V, lane, n_r, B, act, b0, b1, G, bj_flags);
+      bj_factor<R>(V, sm, b0, b1, J0, J1, Cr, G, bj_flags);
+    }
+
+    OFB_PHASE(2)
+    // ---- PCR (optim/pcr_solver.py:170-295), x0 = 0 => r = b ----
+#pragma unroll
+    for (int k = 0; k < R; ++k) { x[k] = 0.f; r[k] = rhs[k]; z[k] = Pi[k] * r[k]; }
+    if (BJ) bj_apply<R>(V, act, Pi, G, r, z);
+    schur_matvec<R>(sm, lane, B, act, b0, b1, J0, J1, Cr, z, Az);
+    float zAz = 0.f, rsq = 0.f;
+#pragma unroll
+    for (int k = 0; k < R; ++k) { p[k] = z[k]; Ap[k] = Az[k]; zAz += z[k] * Az[k]; rsq += r[k] * r[k]; }
+    zAz = wave_sum(zAz);
+    rsq = wave_sum(rsq);
+    float target = latol_sq;
+    if (bsq * tol_sq > target) target = bsq * tol_sq;
+    int it = 0;
+    while (true) {
+      float zAz_old = zAz;
+      float yAp = 0.f;
+#pragma unroll
+      for (int k = 0; k < R; ++k) y[k] = Pi[k] * Ap[k];
+      if (BJ) bj_apply<R>(V, act, Pi, G, Ap, y);
+#pragma unroll
+      for (int k = 0; k < R; ++k) yAp += y[k] * Ap[k];
+      yAp = wave_sum(yAp);
+      float a = yAp > 0.f ? zAz_old / yAp : 0.f;
+#pragma unroll
+      for (int k = 0; k < R; ++k) { x[k] += a * p[k]; r[k] -= a * Ap[k]; z[k] -= a * y[k]; }
+      schur_matvec<R>(sm, lane, B, act, b0, b1, J0, J1, Cr, z, Az);
+      zAz = 0.f;
+#pragma unroll
+      for (int k = 0; k < R; ++k) zAz += z[k] * Az[k];
+      zAz = wave_sum(zAz);
+      float bt = zAz_old > 0.f ? zAz / zAz_old : 0.f;
+      rsq = 0.f;
+#pragma unroll
+      for (int k = 0; k < R; ++k) { p[k] = z[k] + bt * p[k]; Ap[k] = Az[k] + bt * Ap[k]; rsq += r[k] * r[k]; }
+      rsq = wave_sum(rsq);
+      ++it;
+      if (!(rsq > target) || it >= P.cfg.lin_max_iters) break;
+    }
+    pcr_total += it;
+    OFB_PHASE(3)
+
+    // ---- velocity recovery + Newton update (linear_system.py:344-377, newton_step.py:22-54) ----
+    for (int e = lane; e < 6 * B; e += WAVE) sm[(e / 6) * BS + BACC + (e % 6)] = 0.f;
+    wsync();
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      if (act[k]) {
+        if (b0[k] >= 0) atomic_add_sv(sm + b0[k] * BS + BACC, J0[k] * x[k]);
+        if (b1[k] >= 0) atomic_add_sv(sm + b1[k] * BS + BACC, J1[k] * x[k]);
+      }
+      lam[k] = lam[k] + x[k];
+    }
+    wsync();
+    for (int b = lane; b < B; b += WAVE) {
+      float* bb = sm + b * BS;
+      sv acc = ldsv(bb + BACC), hd = ldsv(bb + BHD);
+      sv du = apply_minv(bb, sv{acc.a * dt - hd.a, acc.b * dt - hd.b});
+      sv u = ldsv(bb + BU);
+      u = sv{u.a + du.a, u.b + du.b};
+      stsv(bb + BU, u);
+      sttf(bb + BQ, integrate(u, ldtf(bb + BQP), ld3(bb + BCOM), dt));
+    }
+    wsync();
+
+    OFB_PHASE(4)
+    // ---- residual at the new iterate (core/residual.py:18-173) ----
+    dynamics_bodies(P, sm, lane, false);
+    wsync();
+    float msum = 0.f;
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      int i = lane + WAVE * k;
+      if (i < n_r) {
+        RowOut o = assemble_row(P, sm, lamprev, w, i, n_c, lam[k]);
+        msum += o.hc * o.hc;
+      }
+    }
+    wsync();
+    for (int e = lane; e < 6 * B; e += WAVE) {
+      float v = sm[(e / 6) * BS + BHD + (e % 6)];
+      msum += v * v;
+    }
+    merit = wave_sum(msum);
+    OFB_PHASE(5)
+
+    // ---- candidate bookkeeping (history_group.py + backtracking.py:12-95) ----
+    int kidx = n_it;
+    ++n_it;
+    bool take = (kidx == P.cfg.nr_backtrack_min_iter) || (kidx > P.cfg.nr_backtrack_min_iter && merit < best_merit);
+    if (take) {
+      best_idx = kidx;
+      best_merit = merit;
+      for (int e = lane; e < 13 * B; e += WAVE) {
+        int b = e / 13, c = e % 13;
+        float* bb = sm + b * BS;
+        if (c < 7) bb[BBQ + c] = bb[BQ + c];
+        else bb[BBU + c - 7] = bb[BU + c - 7];
+      }
+#pragma unroll
+      for (int k = 0; k < R; ++k) lamb[k] = lam[k];
+    }
+    // ---- per-world stop rule (base_engine.py:42-80) ----
+    bool keep = merit > atol_sq;
+    if (n_it < P.cfg.nr_backtrack_min_iter) keep = true;
+    if (n_it >= P.cfg.nr_max_iters) keep = false;
+    OFB_PHASE(6)
+    if (!keep) break;
+    wsync();
+  }
+  wsync();
+  OFB_PHASE_FLUSH
+  // ---- pick the candidate ----
+  int picked = n_it - 1;
+  float picked_merit = merit;
+  if (n_it > P.cfg.nr_backtrack_min_iter && best_idx >= 0) {
+    picked = best_idx;
+    picked_merit = best_merit;
+    for (int e = lane; e < 13 * B; e += WAVE) {
+      int b = e / 13, c = e % 13;
+      float* bb = sm + b * BS;
+      if (c < 7) bb[BQ + c] = bb[BBQ + c];
+      else bb[BU + c - 7] = bb[BBU + c - 7];
+    }
+#pragma unroll
+    for (int k = 0; k < R; ++k) lam[k] = lamb[k];
+  }
+  wsync();
+  // ---- outputs (engine.py:68-69) ----
+  for (int e = lane; e < 13 * B; e += WAVE) {
+    int b = e / 13, c = e % 13;
+    const float* bb = sm + b * BS;
+    size_t k = (size_t)w * B + b;
+    if (c < 7) P.out_q[k * 7 + c] = bb[BQ + c];
+    else P.out_qd[k * 6 + c - 7] = bb[BU + c - 7];
+  }
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    int i = lane + WAVE * k;
+    if (i < n_r) P.lambda_out[(size_t)w * L.Nc + original_row(L, i, n_c)] = lam[k];
+  }
+  // contact slots beyond the world's contact count hold lambda = 0
+  for (int c = n_c + lane; c < L.K; c += WAVE) {
+    P.lambda_out[(size_t)w * L.Nc + L.off_n + c] = 0.f;
+    P.lambda_out[(size_t)w * L.Nc + L.off_f + 2 * c] = 0.f;
+    P.lambda_out[(size_t)w * L.Nc + L.off_f + 2 * c + 1] = 0.f;
+  }
+  int bj_any = 0;
+  if (__any(bj_flags & FLAG_FACTOR_FAILURE)) bj_any |= FLAG_FACTOR_FAILURE;
+  if (__any(bj_flags & FLAG_PAIR_OVERFLOW)) bj_any |= FLAG_PAIR_OVERFLOW;
+  if (lane == 0) {
+    P.st_nr[w] = n_it;
+    P.st_pcr[w] = pcr_total;
+    P.st_merit[w] = picked_merit;
+    P.st_best[w] = picked;
+    int fl = (P.st_flags[w] & FLAG_CONTACT_OVERFLOW) | bj_any;
+    if (n_it >= P.cfg.nr_max_iters && merit > atol_sq) fl |= FLAG_NR_CAP;
+    P.st_flags[w] = fl;
+  }
+}
+
+// Host side of one class instance: op 0 launches it, op 1 raises its dynamic-LDS limit
+// to `lds` bytes.  Instances are compiled in several translation units
+// (fb_fused_classes.hip) so that the build runs in parallel.
+template <int R, bool BJ>
+int fused_class_op(int op, const FusedParams& P, int blocks, int threads, size_t lds, hipStream_t stream) {
+  if (op == 1)
+    return hipFuncSetAttribute((const void*)fused_step_kernel<R, BJ>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)lds) == hipSuccess ? 0 : -4;
+  hipLaunchKernelGGL((fused_step_kernel<R, BJ>), dim3(blocks), dim3(threads), lds, stream, P);
+  return 0;
+}
+
+// defined in fb_fused_classes.hip, one per (preconditioner, class set) translation unit;
+// return -1 when R is not in that unit's set
+int fused_class_op_bj0_set0(int R, int op, const FusedParams& P, int blocks, int threads, size_t lds, hipStream_t st);
+int fused_class_op_bj0_set1(int R, int op, const FusedParams& P, int blocks, int threads, size_t lds, hipStream_t st);
+int fused_class_op_bj1_set0(int R, int op, const FusedParams& P, int blocks, int threads, size_t lds, hipStream_t st);
+int fused_class_op_bj1_set1(int R, int op, const FusedParams& P, int blocks, int threads, size_t lds, hipStream_t st);
+
+}  // namespace ofb
