@@ -126,6 +126,11 @@ typedef struct ofb_config {
   int32_t seed_iterate;          /* warm_start.seed_iterate: start NR from the lambda seed */
   int32_t rigid_contact_max;     /* capacity of the flat contact arrays (Newton
                                     Contacts.rigid_contact_max); sizes the packing scratch */
+  float adjoint_regularization;  /* adjoint.regularization: gamma added to C of active rows */
+  int32_t adjoint_pose_vjp;      /* adjoint.pose_vjp: pull w^T dR/dq+ and w^T dR/dq- into the
+                                    pose gradients */
+  float adjoint_pose_eps;        /* central-difference step of that pull-back, in pose
+                                    coordinates (m and quaternion components) */
 } ofb_config;
 
 typedef struct ofb_state_in {
@@ -196,6 +201,55 @@ int ofb_get_stats(const ofb_solver* solver, int32_t* nr_iters, int32_t* pcr_iter
 /* Copies the picked iterate's constraint impulses lambda [W*N_c] (data._constr_force
  * after perform_backtracking) into a caller device buffer. */
 int ofb_get_lambda(const ofb_solver* solver, float* lambda_out, void* stream);
+
+/* Copies lambda_prev_iter [W*N_c] as the last step left it: the snapshot taken at the
+ * start of its last Newton iteration (core/base_engine.py:441), which backtracking does
+ * not restore.  With lambda it is what TrajectoryBuffer.save_step keeps per step
+ * (simulation/trajectory_buffer.py:222-223) for the adjoint.  Fused path only. */
+int ofb_get_lambda_prev(const ofb_solver* solver, float* lambda_prev_out, void* stream);
+
+/* Replaces the per-shape friction coefficients shape_mu [W*(S+G)] (OstrichModel
+ * shape_material_mu, core/model.py) copied at ofb_create, e.g. between steps of a
+ * gradient-based fit of mu.  Device pointer. */
+int ofb_set_shape_mu(ofb_solver* solver, const float* shape_mu, void* stream);
+
+/* Gradients flowing into one step's outputs: dL/dq+ [W*B*7] and dL/du+ [W*B*6]
+ * (data.body_pose_grad / body_vel_grad loaded by TrajectoryBuffer.load_step,
+ * simulation/trajectory_buffer.py:241-242). */
+typedef struct ofb_adjoint_in {
+  const float* body_q_grad;
+  const float* body_qd_grad;
+} ofb_adjoint_in;
+
+/* Gradients with respect to the step's inputs.  Every pointer may be NULL; non-NULL
+ * outputs are overwritten (not accumulated). */
+typedef struct ofb_adjoint_out {
+  float* body_q_grad;           /* dL/dq-       [W*B*7]  body_pose_prev.grad */
+  float* body_qd_grad;          /* dL/du-       [W*B*6]  body_vel_prev.grad */
+  float* body_f_grad;           /* dL/df        [W*B*6]  ext_force.grad */
+  float* joint_target_pos_grad; /* [W*D] */
+  float* joint_target_vel_grad; /* [W*D] */
+  float* joint_target_ke_grad;  /* [W*D] */
+  float* joint_target_kd_grad;  /* [W*D] */
+  float* shape_mu_grad;         /* [W*(S+G)] per world copy of shape_mu */
+  float* adjoint_lambda;        /* w_lambda [W*N_c], reference row layout (data.w.c) */
+  float* adjoint_body;          /* w_u [W*B*6] (data.w.d_spatial) */
+} ofb_adjoint_out;
+
+/* Adjoint of one step (OstrichEngine.step_backward, core/base_engine.py:528-743) at the
+ * state the forward step left: state_in = q-, u-, f of the step start; state_out = q+, u+
+ * after it (read only); lambda / lambda_prev from ofb_get_lambda / ofb_get_lambda_prev of
+ * that step; contacts = the step's contact set (re-packed here).  Rebuilds J, C, h at
+ * that state, forms w_u0 = M^-1 f(dL/dq+, dL/du+), solves (J M^-1 J^T + C + reg) w_lambda
+ * = J w_u0 with the configured PCR and preconditioner, sets w_u = w_u0 - M^-1 J^T w_lambda
+ * and returns w^T dR/dxi for every input xi plus the explicit pose carry
+ * (residual.py:176-406).  Per-world adjoint PCR counts and flags (bit 4: a friction row's
+ * mu derivative has no closed form, that part of shape_mu_grad is missing) are read with
+ * ofb_get_stats.  Fused path only. */
+int ofb_step_backward(ofb_solver* solver, const ofb_state_in* state_in, const ofb_state_out* state_out,
+                      const ofb_control* control, const ofb_contacts* contacts, const float* lambda,
+                      const float* lambda_prev, const ofb_adjoint_in* grad_in, const ofb_adjoint_out* grad_out,
+                      float dt, void* stream);
 
 /* Copies the packed per-world contact set of the last step: count [W] and, for each of
  * the W*C_max slots, shape0/shape1 (world-local columns), point0/point1/normal (after the

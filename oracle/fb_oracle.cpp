@@ -20,6 +20,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <limits>
 #include <vector>
 
 namespace {
@@ -175,6 +176,7 @@ template <class T> struct Batch {
   // rows [W*Nc]
   std::vector<T> lam, lam_prev, dlam, h_c, C, active, rhs, Pinv;
   std::vector<Sp<T>> J0, J1;
+  std::vector<T> dw_dmu;  // [W*Cmax] d(friction weight)/d(mu_x) of the isotropic cone, NaN if not closed-form
   std::vector<int> b0, b1;
 
   // PCR
@@ -228,6 +230,7 @@ template <class T> struct Batch {
     tpos.assign(size_t(W) * D, 0); tvel.assign(size_t(W) * D, 0);
     lam.assign(WC, 0); lam_prev.assign(WC, 0); dlam.assign(WC, 0); h_c.assign(WC, 0); C.assign(WC, 0);
     active.assign(WC, 0); rhs.assign(WC, 0); Pinv.assign(WC, 0);
+    dw_dmu.assign(WK, T(0));
     J0.assign(WC, spzero<T>()); J1.assign(WC, spzero<T>());
     b0.assign(WC, -1); b1.assign(WC, -1);
     r.assign(WC, 0); z.assign(WC, 0); Az.assign(WC, 0); p.assign(WC, 0); Ap.assign(WC, 0); y.assign(WC, 0);
@@ -638,6 +641,7 @@ template <class T> void friction_rows(Batch<T>& Bt, int w, bool full) {  // fric
     size_t k = size_t(w) * Bt.Cmax + c;
     int s0 = Bt.cs0[k], s1 = Bt.cs1[k];
     auto deactivate = [&]() {
+      Bt.dw_dmu[k] = 0;
       if (full) {
         Bt.active[r0] = 0; Bt.active[r1] = 0; Bt.lam[r0] = 0; Bt.lam[r1] = 0;
         Bt.J0[r0] = spzero<T>(); Bt.J0[r1] = spzero<T>(); Bt.J1[r0] = spzero<T>(); Bt.J1[r1] = spzero<T>();
@@ -706,6 +710,7 @@ template <class T> void friction_rows(Batch<T>& Bt, int w, bool full) {  // fric
     T wx, wy;
     if (Bt.cfg.friction_mode != OFB_FRICTION_CONE) {
       wx = 0; wy = 0;
+      Bt.dw_dmu[k] = 0;
     } else {
       const T eps = T(1e-8);
       T denom_eps = T(1e-6) * dt;
@@ -722,6 +727,17 @@ template <class T> void friction_rows(Batch<T>& Bt, int w, bool full) {  // fric
         T den = rr_ * raw + phi + denom_eps;
         T num = dn - phi;
         T wv = rr_ * (num / den);
+        // d wv / d mu for the adjoint: lim = mu fn_prev dt enters through gap
+        {
+          T sb = rr_ * gap, nrm = std::sqrt(dn * dn + sb * sb);
+          T dgap = raw < lim ? T(1) : T(0);
+          T dphi = rr_ * (T(1) - (nrm > T(0) ? sb / nrm : T(0))) * dgap * fn_prev * dt;
+          T dw = (wv < T(0) || wv > T(1e5)) ? T(0) : -rr_ * dphi * (den + num) / (den * den);
+          V3<T> a0 = Bt.saxis(w, s0), a1 = Bt.saxis(w, s1);
+          bool aniso = (dot(a0, a0) > T(1e-6) && bb0 >= 0) || (dot(a1, a1) > T(1e-6) && bb1 >= 0);
+          bool strib = mss > T(0) && vst > T(0);
+          Bt.dw_dmu[k] = (aniso || strib) ? std::numeric_limits<T>::quiet_NaN() : dw;
+        }
         wv = std::max(wv, T(0));
         wv = std::min(wv, T(1e5));
         wx = wv; wy = wv;
@@ -745,6 +761,7 @@ template <class T> void friction_rows(Batch<T>& Bt, int w, bool full) {  // fric
         wt = std::max(wt, T(0));
         wt = std::min(wt, T(1e5));
         wx = wt * imx * imx; wy = wt * imy * imy;
+        Bt.dw_dmu[k] = std::numeric_limits<T>::quiet_NaN();
       }
     }
     Sp<T> d0 = spzero<T>(), d1 = spzero<T>();
@@ -1098,7 +1115,8 @@ template <class T> int run_step(const ofb_model* m, const ofb_config* cfg, const
                                 const float* f_in, const float* tpos, const float* tvel, const ofb_contacts* contacts,
                                 float dt, const double* lam_seed, double* q_out, double* qd_out, double* lam_out,
                                 int32_t* nr_iters, int32_t* pcr_iters, double* merit_out, int32_t* best_out,
-                                int32_t* flags_out, int32_t* ccount_out, double* merit_hist, double* res_out) {
+                                int32_t* flags_out, int32_t* ccount_out, double* merit_hist, double* res_out,
+                                double* lam_prev_out) {
   Batch<T> Bt(*m, *cfg);
   Bt.dt = T(dt);
   const int W = Bt.W, B = Bt.B;
@@ -1181,12 +1199,235 @@ template <class T> int run_step(const ofb_model* m, const ofb_config* cfg, const
       for (int i = 0; i < 6; ++i) qd_out[k * 6 + i] = Bt.u[k].v[i];
     }
     if (lam_out) for (int i = 0; i < Bt.Nc; ++i) lam_out[Bt.ri(w, i)] = Bt.lam[Bt.ri(w, i)];
+    // lambda_prev_iter as the step leaves it: the snapshot taken at the start of the last
+    // Newton iteration (base_engine.py:441), which backtracking does not restore
+    if (lam_prev_out) for (int i = 0; i < Bt.Nc; ++i) lam_prev_out[Bt.ri(w, i)] = Bt.lam_prev[Bt.ri(w, i)];
     if (nr_iters) nr_iters[w] = Bt.nr_iters[w];
     if (pcr_iters) pcr_iters[w] = Bt.pcr_iters[w];
     if (merit_out) merit_out[w] = Bt.merit[w];
     if (best_out) best_out[w] = Bt.best[w];
     if (flags_out) flags_out[w] = Bt.flags[w];
     if (ccount_out) ccount_out[w] = Bt.ccount[w];
+  }
+  return 0;
+}
+
+// ----------------------------------------------------------------------------------------
+// Adjoint of one step (base_engine.py:528-743, adjoint/adjoint_utils.py:11-169,
+// residual.py:176-406) at the state the forward step left: q+ = q_in, u+ = qd_in,
+// lambda, lambda_prev_iter; q-, u-, f of the step start.
+// ----------------------------------------------------------------------------------------
+// driving force -(g_u + dt G(q)^T g_q) - [0; dt (R(q)(-com)) x g_p] (Gt_matvec,
+// kinematic_mapping.py:208-279; compute_body_adjoint_init_kernel, adjoint_utils.py:12-79)
+template <class T> Sp<T> adjoint_drive(const Tf<T>& q, V3<T> com, const T* gq, const T* gu, T dt) {
+  T t2 = q.q.x, t3 = q.q.y, t4 = q.q.z, t1 = q.q.w;
+  T d2 = gq[3], d3 = gq[4], d4 = gq[5], d1 = gq[6];
+  T s = dt * T(0.5);
+  Sp<T> o;
+  for (int i = 0; i < 3; ++i) o.v[i] = gu[i] + gq[i] * dt;
+  o.v[3] = gu[3] + (-t2 * d1 + t1 * d2 - t4 * d3 + t3 * d4) * s;
+  o.v[4] = gu[4] + (-t3 * d1 + t4 * d2 + t1 * d3 - t2 * d4) * s;
+  o.v[5] = gu[5] + (-t4 * d1 - t3 * d2 + t2 * d3 + t1 * d4) * s;
+  for (int i = 0; i < 6; ++i) o.v[i] = -o.v[i];
+  V3<T> lever = qrot(q.q, neg(com));
+  V3<T> tq = cross(lever, v3<T>(gq[0], gq[1], gq[2]));
+  o.v[3] -= tq.x * dt; o.v[4] -= tq.y * dt; o.v[5] -= tq.z * dt;
+  return o;
+}
+
+// explicit carry dq+/dq- of a pose gradient (body_pose_prev_grad_kernel, residual.py:260-339)
+template <class T> void adjoint_carry(const T* g, const Sp<T>& vel, T dt, T* out) {
+  T w1 = vel.v[3], w2 = vel.v[4], w3 = vel.v[5], s = dt * T(0.5);
+  T gx = g[3], gy = g[4], gz = g[5], gw = g[6];
+  out[0] = g[0]; out[1] = g[1]; out[2] = g[2];
+  out[3] = gx + s * (w3 * gy - w2 * gz - w1 * gw);
+  out[4] = gy + s * (-w3 * gx + w1 * gz - w2 * gw);
+  out[5] = gz + s * (w2 * gx - w1 * gy - w3 * gw);
+  out[6] = gw + s * (w1 * gx + w2 * gy + w3 * gz);
+}
+
+// s = sum_rows w_lambda h_c + sum_bodies w_u . h_d at the current state of world w
+template <class T> T adjoint_weighted_residual(Batch<T>& Bt, int w, const std::vector<T>& wl,
+                                               const std::vector<Sp<T>>& wu) {
+  residual(Bt, w);
+  T s = 0;
+  for (int i = 0; i < Bt.Nc; ++i) s += wl[Bt.ri(w, i)] * Bt.h_c[Bt.ri(w, i)];
+  for (int b = 0; b < Bt.B; ++b) s += sdot(wu[Bt.bi(w, b)], Bt.h_d[Bt.bi(w, b)]);
+  return s;
+}
+
+template <class T> T& pose_coord(Tf<T>& X, int k) {
+  switch (k) {
+    case 0: return X.p.x; case 1: return X.p.y; case 2: return X.p.z;
+    case 3: return X.q.x; case 4: return X.q.y; case 5: return X.q.z;
+    default: return X.q.w;
+  }
+}
+
+template <class T> int run_adjoint(const ofb_model* m, const ofb_config* cfg, const double* q_prev_in,
+                                   const double* qd_prev_in, const float* f_in, const double* q_in,
+                                   const double* qd_in, const float* tpos, const float* tvel,
+                                   const ofb_contacts* contacts, float dt, const double* lam_in,
+                                   const double* lam_prev_in, const double* g_q, const double* g_u, double fd_eps,
+                                   double* g_q_prev, double* g_u_prev, double* g_f, double* g_tpos, double* g_tvel,
+                                   double* g_ke, double* g_kd, double* g_mu, double* w_lam_out, double* w_u_out,
+                                   int32_t* pcr_iters, int32_t* flags_out) {
+  Batch<T> Bt(*m, *cfg);
+  Bt.dt = T(dt);
+  const int W = Bt.W, B = Bt.B, D = Bt.D, Nc = Bt.Nc, SC = Bt.SC;
+  pack_contacts(Bt, *contacts);
+  auto ld_tf = [](const double* x) {
+    return Tf<T>{v3<T>(T(x[0]), T(x[1]), T(x[2])), Qt<T>{T(x[3]), T(x[4]), T(x[5]), T(x[6])}};
+  };
+  std::vector<T> wl(size_t(W) * Nc, T(0));
+  std::vector<Sp<T>> wu(size_t(W) * B, spzero<T>());
+  std::vector<int> all(W);
+  for (int w = 0; w < W; ++w) {
+    all[w] = w;
+    for (int b = 0; b < B; ++b) {
+      size_t k = Bt.bi(w, b);
+      Bt.q_prev[k] = ld_tf(q_prev_in + k * 7);
+      Bt.q[k] = ld_tf(q_in + k * 7);
+      for (int i = 0; i < 6; ++i) {
+        Bt.u_prev[k].v[i] = T(qd_prev_in[k * 6 + i]);
+        Bt.u[k].v[i] = T(qd_in[k * 6 + i]);
+        Bt.f_ext[k].v[i] = T(f_in[k * 6 + i]);
+      }
+    }
+    for (int d = 0; d < D; ++d) { Bt.tpos[size_t(w) * D + d] = T(tpos[w * D + d]); Bt.tvel[size_t(w) * D + d] = T(tvel[w * D + d]); }
+    for (int i = 0; i < Nc; ++i) { Bt.lam[Bt.ri(w, i)] = T(lam_in[Bt.ri(w, i)]); Bt.lam_prev[Bt.ri(w, i)] = T(lam_prev_in[Bt.ri(w, i)]); }
+    // compute_linear_system at the converged state (base_engine.py:533-535)
+    linear_system(Bt, w);
+    // adjoint_regularize_compliance_kernel: C += gamma on active rows
+    if (cfg->adjoint_regularization > 0)
+      for (int i = 0; i < Nc; ++i) if (Bt.active[Bt.ri(w, i)] > T(0)) Bt.C[Bt.ri(w, i)] += T(cfg->adjoint_regularization);
+    // w_u0 = M^-1 f_drive (compute_body_adjoint_init_kernel)
+    for (int b = 0; b < B; ++b) {
+      size_t k = Bt.bi(w, b);
+      T gq[7], gu[6];
+      for (int i = 0; i < 7; ++i) gq[i] = T(g_q[k * 7 + i]);
+      for (int i = 0; i < 6; ++i) gu[i] = T(g_u[k * 6 + i]);
+      wu[k] = minv(Bt, w, b, adjoint_drive(Bt.q[k], Bt.com(w, b), gq, gu, Bt.dt));
+    }
+    // b = J w_u0 (compute_adjoint_rhs_kernel)
+    for (int i = 0; i < Nc; ++i) {
+      size_t rr = Bt.ri(w, i);
+      T v = 0;
+      if (Bt.active[rr] > T(0)) {
+        if (Bt.b0[rr] >= 0) v += sdot(Bt.J0[rr], wu[Bt.bi(w, Bt.b0[rr])]);
+        if (Bt.b1[rr] >= 0) v += sdot(Bt.J1[rr], wu[Bt.bi(w, Bt.b1[rr])]);
+      }
+      Bt.rhs[rr] = v;
+      Bt.dlam[rr] = 0;
+    }
+    if (cfg->preconditioner == OFB_PRECOND_PER_BODY_PAIR) pair_update(Bt, w);
+    else jacobi_update(Bt, w);
+  }
+  // A w_lambda = b, same operator and PCR (base_engine.py:633-644)
+  pcr_solve(Bt, all, cfg->termination == OFB_TERM_GLOBAL);
+  for (int w = 0; w < W; ++w) {
+    // w_u -= M^-1 J^T w_lambda (subtract_constraint_feedback_kernel)
+    for (int i = 0; i < Nc; ++i) {
+      size_t rr = Bt.ri(w, i);
+      wl[rr] = Bt.dlam[rr];
+      if (Bt.active[rr] == T(0)) continue;
+      for (int s_ = 0; s_ < 2; ++s_) {
+        int b = s_ == 0 ? Bt.b0[rr] : Bt.b1[rr];
+        if (b < 0) continue;
+        Sp<T> a = minv(Bt, w, b, sscale(s_ == 0 ? Bt.J0[rr] : Bt.J1[rr], wl[rr]));
+        for (int k = 0; k < 6; ++k) wu[Bt.bi(w, b)].v[k] -= a.v[k];
+      }
+    }
+    int flags = 0;
+    // closed-form parts of w^T dR/dxi
+    for (int b = 0; b < B; ++b) {
+      size_t k = Bt.bi(w, b);
+      M3<T> Iw = world_inertia(Bt.q[k].q, Bt.inertia(w, b));
+      Sp<T> gp = spatial_momentum(-Bt.mass(w, b), Iw, wu[k]);  // dynamics: d h_d / d u- = -M
+      for (int i = 3; i < 6; ++i) gp.v[i] = -gp.v[i];
+      for (int i = 0; i < 6; ++i) {
+        if (g_u_prev) g_u_prev[k * 6 + i] = double(gp.v[i]);
+        if (g_f) g_f[k * 6 + i] = double(-Bt.dt * wu[k].v[i]);  // d h_d / d f = -dt
+      }
+    }
+    for (int d = 0; d < D; ++d) {
+      size_t kd = size_t(w) * D + d;
+      if (g_tpos) g_tpos[kd] = 0;
+      if (g_tvel) g_tvel[kd] = 0;
+      if (g_ke) g_ke[kd] = 0;
+      if (g_kd) g_kd[kd] = 0;
+    }
+    for (int j = 0; j < Bt.J; ++j) {  // control_target_grad_kernel (residual.py:201-257)
+      if (Bt.ctrl_off[j] < 0) continue;
+      size_t rr = Bt.ri(w, Bt.Nj + Bt.ctrl_off[j]);
+      if (Bt.active[rr] == T(0)) continue;
+      int qs = m->joint_qd_start[w * Bt.J + j];
+      int mode = m->joint_dof_mode[w * D + qs];
+      size_t kd = size_t(w) * D + qs;
+      T wlam = wl[rr], lam = Bt.lam[rr];
+      T ke = T(m->joint_target_ke[w * D + qs]), kdg = T(m->joint_target_kd[w * D + qs]);
+      if (mode == 1) {
+        if (g_tpos) g_tpos[kd] += double(wlam * (T(-1) / Bt.dt));
+        T den = Bt.dt * Bt.dt * ke + Bt.dt * kdg;
+        if (den > T(1e-6)) {
+          T inv2 = T(1) / (den * den);
+          if (g_ke) g_ke[kd] += double(wlam * (-Bt.dt * Bt.dt * inv2) * lam * Bt.dt);
+          if (g_kd) g_kd[kd] += double(wlam * (-Bt.dt * inv2) * lam * Bt.dt);
+        }
+      } else if (mode == 2) {
+        if (g_tvel) g_tvel[kd] += double(-wlam);
+        T den = Bt.dt * ke;
+        if (den > T(1e-6)) {
+          T inv2 = T(1) / (den * den);
+          if (g_ke) g_ke[kd] += double(wlam * (-Bt.dt * inv2) * lam * Bt.dt);
+        }
+      }
+    }
+    // friction coefficient: h_f = v_t + w(mu) lambda on both tangent rows
+    residual(Bt, w);  // refreshes dw_dmu at the converged state
+    if (g_mu) for (int s_ = 0; s_ < SC; ++s_) g_mu[size_t(w) * SC + s_] = 0;
+    for (int c = 0; c < std::min(Bt.ccount[w], Bt.Cmax); ++c) {
+      size_t kc = size_t(w) * Bt.Cmax + c;
+      size_t r0 = Bt.ri(w, Bt.Nj + Bt.Nctrl + Bt.Cmax + 2 * c), r1 = r0 + 1;
+      T dw = Bt.dw_dmu[kc];
+      T contrib = wl[r0] * Bt.lam[r0] + wl[r1] * Bt.lam[r1];
+      if (std::isnan(dw)) { if (contrib != T(0)) flags |= 16; continue; }
+      if (dw == T(0) || !g_mu) continue;
+      int s0 = Bt.cs0[kc], s1 = Bt.cs1[kc];
+      g_mu[size_t(w) * SC + s0] += double(T(0.5) * dw * contrib);
+      g_mu[size_t(w) * SC + s1] += double(T(0.5) * dw * contrib);
+    }
+    // pose pull-back w^T dR/dq+ and w^T dR/dq- by central differences of the residual
+    std::vector<T> gplus(size_t(B) * 7, T(0)), gminus(size_t(B) * 7, T(0));
+    if (cfg->adjoint_pose_vjp) {
+      T h = T(fd_eps);
+      for (int b = 0; b < B; ++b) {
+        for (int side = 0; side < 2; ++side) {
+          Tf<T>& X = side == 0 ? Bt.q[Bt.bi(w, b)] : Bt.q_prev[Bt.bi(w, b)];
+          for (int kk = 0; kk < 7; ++kk) {
+            T& v = pose_coord(X, kk);
+            T orig = v;
+            v = orig + h;
+            T sp_ = adjoint_weighted_residual(Bt, w, wl, wu);
+            v = orig - h;
+            T sm_ = adjoint_weighted_residual(Bt, w, wl, wu);
+            v = orig;
+            (side == 0 ? gplus : gminus)[b * 7 + kk] = (sp_ - sm_) / (T(2) * h);
+          }
+        }
+      }
+    }
+    for (int b = 0; b < B; ++b) {
+      size_t k = Bt.bi(w, b);
+      T G[7], out[7];
+      for (int i = 0; i < 7; ++i) G[i] = T(g_q[k * 7 + i]) + gplus[b * 7 + i];
+      adjoint_carry(G, Bt.u[k], Bt.dt, out);
+      if (g_q_prev) for (int i = 0; i < 7; ++i) g_q_prev[k * 7 + i] = double(out[i] + gminus[b * 7 + i]);
+      if (w_u_out) for (int i = 0; i < 6; ++i) w_u_out[k * 6 + i] = double(wu[k].v[i]);
+    }
+    if (w_lam_out) for (int i = 0; i < Nc; ++i) w_lam_out[Bt.ri(w, i)] = double(wl[Bt.ri(w, i)]);
+    if (pcr_iters) pcr_iters[w] = Bt.pcr_iters[w];
+    if (flags_out) flags_out[w] = flags;
   }
   return 0;
 }
@@ -1204,18 +1445,19 @@ int oracle_dims(const ofb_model* m, const ofb_config* cfg, int32_t* n_c, int32_t
 
 // One OstrichEngine.step in FP32 (precision 32) or FP64 (precision 64).  State is passed
 // as double so FP64 rollouts are not rounded between steps; model and contacts are f32.
-// Optional diagnostics: merit of every Newton iteration [W*nr_max_iters] and the
-// residual [W*(6B+N_c)] of the last iterate.
+// Optional diagnostics: merit of every Newton iteration [W*nr_max_iters], the
+// residual [W*(6B+N_c)] of the last iterate and lambda_prev_iter [W*N_c] as the step
+// leaves it.
 int oracle_step(const ofb_model* m, const ofb_config* cfg, const double* q_in, const double* qd_in,
                 const float* f_in, const float* tpos, const float* tvel, const ofb_contacts* contacts,
                 float dt, int precision, const double* lam_seed, double* q_out, double* qd_out,
                 double* lam_out, int32_t* nr_iters, int32_t* pcr_iters, double* merit, int32_t* best,
-                int32_t* flags, int32_t* ccount, double* merit_hist, double* res_out) {
+                int32_t* flags, int32_t* ccount, double* merit_hist, double* res_out, double* lam_prev_out) {
   if (precision == 64)
     return run_step<double>(m, cfg, q_in, qd_in, f_in, tpos, tvel, contacts, dt, lam_seed, q_out, qd_out, lam_out,
-                            nr_iters, pcr_iters, merit, best, flags, ccount, merit_hist, res_out);
+                            nr_iters, pcr_iters, merit, best, flags, ccount, merit_hist, res_out, lam_prev_out);
   return run_step<float>(m, cfg, q_in, qd_in, f_in, tpos, tvel, contacts, dt, lam_seed, q_out, qd_out, lam_out,
-                         nr_iters, pcr_iters, merit, best, flags, ccount, merit_hist, res_out);
+                         nr_iters, pcr_iters, merit, best, flags, ccount, merit_hist, res_out, lam_prev_out);
 }
 
 // Scaled FB value and derivatives (complementarity.py:22-47), FP64, for pinning tests.
@@ -1295,6 +1537,62 @@ int oracle_linear_system(const ofb_model* m, const ofb_config* cfg, const double
     }
   }
   return 0;
+}
+
+// s = sum_rows w_lambda h_c + sum_bodies w_u . h_d of R(q+, q-, u+, u-, f, targets, mu,
+// lambda, lambda_prev) in FP64: the quantity whose gradient the adjoint's VJP returns,
+// for finite-difference checks of every term.
+double oracle_weighted_residual(const ofb_model* m, const ofb_config* cfg, const double* q_prev,
+                                const double* qd_prev, const float* f_in, const double* q, const double* qd,
+                                const float* tpos, const float* tvel, const ofb_contacts* contacts, float dt,
+                                const double* lam, const double* lam_prev, const double* wl_in,
+                                const double* wu_in) {
+  Batch<double> Bt(*m, *cfg);
+  Bt.dt = dt;
+  pack_contacts(Bt, *contacts);
+  const int W = Bt.W, B = Bt.B, D = Bt.D, Nc = Bt.Nc;
+  std::vector<double> wl(size_t(W) * Nc);
+  std::vector<Sp<double>> wu(size_t(W) * B);
+  double total = 0;
+  for (int w = 0; w < W; ++w) {
+    for (int b = 0; b < B; ++b) {
+      size_t k = Bt.bi(w, b);
+      const double* a = q_prev + k * 7;
+      const double* c = q + k * 7;
+      Bt.q_prev[k] = Tf<double>{v3<double>(a[0], a[1], a[2]), Qt<double>{a[3], a[4], a[5], a[6]}};
+      Bt.q[k] = Tf<double>{v3<double>(c[0], c[1], c[2]), Qt<double>{c[3], c[4], c[5], c[6]}};
+      for (int i = 0; i < 6; ++i) {
+        Bt.u_prev[k].v[i] = qd_prev[k * 6 + i]; Bt.u[k].v[i] = qd[k * 6 + i]; Bt.f_ext[k].v[i] = f_in[k * 6 + i];
+        wu[k].v[i] = wu_in[k * 6 + i];
+      }
+    }
+    for (int d = 0; d < D; ++d) { Bt.tpos[size_t(w) * D + d] = tpos[w * D + d]; Bt.tvel[size_t(w) * D + d] = tvel[w * D + d]; }
+    for (int i = 0; i < Nc; ++i) {
+      size_t rr = Bt.ri(w, i);
+      Bt.lam[rr] = lam[rr]; Bt.lam_prev[rr] = lam_prev[rr]; wl[rr] = wl_in[rr];
+    }
+    total += adjoint_weighted_residual(Bt, w, wl, wu);
+  }
+  return total;
+}
+
+// Adjoint of one step (OstrichEngine.step_backward), FP32 (precision 32) or FP64 (64).
+// Gradients are written in double; any output pointer may be null.  flags bit 4: a
+// friction row contributes but its mu derivative has no closed form here (anisotropic
+// or Stribeck friction), so that part of g_mu is missing.
+int oracle_adjoint(const ofb_model* m, const ofb_config* cfg, const double* q_prev, const double* qd_prev,
+                   const float* f_in, const double* q, const double* qd, const float* tpos, const float* tvel,
+                   const ofb_contacts* contacts, float dt, const double* lam, const double* lam_prev,
+                   const double* g_q, const double* g_u, int precision, double fd_eps, double* g_q_prev,
+                   double* g_u_prev, double* g_f, double* g_tpos, double* g_tvel, double* g_ke, double* g_kd,
+                   double* g_mu, double* w_lam, double* w_u, int32_t* pcr_iters, int32_t* flags) {
+  if (precision == 64)
+    return run_adjoint<double>(m, cfg, q_prev, qd_prev, f_in, q, qd, tpos, tvel, contacts, dt, lam, lam_prev, g_q,
+                               g_u, fd_eps, g_q_prev, g_u_prev, g_f, g_tpos, g_tvel, g_ke, g_kd, g_mu, w_lam, w_u,
+                               pcr_iters, flags);
+  return run_adjoint<float>(m, cfg, q_prev, qd_prev, f_in, q, qd, tpos, tvel, contacts, dt, lam, lam_prev, g_q, g_u,
+                            fd_eps, g_q_prev, g_u_prev, g_f, g_tpos, g_tvel, g_ke, g_kd, g_mu, w_lam, w_u, pcr_iters,
+                            flags);
 }
 
 }  // extern "C"

@@ -31,7 +31,12 @@ def lib():
         _lib = C.CDLL(LIB_PATH)
         vp = C.c_void_p
         _lib.oracle_step.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, C.c_float, C.c_int, vp,
-                                     vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+                                     vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+        _lib.oracle_adjoint.argtypes = ([vp] * 10 + [C.c_float] + [vp] * 4 + [C.c_int, C.c_double]
+                                        + [vp] * 12)
+        _lib.oracle_adjoint.restype = C.c_int
+        _lib.oracle_weighted_residual.argtypes = [vp] * 10 + [C.c_float] + [vp] * 4
+        _lib.oracle_weighted_residual.restype = C.c_double
         _lib.oracle_step.restype = C.c_int
         _lib.oracle_dims.argtypes = [vp, vp, vp, vp, vp]
         _lib.oracle_fb_diff.argtypes = [C.c_double] * 5 + [vp]
@@ -104,14 +109,56 @@ def step(model, config, state, control, contacts, dt, precision=32, lam_seed=Non
         best_iter=np.zeros(W, np.int32), flags=np.zeros(W, np.int32), contact_count=np.zeros(W, np.int32),
         merit_hist=np.full((W, config.nr.max_iters), np.nan),
         residual=np.zeros((W, 6 * model.bodies_per_world + nc)),
+        lam_prev=np.zeros((W, nc)),
     )
     rc = lib().oracle_step(C.byref(hm.abi), C.byref(cfg), _p(q_in), _p(qd_in), _p(f_in), _p(tp), _p(tv),
                            C.byref(hc.abi), C.c_float(dt), precision, _p(seed), _p(out["body_q"]),
                            _p(out["body_qd"]), _p(out["lam"]), _p(out["nr_iters"]), _p(out["pcr_iters"]),
                            _p(out["merit"]), _p(out["best_iter"]), _p(out["flags"]), _p(out["contact_count"]),
-                           _p(out["merit_hist"]), _p(out["residual"]))
+                           _p(out["merit_hist"]), _p(out["residual"]), _p(out["lam_prev"]))
     if rc != 0:
         raise RuntimeError(f"oracle_step failed ({rc})")
+    return out
+
+
+def adjoint(model, config, state_in, state_out, control, contacts, dt, lam, lam_prev, grad_q, grad_qd,
+            precision=64, fd_eps=None, host_model=None):
+    """Adjoint of one step at the state the forward step left (OstrichEngine.step_backward).
+
+    state_in: body_q, body_qd, body_f of the step start; state_out: body_q, body_qd after
+    the step; grad_q [W*B, 7] and grad_qd [W*B, 6] are dL/dq+ and dL/du+.  Returns dL/dq-,
+    dL/du-, dL/df, dL/dtarget_pos, dL/dtarget_vel, dL/dke, dL/dkd, dL/dmu (per world
+    shape), the adjoint multipliers w_lambda and w_u, PCR iterations and flags."""
+    hm = host_model or HostModel(model)
+    hc = contacts if isinstance(contacts, HostContacts) else HostContacts(contacts)
+    cfg = config.to_abi()
+    W, B, D = model.num_worlds, model.bodies_per_world, model.dofs_per_world
+    SC = model.shapes_per_world + model.num_global_shapes
+    WB = W * B
+    nc, _, _ = dims(model, config)
+    f64 = lambda a, shape: np.ascontiguousarray(np.asarray(a, np.float64).reshape(shape))
+    f32 = lambda a, shape: np.ascontiguousarray(np.asarray(a, np.float32).reshape(shape))
+    qp, qdp = f64(state_in["body_q"], (WB, 7)), f64(state_in["body_qd"], (WB, 6))
+    fi = f32(state_in["body_f"], (WB, 6))
+    q, qd = f64(state_out["body_q"], (WB, 7)), f64(state_out["body_qd"], (WB, 6))
+    tp = f32(control["joint_target_pos"], (W * D,)) if D else np.zeros(1, np.float32)
+    tv = f32(control["joint_target_vel"], (W * D,)) if D else np.zeros(1, np.float32)
+    la, lp = f64(lam, (W, nc)), f64(lam_prev, (W, nc))
+    gq, gu = f64(grad_q, (WB, 7)), f64(grad_qd, (WB, 6))
+    eps = config.adjoint.pose_fd_eps if fd_eps is None else fd_eps
+    out = dict(body_q=np.zeros((WB, 7)), body_qd=np.zeros((WB, 6)), body_f=np.zeros((WB, 6)),
+               joint_target_pos=np.zeros((W, max(D, 1))), joint_target_vel=np.zeros((W, max(D, 1))),
+               joint_target_ke=np.zeros((W, max(D, 1))), joint_target_kd=np.zeros((W, max(D, 1))),
+               shape_mu=np.zeros((W, SC)), w_lambda=np.zeros((W, nc)), w_u=np.zeros((WB, 6)),
+               pcr_iters=np.zeros(W, np.int32), flags=np.zeros(W, np.int32))
+    rc = lib().oracle_adjoint(C.byref(hm.abi), C.byref(cfg), _p(qp), _p(qdp), _p(fi), _p(q), _p(qd), _p(tp), _p(tv),
+                              C.byref(hc.abi), C.c_float(dt), _p(la), _p(lp), _p(gq), _p(gu), precision,
+                              C.c_double(eps), _p(out["body_q"]), _p(out["body_qd"]), _p(out["body_f"]),
+                              _p(out["joint_target_pos"]), _p(out["joint_target_vel"]), _p(out["joint_target_ke"]),
+                              _p(out["joint_target_kd"]), _p(out["shape_mu"]), _p(out["w_lambda"]), _p(out["w_u"]),
+                              _p(out["pcr_iters"]), _p(out["flags"]))
+    if rc != 0:
+        raise RuntimeError(f"oracle_adjoint failed ({rc})")
     return out
 
 
@@ -152,3 +199,23 @@ def integrate_pose(vel, pose, com, dt):
     out = np.zeros(7)
     lib().oracle_integrate_pose(_p(v), _p(p), _p(c), C.c_double(dt), _p(out))
     return out
+
+
+def weighted_residual(model, config, state_in, state_out, control, contacts, dt, lam, lam_prev, w_lambda, w_u,
+                      host_model=None):
+    """FP64 sum_rows w_lambda h_c + sum_bodies w_u . h_d at the given step data."""
+    hm = host_model or HostModel(model)
+    hc = contacts if isinstance(contacts, HostContacts) else HostContacts(contacts)
+    cfg = config.to_abi()
+    W, B, D = model.num_worlds, model.bodies_per_world, model.dofs_per_world
+    WB = W * B
+    nc, _, _ = dims(model, config)
+    f64 = lambda a, shape: np.ascontiguousarray(np.asarray(a, np.float64).reshape(shape))
+    f32 = lambda a, shape: np.ascontiguousarray(np.asarray(a, np.float32).reshape(shape))
+    tp = f32(control["joint_target_pos"], (W * D,)) if D else np.zeros(1, np.float32)
+    tv = f32(control["joint_target_vel"], (W * D,)) if D else np.zeros(1, np.float32)
+    args = [f64(state_in["body_q"], (WB, 7)), f64(state_in["body_qd"], (WB, 6)), f32(state_in["body_f"], (WB, 6)),
+            f64(state_out["body_q"], (WB, 7)), f64(state_out["body_qd"], (WB, 6)), tp, tv]
+    tail = [f64(lam, (W, nc)), f64(lam_prev, (W, nc)), f64(w_lambda, (W, nc)), f64(w_u, (WB, 6))]
+    return lib().oracle_weighted_residual(C.byref(hm.abi), C.byref(cfg), *[_p(a) for a in args], C.byref(hc.abi),
+                                          C.c_float(dt), *[_p(a) for a in tail])

@@ -81,6 +81,9 @@ class OfbConfig(C.Structure):
         ("termination", C.c_int32),
         ("seed_iterate", C.c_int32),
         ("rigid_contact_max", C.c_int32),
+        ("adjoint_regularization", C.c_float),
+        ("adjoint_pose_vjp", C.c_int32),
+        ("adjoint_pose_eps", C.c_float),
     ]
 
 
@@ -107,6 +110,25 @@ class OfbContacts(C.Structure):
         ("shape1", C.c_void_p),
         ("margin0", C.c_void_p),
         ("margin1", C.c_void_p),
+    ]
+
+
+class OfbAdjointIn(C.Structure):
+    _fields_ = [("body_q_grad", C.c_void_p), ("body_qd_grad", C.c_void_p)]
+
+
+class OfbAdjointOut(C.Structure):
+    _fields_ = [
+        ("body_q_grad", C.c_void_p),
+        ("body_qd_grad", C.c_void_p),
+        ("body_f_grad", C.c_void_p),
+        ("joint_target_pos_grad", C.c_void_p),
+        ("joint_target_vel_grad", C.c_void_p),
+        ("joint_target_ke_grad", C.c_void_p),
+        ("joint_target_kd_grad", C.c_void_p),
+        ("shape_mu_grad", C.c_void_p),
+        ("adjoint_lambda", C.c_void_p),
+        ("adjoint_body", C.c_void_p),
     ]
 
 

@@ -11,6 +11,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "lib
 EXPORTS = (
     "ofb_create", "ofb_destroy", "ofb_dims", "ofb_step", "ofb_get_stats", "ofb_get_lambda",
     "ofb_get_packed_contacts", "ofb_set_lambda_seed", "ofb_last_solve_ms", "ofb_last_error",
+    "ofb_get_lambda_prev", "ofb_set_shape_mu", "ofb_step_backward",
 )
 
 _lib = None
@@ -34,6 +35,9 @@ def load():
     lib.ofb_set_lambda_seed.argtypes = [vp, vp, vp]
     lib.ofb_last_solve_ms.argtypes = [vp, C.POINTER(C.c_float)]
     lib.ofb_last_error.restype = C.c_char_p
+    lib.ofb_get_lambda_prev.argtypes = [vp, vp, vp]
+    lib.ofb_set_shape_mu.argtypes = [vp, vp, vp]
+    lib.ofb_step_backward.argtypes = [vp] * 9 + [C.c_float, vp]
     for name in EXPORTS:
         if name != "ofb_last_error":
             getattr(lib, name).restype = C.c_int

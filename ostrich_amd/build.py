@@ -26,6 +26,10 @@ UNITS = [
     ("fb_fused_j1", "fb_fused_classes.hip", ["-DOFB_BJ=0", "-DOFB_RSET=1"]),
     ("fb_fused_b0", "fb_fused_classes.hip", ["-DOFB_BJ=1", "-DOFB_RSET=0"]),
     ("fb_fused_b1", "fb_fused_classes.hip", ["-DOFB_BJ=1", "-DOFB_RSET=1"]),
+    ("fb_adj_j0", "fb_adjoint_classes.hip", ["-DOFB_BJ=0", "-DOFB_RSET=0"]),
+    ("fb_adj_j1", "fb_adjoint_classes.hip", ["-DOFB_BJ=0", "-DOFB_RSET=1"]),
+    ("fb_adj_b0", "fb_adjoint_classes.hip", ["-DOFB_BJ=1", "-DOFB_RSET=0"]),
+    ("fb_adj_b1", "fb_adjoint_classes.hip", ["-DOFB_BJ=1", "-DOFB_RSET=1"]),
 ]
 ARCH = os.environ.get("OFB_OFFLOAD_ARCH", "gfx950")
 

@@ -195,6 +195,9 @@ class AdjointConfig:
     soft_blending_temperature: float = 0.05
     regularization: float = 0.0
     gradient_normalization: bool = False
+    # step of the central differences that form w^T dR/dq+ and w^T dR/dq- on the GPU
+    # (the reference differentiates its residual kernels on a Warp tape instead)
+    pose_fd_eps: float = 2e-3
 
     def __post_init__(self):
         if self.friction_linearization not in ("true", "frozen"):
@@ -209,6 +212,8 @@ class AdjointConfig:
             )
         if self.regularization < 0:
             raise ValueError(f"adjoint.regularization must be >= 0, got {self.regularization}")
+        if not self.pose_fd_eps > 0:
+            raise ValueError(f"adjoint.pose_fd_eps must be > 0, got {self.pose_fd_eps}")
 
 
 @dataclass(frozen=True)
@@ -310,6 +315,9 @@ class OstrichEngineConfig:
         )
         c.seed_iterate = 1 if self.warm_start.seed_iterate else 0
         c.rigid_contact_max = 0
+        c.adjoint_regularization = self.adjoint.regularization
+        c.adjoint_pose_vjp = 1 if self.adjoint.pose_vjp else 0
+        c.adjoint_pose_eps = self.adjoint.pose_fd_eps
         return c
 
     @property

@@ -200,7 +200,7 @@ int ofb_destroy(ofb_solver* s) {
                   m.shape_body, m.shape_mu, m.shape_friction_axis, m.shape_mu_perp, m.shape_mu_stiction_scale,
                   m.shape_v_stribeck, m.shape_stribeck_lateral_only, m.shape_world, s->ccount, s->cf, s->ci,
                   s->pk_keys, s->pk_keys_out, s->pk_vals, s->pk_vals_out, s->pk_seg_start, s->pk_temp, s->topo,
-                  s->lambda, s->lambda_seed, s->st_nr, s->st_pcr, s->st_best, s->st_flags, s->st_merit};
+                  s->lambda, s->lambda_prev, s->lambda_seed, s->st_nr, s->st_pcr, s->st_best, s->st_flags, s->st_merit};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (s->plain) plain_destroy(s);
@@ -301,7 +301,7 @@ int ofb_create(const ofb_model* model, const ofb_config* config, int32_t path, o
 #define ALLOC(ptr, n) if ((rc = dev_alloc(&s->ptr, n)) != 0) { ofb_destroy(s); return rc; }
   const size_t K = L.K;
   ALLOC(ccount, W); ALLOC(cf, (size_t)W * CF * K); ALLOC(ci, (size_t)W * 2 * K);
-  ALLOC(lambda, (size_t)W * L.Nc); ALLOC(lambda_seed, (size_t)W * L.Nc);
+  ALLOC(lambda, (size_t)W * L.Nc); ALLOC(lambda_prev, (size_t)W * L.Nc); ALLOC(lambda_seed, (size_t)W * L.Nc);
   ALLOC(st_nr, W); ALLOC(st_pcr, W); ALLOC(st_best, W); ALLOC(st_flags, W); ALLOC(st_merit, W);
   ALLOC(topo, topo.size());
   int cap = config->rigid_contact_max > 0 ? config->rigid_contact_max : W * L.K;
@@ -391,6 +391,50 @@ int ofb_get_lambda(const ofb_solver* s, float* lambda_out, void* stream) {
   if (!s || !lambda_out) { set_error("null argument"); return -1; }
   OFB_CHECK(hipMemcpyAsync(lambda_out, s->lambda, (size_t)s->m.W * s->L.Nc * 4, hipMemcpyDeviceToDevice,
                            (hipStream_t)stream));
+  return 0;
+}
+
+int ofb_get_lambda_prev(const ofb_solver* s, float* lambda_prev_out, void* stream) {
+  if (!s || !lambda_prev_out) { set_error("null argument"); return -1; }
+  if (s->path != OFB_PATH_FUSED) { set_error("lambda_prev_iter is recorded by the fused path only"); return -2; }
+  OFB_CHECK(hipMemcpyAsync(lambda_prev_out, s->lambda_prev, (size_t)s->m.W * s->L.Nc * 4, hipMemcpyDeviceToDevice,
+                           (hipStream_t)stream));
+  return 0;
+}
+
+int ofb_set_shape_mu(ofb_solver* s, const float* shape_mu, void* stream) {
+  if (!s || !shape_mu) { set_error("null argument"); return -1; }
+  OFB_CHECK(hipMemcpyAsync(s->m.shape_mu, shape_mu, (size_t)s->m.W * s->m.SC * 4, hipMemcpyDeviceToDevice,
+                           (hipStream_t)stream));
+  return 0;
+}
+
+int ofb_step_backward(ofb_solver* s, const ofb_state_in* in, const ofb_state_out* out, const ofb_control* ctl,
+                      const ofb_contacts* contacts, const float* lambda, const float* lambda_prev,
+                      const ofb_adjoint_in* gin, const ofb_adjoint_out* gout, float dt, void* stream) {
+  if (!s || !in || !out || !ctl || !contacts || !lambda || !lambda_prev || !gin || !gout) {
+    set_error("null argument");
+    return -1;
+  }
+  if (s->path != OFB_PATH_FUSED) { set_error("step_backward runs on the fused path"); return -2; }
+  if (!(dt > 0.f)) { set_error("dt must be positive"); return -2; }
+  if (!in->body_q || !in->body_qd || !in->body_f || !out->body_q || !out->body_qd) {
+    set_error("state pointers must be set");
+    return -1;
+  }
+  if (!gin->body_q_grad || !gin->body_qd_grad) { set_error("incoming gradients must be set"); return -1; }
+  if (s->m.D > 0 && (!ctl->joint_target_pos || !ctl->joint_target_vel)) { set_error("control pointers must be set"); return -1; }
+  if (!(s->cfg.adjoint_pose_eps > 0.f) && s->cfg.adjoint_pose_vjp) {
+    set_error("adjoint_pose_eps must be positive");
+    return -2;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  int rc = pack_contacts(s, contacts, st);
+  if (rc) return rc;
+  OFB_CHECK(hipEventRecord(s->ev_solve0, st));
+  rc = fused_adjoint(s, in, out, ctl, lambda, lambda_prev, gin, gout, dt, st);
+  if (rc) return rc;
+  OFB_CHECK(hipEventRecord(s->ev_solve1, st));
   return 0;
 }
 

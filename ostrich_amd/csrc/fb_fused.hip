@@ -1,7 +1,7 @@
 // fb_fused.hip -- host side of the fused persistent solver: per-step bucketing of worlds
 // into row classes, LDS sizing, and the class launches (kernel in fb_fused_kernel.h).
 
-#include "fb_fused_kernel.h"
+#include "fb_adjoint_kernel.h"
 
 namespace ofb {
 
@@ -34,6 +34,23 @@ static int lds_words(const ofb_solver* s, int R) {
   return base + (s->cfg.preconditioner == OFB_PRECOND_PER_BODY_PAIR ? bj_words(WAVE * R) : 0);
 }
 
+// LDS floats per wave of the adjoint kernel: the forward's bodies and lambda_prev, the
+// adjoint scratch, then the block-Jacobi region
+static int adj_lds_words(const ofb_solver* s, int R) {
+  int base = s->m.B * BS + s->L.Nc + 1 + adj_words(s->m.B, s->m.D, s->m.SC);
+  return base + (s->cfg.preconditioner == OFB_PRECOND_PER_BODY_PAIR ? bj_words(WAVE * R) : 0);
+}
+
+static int adj_class_op(int R, int op, const AdjParams& P, int blocks, int threads, size_t lds, hipStream_t st,
+                        bool bj) {
+  int rc = bj ? adjoint_class_op_bj1_set0(R, op, P, blocks, threads, lds, st)
+              : adjoint_class_op_bj0_set0(R, op, P, blocks, threads, lds, st);
+  if (rc == -1)
+    rc = bj ? adjoint_class_op_bj1_set1(R, op, P, blocks, threads, lds, st)
+            : adjoint_class_op_bj0_set1(R, op, P, blocks, threads, lds, st);
+  return rc;
+}
+
 static int class_op(int R, int op, const FusedParams& P, int blocks, int threads, size_t lds, hipStream_t st,
                     bool bj) {
   int rc = bj ? fused_class_op_bj1_set0(R, op, P, blocks, threads, lds, st)
@@ -58,14 +75,25 @@ static int launch_fused(int R, FusedParams P, const ofb_solver* s, int cls, hipS
                   s->cfg.preconditioner == OFB_PRECOND_PER_BODY_PAIR);
 }
 
-// lets a class kernel use more than 64 KB of LDS when one world needs it
+// lets a class kernel (forward and adjoint) use more than 64 KB of LDS when one world
+// needs it
 static int allow_lds(int R, const ofb_solver* s) {
+  const bool bj = s->cfg.preconditioner == OFB_PRECOND_PER_BODY_PAIR;
   size_t bytes = (size_t)lds_words(s, R) * sizeof(float);
-  if (bytes <= 65536) return 0;
-  FusedParams P{};
-  if (class_op(R, 1, P, 0, 0, bytes, nullptr, s->cfg.preconditioner == OFB_PRECOND_PER_BODY_PAIR) != 0) {
-    set_error("fused kernel needs %zu bytes of LDS per world", bytes);
-    return -4;
+  if (bytes > 65536) {
+    FusedParams P{};
+    if (class_op(R, 1, P, 0, 0, bytes, nullptr, bj) != 0) {
+      set_error("fused kernel needs %zu bytes of LDS per world", bytes);
+      return -4;
+    }
+  }
+  bytes = (size_t)adj_lds_words(s, R) * sizeof(float);
+  if (bytes > 65536) {
+    AdjParams A{};
+    if (adj_class_op(R, 1, A, 0, 0, bytes, nullptr, bj) != 0) {
+      set_error("adjoint kernel needs %zu bytes of LDS per world", bytes);
+      return -4;
+    }
   }
   return 0;
 }
@@ -101,6 +129,7 @@ int fused_step(ofb_solver* s, const ofb_state_in* in, const ofb_state_out* out, 
   P.topo = s->topo;
   P.lambda_seed = s->lambda_seed;
   P.lambda_out = s->lambda;
+  P.lambda_prev_out = s->lambda_prev;
   P.st_nr = s->st_nr; P.st_pcr = s->st_pcr; P.st_best = s->st_best; P.st_flags = s->st_flags; P.st_merit = s->st_merit;
   const int W = s->m.W;
   const int r_max = (s->L.Nc + WAVE - 1) / WAVE;
@@ -121,6 +150,56 @@ int fused_step(ofb_solver* s, const ofb_state_in* in, const ofb_state_out* out, 
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) { set_error("fused kernel launch failed: %s", hipGetErrorString(e)); return -4; }
+  return 0;
+}
+
+int fused_adjoint(ofb_solver* s, const ofb_state_in* in, const ofb_state_out* out, const ofb_control* ctl,
+                  const float* lambda, const float* lambda_prev, const ofb_adjoint_in* gin,
+                  const ofb_adjoint_out* gout, float dt, hipStream_t stream) {
+  AdjParams P{};
+  P.m = s->m;
+  P.L = s->L;
+  P.cfg = s->cfg;
+  P.dt = dt;
+  P.target_pos = ctl->joint_target_pos; P.target_vel = ctl->joint_target_vel;
+  P.ccount = s->ccount; P.cf = s->cf; P.ci = s->ci; P.topo = s->topo;
+  P.q_prev = in->body_q; P.qd_prev = in->body_qd; P.body_f = in->body_f;
+  P.q = out->body_q; P.qd = out->body_qd;
+  P.lam = lambda; P.lam_prev = lambda_prev;
+  P.g_q = gin->body_q_grad; P.g_qd = gin->body_qd_grad;
+  P.g_q_prev = gout->body_q_grad; P.g_qd_prev = gout->body_qd_grad; P.g_f = gout->body_f_grad;
+  P.g_tpos = gout->joint_target_pos_grad; P.g_tvel = gout->joint_target_vel_grad;
+  P.g_ke = gout->joint_target_ke_grad; P.g_kd = gout->joint_target_kd_grad;
+  P.g_mu = gout->shape_mu_grad; P.w_lam = gout->adjoint_lambda; P.w_u = gout->adjoint_body;
+  P.st_pcr = s->st_pcr; P.st_flags = s->st_flags;
+  const int W = s->m.W;
+  const int r_max = (s->L.Nc + WAVE - 1) / WAVE;
+  if (r_max > kClassRHost[kNumClasses - 1]) {
+    set_error("fused path supports at most %d constraint rows per world (got %d)",
+              WAVE * kClassRHost[kNumClasses - 1], s->L.Nc);
+    return -3;
+  }
+  const bool bj = s->cfg.preconditioner == OFB_PRECOND_PER_BODY_PAIR;
+  hipLaunchKernelGGL(reset_classes_kernel, dim3(1), dim3(64), 0, stream, s->cls_counts);
+  hipLaunchKernelGGL(classify_worlds_kernel, dim3((W + 255) / 256), dim3(256), 0, stream, W, s->ccount, s->L.K,
+                     s->L.Nj + s->L.Nctrl, s->cls_counts, s->cls_lists);
+  for (int c = 0; c < kNumClasses; ++c) {
+    if (c > 0 && kClassRHost[c - 1] >= r_max) break;
+    const int R = kClassRHost[c];
+    P.wlist = s->cls_lists + (size_t)c * W;
+    P.wcount = s->cls_counts + c;
+    P.rows_cap = WAVE * R;
+    P.lds_per_wave = adj_lds_words(s, R);
+    int wpb = s->fused_worlds_per_block;
+    while (wpb > 1 && (size_t)wpb * P.lds_per_wave * sizeof(float) > 65536) --wpb;
+    size_t lds_bytes = (size_t)wpb * P.lds_per_wave * sizeof(float);
+    if (adj_class_op(R, 0, P, (W + wpb - 1) / wpb, wpb * WAVE, lds_bytes, stream, bj) != 0) {
+      set_error("no adjoint kernel instance for %d row slots per lane", R);
+      return -4;
+    }
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) { set_error("adjoint kernel launch failed: %s", hipGetErrorString(e)); return -4; }
   return 0;
 }
 

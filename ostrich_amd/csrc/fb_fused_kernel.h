@@ -65,6 +65,7 @@ struct FusedParams {
   const int32_t* topo;  // [Nj] joint of each joint row, [Nj] row within joint, [Nctrl] joint of each control row
   const float* lambda_seed;
   float* lambda_out;
+  float* lambda_prev_out;  // lambda_prev_iter as the step leaves it (input of the adjoint)
   int32_t *st_nr, *st_pcr, *st_best, *st_flags;
   float* st_merit;
   int lds_per_wave;  // floats
@@ -83,15 +84,15 @@ struct FusedParams {
 //  * structure (once per step: row activity and bodies do not change inside a step):
 //    pair ids, ranks, sizes, the pair's slot range and packed-block offset;
 //  * factor (once per Newton iteration): lane a builds row a of the packed lower
-//    triangle of A_pair = sum_b J_a,b M_b^-1 J_c,b^T (+ C + reg on the diagonal), then
-//    the wave sweeps the block in place, pivot by pivot (X_kk <- -1/X_kk, X_ik <-
-//    X_ik/X_kk, X_ij <- X_ij - X_ik X_kj / X_kk), which leaves -A_pair^-1.  The k-th
-//    pivot equals the k-th Cholesky pivot of the reference (A_kk - sum L_kq^2), so a
-//    pivot <= 0 is exactly its factorisation failure (:305-311); such a pair uses the
-//    Jacobi diagonal (:374-380);
-//  * apply (twice per PCR iteration): lane a forms -sum_c X_ac v_c over its pair, a
-//    dense row of at most 64 LDS reads with no serial dependence, where the reference
-//    runs two triangular solves (:384-410) - the same operator A_pair^-1.
+//    triangle of A_pair = sum_b J_a,b M_b^-1 J_c,b^T (+ C + reg on the diagonal); the
+//    wave factors it in place with a right-looking Cholesky, pivot by pivot (the
+//    reference's factor, same failure test pivot <= 0, :296-321; a failed pair uses the
+//    Jacobi diagonal, :374-380), then inverts L in place, one column per lane;
+//  * apply (twice per PCR iteration): y = L^-1 v and z = L^-T y as two dense triangular
+//    matvecs of at most 64 LDS reads per lane with no serial dependence, where the
+//    reference runs two triangular solves (:384-410).  An explicit A_pair^-1 would be
+//    cheaper still, but in FP32 it loses an order of magnitude more accuracy on the
+//    near-singular blocks of coplanar contacts (friction rows with C ~ 1e-6).
 // LDS per wave after the bodies and lambda_prev (cap = 64 R rows):
 //   pid[cap] i32, lsz[cap] i32, colbuf[cap], vbuf[cap], rowd[16 cap] (J0 6, J1 6,
 //   b0 bits, b1 bits, C + reg, pad) indexed by member slot, X[R * 2080] packed blocks.
@@ -228,15 +229,18 @@ __device__ void bj_factor(const BJView& V, const float* sm, const int (&b0)[R], 
     }
   }
   wsync();
+  // right-looking Cholesky in place, pivot by pivot (the reference's column Cholesky,
+  // :296-321): column k is snapshotted, L_kk = sqrt(X_kk), L_ak = X_ak / L_kk and the
+  // trailing rows update X_ac -= L_ak L_ck.  Every member lane reads the same pivot, so
+  // the pair agrees on a failure.
   for (int piv = 0; piv < G.mmax; ++piv) {
     float colv[R];
 #pragma unroll
     for (int k = 0; k < R; ++k) {
       colv[k] = 0.f;
-      if (G.slot[k] >= 0 && piv < G.m[k] && !G.fail[k]) {
+      if (G.slot[k] >= 0 && piv < G.m[k] && !G.fail[k] && G.rank[k] >= piv) {
         const int rank = G.rank[k];
-        int e = rank >= piv ? rank * (rank + 1) / 2 + piv : piv * (piv + 1) / 2 + rank;
-        colv[k] = V.X[G.poff[k] + e];
+        colv[k] = V.X[G.poff[k] + rank * (rank + 1) / 2 + piv];
         V.colbuf[G.slot[k]] = colv[k];
       }
     }
@@ -251,24 +255,50 @@ __device__ void bj_factor(const BJView& V, const float* sm, const int (&b0)[R], 
         flags |= FLAG_FACTOR_FAILURE;
         continue;
       }
-      const float inv = 1.f / dpiv;
+      if (rank < piv) continue;
+      const float lkk = sqrtf(dpiv), inv = 1.f / lkk;
       float* Xr = V.X + G.poff[k] + rank * (rank + 1) / 2;
       if (rank == piv) {
-        for (int c = 0; c < rank; ++c) Xr[c] = V.colbuf[base + c] * inv;
-        Xr[rank] = -inv;
+        Xr[piv] = lkk;
       } else {
-        const float f = colv[k] * inv;
-        for (int c = 0; c <= rank; ++c) {
-          if (c == piv) Xr[c] = f;
-          else Xr[c] -= f * V.colbuf[base + c];
-        }
+        const float lak = colv[k] * inv;
+        Xr[piv] = lak;
+        for (int c = piv + 1; c <= rank; ++c) Xr[c] -= lak * (V.colbuf[base + c] * inv);
       }
+    }
+    wsync();
+  }
+  // X <- L^-1 in place, one column per lane (lane of rank c owns column c): at step a
+  // row a of L is copied out, then Z_ac = -(sum_{q=c}^{a-1} L_aq Z_qc) / L_aa
+  for (int a = 0; a < G.mmax; ++a) {
+#pragma unroll
+    for (int k = 0; k < R; ++k)
+      if (G.slot[k] >= 0 && a < G.m[k] && !G.fail[k] && G.rank[k] <= a)
+        V.colbuf[G.slot[k]] = V.X[G.poff[k] + a * (a + 1) / 2 + G.rank[k]];
+    wsync();
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      if (G.slot[k] < 0 || a >= G.m[k] || G.fail[k] || G.rank[k] > a) continue;
+      const int c = G.rank[k], base = G.slot[k] - c;
+      const float* Xp = V.X + G.poff[k];
+      const float laa = V.colbuf[base + a];
+      float z;
+      if (c == a) {
+        z = 1.f / laa;
+      } else {
+        float acc = 0.f;
+        for (int q = c; q < a; ++q) acc += V.colbuf[base + q] * Xp[q * (q + 1) / 2 + c];
+        z = -acc / laa;
+      }
+      V.X[G.poff[k] + a * (a + 1) / 2 + c] = z;
     }
     wsync();
   }
 }
 
-// out = P v (apply of per_body_pair_preconditioner.py:324-410)
+// out = P v = L^-T (L^-1 v) over each pair (apply of per_body_pair_preconditioner.py:
+// 324-410, whose two triangular solves become two dense triangular matvecs with the
+// stored L^-1); failed pairs and rows outside any block use the Jacobi diagonal
 template <int R>
 __device__ void bj_apply(const BJView& V, const bool (&act)[R], const float (&Pi)[R], const BJRegs<R>& G,
                          const float (&v)[R], float (&out)[R]) {
@@ -276,6 +306,18 @@ __device__ void bj_apply(const BJView& V, const bool (&act)[R], const float (&Pi
   for (int k = 0; k < R; ++k)
     if (G.slot[k] >= 0 && !G.fail[k]) V.vbuf[G.slot[k]] = v[k];
   wsync();
+  // y = L^-1 v: row a of the packed triangle
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    if (G.slot[k] < 0 || G.fail[k]) continue;
+    const int rank = G.rank[k], base = G.slot[k] - rank;
+    const float* Zr = V.X + G.poff[k] + rank * (rank + 1) / 2;
+    float y = 0.f;
+    for (int c = 0; c <= rank; ++c) y += Zr[c] * V.vbuf[base + c];
+    V.colbuf[G.slot[k]] = y;
+  }
+  wsync();
+  // z = L^-T y: column a of the packed triangle
 #pragma unroll
   for (int k = 0; k < R; ++k) {
     float o = 0.f;
@@ -283,13 +325,9 @@ __device__ void bj_apply(const BJView& V, const bool (&act)[R], const float (&Pi
       if (G.slot[k] >= 0 && !G.fail[k]) {
         const int rank = G.rank[k], base = G.slot[k] - rank;
         const float* Xp = V.X + G.poff[k];
-        const int rowb = rank * (rank + 1) / 2;
-        float s = 0.f;
-        for (int c = 0; c < G.m[k]; ++c) {
-          int e = c <= rank ? rowb + c : c * (c + 1) / 2 + rank;
-          s += Xp[e] * V.vbuf[base + c];
-        }
-        o = -s;
+        float z = 0.f;
+        for (int c = rank; c < G.m[k]; ++c) z += Xp[c * (c + 1) / 2 + rank] * V.colbuf[base + c];
+        o = z;
       } else {
         o = Pi[k] * v[k];
       }
@@ -331,6 +369,56 @@ __device__ __forceinline__ void schur_matvec(float* sm, int lane, int B, const b
     out[k] = o;
   }
   wsync();
+}
+
+// PCR on A x = rhs from x0 = 0 (optim/pcr_solver.py:170-295): the seven vectors live in
+// VGPRs, dots are DPP wave reductions, A is the LDS matrix-free operator.  Returns the
+// iteration count.
+template <int R, bool BJ>
+__device__ __forceinline__ int pcr_solve(float* sm, int lane, int B, const BJView& V, const BJRegs<R>& G,
+                                         const bool (&act)[R], const int (&b0)[R], const int (&b1)[R],
+                                         const sv (&J0)[R], const sv (&J1)[R], const float (&Cr)[R],
+                                         const float (&Pi)[R], const float (&rhs)[R], float bsq, float tol_sq,
+                                         float latol_sq, int max_iters, float (&x)[R]) {
+  float r[R], z[R], p[R], Ap[R], y[R], Az[R];
+#pragma unroll
+  for (int k = 0; k < R; ++k) { x[k] = 0.f; r[k] = rhs[k]; z[k] = Pi[k] * r[k]; }
+  if (BJ) bj_apply<R>(V, act, Pi, G, r, z);
+  schur_matvec<R>(sm, lane, B, act, b0, b1, J0, J1, Cr, z, Az);
+  float zAz = 0.f, rsq = 0.f;
+#pragma unroll
+  for (int k = 0; k < R; ++k) { p[k] = z[k]; Ap[k] = Az[k]; zAz += z[k] * Az[k]; rsq += r[k] * r[k]; }
+  zAz = wave_sum(zAz);
+  rsq = wave_sum(rsq);
+  float target = latol_sq;
+  if (bsq * tol_sq > target) target = bsq * tol_sq;
+  int it = 0;
+  while (true) {
+    float zAz_old = zAz;
+    float yAp = 0.f;
+#pragma unroll
+    for (int k = 0; k < R; ++k) y[k] = Pi[k] * Ap[k];
+    if (BJ) bj_apply<R>(V, act, Pi, G, Ap, y);
+#pragma unroll
+    for (int k = 0; k < R; ++k) yAp += y[k] * Ap[k];
+    yAp = wave_sum(yAp);
+    float a = yAp > 0.f ? zAz_old / yAp : 0.f;
+#pragma unroll
+    for (int k = 0; k < R; ++k) { x[k] += a * p[k]; r[k] -= a * Ap[k]; z[k] -= a * y[k]; }
+    schur_matvec<R>(sm, lane, B, act, b0, b1, J0, J1, Cr, z, Az);
+    zAz = 0.f;
+#pragma unroll
+    for (int k = 0; k < R; ++k) zAz += z[k] * Az[k];
+    zAz = wave_sum(zAz);
+    float bt = zAz_old > 0.f ? zAz / zAz_old : 0.f;
+    rsq = 0.f;
+#pragma unroll
+    for (int k = 0; k < R; ++k) { p[k] = z[k] + bt * p[k]; Ap[k] = Az[k] + bt * Ap[k]; rsq += r[k] * r[k]; }
+    rsq = wave_sum(rsq);
+    ++it;
+    if (!(rsq > target) || it >= max_iters) break;
+  }
+  return it;
 }
 
 // ------------------------------------------------------------------------------------
@@ -384,7 +472,7 @@ fused_step_kernel(FusedParams P) {
   const int n_r = L.Nj + L.Nctrl + 3 * n_c;
 
   // ---- per-row registers ----
-  float lam[R], lamb[R], x[R], r[R], z[R], p[R], Ap[R], y[R], Az[R], Cr[R], Pi[R], rhs[R];
+  float lam[R], lamb[R], x[R], Cr[R], Pi[R], rhs[R];
   sv J0[R], J1[R];
   int b0[R], b1[R];
   bool act[R];
@@ -471,45 +559,8 @@ fused_step_kernel(FusedParams P) {
     }
 
     OFB_PHASE(2)
-    // ---- PCR (optim/pcr_solver.py:170-295), x0 = 0 => r = b ----
-#pragma unroll
-    for (int k = 0; k < R; ++k) { x[k] = 0.f; r[k] = rhs[k]; z[k] = Pi[k] * r[k]; }
-    if (BJ) bj_apply<R>(V, act, Pi, G, r, z);
-    schur_matvec<R>(sm, lane, B, act, b0, b1, J0, J1, Cr, z, Az);
-    float zAz = 0.f, rsq = 0.f;
-#pragma unroll
-    for (int k = 0; k < R; ++k) { p[k] = z[k]; Ap[k] = Az[k]; zAz += z[k] * Az[k]; rsq += r[k] * r[k]; }
-    zAz = wave_sum(zAz);
-    rsq = wave_sum(rsq);
-    float target = latol_sq;
-    if (bsq * tol_sq > target) target = bsq * tol_sq;
-    int it = 0;
-    while (true) {
-      float zAz_old = zAz;
-      float yAp = 0.f;
-#pragma unroll
-      for (int k = 0; k < R; ++k) y[k] = Pi[k] * Ap[k];
-      if (BJ) bj_apply<R>(V, act, Pi, G, Ap, y);
-#pragma unroll
-      for (int k = 0; k < R; ++k) yAp += y[k] * Ap[k];
-      yAp = wave_sum(yAp);
-      float a = yAp > 0.f ? zAz_old / yAp : 0.f;
-#pragma unroll
-      for (int k = 0; k < R; ++k) { x[k] += a * p[k]; r[k] -= a * Ap[k]; z[k] -= a * y[k]; }
-      schur_matvec<R>(sm, lane, B, act, b0, b1, J0, J1, Cr, z, Az);
-      zAz = 0.f;
-#pragma unroll
-      for (int k = 0; k < R; ++k) zAz += z[k] * Az[k];
-      zAz = wave_sum(zAz);
-      float bt = zAz_old > 0.f ? zAz / zAz_old : 0.f;
-      rsq = 0.f;
-#pragma unroll
-      for (int k = 0; k < R; ++k) { p[k] = z[k] + bt * p[k]; Ap[k] = Az[k] + bt * Ap[k]; rsq += r[k] * r[k]; }
-      rsq = wave_sum(rsq);
-      ++it;
-      if (!(rsq > target) || it >= P.cfg.lin_max_iters) break;
-    }
-    pcr_total += it;
+    pcr_total += pcr_solve<R, BJ>(sm, lane, B, V, G, act, b0, b1, J0, J1, Cr, Pi, rhs, bsq, tol_sq, latol_sq,
+                                  P.cfg.lin_max_iters, x);
     OFB_PHASE(3)
 
     // ---- velocity recovery + Newton update (linear_system.py:344-377, newton_step.py:22-54) ----
@@ -606,16 +657,26 @@ fused_step_kernel(FusedParams P) {
     if (c < 7) P.out_q[k * 7 + c] = bb[BQ + c];
     else P.out_qd[k * 6 + c - 7] = bb[BU + c - 7];
   }
+  // lambda and lambda_prev_iter (the snapshot of the last Newton iteration, which
+  // backtracking does not restore, base_engine.py:441) in the reference row layout
 #pragma unroll
   for (int k = 0; k < R; ++k) {
     int i = lane + WAVE * k;
-    if (i < n_r) P.lambda_out[(size_t)w * L.Nc + original_row(L, i, n_c)] = lam[k];
+    if (i < n_r) {
+      size_t o = (size_t)w * L.Nc + original_row(L, i, n_c);
+      P.lambda_out[o] = lam[k];
+      P.lambda_prev_out[o] = lamprev[i];
+    }
   }
   // contact slots beyond the world's contact count hold lambda = 0
   for (int c = n_c + lane; c < L.K; c += WAVE) {
-    P.lambda_out[(size_t)w * L.Nc + L.off_n + c] = 0.f;
-    P.lambda_out[(size_t)w * L.Nc + L.off_f + 2 * c] = 0.f;
-    P.lambda_out[(size_t)w * L.Nc + L.off_f + 2 * c + 1] = 0.f;
+    size_t o = (size_t)w * L.Nc;
+    P.lambda_out[o + L.off_n + c] = 0.f;
+    P.lambda_out[o + L.off_f + 2 * c] = 0.f;
+    P.lambda_out[o + L.off_f + 2 * c + 1] = 0.f;
+    P.lambda_prev_out[o + L.off_n + c] = 0.f;
+    P.lambda_prev_out[o + L.off_f + 2 * c] = 0.f;
+    P.lambda_prev_out[o + L.off_f + 2 * c + 1] = 0.f;
   }
   int bj_any = 0;
   if (__any(bj_flags & FLAG_FACTOR_FAILURE)) bj_any |= FLAG_FACTOR_FAILURE;

@@ -353,9 +353,13 @@ __device__ RowOut normal_row(const PT& P, float* sm, int w, int c, float lam) {
 }
 
 // friction row t in {0,1} of contact c (constraints/friction_constraint.py:79-908)
+// dwdmu (adjoint only, may be null): d(weight)/d(mu_x) of the isotropic cone, written as
+// NaN where the weight's dependence on mu has no closed form here (anisotropic axes,
+// Stribeck scaling, the elliptic cone) and 0 where the row does not depend on mu.
 template <class PT>
 __device__ RowOut friction_row(const PT& P, float* sm, const float* lamprev, int w, int c, int t,
-                               int n_c, float lam) {
+                               int n_c, float lam, float* dwdmu = nullptr) {
+  if (dwdmu) *dwdmu = 0.f;
   RowOut o;
   o.J0 = svzero(); o.J1 = svzero(); o.hc = 0.f; o.C = 0.f; o.active = false; o.b0 = -1; o.b1 = -1;
   ContactGeo g = load_contact(P, w, c);
@@ -454,7 +458,15 @@ __device__ RowOut friction_row(const PT& P, float* sm, const float* lamprev, int
       float gap = lim - fminf(raw, lim);
       float sa = dn, sb = rr * gap;
       float phi = sa + sb - sqrtf(sa * sa + sb * sb);
-      float wv = rr * ((dn - phi) / (rr * raw + phi + denom_eps));
+      float den = rr * raw + phi + denom_eps;
+      float wv = rr * ((dn - phi) / den);
+      if (dwdmu) {  // lim = mu fn_prev dt enters through gap
+        float nrm = sqrtf(sa * sa + sb * sb);
+        float dphi = rr * (1.f - (nrm > 0.f ? sb / nrm : 0.f)) * (raw < lim ? 1.f : 0.f) * fn_prev * dt;
+        bool closed = !an0 && !an1 && !(mss > 0.f && vst > 0.f);
+        *dwdmu = !closed ? __int_as_float(0x7fc00000)
+                         : ((wv < 0.f || wv > 1e5f) ? 0.f : -rr * dphi * (den + (dn - phi)) / (den * den));
+      }
       wv = fminf(fmaxf(wv, 0.f), 1e5f);
       wx = wv; wy = wv;
     } else {
@@ -472,6 +484,7 @@ __device__ RowOut friction_row(const PT& P, float* sm, const float* lamprev, int
       wt = fminf(fmaxf(wt, 0.f), 1e5f);
       wx = wt * imx * imx;
       wy = wt * imy * imy;
+      if (dwdmu) *dwdmu = __int_as_float(0x7fc00000);
     }
   }
   o.active = true;

@@ -25,6 +25,7 @@ enum {
   FLAG_CONTACT_OVERFLOW = 2,
   FLAG_FACTOR_FAILURE = 4,
   FLAG_PAIR_OVERFLOW = 8,
+  FLAG_MU_GRAD_MISSING = 16,  // adjoint: a friction row's mu derivative has no closed form
 };
 
 constexpr int kMaxJoints = 64;      // joints per world supported by the fused path
@@ -74,6 +75,7 @@ struct ofb_solver {
   // outputs
   int32_t* topo;     // joint of each joint row [Nj], row within joint [Nj], joint of each control row [Nctrl]
   float* lambda;     // [W*Nc]
+  float* lambda_prev;// [W*Nc] lambda_prev_iter as the last step left it (fused path)
   float* lambda_seed;// [W*Nc]
   int32_t *st_nr, *st_pcr, *st_best, *st_flags;
   float* st_merit;
@@ -92,6 +94,9 @@ namespace ofb {
 int fused_step(ofb_solver* s, const ofb_state_in* in, const ofb_state_out* out, const ofb_control* ctl, float dt,
                hipStream_t stream);
 int fused_classes_alloc(ofb_solver* s);
+int fused_adjoint(ofb_solver* s, const ofb_state_in* in, const ofb_state_out* out, const ofb_control* ctl,
+                  const float* lambda, const float* lambda_prev, const ofb_adjoint_in* gin,
+                  const ofb_adjoint_out* gout, float dt, hipStream_t stream);
 int plain_create(ofb_solver* s);
 void plain_destroy(ofb_solver* s);
 int plain_step(ofb_solver* s, const ofb_state_in* in, const ofb_state_out* out, const ofb_control* ctl, float dt,

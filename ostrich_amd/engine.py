@@ -87,16 +87,7 @@ class OstrichEngine:
         self._out.body_q, self._out.body_qd = state_out.body_q.data_ptr(), state_out.body_qd.data_ptr()
         self._ctl.joint_target_pos = control.joint_target_pos.data_ptr()
         self._ctl.joint_target_vel = control.joint_target_vel.data_ptr()
-        c = self._con
-        c.count = contacts.rigid_contact_count.data_ptr()
-        c.max_contacts = contacts.rigid_contact_max
-        c.point0 = contacts.rigid_contact_point0.data_ptr()
-        c.point1 = contacts.rigid_contact_point1.data_ptr()
-        c.normal = contacts.rigid_contact_normal.data_ptr()
-        c.shape0 = contacts.rigid_contact_shape0.data_ptr()
-        c.shape1 = contacts.rigid_contact_shape1.data_ptr()
-        c.margin0 = contacts.rigid_contact_margin0.data_ptr()
-        c.margin1 = contacts.rigid_contact_margin1.data_ptr()
+        c = self._fill_contacts(contacts)
         _lib.check(self.lib.ofb_step(self._h, C.byref(self._in), C.byref(self._out), C.byref(self._ctl),
                                      C.byref(c), C.c_float(dt), self._stream()), "ofb_step")
         self.timestep += 1
@@ -150,5 +141,89 @@ class OstrichEngine:
         _lib.check(self.lib.ofb_last_solve_ms(self._h, C.byref(ms)), "ofb_last_solve_ms")
         return ms.value
 
-    def step_backward(self):
-        raise NotImplementedError("the adjoint (step_backward) is not built on this solver yet")
+    def constraint_impulses_prev(self):
+        """lambda_prev_iter [W, N_c] as the last step left it (the snapshot of its last
+        Newton iteration, base_engine.py:441); with `constraint_impulses()` it is what the
+        reference's TrajectoryBuffer keeps per step for the adjoint."""
+        lam = torch.zeros(self.num_worlds, self.num_constraints, dtype=torch.float32, device=self.device)
+        _lib.check(self.lib.ofb_get_lambda_prev(self._h, _ptr(lam), self._stream()), "ofb_get_lambda_prev")
+        return lam
+
+    def set_shape_mu(self, shape_mu):
+        """Replaces the per-shape friction coefficients [W*(S+G)] copied at construction."""
+        mu = shape_mu.to(self.device, torch.float32).contiguous().reshape(-1)
+        if mu.numel() != self.num_worlds * (self.model.shapes_per_world + self.model.num_global_shapes):
+            raise ValueError("shape_mu must hold W*(S+G) values")
+        _lib.check(self.lib.ofb_set_shape_mu(self._h, _ptr(mu), self._stream()), "ofb_set_shape_mu")
+        self._mu_keepalive = mu
+
+    def _fill_contacts(self, contacts):
+        c = self._con
+        c.count = contacts.rigid_contact_count.data_ptr()
+        c.max_contacts = contacts.rigid_contact_max
+        c.point0 = contacts.rigid_contact_point0.data_ptr()
+        c.point1 = contacts.rigid_contact_point1.data_ptr()
+        c.normal = contacts.rigid_contact_normal.data_ptr()
+        c.shape0 = contacts.rigid_contact_shape0.data_ptr()
+        c.shape1 = contacts.rigid_contact_shape1.data_ptr()
+        c.margin0 = contacts.rigid_contact_margin0.data_ptr()
+        c.margin1 = contacts.rigid_contact_margin1.data_ptr()
+        return c
+
+    def step_backward(self, state_in: State, state_out: State, control: Control, contacts: Contacts, dt: float,
+                      lam, lam_prev, grad_body_q, grad_body_qd, adjoint_multipliers=False):
+        """Adjoint of one step (core/base_engine.py:528-743).
+
+        The reference reads the step's data from engine buffers that its TrajectoryBuffer
+        restores (simulation/trajectory_buffer.py:229-264); here the same data are
+        arguments: the step's input state (q-, u-, f), its output state (q+, u+), control,
+        contacts, lambda and lambda_prev_iter (`constraint_impulses()` /
+        `constraint_impulses_prev()` after that step) and dL/dq+, dL/du+.  Returns a dict of
+        dL/d(input): body_q, body_qd, body_f, joint_target_pos, joint_target_vel,
+        joint_target_ke, joint_target_kd, shape_mu (per world copy), plus w_lambda and w_u
+        when `adjoint_multipliers`.  Per-world adjoint PCR counts and flags are then in
+        `stats()`."""
+        if self.config.execution.path != "fused":
+            raise NotImplementedError("step_backward runs on the fused path")
+        d, W = self.device, self.num_worlds
+        B, D = self.model.bodies_per_world, self.model.dofs_per_world
+        SC = self.model.shapes_per_world + self.model.num_global_shapes
+
+        def f32(t, n):
+            t = t.to(d, torch.float32).contiguous().reshape(-1)
+            if t.numel() != n:
+                raise ValueError(f"expected {n} values, got {t.numel()}")
+            return t
+
+        gq, gu = f32(grad_body_q, W * B * 7), f32(grad_body_qd, W * B * 6)
+        la, lp = f32(lam, W * self.num_constraints), f32(lam_prev, W * self.num_constraints)
+        out = dict(body_q=torch.empty(W * B, 7, device=d), body_qd=torch.empty(W * B, 6, device=d),
+                   body_f=torch.empty(W * B, 6, device=d),
+                   joint_target_pos=torch.empty(W, D, device=d), joint_target_vel=torch.empty(W, D, device=d),
+                   joint_target_ke=torch.empty(W, D, device=d), joint_target_kd=torch.empty(W, D, device=d),
+                   shape_mu=torch.empty(W, SC, device=d))
+        if adjoint_multipliers:
+            out["w_lambda"] = torch.empty(W, self.num_constraints, device=d)
+            out["w_u"] = torch.empty(W * B, 6, device=d)
+        si, so = _abi.OfbStateIn(), _abi.OfbStateOut()
+        si.body_q, si.body_qd, si.body_f = (state_in.body_q.data_ptr(), state_in.body_qd.data_ptr(),
+                                            state_in.body_f.data_ptr())
+        so.body_q, so.body_qd = state_out.body_q.data_ptr(), state_out.body_qd.data_ptr()
+        ctl = _abi.OfbControl()
+        ctl.joint_target_pos = control.joint_target_pos.data_ptr()
+        ctl.joint_target_vel = control.joint_target_vel.data_ptr()
+        gin, gout = _abi.OfbAdjointIn(), _abi.OfbAdjointOut()
+        gin.body_q_grad, gin.body_qd_grad = gq.data_ptr(), gu.data_ptr()
+        for k, field in (("body_q", "body_q_grad"), ("body_qd", "body_qd_grad"), ("body_f", "body_f_grad"),
+                         ("joint_target_pos", "joint_target_pos_grad"),
+                         ("joint_target_vel", "joint_target_vel_grad"),
+                         ("joint_target_ke", "joint_target_ke_grad"), ("joint_target_kd", "joint_target_kd_grad"),
+                         ("shape_mu", "shape_mu_grad"), ("w_lambda", "adjoint_lambda"), ("w_u", "adjoint_body")):
+            if k in out and out[k].numel():
+                setattr(gout, field, out[k].data_ptr())
+        con = self._fill_contacts(contacts)
+        _lib.check(self.lib.ofb_step_backward(self._h, C.byref(si), C.byref(so), C.byref(ctl), C.byref(con),
+                                              _ptr(la), _ptr(lp), C.byref(gin), C.byref(gout), C.c_float(dt),
+                                              self._stream()), "ofb_step_backward")
+        self._bwd_keepalive = (gq, gu, la, lp)
+        return out

@@ -37,7 +37,9 @@ def _struct_fields(name):
 
 @pytest.mark.parametrize("cname,pyc", [("ofb_model", _abi.OfbModel), ("ofb_config", _abi.OfbConfig),
                                        ("ofb_contacts", _abi.OfbContacts), ("ofb_state_in", _abi.OfbStateIn),
-                                       ("ofb_state_out", _abi.OfbStateOut), ("ofb_control", _abi.OfbControl)])
+                                       ("ofb_state_out", _abi.OfbStateOut), ("ofb_control", _abi.OfbControl),
+                                       ("ofb_adjoint_in", _abi.OfbAdjointIn),
+                                       ("ofb_adjoint_out", _abi.OfbAdjointOut)])
 def test_ctypes_structs_match_header(cname, pyc):
     assert _struct_fields(cname) == [f[0] for f in pyc._fields_]
 

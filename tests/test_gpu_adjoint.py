@@ -1,0 +1,131 @@
+"""Parity of the gfx950 step_backward (through the C ABI) with the oracle's adjoint.
+
+The GPU runs a forward step, then its adjoint with a random loss seed (dL/dq+, dL/du+).
+The oracle's adjoint is evaluated at the GPU's own step output (q+, u+, lambda,
+lambda_prev_iter) and the same contacts, so the comparison isolates the backward pass.
+Tolerance: every gradient within 2e-3 of its largest magnitude (per quantity) of the
+FP32 oracle (same pose-difference step), plus four times the FP32-vs-FP64 oracle spread
+of that quantity.  The spread is ~1e-5 on well-conditioned worlds (skid-steer); on box
+stacks the coplanar contacts make the adjoint system near-singular and a 16-iteration
+PCR leaves an FP32 truncation of ~1e-2 in every implementation, the oracle's included.
+"""
+import numpy as np
+import pytest
+import torch
+
+from ostrich_amd import collide, scenes
+from ostrich_amd.containers import Contacts, Control, State
+from tests.helpers import config, no_contacts, single_box, zero_control
+
+pytestmark = pytest.mark.gpu
+
+DT = 0.05
+KEYS = ("body_q", "body_qd", "body_f", "joint_target_pos", "joint_target_vel", "joint_target_ke",
+        "joint_target_kd", "shape_mu", "w_lambda", "w_u")
+
+
+def _engine(model, cfg, device):
+    from ostrich_amd import OstrichEngine
+    return OstrichEngine(model, config=cfg, device=device)
+
+
+def _advance(oracle_mod, model, cfg, state, control, col, steps):
+    st = {k: np.asarray(v, np.float64) for k, v in state.items()}
+    for _ in range(steps):
+        c = col.collide(torch.as_tensor(st["body_q"], dtype=torch.float32))
+        o = oracle_mod.step(model, cfg, st, control, c, DT, precision=32)
+        st = dict(st, body_q=o["body_q"], body_qd=o["body_qd"])
+    return {k: np.asarray(v, np.float32) for k, v in st.items()}
+
+
+def _compare(oracle_mod, model, cfg, state, control, col, device, seed=0, tol=2e-3):
+    eng = _engine(model, cfg, device)
+    c = col.collide(torch.as_tensor(state["body_q"], dtype=torch.float32))
+    c = {k: (v.numpy() if hasattr(v, "numpy") else v) for k, v in c.items()}
+    s_in = State.from_numpy(state, device)
+    s_out = s_in.clone()
+    ctl = Control.from_numpy(control, device)
+    con = Contacts.from_dict(c, device)
+    eng.step(s_in, s_out, ctl, con, DT)
+    lam, lam_prev = eng.constraint_impulses(), eng.constraint_impulses_prev()
+    rng = np.random.default_rng(seed)
+    WB = model.num_worlds * model.bodies_per_world
+    a_q = rng.normal(size=(WB, 7)).astype(np.float32)
+    a_u = rng.normal(size=(WB, 6)).astype(np.float32)
+    print(f"[adjoint parity] W={model.num_worlds} B={model.bodies_per_world} precond="
+          f"{cfg.linear.preconditioner_type}", flush=True)
+    g = eng.step_backward(s_in, s_out, ctl, con, DT, lam, lam_prev, torch.from_numpy(a_q), torch.from_numpy(a_u),
+                          adjoint_multipliers=True)
+    g = {k: v.cpu().numpy().astype(np.float64) for k, v in g.items()}
+    flags = eng.stats()["flags"].cpu().numpy()
+    out = dict(body_q=s_out.body_q.cpu().numpy(), body_qd=s_out.body_qd.cpu().numpy())
+    lam_n, lp_n = lam.cpu().numpy(), lam_prev.cpu().numpy()
+    o32 = oracle_mod.adjoint(model, cfg, state, out, control, c, DT, lam_n, lp_n, a_q, a_u, precision=32)
+    o64 = oracle_mod.adjoint(model, cfg, state, out, control, c, DT, lam_n, lp_n, a_q, a_u, precision=64,
+                             fd_eps=1e-6)
+    np.testing.assert_array_equal(flags & 16, o32["flags"] & 16)
+    for k in KEYS:
+        if o64[k].size == 0 or g[k].size == 0 or k == "w_lambda":
+            continue  # w_lambda is not unique where contact rows are redundant
+        gk = g[k].reshape(o64[k].shape)
+        scale = max(np.abs(o64[k]).max(), 1e-6)
+        err = np.abs(gk - o32[k]).max()
+        spread = np.abs(o32[k] - o64[k]).max()
+        assert err <= 4.0 * spread + tol * scale, (f"{k}: max |gpu - oracle32| {err:.3e}, oracle32-vs-64 "
+                                                   f"{spread:.3e}, scale {scale:.3e}")
+    return g, o64
+
+
+def test_adjoint_box_stack(oracle_mod, gpu_device):
+    model, state, control = scenes.box_stack(16, seed=0)
+    cfg = config(K=16)
+    col = collide.PrimitiveCollider(model, gap=0.02, max_contacts_per_world=16)
+    st = _advance(oracle_mod, model, cfg, state, control, col, 3)
+    _compare(oracle_mod, model, cfg, st, control, col, gpu_device)
+
+
+def test_adjoint_box_stack_block_jacobi(oracle_mod, gpu_device):
+    model, state, control = scenes.box_stack(16, seed=0)
+    cfg = config(K=16, precond="per_body_pair")
+    col = collide.PrimitiveCollider(model, gap=0.02, max_contacts_per_world=16)
+    st = _advance(oracle_mod, model, cfg, state, control, col, 3)
+    _compare(oracle_mod, model, cfg, st, control, col, gpu_device, seed=1)
+
+
+def test_adjoint_skid_steer(oracle_mod, gpu_device):
+    model, state, control = scenes.skid_steer(32, seed=3)
+    cfg = config(K=32)
+    col = collide.PrimitiveCollider(model, gap=0.02, max_contacts_per_world=32)
+    st = _advance(oracle_mod, model, cfg, state, control, col, 4)
+    g, o64 = _compare(oracle_mod, model, cfg, st, control, col, gpu_device, seed=2)
+    assert np.abs(o64["joint_target_vel"]).max() > 0
+
+
+def test_adjoint_free_body_closed_form(gpu_device):
+    """No contacts: w_u = M^-1 f_drive, dL/du- = -M w_u = g_u + dt g_p (translation)."""
+    model, state = single_box(mass=2.0, pose=(0.3, -0.2, 3.0, 0, 0, 0, 1))
+    state = {k: np.asarray(v, np.float32) for k, v in state.items()}
+    cfg = config(nr_iters=4, K=4)
+    eng = _engine(model, cfg, gpu_device)
+    s_in = State.from_numpy(state, gpu_device)
+    s_out = s_in.clone()
+    ctl = Control.from_numpy(zero_control(model), gpu_device)
+    con = Contacts.from_dict(no_contacts(model), gpu_device)
+    eng.step(s_in, s_out, ctl, con, DT)
+    gq = torch.tensor([[1.0, -2.0, 0.5, 0, 0, 0, 0]])
+    gu = torch.tensor([[0.3, 0.1, -0.4, 0, 0, 0]])
+    g = eng.step_backward(s_in, s_out, ctl, con, DT, eng.constraint_impulses(), eng.constraint_impulses_prev(),
+                          gq, gu)
+    expect = gu[0, :3] + DT * gq[0, :3]
+    np.testing.assert_allclose(g["body_qd"][0, :3].cpu().numpy(), expect.numpy(), rtol=1e-6)
+    np.testing.assert_allclose(g["body_f"][0, :3].cpu().numpy(), (DT * expect / 2.0).numpy(), rtol=1e-6)
+    np.testing.assert_allclose(g["body_q"][0, :3].cpu().numpy(), gq[0, :3].numpy(), rtol=1e-6)
+
+
+def test_backward_rejected_on_plain_path(gpu_device):
+    model, state = single_box()
+    eng = _engine(model, config(K=4, path="plain"), gpu_device)
+    s = State.from_numpy({k: np.asarray(v, np.float32) for k, v in state.items()}, gpu_device)
+    with pytest.raises(NotImplementedError):
+        eng.step_backward(s, s, Control.from_numpy(zero_control(model), gpu_device),
+                          Contacts.from_dict(no_contacts(model), gpu_device), DT, None, None, None, None)

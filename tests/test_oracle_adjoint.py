@@ -1,0 +1,230 @@
+"""Pins the oracle's adjoint (oracle/fb_oracle.cpp run_adjoint), the checker of the GPU
+step_backward, before it is trusted.
+
+* Where the reference's adjoint is exact - a free body in translation, a frictionless box
+  on the ground with zero compliance - its gradients equal central finite differences of
+  the FP64 forward step.
+* Every term of the vector-Jacobian product w^T dR/dxi (closed-form and pose terms)
+  equals a central finite difference of the FP64 weighted residual
+  s = w_lambda . h_c + w_u . h_d, evaluated by the oracle's residual restatement with the
+  adjoint's own (w_lambda, w_u).
+* The explicit pose carry equals the transposed Jacobian of the unnormalised quaternion
+  update q + dt/2 Q(q) w, formed here by finite differences.
+* A zero seed gives zero gradients; the FP32 adjoint (pose differences with step 2e-3)
+  agrees with the FP64 one.
+"""
+import copy
+
+import numpy as np
+import pytest
+import torch
+
+from ostrich_amd import collide, scenes
+from ostrich_amd.config import ComplianceConfig
+from tests.helpers import config, no_contacts, single_box, zero_control
+
+DT = 0.05
+
+
+def _loss_seed(model, seed=0):
+    rng = np.random.default_rng(seed)
+    WB = model.num_worlds * model.bodies_per_world
+    return rng.normal(size=(WB, 7)), rng.normal(size=(WB, 6))
+
+
+def _loss(a_q, a_u, out):
+    return float((a_q * out["body_q"]).sum() + (a_u * out["body_qd"]).sum())
+
+
+def _fd_step(oracle_mod, model, cfg, state, control, contacts, a_q, a_u, key, idx, h=1e-6):
+    st = {k: np.array(v, np.float64, copy=True) for k, v in state.items()}
+    flat = st[key].reshape(-1)
+    base = flat[idx]
+    flat[idx] = base + h
+    lp = _loss(a_q, a_u, oracle_mod.step(model, cfg, st, control, contacts, DT, precision=64))
+    flat[idx] = base - h
+    lm = _loss(a_q, a_u, oracle_mod.step(model, cfg, st, control, contacts, DT, precision=64))
+    return (lp - lm) / (2 * h)
+
+
+def _forward_and_adjoint(oracle_mod, model, cfg, state, control, contacts, seed=0, precision=64, eps=1e-6):
+    o = oracle_mod.step(model, cfg, state, control, contacts, DT, precision=64)
+    a_q, a_u = _loss_seed(model, seed)
+    out = dict(body_q=o["body_q"], body_qd=o["body_qd"])
+    g = oracle_mod.adjoint(model, cfg, state, out, control, contacts, DT, o["lam"], o["lam_prev"], a_q, a_u,
+                           precision=precision, fd_eps=eps)
+    return o, out, a_q, a_u, g
+
+
+def test_free_body_translation_matches_fd(oracle_mod):
+    """No constraints, no spin: u+ = u- + dt (g + f/m), q+ = q- + dt u+, so the adjoint is
+    the exact derivative."""
+    model, state = single_box(mass=2.0, pose=(0.3, -0.2, 3.0, 0, 0, 0, 1))
+    state["body_qd"][0] = [0.5, -0.2, 0.1, 0, 0, 0]
+    cfg = config(nr_iters=6, atol=1e-12, K=4)
+    o, _, a_q, a_u, g = _forward_and_adjoint(oracle_mod, model, cfg, state, zero_control(model), no_contacts(model))
+    for key, n in (("body_qd", 6), ("body_f", 6), ("body_q", 3)):
+        fd = [_fd_step(oracle_mod, model, cfg, state, zero_control(model), no_contacts(model), a_q, a_u, key, i)
+              for i in range(n)]
+        np.testing.assert_allclose(g[key].reshape(-1)[:n], fd, rtol=1e-6, atol=1e-8)
+
+
+def test_frictionless_contact_without_compliance_matches_fd(oracle_mod):
+    """Box pressed onto the ground, mu = 0, contact compliance 0: the Newton matrix is the
+    residual's Jacobian at the converged point, so dL/du- and dL/df equal the derivative of
+    the converged step (with compliance > 0 the reference's adjoint softens the contact by
+    C / (J M^-1 J^T + C), a deliberate property of its method)."""
+    model, state = single_box(mass=1.0, mu=0.0, ground_mu=0.0, pose=(0, 0, 0.25, 0, 0, 0, 1))
+    state["body_qd"][0] = [0.2, 0.1, -0.05, 0, 0, 0]
+    col = collide.PrimitiveCollider(model, gap=0.02, max_contacts_per_world=8)
+    c = col.collide(torch.as_tensor(state["body_q"], dtype=torch.float32))
+    cfg = config(nr_iters=40, atol=1e-11, K=8, pcr_iters=80, tol=1e-13, patol=1e-15,
+                 compliance=ComplianceConfig(joint=0.0, contact=0.0, friction=0.0))
+    o, _, a_q, a_u, g = _forward_and_adjoint(oracle_mod, model, cfg, state, zero_control(model), c)
+    assert o["merit"][0] < 1e-18
+    for key in ("body_qd", "body_f"):
+        fd = np.array([_fd_step(oracle_mod, model, cfg, state, zero_control(model), c, a_q, a_u, key, i)
+                       for i in range(6)])
+        scale = np.abs(fd).max()
+        np.testing.assert_allclose(g[key].reshape(-1)[:6], fd, atol=2e-3 * scale)
+
+
+def _skid_state(oracle_mod, worlds=2, steps=4, seed=3, K=16):
+    model, state, control = scenes.skid_steer(worlds, seed=seed)
+    col = collide.PrimitiveCollider(model, gap=0.02, max_contacts_per_world=K)
+    st = {k: np.asarray(v, np.float64) for k, v in state.items()}
+    cfg = config(K=K)
+    for _ in range(steps):
+        c = col.collide(torch.as_tensor(st["body_q"], dtype=torch.float32))
+        o = oracle_mod.step(model, cfg, st, control, c, DT, precision=64)
+        st = dict(st, body_q=o["body_q"], body_qd=o["body_qd"])
+    c = col.collide(torch.as_tensor(st["body_q"], dtype=torch.float32))
+    return model, st, control, c, cfg
+
+
+def _ws(oracle_mod, model, cfg, st_in, st_out, control, c, o, g):
+    return oracle_mod.weighted_residual(model, cfg, st_in, st_out, control, c, DT, o["lam"], o["lam_prev"],
+                                        g["w_lambda"], g["w_u"])
+
+
+def _fd_ws(oracle_mod, model, cfg, st_in, st_out, control, c, o, g, where, key, idx, h=1e-6):
+    """Central difference of the weighted residual in one input."""
+    vals = []
+    for sgn in (1, -1):
+        si = {k: np.array(v, np.float64, copy=True) for k, v in st_in.items()}
+        so = {k: np.array(v, np.float64, copy=True) for k, v in st_out.items()}
+        ct = {k: np.array(v, np.float64, copy=True) for k, v in control.items()}
+        m = model
+        if where == "in":
+            si[key].reshape(-1)[idx] += sgn * h
+        elif where == "out":
+            so[key].reshape(-1)[idx] += sgn * h
+        elif where == "control":
+            ct[key].reshape(-1)[idx] += sgn * h
+        else:  # model array
+            m = copy.deepcopy(model)
+            arr = np.array(m.arrays[key], np.float64)
+            arr.reshape(-1)[idx] += sgn * h
+            m.arrays[key] = arr.astype(model.arrays[key].dtype)
+        vals.append(oracle_mod.weighted_residual(m, cfg, si, so, ct, c, DT, o["lam"], o["lam_prev"],
+                                                 g["w_lambda"], g["w_u"]))
+    return (vals[0] - vals[1]) / (2 * h)
+
+
+def test_vjp_terms_equal_weighted_residual_differences(oracle_mod):
+    """Skid-steer worlds (revolute joints, velocity motors, contacts, friction): every
+    closed-form gradient equals d s / d xi of the residual restatement."""
+    model, st, control, c, cfg = _skid_state(oracle_mod)
+    o, out, a_q, a_u, g = _forward_and_adjoint(oracle_mod, model, cfg, st, control, c)
+    B, D = model.bodies_per_world, model.dofs_per_world
+    # u- and f of the chassis and one wheel, world 1
+    for key in ("body_qd", "body_f"):
+        for idx in (6 * B + 0, 6 * B + 4, 6 * (B + 1) + 5):
+            fd = _fd_ws(oracle_mod, model, cfg, st, out, control, c, o, g, "in", key, idx, h=1e-4)
+            assert g[key].reshape(-1)[idx] == pytest.approx(fd, rel=1e-6, abs=1e-9)
+    # velocity targets (h is f32-representable so the control perturbation is exact)
+    for idx in range(model.num_worlds * D):
+        fd = _fd_ws(oracle_mod, model, cfg, st, out, control, c, o, g, "control", "joint_target_vel", idx,
+                    h=2.0 ** -10)
+        assert g["joint_target_vel"].reshape(-1)[idx] == pytest.approx(fd, rel=1e-6, abs=1e-9)
+    # motor gain ke (f32 model array)
+    for idx in range(D):
+        fd = _fd_ws(oracle_mod, model, cfg, st, out, control, c, o, g, "model", "joint_target_ke", D + idx,
+                    h=2.0 ** -6)
+        assert g["joint_target_ke"].reshape(-1)[D + idx] == pytest.approx(fd, rel=1e-4, abs=1e-9)
+
+
+def test_pose_pullback_and_carry(oracle_mod):
+    """g_q- = carry(g_q + ds/dq+) + ds/dq-, with ds/dq differenced here in FP64 and the
+    carry as the transposed Jacobian of q -> q + dt/2 Q(q) w (differenced here)."""
+    model, st, control, c, cfg = _skid_state(oracle_mod, worlds=1)
+    o, out, a_q, a_u, g = _forward_and_adjoint(oracle_mod, model, cfg, st, control, c)
+    B = model.bodies_per_world
+
+    def quat_update(q, w):
+        x, y, z, s = q
+        return np.array([x + (s * w[0] + z * w[1] - y * w[2]) * DT / 2,
+                         y + (-z * w[0] + s * w[1] + x * w[2]) * DT / 2,
+                         z + (y * w[0] - x * w[1] + s * w[2]) * DT / 2,
+                         s + (-x * w[0] - y * w[1] - z * w[2]) * DT / 2])
+
+    for b in range(B):
+        dplus = np.array([_fd_ws(oracle_mod, model, cfg, st, out, control, c, o, g, "out", "body_q", b * 7 + k)
+                          for k in range(7)])
+        dminus = np.array([_fd_ws(oracle_mod, model, cfg, st, out, control, c, o, g, "in", "body_q", b * 7 + k)
+                           for k in range(7)])
+        G = a_q[b] + dplus
+        q = out["body_q"][b, 3:]
+        w = out["body_qd"][b, 3:]
+        Jq = np.zeros((4, 4))
+        for k in range(4):
+            e = np.zeros(4); e[k] = 1e-6
+            Jq[:, k] = (quat_update(q + e, w) - quat_update(q - e, w)) / 2e-6
+        carry = np.concatenate([G[:3], Jq.T @ G[3:]])
+        np.testing.assert_allclose(g["body_q"][b], carry + dminus, rtol=1e-5, atol=1e-6 * np.abs(carry).max())
+
+
+def test_mu_gradient_inside_the_cone(oracle_mod):
+    """A box held by static friction against a push smaller than mu m g: the friction
+    impulse is inside the cone, where the lagged weight depends on mu; ds/dmu of both
+    shapes matches the closed form.  (Sliding contacts sit at the cone's fixed point,
+    where the reference's weight does not depend on mu: gradient 0.)"""
+    model, state = single_box(mass=1.0, mu=0.8, pose=(0, 0, 0.25, 0, 0, 0, 1))
+    state["body_f"][0] = [2.0, 1.0, 0.0, 0, 0, 0]
+    col = collide.PrimitiveCollider(model, gap=0.02, max_contacts_per_world=8)
+    st = {k: np.asarray(v, np.float64) for k, v in state.items()}
+    cfg = config(K=8, nr_iters=30)
+    for _ in range(3):
+        cc = col.collide(torch.as_tensor(st["body_q"], dtype=torch.float32))
+        oo = oracle_mod.step(model, cfg, st, zero_control(model), cc, DT, precision=64)
+        st = dict(st, body_q=oo["body_q"], body_qd=oo["body_qd"])
+    cc = col.collide(torch.as_tensor(st["body_q"], dtype=torch.float32))
+    o, out, a_q, a_u, g = _forward_and_adjoint(oracle_mod, model, cfg, st, zero_control(model), cc)
+    assert g["flags"][0] == 0
+    assert np.abs(g["shape_mu"]).max() > 1e-9
+    for s in range(g["shape_mu"].shape[1]):
+        fd = _fd_ws(oracle_mod, model, cfg, st, out, zero_control(model), cc, o, g, "model", "shape_mu", s,
+                    h=2.0 ** -12)
+        assert g["shape_mu"][0, s] == pytest.approx(fd, rel=1e-3, abs=1e-12)
+
+
+def test_zero_seed_gives_zero_gradients(oracle_mod):
+    model, st, control, c, cfg = _skid_state(oracle_mod)
+    o = oracle_mod.step(model, cfg, st, control, c, DT, precision=64)
+    WB = model.num_worlds * model.bodies_per_world
+    g = oracle_mod.adjoint(model, cfg, st, dict(body_q=o["body_q"], body_qd=o["body_qd"]), control, c, DT,
+                           o["lam"], o["lam_prev"], np.zeros((WB, 7)), np.zeros((WB, 6)))
+    for k in ("body_q", "body_qd", "body_f", "joint_target_vel", "shape_mu", "w_lambda", "w_u"):
+        assert np.abs(g[k]).max() == 0.0
+
+
+def test_fp32_adjoint_close_to_fp64(oracle_mod):
+    """The FP32 restatement with pose differences of step 2e-3 (the GPU's arithmetic)
+    stays within 1e-4 of the FP64 gradients' scale."""
+    model, st, control, c, cfg = _skid_state(oracle_mod, worlds=3)
+    o, out, a_q, a_u, g64 = _forward_and_adjoint(oracle_mod, model, cfg, st, control, c)
+    g32 = oracle_mod.adjoint(model, cfg, st, out, control, c, DT, o["lam"], o["lam_prev"], a_q, a_u, precision=32,
+                             fd_eps=2e-3)
+    for k in ("body_q", "body_qd", "body_f", "joint_target_vel"):
+        scale = np.abs(g64[k]).max()
+        assert np.abs(g32[k] - g64[k]).max() <= 1e-4 * scale, k
