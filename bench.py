@@ -96,6 +96,52 @@ def run_path(variant, model, state, control, steps, warmup, device, collider):
                 converged_fraction=float(np.mean(conv)), contacts_mean=float(np.mean(ncon)))
 
 
+def adjoint_leg(device, worlds=1024, steps=50, checkpoint_every=10, seed=3):
+    """Differentiable 50-step rollout of skid-steer robots on a flat plane (mu 0.8, wheel
+    velocity targets U(-3,3) per step), loss on the final chassis x, y and yaw.  Times the
+    forward rollout and the checkpointed backward (re-simulation of every window plus
+    one adjoint per step) with HIP events on the solver stream."""
+    from ostrich_amd import OstrichEngine
+    from ostrich_amd.differentiable import CheckpointedRollout
+    model, state, control = scenes.skid_steer(worlds, seed=seed, max_obstacles=0, mu_range=(0.8, 0.8))
+    cfg = solver_config("fused")
+    eng = OstrichEngine(model, config=cfg, device=device)
+    col = collide.PrimitiveCollider(model, device=device, gap=0.02, max_contacts_per_world=K_CONTACTS)
+    collide_fn = lambda q: Contacts.from_dict(col.collide(q), device)
+    rng = np.random.default_rng(seed)
+    D, B = model.dofs_per_world, model.bodies_per_world
+    controls = [Control(torch.as_tensor(control["joint_target_pos"], device=device),
+                        torch.as_tensor(rng.uniform(-3, 3, worlds * D).astype(np.float32), device=device))
+                for _ in range(steps)]
+    s0 = State.from_numpy(state, device)
+    target = torch.as_tensor(np.stack([rng.uniform(-1, 1, worlds), rng.uniform(-1, 1, worlds),
+                                       rng.uniform(-np.pi / 4, np.pi / 4, worlds)], 1), dtype=torch.float32,
+                             device=device)
+    stream = torch.cuda.current_stream(device)
+    ro = CheckpointedRollout(eng, collide_fn, DT, checkpoint_every=checkpoint_every)
+    ro.forward(s0, controls[:2])  # warm-up of both directions
+    ro.backward(torch.zeros_like(s0.body_q), torch.zeros_like(s0.body_qd))
+    e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+    ro = CheckpointedRollout(eng, collide_fn, DT, checkpoint_every=checkpoint_every)
+    e0.record(stream)
+    fin = ro.forward(s0, controls)
+    e1.record(stream)
+    q = fin.body_q.reshape(worlds, B, 7)[:, 0].detach().clone().requires_grad_(True)
+    yaw = torch.atan2(2 * (q[:, 6] * q[:, 5] + q[:, 3] * q[:, 4]), 1 - 2 * (q[:, 4] ** 2 + q[:, 5] ** 2))
+    loss = ((q[:, 0] - target[:, 0]) ** 2 + (q[:, 1] - target[:, 1]) ** 2 + (yaw - target[:, 2]) ** 2).sum()
+    loss.backward()
+    gq = torch.zeros_like(fin.body_q).reshape(worlds, B, 7)
+    gq[:, 0] = q.grad
+    g = ro.backward(gq.reshape(-1, 7), torch.zeros_like(fin.body_qd))
+    e2.record(stream)
+    e2.synchronize()
+    fwd, bwd = e0.elapsed_time(e1), e1.elapsed_time(e2)
+    return dict(worlds=worlds, steps=steps, checkpoint_every=checkpoint_every, forward_ms=fwd, backward_ms=bwd,
+                backward_ms_per_step=bwd / steps, loss=float(loss.item()),
+                grad_target_vel_norm=float(g["joint_target_vel"].norm().item()),
+                note="backward includes re-simulating each 10-step window from its checkpoint")
+
+
 def cpu_baseline(model_fn, sample_worlds, sample_steps, precond="jacobi"):
     """The oracle (FP32 CPU restatement, one thread) on the first `sample_worlds` worlds
     of the same seeded scene; only the solver calls are timed."""
@@ -129,6 +175,7 @@ def main():
     ap.add_argument("--cpu-sample-worlds", type=int, default=64)
     ap.add_argument("--cpu-sample-steps", type=int, default=40)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: worlds are independent, so the solver shards by world range "
@@ -188,6 +235,10 @@ def main():
     fj, pj = results.get("fused:jacobi", {}), results.get("plain:jacobi", {})
     if "world_steps_per_s" in fj and "world_steps_per_s" in pj:
         out["fused_over_plain_jacobi"] = fj["world_steps_per_s"] / pj["world_steps_per_s"]
+    if not args.no_adjoint:
+        print("[bench] adjoint rollout", file=sys.stderr, flush=True)
+        out["adjoint"] = adjoint_leg(device)
+        print(f"[bench] adjoint: {out['adjoint']}", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(lambda n: scenes.skid_steer(n, seed=1), args.cpu_sample_worlds,

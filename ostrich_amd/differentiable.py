@@ -1,0 +1,213 @@
+"""Differentiable stepping: the torch bridge and checkpointed rollouts.
+
+* `OstrichStep` - a `torch.autograd.Function` around one solver step whose backward is
+  the gfx950 adjoint (`OstrichEngine.step_backward`, core/base_engine.py:528-743).  It
+  plays the part of the reference's torch bridges (learning/differentiable_newton_step.py:
+  33-250, learning/differentiable_linear_solve.py:16-103) for the whole step.
+* `TrajectoryBuffer` - per-step records the adjoint needs (input/output state, control,
+  contacts, lambda, lambda_prev_iter), as simulation/trajectory_buffer.py:106-351 keeps
+  them, held on the device.
+* `CheckpointedRollout` - forward over N steps keeping only the states at every k-th step;
+  the backward re-simulates each window of k steps from its checkpoint into a
+  TrajectoryBuffer and runs the adjoint backwards through it
+  (simulation/differentiable_simulator.py:234-271 with the windowed recompute of
+  experiments/4_scalability_box/ostrich_sim_ckpt.py).  Memory is O(N/k + k) step records
+  instead of O(N).  The solver is deterministic (fixed accumulation order), so a
+  re-simulated window reproduces the forward bit for bit.
+* `OstrichRollout` - a `torch.autograd.Function` over a whole checkpointed rollout.
+
+Gradient flow between steps follows the reference: dL/dq- of step t is added to the
+pose gradient of state t, and dL/du- of step t becomes the velocity gradient of state t
+(trajectory_buffer.py:266-297, accumulated here with the loss's own gradient at t).
+With `gradient_normalization` the carried (pose, velocity) gradient is rescaled to unit
+norm per step, as TrajectoryBuffer.normalize_gradients (:299-351).
+"""
+from dataclasses import dataclass
+from typing import Callable, List, Optional
+
+import torch
+
+from .containers import Contacts, Control, State
+
+
+@dataclass
+class StepRecord:
+    state_in: State
+    state_out: State
+    control: Control
+    contacts: Contacts
+    lam: torch.Tensor
+    lam_prev: torch.Tensor
+
+
+class TrajectoryBuffer:
+    """Step records of one window (simulation/trajectory_buffer.py:106-351)."""
+
+    def __init__(self):
+        self.steps: List[StepRecord] = []
+
+    def save_step(self, rec: StepRecord):
+        self.steps.append(rec)
+
+    def __len__(self):
+        return len(self.steps)
+
+    def __getitem__(self, i) -> StepRecord:
+        return self.steps[i]
+
+    def clear(self):
+        self.steps.clear()
+
+
+def _record_step(engine, state_in: State, control: Control, contacts: Contacts, dt: float) -> StepRecord:
+    out = state_in.clone()
+    engine.step(state_in, out, control, contacts, dt)
+    return StepRecord(state_in, out, control, contacts, engine.constraint_impulses(),
+                      engine.constraint_impulses_prev())
+
+
+class OstrichStep(torch.autograd.Function):
+    """(body_q, body_qd, body_f, target_pos, target_vel, shape_mu) -> (body_q+, body_qd+).
+
+    `contacts` are treated as constant (collision is not differentiated, as in the
+    reference).  `shape_mu` (W*(S+G)) may be None to keep the model's friction."""
+
+    @staticmethod
+    def forward(ctx, engine, contacts, dt, body_q, body_qd, body_f, target_pos, target_vel, shape_mu=None):
+        if shape_mu is not None:
+            engine.set_shape_mu(shape_mu.detach())
+        s_in = State(body_q.detach().clone().contiguous(), body_qd.detach().clone().contiguous(),
+                     body_f.detach().clone().contiguous())
+        ctl = Control(target_pos.detach().clone().contiguous(), target_vel.detach().clone().contiguous())
+        rec = _record_step(engine, s_in, ctl, contacts, dt)
+        ctx.engine, ctx.dt, ctx.rec = engine, dt, rec
+        ctx.has_mu = shape_mu is not None
+        return rec.state_out.body_q.clone(), rec.state_out.body_qd.clone()
+
+    @staticmethod
+    def backward(ctx, g_q, g_qd):
+        r = ctx.rec
+        if g_q is None:
+            g_q = torch.zeros_like(r.state_out.body_q)
+        if g_qd is None:
+            g_qd = torch.zeros_like(r.state_out.body_qd)
+        g = ctx.engine.step_backward(r.state_in, r.state_out, r.control, r.contacts, ctx.dt, r.lam, r.lam_prev,
+                                     g_q, g_qd)
+        gmu = g["shape_mu"].reshape(-1) if ctx.has_mu else None
+        return (None, None, None, g["body_q"], g["body_qd"], g["body_f"],
+                g["joint_target_pos"].reshape(r.control.joint_target_pos.shape),
+                g["joint_target_vel"].reshape(r.control.joint_target_vel.shape), gmu)
+
+
+class CheckpointedRollout:
+    """N steps of `engine` with contacts from `collide(body_q) -> Contacts`, checkpoints
+    every `checkpoint_every` steps."""
+
+    def __init__(self, engine, collide: Callable[[torch.Tensor], Contacts], dt: float, checkpoint_every: int = 10,
+                 gradient_normalization: Optional[bool] = None):
+        if checkpoint_every < 1:
+            raise ValueError("checkpoint_every must be >= 1")
+        self.engine, self.collide, self.dt, self.k = engine, collide, dt, checkpoint_every
+        gn = engine.config.adjoint.gradient_normalization if gradient_normalization is None else gradient_normalization
+        self.gradient_normalization = gn
+        self.checkpoints: List[State] = []
+        self.controls: List[Control] = []
+        self.final: Optional[State] = None
+        self.recomputed_steps = 0
+
+    def _step(self, state: State, t: int) -> StepRecord:
+        contacts = self.collide(state.body_q)
+        return _record_step(self.engine, state, self.controls[t], contacts, self.dt)
+
+    def forward(self, state0: State, controls: List[Control]) -> State:
+        """Runs len(controls) steps from state0; keeps the states at steps 0, k, 2k, ..."""
+        self.controls = list(controls)
+        self.checkpoints = []
+        s = state0.clone()
+        for t in range(len(self.controls)):
+            if t % self.k == 0:
+                self.checkpoints.append(s.clone())
+            s = self._step(s, t).state_out
+        self.final = s
+        return s
+
+    def _window(self, j: int) -> TrajectoryBuffer:
+        buf = TrajectoryBuffer()
+        t0 = j * self.k
+        t1 = min(t0 + self.k, len(self.controls))
+        s = self.checkpoints[j].clone()
+        for t in range(t0, t1):
+            rec = self._step(s, t)
+            buf.save_step(rec)
+            s = rec.state_out
+        self.recomputed_steps += t1 - t0
+        return buf
+
+    def backward(self, grad_q: torch.Tensor, grad_qd: torch.Tensor,
+                 step_loss_grads: Optional[Callable[[int], Optional[tuple]]] = None):
+        """Adjoint of the whole rollout from dL/dq_N, dL/du_N.
+
+        `step_loss_grads(t)` may return (dL/dq_t, dL/du_t) of a loss on intermediate state
+        t (1 <= t < N), added to the carried gradient.  Returns dict: body_q, body_qd (of
+        the initial state), body_f [N, ...], joint_target_pos / joint_target_vel [N, W, D],
+        joint_target_ke, joint_target_kd [W, D] and shape_mu [W, S+G] summed over steps."""
+        N = len(self.controls)
+        gq = grad_q.reshape(self.final.body_q.shape).clone()
+        gu = grad_qd.reshape(self.final.body_qd.shape).clone()
+        out = dict(body_f=[None] * N, joint_target_pos=[None] * N, joint_target_vel=[None] * N)
+        acc = {}
+        for j in reversed(range(len(self.checkpoints))):
+            buf = self._window(j)
+            t0 = j * self.k
+            for i in reversed(range(len(buf))):
+                t = t0 + i
+                r = buf[i]
+                g = self.engine.step_backward(r.state_in, r.state_out, r.control, r.contacts, self.dt, r.lam,
+                                              r.lam_prev, gq, gu)
+                out["body_f"][t] = g["body_f"]
+                out["joint_target_pos"][t] = g["joint_target_pos"]
+                out["joint_target_vel"][t] = g["joint_target_vel"]
+                for k in ("joint_target_ke", "joint_target_kd", "shape_mu"):
+                    acc[k] = g[k] if k not in acc else acc[k] + g[k]
+                gq, gu = g["body_q"], g["body_qd"]
+                if step_loss_grads is not None and t > 0:
+                    extra = step_loss_grads(t)
+                    if extra is not None:
+                        gq = gq + extra[0].reshape(gq.shape)
+                        gu = gu + extra[1].reshape(gu.shape)
+                if self.gradient_normalization and t > 0:
+                    nrm = torch.sqrt((gq * gq).sum() + (gu * gu).sum())
+                    scale = torch.where(nrm > 0, 1.0 / nrm, torch.zeros_like(nrm))
+                    gq, gu = gq * scale, gu * scale
+        out = {k: torch.stack(v) for k, v in out.items()}
+        out.update(acc)
+        out["body_q"], out["body_qd"] = gq, gu
+        return out
+
+
+class OstrichRollout(torch.autograd.Function):
+    """(body_q0, body_qd0, target_pos [N, W*D], target_vel [N, W*D]) -> (body_q_N, body_qd_N)
+    through a CheckpointedRollout (body_f is held at zero)."""
+
+    @staticmethod
+    def forward(ctx, engine, collide, dt, checkpoint_every, body_q0, body_qd0, target_pos, target_vel):
+        N = target_vel.shape[0]
+        s0 = State(body_q0.detach().contiguous(), body_qd0.detach().contiguous(), torch.zeros_like(body_qd0))
+        controls = [Control(target_pos[t].detach().contiguous(), target_vel[t].detach().contiguous())
+                    for t in range(N)]
+        ro = CheckpointedRollout(engine, collide, dt, checkpoint_every)
+        fin = ro.forward(s0, controls)
+        ctx.ro = ro
+        return fin.body_q.clone(), fin.body_qd.clone()
+
+    @staticmethod
+    def backward(ctx, g_q, g_qd):
+        ro = ctx.ro
+        if g_q is None:
+            g_q = torch.zeros_like(ro.final.body_q)
+        if g_qd is None:
+            g_qd = torch.zeros_like(ro.final.body_qd)
+        g = ro.backward(g_q, g_qd)
+        N = len(ro.controls)
+        return (None, None, None, None, g["body_q"], g["body_qd"], g["joint_target_pos"].reshape(N, -1),
+                g["joint_target_vel"].reshape(N, -1))
