@@ -131,6 +131,12 @@ typedef struct ofb_config {
                                     pose gradients */
   float adjoint_pose_eps;        /* central-difference step of that pull-back, in pose
                                     coordinates (m and quaternion components) */
+  int32_t warm_start;            /* warm_start.enabled, method position_match: seed each
+                                    step from the previous step's contacts (acts only
+                                    with seed_iterate, as in the reference) */
+  int32_t warm_start_cold_gravity; /* warm_start.cold_gravity */
+  int32_t warm_start_cold_impact;  /* warm_start.cold_impact */
+  float warm_start_cold_friction_v;/* warm_start.cold_friction_v_threshold (<= 0: off) */
 } ofb_config;
 
 typedef struct ofb_state_in {
@@ -250,6 +256,14 @@ int ofb_step_backward(ofb_solver* solver, const ofb_state_in* state_in, const of
                       const ofb_control* control, const ofb_contacts* contacts, const float* lambda,
                       const float* lambda_prev, const ofb_adjoint_in* grad_in, const ofb_adjoint_out* grad_out,
                       float dt, void* stream);
+
+/* Forgets the warm starter's snapshot (ContactWarmStarter state, collision/warm_start.py:
+ * 620-640), e.g. at an episode reset: the next step cold-starts every contact. */
+int ofb_warm_start_reset(ofb_solver* solver, void* stream);
+
+/* Copies the Newton seed [W*N_c] the last step started from (warm starter output, or the
+ * seed set with ofb_set_lambda_seed) - _constr_force after engine.py:55-59. */
+int ofb_get_lambda_seed(const ofb_solver* solver, float* seed_out, void* stream);
 
 /* Copies the packed per-world contact set of the last step: count [W] and, for each of
  * the W*C_max slots, shape0/shape1 (world-local columns), point0/point1/normal (after the

@@ -1432,6 +1432,172 @@ template <class T> int run_adjoint(const ofb_model* m, const ofb_config* cfg, co
   return 0;
 }
 
+// ----------------------------------------------------------------------------------------
+// Cross-step contact warm start, position_match (collision/warm_start.py:84-519).  The
+// snapshot of the previous step lives in caller arrays (per world: count; per slot: b0,
+// b1, world midpoint, normal, lambda_n, lambda_t).  Both hooks use the start-of-step
+// pose and velocity (body_pose_prev / body_vel_prev, warm_start.py:600-609).
+// ----------------------------------------------------------------------------------------
+struct WSPrev {
+  int32_t* count;  // [W]
+  int32_t* b0;     // [W*K]
+  int32_t* b1;
+  double* p;       // [W*K*3]
+  double* n;       // [W*K*3]
+  double* ln;      // [W*K]
+  double* lt;      // [W*K*2]
+};
+
+constexpr double kWsAlpha = 0.1, kWsMinThreshold = 5.0e-3, kWsNDotUpMin = 0.3;  // warm_start.py:520-538
+
+// midpoint of contact c in world frame (_world_midpoint, warm_start.py:51-82)
+inline V3<double> ws_midpoint(const Batch<double>& Bt, int w, int c, const std::vector<Tf<double>>& pose, int b0,
+                              int b1) {
+  size_t k = size_t(w) * Bt.Cmax + c;
+  V3<double> n = Bt.cvec(Bt.cn, w, c);
+  Tf<double> pa = b0 >= 0 ? pose[Bt.bi(w, b0)] : tident<double>();
+  Tf<double> pb = b1 >= 0 ? pose[Bt.bi(w, b1)] : tident<double>();
+  V3<double> a = sub(tpoint(pa, Bt.cvec(Bt.cp0, w, c)), mul(n, Bt.ct0[k]));
+  V3<double> b = add(tpoint(pb, Bt.cvec(Bt.cp1, w, c)), mul(n, Bt.ct1[k]));
+  return mul(add(a, b), 0.5);
+}
+
+// apply(): _match_kernel (:84-297) then _cold_start_kernel (:300-462) -> seed [W*Nc]
+void ws_apply(Batch<double>& Bt, const WSPrev& P, int cold_gravity, int cold_impact, double v_t_threshold,
+              double* seed) {
+  const int W = Bt.W, K = Bt.Cmax, B = Bt.B;
+  const double dt = Bt.dt;
+  V3<double> g = v3<double>(Bt.M.gravity[0], Bt.M.gravity[1], Bt.M.gravity[2]);
+  double gmag = std::sqrt(dot(g, g));
+  V3<double> up = gmag > 1e-6 ? mul(g, -1.0 / gmag) : v3<double>(0, 0, 1);
+  int on = Bt.Nj + Bt.Nctrl, of = on + K;
+  for (int w = 0; w < W; ++w) {
+    for (int i = 0; i < Bt.Nc; ++i) seed[Bt.ri(w, i)] = 0;
+    std::vector<double> support(B, 0.0);
+    std::vector<int> unmatched(B, 0), matched(K, 0);
+    const int n_c = std::min(Bt.ccount[w], K);
+    const int n_prev = std::min(P.count[w], K);
+    for (int c = 0; c < n_c; ++c) {
+      size_t k = size_t(w) * K + c;
+      int s0 = Bt.cs0[k], s1 = Bt.cs1[k];
+      if (s0 == s1) continue;
+      int b0 = s0 >= 0 ? Bt.shape_body(w, s0) : -1, b1 = s1 >= 0 ? Bt.shape_body(w, s1) : -1;
+      V3<double> pc = ws_midpoint(Bt, w, c, Bt.q_prev, b0, b1);
+      V3<double> nc = Bt.cvec(Bt.cn, w, c);
+      V3<double> vt = v3<double>(0, 0, 0);
+      if (b0 >= 0) vt = add(vt, top(Bt.u_prev[Bt.bi(w, b0)]));
+      if (b1 >= 0) vt = sub(vt, top(Bt.u_prev[Bt.bi(w, b1)]));
+      vt = sub(vt, mul(nc, dot(vt, nc)));
+      double thr = std::max(kWsMinThreshold, kWsAlpha * std::sqrt(dot(vt, vt)) * dt);
+      double best_d = thr * thr;
+      int best = -1;
+      for (int j = 0; j < n_prev; ++j) {
+        size_t kj = size_t(w) * K + j;
+        if (P.b0[kj] != b0 || P.b1[kj] != b1) continue;
+        V3<double> pp = add(v3<double>(P.p[kj * 3], P.p[kj * 3 + 1], P.p[kj * 3 + 2]), mul(vt, dt));
+        V3<double> d = sub(pc, pp);
+        double dsq = dot(d, d);
+        if (dsq < best_d) { best_d = dsq; best = j; }
+      }
+      double ndu = dot(nc, up);
+      bool sup0 = -ndu > kWsNDotUpMin, sup1 = ndu > kWsNDotUpMin;
+      if (best < 0) {
+        if (b0 >= 0 && sup0) unmatched[b0] += 1;
+        if (b1 >= 0 && sup1) unmatched[b1] += 1;
+        continue;
+      }
+      size_t kb = size_t(w) * K + best;
+      V3<double> pn = v3<double>(P.n[kb * 3], P.n[kb * 3 + 1], P.n[kb * 3 + 2]);
+      V3<double> pt1, pt2, ct1, ct2;
+      orthogonal_basis(pn, pt1, pt2);
+      orthogonal_basis(nc, ct1, ct2);
+      V3<double> ltw = add(mul(pt1, P.lt[kb * 2]), mul(pt2, P.lt[kb * 2 + 1]));
+      double lnm = P.ln[kb];
+      seed[Bt.ri(w, on + c)] = lnm;
+      seed[Bt.ri(w, of + 2 * c)] = dot(ltw, ct1);
+      seed[Bt.ri(w, of + 2 * c + 1)] = dot(ltw, ct2);
+      matched[c] = 1;
+      if (b0 >= 0 && sup0) support[b0] += -ndu * lnm;
+      if (b1 >= 0 && sup1) support[b1] += ndu * lnm;
+    }
+    for (int c = 0; c < n_c; ++c) {
+      if (matched[c]) continue;
+      size_t k = size_t(w) * K + c;
+      int s0 = Bt.cs0[k], s1 = Bt.cs1[k];
+      if (s0 == s1) continue;
+      int b0 = s0 >= 0 ? Bt.shape_body(w, s0) : -1, b1 = s1 >= 0 ? Bt.shape_body(w, s1) : -1;
+      V3<double> n = Bt.cvec(Bt.cn, w, c);
+      V3<double> p = ws_midpoint(Bt, w, c, Bt.q_prev, b0, b1);
+      V3<double> v0 = v3<double>(0, 0, 0), v1 = v0;
+      if (b0 >= 0) {  // velocity about the body origin, as the reference computes it (:383-392)
+        const Sp<double>& tw = Bt.u_prev[Bt.bi(w, b0)];
+        v0 = add(top(tw), cross(bot(tw), sub(p, Bt.q_prev[Bt.bi(w, b0)].p)));
+      }
+      if (b1 >= 0) {
+        const Sp<double>& tw = Bt.u_prev[Bt.bi(w, b1)];
+        v1 = add(top(tw), cross(bot(tw), sub(p, Bt.q_prev[Bt.bi(w, b1)].p)));
+      }
+      V3<double> vr = sub(v1, v0);
+      double vn = dot(vr, n);
+      V3<double> vtt = sub(vr, mul(n, vn));
+      double l_imp = 0;
+      if (cold_impact && vn < 0) {
+        double im = (b0 >= 0 ? Bt.inv_mass(w, b0) : 0.0) + (b1 >= 0 ? Bt.inv_mass(w, b1) : 0.0);
+        if (im > 1e-9) l_imp = (1.0 / im) * (-vn) / dt;
+      }
+      double ndu = dot(n, up), l_grav = 0;
+      if (cold_gravity) {
+        if (b0 >= 0 && -ndu > kWsNDotUpMin && unmatched[b0] > 0) {
+          double res = Bt.mass(w, b0) * gmag - support[b0];
+          if (res > 0) l_grav = std::max(l_grav, res / (double(unmatched[b0]) * (-ndu)));
+        }
+        if (b1 >= 0 && ndu > kWsNDotUpMin && unmatched[b1] > 0) {
+          double res = Bt.mass(w, b1) * gmag - support[b1];
+          if (res > 0) l_grav = std::max(l_grav, res / (double(unmatched[b1]) * ndu));
+        }
+      }
+      double ln = std::max(l_imp, l_grav);
+      if (ln > 0) seed[Bt.ri(w, on + c)] = ln;
+      if (v_t_threshold > 0 && ln > 0) {
+        double vsq = dot(vtt, vtt);
+        if (vsq > v_t_threshold * v_t_threshold) {
+          V3<double> th = mul(vtt, -1.0 / std::sqrt(vsq));
+          double mu = 0.5 * (Bt.smu(w, s0) + Bt.smu(w, s1));
+          V3<double> ft = mul(th, mu * ln);
+          V3<double> t1, t2;
+          orthogonal_basis(n, t1, t2);
+          seed[Bt.ri(w, of + 2 * c)] = dot(ft, t1);
+          seed[Bt.ri(w, of + 2 * c + 1)] = dot(ft, t2);
+        }
+      }
+    }
+  }
+}
+
+// snapshot(): _snapshot_kernel (:465-519) with the step's lambda
+void ws_snapshot(const Batch<double>& Bt, const double* lam, const WSPrev& P) {
+  const int K = Bt.Cmax;
+  int on = Bt.Nj + Bt.Nctrl, of = on + K;
+  for (int w = 0; w < Bt.W; ++w) {
+    const int n_c = std::min(Bt.ccount[w], K);
+    P.count[w] = n_c;
+    for (int c = 0; c < n_c; ++c) {
+      size_t k = size_t(w) * K + c;
+      int s0 = Bt.cs0[k], s1 = Bt.cs1[k];
+      if (s0 == s1) { P.b0[k] = -2; P.b1[k] = -2; continue; }  // never matched
+      int b0 = s0 >= 0 ? Bt.shape_body(w, s0) : -1, b1 = s1 >= 0 ? Bt.shape_body(w, s1) : -1;
+      V3<double> m = ws_midpoint(Bt, w, c, Bt.q_prev, b0, b1);
+      V3<double> n = Bt.cvec(Bt.cn, w, c);
+      P.b0[k] = b0; P.b1[k] = b1;
+      P.p[k * 3] = m.x; P.p[k * 3 + 1] = m.y; P.p[k * 3 + 2] = m.z;
+      P.n[k * 3] = n.x; P.n[k * 3 + 1] = n.y; P.n[k * 3 + 2] = n.z;
+      P.ln[k] = lam[Bt.ri(w, on + c)];
+      P.lt[k * 2] = lam[Bt.ri(w, of + 2 * c)];
+      P.lt[k * 2 + 1] = lam[Bt.ri(w, of + 2 * c + 1)];
+    }
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -1593,6 +1759,30 @@ int oracle_adjoint(const ofb_model* m, const ofb_config* cfg, const double* q_pr
   return run_adjoint<float>(m, cfg, q_prev, qd_prev, f_in, q, qd, tpos, tvel, contacts, dt, lam, lam_prev, g_q, g_u,
                             fd_eps, g_q_prev, g_u_prev, g_f, g_tpos, g_tvel, g_ke, g_kd, g_mu, w_lam, w_u, pcr_iters,
                             flags);
+}
+
+// Warm start of one step: mode 0 = apply (writes seed [W*N_c] from the snapshot), mode 1 =
+// snapshot (stores the step's contacts and lambda [W*N_c]).  State is the step-start
+// state (q-, u-).  FP64.
+int oracle_warm_start(const ofb_model* m, const ofb_config* cfg, const double* q_prev, const double* qd_prev,
+                      const ofb_contacts* contacts, float dt, int mode, const double* lam, int32_t* prev_count,
+                      int32_t* prev_b0, int32_t* prev_b1, double* prev_p, double* prev_n, double* prev_ln,
+                      double* prev_lt, double* seed) {
+  Batch<double> Bt(*m, *cfg);
+  Bt.dt = dt;
+  pack_contacts(Bt, *contacts);
+  for (int w = 0; w < Bt.W; ++w)
+    for (int b = 0; b < Bt.B; ++b) {
+      size_t k = Bt.bi(w, b);
+      const double* x = q_prev + k * 7;
+      Bt.q_prev[k] = Tf<double>{v3<double>(x[0], x[1], x[2]), Qt<double>{x[3], x[4], x[5], x[6]}};
+      for (int i = 0; i < 6; ++i) Bt.u_prev[k].v[i] = qd_prev[k * 6 + i];
+    }
+  WSPrev P{prev_count, prev_b0, prev_b1, prev_p, prev_n, prev_ln, prev_lt};
+  if (mode == 0) ws_apply(Bt, P, cfg->warm_start_cold_gravity, cfg->warm_start_cold_impact,
+                          cfg->warm_start_cold_friction_v, seed);
+  else ws_snapshot(Bt, lam, P);
+  return 0;
 }
 
 }  // extern "C"

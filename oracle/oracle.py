@@ -37,6 +37,8 @@ def lib():
         _lib.oracle_adjoint.restype = C.c_int
         _lib.oracle_weighted_residual.argtypes = [vp] * 10 + [C.c_float] + [vp] * 4
         _lib.oracle_weighted_residual.restype = C.c_double
+        _lib.oracle_warm_start.argtypes = [vp] * 5 + [C.c_float, C.c_int] + [vp] * 9
+        _lib.oracle_warm_start.restype = C.c_int
         _lib.oracle_step.restype = C.c_int
         _lib.oracle_dims.argtypes = [vp, vp, vp, vp, vp]
         _lib.oracle_fb_diff.argtypes = [C.c_double] * 5 + [vp]
@@ -219,3 +221,43 @@ def weighted_residual(model, config, state_in, state_out, control, contacts, dt,
     tail = [f64(lam, (W, nc)), f64(lam_prev, (W, nc)), f64(w_lambda, (W, nc)), f64(w_u, (WB, 6))]
     return lib().oracle_weighted_residual(C.byref(hm.abi), C.byref(cfg), *[_p(a) for a in args], C.byref(hc.abi),
                                           C.c_float(dt), *[_p(a) for a in tail])
+
+
+class WarmStarter:
+    """Cross-step contact warm start (collision/warm_start.py, position_match) with its
+    snapshot held here.  `apply` returns the seed [W, N_c] of a step from the step-start
+    state and contacts; `snapshot` stores the step's contacts and lambda."""
+
+    def __init__(self, model, config):
+        self.model, self.config = model, config
+        self.hm = HostModel(model)
+        W, K = model.num_worlds, config.contacts.max_per_world
+        self.nc, _, _ = dims(model, config)
+        self.count = np.zeros(W, np.int32)
+        self.b0 = np.zeros(W * K, np.int32)
+        self.b1 = np.zeros(W * K, np.int32)
+        self.p = np.zeros(W * K * 3)
+        self.n = np.zeros(W * K * 3)
+        self.ln = np.zeros(W * K)
+        self.lt = np.zeros(W * K * 2)
+
+    def _call(self, state, contacts, dt, mode, lam=None):
+        hc = contacts if isinstance(contacts, HostContacts) else HostContacts(contacts)
+        cfg = self.config.to_abi()
+        WB = self.model.num_worlds * self.model.bodies_per_world
+        q = np.ascontiguousarray(np.asarray(state["body_q"], np.float64).reshape(WB, 7))
+        qd = np.ascontiguousarray(np.asarray(state["body_qd"], np.float64).reshape(WB, 6))
+        seed = np.zeros((self.model.num_worlds, self.nc))
+        la = None if lam is None else np.ascontiguousarray(np.asarray(lam, np.float64).reshape(-1))
+        rc = lib().oracle_warm_start(C.byref(self.hm.abi), C.byref(cfg), _p(q), _p(qd), C.byref(hc.abi),
+                                     C.c_float(dt), mode, _p(la), _p(self.count), _p(self.b0), _p(self.b1),
+                                     _p(self.p), _p(self.n), _p(self.ln), _p(self.lt), _p(seed))
+        if rc != 0:
+            raise RuntimeError("oracle_warm_start failed")
+        return seed
+
+    def apply(self, state, contacts, dt):
+        return self._call(state, contacts, dt, 0)
+
+    def snapshot(self, state, contacts, dt, lam):
+        self._call(state, contacts, dt, 1, lam)

@@ -84,6 +84,10 @@ class OfbConfig(C.Structure):
         ("adjoint_regularization", C.c_float),
         ("adjoint_pose_vjp", C.c_int32),
         ("adjoint_pose_eps", C.c_float),
+        ("warm_start", C.c_int32),
+        ("warm_start_cold_gravity", C.c_int32),
+        ("warm_start_cold_impact", C.c_int32),
+        ("warm_start_cold_friction_v", C.c_float),
     ]
 
 

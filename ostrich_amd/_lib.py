@@ -11,7 +11,8 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "lib
 EXPORTS = (
     "ofb_create", "ofb_destroy", "ofb_dims", "ofb_step", "ofb_get_stats", "ofb_get_lambda",
     "ofb_get_packed_contacts", "ofb_set_lambda_seed", "ofb_last_solve_ms", "ofb_last_error",
-    "ofb_get_lambda_prev", "ofb_set_shape_mu", "ofb_step_backward",
+    "ofb_get_lambda_prev", "ofb_set_shape_mu", "ofb_step_backward", "ofb_warm_start_reset",
+    "ofb_get_lambda_seed",
 )
 
 _lib = None
@@ -38,6 +39,8 @@ def load():
     lib.ofb_get_lambda_prev.argtypes = [vp, vp, vp]
     lib.ofb_set_shape_mu.argtypes = [vp, vp, vp]
     lib.ofb_step_backward.argtypes = [vp] * 9 + [C.c_float, vp]
+    lib.ofb_warm_start_reset.argtypes = [vp, vp]
+    lib.ofb_get_lambda_seed.argtypes = [vp, vp, vp]
     for name in EXPORTS:
         if name != "ofb_last_error":
             getattr(lib, name).restype = C.c_int

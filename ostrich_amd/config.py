@@ -318,6 +318,12 @@ class OstrichEngineConfig:
         c.adjoint_regularization = self.adjoint.regularization
         c.adjoint_pose_vjp = 1 if self.adjoint.pose_vjp else 0
         c.adjoint_pose_eps = self.adjoint.pose_fd_eps
+        if self.warm_start.method != "position_match" and self.warm_start.enabled and self.warm_start.seed_iterate:
+            raise NotImplementedError("warm_start.method='pair_aggregate' is not built; use 'position_match'")
+        c.warm_start = 1 if self.warm_start.enabled else 0
+        c.warm_start_cold_gravity = 1 if self.warm_start.cold_gravity else 0
+        c.warm_start_cold_impact = 1 if self.warm_start.cold_impact else 0
+        c.warm_start_cold_friction_v = self.warm_start.cold_friction_v_threshold
         return c
 
     @property

@@ -200,7 +200,8 @@ int ofb_destroy(ofb_solver* s) {
                   m.shape_body, m.shape_mu, m.shape_friction_axis, m.shape_mu_perp, m.shape_mu_stiction_scale,
                   m.shape_v_stribeck, m.shape_stribeck_lateral_only, m.shape_world, s->ccount, s->cf, s->ci,
                   s->pk_keys, s->pk_keys_out, s->pk_vals, s->pk_vals_out, s->pk_seg_start, s->pk_temp, s->topo,
-                  s->lambda, s->lambda_prev, s->lambda_seed, s->st_nr, s->st_pcr, s->st_best, s->st_flags, s->st_merit};
+                  s->lambda, s->lambda_prev, s->lambda_seed, s->ws_count, s->ws_b0, s->ws_b1, s->ws_p, s->ws_n,
+                  s->ws_ln, s->ws_lt, s->st_nr, s->st_pcr, s->st_best, s->st_flags, s->st_merit};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (s->plain) plain_destroy(s);
@@ -308,6 +309,15 @@ int ofb_create(const ofb_model* model, const ofb_config* config, int32_t path, o
   s->rigid_contact_cap = cap;
   ALLOC(pk_keys, cap); ALLOC(pk_keys_out, cap); ALLOC(pk_vals, cap); ALLOC(pk_vals_out, cap);
   ALLOC(pk_seg_start, W);
+  if (config->warm_start) {
+    ALLOC(ws_count, W); ALLOC(ws_b0, (size_t)W * K); ALLOC(ws_b1, (size_t)W * K); ALLOC(ws_p, (size_t)W * K * 3);
+    ALLOC(ws_n, (size_t)W * K * 3); ALLOC(ws_ln, (size_t)W * K); ALLOC(ws_lt, (size_t)W * K * 2);
+    if (hipMemset(s->ws_count, 0, (size_t)W * sizeof(int32_t)) != hipSuccess) {
+      set_error("hipMemset failed (warm start)");
+      ofb_destroy(s);
+      return -4;
+    }
+  }
 #undef ALLOC
   if (hipMemcpy(s->topo, topo.data(), topo.size() * sizeof(int32_t), hipMemcpyHostToDevice) != hipSuccess) {
     set_error("topology upload failed");
@@ -360,10 +370,28 @@ int ofb_step(ofb_solver* s, const ofb_state_in* in, const ofb_state_out* out, co
   hipStream_t st = (hipStream_t)stream;
   int rc = pack_contacts(s, contacts, st);
   if (rc) return rc;
+  // the warm starter's seed reaches the solve only through seed_iterate (engine.py:52-59)
+  const bool ws = s->cfg.warm_start && s->cfg.seed_iterate;
   OFB_CHECK(hipEventRecord(s->ev_solve0, st));
+  if (ws && (rc = warm_start_apply(s, in, dt, st)) != 0) return rc;
   rc = s->path == OFB_PATH_FUSED ? fused_step(s, in, out, ctl, dt, st) : plain_step(s, in, out, ctl, dt, st);
   if (rc) return rc;
+  if (ws && (rc = warm_start_snapshot(s, in, dt, st)) != 0) return rc;
   OFB_CHECK(hipEventRecord(s->ev_solve1, st));
+  return 0;
+}
+
+int ofb_warm_start_reset(ofb_solver* s, void* stream) {
+  if (!s) { set_error("null solver"); return -1; }
+  if (!s->ws_count) return 0;
+  OFB_CHECK(hipMemsetAsync(s->ws_count, 0, (size_t)s->m.W * sizeof(int32_t), (hipStream_t)stream));
+  return 0;
+}
+
+int ofb_get_lambda_seed(const ofb_solver* s, float* seed_out, void* stream) {
+  if (!s || !seed_out) { set_error("null argument"); return -1; }
+  OFB_CHECK(hipMemcpyAsync(seed_out, s->lambda_seed, (size_t)s->m.W * s->L.Nc * 4, hipMemcpyDeviceToDevice,
+                           (hipStream_t)stream));
   return 0;
 }
 

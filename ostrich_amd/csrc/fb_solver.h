@@ -85,6 +85,9 @@ struct ofb_solver {
   // per-step bucketing of worlds by active-row class (fused path)
   int32_t* cls_counts;
   int32_t* cls_lists;
+  // cross-step contact warm start: snapshot of the previous step (fb_warmstart.hip)
+  int32_t *ws_count, *ws_b0, *ws_b1;
+  float *ws_p, *ws_n, *ws_ln, *ws_lt;
   // events around the solve of the last step
   hipEvent_t ev_solve0, ev_solve1;
   int events_ready;
@@ -101,5 +104,7 @@ int plain_create(ofb_solver* s);
 void plain_destroy(ofb_solver* s);
 int plain_step(ofb_solver* s, const ofb_state_in* in, const ofb_state_out* out, const ofb_control* ctl, float dt,
                hipStream_t stream);
+int warm_start_apply(ofb_solver* s, const ofb_state_in* in, float dt, hipStream_t stream);
+int warm_start_snapshot(ofb_solver* s, const ofb_state_in* in, float dt, hipStream_t stream);
 void set_error(const char* fmt, ...);
 }  // namespace ofb

@@ -1,0 +1,251 @@
+// fb_warmstart.hip -- cross-step contact warm start (collision/warm_start.py:84-519,
+// method position_match) for both solver paths.
+//
+// apply (before the solve, one wavefront per world): every current contact looks for the
+// previous step's contact of the same body pair whose start-of-step midpoint, moved by
+// the pair's tangential velocity * dt, lies closest within max(5 mm, 0.1 |v_t| dt); a
+// match copies lambda_n and re-expresses lambda_t in the new tangent basis; unmatched
+// contacts get the cold-start seeds (residual gravity share, impact impulse, sliding
+// friction).  The result is the Newton seed lambda_seed [W*N_c] (reference layout; joint
+// and control rows 0).  snapshot (after the solve): the step's contacts, start-of-step
+// midpoints and lambda.  Both hooks read the start-of-step pose and velocity
+// (body_pose_prev / body_vel_prev, warm_start.py:600-609).
+//
+// The cold-start kernel of the reference takes the force on body 0 as -n lambda, the
+// convention from before its contact packing flipped Newton's normals
+// (core/contacts.py:80-83), while its contact rows push body 0 along +n; it is restated
+// as is, so for physical contacts its gravity and impact terms stay silent.
+
+#include <hip/hip_runtime.h>
+
+#include "fb_math.h"
+#include "fb_solver.h"
+
+namespace ofb {
+
+namespace {
+
+constexpr float kWsAlpha = 0.1f, kWsMinThreshold = 5.0e-3f, kWsNDotUpMin = 0.3f;  // warm_start.py:520-538
+
+struct WsParams {
+  DeviceModel m;
+  RowLayout L;
+  float dt;
+  int cold_gravity, cold_impact;
+  float v_t_threshold;
+  const float* q_prev;   // [W*B*7] start of step
+  const float* qd_prev;  // [W*B*6]
+  const int32_t* ccount;
+  const float* cf;
+  const int32_t* ci;
+  const float* lambda;   // [W*Nc] of the finished step (snapshot)
+  int32_t* prev_count;   // [W]
+  int32_t *prev_b0, *prev_b1;  // [W*K]
+  float *prev_p, *prev_n;      // [W*K*3]
+  float *prev_ln, *prev_lt;    // [W*K], [W*K*2]
+  float* seed;                 // [W*Nc]
+};
+
+__device__ __forceinline__ tf body_pose(const WsParams& P, int w, int b) {
+  if (b < 0) return tident();
+  const float* x = P.q_prev + ((size_t)w * P.m.B + b) * 7;
+  return tf{mk(x[0], x[1], x[2]), qt{x[3], x[4], x[5], x[6]}};
+}
+
+__device__ __forceinline__ v3 cvec(const WsParams& P, int w, int c, int k0) {
+  const int K = P.L.K;
+  const float* f = P.cf + (size_t)w * CF * K + c;
+  return mk(f[k0 * K], f[(k0 + 1) * K], f[(k0 + 2) * K]);
+}
+
+// _world_midpoint (warm_start.py:51-82)
+__device__ __forceinline__ v3 midpoint(const WsParams& P, int w, int c, int b0, int b1, v3 n) {
+  const int K = P.L.K;
+  const float* f = P.cf + (size_t)w * CF * K + c;
+  v3 a = tpoint(body_pose(P, w, b0), cvec(P, w, c, CF_P0X)) - n * f[CF_TH0 * K];
+  v3 b = tpoint(body_pose(P, w, b1), cvec(P, w, c, CF_P1X)) + n * f[CF_TH1 * K];
+  return (a + b) * 0.5f;
+}
+
+__device__ __forceinline__ v3 lin_vel(const WsParams& P, int w, int b) {
+  const float* u = P.qd_prev + ((size_t)w * P.m.B + b) * 6;
+  return mk(u[0], u[1], u[2]);
+}
+
+__device__ __forceinline__ v3 ang_vel(const WsParams& P, int w, int b) {
+  const float* u = P.qd_prev + ((size_t)w * P.m.B + b) * 6;
+  return mk(u[3], u[4], u[5]);
+}
+
+__global__ void __launch_bounds__(64) ws_apply_kernel(WsParams P) {
+  extern __shared__ float ws_lds[];
+  const int w = blockIdx.x, lane = threadIdx.x;
+  const int B = P.m.B, K = P.L.K, Nc = P.L.Nc, SC = P.m.SC;
+  float* support = ws_lds;                                   // [B]
+  int* unmatched = reinterpret_cast<int*>(support + B);      // [B]
+  int* matched = unmatched + B;                              // [K]
+  for (int b = lane; b < B; b += 64) { support[b] = 0.f; unmatched[b] = 0; }
+  for (int i = lane; i < Nc; i += 64) P.seed[(size_t)w * Nc + i] = 0.f;
+  __syncthreads();
+  const v3 g = mk(P.m.gravity[0], P.m.gravity[1], P.m.gravity[2]);
+  const float gmag = sqrtf(dot(g, g));
+  const v3 up = gmag > 1e-6f ? g * (-1.f / gmag) : mk(0.f, 0.f, 1.f);
+  const int on = P.L.off_n, of = P.L.off_f;
+  const int n_c = min(P.ccount[w], K), n_prev = min(P.prev_count[w], K);
+  const float dt = P.dt;
+  float* seed = P.seed + (size_t)w * Nc;
+  // _match_kernel (:84-297)
+  for (int c = lane; c < K; c += 64) {
+    matched[c] = 0;
+    if (c >= n_c) continue;
+    const int s0 = P.ci[(size_t)w * 2 * K + c], s1 = P.ci[(size_t)w * 2 * K + K + c];
+    if (s0 == s1) continue;
+    const int b0 = s0 >= 0 ? P.m.shape_body[w * SC + s0] : -1;
+    const int b1 = s1 >= 0 ? P.m.shape_body[w * SC + s1] : -1;
+    const v3 nc = cvec(P, w, c, CF_NX);
+    const v3 pc = midpoint(P, w, c, b0, b1, nc);
+    v3 vt = mk(0.f, 0.f, 0.f);
+    if (b0 >= 0) vt = vt + lin_vel(P, w, b0);
+    if (b1 >= 0) vt = vt - lin_vel(P, w, b1);
+    vt = vt - nc * dot(vt, nc);
+    const float thr = fmaxf(kWsMinThreshold, kWsAlpha * sqrtf(dot(vt, vt)) * dt);
+    float best_d = thr * thr;
+    int best = -1;
+    for (int j = 0; j < n_prev; ++j) {
+      size_t kj = (size_t)w * K + j;
+      if (P.prev_b0[kj] != b0 || P.prev_b1[kj] != b1) continue;
+      v3 pp = mk(P.prev_p[kj * 3], P.prev_p[kj * 3 + 1], P.prev_p[kj * 3 + 2]) + vt * dt;
+      v3 d = pc - pp;
+      float dsq = dot(d, d);
+      if (dsq < best_d) { best_d = dsq; best = j; }
+    }
+    const float ndu = dot(nc, up);
+    const bool sup0 = -ndu > kWsNDotUpMin, sup1 = ndu > kWsNDotUpMin;
+    if (best < 0) {
+      if (b0 >= 0 && sup0) atomicAdd(&unmatched[b0], 1);
+      if (b1 >= 0 && sup1) atomicAdd(&unmatched[b1], 1);
+      continue;
+    }
+    size_t kb = (size_t)w * K + best;
+    v3 pn = mk(P.prev_n[kb * 3], P.prev_n[kb * 3 + 1], P.prev_n[kb * 3 + 2]);
+    v3 pt1, pt2, ct1, ct2;
+    orthogonal_basis(pn, pt1, pt2);
+    orthogonal_basis(nc, ct1, ct2);
+    v3 ltw = pt1 * P.prev_lt[kb * 2] + pt2 * P.prev_lt[kb * 2 + 1];
+    const float lnm = P.prev_ln[kb];
+    seed[on + c] = lnm;
+    seed[of + 2 * c] = dot(ltw, ct1);
+    seed[of + 2 * c + 1] = dot(ltw, ct2);
+    matched[c] = 1;
+    if (b0 >= 0 && sup0) atomicAdd(&support[b0], -ndu * lnm);
+    if (b1 >= 0 && sup1) atomicAdd(&support[b1], ndu * lnm);
+  }
+  __syncthreads();
+  // _cold_start_kernel (:300-462)
+  for (int c = lane; c < n_c; c += 64) {
+    if (matched[c]) continue;
+    const int s0 = P.ci[(size_t)w * 2 * K + c], s1 = P.ci[(size_t)w * 2 * K + K + c];
+    if (s0 == s1) continue;
+    const int b0 = s0 >= 0 ? P.m.shape_body[w * SC + s0] : -1;
+    const int b1 = s1 >= 0 ? P.m.shape_body[w * SC + s1] : -1;
+    const v3 n = cvec(P, w, c, CF_NX);
+    const v3 p = midpoint(P, w, c, b0, b1, n);
+    v3 v0 = mk(0.f, 0.f, 0.f), v1 = v0;
+    // velocity about the body origin, as the reference computes it (:383-392)
+    if (b0 >= 0) v0 = lin_vel(P, w, b0) + cross(ang_vel(P, w, b0), p - body_pose(P, w, b0).p);
+    if (b1 >= 0) v1 = lin_vel(P, w, b1) + cross(ang_vel(P, w, b1), p - body_pose(P, w, b1).p);
+    const v3 vr = v1 - v0;
+    const float vn = dot(vr, n);
+    const v3 vtt = vr - n * vn;
+    float l_imp = 0.f;
+    if (P.cold_impact && vn < 0.f) {
+      float im = (b0 >= 0 ? P.m.body_inv_mass[w * B + b0] : 0.f) + (b1 >= 0 ? P.m.body_inv_mass[w * B + b1] : 0.f);
+      if (im > 1e-9f) l_imp = (1.f / im) * (-vn) / dt;
+    }
+    const float ndu = dot(n, up);
+    float l_grav = 0.f;
+    if (P.cold_gravity) {
+      if (b0 >= 0 && -ndu > kWsNDotUpMin && unmatched[b0] > 0) {
+        float res = P.m.body_mass[w * B + b0] * gmag - support[b0];
+        if (res > 0.f) l_grav = fmaxf(l_grav, res / ((float)unmatched[b0] * (-ndu)));
+      }
+      if (b1 >= 0 && ndu > kWsNDotUpMin && unmatched[b1] > 0) {
+        float res = P.m.body_mass[w * B + b1] * gmag - support[b1];
+        if (res > 0.f) l_grav = fmaxf(l_grav, res / ((float)unmatched[b1] * ndu));
+      }
+    }
+    const float ln = fmaxf(l_imp, l_grav);
+    if (ln > 0.f) seed[on + c] = ln;
+    if (P.v_t_threshold > 0.f && ln > 0.f) {
+      float vsq = dot(vtt, vtt);
+      if (vsq > P.v_t_threshold * P.v_t_threshold) {
+        v3 th = vtt * (-1.f / sqrtf(vsq));
+        float mu = 0.5f * (P.m.shape_mu[w * SC + s0] + P.m.shape_mu[w * SC + s1]);
+        v3 ft = th * (mu * ln);
+        v3 t1, t2;
+        orthogonal_basis(n, t1, t2);
+        seed[of + 2 * c] = dot(ft, t1);
+        seed[of + 2 * c + 1] = dot(ft, t2);
+      }
+    }
+  }
+}
+
+// _snapshot_kernel (:465-519); one thread per (world, slot)
+__global__ void ws_snapshot_kernel(WsParams P) {
+  const int K = P.L.K, W = P.m.W, SC = P.m.SC;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= W * K) return;
+  const int w = t / K, c = t % K;
+  const int n_c = min(P.ccount[w], K);
+  if (c == 0) P.prev_count[w] = n_c;
+  if (c >= n_c) return;
+  const size_t k = (size_t)w * K + c;
+  const int s0 = P.ci[(size_t)w * 2 * K + c], s1 = P.ci[(size_t)w * 2 * K + K + c];
+  if (s0 == s1) { P.prev_b0[k] = -2; P.prev_b1[k] = -2; return; }  // never matched
+  const int b0 = s0 >= 0 ? P.m.shape_body[w * SC + s0] : -1;
+  const int b1 = s1 >= 0 ? P.m.shape_body[w * SC + s1] : -1;
+  const v3 n = cvec(P, w, c, CF_NX);
+  const v3 mp = midpoint(P, w, c, b0, b1, n);
+  P.prev_b0[k] = b0; P.prev_b1[k] = b1;
+  P.prev_p[k * 3] = mp.x; P.prev_p[k * 3 + 1] = mp.y; P.prev_p[k * 3 + 2] = mp.z;
+  P.prev_n[k * 3] = n.x; P.prev_n[k * 3 + 1] = n.y; P.prev_n[k * 3 + 2] = n.z;
+  const float* lam = P.lambda + (size_t)w * P.L.Nc;
+  P.prev_ln[k] = lam[P.L.off_n + c];
+  P.prev_lt[k * 2] = lam[P.L.off_f + 2 * c];
+  P.prev_lt[k * 2 + 1] = lam[P.L.off_f + 2 * c + 1];
+}
+
+WsParams ws_params(ofb_solver* s, const ofb_state_in* in, float dt) {
+  WsParams P;
+  P.m = s->m; P.L = s->L; P.dt = dt;
+  P.cold_gravity = s->cfg.warm_start_cold_gravity; P.cold_impact = s->cfg.warm_start_cold_impact;
+  P.v_t_threshold = s->cfg.warm_start_cold_friction_v;
+  P.q_prev = in->body_q; P.qd_prev = in->body_qd;
+  P.ccount = s->ccount; P.cf = s->cf; P.ci = s->ci; P.lambda = s->lambda;
+  P.prev_count = s->ws_count; P.prev_b0 = s->ws_b0; P.prev_b1 = s->ws_b1; P.prev_p = s->ws_p; P.prev_n = s->ws_n;
+  P.prev_ln = s->ws_ln; P.prev_lt = s->ws_lt; P.seed = s->lambda_seed;
+  return P;
+}
+
+}  // namespace
+
+int warm_start_apply(ofb_solver* s, const ofb_state_in* in, float dt, hipStream_t stream) {
+  WsParams P = ws_params(s, in, dt);
+  size_t lds = (size_t)(2 * s->m.B + s->L.K) * 4;
+  hipLaunchKernelGGL(ws_apply_kernel, dim3(s->m.W), dim3(64), lds, stream, P);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) { set_error("warm start launch failed: %s", hipGetErrorString(e)); return -4; }
+  return 0;
+}
+
+int warm_start_snapshot(ofb_solver* s, const ofb_state_in* in, float dt, hipStream_t stream) {
+  WsParams P = ws_params(s, in, dt);
+  const int n = s->m.W * s->L.K;
+  hipLaunchKernelGGL(ws_snapshot_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, P);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) { set_error("warm start snapshot failed: %s", hipGetErrorString(e)); return -4; }
+  return 0;
+}
+
+}  // namespace ofb

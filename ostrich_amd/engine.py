@@ -113,6 +113,16 @@ class OstrichEngine:
         _lib.check(self.lib.ofb_get_lambda(self._h, _ptr(lam), self._stream()), "ofb_get_lambda")
         return lam
 
+    def reset_warm_start(self):
+        """Forgets the warm starter's snapshot (next step cold-starts every contact)."""
+        _lib.check(self.lib.ofb_warm_start_reset(self._h, self._stream()), "ofb_warm_start_reset")
+
+    def lambda_seed(self):
+        """The Newton seed [W, N_c] the last step started from."""
+        out = torch.zeros(self.num_worlds, self.num_constraints, dtype=torch.float32, device=self.device)
+        _lib.check(self.lib.ofb_get_lambda_seed(self._h, _ptr(out), self._stream()), "ofb_get_lambda_seed")
+        return out
+
     def set_lambda_seed(self, seed):
         seed = seed.to(self.device, torch.float32).contiguous()
         _lib.check(self.lib.ofb_set_lambda_seed(self._h, _ptr(seed), self._stream()), "ofb_set_lambda_seed")
