@@ -265,6 +265,24 @@ int ofb_warm_start_reset(ofb_solver* solver, void* stream);
  * seed set with ofb_set_lambda_seed) - _constr_force after engine.py:55-59. */
 int ofb_get_lambda_seed(const ofb_solver* solver, float* seed_out, void* stream);
 
+/* ---- per-contact warm-start network (no counterpart in the reference) ----------------
+ * 12 -> 64 -> 64 -> 3 ReLU MLP in BF16 with FP32 accumulation on MFMA (fb_mlp.hip).  x is
+ * [n*12] features, y and target [n*3] (device pointers); parameters are [num_params] FP32
+ * master weights (W1 16x64, b1, W2 64x64, b2, W3 64x16, b3; padding rows/columns zero). */
+typedef struct ofb_mlp ofb_mlp;
+int ofb_mlp_create(ofb_mlp** out);
+int ofb_mlp_destroy(ofb_mlp* net);
+int ofb_mlp_num_params(void);
+int ofb_mlp_set_params(ofb_mlp* net, const float* params, void* stream);
+int ofb_mlp_get_params(const ofb_mlp* net, float* params, void* stream);
+int ofb_mlp_forward(const ofb_mlp* net, const float* x, int n, float* y, void* stream);
+/* gradient of mean_rows sum_c (y_c - target_c)^2 w.r.t. the parameters; loss may be NULL */
+int ofb_mlp_gradient(ofb_mlp* net, const float* x, const float* target, int n, float* grad, float* loss,
+                     void* stream);
+/* one Adam step (beta 0.9 / 0.999, eps 1e-8) on that loss */
+int ofb_mlp_train_step(ofb_mlp* net, const float* x, const float* target, int n, float lr, float* loss,
+                       void* stream);
+
 /* Copies the packed per-world contact set of the last step: count [W] and, for each of
  * the W*C_max slots, shape0/shape1 (world-local columns), point0/point1/normal (after the
  * normal flip, core/contacts.py:80-83) and thickness0/thickness1. Any pointer may be NULL. */

@@ -12,7 +12,8 @@ EXPORTS = (
     "ofb_create", "ofb_destroy", "ofb_dims", "ofb_step", "ofb_get_stats", "ofb_get_lambda",
     "ofb_get_packed_contacts", "ofb_set_lambda_seed", "ofb_last_solve_ms", "ofb_last_error",
     "ofb_get_lambda_prev", "ofb_set_shape_mu", "ofb_step_backward", "ofb_warm_start_reset",
-    "ofb_get_lambda_seed",
+    "ofb_get_lambda_seed", "ofb_mlp_create", "ofb_mlp_destroy", "ofb_mlp_num_params", "ofb_mlp_set_params",
+    "ofb_mlp_get_params", "ofb_mlp_forward", "ofb_mlp_gradient", "ofb_mlp_train_step",
 )
 
 _lib = None
@@ -41,6 +42,14 @@ def load():
     lib.ofb_step_backward.argtypes = [vp] * 9 + [C.c_float, vp]
     lib.ofb_warm_start_reset.argtypes = [vp, vp]
     lib.ofb_get_lambda_seed.argtypes = [vp, vp, vp]
+    lib.ofb_mlp_create.argtypes = [C.POINTER(vp)]
+    lib.ofb_mlp_destroy.argtypes = [vp]
+    lib.ofb_mlp_num_params.argtypes = []
+    lib.ofb_mlp_set_params.argtypes = [vp, vp, vp]
+    lib.ofb_mlp_get_params.argtypes = [vp, vp, vp]
+    lib.ofb_mlp_forward.argtypes = [vp, vp, C.c_int, vp, vp]
+    lib.ofb_mlp_gradient.argtypes = [vp, vp, vp, C.c_int, vp, vp, vp]
+    lib.ofb_mlp_train_step.argtypes = [vp, vp, vp, C.c_int, C.c_float, vp, vp]
     for name in EXPORTS:
         if name != "ofb_last_error":
             getattr(lib, name).restype = C.c_int
