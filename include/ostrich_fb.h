@@ -172,6 +172,7 @@ typedef struct ofb_contacts {
 
 /* Opaque solver handle (engine state + scratch), one per OstrichEngine instance. */
 typedef struct ofb_solver ofb_solver;
+typedef struct ofb_mlp ofb_mlp;
 
 /* Replaces OstrichEngineBase.__init__ (core/base_engine.py:112-237). Copies the model
  * into solver-owned device memory and allocates every scratch buffer. `path` selects
@@ -269,7 +270,6 @@ int ofb_get_lambda_seed(const ofb_solver* solver, float* seed_out, void* stream)
  * 12 -> 64 -> 64 -> 3 ReLU MLP in BF16 with FP32 accumulation on MFMA (fb_mlp.hip).  x is
  * [n*12] features, y and target [n*3] (device pointers); parameters are [num_params] FP32
  * master weights (W1 16x64, b1, W2 64x64, b2, W3 64x16, b3; padding rows/columns zero). */
-typedef struct ofb_mlp ofb_mlp;
 int ofb_mlp_create(ofb_mlp** out);
 int ofb_mlp_destroy(ofb_mlp* net);
 int ofb_mlp_num_params(void);
@@ -282,6 +282,17 @@ int ofb_mlp_gradient(ofb_mlp* net, const float* x, const float* target, int n, f
 /* one Adam step (beta 0.9 / 0.999, eps 1e-8) on that loss */
 int ofb_mlp_train_step(ofb_mlp* net, const float* x, const float* target, int n, float lr, float* loss,
                        void* stream);
+
+/* Attaches (net != NULL) or detaches the learned warm start (no counterpart in the
+ * reference): with warm_start and seed_iterate on, each step the network predicts the
+ * contact impulses from the warm starter's features and every world starts Newton from
+ * whichever of the two seeds (network, previous step) has the lower initial residual
+ * (stats flag bit 5: the network's).  The network is not owned.  Fused path only. */
+int ofb_set_warm_start_net(ofb_solver* solver, const ofb_mlp* net);
+
+/* Copies the learned warm start's per-contact features [W*C_max*12] of the last step and
+ * their impulse scale m_eff |g| [W*C_max] (dataset logging).  Needs warm_start. */
+int ofb_get_contact_features(const ofb_solver* solver, float* features, float* scale, void* stream);
 
 /* Copies the packed per-world contact set of the last step: count [W] and, for each of
  * the W*C_max slots, shape0/shape1 (world-local columns), point0/point1/normal (after the

@@ -201,7 +201,7 @@ int ofb_destroy(ofb_solver* s) {
                   m.shape_v_stribeck, m.shape_stribeck_lateral_only, m.shape_world, s->ccount, s->cf, s->ci,
                   s->pk_keys, s->pk_keys_out, s->pk_vals, s->pk_vals_out, s->pk_seg_start, s->pk_temp, s->topo,
                   s->lambda, s->lambda_prev, s->lambda_seed, s->ws_count, s->ws_b0, s->ws_b1, s->ws_p, s->ws_n,
-                  s->ws_ln, s->ws_lt, s->st_nr, s->st_pcr, s->st_best, s->st_flags, s->st_merit};
+                  s->ws_ln, s->ws_lt, s->ws_feat, s->ws_fscale, s->ws_y, s->lambda_seed2, s->st_nr, s->st_pcr, s->st_best, s->st_flags, s->st_merit};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (s->plain) plain_destroy(s);
@@ -312,6 +312,8 @@ int ofb_create(const ofb_model* model, const ofb_config* config, int32_t path, o
   if (config->warm_start) {
     ALLOC(ws_count, W); ALLOC(ws_b0, (size_t)W * K); ALLOC(ws_b1, (size_t)W * K); ALLOC(ws_p, (size_t)W * K * 3);
     ALLOC(ws_n, (size_t)W * K * 3); ALLOC(ws_ln, (size_t)W * K); ALLOC(ws_lt, (size_t)W * K * 2);
+    ALLOC(ws_feat, (size_t)W * K * 12); ALLOC(ws_fscale, (size_t)W * K); ALLOC(ws_y, (size_t)W * K * 3);
+    ALLOC(lambda_seed2, (size_t)W * L.Nc);
     if (hipMemset(s->ws_count, 0, (size_t)W * sizeof(int32_t)) != hipSuccess) {
       set_error("hipMemset failed (warm start)");
       ofb_destroy(s);
@@ -374,6 +376,10 @@ int ofb_step(ofb_solver* s, const ofb_state_in* in, const ofb_state_out* out, co
   const bool ws = s->cfg.warm_start && s->cfg.seed_iterate;
   OFB_CHECK(hipEventRecord(s->ev_solve0, st));
   if (ws && (rc = warm_start_apply(s, in, dt, st)) != 0) return rc;
+  if (ws && s->ws_net && s->path == OFB_PATH_FUSED) {
+    if ((rc = ofb_mlp_forward(s->ws_net, s->ws_feat, s->m.W * s->L.K, s->ws_y, st)) != 0) return rc;
+    if ((rc = warm_start_learned_seed(s, s->ws_y, st)) != 0) return rc;
+  }
   rc = s->path == OFB_PATH_FUSED ? fused_step(s, in, out, ctl, dt, st) : plain_step(s, in, out, ctl, dt, st);
   if (rc) return rc;
   if (ws && (rc = warm_start_snapshot(s, in, dt, st)) != 0) return rc;
@@ -385,6 +391,24 @@ int ofb_warm_start_reset(ofb_solver* s, void* stream) {
   if (!s) { set_error("null solver"); return -1; }
   if (!s->ws_count) return 0;
   OFB_CHECK(hipMemsetAsync(s->ws_count, 0, (size_t)s->m.W * sizeof(int32_t), (hipStream_t)stream));
+  return 0;
+}
+
+int ofb_set_warm_start_net(ofb_solver* s, const ofb_mlp* net) {
+  if (!s) { set_error("null solver"); return -1; }
+  if (net && !s->cfg.warm_start) { set_error("the learned warm start needs warm_start enabled"); return -2; }
+  if (net && s->path != OFB_PATH_FUSED) { set_error("the learned warm start runs on the fused path"); return -2; }
+  s->ws_net = net;
+  return 0;
+}
+
+int ofb_get_contact_features(const ofb_solver* s, float* features, float* scale, void* stream) {
+  if (!s) { set_error("null solver"); return -1; }
+  if (!s->ws_feat) { set_error("contact features need warm_start enabled"); return -2; }
+  hipStream_t st = (hipStream_t)stream;
+  const size_t n = (size_t)s->m.W * s->L.K;
+  if (features) OFB_CHECK(hipMemcpyAsync(features, s->ws_feat, n * 12 * 4, hipMemcpyDeviceToDevice, st));
+  if (scale) OFB_CHECK(hipMemcpyAsync(scale, s->ws_fscale, n * 4, hipMemcpyDeviceToDevice, st));
   return 0;
 }
 

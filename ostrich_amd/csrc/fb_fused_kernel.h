@@ -64,6 +64,7 @@ struct FusedParams {
   const int32_t* ci;
   const int32_t* topo;  // [Nj] joint of each joint row, [Nj] row within joint, [Nctrl] joint of each control row
   const float* lambda_seed;
+  const float* lambda_seed2;  // learned seed (null: none); the lower initial residual wins
   float* lambda_out;
   float* lambda_prev_out;  // lambda_prev_iter as the step leaves it (input of the adjoint)
   int32_t *st_nr, *st_pcr, *st_best, *st_flags;
@@ -371,6 +372,36 @@ __device__ __forceinline__ void schur_matvec(float* sm, int lane, int B, const b
   wsync();
 }
 
+// ||[h_d | h_c]||^2 at the step-start state with lambda = l (and lambda_prev = l, as the
+// first Newton iteration snapshots it): the gate between two Newton seeds.
+template <int R>
+__device__ __forceinline__ float initial_merit(const FusedParams& P, float* sm, float* lamprev, int lane, int w,
+                                               int n_r, int n_c, const float (&l)[R]) {
+  const int B = P.m.B;
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    int i = lane + WAVE * k;
+    if (i < n_r) lamprev[i] = l[k];
+  }
+  dynamics_bodies(P, sm, lane, false);
+  wsync();
+  float s = 0.f;
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    int i = lane + WAVE * k;
+    if (i < n_r) {
+      RowOut o = assemble_row(P, sm, lamprev, w, i, n_c, l[k]);
+      s += o.hc * o.hc;
+    }
+  }
+  wsync();
+  for (int e = lane; e < 6 * B; e += WAVE) {
+    float v = sm[(e / 6) * BS + BHD + (e % 6)];
+    s += v * v;
+  }
+  return wave_sum(s);
+}
+
 // PCR on A x = rhs from x0 = 0 (optim/pcr_solver.py:170-295): the seven vectors live in
 // VGPRs, dots are DPP wave reductions, A is the LDS matrix-free operator.  Returns the
 // iteration count.
@@ -485,6 +516,25 @@ fused_step_kernel(FusedParams P) {
     act[k] = false; b0[k] = -1; b1[k] = -1;
   }
   wsync();
+  // learned warm start: keep whichever seed starts from the lower residual, so the
+  // network never replaces a better previous-step seed
+  int seed_flag = 0;
+  if (P.cfg.seed_iterate && P.lambda_seed2) {
+    float alt[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      int i = lane + WAVE * k;
+      alt[k] = i < n_r ? P.lambda_seed2[(size_t)w * L.Nc + original_row(L, i, n_c)] : 0.f;
+    }
+    const float ma = initial_merit<R>(P, sm, lamprev, lane, w, n_r, n_c, lam);
+    const float mb = initial_merit<R>(P, sm, lamprev, lane, w, n_r, n_c, alt);
+    if (mb < ma) {
+      seed_flag = FLAG_LEARNED_SEED;
+#pragma unroll
+      for (int k = 0; k < R; ++k) { lam[k] = alt[k]; lamb[k] = alt[k]; }
+    }
+    wsync();
+  }
 
   const float atol_sq = P.cfg.nr_atol * P.cfg.nr_atol;
   const float tol_sq = P.cfg.lin_tol >= 0.f ? P.cfg.lin_tol * P.cfg.lin_tol : -1.f;
@@ -686,7 +736,7 @@ fused_step_kernel(FusedParams P) {
     P.st_pcr[w] = pcr_total;
     P.st_merit[w] = picked_merit;
     P.st_best[w] = picked;
-    int fl = (P.st_flags[w] & FLAG_CONTACT_OVERFLOW) | bj_any;
+    int fl = (P.st_flags[w] & FLAG_CONTACT_OVERFLOW) | bj_any | seed_flag;
     if (n_it >= P.cfg.nr_max_iters && merit > atol_sq) fl |= FLAG_NR_CAP;
     P.st_flags[w] = fl;
   }

@@ -26,6 +26,7 @@ enum {
   FLAG_FACTOR_FAILURE = 4,
   FLAG_PAIR_OVERFLOW = 8,
   FLAG_MU_GRAD_MISSING = 16,  // adjoint: a friction row's mu derivative has no closed form
+  FLAG_LEARNED_SEED = 32,     // the step started from the learned seed (lower initial residual)
 };
 
 constexpr int kMaxJoints = 64;      // joints per world supported by the fused path
@@ -88,6 +89,9 @@ struct ofb_solver {
   // cross-step contact warm start: snapshot of the previous step (fb_warmstart.hip)
   int32_t *ws_count, *ws_b0, *ws_b1;
   float *ws_p, *ws_n, *ws_ln, *ws_lt;
+  float *ws_feat, *ws_fscale, *ws_y;  // learned warm start: features [W*K*12], scale [W*K], output [W*K*3]
+  float* lambda_seed2;                // [W*Nc] learned seed
+  const struct ofb_mlp* ws_net;       // attached network (not owned)
   // events around the solve of the last step
   hipEvent_t ev_solve0, ev_solve1;
   int events_ready;
@@ -106,5 +110,6 @@ int plain_step(ofb_solver* s, const ofb_state_in* in, const ofb_state_out* out, 
                hipStream_t stream);
 int warm_start_apply(ofb_solver* s, const ofb_state_in* in, float dt, hipStream_t stream);
 int warm_start_snapshot(ofb_solver* s, const ofb_state_in* in, float dt, hipStream_t stream);
+int warm_start_learned_seed(ofb_solver* s, const float* y, hipStream_t stream);
 void set_error(const char* fmt, ...);
 }  // namespace ofb

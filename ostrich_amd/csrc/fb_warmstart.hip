@@ -44,6 +44,8 @@ struct WsParams {
   float *prev_p, *prev_n;      // [W*K*3]
   float *prev_ln, *prev_lt;    // [W*K], [W*K*2]
   float* seed;                 // [W*Nc]
+  float* feat;                 // [W*K*12] learned warm-start features (may be null)
+  float* fscale;               // [W*K] m_eff |g| of each contact
 };
 
 __device__ __forceinline__ tf body_pose(const WsParams& P, int w, int b) {
@@ -77,6 +79,44 @@ __device__ __forceinline__ v3 ang_vel(const WsParams& P, int w, int b) {
   return mk(u[3], u[4], u[5]);
 }
 
+// Features of contact c for the learned warm start (ostrich_amd/learned.py): gap,
+// normal, tangential relative velocity in the contact basis, the matched previous
+// impulse, mu, dt and the mass ratio, made dimensionless with |g|, dt and m_eff.
+__device__ void write_features(const WsParams& P, int w, int c, int b0, int b1, int s0, int s1, v3 n, v3 t1, v3 t2,
+                               float gmag, float lnm, float lt1, float lt2) {
+  const int K = P.L.K, B = P.m.B, SC = P.m.SC;
+  const float* f = P.cf + (size_t)w * CF * K + c;
+  const tf q0 = body_pose(P, w, b0), q1 = body_pose(P, w, b1);
+  const v3 p0 = tpoint(q0, cvec(P, w, c, CF_P0X)) - n * f[CF_TH0 * K];
+  const v3 p1 = tpoint(q1, cvec(P, w, c, CF_P1X)) + n * f[CF_TH1 * K];
+  const float gap = dot(n, p0 - p1);
+  v3 v0 = mk(0.f, 0.f, 0.f), v1 = v0;
+  if (b0 >= 0) {
+    const float* cm = P.m.body_com + ((size_t)w * B + b0) * 3;
+    v0 = lin_vel(P, w, b0) + cross(ang_vel(P, w, b0), p0 - tpoint(q0, mk(cm[0], cm[1], cm[2])));
+  }
+  if (b1 >= 0) {
+    const float* cm = P.m.body_com + ((size_t)w * B + b1) * 3;
+    v1 = lin_vel(P, w, b1) + cross(ang_vel(P, w, b1), p1 - tpoint(q1, mk(cm[0], cm[1], cm[2])));
+  }
+  const v3 vr = v0 - v1;
+  const float im0 = b0 >= 0 ? P.m.body_inv_mass[w * B + b0] : 0.f;
+  const float im1 = b1 >= 0 ? P.m.body_inv_mass[w * B + b1] : 0.f;
+  const float meff = (im0 + im1) > 0.f ? 1.f / (im0 + im1) : 1.f;
+  const float g = gmag > 0.f ? gmag : 1.f, dt = P.dt;
+  const float sc = meff * g;
+  float* o = P.feat + ((size_t)w * K + c) * 12;
+  o[0] = gap / (g * dt * dt);
+  o[1] = n.x; o[2] = n.y; o[3] = n.z;
+  o[4] = dot(vr, t1) / (g * dt);
+  o[5] = dot(vr, t2) / (g * dt);
+  o[6] = lnm / sc; o[7] = lt1 / sc; o[8] = lt2 / sc;
+  o[9] = 0.5f * (P.m.shape_mu[w * SC + s0] + P.m.shape_mu[w * SC + s1]);
+  o[10] = dt / 0.05f;
+  o[11] = (im0 + im1) > 0.f ? im1 / (im0 + im1) : 0.5f;
+  P.fscale[(size_t)w * K + c] = sc;
+}
+
 __global__ void __launch_bounds__(64) ws_apply_kernel(WsParams P) {
   extern __shared__ float ws_lds[];
   const int w = blockIdx.x, lane = threadIdx.x;
@@ -86,6 +126,10 @@ __global__ void __launch_bounds__(64) ws_apply_kernel(WsParams P) {
   int* matched = unmatched + B;                              // [K]
   for (int b = lane; b < B; b += 64) { support[b] = 0.f; unmatched[b] = 0; }
   for (int i = lane; i < Nc; i += 64) P.seed[(size_t)w * Nc + i] = 0.f;
+  if (P.feat)
+    for (int e = lane; e < K * 12; e += 64) P.feat[(size_t)w * K * 12 + e] = 0.f;
+  if (P.fscale)
+    for (int e = lane; e < K; e += 64) P.fscale[(size_t)w * K + e] = 0.f;
   __syncthreads();
   const v3 g = mk(P.m.gravity[0], P.m.gravity[1], P.m.gravity[2]);
   const float gmag = sqrtf(dot(g, g));
@@ -121,21 +165,28 @@ __global__ void __launch_bounds__(64) ws_apply_kernel(WsParams P) {
     }
     const float ndu = dot(nc, up);
     const bool sup0 = -ndu > kWsNDotUpMin, sup1 = ndu > kWsNDotUpMin;
+    v3 ct1, ct2;
+    orthogonal_basis(nc, ct1, ct2);
+    float lnm = 0.f, lt1 = 0.f, lt2 = 0.f;
+    if (best >= 0) {
+      size_t kb = (size_t)w * K + best;
+      v3 pn = mk(P.prev_n[kb * 3], P.prev_n[kb * 3 + 1], P.prev_n[kb * 3 + 2]);
+      v3 pt1, pt2;
+      orthogonal_basis(pn, pt1, pt2);
+      v3 ltw = pt1 * P.prev_lt[kb * 2] + pt2 * P.prev_lt[kb * 2 + 1];
+      lnm = P.prev_ln[kb];
+      lt1 = dot(ltw, ct1);
+      lt2 = dot(ltw, ct2);
+    }
+    if (P.feat) write_features(P, w, c, b0, b1, s0, s1, nc, ct1, ct2, gmag, lnm, lt1, lt2);
     if (best < 0) {
       if (b0 >= 0 && sup0) atomicAdd(&unmatched[b0], 1);
       if (b1 >= 0 && sup1) atomicAdd(&unmatched[b1], 1);
       continue;
     }
-    size_t kb = (size_t)w * K + best;
-    v3 pn = mk(P.prev_n[kb * 3], P.prev_n[kb * 3 + 1], P.prev_n[kb * 3 + 2]);
-    v3 pt1, pt2, ct1, ct2;
-    orthogonal_basis(pn, pt1, pt2);
-    orthogonal_basis(nc, ct1, ct2);
-    v3 ltw = pt1 * P.prev_lt[kb * 2] + pt2 * P.prev_lt[kb * 2 + 1];
-    const float lnm = P.prev_ln[kb];
     seed[on + c] = lnm;
-    seed[of + 2 * c] = dot(ltw, ct1);
-    seed[of + 2 * c + 1] = dot(ltw, ct2);
+    seed[of + 2 * c] = lt1;
+    seed[of + 2 * c + 1] = lt2;
     matched[c] = 1;
     if (b0 >= 0 && sup0) atomicAdd(&support[b0], -ndu * lnm);
     if (b1 >= 0 && sup1) atomicAdd(&support[b1], ndu * lnm);
@@ -191,6 +242,26 @@ __global__ void __launch_bounds__(64) ws_apply_kernel(WsParams P) {
   }
 }
 
+// learned seed: contact rows from the network's (n, t1, t2) outputs times m_eff |g|,
+// joint and control rows 0 (as the warm starter leaves them)
+__global__ void ws_learned_seed_kernel(int W, int K, RowLayout L, const int32_t* __restrict__ ccount,
+                                       const float* __restrict__ y, const float* __restrict__ fscale,
+                                       float* __restrict__ seed2) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= W * L.Nc) return;
+  const int w = t / L.Nc, i = t % L.Nc;
+  const int n_c = min(ccount[w], K);
+  float v = 0.f;
+  int c = -1, comp = 0;
+  if (i >= L.off_n && i < L.off_n + K) { c = i - L.off_n; comp = 0; }
+  else if (i >= L.off_f && i < L.off_f + 2 * K) { c = (i - L.off_f) >> 1; comp = 1 + ((i - L.off_f) & 1); }
+  if (c >= 0 && c < n_c) {
+    const size_t k = (size_t)w * K + c;
+    v = y[k * 3 + comp] * fscale[k];
+  }
+  seed2[t] = v;
+}
+
 // _snapshot_kernel (:465-519); one thread per (world, slot)
 __global__ void ws_snapshot_kernel(WsParams P) {
   const int K = P.L.K, W = P.m.W, SC = P.m.SC;
@@ -225,6 +296,7 @@ WsParams ws_params(ofb_solver* s, const ofb_state_in* in, float dt) {
   P.ccount = s->ccount; P.cf = s->cf; P.ci = s->ci; P.lambda = s->lambda;
   P.prev_count = s->ws_count; P.prev_b0 = s->ws_b0; P.prev_b1 = s->ws_b1; P.prev_p = s->ws_p; P.prev_n = s->ws_n;
   P.prev_ln = s->ws_ln; P.prev_lt = s->ws_lt; P.seed = s->lambda_seed;
+  P.feat = s->ws_feat; P.fscale = s->ws_fscale;
   return P;
 }
 
@@ -236,6 +308,15 @@ int warm_start_apply(ofb_solver* s, const ofb_state_in* in, float dt, hipStream_
   hipLaunchKernelGGL(ws_apply_kernel, dim3(s->m.W), dim3(64), lds, stream, P);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) { set_error("warm start launch failed: %s", hipGetErrorString(e)); return -4; }
+  return 0;
+}
+
+int warm_start_learned_seed(ofb_solver* s, const float* y, hipStream_t stream) {
+  const int n = s->m.W * s->L.Nc;
+  hipLaunchKernelGGL(ws_learned_seed_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, s->m.W, s->L.K, s->L,
+                     s->ccount, y, s->ws_fscale, s->lambda_seed2);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) { set_error("learned seed launch failed: %s", hipGetErrorString(e)); return -4; }
   return 0;
 }
 

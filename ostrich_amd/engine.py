@@ -113,6 +113,23 @@ class OstrichEngine:
         _lib.check(self.lib.ofb_get_lambda(self._h, _ptr(lam), self._stream()), "ofb_get_lambda")
         return lam
 
+    def set_warm_start_net(self, net):
+        """Attaches a learned warm start (`ostrich_amd.learned.WarmStartNet`) or detaches it
+        (None).  Acts with warm_start.enabled and warm_start.seed_iterate."""
+        _lib.check(self.lib.ofb_set_warm_start_net(self._h, net._h if net is not None else None),
+                   "ofb_set_warm_start_net")
+        self._net_keepalive = net
+
+    def contact_features(self):
+        """Learned-warm-start features [W, C_max, 12] of the last step and the impulse scale
+        m_eff |g| [W, C_max]."""
+        W, K = self.num_worlds, self.config.contacts.max_per_world
+        f = torch.zeros(W, K, 12, device=self.device)
+        sc = torch.zeros(W, K, device=self.device)
+        _lib.check(self.lib.ofb_get_contact_features(self._h, _ptr(f), _ptr(sc), self._stream()),
+                   "ofb_get_contact_features")
+        return f, sc
+
     def reset_warm_start(self):
         """Forgets the warm starter's snapshot (next step cold-starts every contact)."""
         _lib.check(self.lib.ofb_warm_start_reset(self._h, self._stream()), "ofb_warm_start_reset")
