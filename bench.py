@@ -3,7 +3,10 @@
 
 Workload: the uneven-batch skid-steer scene - 4096 worlds, each a 10 kg chassis with three
 velocity-driven wheels, 0-24 static boxes, mu U(0.4,1.0), dt 0.05 s, Newton <= 20
-iterations to ||h|| <= 1e-5, PCR <= 16 iterations (Jacobi).  A step is one full solver
+iterations to ||h|| <= 1e-5, PCR <= 16 iterations (Jacobi).  The headline variant seeds
+Newton with the cross-step contact warm starter ("fused + warm start"); the other
+variants (plain per-kernel path, cold start, block-Jacobi, learned warm start) are
+reported beside it.  A step is one full solver
 step (contact packing + Newton loop + backtracking) over all worlds; contacts are
 generated between steps by the primitive collider on the GPU and are not timed, as the
 reference times collision separately (profiling/engine_profiler.py:46-53).
@@ -25,7 +28,7 @@ sys.path.insert(0, ROOT)
 
 from ostrich_amd import collide, scenes  # noqa: E402
 from ostrich_amd.config import (ContactsConfig, ExecutionConfig, LinearSolverConfig,  # noqa: E402
-                                NewtonRaphsonConfig, OstrichEngineConfig)
+                                NewtonRaphsonConfig, OstrichEngineConfig, WarmStartConfig)
 from ostrich_amd.containers import Contacts, Control, State  # noqa: E402
 
 DT = 0.05
@@ -33,11 +36,15 @@ K_CONTACTS = 128
 HBM_PEAK_GBS = 8000.0
 
 
-def solver_config(path, termination="per_world", precond="jacobi"):
+def solver_config(path, termination="per_world", precond="jacobi", seed="cold"):
+    """seed: "cold" (lambda = 0, the reference default), "prev" (the cross-step contact warm
+    starter seeds Newton: warm_start.seed_iterate), "learned" (prev + the learned
+    per-contact network, gated per world)."""
     return OstrichEngineConfig(
         nr=NewtonRaphsonConfig(max_iters=20, atol=1e-5),
         linear=LinearSolverConfig(max_iters=16, preconditioner_type=precond),
         contacts=ContactsConfig(max_per_world=K_CONTACTS),
+        warm_start=WarmStartConfig(enabled=True, seed_iterate=seed in ("prev", "learned")),
         execution=ExecutionConfig(path=path, termination=termination),
     )
 
@@ -60,17 +67,27 @@ def operand_bytes_per_world(n_rows_active, nr_iters, pcr_iters):
     return n_rows_active * (400.0 * nr_iters + 304.0 * pcr_iters)
 
 
+NET_PATH = os.path.join(ROOT, "ostrich_amd", "data", "warm_start_net.npy")
+
+
 def run_path(variant, model, state, control, steps, warmup, device, collider):
     from ostrich_amd import OstrichEngine
-    path, precond = variant.split(":")
-    cfg = solver_config(path, precond=precond)
+    parts = variant.split(":")
+    path, precond = parts[0], parts[1]
+    seed = parts[2] if len(parts) > 2 else "cold"
+    cfg = solver_config(path, precond=precond, seed=seed)
     eng = OstrichEngine(model, config=cfg, device=device)
+    if seed == "learned":
+        from ostrich_amd.learned import WarmStartNet
+        if not os.path.exists(NET_PATH):
+            raise RuntimeError(f"{NET_PATH} missing: train it with python -m ostrich_amd.train_warm_start")
+        eng.set_warm_start_net(WarmStartNet(device, params=np.load(NET_PATH)))
     s_a = State.from_numpy(state, device)
     s_b = s_a.clone()
     ctl = Control.from_numpy(control, device)
     stream = torch.cuda.current_stream(device)
     W = model.num_worlds
-    step_ms, solve_ms, nr, pcr, conv, ncon = [], [], [], [], [], []
+    step_ms, solve_ms, nr, pcr, conv, ncon, lrn = [], [], [], [], [], [], []
     for k in range(warmup + steps):
         con = Contacts.from_dict(collider.collide(s_a.body_q), device)
         e0 = torch.cuda.Event(enable_timing=True)
@@ -87,13 +104,15 @@ def run_path(variant, model, state, control, steps, warmup, device, collider):
             pcr.append(st["pcr_iters"].float().mean().item())
             conv.append((st["merit"] <= cfg.nr.atol ** 2).float().mean().item())
             ncon.append(eng.packed_contacts()["count"].clamp(max=K_CONTACTS).float().mean().item())
+            lrn.append(((st["flags"] & 32) != 0).float().mean().item())
         s_a, s_b = s_b, s_a
     torch.cuda.synchronize()
     return dict(step_ms=float(np.sum(step_ms)), solve_ms=float(np.sum(solve_ms)), steps=steps,
                 world_steps_per_s=W * steps / (np.sum(step_ms) / 1e3),
                 ms_per_step=float(np.mean(step_ms)), kernel_ms=float(np.mean(solve_ms)),
                 nr_mean=float(np.mean(nr)), pcr_per_step_mean=float(np.mean(pcr)),
-                converged_fraction=float(np.mean(conv)), contacts_mean=float(np.mean(ncon)))
+                converged_fraction=float(np.mean(conv)), contacts_mean=float(np.mean(ncon)),
+                learned_seed_fraction=float(np.mean(lrn)))
 
 
 def adjoint_leg(device, worlds=1024, steps=50, checkpoint_every=10, seed=3):
@@ -142,26 +161,32 @@ def adjoint_leg(device, worlds=1024, steps=50, checkpoint_every=10, seed=3):
                 note="backward includes re-simulating each 10-step window from its checkpoint")
 
 
-def cpu_baseline(model_fn, sample_worlds, sample_steps, precond="jacobi"):
-    """The oracle (FP32 CPU restatement, one thread) on the first `sample_worlds` worlds
-    of the same seeded scene; only the solver calls are timed."""
+def cpu_baseline(model_fn, sample_worlds, sample_steps, precond="jacobi", seed="prev"):
+    """The oracle (FP32 CPU restatement of the solve, one thread; FP64 warm starter) on the
+    first `sample_worlds` worlds of the same seeded scene, with the same Newton seeding as
+    the headline; warm start + solver calls are timed, collision is not."""
     from oracle import oracle
     model, state, control = model_fn(sample_worlds)
-    cfg = solver_config("fused", precond=precond)
+    cfg = solver_config("fused", precond=precond, seed=seed)
     col = collide.PrimitiveCollider(model, device="cpu", gap=0.02, max_contacts_per_world=K_CONTACTS)
     hm = oracle.HostModel(model)
+    ws = oracle.WarmStarter(model, cfg) if seed != "cold" else None
     st = {k: np.asarray(v, np.float64) for k, v in state.items()}
     t = 0.0
     for _ in range(sample_steps):
         c = col.collide(torch.as_tensor(st["body_q"], dtype=torch.float32))
         hc = oracle.HostContacts(c)
         t0 = time.perf_counter()
-        o = oracle.step(model, cfg, st, control, hc, DT, precision=32, host_model=hm)
+        lam_seed = ws.apply(st, hc, DT) if ws is not None else None
+        o = oracle.step(model, cfg, st, control, hc, DT, precision=32, lam_seed=lam_seed, host_model=hm)
+        if ws is not None:
+            ws.snapshot(st, hc, DT, o["lam"])
         t += time.perf_counter() - t0
         st["body_q"], st["body_qd"] = o["body_q"], o["body_qd"]
     return dict(value=sample_worlds * sample_steps / t, unit="world-steps/s", cores=1, kind="port",
                 sample=f"oracle FP32 (C++ CPU restatement, 1 thread), first {sample_worlds} worlds of the "
-                       f"same seeded scene, {sample_steps} closed-loop steps, solver calls only ({t:.1f} s)")
+                       f"same seeded scene, {sample_steps} closed-loop steps, Newton seed '{seed}', solver and "
+                       f"warm-start calls only ({t:.1f} s)")
 
 
 def main():
@@ -170,12 +195,16 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--worlds", type=int, default=4096)
-    ap.add_argument("--paths", default="fused:jacobi,plain:jacobi,fused:per_body_pair",
+    ap.add_argument("--paths",
+                    default="fused:jacobi:prev,fused:jacobi:cold,plain:jacobi:cold,fused:per_body_pair:cold,"
+                            "fused:jacobi:learned",
                     help="comma-separated path:preconditioner variants; the first is the headline")
     ap.add_argument("--cpu-sample-worlds", type=int, default=64)
     ap.add_argument("--cpu-sample-steps", type=int, default=40)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a 256 MiB device copy (reference kernel for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: worlds are independent, so the solver shards by world range "
@@ -206,7 +235,7 @@ def main():
     achieved = W * bws / (r["kernel_ms"] / 1e3) / 1e9
     lds_peak = 256 * 128 * 2.4e9 / 1e9  # 256 CUs x 128 B/clk x 2.4 GHz
     out = {
-        "metric": "world-steps/s at FB residual<=1e-5 (fused persistent solver)",
+        "metric": "world-steps/s at FB residual<=1e-5 (fused persistent solver + cross-step warm start)",
         "value": r["world_steps_per_s"],
         "unit": "world-steps/s",
         "n_gpus": 1,
@@ -219,7 +248,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded skid-steer scene, primitive collider)",
         "config": {"workload": "uneven-batch skid-steer, 4096 worlds, dt 0.05, NR<=20 to ||h||<=1e-5, "
-                               "PCR<=16 (Jacobi, the library default), 128 contact slots", "worlds": W,
+                               "PCR<=16 (Jacobi, the library default), 128 contact slots, Newton seeded by "
+                               "the previous step's contacts", "worlds": W,
                    "dt": DT, "variant": main_path, "termination": "per_world"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
@@ -232,17 +262,27 @@ def main():
                    "converged_fraction": r["converged_fraction"], "contacts_mean": n_c},
         "paths": {p: results[p] for p in paths},
     }
-    fj, pj = results.get("fused:jacobi", {}), results.get("plain:jacobi", {})
+    fj, pj = results.get("fused:jacobi:cold", {}), results.get("plain:jacobi:cold", {})
     if "world_steps_per_s" in fj and "world_steps_per_s" in pj:
-        out["fused_over_plain_jacobi"] = fj["world_steps_per_s"] / pj["world_steps_per_s"]
+        out["fused_over_plain_cold"] = fj["world_steps_per_s"] / pj["world_steps_per_s"]
+    fw = results.get(main_path, {})
+    if "world_steps_per_s" in fw and "world_steps_per_s" in pj:
+        out["fused_warm_over_plain_cold"] = fw["world_steps_per_s"] / pj["world_steps_per_s"]
     if not args.no_adjoint:
         print("[bench] adjoint rollout", file=sys.stderr, flush=True)
         out["adjoint"] = adjoint_leg(device)
         print(f"[bench] adjoint: {out['adjoint']}", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
+        mp = main_path.split(":")
         out["cpu_baseline"] = cpu_baseline(lambda n: scenes.skid_steer(n, seed=1), args.cpu_sample_worlds,
-                                           args.cpu_sample_steps, precond=main_path.split(":")[1])
+                                           args.cpu_sample_steps, precond=mp[1],
+                                           seed=mp[2] if len(mp) > 2 and mp[2] != "learned" else "cold")
+    if args.copy_calibration:
+        src = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device=device)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
     print(json.dumps(out))
 
 

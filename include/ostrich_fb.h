@@ -137,6 +137,13 @@ typedef struct ofb_config {
   int32_t warm_start_cold_gravity; /* warm_start.cold_gravity */
   int32_t warm_start_cold_impact;  /* warm_start.cold_impact */
   float warm_start_cold_friction_v;/* warm_start.cold_friction_v_threshold (<= 0: off) */
+  int32_t linesearch;            /* linesearch.enabled (fused path only) */
+  int32_t linesearch_conservative_count;   /* linesearch.conservative_step_count */
+  int32_t linesearch_optimistic_count;     /* linesearch.optimistic_step_count */
+  double linesearch_min_step;              /* linesearch.min_step (double: the candidate set
+                                              is built in double, as the reference's numpy) */
+  double linesearch_conservative_upper;    /* linesearch.conservative_upper_bound */
+  double linesearch_optimistic_window;     /* linesearch.optimistic_window */
 } ofb_config;
 
 typedef struct ofb_state_in {
@@ -282,6 +289,11 @@ int ofb_mlp_gradient(ofb_mlp* net, const float* x, const float* target, int n, f
 /* one Adam step (beta 0.9 / 0.999, eps 1e-8) on that loss */
 int ofb_mlp_train_step(ofb_mlp* net, const float* x, const float* target, int n, float lr, float* loss,
                        void* stream);
+
+/* Copies the merit line search's candidate step sizes (core/engine_data.py:15-39, the
+ * set the solver evaluates when linesearch is enabled) into a host buffer of at least
+ * conservative_count + optimistic_count floats; *count receives their number. */
+int ofb_get_linesearch_steps(const ofb_solver* solver, float* steps, int32_t* count);
 
 /* Attaches (net != NULL) or detaches the learned warm start (no counterpart in the
  * reference): with warm_start and seed_iterate on, each step the network predicts the

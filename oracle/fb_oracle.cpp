@@ -1084,6 +1084,62 @@ template <class T> void newton_update(Batch<T>& Bt, int w) {
   }
 }
 
+// _compute_linesearch_step_size_array (core/engine_data.py:15-39), in double, as float
+std::vector<float> linesearch_step_list(const ofb_config& c) {
+  const int nc = c.linesearch_conservative_count, no = c.linesearch_optimistic_count;
+  std::vector<double> v;
+  const double a = std::log10((double)c.linesearch_min_step), b = std::log10((double)c.linesearch_conservative_upper);
+  for (int i = 0; i < nc; ++i) v.push_back(std::pow(10.0, nc == 1 ? a : (i == nc - 1 ? b : a + i * ((b - a) / (nc - 1)))));
+  const double h = c.linesearch_optimistic_window / 2.0, lo = 1.0 - h, hi = 1.0 + h;
+  for (int i = 0; i < no; ++i) v.push_back(no == 1 ? lo : (i == no - 1 ? hi : lo + i * ((hi - lo) / (no - 1))));
+  std::sort(v.begin(), v.end());
+  size_t best = 0;
+  for (size_t i = 1; i < v.size(); ++i)
+    if (std::fabs(v[i] - 1.0) < std::fabs(v[best] - 1.0)) best = i;
+  v[best] = 1.0;
+  return std::vector<float>(v.begin(), v.end());
+}
+
+// perform_linesearch (core/linesearch.py:330-339): candidates u0 + a du, integrate from
+// the step-start pose, lambda0 + a dlambda, lambda_prev unchanged; first argmin of the
+// merit; the chosen iterate's residual stays in h_d / h_c.
+template <class T> void newton_update_ls(Batch<T>& Bt, int w, const std::vector<float>& steps) {
+  for (int b = 0; b < Bt.B; ++b) Bt.tmp_body[Bt.bi(w, b)] = spzero<T>();
+  for (int i = 0; i < Bt.Nc; ++i) {
+    size_t rr = Bt.ri(w, i);
+    T dl = Bt.dlam[rr];
+    if (Bt.b0[rr] >= 0) { Sp<T>& t = Bt.tmp_body[Bt.bi(w, Bt.b0[rr])]; for (int k = 0; k < 6; ++k) t.v[k] += dl * Bt.J0[rr].v[k]; }
+    if (Bt.b1[rr] >= 0) { Sp<T>& t = Bt.tmp_body[Bt.bi(w, Bt.b1[rr])]; for (int k = 0; k < 6; ++k) t.v[k] += dl * Bt.J1[rr].v[k]; }
+  }
+  std::vector<Sp<T>> u0(Bt.B);
+  std::vector<T> l0(Bt.Nc);
+  for (int b = 0; b < Bt.B; ++b) {
+    size_t k = Bt.bi(w, b);
+    Sp<T> a;
+    for (int i = 0; i < 6; ++i) a.v[i] = Bt.tmp_body[k].v[i] * Bt.dt - Bt.h_d[k].v[i];
+    Bt.du[k] = minv(Bt, w, b, a);
+    u0[b] = Bt.u[k];
+  }
+  for (int i = 0; i < Bt.Nc; ++i) l0[i] = Bt.lam[Bt.ri(w, i)];
+  auto set_alpha = [&](T a) {
+    for (int b = 0; b < Bt.B; ++b) {
+      size_t k = Bt.bi(w, b);
+      for (int i = 0; i < 6; ++i) Bt.u[k].v[i] = u0[b].v[i] + a * Bt.du[k].v[i];
+      Bt.q[k] = integrate_pose(Bt.u[k], Bt.q_prev[k], Bt.com(w, b), Bt.dt);
+    }
+    for (int i = 0; i < Bt.Nc; ++i) Bt.lam[Bt.ri(w, i)] = l0[i] + a * Bt.dlam[Bt.ri(w, i)];
+  };
+  size_t best = 0;
+  T best_m = 0;
+  for (size_t si = 0; si < steps.size(); ++si) {
+    set_alpha(T(steps[si]));
+    residual(Bt, w);
+    T m = merit_of(Bt, w);
+    if (si == 0 || m < best_m) { best_m = m; best = si; }
+  }
+  set_alpha(T(steps[best]));
+}
+
 template <class T> void save_candidate(Batch<T>& Bt, int w, int idx) {  // history_group.py:12-79
   if (idx >= Bt.cfg.nr_max_iters) return;
   for (int b = 0; b < Bt.B; ++b) { size_t k = Bt.bi(w, b); Bt.cand_q[idx][k] = Bt.q[k]; Bt.cand_u[idx][k] = Bt.u[k]; }
@@ -1141,6 +1197,7 @@ template <class T> int run_step(const ofb_model* m, const ofb_config* cfg, const
   }
   const bool global = cfg->termination == OFB_TERM_GLOBAL;
   const T atol_sq = T(cfg->nr_atol) * T(cfg->nr_atol);
+  const std::vector<float> ls_steps = cfg->linesearch ? linesearch_step_list(*cfg) : std::vector<float>();
   std::vector<char> running(W, 1);
   int iter = 0;
   std::vector<int> all(W);
@@ -1159,7 +1216,8 @@ template <class T> int run_step(const ofb_model* m, const ofb_config* cfg, const
     pcr_solve(Bt, act, global);
     bool any = false;
     for (int w : act) {
-      newton_update(Bt, w);
+      if (cfg->linesearch) newton_update_ls(Bt, w, ls_steps);
+      else newton_update(Bt, w);
       residual(Bt, w);
       Bt.merit[w] = merit_of(Bt, w);
       if (merit_hist && Bt.nr_iters[w] < cfg->nr_max_iters) merit_hist[size_t(w) * cfg->nr_max_iters + Bt.nr_iters[w]] = Bt.merit[w];
@@ -1759,6 +1817,13 @@ int oracle_adjoint(const ofb_model* m, const ofb_config* cfg, const double* q_pr
   return run_adjoint<float>(m, cfg, q_prev, qd_prev, f_in, q, qd, tpos, tvel, contacts, dt, lam, lam_prev, g_q, g_u,
                             fd_eps, g_q_prev, g_u_prev, g_f, g_tpos, g_tvel, g_ke, g_kd, g_mu, w_lam, w_u, pcr_iters,
                             flags);
+}
+
+// The merit line search's candidate steps of a config; returns their number.
+int oracle_linesearch_steps(const ofb_config* cfg, float* out) {
+  std::vector<float> v = linesearch_step_list(*cfg);
+  for (size_t i = 0; i < v.size(); ++i) out[i] = v[i];
+  return (int)v.size();
 }
 
 // Warm start of one step: mode 0 = apply (writes seed [W*N_c] from the snapshot), mode 1 =

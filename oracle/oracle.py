@@ -39,6 +39,8 @@ def lib():
         _lib.oracle_weighted_residual.restype = C.c_double
         _lib.oracle_warm_start.argtypes = [vp] * 5 + [C.c_float, C.c_int] + [vp] * 9
         _lib.oracle_warm_start.restype = C.c_int
+        _lib.oracle_linesearch_steps.argtypes = [vp, vp]
+        _lib.oracle_linesearch_steps.restype = C.c_int
         _lib.oracle_step.restype = C.c_int
         _lib.oracle_dims.argtypes = [vp, vp, vp, vp, vp]
         _lib.oracle_fb_diff.argtypes = [C.c_double] * 5 + [vp]
@@ -261,3 +263,10 @@ class WarmStarter:
 
     def snapshot(self, state, contacts, dt, lam):
         self._call(state, contacts, dt, 1, lam)
+
+
+def linesearch_steps(config):
+    cfg = config.to_abi()
+    out = np.zeros(1024, np.float32)
+    n = lib().oracle_linesearch_steps(C.byref(cfg), _p(out))
+    return out[:n].copy()

@@ -88,6 +88,12 @@ class OfbConfig(C.Structure):
         ("warm_start_cold_gravity", C.c_int32),
         ("warm_start_cold_impact", C.c_int32),
         ("warm_start_cold_friction_v", C.c_float),
+        ("linesearch", C.c_int32),
+        ("linesearch_conservative_count", C.c_int32),
+        ("linesearch_optimistic_count", C.c_int32),
+        ("linesearch_min_step", C.c_double),
+        ("linesearch_conservative_upper", C.c_double),
+        ("linesearch_optimistic_window", C.c_double),
     ]
 
 

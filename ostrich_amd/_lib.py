@@ -14,7 +14,7 @@ EXPORTS = (
     "ofb_get_lambda_prev", "ofb_set_shape_mu", "ofb_step_backward", "ofb_warm_start_reset",
     "ofb_get_lambda_seed", "ofb_mlp_create", "ofb_mlp_destroy", "ofb_mlp_num_params", "ofb_mlp_set_params",
     "ofb_mlp_get_params", "ofb_mlp_forward", "ofb_mlp_gradient", "ofb_mlp_train_step",
-    "ofb_set_warm_start_net", "ofb_get_contact_features",
+    "ofb_set_warm_start_net", "ofb_get_contact_features", "ofb_get_linesearch_steps",
 )
 
 _lib = None
@@ -53,6 +53,7 @@ def load():
     lib.ofb_mlp_train_step.argtypes = [vp, vp, vp, C.c_int, C.c_float, vp, vp]
     lib.ofb_set_warm_start_net.argtypes = [vp, vp]
     lib.ofb_get_contact_features.argtypes = [vp, vp, vp, vp]
+    lib.ofb_get_linesearch_steps.argtypes = [vp, vp, vp]
     for name in EXPORTS:
         if name != "ofb_last_error":
             getattr(lib, name).restype = C.c_int

@@ -135,7 +135,7 @@ class RelaxationConfig:
 
 @dataclass(frozen=True)
 class LinesearchConfig:
-    """engine_config.py:290-333 (the merit line search is not built on this path yet)."""
+    """engine_config.py:290-333 (merit line search over a fixed candidate set; fused path)."""
 
     enabled: bool = False
     min_step: float = 1e-6
@@ -144,9 +144,24 @@ class LinesearchConfig:
     optimistic_step_count: int = 32
     optimistic_window: float = 0.2
 
+    @property
+    def num_steps(self) -> int:
+        return self.conservative_step_count + self.optimistic_step_count
+
     def __post_init__(self):
-        if self.enabled:
-            raise NotImplementedError("linesearch.enabled=True is not supported by this solver yet")
+        if not self.enabled:
+            return  # engine_config.py:306-308: no validation when off
+        if self.conservative_step_count < 1:
+            raise ValueError(f"linesearch.conservative_step_count must be >= 1, got {self.conservative_step_count}")
+        if self.optimistic_step_count < 1:
+            raise ValueError(f"linesearch.optimistic_step_count must be >= 1, got {self.optimistic_step_count}")
+        if self.min_step < 0:
+            raise ValueError(f"linesearch.min_step must be >= 0, got {self.min_step}")
+        if self.conservative_upper_bound <= self.min_step:
+            raise ValueError("linesearch.conservative_upper_bound must be > min_step, "
+                             f"got {self.conservative_upper_bound} and {self.min_step}")
+        if self.optimistic_window < 0:
+            raise ValueError(f"linesearch.optimistic_window must be >= 0, got {self.optimistic_window}")
 
 
 @dataclass(frozen=True)
@@ -324,6 +339,13 @@ class OstrichEngineConfig:
         c.warm_start_cold_gravity = 1 if self.warm_start.cold_gravity else 0
         c.warm_start_cold_impact = 1 if self.warm_start.cold_impact else 0
         c.warm_start_cold_friction_v = self.warm_start.cold_friction_v_threshold
+        ls = self.linesearch
+        c.linesearch = 1 if ls.enabled else 0
+        c.linesearch_min_step = ls.min_step
+        c.linesearch_conservative_count = ls.conservative_step_count
+        c.linesearch_conservative_upper = ls.conservative_upper_bound
+        c.linesearch_optimistic_count = ls.optimistic_step_count
+        c.linesearch_optimistic_window = ls.optimistic_window
         return c
 
     @property

@@ -1,6 +1,8 @@
 // fb_api.hip -- C ABI entry points (include/ostrich_fb.h): solver lifetime, contact
 // packing and step dispatch.
 
+#include <algorithm>
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -30,6 +32,30 @@ void set_error(const char* fmt, ...) {
       return -4;                                                                     \
     }                                                                                \
   } while (0)
+
+// Line-search candidates (_compute_linesearch_step_size_array, core/engine_data.py:15-39):
+// `nc` log-spaced values in [min_step, upper] and `no` linear ones in 1 -/+ window/2,
+// sorted ascending, the one closest to 1 set to exactly 1; computed in double, stored
+// as float.
+static int linesearch_steps(const ofb_config* c, float* out, int cap) {
+  const int nc = c->linesearch_conservative_count, no = c->linesearch_optimistic_count;
+  if (nc < 1 || no < 1 || nc + no > cap) return -1;
+  std::vector<double> v;
+  const double a = std::log10((double)c->linesearch_min_step), b = std::log10((double)c->linesearch_conservative_upper);
+  for (int i = 0; i < nc; ++i) {
+    double e = nc == 1 ? a : (i == nc - 1 ? b : a + i * ((b - a) / (nc - 1)));
+    v.push_back(std::pow(10.0, e));
+  }
+  const double h = (double)c->linesearch_optimistic_window / 2.0, lo = 1.0 - h, hi = 1.0 + h;
+  for (int i = 0; i < no; ++i) v.push_back(no == 1 ? lo : (i == no - 1 ? hi : lo + i * ((hi - lo) / (no - 1))));
+  std::sort(v.begin(), v.end());
+  size_t best = 0;
+  for (size_t i = 1; i < v.size(); ++i)
+    if (std::fabs(v[i] - 1.0) < std::fabs(v[best] - 1.0)) best = i;
+  v[best] = 1.0;
+  for (size_t i = 0; i < v.size(); ++i) out[i] = (float)v[i];
+  return (int)v.size();
+}
 
 static int joint_row_count(int type) {  // core/model.py:88-99
   static const int m[8] = {5, 5, 3, 6, 0, 1, 6, 0};
@@ -201,7 +227,7 @@ int ofb_destroy(ofb_solver* s) {
                   m.shape_v_stribeck, m.shape_stribeck_lateral_only, m.shape_world, s->ccount, s->cf, s->ci,
                   s->pk_keys, s->pk_keys_out, s->pk_vals, s->pk_vals_out, s->pk_seg_start, s->pk_temp, s->topo,
                   s->lambda, s->lambda_prev, s->lambda_seed, s->ws_count, s->ws_b0, s->ws_b1, s->ws_p, s->ws_n,
-                  s->ws_ln, s->ws_lt, s->ws_feat, s->ws_fscale, s->ws_y, s->lambda_seed2, s->st_nr, s->st_pcr, s->st_best, s->st_flags, s->st_merit};
+                  s->ws_ln, s->ws_lt, s->ws_feat, s->ws_fscale, s->ws_y, s->lambda_seed2, s->ls_steps, s->st_nr, s->st_pcr, s->st_best, s->st_flags, s->st_merit};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (s->plain) plain_destroy(s);
@@ -235,6 +261,10 @@ int ofb_create(const ofb_model* model, const ofb_config* config, int32_t path, o
     return -2;
   }
   if (path == OFB_PATH_FUSED && B > 64) { set_error("the fused path supports at most 64 bodies per world"); return -2; }
+  if (config->linesearch && path != OFB_PATH_FUSED) {
+    set_error("the merit line search runs on the fused path");
+    return -2;
+  }
   if (path == OFB_PATH_PLAIN && config->preconditioner == OFB_PRECOND_PER_BODY_PAIR) {
     set_error("the plain path preconditions with Jacobi only: use the fused path for per_body_pair");
     return -2;
@@ -309,6 +339,17 @@ int ofb_create(const ofb_model* model, const ofb_config* config, int32_t path, o
   s->rigid_contact_cap = cap;
   ALLOC(pk_keys, cap); ALLOC(pk_keys_out, cap); ALLOC(pk_vals, cap); ALLOC(pk_vals_out, cap);
   ALLOC(pk_seg_start, W);
+  if (config->linesearch) {
+    int n = linesearch_steps(config, s->ls_host, 512);
+    if (n < 1) { set_error("invalid line-search step counts (at most 512 candidates)"); ofb_destroy(s); return -2; }
+    s->ls_count = n;
+    ALLOC(ls_steps, (size_t)n);
+    if (hipMemcpy(s->ls_steps, s->ls_host, n * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) {
+      set_error("hipMemcpy failed (line search)");
+      ofb_destroy(s);
+      return -4;
+    }
+  }
   if (config->warm_start) {
     ALLOC(ws_count, W); ALLOC(ws_b0, (size_t)W * K); ALLOC(ws_b1, (size_t)W * K); ALLOC(ws_p, (size_t)W * K * 3);
     ALLOC(ws_n, (size_t)W * K * 3); ALLOC(ws_ln, (size_t)W * K); ALLOC(ws_lt, (size_t)W * K * 2);
@@ -391,6 +432,13 @@ int ofb_warm_start_reset(ofb_solver* s, void* stream) {
   if (!s) { set_error("null solver"); return -1; }
   if (!s->ws_count) return 0;
   OFB_CHECK(hipMemsetAsync(s->ws_count, 0, (size_t)s->m.W * sizeof(int32_t), (hipStream_t)stream));
+  return 0;
+}
+
+int ofb_get_linesearch_steps(const ofb_solver* s, float* steps, int32_t* count) {
+  if (!s || !count) { set_error("null argument"); return -1; }
+  *count = s->ls_count;
+  if (steps) std::memcpy(steps, s->ls_host, (size_t)s->ls_count * sizeof(float));
   return 0;
 }
 

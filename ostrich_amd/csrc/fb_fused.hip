@@ -131,6 +131,8 @@ int fused_step(ofb_solver* s, const ofb_state_in* in, const ofb_state_out* out, 
   P.lambda_out = s->lambda;
   P.lambda_prev_out = s->lambda_prev;
   P.lambda_seed2 = (s->cfg.warm_start && s->ws_net) ? s->lambda_seed2 : nullptr;
+  P.ls_steps = s->ls_steps;
+  P.ls_count = s->cfg.linesearch ? s->ls_count : 0;
   P.st_nr = s->st_nr; P.st_pcr = s->st_pcr; P.st_best = s->st_best; P.st_flags = s->st_flags; P.st_merit = s->st_merit;
   const int W = s->m.W;
   const int r_max = (s->L.Nc + WAVE - 1) / WAVE;

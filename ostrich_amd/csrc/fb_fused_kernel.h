@@ -65,6 +65,8 @@ struct FusedParams {
   const int32_t* topo;  // [Nj] joint of each joint row, [Nj] row within joint, [Nctrl] joint of each control row
   const float* lambda_seed;
   const float* lambda_seed2;  // learned seed (null: none); the lower initial residual wins
+  const float* ls_steps;      // merit line-search candidates (ls_count 0: full Newton step)
+  int ls_count;
   float* lambda_out;
   float* lambda_prev_out;  // lambda_prev_iter as the step leaves it (input of the adjoint)
   int32_t *st_nr, *st_pcr, *st_best, *st_flags;
@@ -622,19 +624,77 @@ fused_step_kernel(FusedParams P) {
         if (b0[k] >= 0) atomic_add_sv(sm + b0[k] * BS + BACC, J0[k] * x[k]);
         if (b1[k] >= 0) atomic_add_sv(sm + b1[k] * BS + BACC, J1[k] * x[k]);
       }
-      lam[k] = lam[k] + x[k];
+      if (P.ls_count == 0) lam[k] = lam[k] + x[k];
     }
     wsync();
-    for (int b = lane; b < B; b += WAVE) {
-      float* bb = sm + b * BS;
-      sv acc = ldsv(bb + BACC), hd = ldsv(bb + BHD);
-      sv du = apply_minv(bb, sv{acc.a * dt - hd.a, acc.b * dt - hd.b});
-      sv u = ldsv(bb + BU);
-      u = sv{u.a + du.a, u.b + du.b};
-      stsv(bb + BU, u);
-      sttf(bb + BQ, integrate(u, ldtf(bb + BQP), ld3(bb + BCOM), dt));
+    if (P.ls_count == 0) {
+      for (int b = lane; b < B; b += WAVE) {
+        float* bb = sm + b * BS;
+        sv acc = ldsv(bb + BACC), hd = ldsv(bb + BHD);
+        sv du = apply_minv(bb, sv{acc.a * dt - hd.a, acc.b * dt - hd.b});
+        sv u = ldsv(bb + BU);
+        u = sv{u.a + du.a, u.b + du.b};
+        stsv(bb + BU, u);
+        sttf(bb + BQ, integrate(u, ldtf(bb + BQP), ld3(bb + BCOM), dt));
+      }
+      wsync();
+    } else {
+      // merit line search (core/linesearch.py:330-339): every candidate alpha gives
+      // u = u0 + alpha du, q = integrate(u, q_prev), lambda = lambda0 + alpha dlambda, with
+      // lambda_prev unchanged (:222-223); the first argmin of ||h||^2 is kept
+      // (reductions.py batched_argmin)
+      for (int b = lane; b < B; b += WAVE) {
+        float* bb = sm + b * BS;
+        sv acc = ldsv(bb + BACC), hd = ldsv(bb + BHD);
+        stsv(bb + BVV, apply_minv(bb, sv{acc.a * dt - hd.a, acc.b * dt - hd.b}));
+        stsv(bb + BU0, ldsv(bb + BU));
+      }
+      wsync();
+      float best_a = 0.f, best_m = 0.f;
+      for (int si = 0; si < P.ls_count; ++si) {
+        const float a = P.ls_steps[si];
+        for (int b = lane; b < B; b += WAVE) {
+          float* bb = sm + b * BS;
+          sv u0 = ldsv(bb + BU0), du = ldsv(bb + BVV);
+          sv u = sv{u0.a + du.a * a, u0.b + du.b * a};
+          stsv(bb + BU, u);
+          sttf(bb + BQ, integrate(u, ldtf(bb + BQP), ld3(bb + BCOM), dt));
+        }
+        wsync();
+        float la[R];
+#pragma unroll
+        for (int k = 0; k < R; ++k) la[k] = lam[k] + a * x[k];
+        dynamics_bodies(P, sm, lane, false);
+        wsync();
+        float ms = 0.f;
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+          int i = lane + WAVE * k;
+          if (i < n_r) {
+            RowOut o = assemble_row(P, sm, lamprev, w, i, n_c, la[k]);
+            ms += o.hc * o.hc;
+          }
+        }
+        wsync();
+        for (int e = lane; e < 6 * B; e += WAVE) {
+          float v = sm[(e / 6) * BS + BHD + (e % 6)];
+          ms += v * v;
+        }
+        ms = wave_sum(ms);
+        if (si == 0 || ms < best_m) { best_m = ms; best_a = a; }
+        wsync();
+      }
+#pragma unroll
+      for (int k = 0; k < R; ++k) lam[k] += best_a * x[k];
+      for (int b = lane; b < B; b += WAVE) {
+        float* bb = sm + b * BS;
+        sv u0 = ldsv(bb + BU0), du = ldsv(bb + BVV);
+        sv u = sv{u0.a + du.a * best_a, u0.b + du.b * best_a};
+        stsv(bb + BU, u);
+        sttf(bb + BQ, integrate(u, ldtf(bb + BQP), ld3(bb + BCOM), dt));
+      }
+      wsync();
     }
-    wsync();
 
     OFB_PHASE(4)
     // ---- residual at the new iterate (core/residual.py:18-173) ----

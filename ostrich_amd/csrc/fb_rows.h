@@ -18,7 +18,7 @@ namespace ofb {
 namespace rows {
 
 constexpr int WAVE = 64;
-constexpr int BS = 97;  // LDS floats per body (odd stride spreads bodies over banks)
+constexpr int BS = 101;  // LDS floats per body (odd stride spreads bodies over banks)
 
 // body block offsets (floats)
 enum {
@@ -38,6 +38,7 @@ enum {
   BCOM = 74,    // com (3)
   BIB = 77,     // body inertia (9)
   BIIB = 86,    // body inverse inertia (9)
+  BU0 = 95,     // velocity before the line-searched step (6)
 };
 
 

@@ -92,6 +92,10 @@ struct ofb_solver {
   float *ws_feat, *ws_fscale, *ws_y;  // learned warm start: features [W*K*12], scale [W*K], output [W*K*3]
   float* lambda_seed2;                // [W*Nc] learned seed
   const struct ofb_mlp* ws_net;       // attached network (not owned)
+  // merit line search candidates (core/engine_data.py:15-39), device and host copies
+  float* ls_steps;
+  int32_t ls_count;
+  float ls_host[512];
   // events around the solve of the last step
   hipEvent_t ev_solve0, ev_solve1;
   int events_ready;

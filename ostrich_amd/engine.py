@@ -130,6 +130,14 @@ class OstrichEngine:
                    "ofb_get_contact_features")
         return f, sc
 
+    def linesearch_steps(self):
+        """The merit line search's candidate step sizes (empty when it is off)."""
+        import numpy as np
+        n = C.c_int32()
+        buf = (C.c_float * 512)()
+        _lib.check(self.lib.ofb_get_linesearch_steps(self._h, buf, C.byref(n)), "ofb_get_linesearch_steps")
+        return np.array(buf[:n.value], np.float32)
+
     def reset_warm_start(self):
         """Forgets the warm starter's snapshot (next step cold-starts every contact)."""
         _lib.check(self.lib.ofb_warm_start_reset(self._h, self._stream()), "ofb_warm_start_reset")

@@ -225,3 +225,30 @@ def test_block_jacobi_fixed_iterations(oracle_mod, gpu_device, nr_iters):
     lscale = np.maximum(np.abs(o["lam"]).max(1, keepdims=True), 1.0)
     assert np.all(np.abs(g["lam"] - o["lam"]) <= 1e-4 * lscale)
     assert np.array_equal(g["flags"] & 12, o["flags"] & 12)
+
+
+@pytest.mark.parametrize("nr_iters", [1, 2])
+def test_linesearch_fixed_newton_iterations(oracle_mod, gpu_device, nr_iters):
+    """Merit line search on (64 candidates per Newton iteration): after 1-2 Newton
+    iterations from identical inputs the GPU picks the same step sizes as the FP32 oracle
+    (identical iterates to 1e-4 of scale) from the same candidate set."""
+    from ostrich_amd.config import LinesearchConfig, NewtonRaphsonConfig
+    model, state, control = scenes.skid_steer(48, seed=6)
+    base = config(K=128)
+    cfg = type(base)(nr=NewtonRaphsonConfig(max_iters=nr_iters, backtrack_min_iter=0, atol=1e-5),
+                     linear=base.linear, contacts=base.contacts, execution=base.execution,
+                     linesearch=LinesearchConfig(enabled=True))
+    col = collide.PrimitiveCollider(model, gap=0.02, max_contacts_per_world=128)
+    rng = np.random.default_rng(10 + nr_iters)
+    st = {k: np.asarray(v, np.float32).copy() for k, v in state.items()}
+    st["body_qd"] = rng.normal(scale=0.3, size=st["body_qd"].shape).astype(np.float32)
+    c = col.collide(torch.as_tensor(st["body_q"], dtype=torch.float32))
+    c = {k: (v.numpy() if hasattr(v, "numpy") else v) for k, v in c.items()}
+    eng = _engine(model, cfg, gpu_device)
+    np.testing.assert_array_equal(eng.linesearch_steps(), oracle_mod.linesearch_steps(cfg))
+    g = _gpu_step(eng, st, control, c, 0.05, gpu_device)
+    o = oracle_mod.step(model, cfg, st, control, c, 0.05, precision=32)
+    np.testing.assert_allclose(g["body_q"], o["body_q"], atol=1e-4)
+    np.testing.assert_allclose(g["body_qd"], o["body_qd"], atol=1e-4 * G * 0.05)
+    lscale = np.maximum(np.abs(o["lam"]).max(1, keepdims=True), 1.0)
+    assert np.all(np.abs(g["lam"] - o["lam"]) <= 1e-4 * lscale)
