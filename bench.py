@@ -234,6 +234,12 @@ def main():
     aws = operand_bytes_per_world(n_rows_active, r["nr_mean"], r["pcr_per_step_mean"])
     achieved = W * bws / (r["kernel_ms"] / 1e3) / 1e9
     lds_peak = 256 * 128 * 2.4e9 / 1e9  # 256 CUs x 128 B/clk x 2.4 GHz
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "fused_step_kernel_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            t = json.load(f)
+        traffic = t["fetch_bytes"] + t["write_bytes"]
     out = {
         "metric": "world-steps/s at FB residual<=1e-5 (fused persistent solver + cross-step warm start)",
         "value": r["world_steps_per_s"],
@@ -252,7 +258,10 @@ def main():
                                "the previous step's contacts", "worlds": W,
                    "dt": DT, "variant": main_path, "termination": "per_world"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_note": "HBM bytes per launch of fused_step_kernel<1,false> at 4096 worlds from "
+                                     "rocprofv3 FETCH_SIZE + WRITE_SIZE scaled by a 256 MiB copy "
+                                     "(profiles/fused_step_kernel_skid_steer_4096.md)",
                      "kernel": "fused_step_kernel", "kernel_ms": r["kernel_ms"],
                      "bytes_per_world_step": bws,
                      "note": "compulsory HBM bytes; the fused kernel is latency/LDS bound",
