@@ -36,16 +36,17 @@ K_CONTACTS = 128
 HBM_PEAK_GBS = 8000.0
 
 
-def solver_config(path, termination="per_world", precond="jacobi", seed="cold"):
+def solver_config(path, termination="per_world", precond="jacobi", seed="cold", paired=True):
     """seed: "cold" (lambda = 0, the reference default), "prev" (the cross-step contact warm
     starter seeds Newton: warm_start.seed_iterate), "learned" (prev + the learned
-    per-contact network, gated per world)."""
+    per-contact network, gated per world).  paired: worlds of at most 32 rows run two per
+    wavefront on the assembled Schur matrix (fused path)."""
     return OstrichEngineConfig(
         nr=NewtonRaphsonConfig(max_iters=20, atol=1e-5),
         linear=LinearSolverConfig(max_iters=16, preconditioner_type=precond),
         contacts=ContactsConfig(max_per_world=K_CONTACTS),
         warm_start=WarmStartConfig(enabled=True, seed_iterate=seed in ("prev", "learned")),
-        execution=ExecutionConfig(path=path, termination=termination),
+        execution=ExecutionConfig(path=path, termination=termination, pair_small_worlds=paired),
     )
 
 
@@ -75,7 +76,8 @@ def run_path(variant, model, state, control, steps, warmup, device, collider):
     parts = variant.split(":")
     path, precond = parts[0], parts[1]
     seed = parts[2] if len(parts) > 2 else "cold"
-    cfg = solver_config(path, precond=precond, seed=seed)
+    paired = not (len(parts) > 3 and parts[3] == "unpaired")
+    cfg = solver_config(path, precond=precond, seed=seed, paired=paired)
     eng = OstrichEngine(model, config=cfg, device=device)
     if seed == "learned":
         from ostrich_amd.learned import WarmStartNet
@@ -196,8 +198,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--worlds", type=int, default=4096)
     ap.add_argument("--paths",
-                    default="fused:jacobi:prev,fused:jacobi:cold,plain:jacobi:cold,fused:per_body_pair:cold,"
-                            "fused:jacobi:learned",
+                    default="fused:jacobi:prev,fused:jacobi:prev:unpaired,fused:jacobi:cold,plain:jacobi:cold,"
+                            "fused:per_body_pair:cold,fused:jacobi:learned",
                     help="comma-separated path:preconditioner variants; the first is the headline")
     ap.add_argument("--cpu-sample-worlds", type=int, default=64)
     ap.add_argument("--cpu-sample-steps", type=int, default=40)

@@ -144,6 +144,9 @@ typedef struct ofb_config {
                                               is built in double, as the reference's numpy) */
   double linesearch_conservative_upper;    /* linesearch.conservative_upper_bound */
   double linesearch_optimistic_window;     /* linesearch.optimistic_window */
+  int32_t pair_small_worlds;     /* fused path: worlds of at most 32 rows run two per
+                                    wavefront on the assembled (dense) Schur operator;
+                                    0: one world per wavefront, matrix-free operator */
 } ofb_config;
 
 typedef struct ofb_state_in {

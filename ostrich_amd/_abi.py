@@ -94,6 +94,7 @@ class OfbConfig(C.Structure):
         ("linesearch_min_step", C.c_double),
         ("linesearch_conservative_upper", C.c_double),
         ("linesearch_optimistic_window", C.c_double),
+        ("pair_small_worlds", C.c_int32),
     ]
 
 

@@ -240,10 +240,14 @@ class ExecutionConfig:
     termination: "per_world" - each world leaves the Newton / PCR loops on its own
                  criterion; "global" - the reference's batch-global stop flags
                  (base_engine.py:42-80, pcr_solver.py:84-117).
+    pair_small_worlds: fused path - worlds of at most 32 constraint rows run two per
+                 wavefront (32 lanes each) and solve PCR on the assembled Schur matrix;
+                 False keeps one world per wavefront and the matrix-free operator.
     """
 
     path: str = "fused"
     termination: str = "per_world"
+    pair_small_worlds: bool = True
 
     def __post_init__(self):
         if self.path not in ("fused", "plain"):
@@ -346,6 +350,7 @@ class OstrichEngineConfig:
         c.linesearch_conservative_upper = ls.conservative_upper_bound
         c.linesearch_optimistic_count = ls.optimistic_step_count
         c.linesearch_optimistic_window = ls.optimistic_window
+        c.pair_small_worlds = 1 if self.execution.pair_small_worlds else 0
         return c
 
     @property

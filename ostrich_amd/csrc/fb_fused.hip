@@ -7,31 +7,45 @@ namespace ofb {
 
 // Row classes: a world with n_r active rows runs the instance with R = the smallest
 // class >= ceil(n_r / 64) slots per lane, so small worlds do not pay for the padded
-// row count's registers.  Worlds are bucketed on the device every step.
-constexpr int kNumClasses = 11;
-__constant__ int kClassR[kNumClasses] = {1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 16};
-static const int kClassRHost[kNumClasses] = {1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 16};
+// row count's registers; with pair_small_worlds, worlds of at most 32 rows take class
+// code 0 (two worlds per wavefront, dense Schur operator).  Worlds are bucketed on the
+// device every step.
+constexpr int kNumClasses = 12;
+constexpr int kPairRows = kDenseLd;
+__constant__ int kClassR[kNumClasses] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 16};
+static const int kClassRHost[kNumClasses] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 16};
 
 __global__ void reset_classes_kernel(int32_t* counts) {
   if (threadIdx.x < kNumClasses) counts[threadIdx.x] = 0;
 }
 
 __global__ void classify_worlds_kernel(int W, const int32_t* __restrict__ ccount, int K, int base_rows,
-                                       int32_t* __restrict__ counts, int32_t* __restrict__ lists) {
+                                       int pair_rows, int32_t* __restrict__ counts, int32_t* __restrict__ lists) {
   int w = blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= W) return;
   int n_r = base_rows + 3 * min(ccount[w], K);
-  int r = max(1, (n_r + WAVE - 1) / WAVE);
   int c = 0;
-  while (c < kNumClasses - 1 && kClassR[c] < r) ++c;
+  if (n_r > pair_rows) {
+    int r = max(1, (n_r + WAVE - 1) / WAVE);
+    c = 1;
+    while (c < kNumClasses - 1 && kClassR[c] < r) ++c;
+  }
   int slot = atomicAdd(counts + c, 1);
   lists[(size_t)c * W + slot] = w;
 }
 
-// LDS floats per wave of the class with R row slots per lane
-static int lds_words(const ofb_solver* s, int R) {
-  int base = s->m.B * BS + s->L.Nc + 1;
-  return base + (s->cfg.preconditioner == OFB_PRECOND_PER_BODY_PAIR ? bj_words(WAVE * R) : 0);
+// lanes per world and row capacity of a class code
+static int class_lanes(int code) { return code == 0 ? kPairRows : WAVE; }
+static int class_rows(int code) { return code == 0 ? kPairRows : WAVE * code; }
+
+// LDS floats per world of the forward kernel (a multiple of 4, so that every world's
+// region stays 16-byte aligned): bodies, lambda_prev, block-Jacobi, dense operator
+static int lds_words(const ofb_solver* s, int code) {
+  const bool bj = s->cfg.preconditioner == OFB_PRECOND_PER_BODY_PAIR;
+  const int cap = class_rows(code);
+  int words = code == 0 ? fused_dense_offset(s->m.B, cap, bj) + dense_words()
+                        : s->m.B * BS + cap + 1 + (bj ? bj_words(cap) : 0);
+  return (words + 3) & ~3;
 }
 
 // LDS floats per wave of the adjoint kernel: the forward's bodies and lambda_prev, the
@@ -61,32 +75,35 @@ static int class_op(int R, int op, const FusedParams& P, int blocks, int threads
   return rc;
 }
 
-static int launch_fused(int R, FusedParams P, const ofb_solver* s, int cls, hipStream_t stream) {
+static int launch_fused(int code, FusedParams P, const ofb_solver* s, int cls, hipStream_t stream) {
   P.wlist = s->cls_lists + (size_t)cls * P.m.W;
   P.wcount = s->cls_counts + cls;
-  P.rows_cap = WAVE * R;
-  P.lds_per_wave = lds_words(s, R);
-  // worlds (waves) per workgroup: as many as fit in 64 KB of LDS, at most the tuned value
-  int wpb = s->fused_worlds_per_block;
-  while (wpb > 1 && (size_t)wpb * P.lds_per_wave * sizeof(float) > 65536) --wpb;
-  size_t lds_bytes = (size_t)wpb * P.lds_per_wave * sizeof(float);
+  P.rows_cap = class_rows(code);
+  P.lds_per_world = lds_words(s, code);
+  const int per_wave = WAVE / class_lanes(code);
+  // worlds per workgroup: as many as fit in 64 KB of LDS, at most the tuned number of
+  // waves' worth
+  int wpb = s->fused_worlds_per_block * per_wave;
+  while (wpb > per_wave && (size_t)wpb * P.lds_per_world * sizeof(float) > 65536) wpb -= per_wave;
+  size_t lds_bytes = (size_t)wpb * P.lds_per_world * sizeof(float);
   int blocks = (P.m.W + wpb - 1) / wpb;
-  return class_op(R, 0, P, blocks, wpb * WAVE, lds_bytes, stream,
+  return class_op(code, 0, P, blocks, wpb * class_lanes(code), lds_bytes, stream,
                   s->cfg.preconditioner == OFB_PRECOND_PER_BODY_PAIR);
 }
 
 // lets a class kernel (forward and adjoint) use more than 64 KB of LDS when one world
 // needs it
-static int allow_lds(int R, const ofb_solver* s) {
+static int allow_lds(int code, const ofb_solver* s) {
   const bool bj = s->cfg.preconditioner == OFB_PRECOND_PER_BODY_PAIR;
-  size_t bytes = (size_t)lds_words(s, R) * sizeof(float);
+  size_t bytes = (size_t)lds_words(s, code) * sizeof(float) * (WAVE / class_lanes(code));
   if (bytes > 65536) {
     FusedParams P{};
-    if (class_op(R, 1, P, 0, 0, bytes, nullptr, bj) != 0) {
-      set_error("fused kernel needs %zu bytes of LDS per world", bytes);
+    if (class_op(code, 1, P, 0, 0, bytes, nullptr, bj) != 0) {
+      set_error("fused kernel needs %zu bytes of LDS per wavefront", bytes);
       return -4;
     }
   }
+  const int R = code == 0 ? 1 : code;
   bytes = (size_t)adj_lds_words(s, R) * sizeof(float);
   if (bytes > 65536) {
     AdjParams A{};
@@ -98,13 +115,17 @@ static int allow_lds(int R, const ofb_solver* s) {
   return 0;
 }
 
+// classes a world of this solver can fall in (class codes are increasing)
+static bool class_used(const ofb_solver* s, int c) {
+  const int r_max = (s->L.Nc + WAVE - 1) / WAVE;
+  if (c == 0) return s->cfg.pair_small_worlds != 0;
+  return c == 1 || kClassRHost[c - 1] < r_max;
+}
 
 int fused_classes_alloc(ofb_solver* s) {
-  int rc = 0;
-  const int r_max = (s->L.Nc + WAVE - 1) / WAVE;
   for (int c = 0; c < kNumClasses; ++c) {
-    if (c > 0 && kClassRHost[c - 1] >= r_max) break;
-    rc = allow_lds(kClassRHost[c], s);
+    if (!class_used(s, c)) continue;
+    int rc = allow_lds(kClassRHost[c], s);
     if (rc) return rc;
   }
   if (hipMalloc((void**)&s->cls_counts, kNumClasses * sizeof(int32_t)) != hipSuccess ||
@@ -143,11 +164,12 @@ int fused_step(ofb_solver* s, const ofb_state_in* in, const ofb_state_out* out, 
   }
   hipLaunchKernelGGL(reset_classes_kernel, dim3(1), dim3(64), 0, stream, s->cls_counts);
   hipLaunchKernelGGL(classify_worlds_kernel, dim3((W + 255) / 256), dim3(256), 0, stream, W, s->ccount, s->L.K,
-                     s->L.Nj + s->L.Nctrl, s->cls_counts, s->cls_lists);
+                     s->L.Nj + s->L.Nctrl, s->cfg.pair_small_worlds ? kPairRows : -1, s->cls_counts,
+                     s->cls_lists);
   for (int c = 0; c < kNumClasses; ++c) {
-    if (c > 0 && kClassRHost[c - 1] >= r_max) break;  // no world can need this class
+    if (!class_used(s, c)) continue;  // no world can fall in this class
     if (launch_fused(kClassRHost[c], P, s, c, stream) != 0) {
-      set_error("no fused kernel instance for %d row slots per lane", kClassRHost[c]);
+      set_error("no fused kernel instance for class code %d", kClassRHost[c]);
       return -4;
     }
   }
@@ -183,11 +205,12 @@ int fused_adjoint(ofb_solver* s, const ofb_state_in* in, const ofb_state_out* ou
     return -3;
   }
   const bool bj = s->cfg.preconditioner == OFB_PRECOND_PER_BODY_PAIR;
+  // the adjoint runs one world per wavefront: no world is put in the paired class
   hipLaunchKernelGGL(reset_classes_kernel, dim3(1), dim3(64), 0, stream, s->cls_counts);
   hipLaunchKernelGGL(classify_worlds_kernel, dim3((W + 255) / 256), dim3(256), 0, stream, W, s->ccount, s->L.K,
-                     s->L.Nj + s->L.Nctrl, s->cls_counts, s->cls_lists);
-  for (int c = 0; c < kNumClasses; ++c) {
-    if (c > 0 && kClassRHost[c - 1] >= r_max) break;
+                     s->L.Nj + s->L.Nctrl, -1, s->cls_counts, s->cls_lists);
+  for (int c = 1; c < kNumClasses; ++c) {
+    if (!class_used(s, c)) continue;
     const int R = kClassRHost[c];
     P.wlist = s->cls_lists + (size_t)c * W;
     P.wcount = s->cls_counts + c;

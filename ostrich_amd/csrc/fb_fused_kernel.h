@@ -71,10 +71,10 @@ struct FusedParams {
   float* lambda_prev_out;  // lambda_prev_iter as the step leaves it (input of the adjoint)
   int32_t *st_nr, *st_pcr, *st_best, *st_flags;
   float* st_merit;
-  int lds_per_wave;  // floats
+  int lds_per_world;       // floats (a multiple of 4)
   const int32_t* wlist;   // worlds of this launch's row class
   const int32_t* wcount;  // number of them
-  int rows_cap;           // 64 R: row capacity of this launch's class
+  int rows_cap;           // G R: row capacity of this launch's class
 };
 
 // ---------------------------------------------------------------------------------
@@ -103,7 +103,11 @@ struct FusedParams {
 constexpr int kPairCap = 64;
 constexpr int kPairL = kPairCap * (kPairCap + 1) / 2;
 
-__host__ __device__ inline int bj_words(int cap) { return 20 * cap + (cap / 64) * kPairL; }
+// packed blocks: at most cap / 64 full 64-row pairs (cap a multiple of 64), or one pair
+// holding every row (cap < 64)
+__host__ __device__ inline int bj_words(int cap) {
+  return 20 * cap + (cap >= kPairCap ? (cap / kPairCap) * kPairL : cap * (cap + 1) / 2);
+}
 
 struct BJView {
   int* pid;
@@ -136,19 +140,14 @@ struct BJRegs {
   int mmax;     // largest block of the wave
 };
 
-__device__ __forceinline__ int wave_max_int(int v) {
-  for (int off = 32; off >= 1; off >>= 1) v = max(v, __shfl_xor(v, off, WAVE));
-  return v;
-}
-
 // pair ids (_compute_pair_ids_kernel :62-109), member ranks and sizes (:112-144)
-template <int R>
+template <int R, int GW = WAVE>
 __device__ void bj_structure(const BJView& V, int lane, int n_r, int B, const bool (&act)[R], const int (&b0)[R],
                              const int (&b1)[R], BJRegs<R>& G, int& flags) {
   int mypid[R], leader[R];
 #pragma unroll
   for (int k = 0; k < R; ++k) {
-    int i = lane + WAVE * k;
+    int i = lane + GW * k;
     mypid[k] = -1;
     if (i < n_r) {
       if (act[k]) {
@@ -161,7 +160,7 @@ __device__ void bj_structure(const BJView& V, int lane, int n_r, int B, const bo
   wsync();
 #pragma unroll
   for (int k = 0; k < R; ++k) {
-    int i = lane + WAVE * k;
+    int i = lane + GW * k;
     int rank = 0, size = 0, lead = -1;
     if (i < n_r && mypid[k] >= 0) {
       for (int j = 0; j < n_r; ++j) {
@@ -192,7 +191,7 @@ __device__ void bj_structure(const BJView& V, int lane, int n_r, int B, const bo
     G.poff[k] = po;
     if (G.slot[k] >= 0) mm = max(mm, G.m[k]);
   }
-  G.mmax = wave_max_int(mm);
+  G.mmax = group_max_int<GW>(mm);
   wsync();
 }
 
@@ -342,11 +341,11 @@ __device__ void bj_apply(const BJView& V, const bool (&act)[R], const float (&Pi
 
 // out = (J M^-1 J^T + C + reg) v over the rows of one world (optim/system_operator.py:36-124):
 // scatter J^T v into the LDS body accumulator, apply M^-1 per body, gather.
-template <int R>
+template <int R, int GW = WAVE>
 __device__ __forceinline__ void schur_matvec(float* sm, int lane, int B, const bool (&act)[R], const int (&b0)[R],
                                              const int (&b1)[R], const sv (&J0)[R], const sv (&J1)[R],
                                              const float (&Cr)[R], const float (&v)[R], float (&out)[R]) {
-  for (int e = lane; e < 6 * B; e += WAVE) sm[(e / 6) * BS + BACC + (e % 6)] = 0.f;
+  for (int e = lane; e < 6 * B; e += GW) sm[(e / 6) * BS + BACC + (e % 6)] = 0.f;
   wsync();
 #pragma unroll
   for (int k = 0; k < R; ++k) {
@@ -356,7 +355,7 @@ __device__ __forceinline__ void schur_matvec(float* sm, int lane, int B, const b
     }
   }
   wsync();
-  for (int b = lane; b < B; b += WAVE) {
+  for (int b = lane; b < B; b += GW) {
     float* bb = sm + b * BS;
     stsv(bb + BVV, apply_minv(bb, ldsv(bb + BACC)));
   }
@@ -374,55 +373,172 @@ __device__ __forceinline__ void schur_matvec(float* sm, int lane, int B, const b
   wsync();
 }
 
+// ---------------------------------------------------------------------------------
+// Dense Schur operator for worlds of at most 32 rows (one row per lane, R = 1).  Once per
+// Newton iteration the group assembles A = J M^-1 J^T + diag(C + reg) (the operator of
+// optim/system_operator.py:36-124 written out) into LDS, column j at A + 32 j, so that a
+// PCR matvec is one burst of 32 conflict-free column reads against a broadcast of v,
+// with a single wait: no LDS atomics and no per-body pass inside the Krylov loop, whose
+// cost is LDS latency rather than bandwidth.  Row i of A is built by lane i from every
+// row j's M^-1 J_j^T (two bodies, staged in LDS) and body indices; rows that share no
+// body give exact zeros, inactive rows and the padding up to 32 give zero rows and
+// columns (their PCR entries are zero, as in the matrix-free operator).
+// LDS: A[32 * 32], rowd[16 * 32] (M^-1 J0^T 6, M^-1 J1^T 6, b0 bits, b1 bits, pad), v[32].
+// ---------------------------------------------------------------------------------
+constexpr int kDenseLd = 32;
+__host__ __device__ inline int dense_words() { return kDenseLd * kDenseLd + 17 * kDenseLd; }
+
+struct DenseView {
+  float* A;
+  float* rowd;
+  float* v;
+  int n_r;
+};
+
+__device__ __forceinline__ DenseView dense_view(float* base, int n_r) {
+  return DenseView{base, base + kDenseLd * kDenseLd, base + kDenseLd * kDenseLd + 16 * kDenseLd, n_r};
+}
+
+// one entry A_ij from row j's staged data (x0..x3) and row i's Jacobian and bodies
+__device__ __forceinline__ float dense_entry(const float4& x0, const float4& x1, const float4& x2, const float4& x3,
+                                             int m0, int m1, const sv& J0, const sv& J1) {
+  const sv w0{mk(x0.x, x0.y, x0.z), mk(x0.w, x1.x, x1.y)};
+  const sv w1{mk(x1.z, x1.w, x2.x), mk(x2.y, x2.z, x2.w)};
+  const int c0 = __float_as_int(x3.x), c1 = __float_as_int(x3.y);
+  float a = 0.f;
+  if (m0 == c0) a += sdot(J0, w0);
+  if (m0 == c1) a += sdot(J0, w1);
+  if (m1 == c0) a += sdot(J1, w0);
+  if (m1 == c1) a += sdot(J1, w1);
+  return a;
+}
+
+__device__ __forceinline__ void dense_stage(float* __restrict__ rowd, const float* sm, int lane, bool on, int b0,
+                                            int b1, const sv& J0, const sv& J1) {
+  sv w0 = svzero(), w1 = svzero();
+  int c0 = -2, c1 = -2;  // an inactive row matches no body
+  if (on && b0 >= 0) { w0 = apply_minv(sm + b0 * BS, J0); c0 = b0; }
+  if (on && b1 >= 0) { w1 = apply_minv(sm + b1 * BS, J1); c1 = b1; }
+  float4* r = reinterpret_cast<float4*>(rowd + 16 * lane);
+  r[0] = make_float4(w0.a.x, w0.a.y, w0.a.z, w0.b.x);
+  r[1] = make_float4(w0.b.y, w0.b.z, w1.a.x, w1.a.y);
+  r[2] = make_float4(w1.a.z, w1.b.x, w1.b.y, w1.b.z);
+  r[3] = make_float4(__int_as_float(c0), __int_as_float(c1), 0.f, 0.f);
+}
+
+// rows j, j + 1 per pass: their staged data is loaded in one burst before the two
+// stores (A and rowd do not alias)
+__device__ __forceinline__ void dense_fill(float* __restrict__ A, const float* __restrict__ rowd, int lane, int n_r,
+                                           bool on, int b0, int b1, const sv& J0, const sv& J1, float Cr) {
+  const int m0 = (on && b0 >= 0) ? b0 : -3, m1 = (on && b1 >= 0) ? b1 : -3;
+  const float4* r = reinterpret_cast<const float4*>(rowd);
+  for (int j = 0; j < kDenseLd; j += 2) {
+    float a0 = 0.f, a1 = 0.f;
+    if (j < n_r) {
+      const float4 x0 = r[4 * j], x1 = r[4 * j + 1], x2 = r[4 * j + 2], x3 = r[4 * j + 3];
+      const float4 y0 = r[4 * j + 4], y1 = r[4 * j + 5], y2 = r[4 * j + 6], y3 = r[4 * j + 7];
+      a0 = dense_entry(x0, x1, x2, x3, m0, m1, J0, J1);
+      a1 = j + 1 < n_r ? dense_entry(y0, y1, y2, y3, m0, m1, J0, J1) : 0.f;
+      if (j == lane && on) a0 += Cr;
+      if (j + 1 == lane && on) a1 += Cr;
+    }
+    A[j * kDenseLd + lane] = a0;
+    A[(j + 1) * kDenseLd + lane] = a1;
+  }
+}
+
+__device__ __forceinline__ void dense_build(const DenseView& D, const float* sm, int lane, bool act, int b0, int b1,
+                                            const sv& J0, const sv& J1, float Cr) {
+  const bool on = lane < D.n_r && act;
+  dense_stage(D.rowd, sm, lane, on, b0, b1, J0, J1);
+  wsync();
+  dense_fill(D.A, D.rowd, lane, D.n_r, on, b0, b1, J0, J1, Cr);
+  wsync();
+}
+
+// out_i = sum_j A_ij v_j: v goes through LDS once, then all 32 column entries and the 8
+// broadcast quads of v are in flight together
+__device__ __forceinline__ float dense_apply(const float* __restrict__ A, float* __restrict__ vb, int lane, bool on,
+                                             float v) {
+  vb[lane] = on ? v : 0.f;
+  wsync();
+  const float4* v4 = reinterpret_cast<const float4*>(vb);
+  float4 x[kDenseLd / 4];
+#pragma unroll
+  for (int q = 0; q < kDenseLd / 4; ++q) x[q] = v4[q];
+  float a[kDenseLd];
+#pragma unroll
+  for (int j = 0; j < kDenseLd; ++j) a[j] = A[j * kDenseLd + lane];
+  float o[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int q = 0; q < kDenseLd / 4; ++q) {
+    o[0] += a[4 * q] * x[q].x;
+    o[1] += a[4 * q + 1] * x[q].y;
+    o[2] += a[4 * q + 2] * x[q].z;
+    o[3] += a[4 * q + 3] * x[q].w;
+  }
+  wsync();
+  return on ? (o[0] + o[1]) + (o[2] + o[3]) : 0.f;
+}
+
+__device__ __forceinline__ float dense_matvec(const DenseView& D, int lane, bool act, float v) {
+  return dense_apply(D.A, D.v, lane, lane < D.n_r && act, v);
+}
+
 // ||[h_d | h_c]||^2 at the step-start state with lambda = l (and lambda_prev = l, as the
 // first Newton iteration snapshots it): the gate between two Newton seeds.
-template <int R>
+template <int R, int GW>
 __device__ __forceinline__ float initial_merit(const FusedParams& P, float* sm, float* lamprev, int lane, int w,
                                                int n_r, int n_c, const float (&l)[R]) {
   const int B = P.m.B;
 #pragma unroll
   for (int k = 0; k < R; ++k) {
-    int i = lane + WAVE * k;
+    int i = lane + GW * k;
     if (i < n_r) lamprev[i] = l[k];
   }
-  dynamics_bodies(P, sm, lane, false);
+  dynamics_bodies(P, sm, lane, false, GW);
   wsync();
   float s = 0.f;
 #pragma unroll
   for (int k = 0; k < R; ++k) {
-    int i = lane + WAVE * k;
+    int i = lane + GW * k;
     if (i < n_r) {
       RowOut o = assemble_row(P, sm, lamprev, w, i, n_c, l[k]);
       s += o.hc * o.hc;
     }
   }
   wsync();
-  for (int e = lane; e < 6 * B; e += WAVE) {
+  for (int e = lane; e < 6 * B; e += GW) {
     float v = sm[(e / 6) * BS + BHD + (e % 6)];
     s += v * v;
   }
-  return wave_sum(s);
+  return group_sum<GW>(s);
 }
 
 // PCR on A x = rhs from x0 = 0 (optim/pcr_solver.py:170-295): the seven vectors live in
-// VGPRs, dots are DPP wave reductions, A is the LDS matrix-free operator.  Returns the
-// iteration count.
-template <int R, bool BJ>
+// VGPRs, dots are DPP group reductions, A is the LDS matrix-free operator (or, DENSE, the
+// assembled matrix of the dense view).  Returns the iteration count.
+template <int R, bool BJ, int GW = WAVE, bool DENSE = false>
 __device__ __forceinline__ int pcr_solve(float* sm, int lane, int B, const BJView& V, const BJRegs<R>& G,
                                          const bool (&act)[R], const int (&b0)[R], const int (&b1)[R],
                                          const sv (&J0)[R], const sv (&J1)[R], const float (&Cr)[R],
                                          const float (&Pi)[R], const float (&rhs)[R], float bsq, float tol_sq,
-                                         float latol_sq, int max_iters, float (&x)[R]) {
+                                         float latol_sq, int max_iters, float (&x)[R],
+                                         const DenseView& D = DenseView{}) {
+  static_assert(!DENSE || R == 1, "the dense operator holds one row per lane");
   float r[R], z[R], p[R], Ap[R], y[R], Az[R];
+  auto matvec = [&](const float (&v)[R], float (&out)[R]) {
+    if constexpr (DENSE) out[0] = dense_matvec(D, lane, act[0], v[0]);
+    else schur_matvec<R, GW>(sm, lane, B, act, b0, b1, J0, J1, Cr, v, out);
+  };
 #pragma unroll
   for (int k = 0; k < R; ++k) { x[k] = 0.f; r[k] = rhs[k]; z[k] = Pi[k] * r[k]; }
   if (BJ) bj_apply<R>(V, act, Pi, G, r, z);
-  schur_matvec<R>(sm, lane, B, act, b0, b1, J0, J1, Cr, z, Az);
+  matvec(z, Az);
   float zAz = 0.f, rsq = 0.f;
 #pragma unroll
   for (int k = 0; k < R; ++k) { p[k] = z[k]; Ap[k] = Az[k]; zAz += z[k] * Az[k]; rsq += r[k] * r[k]; }
-  zAz = wave_sum(zAz);
-  rsq = wave_sum(rsq);
+  group_sum2<GW>(zAz, rsq);
   float target = latol_sq;
   if (bsq * tol_sq > target) target = bsq * tol_sq;
   int it = 0;
@@ -434,56 +550,69 @@ __device__ __forceinline__ int pcr_solve(float* sm, int lane, int B, const BJVie
     if (BJ) bj_apply<R>(V, act, Pi, G, Ap, y);
 #pragma unroll
     for (int k = 0; k < R; ++k) yAp += y[k] * Ap[k];
-    yAp = wave_sum(yAp);
+    yAp = group_sum<GW>(yAp);
     float a = yAp > 0.f ? zAz_old / yAp : 0.f;
+    rsq = 0.f;
 #pragma unroll
-    for (int k = 0; k < R; ++k) { x[k] += a * p[k]; r[k] -= a * Ap[k]; z[k] -= a * y[k]; }
-    schur_matvec<R>(sm, lane, B, act, b0, b1, J0, J1, Cr, z, Az);
+    for (int k = 0; k < R; ++k) {
+      x[k] += a * p[k]; r[k] -= a * Ap[k]; z[k] -= a * y[k];
+      rsq += r[k] * r[k];
+    }
+    matvec(z, Az);
     zAz = 0.f;
 #pragma unroll
     for (int k = 0; k < R; ++k) zAz += z[k] * Az[k];
-    zAz = wave_sum(zAz);
+    group_sum2<GW>(zAz, rsq);  // ||r||^2 of the updated residual rides along
     float bt = zAz_old > 0.f ? zAz / zAz_old : 0.f;
-    rsq = 0.f;
 #pragma unroll
-    for (int k = 0; k < R; ++k) { p[k] = z[k] + bt * p[k]; Ap[k] = Az[k] + bt * Ap[k]; rsq += r[k] * r[k]; }
-    rsq = wave_sum(rsq);
+    for (int k = 0; k < R; ++k) { p[k] = z[k] + bt * p[k]; Ap[k] = Az[k] + bt * Ap[k]; }
     ++it;
     if (!(rsq > target) || it >= max_iters) break;
   }
   return it;
 }
 
+// LDS floats of one world before its dense region: bodies, lambda_prev (rows_cap + 1),
+// the block-Jacobi region, rounded up to a 16-byte boundary
+__host__ __device__ inline int fused_dense_offset(int B, int rows_cap, bool bj) {
+  int off = B * BS + rows_cap + 1 + (bj ? bj_words(rows_cap) : 0);
+  return (off + 3) & ~3;
+}
+
 // ------------------------------------------------------------------------------------
-// The kernel.  R = row slots per lane (rows per world <= 64 R); BJ = per-body-pair
-// block-Jacobi preconditioner (else Jacobi), a template flag so that the Jacobi kernel
-// carries none of the block code's registers.  The R = 1 instance (most worlds) is
-// held to 128 VGPRs so that four waves share a SIMD; the few spilled values cost less
-// than the latency hiding gained (3.68 -> 3.47 ms per 4096-world skid-steer step).
+// The kernel.  GW = lanes per world (64: one world per wave; 32: two worlds per wave, one
+// per half, for worlds of at most 32 rows); R = row slots per lane (rows per world <=
+// GW R); BJ = per-body-pair block-Jacobi preconditioner (else Jacobi), a template flag
+// so that the Jacobi kernel carries none of the block code's registers; DENSE = the PCR
+// operator is the assembled Schur matrix (R = 1 only).  The R = 1 instances (most worlds)
+// are held to 128 VGPRs so that four waves share a SIMD; the few spilled values cost
+// less than the latency hiding gained (3.68 -> 3.47 ms per 4096-world skid-steer step).
 // ------------------------------------------------------------------------------------
 #ifndef OFB_R1_WAVES_PER_EU
 #define OFB_R1_WAVES_PER_EU 4
 #endif
-template <int R, bool BJ>
+template <int R, bool BJ, int GW, bool DENSE>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(R == 1 ? OFB_R1_WAVES_PER_EU : 1)))
 fused_step_kernel(FusedParams P) {
-  extern __shared__ float lds[];
-  const int lane = threadIdx.x & (WAVE - 1);
-  const int wave = threadIdx.x / WAVE;
-  const int idx = blockIdx.x * (blockDim.x / WAVE) + wave;
+  extern __shared__ __align__(16) float lds[];
+  static_assert(GW == 64 || (GW == 32 && R == 1), "two worlds per wave hold one row slot per lane");
+  const int lane = threadIdx.x & (GW - 1);
+  const int grp = threadIdx.x / GW;
+  const int idx = blockIdx.x * (blockDim.x / GW) + grp;
   if (idx >= *P.wcount) return;
   const int w = P.wlist[idx];
-  float* sm = lds + wave * P.lds_per_wave;
+  float* sm = lds + grp * P.lds_per_world;
   const int B = P.m.B;
   const RowLayout& L = P.L;
-  float* lamprev = sm + B * BS;  // compacted row index
+  float* lamprev = sm + B * BS;  // compacted row index, rows_cap + 1 entries
   const float dt = P.dt;
-  const BJView V = bj_view(lamprev + L.Nc + 1, P.rows_cap);
+  float* bj_base = lamprev + P.rows_cap + 1;
+  const BJView V = bj_view(bj_base, P.rows_cap);
   BJRegs<R> G;
   int bj_flags = 0;
 
   // ---- load bodies (load_data, base_engine.py:303-308; engine.py:48-50) ----
-  for (int b = lane; b < B; b += WAVE) {
+  for (int b = lane; b < B; b += GW) {
     float* bb = sm + b * BS;
     size_t k = (size_t)w * B + b;
 #pragma unroll
@@ -503,6 +632,7 @@ fused_step_kernel(FusedParams P) {
   }
   const int n_c = min(P.ccount[w], L.K);
   const int n_r = L.Nj + L.Nctrl + 3 * n_c;
+  const DenseView D = dense_view(sm + fused_dense_offset(B, P.rows_cap, BJ), n_r);
 
   // ---- per-row registers ----
   float lam[R], lamb[R], x[R], Cr[R], Pi[R], rhs[R];
@@ -511,7 +641,7 @@ fused_step_kernel(FusedParams P) {
   bool act[R];
 #pragma unroll
   for (int k = 0; k < R; ++k) {
-    int i = lane + WAVE * k;
+    int i = lane + GW * k;
     float l0 = 0.f;
     if (P.cfg.seed_iterate && i < n_r) l0 = P.lambda_seed[(size_t)w * L.Nc + original_row(L, i, n_c)];
     lam[k] = l0; lamb[k] = l0;
@@ -525,11 +655,11 @@ fused_step_kernel(FusedParams P) {
     float alt[R];
 #pragma unroll
     for (int k = 0; k < R; ++k) {
-      int i = lane + WAVE * k;
+      int i = lane + GW * k;
       alt[k] = i < n_r ? P.lambda_seed2[(size_t)w * L.Nc + original_row(L, i, n_c)] : 0.f;
     }
-    const float ma = initial_merit<R>(P, sm, lamprev, lane, w, n_r, n_c, lam);
-    const float mb = initial_merit<R>(P, sm, lamprev, lane, w, n_r, n_c, alt);
+    const float ma = initial_merit<R, GW>(P, sm, lamprev, lane, w, n_r, n_c, lam);
+    const float mb = initial_merit<R, GW>(P, sm, lamprev, lane, w, n_r, n_c, alt);
     if (mb < ma) {
       seed_flag = FLAG_LEARNED_SEED;
 #pragma unroll
@@ -550,17 +680,17 @@ fused_step_kernel(FusedParams P) {
     // ---- friction-lag snapshot (base_engine.py:441) ----
 #pragma unroll
     for (int k = 0; k < R; ++k) {
-      int i = lane + WAVE * k;
+      int i = lane + GW * k;
       if (i < n_r) lamprev[i] = lam[k];
     }
-    dynamics_bodies(P, sm, lane, true);
+    dynamics_bodies(P, sm, lane, true, GW);
     wsync();
     OFB_PHASE(0)
     // ---- assembly (linear_system.py:140-322) ----
     float hc[R];
 #pragma unroll
     for (int k = 0; k < R; ++k) {
-      int i = lane + WAVE * k;
+      int i = lane + GW * k;
       hc[k] = 0.f;
       if (i < n_r) {
         RowOut o = assemble_row(P, sm, lamprev, w, i, n_c, lam[k]);
@@ -577,7 +707,7 @@ fused_step_kernel(FusedParams P) {
     wsync();
     OFB_PHASE(1)
     // M^-1 h_d per body
-    for (int b = lane; b < B; b += WAVE) {
+    for (int b = lane; b < B; b += GW) {
       float* bb = sm + b * BS;
       stsv(bb + BVV, apply_minv(bb, ldsv(bb + BHD)));
     }
@@ -604,19 +734,21 @@ fused_step_kernel(FusedParams P) {
       }
       bsq += rhs[k] * rhs[k];
     }
-    bsq = wave_sum(bsq);
+    bsq = group_sum<GW>(bsq);
     if (BJ) {
-      if (n_it == 0) bj_structure<R>(V, lane, n_r, B, act, b0, b1, G, bj_flags);
+      if (n_it == 0) bj_structure<R, GW>(V, lane, n_r, B, act, b0, b1, G, bj_flags);
       bj_factor<R>(V, sm, b0, b1, J0, J1, Cr, G, bj_flags);
     }
 
+    if constexpr (DENSE) dense_build(D, sm, lane, act[0], b0[0], b1[0], J0[0], J1[0], Cr[0]);
+
     OFB_PHASE(2)
-    pcr_total += pcr_solve<R, BJ>(sm, lane, B, V, G, act, b0, b1, J0, J1, Cr, Pi, rhs, bsq, tol_sq, latol_sq,
-                                  P.cfg.lin_max_iters, x);
+    pcr_total += pcr_solve<R, BJ, GW, DENSE>(sm, lane, B, V, G, act, b0, b1, J0, J1, Cr, Pi, rhs, bsq, tol_sq,
+                                             latol_sq, P.cfg.lin_max_iters, x, D);
     OFB_PHASE(3)
 
     // ---- velocity recovery + Newton update (linear_system.py:344-377, newton_step.py:22-54) ----
-    for (int e = lane; e < 6 * B; e += WAVE) sm[(e / 6) * BS + BACC + (e % 6)] = 0.f;
+    for (int e = lane; e < 6 * B; e += GW) sm[(e / 6) * BS + BACC + (e % 6)] = 0.f;
     wsync();
 #pragma unroll
     for (int k = 0; k < R; ++k) {
@@ -628,7 +760,7 @@ fused_step_kernel(FusedParams P) {
     }
     wsync();
     if (P.ls_count == 0) {
-      for (int b = lane; b < B; b += WAVE) {
+      for (int b = lane; b < B; b += GW) {
         float* bb = sm + b * BS;
         sv acc = ldsv(bb + BACC), hd = ldsv(bb + BHD);
         sv du = apply_minv(bb, sv{acc.a * dt - hd.a, acc.b * dt - hd.b});
@@ -643,7 +775,7 @@ fused_step_kernel(FusedParams P) {
       // u = u0 + alpha du, q = integrate(u, q_prev), lambda = lambda0 + alpha dlambda, with
       // lambda_prev unchanged (:222-223); the first argmin of ||h||^2 is kept
       // (reductions.py batched_argmin)
-      for (int b = lane; b < B; b += WAVE) {
+      for (int b = lane; b < B; b += GW) {
         float* bb = sm + b * BS;
         sv acc = ldsv(bb + BACC), hd = ldsv(bb + BHD);
         stsv(bb + BVV, apply_minv(bb, sv{acc.a * dt - hd.a, acc.b * dt - hd.b}));
@@ -653,7 +785,7 @@ fused_step_kernel(FusedParams P) {
       float best_a = 0.f, best_m = 0.f;
       for (int si = 0; si < P.ls_count; ++si) {
         const float a = P.ls_steps[si];
-        for (int b = lane; b < B; b += WAVE) {
+        for (int b = lane; b < B; b += GW) {
           float* bb = sm + b * BS;
           sv u0 = ldsv(bb + BU0), du = ldsv(bb + BVV);
           sv u = sv{u0.a + du.a * a, u0.b + du.b * a};
@@ -664,29 +796,29 @@ fused_step_kernel(FusedParams P) {
         float la[R];
 #pragma unroll
         for (int k = 0; k < R; ++k) la[k] = lam[k] + a * x[k];
-        dynamics_bodies(P, sm, lane, false);
+        dynamics_bodies(P, sm, lane, false, GW);
         wsync();
         float ms = 0.f;
 #pragma unroll
         for (int k = 0; k < R; ++k) {
-          int i = lane + WAVE * k;
+          int i = lane + GW * k;
           if (i < n_r) {
             RowOut o = assemble_row(P, sm, lamprev, w, i, n_c, la[k]);
             ms += o.hc * o.hc;
           }
         }
         wsync();
-        for (int e = lane; e < 6 * B; e += WAVE) {
+        for (int e = lane; e < 6 * B; e += GW) {
           float v = sm[(e / 6) * BS + BHD + (e % 6)];
           ms += v * v;
         }
-        ms = wave_sum(ms);
+        ms = group_sum<GW>(ms);
         if (si == 0 || ms < best_m) { best_m = ms; best_a = a; }
         wsync();
       }
 #pragma unroll
       for (int k = 0; k < R; ++k) lam[k] += best_a * x[k];
-      for (int b = lane; b < B; b += WAVE) {
+      for (int b = lane; b < B; b += GW) {
         float* bb = sm + b * BS;
         sv u0 = ldsv(bb + BU0), du = ldsv(bb + BVV);
         sv u = sv{u0.a + du.a * best_a, u0.b + du.b * best_a};
@@ -698,23 +830,23 @@ fused_step_kernel(FusedParams P) {
 
     OFB_PHASE(4)
     // ---- residual at the new iterate (core/residual.py:18-173) ----
-    dynamics_bodies(P, sm, lane, false);
+    dynamics_bodies(P, sm, lane, false, GW);
     wsync();
     float msum = 0.f;
 #pragma unroll
     for (int k = 0; k < R; ++k) {
-      int i = lane + WAVE * k;
+      int i = lane + GW * k;
       if (i < n_r) {
         RowOut o = assemble_row(P, sm, lamprev, w, i, n_c, lam[k]);
         msum += o.hc * o.hc;
       }
     }
     wsync();
-    for (int e = lane; e < 6 * B; e += WAVE) {
+    for (int e = lane; e < 6 * B; e += GW) {
       float v = sm[(e / 6) * BS + BHD + (e % 6)];
       msum += v * v;
     }
-    merit = wave_sum(msum);
+    merit = group_sum<GW>(msum);
     OFB_PHASE(5)
 
     // ---- candidate bookkeeping (history_group.py + backtracking.py:12-95) ----
@@ -724,7 +856,7 @@ fused_step_kernel(FusedParams P) {
     if (take) {
       best_idx = kidx;
       best_merit = merit;
-      for (int e = lane; e < 13 * B; e += WAVE) {
+      for (int e = lane; e < 13 * B; e += GW) {
         int b = e / 13, c = e % 13;
         float* bb = sm + b * BS;
         if (c < 7) bb[BBQ + c] = bb[BQ + c];
@@ -749,7 +881,7 @@ fused_step_kernel(FusedParams P) {
   if (n_it > P.cfg.nr_backtrack_min_iter && best_idx >= 0) {
     picked = best_idx;
     picked_merit = best_merit;
-    for (int e = lane; e < 13 * B; e += WAVE) {
+    for (int e = lane; e < 13 * B; e += GW) {
       int b = e / 13, c = e % 13;
       float* bb = sm + b * BS;
       if (c < 7) bb[BQ + c] = bb[BBQ + c];
@@ -760,7 +892,7 @@ fused_step_kernel(FusedParams P) {
   }
   wsync();
   // ---- outputs (engine.py:68-69) ----
-  for (int e = lane; e < 13 * B; e += WAVE) {
+  for (int e = lane; e < 13 * B; e += GW) {
     int b = e / 13, c = e % 13;
     const float* bb = sm + b * BS;
     size_t k = (size_t)w * B + b;
@@ -771,7 +903,7 @@ fused_step_kernel(FusedParams P) {
   // backtracking does not restore, base_engine.py:441) in the reference row layout
 #pragma unroll
   for (int k = 0; k < R; ++k) {
-    int i = lane + WAVE * k;
+    int i = lane + GW * k;
     if (i < n_r) {
       size_t o = (size_t)w * L.Nc + original_row(L, i, n_c);
       P.lambda_out[o] = lam[k];
@@ -779,7 +911,7 @@ fused_step_kernel(FusedParams P) {
     }
   }
   // contact slots beyond the world's contact count hold lambda = 0
-  for (int c = n_c + lane; c < L.K; c += WAVE) {
+  for (int c = n_c + lane; c < L.K; c += GW) {
     size_t o = (size_t)w * L.Nc;
     P.lambda_out[o + L.off_n + c] = 0.f;
     P.lambda_out[o + L.off_f + 2 * c] = 0.f;
@@ -789,8 +921,8 @@ fused_step_kernel(FusedParams P) {
     P.lambda_prev_out[o + L.off_f + 2 * c + 1] = 0.f;
   }
   int bj_any = 0;
-  if (__any(bj_flags & FLAG_FACTOR_FAILURE)) bj_any |= FLAG_FACTOR_FAILURE;
-  if (__any(bj_flags & FLAG_PAIR_OVERFLOW)) bj_any |= FLAG_PAIR_OVERFLOW;
+  if (group_any<GW>(bj_flags & FLAG_FACTOR_FAILURE)) bj_any |= FLAG_FACTOR_FAILURE;
+  if (group_any<GW>(bj_flags & FLAG_PAIR_OVERFLOW)) bj_any |= FLAG_PAIR_OVERFLOW;
   if (lane == 0) {
     P.st_nr[w] = n_it;
     P.st_pcr[w] = pcr_total;
@@ -805,17 +937,19 @@ fused_step_kernel(FusedParams P) {
 // Host side of one class instance: op 0 launches it, op 1 raises its dynamic-LDS limit
 // to `lds` bytes.  Instances are compiled in several translation units
 // (fb_fused_classes.hip) so that the build runs in parallel.
-template <int R, bool BJ>
+template <int R, bool BJ, int GW, bool DENSE>
 int fused_class_op(int op, const FusedParams& P, int blocks, int threads, size_t lds, hipStream_t stream) {
   if (op == 1)
-    return hipFuncSetAttribute((const void*)fused_step_kernel<R, BJ>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)lds) == hipSuccess ? 0 : -4;
-  hipLaunchKernelGGL((fused_step_kernel<R, BJ>), dim3(blocks), dim3(threads), lds, stream, P);
+    return hipFuncSetAttribute((const void*)fused_step_kernel<R, BJ, GW, DENSE>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) == hipSuccess ? 0 : -4;
+  hipLaunchKernelGGL((fused_step_kernel<R, BJ, GW, DENSE>), dim3(blocks), dim3(threads), lds, stream, P);
   return 0;
 }
 
 // defined in fb_fused_classes.hip, one per (preconditioner, class set) translation unit;
-// return -1 when R is not in that unit's set
+// return -1 when the class code is not in that unit's set.  Class code 0 is the
+// two-worlds-per-wave dense instance (at most 32 rows), code R >= 1 the one-world-per-wave
+// instance with R row slots per lane.
 int fused_class_op_bj0_set0(int R, int op, const FusedParams& P, int blocks, int threads, size_t lds, hipStream_t st);
 int fused_class_op_bj0_set1(int R, int op, const FusedParams& P, int blocks, int threads, size_t lds, hipStream_t st);
 int fused_class_op_bj1_set0(int R, int op, const FusedParams& P, int blocks, int threads, size_t lds, hipStream_t st);

@@ -65,6 +65,68 @@ __device__ __forceinline__ float wave_sum(float v) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
 }
 
+// Lane groups: a fused kernel runs one world per group of G lanes, G = 64 (the wave) or
+// G = 32 (each half of the wave holds its own world).  The group reductions below stay
+// inside a group, so the two worlds of a wave never mix; every lane of an active group
+// must be active (the other group may have retired).
+template <int G>
+__device__ __forceinline__ float group_sum(float v) {
+  static_assert(G == 32 || G == 64, "lane groups are 32 or 64 lanes");
+  if (G == 64) return wave_sum(v);
+  v = v + OFB_DPP(v, 0xB1, 0xF);   // quad_perm [1,0,3,2]
+  v = v + OFB_DPP(v, 0x4E, 0xF);   // quad_perm [2,3,0,1]
+  v = v + OFB_DPP(v, 0x124, 0xF);  // row_ror:4
+  v = v + OFB_DPP(v, 0x128, 0xF);  // row_ror:8
+  v = v + OFB_DPP(v, 0x142, 0xA);  // row_bcast:15: lane 31 holds half 0's sum, lane 63 half 1's
+  const float lo = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 31));
+  const float hi = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+  return (threadIdx.x & 32) ? hi : lo;
+}
+
+// two group sums whose DPP chains interleave (one chain's wait states are the other's
+// issue slots)
+template <int G>
+__device__ __forceinline__ void group_sum2(float& a, float& b) {
+  static_assert(G == 32 || G == 64, "lane groups are 32 or 64 lanes");
+  a = a + OFB_DPP(a, 0xB1, 0xF);
+  b = b + OFB_DPP(b, 0xB1, 0xF);
+  a = a + OFB_DPP(a, 0x4E, 0xF);
+  b = b + OFB_DPP(b, 0x4E, 0xF);
+  a = a + OFB_DPP(a, 0x124, 0xF);
+  b = b + OFB_DPP(b, 0x124, 0xF);
+  a = a + OFB_DPP(a, 0x128, 0xF);
+  b = b + OFB_DPP(b, 0x128, 0xF);
+  a = a + OFB_DPP(a, 0x142, 0xA);
+  b = b + OFB_DPP(b, 0x142, 0xA);
+  if (G == 64) {
+    a = a + OFB_DPP(a, 0x143, 0xC);
+    b = b + OFB_DPP(b, 0x143, 0xC);
+    a = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(a), 63));
+    b = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(b), 63));
+    return;
+  }
+  const float alo = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(a), 31));
+  const float ahi = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(a), 63));
+  const float blo = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(b), 31));
+  const float bhi = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(b), 63));
+  const bool hi = (threadIdx.x & 32) != 0;
+  a = hi ? ahi : alo;
+  b = hi ? bhi : blo;
+}
+
+template <int G>
+__device__ __forceinline__ int group_max_int(int v) {
+  for (int off = G / 2; off >= 1; off >>= 1) v = max(v, __shfl_xor(v, off, G));
+  return v;
+}
+
+template <int G>
+__device__ __forceinline__ bool group_any(bool pred) {
+  const unsigned long long b = __ballot(pred);
+  if (G == 64) return b != 0ull;
+  return ((threadIdx.x & 32) ? (b >> 32) : (b & 0xffffffffull)) != 0ull;
+}
+
 __device__ __forceinline__ v3 ld3(const float* p) { return mk(p[0], p[1], p[2]); }
 __device__ __forceinline__ tf ldtf(const float* p) { return tf{mk(p[0], p[1], p[2]), qt{p[3], p[4], p[5], p[6]}}; }
 __device__ __forceinline__ sv ldsv(const float* p) { return sv{mk(p[0], p[1], p[2]), mk(p[3], p[4], p[5])}; }
@@ -524,10 +586,10 @@ __device__ __forceinline__ int original_row(const RowLayout& L, int i, int n_c) 
 
 // dynamics residual of every body (constraints/dynamics_constraint.py:6-50); lanes over bodies
 template <class PT>
-__device__ __forceinline__ void dynamics_bodies(const PT& P, float* sm, int lane, bool with_inv) {
+__device__ __forceinline__ void dynamics_bodies(const PT& P, float* sm, int lane, bool with_inv, int stride = WAVE) {
   const int B = P.m.B;
   const v3 g = mk(P.m.gravity[0], P.m.gravity[1], P.m.gravity[2]);
-  for (int b = lane; b < B; b += WAVE) {
+  for (int b = lane; b < B; b += stride) {
     float* bb = sm + b * BS;
     qt q = qt{bb[BQ + 3], bb[BQ + 4], bb[BQ + 5], bb[BQ + 6]};
     if (with_inv) {

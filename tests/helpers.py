@@ -15,7 +15,8 @@ def config(nr_iters=20, atol=1e-5, pcr_iters=16, tol=1e-5, patol=1e-5, K=16, ter
         nr=NewtonRaphsonConfig(max_iters=nr_iters, atol=atol),
         linear=LinearSolverConfig(max_iters=pcr_iters, tol=tol, atol=patol, preconditioner_type=precond),
         contacts=ContactsConfig(max_per_world=K),
-        execution=ExecutionConfig(termination=termination, path=kw.pop("path", "fused")),
+        execution=ExecutionConfig(termination=termination, path=kw.pop("path", "fused"),
+                                  pair_small_worlds=kw.pop("pair_small_worlds", True)),
         **kw)
 
 

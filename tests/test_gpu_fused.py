@@ -88,6 +88,30 @@ def test_box_stack_parity(oracle_mod, gpu_device):
     assert np.mean(nr_diff <= 1) >= 0.9 and nr_diff.max() <= 3
 
 
+@pytest.mark.parametrize("pair", [True, False])
+def test_mixed_row_classes_parity(oracle_mod, gpu_device, pair):
+    """One batch with worlds of at most 32 rows (a single cube on the ground: 12 rows)
+    and of more (three stacked cubes: up to 36 rows).  With pair_small_worlds the small
+    worlds run two per wavefront on the assembled Schur matrix and the others one per
+    wavefront on the matrix-free operator; without it all run one per wavefront.  Both
+    layouts must match the oracle."""
+    model, state, control = scenes.box_stack(32, 3, seed=5)
+    st = {k: np.asarray(v, np.float32).copy() for k, v in state.items()}
+    q = st["body_q"].reshape(32, 3, 7)
+    q[::2, 1:, 2] += 1.0  # every other world: the upper cubes start out of contact
+    st["body_q"] = q.reshape(-1, 7)
+    cfg = config(K=16, pair_small_worlds=pair)
+    col = collide.PrimitiveCollider(model, gap=0.02, max_contacts_per_world=16)
+    c = col.collide(torch.as_tensor(st["body_q"]))
+    c = {k: (v.numpy() if hasattr(v, "numpy") else v) for k, v in c.items()}
+    rows = 3 * oracle_mod.step(model, cfg, st, control, c, 0.05, precision=32)["contact_count"]
+    print(f"[mixed classes] pair={pair} rows per world {np.bincount(rows)}", flush=True)
+    assert (rows <= 32).any() and (rows > 32).any()
+    nr_diff, conv = _run_compare(oracle_mod, model, cfg, st, control, steps=3, dt=0.05, device=gpu_device, K=16)
+    assert conv.mean() > 0.9
+    assert np.mean(nr_diff <= 1) >= 0.9 and nr_diff.max() <= 3
+
+
 @pytest.mark.parametrize("nr_iters", [1, 2])
 def test_skid_steer_fixed_newton_iterations(oracle_mod, gpu_device, nr_iters):
     """Skid-steer worlds after 1-2 Newton iterations from identical inputs (friction enters
@@ -122,7 +146,10 @@ def test_skid_steer_rollout_parity(oracle_mod, gpu_device):
     """Full Newton loop (up to 20 iterations) on skid-steer worlds.  The GPU result must
     be another FP32 rounding of the same algorithm: its per-world deviation from the FP32
     oracle is distributed like the FP32 oracle's deviation from the FP64 oracle (median
-    within 3x, maximum within 4x, plus 1e-4 of scale)."""
+    within 3x, maximum within 4x, plus 1e-4 of scale).  A world that the GPU solves to
+    ||h||^2 <= atol^2 while the FP32 oracle stops at the iteration cap holds a solution
+    the oracle did not reach (the capped iterate is the best of an unconverged sequence),
+    so it is left out of the maximum; at most a tenth of the worlds may be such."""
     model, state, control = scenes.skid_steer(64, seed=1)
     W = model.num_worlds
     eng = _engine(model, config(K=128), gpu_device)
@@ -135,11 +162,14 @@ def test_skid_steer_rollout_parity(oracle_mod, gpu_device):
         g = _gpu_step(eng, st32, control, c, 0.05, gpu_device)
         o32 = oracle_mod.step(model, config(K=128), st32, control, c, 0.05, precision=32)
         o64 = oracle_mod.step(model, config(K=128), st32, control, c, 0.05, precision=64)
+        atol_sq = config(K=128).nr.atol ** 2
+        better = (g["merit"] <= atol_sq) & (o32["merit"] > atol_sq)
+        assert better.mean() <= 0.1, better.mean()
         for key, scale in (("body_q", 1.0), ("body_qd", G * 0.05)):
             eg = np.abs(g[key] - o32[key]).reshape(W, -1).max(1)
             ep = np.abs(o32[key] - o64[key]).reshape(W, -1).max(1)
             assert np.median(eg) <= 3.0 * np.median(ep) + 1e-4 * scale, (key, np.median(eg), np.median(ep))
-            assert eg.max() <= 4.0 * ep.max() + 1e-4 * scale, (key, eg.max(), ep.max())
+            assert eg[~better].max() <= 4.0 * ep.max() + 1e-4 * scale, (key, eg[~better].max(), ep.max())
         st = dict(st, body_q=o32["body_q"], body_qd=o32["body_qd"])
 
 
