@@ -232,6 +232,11 @@ int ofb_destroy(ofb_solver* s) {
     if (p) (void)hipFree(p);
   if (s->plain) plain_destroy(s);
   if (s->events_ready) { (void)hipEventDestroy(s->ev_solve0); (void)hipEventDestroy(s->ev_solve1); }
+  if (s->side_ready) {
+    (void)hipEventDestroy(s->ev_fork);
+    (void)hipEventDestroy(s->ev_join);
+    (void)hipStreamDestroy(s->side_stream);
+  }
   if (s->cls_counts) (void)hipFree(s->cls_counts);
   if (s->cls_lists) (void)hipFree(s->cls_lists);
   delete s;

@@ -1,17 +1,19 @@
 // fb_fused.hip -- host side of the fused persistent solver: per-step bucketing of worlds
 // into row classes, LDS sizing, and the class launches (kernel in fb_fused_kernel.h).
 
+#include <algorithm>
+
 #include "fb_adjoint_kernel.h"
 
 namespace ofb {
 
 // Row classes: a world with n_r active rows runs the instance with R = the smallest
 // class >= ceil(n_r / 64) slots per lane, so small worlds do not pay for the padded
-// row count's registers; with pair_small_worlds, worlds of at most 32 rows take class
-// code 0 (two worlds per wavefront, dense Schur operator).  Worlds are bucketed on the
-// device every step.
+// row count's registers.  Worlds of at most 64 rows (R = 1) solve PCR on the assembled
+// (dense) Schur matrix; with pair_small_worlds, those of at most 32 rows take class code
+// 0 and run two per wavefront.  Worlds are bucketed on the device every step.
 constexpr int kNumClasses = 12;
-constexpr int kPairRows = kDenseLd;
+constexpr int kPairRows = 32;
 __constant__ int kClassR[kNumClasses] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 16};
 static const int kClassRHost[kNumClasses] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 16};
 
@@ -19,33 +21,80 @@ __global__ void reset_classes_kernel(int32_t* counts) {
   if (threadIdx.x < kNumClasses) counts[threadIdx.x] = 0;
 }
 
-__global__ void classify_worlds_kernel(int W, const int32_t* __restrict__ ccount, int K, int base_rows,
-                                       int pair_rows, int32_t* __restrict__ counts, int32_t* __restrict__ lists) {
-  int w = blockIdx.x * blockDim.x + threadIdx.x;
-  if (w >= W) return;
-  int n_r = base_rows + 3 * min(ccount[w], K);
-  int c = 0;
-  if (n_r > pair_rows) {
-    int r = max(1, (n_r + WAVE - 1) / WAVE);
-    c = 1;
-    while (c < kNumClasses - 1 && kClassR[c] < r) ++c;
+// One pass over the worlds: each wave ranks its worlds per class with ballots, the
+// block sums its waves in LDS, and one global atomic per class and block reserves the
+// block's range of each list (one atomic per world on a single counter serialises).
+// The order of a class list is not significant: every world is solved on its own.
+__global__ void __launch_bounds__(256) classify_worlds_kernel(int W, const int32_t* __restrict__ ccount, int K,
+                                                              int base_rows, int pair_rows,
+                                                              int32_t* __restrict__ counts,
+                                                              int32_t* __restrict__ lists) {
+  __shared__ int wcnt[4][kNumClasses];  // worlds per (wave, class)
+  __shared__ int bbase[kNumClasses];    // block's base in each list
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
+  int c = -1;
+  if (w < W) {
+    int n_r = base_rows + 3 * min(ccount[w], K);
+    c = 0;
+    if (n_r > pair_rows) {
+      int r = max(1, (n_r + WAVE - 1) / WAVE);
+      c = 1;
+      while (c < kNumClasses - 1 && kClassR[c] < r) ++c;
+    }
   }
-  int slot = atomicAdd(counts + c, 1);
-  lists[(size_t)c * W + slot] = w;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  int rank = 0;
+  for (int k = 0; k < kNumClasses; ++k) {
+    const unsigned long long m = __ballot(c == k);
+    if (c == k) rank = __popcll(m & below);
+    if (lane == 0) wcnt[wave][k] = __popcll(m);
+  }
+  __syncthreads();
+  if (threadIdx.x < kNumClasses) {
+    const int k = threadIdx.x;
+    int tot = 0;
+    for (int v = 0; v < (int)(blockDim.x / WAVE); ++v) tot += wcnt[v][k];
+    bbase[k] = tot > 0 ? atomicAdd(counts + k, tot) : 0;
+  }
+  __syncthreads();
+  if (c >= 0) {
+    int off = bbase[c];
+    for (int v = 0; v < wave; ++v) off += wcnt[v][c];
+    lists[(size_t)c * W + off + rank] = w;
+  }
 }
 
 // lanes per world and row capacity of a class code
 static int class_lanes(int code) { return code == 0 ? kPairRows : WAVE; }
 static int class_rows(int code) { return code == 0 ? kPairRows : WAVE * code; }
 
-// LDS floats per world of the forward kernel (a multiple of 4, so that every world's
-// region stays 16-byte aligned): bodies, lambda_prev, block-Jacobi, dense operator
-static int lds_words(const ofb_solver* s, int code) {
+// LDS layout of one world of a class: a world of the class has at most
+// (rows_cap - joint and control rows) / 3 contacts
+static FusedLayout class_layout(const ofb_solver* s, int code) {
   const bool bj = s->cfg.preconditioner == OFB_PRECOND_PER_BODY_PAIR;
   const int cap = class_rows(code);
-  int words = code == 0 ? fused_dense_offset(s->m.B, cap, bj) + dense_words()
-                        : s->m.B * BS + cap + 1 + (bj ? bj_words(cap) : 0);
-  return (words + 3) & ~3;
+  const int base = s->L.Nj + s->L.Nctrl;
+  const int cs_cap = std::min(s->L.K, std::max(0, (cap - base) / 3));
+  return fused_layout(s->m.B, s->m.J, cap, cs_cap, bj, code <= 1);
+}
+static int lds_words(const ofb_solver* s, int code) { return class_layout(s, code).words; }
+
+// worlds per workgroup of a class: at most the tuned number of waves' worth, a whole
+// number of waves, within 64 KB of LDS per workgroup, and among those the choice that
+// keeps the most worlds resident in a CU's 160 KB (larger workgroups on ties)
+static int choose_wpb(const ofb_solver* s, int code) {
+  const int per_wave = WAVE / class_lanes(code);
+  const size_t world_bytes = (size_t)lds_words(s, code) * sizeof(float);
+  int best = per_wave, best_res = -1;
+  for (int n = 1; n <= s->fused_worlds_per_block; ++n) {
+    const int wpb = n * per_wave;
+    const size_t bytes = wpb * world_bytes;
+    if (bytes > 65536 && n > 1) break;
+    const int res = (int)(163840 / bytes) * wpb;
+    if (res >= best_res) { best_res = res; best = wpb; }
+  }
+  return best;
 }
 
 // LDS floats per wave of the adjoint kernel: the forward's bodies and lambda_prev, the
@@ -79,13 +128,9 @@ static int launch_fused(int code, FusedParams P, const ofb_solver* s, int cls, h
   P.wlist = s->cls_lists + (size_t)cls * P.m.W;
   P.wcount = s->cls_counts + cls;
   P.rows_cap = class_rows(code);
-  P.lds_per_world = lds_words(s, code);
-  const int per_wave = WAVE / class_lanes(code);
-  // worlds per workgroup: as many as fit in 64 KB of LDS, at most the tuned number of
-  // waves' worth
-  int wpb = s->fused_worlds_per_block * per_wave;
-  while (wpb > per_wave && (size_t)wpb * P.lds_per_world * sizeof(float) > 65536) wpb -= per_wave;
-  size_t lds_bytes = (size_t)wpb * P.lds_per_world * sizeof(float);
+  P.lay = class_layout(s, code);
+  const int wpb = s->cls_wpb[cls];
+  size_t lds_bytes = (size_t)wpb * P.lay.words * sizeof(float);
   int blocks = (P.m.W + wpb - 1) / wpb;
   return class_op(code, 0, P, blocks, wpb * class_lanes(code), lds_bytes, stream,
                   s->cfg.preconditioner == OFB_PRECOND_PER_BODY_PAIR);
@@ -124,10 +169,28 @@ static bool class_used(const ofb_solver* s, int c) {
 
 int fused_classes_alloc(ofb_solver* s) {
   for (int c = 0; c < kNumClasses; ++c) {
+    s->cls_wpb[c] = 1;
     if (!class_used(s, c)) continue;
     int rc = allow_lds(kClassRHost[c], s);
     if (rc) return rc;
+    s->cls_wpb[c] = choose_wpb(s, kClassRHost[c]);
   }
+  if (hipStreamCreateWithFlags(&s->side_stream, hipStreamNonBlocking) != hipSuccess) {
+    set_error("hipStreamCreate failed");
+    return -4;
+  }
+  if (hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming) != hipSuccess) {
+    (void)hipStreamDestroy(s->side_stream);
+    set_error("hipEventCreate failed");
+    return -4;
+  }
+  if (hipEventCreateWithFlags(&s->ev_join, hipEventDisableTiming) != hipSuccess) {
+    (void)hipEventDestroy(s->ev_fork);
+    (void)hipStreamDestroy(s->side_stream);
+    set_error("hipEventCreate failed");
+    return -4;
+  }
+  s->side_ready = 1;
   if (hipMalloc((void**)&s->cls_counts, kNumClasses * sizeof(int32_t)) != hipSuccess ||
       hipMalloc((void**)&s->cls_lists, (size_t)kNumClasses * s->m.W * sizeof(int32_t)) != hipSuccess) {
     set_error("hipMalloc failed (world classes)");
@@ -166,12 +229,32 @@ int fused_step(ofb_solver* s, const ofb_state_in* in, const ofb_state_out* out, 
   hipLaunchKernelGGL(classify_worlds_kernel, dim3((W + 255) / 256), dim3(256), 0, stream, W, s->ccount, s->L.K,
                      s->L.Nj + s->L.Nctrl, s->cfg.pair_small_worlds ? kPairRows : -1, s->cls_counts,
                      s->cls_lists);
-  for (int c = 0; c < kNumClasses; ++c) {
+  // The one-world-per-wave classes hold few worlds on a typical batch, but each of their
+  // waves takes a whole Newton solve: they run on a side stream, launched first so that
+  // their waves are resident beside the paired class's, instead of after it.
+  if (hipEventRecord(s->ev_fork, stream) != hipSuccess ||
+      hipStreamWaitEvent(s->side_stream, s->ev_fork, 0) != hipSuccess) {
+    set_error("fused step: stream fork failed");
+    return -4;
+  }
+  for (int c = 1; c < kNumClasses; ++c) {
     if (!class_used(s, c)) continue;  // no world can fall in this class
-    if (launch_fused(kClassRHost[c], P, s, c, stream) != 0) {
+    if (launch_fused(kClassRHost[c], P, s, c, s->side_stream) != 0) {
       set_error("no fused kernel instance for class code %d", kClassRHost[c]);
       return -4;
     }
+  }
+  if (hipEventRecord(s->ev_join, s->side_stream) != hipSuccess) {
+    set_error("fused step: stream join failed");
+    return -4;
+  }
+  if (class_used(s, 0) && launch_fused(kClassRHost[0], P, s, 0, stream) != 0) {
+    set_error("no fused kernel instance for class code %d", kClassRHost[0]);
+    return -4;
+  }
+  if (hipStreamWaitEvent(stream, s->ev_join, 0) != hipSuccess) {
+    set_error("fused step: stream join failed");
+    return -4;
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) { set_error("fused kernel launch failed: %s", hipGetErrorString(e)); return -4; }

@@ -47,6 +47,15 @@ __device__ unsigned long long g_phase_cycles[8];
 #define OFB_PHASE_FLUSH
 #endif
 
+// LDS layout of one world (floats; regions that are read as quads start on 16-byte
+// boundaries): bodies, lambda_prev (rows_cap + 1, compacted row index), one static record
+// per joint, one per contact slot of the class (cs_cap), the block-Jacobi region, the
+// dense Schur operator.  `words` is a multiple of 4, so every world's base stays aligned.
+struct FusedLayout {
+  int lamprev, js, cs, bj, dense, words;
+};
+__host__ __device__ inline int up4(int x) { return (x + 3) & ~3; }
+
 struct FusedParams {
   DeviceModel m;
   RowLayout L;
@@ -71,7 +80,7 @@ struct FusedParams {
   float* lambda_prev_out;  // lambda_prev_iter as the step leaves it (input of the adjoint)
   int32_t *st_nr, *st_pcr, *st_best, *st_flags;
   float* st_merit;
-  int lds_per_world;       // floats (a multiple of 4)
+  FusedLayout lay;        // LDS regions of one world (floats)
   const int32_t* wlist;   // worlds of this launch's row class
   const int32_t* wcount;  // number of them
   int rows_cap;           // G R: row capacity of this launch's class
@@ -374,30 +383,37 @@ __device__ __forceinline__ void schur_matvec(float* sm, int lane, int B, const b
 }
 
 // ---------------------------------------------------------------------------------
-// Dense Schur operator for worlds of at most 32 rows (one row per lane, R = 1).  Once per
-// Newton iteration the group assembles A = J M^-1 J^T + diag(C + reg) (the operator of
-// optim/system_operator.py:36-124 written out) into LDS, column j at A + 32 j, so that a
-// PCR matvec is one burst of 32 conflict-free column reads against a broadcast of v,
-// with a single wait: no LDS atomics and no per-body pass inside the Krylov loop, whose
-// cost is LDS latency rather than bandwidth.  Row i of A is built by lane i from every
-// row j's M^-1 J_j^T (two bodies, staged in LDS) and body indices; rows that share no
-// body give exact zeros, inactive rows and the padding up to 32 give zero rows and
-// columns (their PCR entries are zero, as in the matrix-free operator).
-// LDS: A[32 * 32], rowd[16 * 32] (M^-1 J0^T 6, M^-1 J1^T 6, b0 bits, b1 bits, pad), v[32].
+// Dense Schur operator for worlds of at most LD rows (LD = 32: two worlds per wave; 64:
+// one world per wave; one row per lane, R = 1).  Once per Newton iteration the group
+// assembles A = J M^-1 J^T + diag(C + reg) (the operator of optim/system_operator.py:
+// 36-124 written out), lane i holding row i in LD VGPRs, so a PCR matvec is one LDS round
+// trip for the broadcast of v followed by LD FMAs: no LDS atomics and no per-body pass
+// inside the Krylov loop, and the LDS pipe, which the worlds of a CU otherwise saturate,
+// carries only v.  Row i is built by lane i from every row j's M^-1 J_j^T (two bodies,
+// staged in LDS) and body indices; rows that share no body give exact zeros, inactive
+// rows and the padding up to LD give zero rows and columns (their PCR entries are zero,
+// as in the matrix-free operator).  Columns are handled in blocks of 8, and blocks past
+// the world's rows are skipped.
+// LDS: rowd[16 LD] (M^-1 J0^T 6, M^-1 J1^T 6, b0 bits, b1 bits, pad), v[LD].
 // ---------------------------------------------------------------------------------
-constexpr int kDenseLd = 32;
-__host__ __device__ inline int dense_words() { return kDenseLd * kDenseLd + 17 * kDenseLd; }
+__host__ __device__ inline int dense_words(int ld) { return 17 * ld; }
 
 struct DenseView {
-  float* A;
   float* rowd;
   float* v;
   int n_r;
 };
 
+template <int LD>
 __device__ __forceinline__ DenseView dense_view(float* base, int n_r) {
-  return DenseView{base, base + kDenseLd * kDenseLd, base + kDenseLd * kDenseLd + 16 * kDenseLd, n_r};
+  return DenseView{base, base + 16 * LD, n_r};
 }
+
+// row i of A, in registers
+template <int LD>
+struct DenseRow {
+  float a[LD];
+};
 
 // one entry A_ij from row j's staged data (x0..x3) and row i's Jacobian and bodies
 __device__ __forceinline__ float dense_entry(const float4& x0, const float4& x1, const float4& x2, const float4& x3,
@@ -413,12 +429,12 @@ __device__ __forceinline__ float dense_entry(const float4& x0, const float4& x1,
   return a;
 }
 
-__device__ __forceinline__ void dense_stage(float* __restrict__ rowd, const float* sm, int lane, bool on, int b0,
-                                            int b1, const sv& J0, const sv& J1) {
+__device__ __forceinline__ void dense_stage(float* __restrict__ rowd, int lane, bool on, int b0, int b1,
+                                            const sv& W0, const sv& W1) {
   sv w0 = svzero(), w1 = svzero();
-  int c0 = -2, c1 = -2;  // an inactive row matches no body
-  if (on && b0 >= 0) { w0 = apply_minv(sm + b0 * BS, J0); c0 = b0; }
-  if (on && b1 >= 0) { w1 = apply_minv(sm + b1 * BS, J1); c1 = b1; }
+  int c0 = -2, c1 = -2;  // an inactive row (or a lane past the world's rows) matches no body
+  if (on && b0 >= 0) { w0 = W0; c0 = b0; }
+  if (on && b1 >= 0) { w1 = W1; c1 = b1; }
   float4* r = reinterpret_cast<float4*>(rowd + 16 * lane);
   r[0] = make_float4(w0.a.x, w0.a.y, w0.a.z, w0.b.x);
   r[1] = make_float4(w0.b.y, w0.b.z, w1.a.x, w1.a.y);
@@ -426,69 +442,66 @@ __device__ __forceinline__ void dense_stage(float* __restrict__ rowd, const floa
   r[3] = make_float4(__int_as_float(c0), __int_as_float(c1), 0.f, 0.f);
 }
 
-// rows j, j + 1 per pass: their staged data is loaded in one burst before the two
-// stores (A and rowd do not alias)
-__device__ __forceinline__ void dense_fill(float* __restrict__ A, const float* __restrict__ rowd, int lane, int n_r,
-                                           bool on, int b0, int b1, const sv& J0, const sv& J1, float Cr) {
-  const int m0 = (on && b0 >= 0) ? b0 : -3, m1 = (on && b1 >= 0) ? b1 : -3;
-  const float4* r = reinterpret_cast<const float4*>(rowd);
-  for (int j = 0; j < kDenseLd; j += 2) {
-    float a0 = 0.f, a1 = 0.f;
-    if (j < n_r) {
-      const float4 x0 = r[4 * j], x1 = r[4 * j + 1], x2 = r[4 * j + 2], x3 = r[4 * j + 3];
-      const float4 y0 = r[4 * j + 4], y1 = r[4 * j + 5], y2 = r[4 * j + 6], y3 = r[4 * j + 7];
-      a0 = dense_entry(x0, x1, x2, x3, m0, m1, J0, J1);
-      a1 = j + 1 < n_r ? dense_entry(y0, y1, y2, y3, m0, m1, J0, J1) : 0.f;
-      if (j == lane && on) a0 += Cr;
-      if (j + 1 == lane && on) a1 += Cr;
-    }
-    A[j * kDenseLd + lane] = a0;
-    A[(j + 1) * kDenseLd + lane] = a1;
-  }
-}
-
-__device__ __forceinline__ void dense_build(const DenseView& D, const float* sm, int lane, bool act, int b0, int b1,
-                                            const sv& J0, const sv& J1, float Cr) {
+// W0, W1: M^-1 J0^T, M^-1 J1^T of the lane's row (as computed for the Jacobi diagonal).
+// Every lane of the group stages its row, so rows past n_r inside a block carry bodies
+// that match nothing.
+template <int LD>
+__device__ __forceinline__ void dense_build(const DenseView& D, DenseRow<LD>& A, int lane, bool act, int b0, int b1,
+                                            const sv& J0, const sv& J1, const sv& W0, const sv& W1, float Cr) {
   const bool on = lane < D.n_r && act;
-  dense_stage(D.rowd, sm, lane, on, b0, b1, J0, J1);
+  dense_stage(D.rowd, lane, on, b0, b1, W0, W1);
   wsync();
-  dense_fill(D.A, D.rowd, lane, D.n_r, on, b0, b1, J0, J1, Cr);
+  const int m0 = (on && b0 >= 0) ? b0 : -3, m1 = (on && b1 >= 0) ? b1 : -3;
+  const float4* r = reinterpret_cast<const float4*>(D.rowd);
+#pragma unroll
+  for (int j0 = 0; j0 < LD; j0 += 8) {
+    if (j0 < D.n_r) {
+#pragma unroll
+      for (int j = j0; j < j0 + 8; ++j) {
+        float a = dense_entry(r[4 * j], r[4 * j + 1], r[4 * j + 2], r[4 * j + 3], m0, m1, J0, J1);
+        if (j == lane && on) a += Cr;
+        A.a[j] = a;
+      }
+    } else {
+#pragma unroll
+      for (int j = j0; j < j0 + 8; ++j) A.a[j] = 0.f;
+    }
+  }
   wsync();
 }
 
-// out_i = sum_j A_ij v_j: v goes through LDS once, then all 32 column entries and the 8
-// broadcast quads of v are in flight together
-__device__ __forceinline__ float dense_apply(const float* __restrict__ A, float* __restrict__ vb, int lane, bool on,
-                                             float v) {
-  vb[lane] = on ? v : 0.f;
+// out_i = sum_j A_ij v_j: v goes through LDS once, its broadcast quads in flight together
+template <int LD>
+__device__ __forceinline__ float dense_matvec(const DenseView& D, const DenseRow<LD>& A, int lane, bool act,
+                                              float v) {
+  const bool on = lane < D.n_r && act;
+  D.v[lane] = on ? v : 0.f;
   wsync();
-  const float4* v4 = reinterpret_cast<const float4*>(vb);
-  float4 x[kDenseLd / 4];
-#pragma unroll
-  for (int q = 0; q < kDenseLd / 4; ++q) x[q] = v4[q];
-  float a[kDenseLd];
-#pragma unroll
-  for (int j = 0; j < kDenseLd; ++j) a[j] = A[j * kDenseLd + lane];
+  const float4* v4 = reinterpret_cast<const float4*>(D.v);
   float o[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-  for (int q = 0; q < kDenseLd / 4; ++q) {
-    o[0] += a[4 * q] * x[q].x;
-    o[1] += a[4 * q + 1] * x[q].y;
-    o[2] += a[4 * q + 2] * x[q].z;
-    o[3] += a[4 * q + 3] * x[q].w;
+  for (int j0 = 0; j0 < LD; j0 += 8) {
+    if (j0 < D.n_r) {
+      const float4 x = v4[j0 / 4], y = v4[j0 / 4 + 1];
+      o[0] += A.a[j0] * x.x;
+      o[1] += A.a[j0 + 1] * x.y;
+      o[2] += A.a[j0 + 2] * x.z;
+      o[3] += A.a[j0 + 3] * x.w;
+      o[0] += A.a[j0 + 4] * y.x;
+      o[1] += A.a[j0 + 5] * y.y;
+      o[2] += A.a[j0 + 6] * y.z;
+      o[3] += A.a[j0 + 7] * y.w;
+    }
   }
   wsync();
   return on ? (o[0] + o[1]) + (o[2] + o[3]) : 0.f;
 }
 
-__device__ __forceinline__ float dense_matvec(const DenseView& D, int lane, bool act, float v) {
-  return dense_apply(D.A, D.v, lane, lane < D.n_r && act, v);
-}
-
 // ||[h_d | h_c]||^2 at the step-start state with lambda = l (and lambda_prev = l, as the
 // first Newton iteration snapshots it): the gate between two Newton seeds.
 template <int R, int GW>
-__device__ __forceinline__ float initial_merit(const FusedParams& P, float* sm, float* lamprev, int lane, int w,
+__device__ __forceinline__ float initial_merit(const FusedParams& P, float* sm, float* lamprev, const float* js,
+                                               const float* cs, const int (&rj)[R], const int (&rk)[R], int lane,
                                                int n_r, int n_c, const float (&l)[R]) {
   const int B = P.m.B;
 #pragma unroll
@@ -503,7 +516,7 @@ __device__ __forceinline__ float initial_merit(const FusedParams& P, float* sm, 
   for (int k = 0; k < R; ++k) {
     int i = lane + GW * k;
     if (i < n_r) {
-      RowOut o = assemble_row(P, sm, lamprev, w, i, n_c, l[k]);
+      RowOut o = assemble_row_cached(P, sm, lamprev, js, cs, i, n_c, rj[k], rk[k], l[k]);
       s += o.hc * o.hc;
     }
   }
@@ -524,11 +537,12 @@ __device__ __forceinline__ int pcr_solve(float* sm, int lane, int B, const BJVie
                                          const sv (&J0)[R], const sv (&J1)[R], const float (&Cr)[R],
                                          const float (&Pi)[R], const float (&rhs)[R], float bsq, float tol_sq,
                                          float latol_sq, int max_iters, float (&x)[R],
-                                         const DenseView& D = DenseView{}) {
+                                         const DenseView& D = DenseView{},
+                                         const DenseRow<GW>& Ad = DenseRow<GW>{}) {
   static_assert(!DENSE || R == 1, "the dense operator holds one row per lane");
   float r[R], z[R], p[R], Ap[R], y[R], Az[R];
   auto matvec = [&](const float (&v)[R], float (&out)[R]) {
-    if constexpr (DENSE) out[0] = dense_matvec(D, lane, act[0], v[0]);
+    if constexpr (DENSE) out[0] = dense_matvec(D, Ad, lane, act[0], v[0]);
     else schur_matvec<R, GW>(sm, lane, B, act, b0, b1, J0, J1, Cr, v, out);
   };
 #pragma unroll
@@ -572,11 +586,15 @@ __device__ __forceinline__ int pcr_solve(float* sm, int lane, int B, const BJVie
   return it;
 }
 
-// LDS floats of one world before its dense region: bodies, lambda_prev (rows_cap + 1),
-// the block-Jacobi region, rounded up to a 16-byte boundary
-__host__ __device__ inline int fused_dense_offset(int B, int rows_cap, bool bj) {
-  int off = B * BS + rows_cap + 1 + (bj ? bj_words(rows_cap) : 0);
-  return (off + 3) & ~3;
+__host__ __device__ inline FusedLayout fused_layout(int B, int J, int rows_cap, int cs_cap, bool bj, bool dense) {
+  FusedLayout l;
+  l.lamprev = B * BS;
+  l.js = up4(l.lamprev + rows_cap + 1);
+  l.cs = l.js + JS_WORDS * J;
+  l.bj = l.cs + CS_WORDS * cs_cap;
+  l.dense = up4(l.bj + (bj ? bj_words(rows_cap) : 0));
+  l.words = up4(l.dense + (dense ? dense_words(rows_cap) : 0));
+  return l;
 }
 
 // ------------------------------------------------------------------------------------
@@ -584,15 +602,20 @@ __host__ __device__ inline int fused_dense_offset(int B, int rows_cap, bool bj) 
 // per half, for worlds of at most 32 rows); R = row slots per lane (rows per world <=
 // GW R); BJ = per-body-pair block-Jacobi preconditioner (else Jacobi), a template flag
 // so that the Jacobi kernel carries none of the block code's registers; DENSE = the PCR
-// operator is the assembled Schur matrix (R = 1 only).  The R = 1 instances (most worlds)
-// are held to 128 VGPRs so that four waves share a SIMD; the few spilled values cost
-// less than the latency hiding gained (3.68 -> 3.47 ms per 4096-world skid-steer step).
+// operator is the assembled Schur matrix (R = 1 only, a matrix row of GW floats per
+// lane).  The dense instances hold the matrix row in VGPRs and are allowed 256 of them
+// (two waves per SIMD, no spills); a matrix-free R = 1 instance would be held to 128 so
+// that four waves share a SIMD.
 // ------------------------------------------------------------------------------------
 #ifndef OFB_R1_WAVES_PER_EU
 #define OFB_R1_WAVES_PER_EU 4
 #endif
+#ifndef OFB_DENSE_WAVES_PER_EU
+#define OFB_DENSE_WAVES_PER_EU 2
+#endif
 template <int R, bool BJ, int GW, bool DENSE>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(R == 1 ? OFB_R1_WAVES_PER_EU : 1)))
+__global__ void __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(R == 1 ? (DENSE ? OFB_DENSE_WAVES_PER_EU : OFB_R1_WAVES_PER_EU) : 1)))
 fused_step_kernel(FusedParams P) {
   extern __shared__ __align__(16) float lds[];
   static_assert(GW == 64 || (GW == 32 && R == 1), "two worlds per wave hold one row slot per lane");
@@ -601,13 +624,14 @@ fused_step_kernel(FusedParams P) {
   const int idx = blockIdx.x * (blockDim.x / GW) + grp;
   if (idx >= *P.wcount) return;
   const int w = P.wlist[idx];
-  float* sm = lds + grp * P.lds_per_world;
+  float* sm = lds + grp * P.lay.words;
   const int B = P.m.B;
   const RowLayout& L = P.L;
-  float* lamprev = sm + B * BS;  // compacted row index, rows_cap + 1 entries
+  float* lamprev = sm + P.lay.lamprev;  // compacted row index, rows_cap + 1 entries
+  float* js = sm + P.lay.js;            // static record of each joint
+  float* cs = sm + P.lay.cs;            // static record of each contact
   const float dt = P.dt;
-  float* bj_base = lamprev + P.rows_cap + 1;
-  const BJView V = bj_view(bj_base, P.rows_cap);
+  const BJView V = bj_view(sm + P.lay.bj, P.rows_cap);
   BJRegs<R> G;
   int bj_flags = 0;
 
@@ -632,12 +656,20 @@ fused_step_kernel(FusedParams P) {
   }
   const int n_c = min(P.ccount[w], L.K);
   const int n_r = L.Nj + L.Nctrl + 3 * n_c;
-  const DenseView D = dense_view(sm + fused_dense_offset(B, P.rows_cap, BJ), n_r);
+  const DenseView D = dense_view<GW>(sm + P.lay.dense, n_r);
+  DenseRow<GW> Adense;  // row of the assembled Schur matrix (DENSE)
+  wsync();
+  // ---- step-constant row data (model, control, contacts, Jacobians at the step-start
+  // poses) into LDS, once per step ----
+  for (int j = lane; j < P.m.J; j += GW) store_joint_static(js + JS_WORDS * j, joint_static(P, w, j));
+  for (int c = lane; c < n_c; c += GW)
+    store_contact_static(cs + CS_WORDS * c, normal_static(P, sm, w, c), friction_static(P, sm, w, c));
 
   // ---- per-row registers ----
   float lam[R], lamb[R], x[R], Cr[R], Pi[R], rhs[R];
   sv J0[R], J1[R];
   int b0[R], b1[R];
+  int rj[R], rk[R];  // joint of a joint / control row, and the row within the joint
   bool act[R];
 #pragma unroll
   for (int k = 0; k < R; ++k) {
@@ -646,6 +678,9 @@ fused_step_kernel(FusedParams P) {
     if (P.cfg.seed_iterate && i < n_r) l0 = P.lambda_seed[(size_t)w * L.Nc + original_row(L, i, n_c)];
     lam[k] = l0; lamb[k] = l0;
     act[k] = false; b0[k] = -1; b1[k] = -1;
+    rj[k] = 0; rk[k] = 0;
+    if (i < L.Nj) { rj[k] = P.topo[i]; rk[k] = P.topo[L.Nj + i]; }
+    else if (i < L.Nj + L.Nctrl) rj[k] = P.topo[2 * L.Nj + (i - L.Nj)];
   }
   wsync();
   // learned warm start: keep whichever seed starts from the lower residual, so the
@@ -658,8 +693,8 @@ fused_step_kernel(FusedParams P) {
       int i = lane + GW * k;
       alt[k] = i < n_r ? P.lambda_seed2[(size_t)w * L.Nc + original_row(L, i, n_c)] : 0.f;
     }
-    const float ma = initial_merit<R, GW>(P, sm, lamprev, lane, w, n_r, n_c, lam);
-    const float mb = initial_merit<R, GW>(P, sm, lamprev, lane, w, n_r, n_c, alt);
+    const float ma = initial_merit<R, GW>(P, sm, lamprev, js, cs, rj, rk, lane, n_r, n_c, lam);
+    const float mb = initial_merit<R, GW>(P, sm, lamprev, js, cs, rj, rk, lane, n_r, n_c, alt);
     if (mb < ma) {
       seed_flag = FLAG_LEARNED_SEED;
 #pragma unroll
@@ -674,6 +709,11 @@ fused_step_kernel(FusedParams P) {
   const float reg = P.cfg.regularization;
   int n_it = 0, pcr_total = 0, best_idx = -1;
   float best_merit = 0.f, merit = 0.f;
+
+  // rows of the last residual pass (the dense instances have the registers for them)
+  constexpr bool kRowCache = DENSE;
+  RowOut rcache[R];
+  bool rvalid = false;
 
   OFB_PHASE_INIT
   while (true) {
@@ -693,7 +733,20 @@ fused_step_kernel(FusedParams P) {
       int i = lane + GW * k;
       hc[k] = 0.f;
       if (i < n_r) {
-        RowOut o = assemble_row(P, sm, lamprev, w, i, n_c, lam[k]);
+        RowOut o;
+        if (kRowCache && rvalid && i < L.Nj + L.Nctrl + n_c) {
+          // joint, control and normal rows do not read lambda_prev: the residual pass of
+          // the previous iteration evaluated them at this same (q, u, lambda); replay
+          // their -J^T lambda dt into h_d
+          o = rcache[k];
+          if (o.active) {
+            const float sc = -lam[k] * dt;
+            if (o.b0 >= 0) atomic_add_sv(sm + o.b0 * BS + BHD, o.J0 * sc);
+            if (o.b1 >= 0) atomic_add_sv(sm + o.b1 * BS + BHD, o.J1 * sc);
+          }
+        } else {
+          o = assemble_row_cached(P, sm, lamprev, js, cs, i, n_c, rj[k], rk[k], lam[k]);
+        }
         J0[k] = o.J0; J1[k] = o.J1; b0[k] = o.b0; b1[k] = o.b1; act[k] = o.active;
         hc[k] = o.hc;
         Cr[k] = o.C + reg;
@@ -714,20 +767,24 @@ fused_step_kernel(FusedParams P) {
     wsync();
     // Schur RHS (linear_system.py:17-65) and Jacobi diagonal (preconditioner.py:9-50)
     float bsq = 0.f;
+    sv W0[R], W1[R];  // M^-1 J^T of each row's two sides
 #pragma unroll
     for (int k = 0; k < R; ++k) {
       rhs[k] = 0.f; Pi[k] = 0.f;
+      W0[k] = svzero(); W1[k] = svzero();
       if (act[k]) {
         float s = 0.f, d = 0.f;
         if (b0[k] >= 0) {
           const float* bb = sm + b0[k] * BS;
           s += sdot(J0[k], ldsv(bb + BVV));
-          d += sdot(J0[k], apply_minv(bb, J0[k]));
+          W0[k] = apply_minv(bb, J0[k]);
+          d += sdot(J0[k], W0[k]);
         }
         if (b1[k] >= 0) {
           const float* bb = sm + b1[k] * BS;
           s += sdot(J1[k], ldsv(bb + BVV));
-          d += sdot(J1[k], apply_minv(bb, J1[k]));
+          W1[k] = apply_minv(bb, J1[k]);
+          d += sdot(J1[k], W1[k]);
         }
         rhs[k] = (s - hc[k]) / dt;
         Pi[k] = 1.f / (d + Cr[k] + 1e-6f);
@@ -740,11 +797,11 @@ fused_step_kernel(FusedParams P) {
       bj_factor<R>(V, sm, b0, b1, J0, J1, Cr, G, bj_flags);
     }
 
-    if constexpr (DENSE) dense_build(D, sm, lane, act[0], b0[0], b1[0], J0[0], J1[0], Cr[0]);
-
     OFB_PHASE(2)
+    if constexpr (DENSE) dense_build(D, Adense, lane, act[0], b0[0], b1[0], J0[0], J1[0], W0[0], W1[0], Cr[0]);
+    OFB_PHASE(7)
     pcr_total += pcr_solve<R, BJ, GW, DENSE>(sm, lane, B, V, G, act, b0, b1, J0, J1, Cr, Pi, rhs, bsq, tol_sq,
-                                             latol_sq, P.cfg.lin_max_iters, x, D);
+                                             latol_sq, P.cfg.lin_max_iters, x, D, Adense);
     OFB_PHASE(3)
 
     // ---- velocity recovery + Newton update (linear_system.py:344-377, newton_step.py:22-54) ----
@@ -803,7 +860,7 @@ fused_step_kernel(FusedParams P) {
         for (int k = 0; k < R; ++k) {
           int i = lane + GW * k;
           if (i < n_r) {
-            RowOut o = assemble_row(P, sm, lamprev, w, i, n_c, la[k]);
+            RowOut o = assemble_row_cached(P, sm, lamprev, js, cs, i, n_c, rj[k], rk[k], la[k]);
             ms += o.hc * o.hc;
           }
         }
@@ -837,10 +894,12 @@ fused_step_kernel(FusedParams P) {
     for (int k = 0; k < R; ++k) {
       int i = lane + GW * k;
       if (i < n_r) {
-        RowOut o = assemble_row(P, sm, lamprev, w, i, n_c, lam[k]);
+        RowOut o = assemble_row_cached(P, sm, lamprev, js, cs, i, n_c, rj[k], rk[k], lam[k]);
         msum += o.hc * o.hc;
+        if (kRowCache) rcache[k] = o;
       }
     }
+    rvalid = true;
     wsync();
     for (int e = lane; e < 6 * B; e += GW) {
       float v = sm[(e / 6) * BS + BHD + (e % 6)];
@@ -948,7 +1007,8 @@ int fused_class_op(int op, const FusedParams& P, int blocks, int threads, size_t
 
 // defined in fb_fused_classes.hip, one per (preconditioner, class set) translation unit;
 // return -1 when the class code is not in that unit's set.  Class code 0 is the
-// two-worlds-per-wave dense instance (at most 32 rows), code R >= 1 the one-world-per-wave
+// two-worlds-per-wave dense instance (at most 32 rows), code 1 the one-world-per-wave
+// dense instance (at most 64 rows), code R >= 2 the one-world-per-wave matrix-free
 // instance with R row slots per lane.
 int fused_class_op_bj0_set0(int R, int op, const FusedParams& P, int blocks, int threads, size_t lds, hipStream_t st);
 int fused_class_op_bj0_set1(int R, int op, const FusedParams& P, int blocks, int threads, size_t lds, hipStream_t st);

@@ -163,45 +163,115 @@ struct RowOut {
   bool active;
 };
 
-// joint rows (constraints/joint_constraint.py:62-162, joint_kinematics.py:5-261)
+// ---------------------------------------------------------------------------------
+// Step-constant row data.  Everything a row reads from the model, the control input and
+// the packed contacts is gathered into a static record, together with everything of a
+// contact row that depends only on the step-start poses: the reference evaluates the
+// contact and friction Jacobians, the effective masses and the friction frame at
+// body_q_prev (contact_constraint.py:118-166, friction_constraint.py:79-156 and
+// :379-469, constraints/utils.py:6-33), so they are constant over the Newton loop.  The
+// per-iteration row functions (*_row_s) work from a record and the current body state,
+// with the same arithmetic in the same order as evaluating everything in place.  The
+// fused kernel builds the records once per step into LDS; the plain kernels and the
+// adjoint build them on the fly (the *_row wrappers).
+// ---------------------------------------------------------------------------------
+
+// joints (joint_constraint.py:62-162, control_constraint.py:117-338)
+struct JointStatic {
+  int parent, child, enabled, type;
+  float comp;       // compliance (the joint's own, or the config's when negative)
+  int dof_mode;     // mode of the first dof (revolute / prismatic)
+  float ke, kd;     // its gains
+  float target;     // its control target: position (mode 1) or velocity (mode 2)
+  tf Xp, Xc;        // joint frames in the parent and child bodies
+  v3 ax;            // axis of the first dof (revolute / prismatic), else 0
+};
+
 template <class PT>
-__device__ void joint_frames(const PT& P, const float* sm, int w, int j, tf& Xwp, tf& Xwc, v3& r_p,
-                             v3& r_c, v3& pos_p, v3& pos_c, v3& com_p) {
-  const int J = P.m.J;
-  int p_ = P.m.joint_parent[w * J + j], c_ = P.m.joint_child[w * J + j];
-  const float* bc = sm + c_ * BS;
-  tf Xc_l = ldtf(P.m.joint_X_c + (size_t)(w * J + j) * 7);
+__device__ __forceinline__ JointStatic joint_static(const PT& P, int w, int j) {
+  const int J = P.m.J, D = P.m.D;
+  const int k = w * J + j;
+  JointStatic s;
+  s.parent = P.m.joint_parent[k];
+  s.child = P.m.joint_child[k];
+  s.enabled = P.m.joint_enabled[k];
+  s.type = P.m.joint_type[k];
+  float c = P.m.joint_compliance[k];
+  s.comp = c < 0.f ? P.cfg.compliance_joint : c;
+  s.Xp = ldtf(P.m.joint_X_p + (size_t)k * 7);
+  s.Xc = ldtf(P.m.joint_X_c + (size_t)k * 7);
+  s.ax = mk(0.f, 0.f, 0.f);
+  s.dof_mode = 0;
+  s.ke = 0.f; s.kd = 0.f; s.target = 0.f;
+  if (s.type == 0 || s.type == 1) {
+    const int qs = P.m.joint_qd_start[k];
+    s.ax = ld3(P.m.joint_axis + (size_t)(w * D + qs) * 3);
+    s.dof_mode = P.m.joint_dof_mode[w * D + qs];
+    s.ke = P.m.joint_target_ke[w * D + qs];
+    s.kd = P.m.joint_target_kd[w * D + qs];
+    if (s.dof_mode == 1) s.target = P.target_pos[w * D + qs];
+    else if (s.dof_mode == 2) s.target = P.target_vel[w * D + qs];
+  }
+  return s;
+}
+
+// LDS record of a joint: 28 floats (7 quads)
+constexpr int JS_WORDS = 28;
+__device__ __forceinline__ void store_joint_static(float* d, const JointStatic& s) {
+  float4* q = reinterpret_cast<float4*>(d);
+  q[0] = make_float4(__int_as_float(s.parent), __int_as_float(s.child), __int_as_float(s.enabled),
+                     __int_as_float(s.type));
+  q[1] = make_float4(s.comp, __int_as_float(s.dof_mode), s.ke, s.kd);
+  q[2] = make_float4(s.Xp.p.x, s.Xp.p.y, s.Xp.p.z, s.Xp.q.x);
+  q[3] = make_float4(s.Xp.q.y, s.Xp.q.z, s.Xp.q.w, s.Xc.p.x);
+  q[4] = make_float4(s.Xc.p.y, s.Xc.p.z, s.Xc.q.x, s.Xc.q.y);
+  q[5] = make_float4(s.Xc.q.z, s.Xc.q.w, s.ax.x, s.ax.y);
+  q[6] = make_float4(s.ax.z, s.target, 0.f, 0.f);
+}
+__device__ __forceinline__ JointStatic load_joint_static(const float* d) {
+  const float4* q = reinterpret_cast<const float4*>(d);
+  const float4 a = q[0], b = q[1], c = q[2], e = q[3], f = q[4], g = q[5], h = q[6];
+  JointStatic s;
+  s.parent = __float_as_int(a.x); s.child = __float_as_int(a.y);
+  s.enabled = __float_as_int(a.z); s.type = __float_as_int(a.w);
+  s.comp = b.x; s.dof_mode = __float_as_int(b.y); s.ke = b.z; s.kd = b.w;
+  s.Xp = tf{mk(c.x, c.y, c.z), qt{c.w, e.x, e.y, e.z}};
+  s.Xc = tf{mk(e.w, f.x, f.y), qt{f.z, f.w, g.x, g.y}};
+  s.ax = mk(g.z, g.w, h.x);
+  s.target = h.y;
+  return s;
+}
+
+// world frames of a joint at the current poses (joint_kinematics.py:5-40)
+__device__ __forceinline__ void joint_frames(const JointStatic& s, const float* sm, tf& Xwp, tf& Xwc, v3& r_p,
+                                             v3& r_c, v3& pos_p, v3& pos_c, v3& com_p) {
+  const float* bc = sm + s.child * BS;
   tf bq = ldtf(bc + BQ);
-  Xwc = tmul(bq, Xc_l);
+  Xwc = tmul(bq, s.Xc);
   pos_c = Xwc.p;
   r_c = pos_c - tpoint(bq, ld3(bc + BCOM));
-  tf Xp_l = ldtf(P.m.joint_X_p + (size_t)(w * J + j) * 7);
   tf bp = tident();
   com_p = mk(0.f, 0.f, 0.f);
-  if (p_ >= 0) { bp = ldtf(sm + p_ * BS + BQ); com_p = ld3(sm + p_ * BS + BCOM); }
-  Xwp = tmul(bp, Xp_l);
+  if (s.parent >= 0) { bp = ldtf(sm + s.parent * BS + BQ); com_p = ld3(sm + s.parent * BS + BCOM); }
+  Xwp = tmul(bp, s.Xp);
   pos_p = Xwp.p;
   r_p = pos_p - tpoint(bp, com_p);
 }
 
+// row k of a joint (constraints/joint_constraint.py:62-162, joint_kinematics.py:5-261)
 template <class PT>
-__device__ RowOut joint_row(const PT& P, float* sm, int w, int row, float lam, bool full) {
+__device__ RowOut joint_row_s(const PT& P, float* sm, const JointStatic& s, int k, float lam) {
   RowOut o;
   o.J0 = svzero(); o.J1 = svzero(); o.hc = 0.f; o.C = 0.f; o.active = false;
-  const int J = P.m.J;
-  int j = P.topo[row];
-  int k = P.topo[P.L.Nj + row];
-  o.b0 = P.m.joint_parent[w * J + j];
-  o.b1 = P.m.joint_child[w * J + j];
-  if (P.m.joint_enabled[w * J + j] == 0 || o.b1 < 0) return o;
-  int type = P.m.joint_type[w * J + j];
+  o.b0 = s.parent;
+  o.b1 = s.child;
+  if (s.enabled == 0 || o.b1 < 0) return o;
+  const int type = s.type;
   if (type < 0 || type > 3) return o;
-  float comp_ = P.m.joint_compliance[w * J + j];
-  if (comp_ < 0.f) comp_ = P.cfg.compliance_joint;
+  const float comp_ = s.comp;
   tf Xwp, Xwc; v3 r_p, r_c, pos_p, pos_c, com_p;
-  joint_frames(P, sm, w, j, Xwp, Xwc, r_p, r_c, pos_p, pos_c, com_p);
-  v3 ax = mk(0.f, 0.f, 0.f);
-  if (type == 0 || type == 1) ax = ld3(P.m.joint_axis + (size_t)(w * P.m.D + P.m.joint_qd_start[w * J + j]) * 3);
+  joint_frames(s, sm, Xwp, Xwc, r_p, r_c, pos_p, pos_c, com_p);
+  const v3 ax = s.ax;
   sv Jp = svzero(), Jc = svzero();
   float err = 0.f;
   bool act = false;
@@ -255,31 +325,31 @@ __device__ RowOut joint_row(const PT& P, float* sm, int w, int row, float lam, b
   return o;
 }
 
-// control rows (constraints/control_constraint.py:117-338)
 template <class PT>
-__device__ RowOut control_row(const PT& P, float* sm, int w, int crow, float lam) {
+__device__ RowOut joint_row(const PT& P, float* sm, int w, int row, float lam, bool /*full*/) {
+  return joint_row_s(P, sm, joint_static(P, w, P.topo[row]), P.topo[P.L.Nj + row], lam);
+}
+
+// control row of a joint (constraints/control_constraint.py:117-338)
+template <class PT>
+__device__ RowOut control_row_s(const PT& P, float* sm, const JointStatic& s, float lam) {
   RowOut o;
   o.J0 = svzero(); o.J1 = svzero(); o.hc = 0.f; o.C = 0.f; o.active = false;
-  const int J = P.m.J, D = P.m.D;
-  int j = P.topo[2 * P.L.Nj + crow];
-  o.b0 = P.m.joint_parent[w * J + j];
-  o.b1 = P.m.joint_child[w * J + j];
-  if (P.m.joint_enabled[w * J + j] == 0) return o;
-  int type = P.m.joint_type[w * J + j];
+  o.b0 = s.parent;
+  o.b1 = s.child;
+  if (s.enabled == 0) return o;
+  const int type = s.type;
   if (type != 0 && type != 1) return o;
-  int qs = P.m.joint_qd_start[w * J + j];
-  int mode = P.m.joint_dof_mode[w * D + qs];
+  const int mode = s.dof_mode;
   if (mode == 0 || o.b1 < 0) return o;
   tf Xwp, Xwc; v3 r_p, r_c, pos_p, pos_c, com_p;
-  joint_frames(P, sm, w, j, Xwp, Xwc, r_p, r_c, pos_p, pos_c, com_p);
-  v3 ax = ld3(P.m.joint_axis + (size_t)(w * D + qs) * 3);
+  joint_frames(s, sm, Xwp, Xwc, r_p, r_c, pos_p, pos_c, com_p);
+  const v3 ax = s.ax;
   sv up = svzero();
   if (o.b0 >= 0) up = ldsv(sm + o.b0 * BS + BU);
   sv uc = ldsv(sm + o.b1 * BS + BU);
-  float target = 0.f;
-  if (mode == 1) target = P.target_pos[w * D + qs];
-  else if (mode == 2) target = P.target_vel[w * D + qs];
-  float ke = P.m.joint_target_ke[w * D + qs], kd = P.m.joint_target_kd[w * D + qs];
+  const float target = s.target;
+  const float ke = s.ke, kd = s.kd;
   float dt = P.dt;
   float qv, qdv;
   sv Jp, Jc;
@@ -309,8 +379,8 @@ __device__ RowOut control_row(const PT& P, float* sm, int w, int crow, float lam
     float raw = qv - target;
     if (angular) {
       const float PI = 3.141592653589793f, TWO_PI = 6.283185307179586f;
-      float s = raw + PI;
-      err = (s - TWO_PI * floorf(s / TWO_PI)) - PI;
+      float s_ = raw + PI;
+      err = (s_ - TWO_PI * floorf(s_ / TWO_PI)) - PI;
     } else {
       err = raw;
     }
@@ -329,6 +399,11 @@ __device__ RowOut control_row(const PT& P, float* sm, int w, int crow, float lam
   o.J0 = Jp; o.J1 = Jc;
   o.C = alpha;
   return o;
+}
+
+template <class PT>
+__device__ RowOut control_row(const PT& P, float* sm, int w, int crow, float lam) {
+  return control_row_s(P, sm, joint_static(P, w, P.topo[2 * P.L.Nj + crow]), lam);
 }
 
 struct ContactGeo {
@@ -373,28 +448,51 @@ __device__ __forceinline__ float eff_mass_prev(const float* sm, int b0, int b1, 
   return val;
 }
 
-// contact normal row (constraints/contact_constraint.py:48-374)
+// contact normal row: the step-constant part (contact_constraint.py:118-166): geometry,
+// bodies, the Jacobian at the step-start poses and the FB preconditioner dt^2 J M^-1 J^T
+struct NormalStatic {
+  int b0, b1, skip;
+  v3 p0, p1, n;
+  float th0, th1, precond;
+  sv Ja, Jb;
+};
+
 template <class PT>
-__device__ RowOut normal_row(const PT& P, float* sm, int w, int c, float lam) {
-  RowOut o;
-  o.J0 = svzero(); o.J1 = svzero(); o.hc = 0.f; o.C = 0.f; o.active = false; o.b0 = -1; o.b1 = -1;
+__device__ __forceinline__ NormalStatic normal_static(const PT& P, const float* sm, int w, int c) {
   ContactGeo g = load_contact(P, w, c);
-  if (g.s0 == g.s1) return o;
-  o.b0 = g.b0; o.b1 = g.b1;
-  tf q0 = tident(), q0p = tident(), q1 = tident(), q1p = tident();
+  NormalStatic s;
+  s.skip = g.s0 == g.s1;
+  s.b0 = g.b0; s.b1 = g.b1;
+  s.p0 = g.p0; s.p1 = g.p1; s.n = g.n; s.th0 = g.th0; s.th1 = g.th1;
+  tf q0p = tident(), q1p = tident();
   v3 com0 = mk(0.f, 0.f, 0.f), com1 = com0;
-  if (g.b0 >= 0) { const float* bb = sm + g.b0 * BS; q0 = ldtf(bb + BQ); q0p = ldtf(bb + BQP); com0 = ld3(bb + BCOM); }
-  if (g.b1 >= 0) { const float* bb = sm + g.b1 * BS; q1 = ldtf(bb + BQ); q1p = ldtf(bb + BQP); com1 = ld3(bb + BCOM); }
+  if (g.b0 >= 0) { const float* bb = sm + g.b0 * BS; q0p = ldtf(bb + BQP); com0 = ld3(bb + BCOM); }
+  if (g.b1 >= 0) { const float* bb = sm + g.b1 * BS; q1p = ldtf(bb + BQP); com1 = ld3(bb + BCOM); }
   v3 p0_adj_prev = tpoint(q0p, g.p0) - g.n * g.th0;
   v3 p1_adj_prev = tpoint(q1p, g.p1) + g.n * g.th1;
-  v3 p0_adj = tpoint(q0, g.p0) - g.n * g.th0;
-  v3 p1_adj = tpoint(q1, g.p1) + g.n * g.th1;
-  sv Ja = svzero(), Jb = svzero();
-  if (g.b0 >= 0) Ja = sv{g.n, cross(p0_adj_prev - tpoint(q0p, com0), g.n)};
-  if (g.b1 >= 0) Jb = sv{-g.n, cross(p1_adj_prev - tpoint(q1p, com1), -g.n)};
-  float gap = dot(g.n, p0_adj - p1_adj);
+  s.Ja = svzero(); s.Jb = svzero();
+  if (g.b0 >= 0) s.Ja = sv{g.n, cross(p0_adj_prev - tpoint(q0p, com0), g.n)};
+  if (g.b1 >= 0) s.Jb = sv{-g.n, cross(p1_adj_prev - tpoint(q1p, com1), -g.n)};
+  const float dt = P.dt;
+  s.precond = dt * dt * eff_mass_prev(sm, g.b0, g.b1, s.Ja, s.Jb);
+  return s;
+}
+
+// contact normal row at the current poses (constraints/contact_constraint.py:48-374)
+template <class PT>
+__device__ RowOut normal_row_s(const PT& P, float* sm, const NormalStatic& s, float lam) {
+  RowOut o;
+  o.J0 = svzero(); o.J1 = svzero(); o.hc = 0.f; o.C = 0.f; o.active = false; o.b0 = -1; o.b1 = -1;
+  if (s.skip) return o;
+  o.b0 = s.b0; o.b1 = s.b1;
+  tf q0 = tident(), q1 = tident();
+  if (s.b0 >= 0) q0 = ldtf(sm + s.b0 * BS + BQ);
+  if (s.b1 >= 0) q1 = ldtf(sm + s.b1 * BS + BQ);
+  v3 p0_adj = tpoint(q0, s.p0) - s.n * s.th0;
+  v3 p1_adj = tpoint(q1, s.p1) + s.n * s.th1;
+  float gap = dot(s.n, p0_adj - p1_adj);
   float dt = P.dt;
-  float precond = dt * dt * eff_mass_prev(sm, g.b0, g.b1, Ja, Jb);
+  const float precond = s.precond;
   float phi, dphi_dg, dphi_dl;
   if (P.cfg.contact_mode == OFB_CONTACT_EQUALITY) {
     phi = gap; dphi_dg = 1.f; dphi_dl = 0.f;
@@ -405,27 +503,39 @@ __device__ RowOut normal_row(const PT& P, float* sm, int w, int c, float lam) {
     dphi_dg = P.cfg.contact_fb_alpha * (1.f - sa / nrm);
     dphi_dl = precond * (1.f - sb / nrm);
   }
-  o.J0 = Ja * dphi_dg;
-  o.J1 = Jb * dphi_dg;
+  o.J0 = s.Ja * dphi_dg;
+  o.J1 = s.Jb * dphi_dg;
   o.active = true;
-  if (g.b0 >= 0) atomic_add_sv(sm + g.b0 * BS + BHD, o.J0 * (-lam * dt));
-  if (g.b1 >= 0) atomic_add_sv(sm + g.b1 * BS + BHD, o.J1 * (-lam * dt));
+  if (s.b0 >= 0) atomic_add_sv(sm + s.b0 * BS + BHD, o.J0 * (-lam * dt));
+  if (s.b1 >= 0) atomic_add_sv(sm + s.b1 * BS + BHD, o.J1 * (-lam * dt));
   o.hc = phi / dt;
   o.C = (dphi_dl + P.cfg.compliance_contact) / (dt * dt);
   return o;
 }
 
-// friction row t in {0,1} of contact c (constraints/friction_constraint.py:79-908)
-// dwdmu (adjoint only, may be null): d(weight)/d(mu_x) of the isotropic cone, written as
-// NaN where the weight's dependence on mu has no closed form here (anisotropic axes,
-// Stribeck scaling, the elliptic cone) and 0 where the row does not depend on mu.
 template <class PT>
-__device__ RowOut friction_row(const PT& P, float* sm, const float* lamprev, int w, int c, int t,
-                               int n_c, float lam, float* dwdmu = nullptr) {
-  if (dwdmu) *dwdmu = 0.f;
-  RowOut o;
-  o.J0 = svzero(); o.J1 = svzero(); o.hc = 0.f; o.C = 0.f; o.active = false; o.b0 = -1; o.b1 = -1;
+__device__ RowOut normal_row(const PT& P, float* sm, int w, int c, float lam) {
+  return normal_row_s(P, sm, normal_static(P, sm, w, c), lam);
+}
+
+// friction rows of a contact: the step-constant part - skip rule, shape parameters, the
+// friction frame and both tangent Jacobians at the step-start poses, the effective-mass
+// scale r (friction_constraint.py:39-194, :379-469)
+struct FrictionStatic {
+  int b0, b1, skip;
+  int closed;  // isotropic frame without anisotropy axes (d w / d mu has a closed form)
+  sv Jt1a, Jt2a, Jt1b, Jt2b;
+  float rr, mux, muy, gate_mu, mss, vst, lat;
+};
+
+template <class PT>
+__device__ __forceinline__ FrictionStatic friction_static(const PT& P, const float* sm, int w, int c) {
   ContactGeo g = load_contact(P, w, c);
+  FrictionStatic s;
+  s.b0 = g.b0; s.b1 = g.b1;
+  s.closed = 1;
+  s.Jt1a = svzero(); s.Jt2a = svzero(); s.Jt1b = svzero(); s.Jt2b = svzero();
+  s.rr = 0.f; s.mux = 0.f; s.muy = 0.f; s.gate_mu = 0.f; s.mss = -1.f; s.vst = -1.f; s.lat = -1.f;
   bool skip = g.s0 == g.s1;
   if (P.cfg.friction_mode == OFB_FRICTION_BILATERAL_PATCH && !skip) {  // bilateral_row_is_skipped :39-76
     for (int jj = 0; jj < c; ++jj) {
@@ -433,20 +543,15 @@ __device__ RowOut friction_row(const PT& P, float* sm, const float* lamprev, int
       if (ii[0] == g.s0 && ii[P.L.K] == g.s1) { skip = true; break; }
     }
   }
-  if (skip) return o;
+  s.skip = skip;
+  if (skip) return s;
   const int SC = P.m.SC;
   float mu0 = P.m.shape_mu[w * SC + g.s0], mu1 = P.m.shape_mu[w * SC + g.s1];
-  // lambda_prev of this contact: normal row then the two friction rows (compacted index)
-  const int base_n = P.L.Nj + P.L.Nctrl;
-  float fn_prev = lamprev[base_n + c];
-  if (fmaxf(mu0, mu1) * fn_prev <= P.cfg.friction_load_gate) return o;
-  o.b0 = g.b0; o.b1 = g.b1;
-  float l1p = lamprev[base_n + n_c + 2 * c], l2p = lamprev[base_n + n_c + 2 * c + 1];
+  s.gate_mu = fmaxf(mu0, mu1);
   tf q0p = tident(), q1p = tident();
-  sv vel0 = svzero(), vel1 = svzero();
   v3 com0 = mk(0.f, 0.f, 0.f), com1 = com0;
-  if (g.b0 >= 0) { const float* bb = sm + g.b0 * BS; q0p = ldtf(bb + BQP); vel0 = ldsv(bb + BU); com0 = ld3(bb + BCOM); }
-  if (g.b1 >= 0) { const float* bb = sm + g.b1 * BS; q1p = ldtf(bb + BQP); vel1 = ldsv(bb + BU); com1 = ld3(bb + BCOM); }
+  if (g.b0 >= 0) { const float* bb = sm + g.b0 * BS; q0p = ldtf(bb + BQP); com0 = ld3(bb + BCOM); }
+  if (g.b1 >= 0) { const float* bb = sm + g.b1 * BS; q1p = ldtf(bb + BQP); com1 = ld3(bb + BCOM); }
   // resolve_friction_frame :79-156
   v3 ax0 = ld3(P.m.shape_friction_axis + (size_t)(w * SC + g.s0) * 3);
   v3 ax1 = ld3(P.m.shape_friction_axis + (size_t)(w * SC + g.s1) * 3);
@@ -475,6 +580,8 @@ __device__ RowOut friction_row(const PT& P, float* sm, const float* lamprev, int
       muy = (mpa + mot) * 0.5f;
     }
   }
+  s.closed = (!an0 && !an1) ? 1 : 0;
+  s.mux = mux; s.muy = muy;
   // resolve_stribeck_params :159-176 / resolve_stribeck_lateral_only :179-194
   float ss0 = P.m.shape_mu_stiction_scale[w * SC + g.s0], vs0 = P.m.shape_v_stribeck[w * SC + g.s0];
   float ss1 = P.m.shape_mu_stiction_scale[w * SC + g.s1], vs1 = P.m.shape_v_stribeck[w * SC + g.s1];
@@ -482,28 +589,105 @@ __device__ RowOut friction_row(const PT& P, float* sm, const float* lamprev, int
   if (ss0 > 0.f && vs0 > 0.f) { mss = ss0; vst = vs0; }
   else if (ss1 > 0.f && vs1 > 0.f) { mss = ss1; vst = vs1; }
   float lo0 = P.m.shape_stribeck_lateral_only[w * SC + g.s0], lo1 = P.m.shape_stribeck_lateral_only[w * SC + g.s1];
-  float lat = lo0 > 0.f ? lo0 : (lo1 > 0.f ? lo1 : -1.f);
+  s.lat = lo0 > 0.f ? lo0 : (lo1 > 0.f ? lo1 : -1.f);
+  s.mss = mss; s.vst = vst;
   // compute_friction_core :379-469
-  sv Jt1a = svzero(), Jt2a = svzero(), Jt1b = svzero(), Jt2b = svzero();
   if (g.b0 >= 0) {
     v3 r_ = (tpoint(q0p, g.p0) - g.n * g.th0) - tpoint(q0p, com0);
-    Jt1a = sv{t1, cross(r_, t1)};
-    Jt2a = sv{t2, cross(r_, t2)};
+    s.Jt1a = sv{t1, cross(r_, t1)};
+    s.Jt2a = sv{t2, cross(r_, t2)};
   }
   if (g.b1 >= 0) {
     v3 r_ = (tpoint(q1p, g.p1) + g.n * g.th1) - tpoint(q1p, com1);
-    Jt1b = sv{-t1, -cross(r_, t1)};
-    Jt2b = sv{-t2, -cross(r_, t2)};
+    s.Jt1b = sv{-t1, -cross(r_, t1)};
+    s.Jt2b = sv{-t2, -cross(r_, t2)};
   }
-  float wt1 = eff_mass_prev(sm, g.b0, g.b1, Jt1a, Jt1b);
-  float wt2 = eff_mass_prev(sm, g.b0, g.b1, Jt2a, Jt2b);
-  float rr = (wt1 + wt2) * 0.5f;
+  float wt1 = eff_mass_prev(sm, g.b0, g.b1, s.Jt1a, s.Jt1b);
+  float wt2 = eff_mass_prev(sm, g.b0, g.b1, s.Jt2a, s.Jt2b);
+  s.rr = (wt1 + wt2) * 0.5f;
+  return s;
+}
+
+// LDS record of a contact: 64 floats (16 quads), the normal row's 28 then the friction
+// rows' 36
+constexpr int CS_WORDS = 64;
+constexpr int CS_FRICTION = 28;
+__device__ __forceinline__ void st_sv2(float4* q, const sv& a, const sv& b) {
+  q[0] = make_float4(a.a.x, a.a.y, a.a.z, a.b.x);
+  q[1] = make_float4(a.b.y, a.b.z, b.a.x, b.a.y);
+  q[2] = make_float4(b.a.z, b.b.x, b.b.y, b.b.z);
+}
+__device__ __forceinline__ void ld_sv2(const float4* q, sv& a, sv& b) {
+  const float4 x = q[0], y = q[1], z = q[2];
+  a = sv{mk(x.x, x.y, x.z), mk(x.w, y.x, y.y)};
+  b = sv{mk(y.z, y.w, z.x), mk(z.y, z.z, z.w)};
+}
+__device__ __forceinline__ void store_contact_static(float* d, const NormalStatic& n, const FrictionStatic& f) {
+  float4* q = reinterpret_cast<float4*>(d);
+  q[0] = make_float4(__int_as_float(n.b0), __int_as_float(n.b1), __int_as_float(n.skip), 0.f);
+  q[1] = make_float4(n.p0.x, n.p0.y, n.p0.z, n.p1.x);
+  q[2] = make_float4(n.p1.y, n.p1.z, n.n.x, n.n.y);
+  q[3] = make_float4(n.n.z, n.th0, n.th1, n.precond);
+  st_sv2(q + 4, n.Ja, n.Jb);
+  q[7] = make_float4(__int_as_float(f.b0), __int_as_float(f.b1), __int_as_float(f.skip), __int_as_float(f.closed));
+  st_sv2(q + 8, f.Jt1a, f.Jt2a);
+  st_sv2(q + 11, f.Jt1b, f.Jt2b);
+  q[14] = make_float4(f.rr, f.mux, f.muy, f.gate_mu);
+  q[15] = make_float4(f.mss, f.vst, f.lat, 0.f);
+}
+__device__ __forceinline__ NormalStatic load_normal_static(const float* d) {
+  const float4* q = reinterpret_cast<const float4*>(d);
+  const float4 a = q[0], b = q[1], c = q[2], e = q[3];
+  NormalStatic s;
+  s.b0 = __float_as_int(a.x); s.b1 = __float_as_int(a.y); s.skip = __float_as_int(a.z);
+  s.p0 = mk(b.x, b.y, b.z);
+  s.p1 = mk(b.w, c.x, c.y);
+  s.n = mk(c.z, c.w, e.x);
+  s.th0 = e.y; s.th1 = e.z; s.precond = e.w;
+  ld_sv2(q + 4, s.Ja, s.Jb);
+  return s;
+}
+__device__ __forceinline__ FrictionStatic load_friction_static(const float* d) {
+  const float4* q = reinterpret_cast<const float4*>(d);
+  const float4 a = q[7], b = q[14], c = q[15];
+  FrictionStatic s;
+  s.b0 = __float_as_int(a.x); s.b1 = __float_as_int(a.y); s.skip = __float_as_int(a.z); s.closed = __float_as_int(a.w);
+  ld_sv2(q + 8, s.Jt1a, s.Jt2a);
+  ld_sv2(q + 11, s.Jt1b, s.Jt2b);
+  s.rr = b.x; s.mux = b.y; s.muy = b.z; s.gate_mu = b.w;
+  s.mss = c.x; s.vst = c.y; s.lat = c.z;
+  return s;
+}
+
+// friction row t in {0,1} of contact c at the current velocities and lambda_prev
+// (constraints/friction_constraint.py:79-908).
+// dwdmu (adjoint only, may be null): d(weight)/d(mu_x) of the isotropic cone, written as
+// NaN where the weight's dependence on mu has no closed form here (anisotropic axes,
+// Stribeck scaling, the elliptic cone) and 0 where the row does not depend on mu.
+template <class PT>
+__device__ RowOut friction_row_s(const PT& P, float* sm, const float* lamprev, const FrictionStatic& s, int c,
+                                 int t, int n_c, float lam, float* dwdmu = nullptr) {
+  if (dwdmu) *dwdmu = 0.f;
+  RowOut o;
+  o.J0 = svzero(); o.J1 = svzero(); o.hc = 0.f; o.C = 0.f; o.active = false; o.b0 = -1; o.b1 = -1;
+  if (s.skip) return o;
+  // lambda_prev of this contact: normal row then the two friction rows (compacted index)
+  const int base_n = P.L.Nj + P.L.Nctrl;
+  float fn_prev = lamprev[base_n + c];
+  if (s.gate_mu * fn_prev <= P.cfg.friction_load_gate) return o;
+  o.b0 = s.b0; o.b1 = s.b1;
+  float l1p = lamprev[base_n + n_c + 2 * c], l2p = lamprev[base_n + n_c + 2 * c + 1];
+  sv vel0 = svzero(), vel1 = svzero();
+  if (s.b0 >= 0) vel0 = ldsv(sm + s.b0 * BS + BU);
+  if (s.b1 >= 0) vel1 = ldsv(sm + s.b1 * BS + BU);
+  float mux = s.mux, muy = s.muy;
+  const float mss = s.mss, vst = s.vst, lat = s.lat, rr = s.rr;
   // compute_friction_model :197-371
-  float vt0 = sdot(Jt1a, vel0) + sdot(Jt1b, vel1);
-  float vt1 = sdot(Jt2a, vel0) + sdot(Jt2b, vel1);
+  float vt0 = sdot(s.Jt1a, vel0) + sdot(s.Jt1b, vel1);
+  float vt1 = sdot(s.Jt2a, vel0) + sdot(s.Jt2b, vel1);
   if (mss > 0.f && vst > 0.f) {
-    float s = sqrtf(vt0 * vt0 + vt1 * vt1);
-    float factor = 1.f + (mss - 1.f) * expf(-s / vst);
+    float sp = sqrtf(vt0 * vt0 + vt1 * vt1);
+    float factor = 1.f + (mss - 1.f) * expf(-sp / vst);
     if (lat > 0.f && mux != muy) { mux = mux * factor; }
     else { mux = mux * factor; muy = muy * factor; }
   }
@@ -526,7 +710,7 @@ __device__ RowOut friction_row(const PT& P, float* sm, const float* lamprev, int
       if (dwdmu) {  // lim = mu fn_prev dt enters through gap
         float nrm = sqrtf(sa * sa + sb * sb);
         float dphi = rr * (1.f - (nrm > 0.f ? sb / nrm : 0.f)) * (raw < lim ? 1.f : 0.f) * fn_prev * dt;
-        bool closed = !an0 && !an1 && !(mss > 0.f && vst > 0.f);
+        bool closed = s.closed && !(mss > 0.f && vst > 0.f);
         *dwdmu = !closed ? __int_as_float(0x7fc00000)
                          : ((wv < 0.f || wv > 1e5f) ? 0.f : -rr * dphi * (den + (dn - phi)) / (den * den));
       }
@@ -551,16 +735,22 @@ __device__ RowOut friction_row(const PT& P, float* sm, const float* lamprev, int
     }
   }
   o.active = true;
-  sv Ja = t == 0 ? Jt1a : Jt2a;
-  sv Jb = t == 0 ? Jt1b : Jt2b;
+  sv Ja = t == 0 ? s.Jt1a : s.Jt2a;
+  sv Jb = t == 0 ? s.Jt1b : s.Jt2b;
   float wgt = t == 0 ? wx : wy;
   float vt = t == 0 ? vt0 : vt1;
-  if (g.b0 >= 0) atomic_add_sv(sm + g.b0 * BS + BHD, Ja * (-dt * lam));
-  if (g.b1 >= 0) atomic_add_sv(sm + g.b1 * BS + BHD, Jb * (-dt * lam));
+  if (s.b0 >= 0) atomic_add_sv(sm + s.b0 * BS + BHD, Ja * (-dt * lam));
+  if (s.b1 >= 0) atomic_add_sv(sm + s.b1 * BS + BHD, Jb * (-dt * lam));
   o.hc = vt + wgt * lam;
   o.J0 = Ja; o.J1 = Jb;
   o.C = wgt / dt + P.cfg.compliance_friction;
   return o;
+}
+
+template <class PT>
+__device__ RowOut friction_row(const PT& P, float* sm, const float* lamprev, int w, int c, int t, int n_c,
+                               float lam, float* dwdmu = nullptr) {
+  return friction_row_s(P, sm, lamprev, friction_static(P, sm, w, c), c, t, n_c, lam, dwdmu);
 }
 
 // dispatch on the compacted row index
@@ -574,6 +764,21 @@ __device__ __forceinline__ RowOut assemble_row(const PT& P, float* sm, const flo
   if (c < n_c) return normal_row(P, sm, w, c, lam);
   int f = c - n_c;
   return friction_row(P, sm, lamprev, w, f >> 1, f & 1, n_c, lam);
+}
+
+// dispatch on the compacted row index with the step's static records in LDS (js: one
+// record per joint, cs: one per contact); rj, rk: the row's joint and its row within the
+// joint (joint rows), the row's joint (control rows)
+template <class PT>
+__device__ __forceinline__ RowOut assemble_row_cached(const PT& P, float* sm, const float* lamprev, const float* js,
+                                                      const float* cs, int i, int n_c, int rj, int rk, float lam) {
+  const int Nj = P.L.Nj, Nct = P.L.Nctrl;
+  if (i < Nj) return joint_row_s(P, sm, load_joint_static(js + JS_WORDS * rj), rk, lam);
+  if (i < Nj + Nct) return control_row_s(P, sm, load_joint_static(js + JS_WORDS * rj), lam);
+  int c = i - Nj - Nct;
+  if (c < n_c) return normal_row_s(P, sm, load_normal_static(cs + CS_WORDS * c), lam);
+  int f = c - n_c;
+  return friction_row_s(P, sm, lamprev, load_friction_static(cs + CS_WORDS * (f >> 1)), f >> 1, f & 1, n_c, lam);
 }
 
 __device__ __forceinline__ int original_row(const RowLayout& L, int i, int n_c) {

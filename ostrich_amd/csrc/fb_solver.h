@@ -86,6 +86,11 @@ struct ofb_solver {
   // per-step bucketing of worlds by active-row class (fused path)
   int32_t* cls_counts;
   int32_t* cls_lists;
+  int cls_wpb[16];   // worlds per workgroup of each class launch (LDS-limited)
+  // the one-world-per-wave classes run on a side stream beside the paired class
+  hipStream_t side_stream;
+  hipEvent_t ev_fork, ev_join;
+  int side_ready;
   // cross-step contact warm start: snapshot of the previous step (fb_warmstart.hip)
   int32_t *ws_count, *ws_b0, *ws_b1;
   float *ws_p, *ws_n, *ws_ln, *ws_lt;
