@@ -395,18 +395,36 @@ __device__ __forceinline__ void schur_matvec(float* sm, int lane, int B, const b
 // as in the matrix-free operator).  Columns are handled in blocks of 8, and blocks past
 // the world's rows are skipped.
 // LDS: rowd[16 LD] (M^-1 J0^T 6, M^-1 J1^T 6, b0 bits, b1 bits, pad), v[LD].
+//
+// Worlds of at most kBodyMajorMax bodies build A body-major instead: row j stages
+// M_b^-1 J_j,b^T for every body b (zeros where row j does not touch b) at wb[b][j], so
+// lane i forms A_ij = J_i,m0 . wb[m0][j] + J_i,m1 . wb[m1][j] from its own two bodies:
+// two 6-dots and no body matching per entry (the same sum: a row couples two distinct
+// bodies, so at most one staged vector matches each side).  wb aliases rowd and v
+// (neither is live while A is built): B (8 LD + 4) floats, the + 4 staggering the bodies
+// over LDS banks.
 // ---------------------------------------------------------------------------------
-__host__ __device__ inline int dense_words(int ld) { return 17 * ld; }
+#ifndef OFB_BODY_MAJOR_MAX
+#define OFB_BODY_MAJOR_MAX 8
+#endif
+constexpr int kBodyMajorMax = OFB_BODY_MAJOR_MAX;
+__host__ __device__ inline bool dense_body_major(int B) { return B <= kBodyMajorMax; }
+__host__ __device__ inline int dense_wb_stride(int ld) { return 8 * ld + 4; }
+__host__ __device__ inline int dense_words(int ld, int B) {
+  const int w = 17 * ld;
+  return dense_body_major(B) && B * dense_wb_stride(ld) > w ? B * dense_wb_stride(ld) : w;
+}
 
 struct DenseView {
   float* rowd;
   float* v;
+  float* wb;  // body-major staging (aliases rowd), null for the row-major build
   int n_r;
 };
 
 template <int LD>
-__device__ __forceinline__ DenseView dense_view(float* base, int n_r) {
-  return DenseView{base, base + 16 * LD, n_r};
+__device__ __forceinline__ DenseView dense_view(float* base, int n_r, int B) {
+  return DenseView{base, base + 16 * LD, dense_body_major(B) ? base : nullptr, n_r};
 }
 
 // row i of A, in registers
@@ -446,16 +464,67 @@ __device__ __forceinline__ void dense_stage(float* __restrict__ rowd, int lane, 
 // Every lane of the group stages its row, so rows past n_r inside a block carry bodies
 // that match nothing.
 template <int LD>
-__device__ __forceinline__ void dense_build(const DenseView& D, DenseRow<LD>& A, int lane, bool act, int b0, int b1,
-                                            const sv& J0, const sv& J1, const sv& W0, const sv& W1, float Cr) {
+__device__ __forceinline__ void dense_build_body_major(const DenseView& D, DenseRow<LD>& A, int lane, int B, bool on,
+                                                       int b0, int b1, const sv& J0, const sv& J1, const sv& W0,
+                                                       const sv& W1, float Cr, int n_hi) {
+  const int stride = dense_wb_stride(LD);
+  for (int b = 0; b < B; ++b) {
+    sv w = svzero();
+    if (on && b == b0) w = W0;
+    else if (on && b == b1) w = W1;
+    float4* r = reinterpret_cast<float4*>(D.wb + b * stride + 8 * lane);
+    r[0] = make_float4(w.a.x, w.a.y, w.a.z, w.b.x);
+    r[1] = make_float4(w.b.y, w.b.z, 0.f, 0.f);
+  }
+  wsync();
+  const bool s0 = on && b0 >= 0, s1 = on && b1 >= 0;  // a static side contributes nothing
+  const sv Ja = s0 ? J0 : svzero(), Jb = s1 ? J1 : svzero();
+  const float* p0 = D.wb + (s0 ? b0 : 0) * stride;
+  const float* p1 = D.wb + (s1 ? b1 : 0) * stride;
+#pragma unroll
+  for (int j0 = 0; j0 < LD; j0 += 8) {
+    if (j0 < n_hi) {
+#pragma unroll
+      for (int j = j0; j < j0 + 8; ++j) {
+        const float4 x0 = *reinterpret_cast<const float4*>(p0 + 8 * j);
+        const float2 y0 = *reinterpret_cast<const float2*>(p0 + 8 * j + 4);
+        const float4 x1 = *reinterpret_cast<const float4*>(p1 + 8 * j);
+        const float2 y1 = *reinterpret_cast<const float2*>(p1 + 8 * j + 4);
+        float a = sdot(Ja, sv{mk(x0.x, x0.y, x0.z), mk(x0.w, y0.x, y0.y)});
+        a += sdot(Jb, sv{mk(x1.x, x1.y, x1.z), mk(x1.w, y1.x, y1.y)});
+        if (j == lane && on) a += Cr;
+        A.a[j] = a;
+      }
+    } else {
+#pragma unroll
+      for (int j = j0; j < j0 + 8; ++j) A.a[j] = 0.f;
+    }
+  }
+  wsync();
+}
+
+template <int LD>
+__device__ __forceinline__ void dense_build(const DenseView& D, DenseRow<LD>& A, int lane, int B, bool act, int b0,
+                                            int b1, const sv& J0, const sv& J1, const sv& W0, const sv& W1,
+                                            float Cr) {
   const bool on = lane < D.n_r && act;
+  // column blocks past every group's rows are skipped on a wave-uniform bound (the
+  // larger row count of the two groups of a wave: a retired group's stale count only
+  // widens it), so the branch never splits the wave; blocks below the bound but past a
+  // group's own rows read staged rows that match no body and give zeros
+  const int n_hi = LD == 32 ? max(__builtin_amdgcn_readlane(D.n_r, 0), __builtin_amdgcn_readlane(D.n_r, 32))
+                            : __builtin_amdgcn_readfirstlane(D.n_r);
+  if (D.wb) {
+    dense_build_body_major<LD>(D, A, lane, B, on, b0, b1, J0, J1, W0, W1, Cr, n_hi);
+    return;
+  }
   dense_stage(D.rowd, lane, on, b0, b1, W0, W1);
   wsync();
   const int m0 = (on && b0 >= 0) ? b0 : -3, m1 = (on && b1 >= 0) ? b1 : -3;
   const float4* r = reinterpret_cast<const float4*>(D.rowd);
 #pragma unroll
   for (int j0 = 0; j0 < LD; j0 += 8) {
-    if (j0 < D.n_r) {
+    if (j0 < n_hi) {
 #pragma unroll
       for (int j = j0; j < j0 + 8; ++j) {
         float a = dense_entry(r[4 * j], r[4 * j + 1], r[4 * j + 2], r[4 * j + 3], m0, m1, J0, J1);
@@ -470,39 +539,80 @@ __device__ __forceinline__ void dense_build(const DenseView& D, DenseRow<LD>& A,
   wsync();
 }
 
-// out_i = sum_j A_ij v_j: v goes through LDS once, its broadcast quads in flight together
+// out_i = sum_j A_ij v_j: v goes through LDS once and its broadcast quads are read in
+// batches of eight with no branch between them, so one LDS latency is paid per batch
+// (one batch for LD = 32).  Columns past the world's rows hold zeros in A and in v, so
+// reading every column leaves the sum unchanged.
 template <int LD>
 __device__ __forceinline__ float dense_matvec(const DenseView& D, const DenseRow<LD>& A, int lane, bool act,
                                               float v) {
+  static_assert(LD % 32 == 0, "dense rows are read in batches of 32 columns");
   const bool on = lane < D.n_r && act;
   D.v[lane] = on ? v : 0.f;
   wsync();
   const float4* v4 = reinterpret_cast<const float4*>(D.v);
   float o[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-  for (int j0 = 0; j0 < LD; j0 += 8) {
-    if (j0 < D.n_r) {
-      const float4 x = v4[j0 / 4], y = v4[j0 / 4 + 1];
-      o[0] += A.a[j0] * x.x;
-      o[1] += A.a[j0 + 1] * x.y;
-      o[2] += A.a[j0 + 2] * x.z;
-      o[3] += A.a[j0 + 3] * x.w;
-      o[0] += A.a[j0 + 4] * y.x;
-      o[1] += A.a[j0 + 5] * y.y;
-      o[2] += A.a[j0 + 6] * y.z;
-      o[3] += A.a[j0 + 7] * y.w;
+  for (int j0 = 0; j0 < LD; j0 += 32) {
+    float4 x[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) x[q] = v4[j0 / 4 + q];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      o[0] += A.a[j0 + 4 * q] * x[q].x;
+      o[1] += A.a[j0 + 4 * q + 1] * x[q].y;
+      o[2] += A.a[j0 + 4 * q + 2] * x[q].z;
+      o[3] += A.a[j0 + 4 * q + 3] * x[q].w;
     }
   }
   wsync();
   return on ? (o[0] + o[1]) + (o[2] + o[3]) : 0.f;
 }
 
+// ---------------------------------------------------------------------------------
+// Row -> body sums without LDS atomics (dense instances).  Every row couples at most two
+// bodies, and many rows share one (all joint rows of a robot touch its chassis), so
+// ds_add_f32 from the lanes of a wave serialises on a few addresses and stalls the LDS
+// pipe for every wave of the CU.  Instead each row stages its two body indices and
+// contributions in a 16-float record (reusing the dense view's staging rows, which are
+// free outside dense_build), and lanes over (body, component) sum the records in row
+// order: a fixed order, the one of a sequential loop over rows.
+// ---------------------------------------------------------------------------------
+__device__ __forceinline__ void stage_contrib(float* __restrict__ rec, int i, bool on, int b0, int b1, const sv& c0,
+                                              const sv& c1) {
+  float4* r = reinterpret_cast<float4*>(rec + 16 * i);
+  r[0] = make_float4(__int_as_float(on ? b0 : -1), __int_as_float(on ? b1 : -1), 0.f, 0.f);
+  r[1] = make_float4(c0.a.x, c0.a.y, c0.a.z, c0.b.x);
+  r[2] = make_float4(c0.b.y, c0.b.z, c1.a.x, c1.a.y);
+  r[3] = make_float4(c1.a.z, c1.b.x, c1.b.y, c1.b.z);
+}
+
+// field[b] (6 floats at body offset `field`) = (zero ? 0 : field[b]) + the records' sums,
+// rows 0..n_r-1 in order; a body index < 0 (static side, inactive row) matches nothing
+template <int GW>
+__device__ __forceinline__ void gather_contrib(float* sm, const float* __restrict__ rec, int lane, int n_r, int B,
+                                               int field, bool zero) {
+  for (int e = lane; e < 6 * B; e += GW) {
+    const int b = e / 6, k = e % 6;
+    float acc = zero ? 0.f : sm[b * BS + field + k];
+    const float* r = rec;
+#pragma unroll 4
+    for (int j = 0; j < n_r; ++j, r += 16) {
+      const float2 bb = *reinterpret_cast<const float2*>(r);
+      const float v0 = r[4 + k], v1 = r[10 + k];
+      if (__float_as_int(bb.x) == b) acc += v0;
+      if (__float_as_int(bb.y) == b) acc += v1;
+    }
+    sm[b * BS + field + k] = acc;
+  }
+}
+
 // ||[h_d | h_c]||^2 at the step-start state with lambda = l (and lambda_prev = l, as the
 // first Newton iteration snapshots it): the gate between two Newton seeds.
-template <int R, int GW>
+template <int R, int GW, bool GATHER>
 __device__ __forceinline__ float initial_merit(const FusedParams& P, float* sm, float* lamprev, const float* js,
                                                const float* cs, const int (&rj)[R], const int (&rk)[R], int lane,
-                                               int n_r, int n_c, const float (&l)[R]) {
+                                               int n_r, int n_c, const float (&l)[R], float* rec) {
   const int B = P.m.B;
 #pragma unroll
   for (int k = 0; k < R; ++k) {
@@ -516,11 +626,16 @@ __device__ __forceinline__ float initial_merit(const FusedParams& P, float* sm, 
   for (int k = 0; k < R; ++k) {
     int i = lane + GW * k;
     if (i < n_r) {
-      RowOut o = assemble_row_cached(P, sm, lamprev, js, cs, i, n_c, rj[k], rk[k], l[k]);
+      RowOut o = assemble_row_cached(P, sm, lamprev, js, cs, i, n_c, rj[k], rk[k], l[k], !GATHER);
       s += o.hc * o.hc;
+      if (GATHER) stage_contrib(rec, i, o.active, o.b0, o.b1, o.J0 * (-l[k] * P.dt), o.J1 * (-l[k] * P.dt));
     }
   }
   wsync();
+  if (GATHER) {
+    gather_contrib<GW>(sm, rec, lane, n_r, B, BHD, false);
+    wsync();
+  }
   for (int e = lane; e < 6 * B; e += GW) {
     float v = sm[(e / 6) * BS + BHD + (e % 6)];
     s += v * v;
@@ -593,7 +708,7 @@ __host__ __device__ inline FusedLayout fused_layout(int B, int J, int rows_cap, 
   l.cs = l.js + JS_WORDS * J;
   l.bj = l.cs + CS_WORDS * cs_cap;
   l.dense = up4(l.bj + (bj ? bj_words(rows_cap) : 0));
-  l.words = up4(l.dense + (dense ? dense_words(rows_cap) : 0));
+  l.words = up4(l.dense + (dense ? dense_words(rows_cap, B) : 0));
   return l;
 }
 
@@ -656,8 +771,10 @@ fused_step_kernel(FusedParams P) {
   }
   const int n_c = min(P.ccount[w], L.K);
   const int n_r = L.Nj + L.Nctrl + 3 * n_c;
-  const DenseView D = dense_view<GW>(sm + P.lay.dense, n_r);
+  const DenseView D = dense_view<GW>(sm + P.lay.dense, n_r, B);
   DenseRow<GW> Adense;  // row of the assembled Schur matrix (DENSE)
+  // row -> body sums by staged records instead of LDS atomics (DENSE: at most 64 rows)
+  constexpr bool kGather = DENSE;
   wsync();
   // ---- step-constant row data (model, control, contacts, Jacobians at the step-start
   // poses) into LDS, once per step ----
@@ -693,8 +810,8 @@ fused_step_kernel(FusedParams P) {
       int i = lane + GW * k;
       alt[k] = i < n_r ? P.lambda_seed2[(size_t)w * L.Nc + original_row(L, i, n_c)] : 0.f;
     }
-    const float ma = initial_merit<R, GW>(P, sm, lamprev, js, cs, rj, rk, lane, n_r, n_c, lam);
-    const float mb = initial_merit<R, GW>(P, sm, lamprev, js, cs, rj, rk, lane, n_r, n_c, alt);
+    const float ma = initial_merit<R, GW, kGather>(P, sm, lamprev, js, cs, rj, rk, lane, n_r, n_c, lam, D.rowd);
+    const float mb = initial_merit<R, GW, kGather>(P, sm, lamprev, js, cs, rj, rk, lane, n_r, n_c, alt, D.rowd);
     if (mb < ma) {
       seed_flag = FLAG_LEARNED_SEED;
 #pragma unroll
@@ -739,14 +856,15 @@ fused_step_kernel(FusedParams P) {
           // the previous iteration evaluated them at this same (q, u, lambda); replay
           // their -J^T lambda dt into h_d
           o = rcache[k];
-          if (o.active) {
+          if (o.active && !kGather) {
             const float sc = -lam[k] * dt;
             if (o.b0 >= 0) atomic_add_sv(sm + o.b0 * BS + BHD, o.J0 * sc);
             if (o.b1 >= 0) atomic_add_sv(sm + o.b1 * BS + BHD, o.J1 * sc);
           }
         } else {
-          o = assemble_row_cached(P, sm, lamprev, js, cs, i, n_c, rj[k], rk[k], lam[k]);
+          o = assemble_row_cached(P, sm, lamprev, js, cs, i, n_c, rj[k], rk[k], lam[k], !kGather);
         }
+        if (kGather) stage_contrib(D.rowd, i, o.active, o.b0, o.b1, o.J0 * (-lam[k] * dt), o.J1 * (-lam[k] * dt));
         J0[k] = o.J0; J1[k] = o.J1; b0[k] = o.b0; b1[k] = o.b1; act[k] = o.active;
         hc[k] = o.hc;
         Cr[k] = o.C + reg;
@@ -758,6 +876,10 @@ fused_step_kernel(FusedParams P) {
       }
     }
     wsync();
+    if (kGather) {
+      gather_contrib<GW>(sm, D.rowd, lane, n_r, B, BHD, false);
+      wsync();
+    }
     OFB_PHASE(1)
     // M^-1 h_d per body
     for (int b = lane; b < B; b += GW) {
@@ -798,22 +920,29 @@ fused_step_kernel(FusedParams P) {
     }
 
     OFB_PHASE(2)
-    if constexpr (DENSE) dense_build(D, Adense, lane, act[0], b0[0], b1[0], J0[0], J1[0], W0[0], W1[0], Cr[0]);
+    if constexpr (DENSE) dense_build(D, Adense, lane, B, act[0], b0[0], b1[0], J0[0], J1[0], W0[0], W1[0], Cr[0]);
     OFB_PHASE(7)
     pcr_total += pcr_solve<R, BJ, GW, DENSE>(sm, lane, B, V, G, act, b0, b1, J0, J1, Cr, Pi, rhs, bsq, tol_sq,
                                              latol_sq, P.cfg.lin_max_iters, x, D, Adense);
     OFB_PHASE(3)
 
     // ---- velocity recovery + Newton update (linear_system.py:344-377, newton_step.py:22-54) ----
-    for (int e = lane; e < 6 * B; e += GW) sm[(e / 6) * BS + BACC + (e % 6)] = 0.f;
-    wsync();
+    if (kGather) {
+      stage_contrib(D.rowd, lane, act[0] && lane < n_r, b0[0], b1[0], J0[0] * x[0], J1[0] * x[0]);
+      if (P.ls_count == 0) lam[0] = lam[0] + x[0];
+      wsync();
+      gather_contrib<GW>(sm, D.rowd, lane, n_r, B, BACC, true);
+    } else {
+      for (int e = lane; e < 6 * B; e += GW) sm[(e / 6) * BS + BACC + (e % 6)] = 0.f;
+      wsync();
 #pragma unroll
-    for (int k = 0; k < R; ++k) {
-      if (act[k]) {
-        if (b0[k] >= 0) atomic_add_sv(sm + b0[k] * BS + BACC, J0[k] * x[k]);
-        if (b1[k] >= 0) atomic_add_sv(sm + b1[k] * BS + BACC, J1[k] * x[k]);
+      for (int k = 0; k < R; ++k) {
+        if (act[k]) {
+          if (b0[k] >= 0) atomic_add_sv(sm + b0[k] * BS + BACC, J0[k] * x[k]);
+          if (b1[k] >= 0) atomic_add_sv(sm + b1[k] * BS + BACC, J1[k] * x[k]);
+        }
+        if (P.ls_count == 0) lam[k] = lam[k] + x[k];
       }
-      if (P.ls_count == 0) lam[k] = lam[k] + x[k];
     }
     wsync();
     if (P.ls_count == 0) {
@@ -860,11 +989,16 @@ fused_step_kernel(FusedParams P) {
         for (int k = 0; k < R; ++k) {
           int i = lane + GW * k;
           if (i < n_r) {
-            RowOut o = assemble_row_cached(P, sm, lamprev, js, cs, i, n_c, rj[k], rk[k], la[k]);
+            RowOut o = assemble_row_cached(P, sm, lamprev, js, cs, i, n_c, rj[k], rk[k], la[k], !kGather);
             ms += o.hc * o.hc;
+            if (kGather) stage_contrib(D.rowd, i, o.active, o.b0, o.b1, o.J0 * (-la[k] * dt), o.J1 * (-la[k] * dt));
           }
         }
         wsync();
+        if (kGather) {
+          gather_contrib<GW>(sm, D.rowd, lane, n_r, B, BHD, false);
+          wsync();
+        }
         for (int e = lane; e < 6 * B; e += GW) {
           float v = sm[(e / 6) * BS + BHD + (e % 6)];
           ms += v * v;
@@ -894,13 +1028,18 @@ fused_step_kernel(FusedParams P) {
     for (int k = 0; k < R; ++k) {
       int i = lane + GW * k;
       if (i < n_r) {
-        RowOut o = assemble_row_cached(P, sm, lamprev, js, cs, i, n_c, rj[k], rk[k], lam[k]);
+        RowOut o = assemble_row_cached(P, sm, lamprev, js, cs, i, n_c, rj[k], rk[k], lam[k], !kGather);
         msum += o.hc * o.hc;
         if (kRowCache) rcache[k] = o;
+        if (kGather) stage_contrib(D.rowd, i, o.active, o.b0, o.b1, o.J0 * (-lam[k] * dt), o.J1 * (-lam[k] * dt));
       }
     }
     rvalid = true;
     wsync();
+    if (kGather) {
+      gather_contrib<GW>(sm, D.rowd, lane, n_r, B, BHD, false);
+      wsync();
+    }
     for (int e = lane; e < 6 * B; e += GW) {
       float v = sm[(e / 6) * BS + BHD + (e % 6)];
       msum += v * v;

@@ -154,7 +154,8 @@ __device__ __forceinline__ sv apply_minv(const float* body, const sv& v) {
 
 // ---------------------------------------------------------------------------------
 // Per-row computations.  Each returns the linearisation (Jhat pair, C, active) and the
-// residual h_c, and scatters -Jhat^T lambda dt into the dynamics residual.
+// residual h_c, and scatters -Jhat^T lambda dt into the dynamics residual with LDS atomics
+// (with scatter = false the caller sums the contributions itself).
 // ---------------------------------------------------------------------------------
 struct RowOut {
   sv J0, J1;
@@ -260,7 +261,7 @@ __device__ __forceinline__ void joint_frames(const JointStatic& s, const float* 
 
 // row k of a joint (constraints/joint_constraint.py:62-162, joint_kinematics.py:5-261)
 template <class PT>
-__device__ RowOut joint_row_s(const PT& P, float* sm, const JointStatic& s, int k, float lam) {
+__device__ RowOut joint_row_s(const PT& P, float* sm, const JointStatic& s, int k, float lam, bool scatter = true) {
   RowOut o;
   o.J0 = svzero(); o.J1 = svzero(); o.hc = 0.f; o.C = 0.f; o.active = false;
   o.b0 = s.parent;
@@ -317,8 +318,10 @@ __device__ RowOut joint_row_s(const PT& P, float* sm, const JointStatic& s, int 
   if (!act) return o;
   o.active = true;
   float dt = P.dt;
-  if (o.b0 >= 0) atomic_add_sv(sm + o.b0 * BS + BHD, Jp * (-lam * dt));
-  atomic_add_sv(sm + o.b1 * BS + BHD, Jc * (-lam * dt));
+  if (scatter) {
+    if (o.b0 >= 0) atomic_add_sv(sm + o.b0 * BS + BHD, Jp * (-lam * dt));
+    atomic_add_sv(sm + o.b1 * BS + BHD, Jc * (-lam * dt));
+  }
   o.hc = (err + comp_ * lam) / dt;
   o.J0 = Jp; o.J1 = Jc;
   o.C = comp_ / (dt * dt);
@@ -332,7 +335,7 @@ __device__ RowOut joint_row(const PT& P, float* sm, int w, int row, float lam, b
 
 // control row of a joint (constraints/control_constraint.py:117-338)
 template <class PT>
-__device__ RowOut control_row_s(const PT& P, float* sm, const JointStatic& s, float lam) {
+__device__ RowOut control_row_s(const PT& P, float* sm, const JointStatic& s, float lam, bool scatter = true) {
   RowOut o;
   o.J0 = svzero(); o.J1 = svzero(); o.hc = 0.f; o.C = 0.f; o.active = false;
   o.b0 = s.parent;
@@ -393,8 +396,10 @@ __device__ RowOut control_row_s(const PT& P, float* sm, const JointStatic& s, fl
     alpha = den > 1e-6f ? 1.f / den : 1.0e8f;
   }
   o.active = true;
-  if (o.b0 >= 0) atomic_add_sv(sm + o.b0 * BS + BHD, Jp * (-lam * dt));
-  atomic_add_sv(sm + o.b1 * BS + BHD, Jc * (-lam * dt));
+  if (scatter) {
+    if (o.b0 >= 0) atomic_add_sv(sm + o.b0 * BS + BHD, Jp * (-lam * dt));
+    atomic_add_sv(sm + o.b1 * BS + BHD, Jc * (-lam * dt));
+  }
   o.hc = err + alpha * lam * dt;
   o.J0 = Jp; o.J1 = Jc;
   o.C = alpha;
@@ -480,7 +485,7 @@ __device__ __forceinline__ NormalStatic normal_static(const PT& P, const float* 
 
 // contact normal row at the current poses (constraints/contact_constraint.py:48-374)
 template <class PT>
-__device__ RowOut normal_row_s(const PT& P, float* sm, const NormalStatic& s, float lam) {
+__device__ RowOut normal_row_s(const PT& P, float* sm, const NormalStatic& s, float lam, bool scatter = true) {
   RowOut o;
   o.J0 = svzero(); o.J1 = svzero(); o.hc = 0.f; o.C = 0.f; o.active = false; o.b0 = -1; o.b1 = -1;
   if (s.skip) return o;
@@ -506,8 +511,10 @@ __device__ RowOut normal_row_s(const PT& P, float* sm, const NormalStatic& s, fl
   o.J0 = s.Ja * dphi_dg;
   o.J1 = s.Jb * dphi_dg;
   o.active = true;
-  if (s.b0 >= 0) atomic_add_sv(sm + s.b0 * BS + BHD, o.J0 * (-lam * dt));
-  if (s.b1 >= 0) atomic_add_sv(sm + s.b1 * BS + BHD, o.J1 * (-lam * dt));
+  if (scatter) {
+    if (s.b0 >= 0) atomic_add_sv(sm + s.b0 * BS + BHD, o.J0 * (-lam * dt));
+    if (s.b1 >= 0) atomic_add_sv(sm + s.b1 * BS + BHD, o.J1 * (-lam * dt));
+  }
   o.hc = phi / dt;
   o.C = (dphi_dl + P.cfg.compliance_contact) / (dt * dt);
   return o;
@@ -666,7 +673,7 @@ __device__ __forceinline__ FrictionStatic load_friction_static(const float* d) {
 // Stribeck scaling, the elliptic cone) and 0 where the row does not depend on mu.
 template <class PT>
 __device__ RowOut friction_row_s(const PT& P, float* sm, const float* lamprev, const FrictionStatic& s, int c,
-                                 int t, int n_c, float lam, float* dwdmu = nullptr) {
+                                 int t, int n_c, float lam, float* dwdmu = nullptr, bool scatter = true) {
   if (dwdmu) *dwdmu = 0.f;
   RowOut o;
   o.J0 = svzero(); o.J1 = svzero(); o.hc = 0.f; o.C = 0.f; o.active = false; o.b0 = -1; o.b1 = -1;
@@ -739,8 +746,10 @@ __device__ RowOut friction_row_s(const PT& P, float* sm, const float* lamprev, c
   sv Jb = t == 0 ? s.Jt1b : s.Jt2b;
   float wgt = t == 0 ? wx : wy;
   float vt = t == 0 ? vt0 : vt1;
-  if (s.b0 >= 0) atomic_add_sv(sm + s.b0 * BS + BHD, Ja * (-dt * lam));
-  if (s.b1 >= 0) atomic_add_sv(sm + s.b1 * BS + BHD, Jb * (-dt * lam));
+  if (scatter) {
+    if (s.b0 >= 0) atomic_add_sv(sm + s.b0 * BS + BHD, Ja * (-dt * lam));
+    if (s.b1 >= 0) atomic_add_sv(sm + s.b1 * BS + BHD, Jb * (-dt * lam));
+  }
   o.hc = vt + wgt * lam;
   o.J0 = Ja; o.J1 = Jb;
   o.C = wgt / dt + P.cfg.compliance_friction;
@@ -771,14 +780,16 @@ __device__ __forceinline__ RowOut assemble_row(const PT& P, float* sm, const flo
 // joint (joint rows), the row's joint (control rows)
 template <class PT>
 __device__ __forceinline__ RowOut assemble_row_cached(const PT& P, float* sm, const float* lamprev, const float* js,
-                                                      const float* cs, int i, int n_c, int rj, int rk, float lam) {
+                                                      const float* cs, int i, int n_c, int rj, int rk, float lam,
+                                                      bool scatter = true) {
   const int Nj = P.L.Nj, Nct = P.L.Nctrl;
-  if (i < Nj) return joint_row_s(P, sm, load_joint_static(js + JS_WORDS * rj), rk, lam);
-  if (i < Nj + Nct) return control_row_s(P, sm, load_joint_static(js + JS_WORDS * rj), lam);
+  if (i < Nj) return joint_row_s(P, sm, load_joint_static(js + JS_WORDS * rj), rk, lam, scatter);
+  if (i < Nj + Nct) return control_row_s(P, sm, load_joint_static(js + JS_WORDS * rj), lam, scatter);
   int c = i - Nj - Nct;
-  if (c < n_c) return normal_row_s(P, sm, load_normal_static(cs + CS_WORDS * c), lam);
+  if (c < n_c) return normal_row_s(P, sm, load_normal_static(cs + CS_WORDS * c), lam, scatter);
   int f = c - n_c;
-  return friction_row_s(P, sm, lamprev, load_friction_static(cs + CS_WORDS * (f >> 1)), f >> 1, f & 1, n_c, lam);
+  return friction_row_s(P, sm, lamprev, load_friction_static(cs + CS_WORDS * (f >> 1)), f >> 1, f & 1, n_c, lam,
+                        nullptr, scatter);
 }
 
 __device__ __forceinline__ int original_row(const RowLayout& L, int i, int n_c) {
