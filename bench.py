@@ -36,7 +36,7 @@ K_CONTACTS = 128
 HBM_PEAK_GBS = 8000.0
 
 
-def solver_config(path, termination="per_world", precond="jacobi", seed="cold", paired=True):
+def solver_config(path, termination="per_world", precond="jacobi", seed="cold", paired=True, K=K_CONTACTS):
     """seed: "cold" (lambda = 0, the reference default), "prev" (the cross-step contact warm
     starter seeds Newton: warm_start.seed_iterate), "learned" (prev + the learned
     per-contact network, gated per world).  paired: worlds of at most 32 rows run two per
@@ -44,7 +44,7 @@ def solver_config(path, termination="per_world", precond="jacobi", seed="cold", 
     return OstrichEngineConfig(
         nr=NewtonRaphsonConfig(max_iters=20, atol=1e-5),
         linear=LinearSolverConfig(max_iters=16, preconditioner_type=precond),
-        contacts=ContactsConfig(max_per_world=K_CONTACTS),
+        contacts=ContactsConfig(max_per_world=K),
         warm_start=WarmStartConfig(enabled=True, seed_iterate=seed in ("prev", "learned")),
         execution=ExecutionConfig(path=path, termination=termination, pair_small_worlds=paired),
     )
@@ -71,13 +71,13 @@ def operand_bytes_per_world(n_rows_active, nr_iters, pcr_iters):
 NET_PATH = os.path.join(ROOT, "ostrich_amd", "data", "warm_start_net.npy")
 
 
-def run_path(variant, model, state, control, steps, warmup, device, collider):
+def run_path(variant, model, state, control, steps, warmup, device, collider, K=K_CONTACTS):
     from ostrich_amd import OstrichEngine
     parts = variant.split(":")
     path, precond = parts[0], parts[1]
     seed = parts[2] if len(parts) > 2 else "cold"
     paired = not (len(parts) > 3 and parts[3] == "unpaired")
-    cfg = solver_config(path, precond=precond, seed=seed, paired=paired)
+    cfg = solver_config(path, precond=precond, seed=seed, paired=paired, K=K)
     eng = OstrichEngine(model, config=cfg, device=device)
     if seed == "learned":
         from ostrich_amd.learned import WarmStartNet
@@ -105,7 +105,7 @@ def run_path(variant, model, state, control, steps, warmup, device, collider):
             nr.append(st["nr_iters"].float().mean().item())
             pcr.append(st["pcr_iters"].float().mean().item())
             conv.append((st["merit"] <= cfg.nr.atol ** 2).float().mean().item())
-            ncon.append(eng.packed_contacts()["count"].clamp(max=K_CONTACTS).float().mean().item())
+            ncon.append(eng.packed_contacts()["count"].clamp(max=K).float().mean().item())
             lrn.append(((st["flags"] & 32) != 0).float().mean().item())
         s_a, s_b = s_b, s_a
     torch.cuda.synchronize()
@@ -115,6 +115,19 @@ def run_path(variant, model, state, control, steps, warmup, device, collider):
                 nr_mean=float(np.mean(nr)), pcr_per_step_mean=float(np.mean(pcr)),
                 converged_fraction=float(np.mean(conv)), contacts_mean=float(np.mean(ncon)),
                 learned_seed_fraction=float(np.mean(lrn)))
+
+
+def scene_leg(name, model_fn, worlds, K, steps, warmup, device, variants=("fused:jacobi:prev",)):
+    """Another workload through the same solver paths (a parity-test configuration, not
+    the headline): world-steps/s, Newton counts and contact counts per variant."""
+    model, state, control = model_fn(worlds)
+    collider = collide.PrimitiveCollider(model, device=device, gap=0.02, max_contacts_per_world=K)
+    out = dict(worlds=worlds, contact_slots=K, steps=steps, warmup=warmup)
+    for v in variants:
+        print(f"[bench] {name} path={v} worlds={worlds}", file=sys.stderr, flush=True)
+        out[v] = run_path(v, model, state, control, steps, warmup, device, collider, K=K)
+        print(f"[bench] {name} {v}: {out[v]}", file=sys.stderr, flush=True)
+    return out
 
 
 def adjoint_leg(device, worlds=1024, steps=50, checkpoint_every=10, seed=3):
@@ -205,6 +218,7 @@ def main():
     ap.add_argument("--cpu-sample-steps", type=int, default=40)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-scenes", action="store_true", help="skip the tracked-robot and box-stack legs")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 256 MiB device copy (reference kernel for HBM byte counters)")
     args = ap.parse_args()
@@ -279,6 +293,15 @@ def main():
     fw = results.get(main_path, {})
     if "world_steps_per_s" in fw and "world_steps_per_s" in pj:
         out["fused_warm_over_plain_cold"] = fw["world_steps_per_s"] / pj["world_steps_per_s"]
+    if not args.no_scenes:
+        # tracked flipper robots on a Gaussian-bump heightfield (1024 worlds, anisotropic
+        # track friction, position-controlled flippers) and 3-cube box stacks (16 worlds)
+        out["tracked_flipper"] = scene_leg("tracked_flipper", lambda n: scenes.tracked_flipper(n, seed=2), 1024, 64,
+                                           args.steps, max(args.warmup, 6), device,
+                                           ("fused:jacobi:prev", "fused:jacobi:cold"))
+        out["box_stack"] = scene_leg("box_stack", lambda n: scenes.box_stack(n, 3, seed=0), 16, 16, args.steps,
+                                     args.warmup, device, ("fused:jacobi:prev", "fused:jacobi:cold",
+                                                           "plain:jacobi:cold"))
     if not args.no_adjoint:
         print("[bench] adjoint rollout", file=sys.stderr, flush=True)
         out["adjoint"] = adjoint_leg(device)

@@ -1,4 +1,5 @@
-"""Primitive collider for the benchmark scenes (plane / box / cylinder / sphere).
+"""Primitive collider for the benchmark scenes (plane / box / cylinder / sphere /
+heightfield).
 
 It produces Newton-format rigid contacts (flat arrays; normal from shape0 to shape1;
 point0/point1 in the frame of the owning body, world frame for static shapes; margins),
@@ -17,7 +18,7 @@ import math
 import numpy as np
 import torch
 
-from .model import GEO_BOX, GEO_CYLINDER, GEO_PLANE, GEO_SPHERE
+from .model import GEO_BOX, GEO_CYLINDER, GEO_HEIGHTFIELD, GEO_PLANE, GEO_SPHERE
 
 
 def _quat_rotate(q, v):
@@ -100,6 +101,16 @@ class PrimitiveCollider:
                         continue
                 pairs.append((a, b))
         self.pairs = pairs
+        # spheres on bodies against the heightfield (one contact per sphere: the point of
+        # the bilinear surface below its centre, with the surface normal there)
+        hcols = [c for c in range(SC) if g0[c] == GEO_HEIGHTFIELD]
+        self.hf_col = hcols[0] if hcols else -1
+        self.hf_spheres = [a for a in range(SC) if b0[a] >= 0 and g0[a] == GEO_SPHERE] if hcols else []
+        if self.hf_spheres:
+            hf = model.heightfield
+            self.hf_h = torch.as_tensor(np.asarray(hf["heights"], np.float32), device=self.device)
+            self.hf_origin = tuple(float(v) for v in hf["origin"])
+            self.hf_spacing = tuple(float(v) for v in hf["spacing"])
 
     def _shape_pose(self, body_q, col):
         """World pose [W,7] of shape column `col` for every world."""
@@ -134,6 +145,41 @@ class PrimitiveCollider:
         if geo == GEO_SPHERE:
             return torch.zeros(size.shape[0], 1, 3, dtype=size.dtype, device=self.device)
         raise ValueError(f"no samples for geometry {geo}")
+
+    def heightfield_surface(self, x, y):
+        """Bilinear height and its gradient at world (x, y) (clamped to the grid)."""
+        h = self.hf_h
+        ny, nx = h.shape
+        u = ((x - self.hf_origin[0]) / self.hf_spacing[0]).clamp(0.0, nx - 1.0)
+        v = ((y - self.hf_origin[1]) / self.hf_spacing[1]).clamp(0.0, ny - 1.0)
+        i = u.floor().long().clamp(max=nx - 2)
+        j = v.floor().long().clamp(max=ny - 2)
+        fu, fv = u - i, v - j
+        h00, h10, h01, h11 = h[j, i], h[j, i + 1], h[j + 1, i], h[j + 1, i + 1]
+        z = h00 * (1 - fu) * (1 - fv) + h10 * fu * (1 - fv) + h01 * (1 - fu) * fv + h11 * fu * fv
+        dzdx = ((h10 - h00) * (1 - fv) + (h11 - h01) * fv) / self.hf_spacing[0]
+        dzdy = ((h01 - h00) * (1 - fu) + (h11 - h10) * fu) / self.hf_spacing[1]
+        return z, dzdx, dzdy
+
+    def _heightfield_contacts(self, body_q):
+        W, B = self.model.num_worlds, self.model.bodies_per_world
+        cols = torch.as_tensor(self.hf_spheres, device=self.device)
+        bidx = self.sbody[:, cols].long()                                          # [W,Ns]
+        bq = body_q.view(W, B, 7)[torch.arange(W, device=self.device)[:, None], bidx]  # [W,Ns,7]
+        c = bq[..., :3] + _quat_rotate(bq[..., 3:], self.xform[:, cols, :3])
+        r = self.size[:, cols, 0]                                                   # [W,Ns]
+        z, gx, gy = self.heightfield_surface(c[..., 0], c[..., 1])
+        nrm = torch.stack([-gx, -gy, torch.ones_like(z)], dim=-1)
+        nrm = nrm / nrm.norm(dim=-1, keepdim=True)
+        ground = torch.stack([c[..., 0], c[..., 1], z], dim=-1)
+        dist = ((c - ground) * nrm).sum(-1) - r
+        pw = c - nrm * r[..., None]
+        p1w = pw - nrm * dist[..., None]
+        p0 = _quat_rotate_inv(bq[..., 3:], pw - bq[..., :3])
+        fa = torch.as_tensor(self.model.flat_shape_id(np.arange(W)[:, None], np.array(self.hf_spheres)[None, :]),
+                             device=self.device)
+        fb = torch.as_tensor(self.model.flat_shape_id(np.arange(W), np.full(W, self.hf_col)), device=self.device)
+        return p0, p1w, -nrm, fa, fb[:, None].expand_as(fa), dist < self.gap
 
     def collide(self, body_q):
         """body_q: [W*B, 7] tensor on the collider device.  Returns a dict of flat
@@ -209,6 +255,14 @@ class PrimitiveCollider:
             fb = torch.as_tensor(self.model.flat_shape_id(np.arange(W), np.full(W, b)), device=self.device)
             out_s0.append(fa[:, None].expand(W, P))
             out_s1.append(fb[:, None].expand(W, P))
+            out_valid.append(valid)
+        if self.hf_spheres:
+            p0, p1, n_neg, s0, s1, valid = self._heightfield_contacts(body_q)
+            out_p0.append(p0)
+            out_p1.append(p1)
+            out_n.append(n_neg)
+            out_s0.append(s0)
+            out_s1.append(s1)
             out_valid.append(valid)
         p0 = torch.cat(out_p0, 1)
         p1 = torch.cat(out_p1, 1)

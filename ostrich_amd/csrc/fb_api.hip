@@ -9,7 +9,6 @@
 #include <vector>
 
 #include <hip/hip_runtime.h>
-#include <rocprim/device/device_radix_sort.hpp>
 
 #include "fb_solver.h"
 
@@ -65,64 +64,51 @@ static int joint_row_count(int type) {  // core/model.py:88-99
 // ---------------------------------------------------------------------------------
 // Contact packing (core/contacts.py:25-99).  The reference assigns slots with a
 // per-world atomic counter, which orders a world's contacts by thread id on Warp's CPU
-// device and arbitrarily on a GPU.  Here the order is the input order, on every device:
-// contacts are keyed by world, stably partitioned with a radix sort (keys only span
-// ceil(log2(W+1)) bits), and each contact's slot is its position inside its world's
-// segment.
+// device and arbitrarily on a GPU.  Here the order is the input order, on every device,
+// in two passes over the valid prefix of the flat arrays (the producer's count):
+//   1. each contact finds its world and reserves a member slot with a per-world atomic
+//      counter (the count is order-independent; the member order is not);
+//   2. one wavefront per world ranks its members by contact index (ascending = input
+//      order) and writes contact r to slot r.
+// A world with more than K contacts keeps its first K in input order: its wavefront
+// scans the world keys of the valid prefix with ballots (a rare path, flagged with
+// FLAG_CONTACT_OVERFLOW).  The counter keeps counting past K, as the reference's does
+// (contacts.py:73-76).
 // ---------------------------------------------------------------------------------
-__global__ void pack_keys_kernel(int n_max, const int32_t* __restrict__ count, const int32_t* __restrict__ s0,
-                                 const int32_t* __restrict__ s1, const int32_t* __restrict__ shape_world, int W,
-                                 int32_t* __restrict__ keys, int32_t* __restrict__ vals) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n_max) return;
-  int key = W;
-  int a = s0[i], b = s1[i];
-  if (i < count[0] && a != b) {
-    int w0 = a >= 0 ? shape_world[a] : -1;
-    int w1 = b >= 0 ? shape_world[b] : -1;
-    int w = w0 >= 0 ? w0 : w1;
-    if (w >= 0) key = w;
-  }
-  keys[i] = key;
-  vals[i] = i;
+constexpr int kPackWave = 64;  // lanes per wavefront (gfx950)
+
+__device__ __forceinline__ int contact_world(int a, int b, const int32_t* __restrict__ shape_world) {
+  if (a == b) return -1;
+  int w0 = a >= 0 ? shape_world[a] : -1;
+  int w1 = b >= 0 ? shape_world[b] : -1;
+  return w0 >= 0 ? w0 : w1;
 }
 
-__global__ void pack_reset_kernel(int W, int32_t* __restrict__ ccount, int32_t* __restrict__ seg_start,
-                                  int32_t* __restrict__ flags) {
+__global__ void pack_reset_kernel(int W, int32_t* __restrict__ ccount, int32_t* __restrict__ flags) {
   int w = blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= W) return;
   ccount[w] = 0;
-  seg_start[w] = 0;
   flags[w] = 0;
 }
 
-__global__ void pack_segments_kernel(int n_max, int W, const int32_t* __restrict__ keys,
-                                     int32_t* __restrict__ seg_start, int32_t* __restrict__ ccount) {
-  int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n_max) return;
-  int k = keys[j];
-  if (k >= W) return;
-  if (j == 0 || keys[j - 1] != k) seg_start[k] = j;
-  if (j == n_max - 1 || keys[j + 1] != k) ccount[k] = j + 1;  // end, fixed up below
+__global__ void pack_members_kernel(int n_max, const int32_t* __restrict__ count, const int32_t* __restrict__ s0,
+                                    const int32_t* __restrict__ s1, const int32_t* __restrict__ shape_world, int W,
+                                    int K, int32_t* __restrict__ ccount, int32_t* __restrict__ members,
+                                    int32_t* __restrict__ keys) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_max || i >= count[0]) return;
+  int w = contact_world(s0[i], s1[i], shape_world);
+  keys[i] = w;
+  if (w < 0 || w >= W) return;
+  int slot = atomicAdd(ccount + w, 1);
+  if (slot < K) members[(size_t)w * K + slot] = i;
 }
 
-__global__ void pack_scatter_kernel(int n_max, int W, int G, int S, int K, const int32_t* __restrict__ keys,
-                                    const int32_t* __restrict__ vals, const int32_t* __restrict__ seg_start,
-                                    const float* __restrict__ p0, const float* __restrict__ p1,
-                                    const float* __restrict__ nrm, const int32_t* __restrict__ s0,
-                                    const int32_t* __restrict__ s1, const float* __restrict__ m0,
-                                    const float* __restrict__ m1, float* __restrict__ cf, int32_t* __restrict__ ci,
-                                    int32_t* __restrict__ flags) {
-  int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n_max) return;
-  int w = keys[j];
-  if (w >= W) return;
-  int slot = j - seg_start[w];
-  if (slot >= K) {
-    if (slot == K) flags[w] |= FLAG_CONTACT_OVERFLOW;
-    return;
-  }
-  int i = vals[j];
+__device__ __forceinline__ void pack_write(int w, int slot, int i, int G, int S, int K, const float* __restrict__ p0,
+                                           const float* __restrict__ p1, const float* __restrict__ nrm,
+                                           const int32_t* __restrict__ s0, const int32_t* __restrict__ s1,
+                                           const float* __restrict__ m0, const float* __restrict__ m1,
+                                           float* __restrict__ cf, int32_t* __restrict__ ci) {
   float* f = cf + (size_t)w * CF * K + slot;
   f[CF_P0X * K] = p0[i * 3 + 0]; f[CF_P0Y * K] = p0[i * 3 + 1]; f[CF_P0Z * K] = p0[i * 3 + 2];
   f[CF_P1X * K] = p1[i * 3 + 0]; f[CF_P1Y * K] = p1[i * 3 + 1]; f[CF_P1Z * K] = p1[i * 3 + 2];
@@ -137,11 +123,52 @@ __global__ void pack_scatter_kernel(int n_max, int W, int G, int S, int K, const
   ci[(size_t)w * 2 * K + K + slot] = b >= 0 ? local(b) : b;
 }
 
-__global__ void pack_counts_kernel(int W, const int32_t* __restrict__ seg_start, int32_t* __restrict__ ccount) {
-  int w = blockIdx.x * blockDim.x + threadIdx.x;
+// one wavefront per world; LDS: K member indices per wavefront
+__global__ void __launch_bounds__(256) pack_world_kernel(int W, int G, int S, int K, int n_max,
+                                                         const int32_t* __restrict__ count,
+                                                         const int32_t* __restrict__ ccount,
+                                                         const int32_t* __restrict__ members,
+                                                         const int32_t* __restrict__ keys,
+                                                         const float* __restrict__ p0, const float* __restrict__ p1,
+                                                         const float* __restrict__ nrm,
+                                                         const int32_t* __restrict__ s0,
+                                                         const int32_t* __restrict__ s1,
+                                                         const float* __restrict__ m0, const float* __restrict__ m1,
+                                                         float* __restrict__ cf, int32_t* __restrict__ ci,
+                                                         int32_t* __restrict__ flags) {
+  extern __shared__ int32_t mem_lds[];
+  const int lane = threadIdx.x & (kPackWave - 1), wave = threadIdx.x / kPackWave;
+  const int w = blockIdx.x * (blockDim.x / kPackWave) + wave;
   if (w >= W) return;
-  int end = ccount[w];
-  ccount[w] = end > 0 ? end - seg_start[w] : 0;
+  const int c = ccount[w];
+  if (c <= 0) return;
+  if (c <= K) {
+    int32_t* m = mem_lds + wave * K;
+    for (int e = lane; e < c; e += kPackWave) m[e] = members[(size_t)w * K + e];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int e = lane; e < c; e += kPackWave) {
+      const int i = m[e];
+      int rank = 0;
+      for (int q = 0; q < c; ++q) rank += m[q] < i ? 1 : 0;
+      pack_write(w, rank, i, G, S, K, p0, p1, nrm, s0, s1, m0, m1, cf, ci);
+    }
+    return;
+  }
+  // overflow: the first K contacts of the world in input order
+  if (lane == 0) flags[w] |= FLAG_CONTACT_OVERFLOW;
+  const int n = min(n_max, count[0]);
+  const unsigned long long below = (1ull << lane) - 1ull;
+  int found = 0;
+  for (int base = 0; base < n && found < K; base += kPackWave) {
+    const int i = base + lane;
+    const bool mine = i < n && keys[i] == w;
+    const unsigned long long mask = __ballot(mine);
+    const int rank = found + __popcll(mask & below);
+    if (mine && rank < K) pack_write(w, rank, i, G, S, K, p0, p1, nrm, s0, s1, m0, m1, cf, ci);
+    found += __popcll(mask);
+  }
 }
 
 static int pack_contacts(ofb_solver* s, const ofb_contacts* c, hipStream_t stream) {
@@ -151,22 +178,15 @@ static int pack_contacts(ofb_solver* s, const ofb_contacts* c, hipStream_t strea
     return -2;
   }
   const int T = 256;
-  hipLaunchKernelGGL(pack_reset_kernel, dim3((W + T - 1) / T), dim3(T), 0, stream, W, s->ccount, s->pk_seg_start,
-                     s->st_flags);
+  hipLaunchKernelGGL(pack_reset_kernel, dim3((W + T - 1) / T), dim3(T), 0, stream, W, s->ccount, s->st_flags);
   if (n == 0) return 0;
-  hipLaunchKernelGGL(pack_keys_kernel, dim3((n + T - 1) / T), dim3(T), 0, stream, n, c->count, c->shape0, c->shape1,
-                     s->m.shape_world, W, s->pk_keys, s->pk_vals);
-  unsigned end_bit = 1;
-  while ((1u << end_bit) <= (unsigned)W) ++end_bit;
-  size_t tb = s->pk_temp_bytes;
-  OFB_CHECK(rocprim::radix_sort_pairs(s->pk_temp, tb, s->pk_keys, s->pk_keys_out, s->pk_vals, s->pk_vals_out,
-                                      (unsigned)n, 0, end_bit, stream));
-  hipLaunchKernelGGL(pack_segments_kernel, dim3((n + T - 1) / T), dim3(T), 0, stream, n, W, s->pk_keys_out,
-                     s->pk_seg_start, s->ccount);
-  hipLaunchKernelGGL(pack_counts_kernel, dim3((W + T - 1) / T), dim3(T), 0, stream, W, s->pk_seg_start, s->ccount);
-  hipLaunchKernelGGL(pack_scatter_kernel, dim3((n + T - 1) / T), dim3(T), 0, stream, n, W, s->m.G, s->m.S,
-                     s->L.K, s->pk_keys_out, s->pk_vals_out, s->pk_seg_start, c->point0, c->point1, c->normal,
-                     c->shape0, c->shape1, c->margin0, c->margin1, s->cf, s->ci, s->st_flags);
+  const int K = s->L.K;
+  hipLaunchKernelGGL(pack_members_kernel, dim3((n + T - 1) / T), dim3(T), 0, stream, n, c->count, c->shape0,
+                     c->shape1, s->m.shape_world, W, K, s->ccount, s->pk_members, s->pk_keys);
+  const int wpb = T / kPackWave;
+  hipLaunchKernelGGL(pack_world_kernel, dim3((W + wpb - 1) / wpb), dim3(T), (size_t)wpb * K * sizeof(int32_t), stream,
+                     W, s->m.G, s->m.S, K, n, c->count, s->ccount, s->pk_members, s->pk_keys, c->point0, c->point1,
+                     c->normal, c->shape0, c->shape1, c->margin0, c->margin1, s->cf, s->ci, s->st_flags);
   OFB_CHECK(hipGetLastError());
   return 0;
 }
@@ -225,7 +245,7 @@ int ofb_destroy(ofb_solver* s) {
                   m.joint_compliance, m.joint_axis, m.joint_dof_mode, m.joint_target_ke, m.joint_target_kd,
                   m.shape_body, m.shape_mu, m.shape_friction_axis, m.shape_mu_perp, m.shape_mu_stiction_scale,
                   m.shape_v_stribeck, m.shape_stribeck_lateral_only, m.shape_world, s->ccount, s->cf, s->ci,
-                  s->pk_keys, s->pk_keys_out, s->pk_vals, s->pk_vals_out, s->pk_seg_start, s->pk_temp, s->topo,
+                  s->pk_keys, s->pk_members, s->topo,
                   s->lambda, s->lambda_prev, s->lambda_seed, s->ws_count, s->ws_b0, s->ws_b1, s->ws_p, s->ws_n,
                   s->ws_ln, s->ws_lt, s->ws_feat, s->ws_fscale, s->ws_y, s->lambda_seed2, s->ls_steps, s->st_nr, s->st_pcr, s->st_best, s->st_flags, s->st_merit};
   for (void* p : ptrs)
@@ -342,8 +362,7 @@ int ofb_create(const ofb_model* model, const ofb_config* config, int32_t path, o
   ALLOC(topo, topo.size());
   int cap = config->rigid_contact_max > 0 ? config->rigid_contact_max : W * L.K;
   s->rigid_contact_cap = cap;
-  ALLOC(pk_keys, cap); ALLOC(pk_keys_out, cap); ALLOC(pk_vals, cap); ALLOC(pk_vals_out, cap);
-  ALLOC(pk_seg_start, W);
+  ALLOC(pk_keys, cap); ALLOC(pk_members, (size_t)W * K);
   if (config->linesearch) {
     int n = linesearch_steps(config, s->ls_host, 512);
     if (n < 1) { set_error("invalid line-search step counts (at most 512 candidates)"); ofb_destroy(s); return -2; }
@@ -372,15 +391,6 @@ int ofb_create(const ofb_model* model, const ofb_config* config, int32_t path, o
     ofb_destroy(s);
     return -4;
   }
-  size_t tb = 0;
-  if (rocprim::radix_sort_pairs((void*)nullptr, tb, s->pk_keys, s->pk_keys_out, s->pk_vals, s->pk_vals_out,
-                                (unsigned)cap, 0, 31) != hipSuccess) {
-    set_error("radix sort sizing failed");
-    ofb_destroy(s);
-    return -4;
-  }
-  s->pk_temp_bytes = tb;
-  if (hipMalloc(&s->pk_temp, tb > 0 ? tb : 1) != hipSuccess) { set_error("hipMalloc failed"); ofb_destroy(s); return -4; }
   s->fused_worlds_per_block = 4;
   if (hipEventCreate(&s->ev_solve0) != hipSuccess || hipEventCreate(&s->ev_solve1) != hipSuccess) {
     set_error("hipEventCreate failed");

@@ -70,9 +70,8 @@ struct ofb_solver {
   int32_t* ci;       // [W*2*K]
   // contact packing scratch (stable partition by world)
   int32_t rigid_contact_cap;
-  int32_t *pk_keys, *pk_keys_out, *pk_vals, *pk_vals_out, *pk_seg_start;
-  void* pk_temp;
-  size_t pk_temp_bytes;
+  int32_t* pk_keys;     // [rigid_contact_cap] world of each flat contact (valid prefix)
+  int32_t* pk_members;  // [W*K] flat index of each world's contacts, in reservation order
   // outputs
   int32_t* topo;     // joint of each joint row [Nj], row within joint [Nj], joint of each control row [Nctrl]
   float* lambda;     // [W*Nc]

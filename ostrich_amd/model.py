@@ -15,7 +15,7 @@ from . import _abi
 
 JOINT_PRISMATIC, JOINT_REVOLUTE, JOINT_BALL, JOINT_FIXED, JOINT_FREE = 0, 1, 2, 3, 4
 DOF_NONE, DOF_TARGET_POSITION, DOF_TARGET_VELOCITY = 0, 1, 2
-GEO_PLANE, GEO_BOX, GEO_CYLINDER, GEO_SPHERE = 0, 1, 2, 3
+GEO_PLANE, GEO_BOX, GEO_CYLINDER, GEO_SPHERE, GEO_HEIGHTFIELD = 0, 1, 2, 3, 4
 
 # joint type -> number of DOFs (Newton joint_dof_dim)
 _JOINT_DOFS = {JOINT_PRISMATIC: 1, JOINT_REVOLUTE: 1, JOINT_BALL: 3, JOINT_FIXED: 0, JOINT_FREE: 6}
@@ -106,8 +106,13 @@ class Model:
         self.arrays = {}
 
     @staticmethod
-    def from_worlds(worlds, global_shapes=(), gravity=(0.0, 0.0, -9.81)):
+    def from_worlds(worlds, global_shapes=(), gravity=(0.0, 0.0, -9.81), heightfield=None):
+        """heightfield: geometry of the GEO_HEIGHTFIELD global shape, if there is one: a dict
+        with `heights` [ny, nx] (metres, row = y index), `origin` (x, y) of node (0, 0) and
+        `spacing` (dx, dy).  Only the collider reads it; the solver sees the shape's
+        material (mu) like any other static shape."""
         m = Model()
+        m.heightfield = heightfield
         W = len(worlds)
         w0 = worlds[0]
         B, S, J, D = len(w0.bodies), len(w0.shapes), len(w0.joints), len(w0.dofs)

@@ -9,6 +9,11 @@ skid_steer  - a 10 kg chassis box 0.6x0.4x0.2 m with three 1 kg cylinder wheels
               free base + three revolute wheels, axis y), wheel targets U(-5,5) rad/s,
               0-24 static boxes with edge U(0.05,0.3) m placed uniformly in a 4x4 m area,
               mu U(0.4,1.0).
+tracked_flipper - a 40 kg body box 0.9x0.5x0.3 m with four 3 kg flippers on
+              position-controlled revolute joints (targets U(-0.8,0.8) rad), six tracks of ten
+              r = 0.05 m sphere contact points with anisotropic friction (mu 1.0 along the
+              track, 0.5 across), on a shared 256x256 heightfield at 2 cm spacing made of 32
+              Gaussian bumps (amplitude U(0,0.15) m, sigma U(0.1,0.5) m).
 
 Each builder returns (model, state, control) with numpy arrays; `state` holds body_q,
 body_qd, body_f in Newton's layout and `control` the joint targets.
@@ -17,8 +22,9 @@ import math
 
 import numpy as np
 
-from .model import (DOF_TARGET_VELOCITY, GEO_BOX, GEO_CYLINDER, GEO_PLANE, IDENTITY7, Model,
-                    Shape, WorldBuilder, box_inertia, cylinder_inertia_y)
+from .model import (DOF_TARGET_POSITION, DOF_TARGET_VELOCITY, GEO_BOX, GEO_CYLINDER, GEO_HEIGHTFIELD,
+                    GEO_PLANE, GEO_SPHERE, IDENTITY7, Model, Shape, WorldBuilder, box_inertia,
+                    cylinder_inertia_y)
 
 
 def _yaw_quat(yaw):
@@ -111,3 +117,115 @@ def skid_steer(num_worlds=4096, seed=1, max_obstacles=SKID_MAX_OBSTACLES, ke=150
                    joint_target_vel=np.concatenate(targets).astype(np.float32))
     return model, state, control
 
+
+
+# tracked flipper robot on rough terrain
+TF_BODY_HALF = (0.45, 0.25, 0.15)
+TF_BODY_MASS = 40.0
+TF_FLIPPER_MASS = 3.0
+TF_FLIPPER_LEN = 0.4
+TF_FLIPPER_HALF = (0.2, 0.04, 0.06)
+TF_SPHERE_R = 0.05
+TF_TRACK_N = 10
+TF_MAIN_TRACK_Y = 0.2
+TF_HINGES = ((0.45, 0.33, -0.05, 1.0), (0.45, -0.33, -0.05, 1.0),   # (x, y, z, direction along x)
+             (-0.45, 0.33, -0.05, -1.0), (-0.45, -0.33, -0.05, -1.0))
+TF_KE, TF_KD = 2000.0, 50.0   # flipper position gains (N m / rad, N m s / rad)
+
+
+def gaussian_bump_heightfield(seed, n=256, spacing=0.02, bumps=32, amp=(0.0, 0.15), sigma=(0.1, 0.5)):
+    """Heights [n, n] of a sum of Gaussian bumps on a square grid centred on the origin."""
+    rng = np.random.default_rng(seed)
+    ext = (n - 1) * spacing
+    origin = (-0.5 * ext, -0.5 * ext)
+    xs = origin[0] + spacing * np.arange(n)
+    X, Y = np.meshgrid(xs, xs)  # row = y index
+    h = np.zeros((n, n))
+    for _ in range(bumps):
+        a = rng.uniform(*amp)
+        sg = rng.uniform(*sigma)
+        cx, cy = rng.uniform(origin[0], origin[0] + ext, size=2)
+        h += a * np.exp(-((X - cx) ** 2 + (Y - cy) ** 2) / (2.0 * sg * sg))
+    return dict(heights=h.astype(np.float32), origin=origin, spacing=(spacing, spacing))
+
+
+def heightfield_height(hf, x, y):
+    """Bilinear height of the heightfield at world (x, y) (numpy; clamped to the grid)."""
+    h = np.asarray(hf["heights"], np.float64)
+    ny, nx = h.shape
+    u = np.clip((np.asarray(x) - hf["origin"][0]) / hf["spacing"][0], 0.0, nx - 1.0)
+    v = np.clip((np.asarray(y) - hf["origin"][1]) / hf["spacing"][1], 0.0, ny - 1.0)
+    i = np.minimum(np.floor(u).astype(int), nx - 2)
+    j = np.minimum(np.floor(v).astype(int), ny - 2)
+    fu, fv = u - i, v - j
+    return (h[j, i] * (1 - fu) * (1 - fv) + h[j, i + 1] * fu * (1 - fv) + h[j + 1, i] * (1 - fu) * fv
+            + h[j + 1, i + 1] * fu * fv)
+
+
+def _quat_apply(q, v):
+    x, y, z, w = q
+    u = np.array([x, y, z])
+    v = np.asarray(v, np.float64)
+    return v * (2 * w * w - 1) + u * (2 * u.dot(v)) + np.cross(u, v) * (2 * w)
+
+
+def tracked_flipper(num_worlds=1024, seed=2, target_range=0.8, place=1.2, terrain_seed=None, ke=TF_KE, kd=TF_KD):
+    """Tracked flipper robots (5 bodies, 20 joint rows, 4 position motor rows) on one shared
+    heightfield; each world places its robot at a seeded position and yaw, just above the
+    terrain.  Every track sphere carries friction_axis = its body's x axis with mu 1.0 and
+    mu_perp 0.0 against a terrain of mu 1.0: the reference averages the two shapes'
+    coefficients (friction_constraint.py:143-156), so the effective coefficients are 1.0
+    along the track and 0.5 across it."""
+    rng = np.random.default_rng(seed)
+    hf = gaussian_bump_heightfield(seed if terrain_seed is None else terrain_seed)
+    terrain = Shape(GEO_HEIGHTFIELD, -1, IDENTITY7, (0.0, 0.0, 0.0), mu=1.0)
+    sphere_kw = dict(mu=1.0, friction_axis=(1.0, 0.0, 0.0), mu_perp=0.0)
+    bh = TF_BODY_HALF
+    worlds, qs, targets = [], [], []
+    for _ in range(num_worlds):
+        wb = WorldBuilder()
+        body = wb.add_body(TF_BODY_MASS, box_inertia(TF_BODY_MASS, *bh))
+        wb.add_joint_free(body)
+        pts = []  # (body, local point) of every track sphere, for the placement
+        for side in (1.0, -1.0):
+            for k in range(TF_TRACK_N):
+                x = -0.4 + 0.8 * k / (TF_TRACK_N - 1)
+                p = (x, side * TF_MAIN_TRACK_Y, -bh[2])
+                wb.add_shape(Shape(GEO_SPHERE, body, p + (0.0, 0.0, 0.0, 1.0), (TF_SPHERE_R, 0.0, 0.0), **sphere_kw))
+                pts.append(p)
+        hinges = []
+        for hx, hy, hz, d in TF_HINGES:
+            fl = wb.add_body(TF_FLIPPER_MASS, box_inertia(TF_FLIPPER_MASS, *TF_FLIPPER_HALF))
+            c_off = (d * 0.5 * TF_FLIPPER_LEN, 0.0, 0.0)  # flipper centre from its hinge
+            wb.add_joint_revolute(body, fl, X_p=(hx, hy, hz, 0.0, 0.0, 0.0, 1.0),
+                                  X_c=(-c_off[0], 0.0, 0.0, 0.0, 0.0, 0.0, 1.0), axis=(0.0, 1.0, 0.0),
+                                  mode=DOF_TARGET_POSITION, ke=ke, kd=kd)
+            for k in range(TF_TRACK_N):
+                s = 0.04 + (TF_FLIPPER_LEN - 0.04) * k / (TF_TRACK_N - 1)
+                p = (d * s - c_off[0], 0.0, -TF_FLIPPER_HALF[2])
+                wb.add_shape(Shape(GEO_SPHERE, fl, p + (0.0, 0.0, 0.0, 1.0), (TF_SPHERE_R, 0.0, 0.0), **sphere_kw))
+                pts.append((hx + d * s, hy, hz - TF_FLIPPER_HALF[2]))
+            hinges.append((hx + c_off[0], hy, hz))
+        x0, y0 = rng.uniform(-place, place, size=2)
+        yaw = rng.uniform(-np.pi, np.pi)
+        q = _yaw_quat(yaw)
+        pw = np.array([_quat_apply(q, p) for p in pts]) + np.array([x0, y0, 0.0])
+        ground = heightfield_height(hf, pw[:, 0], pw[:, 1])
+        z0 = float(np.max(ground - pw[:, 2])) + TF_SPHERE_R + 0.002
+        poses = [(x0, y0, z0) + q]
+        for c in hinges:
+            cw = _quat_apply(q, c)
+            poses.append((x0 + cw[0], y0 + cw[1], z0 + cw[2]) + q)
+        worlds.append(wb)
+        qs.append(poses)
+        t = np.zeros(len(wb.dofs), np.float32)
+        t[6:10] = rng.uniform(-target_range, target_range, size=4)
+        targets.append(t)
+    model = Model.from_worlds(worlds, [terrain], heightfield=hf)
+    B = model.bodies_per_world
+    body_q = np.array(qs, np.float32).reshape(num_worlds * B, 7)
+    state = dict(body_q=body_q, body_qd=np.zeros((num_worlds * B, 6), np.float32),
+                 body_f=np.zeros((num_worlds * B, 6), np.float32))
+    control = dict(joint_target_pos=np.concatenate(targets).astype(np.float32),
+                   joint_target_vel=np.zeros(num_worlds * model.dofs_per_world, np.float32))
+    return model, state, control
