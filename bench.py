@@ -61,22 +61,33 @@ def compulsory_bytes_per_world(model, n_c, n_rows):
 
 
 def operand_bytes_per_world(n_rows_active, nr_iters, pcr_iters):
-    """Operand bytes the algorithm touches per world-step (read from on-chip memory in
-    the fused kernel): 400 B per active row per Newton iteration (assembly, RHS, update,
-    residual operands) and 304 B per active row per PCR iteration (matvec 224, Jacobi 12,
-    three dots 20... see the design notes for the count)."""
+    """Operand bytes the algorithm touches per world-step (on-chip in the fused kernel):
+    400 B per active row per Newton iteration (assembly, RHS, update and residual
+    operands) and 304 B per active row per PCR iteration: the matvec (Jacobian pair read
+    twice 96, body indices 8, x / C / Az 12, body scatter and gather 96), the Jacobi
+    scaling 12, three dot products 20 and two vector updates 60."""
     return n_rows_active * (400.0 * nr_iters + 304.0 * pcr_iters)
 
 
 NET_PATH = os.path.join(ROOT, "ostrich_amd", "data", "warm_start_net.npy")
+GRAPH_PATH = "fused:jacobi:prev:graph"
 
 
-def run_path(variant, model, state, control, steps, warmup, device, collider, K=K_CONTACTS):
+def run_path(variant, model, state, control, steps, warmup, device, collider, K=K_CONTACTS, tune=False):
+    """One solver variant "path:preconditioner[:seed[:option...]]" over warmup + steps
+    closed-loop steps.  Options: "unpaired" (worlds of at most 32 rows one per wavefront),
+    "graph" (each step is a replay of a HIP graph recorded over static tensors; the
+    collider's contacts are copied into them before each replay, untimed).  tune: run the
+    launch-shape autotuner on the first step's inputs before timing.  Step times are HIP
+    events on the stream around each step (or replay); kernel times are the paired-class
+    kernel's own events (eager steps only)."""
     from ostrich_amd import OstrichEngine
     parts = variant.split(":")
     path, precond = parts[0], parts[1]
     seed = parts[2] if len(parts) > 2 else "cold"
-    paired = not (len(parts) > 3 and parts[3] == "unpaired")
+    opts = set(parts[3:])
+    paired = "unpaired" not in opts
+    graph = "graph" in opts
     cfg = solver_config(path, precond=precond, seed=seed, paired=paired, K=K)
     eng = OstrichEngine(model, config=cfg, device=device)
     if seed == "learned":
@@ -89,32 +100,83 @@ def run_path(variant, model, state, control, steps, warmup, device, collider, K=
     ctl = Control.from_numpy(control, device)
     stream = torch.cuda.current_stream(device)
     W = model.num_worlds
-    step_ms, solve_ms, nr, pcr, conv, ncon, lrn = [], [], [], [], [], [], []
+    base_rows = model_rows(model)
+    launch = None
+    if tune and path == "fused":
+        con0 = Contacts.from_dict(collider.collide(s_a.body_q), device)
+        launch, times = eng.autotune(s_a, ctl, con0, DT, pairing=paired)
+        launch = dict(waves_per_block=launch[0], pair_small_worlds=launch[1],
+                      candidates_ms={f"{w}x{'paired' if p else 'single'}": round(t, 4) for (w, p), t in times.items()})
+        print(f"[bench] autotune {variant}: {launch}", file=sys.stderr, flush=True)
+    g, con_static = None, None
+    if graph:
+        con_static = Contacts.from_dict(collider.collide(s_a.body_q), device)
+        g = eng.capture(s_a, s_b, ctl, con_static, DT)
+    step_ms, solve_ms, kern_ms, nr, pcr, conv, ncon, lrn, pbytes, pops = ([] for _ in range(10))
     for k in range(warmup + steps):
         con = Contacts.from_dict(collider.collide(s_a.body_q), device)
+        if graph:
+            con_static.copy_(con)
         e0 = torch.cuda.Event(enable_timing=True)
         e1 = torch.cuda.Event(enable_timing=True)
         e0.record(stream)
-        eng.step(s_a, s_b, ctl, con, DT)
+        if graph:
+            g.replay()
+        else:
+            eng.step(s_a, s_b, ctl, con, DT)
         e1.record(stream)
         e1.synchronize()
         if k >= warmup:
             step_ms.append(e0.elapsed_time(e1))
-            solve_ms.append(eng.last_solve_ms())
+            if not graph:
+                solve_ms.append(eng.last_solve_ms())
             st = eng.stats()
+            cnt = eng.packed_contacts()["count"].clamp(max=K)
             nr.append(st["nr_iters"].float().mean().item())
             pcr.append(st["pcr_iters"].float().mean().item())
             conv.append((st["merit"] <= cfg.nr.atol ** 2).float().mean().item())
-            ncon.append(eng.packed_contacts()["count"].clamp(max=K).float().mean().item())
+            ncon.append(cnt.float().mean().item())
             lrn.append(((st["flags"] & 32) != 0).float().mean().item())
-        s_a, s_b = s_b, s_a
+            if path == "fused" and not graph and eng.fused_launch()[1]:
+                kern_ms.append(eng.last_kernel_ms())
+                rows = base_rows + 3 * cnt
+                sel = rows <= 32  # the paired class
+                nc = cnt[sel].double()
+                pbytes.append(float(compulsory_bytes_per_world(model, nc, eng.num_constraints).sum()))
+                pops.append(float(operand_bytes_per_world(rows[sel].double(), st["nr_iters"][sel].double(),
+                                                          st["pcr_iters"][sel].double()).sum()))
+        if graph:
+            s_a.copy_(s_b)  # the graph reads s_a and writes s_b
+        else:
+            s_a, s_b = s_b, s_a
     torch.cuda.synchronize()
-    return dict(step_ms=float(np.sum(step_ms)), solve_ms=float(np.sum(solve_ms)), steps=steps,
-                world_steps_per_s=W * steps / (np.sum(step_ms) / 1e3),
-                ms_per_step=float(np.mean(step_ms)), kernel_ms=float(np.mean(solve_ms)),
-                nr_mean=float(np.mean(nr)), pcr_per_step_mean=float(np.mean(pcr)),
-                converged_fraction=float(np.mean(conv)), contacts_mean=float(np.mean(ncon)),
-                learned_seed_fraction=float(np.mean(lrn)))
+    out = dict(step_ms=float(np.sum(step_ms)), steps=steps,
+               world_steps_per_s=W * steps / (np.sum(step_ms) / 1e3),
+               ms_per_step=float(np.mean(step_ms)),
+               nr_mean=float(np.mean(nr)), pcr_per_step_mean=float(np.mean(pcr)),
+               converged_fraction=float(np.mean(conv)), contacts_mean=float(np.mean(ncon)),
+               learned_seed_fraction=float(np.mean(lrn)))
+    if solve_ms:
+        out.update(solve_ms=float(np.sum(solve_ms)), kernel_ms=float(np.mean(solve_ms)))
+    if kern_ms:
+        out.update(paired_kernel_ms=float(np.mean(kern_ms)), paired_bytes=float(np.mean(pbytes)),
+                   paired_operand_bytes=float(np.mean(pops)))
+    if launch is not None:
+        out["launch"] = launch
+    return out
+
+
+def model_rows(model):
+    """Joint and control rows of one world (every world shares the topology)."""
+    from ostrich_amd.model import JOINT_BALL, JOINT_FIXED, JOINT_PRISMATIC, JOINT_REVOLUTE
+    J, D = model.joints_per_world, model.dofs_per_world
+    per = {JOINT_PRISMATIC: 5, JOINT_REVOLUTE: 5, JOINT_BALL: 3, JOINT_FIXED: 6}
+    jt = model.arrays["joint_type"][:J]
+    nj = sum(per.get(int(t), 0) for t in jt)
+    qs = model.arrays["joint_qd_start"][:J]
+    modes = model.arrays["joint_dof_mode"][:D] if D else []
+    nctrl = sum(1 for t, q in zip(jt, qs) if int(t) in (JOINT_PRISMATIC, JOINT_REVOLUTE) and int(modes[q]) != 0)
+    return nj + nctrl
 
 
 def scene_leg(name, model_fn, worlds, K, steps, warmup, device, variants=("fused:jacobi:prev",)):
@@ -211,11 +273,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--worlds", type=int, default=4096)
     ap.add_argument("--paths",
-                    default="fused:jacobi:prev,fused:jacobi:prev:unpaired,fused:jacobi:cold,plain:jacobi:cold,"
-                            "fused:per_body_pair:cold,fused:jacobi:learned",
-                    help="comma-separated path:preconditioner variants; the first is the headline")
-    ap.add_argument("--cpu-sample-worlds", type=int, default=64)
-    ap.add_argument("--cpu-sample-steps", type=int, default=40)
+                    default=f"fused:jacobi:prev,{GRAPH_PATH},fused:jacobi:prev:unpaired,fused:jacobi:cold,"
+                            "plain:jacobi:cold,fused:per_body_pair:cold,fused:jacobi:learned",
+                    help="comma-separated path:preconditioner[:seed[:option]] variants; the headline is "
+                         f"{GRAPH_PATH} when listed, else the first")
+    ap.add_argument("--cpu-sample-worlds", type=int, default=128)
+    ap.add_argument("--cpu-sample-steps", type=int, default=120)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-scenes", action="store_true", help="skip the tracked-robot and box-stack legs")
@@ -235,20 +298,19 @@ def main():
     for p in paths:
         print(f"[bench] path={p} worlds={W} warmup={args.warmup} steps={args.steps}", file=sys.stderr, flush=True)
         try:
-            results[p] = run_path(p, model, state, control, args.steps, args.warmup, device, collider)
+            results[p] = run_path(p, model, state, control, args.steps, args.warmup, device, collider,
+                                  tune=p in (paths[0], GRAPH_PATH))
         except RuntimeError as e:
             results[p] = dict(error=str(e))
         print(f"[bench] {p}: {results[p]}", file=sys.stderr, flush=True)
-    main_path = paths[0]
+    # headline: the autotuned fused step replayed from a HIP graph when that variant ran,
+    # else the first variant; the roofline comes from the eager variant's paired-class
+    # kernel events (a graph records no events)
+    eager_path = paths[0]
+    main_path = GRAPH_PATH if "world_steps_per_s" in results.get(GRAPH_PATH, {}) else eager_path
     r = results[main_path]
+    re_ = results[eager_path]
     n_c = r["contacts_mean"]
-    from ostrich_amd.engine import OstrichEngine  # noqa: F401  (library loaded above)
-    nj, nctrl = 15, 3
-    n_rows_ref = nj + nctrl + 3 * K_CONTACTS
-    n_rows_active = nj + nctrl + 3 * n_c
-    bws = compulsory_bytes_per_world(model, n_c, n_rows_ref)
-    aws = operand_bytes_per_world(n_rows_active, r["nr_mean"], r["pcr_per_step_mean"])
-    achieved = W * bws / (r["kernel_ms"] / 1e3) / 1e9
     lds_peak = 256 * 128 * 2.4e9 / 1e9  # 256 CUs x 128 B/clk x 2.4 GHz
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "fused_step_kernel_traffic.json")
@@ -256,6 +318,19 @@ def main():
         with open(tpath) as f:
             t = json.load(f)
         traffic = t["fetch_bytes"] + t["write_bytes"]
+    roof = {"bound": "hbm", "peak": HBM_PEAK_GBS, "unit": "GB/s", "traffic": traffic}
+    if "paired_kernel_ms" in re_:
+        kms = re_["paired_kernel_ms"]
+        achieved = re_["paired_bytes"] / (kms / 1e3) / 1e9
+        roof.update(achieved=achieved, frac=achieved / HBM_PEAK_GBS,
+                    kernel="fused_step_kernel<1, false, 32, true> (worlds of at most 32 rows, two per wavefront)",
+                    kernel_ms=kms, bytes_per_launch=re_["paired_bytes"],
+                    traffic_note="HBM bytes per launch of that kernel from rocprofv3 FETCH_SIZE + WRITE_SIZE "
+                                 "scaled by a 256 MiB copy (profiles/fused_step_kernel_skid_steer_4096.md)",
+                    note="compulsory HBM bytes of the class's worlds per launch; the kernel is bound by "
+                         "instruction issue and LDS latency of one wave per two worlds, not by HBM",
+                    operand_bytes_per_launch=re_["paired_operand_bytes"],
+                    operand_GBs=re_["paired_operand_bytes"] / (kms / 1e3) / 1e9, lds_peak_GBs=lds_peak)
     out = {
         "metric": "world-steps/s at FB residual<=1e-5 (fused persistent solver + cross-step warm start)",
         "value": r["world_steps_per_s"],
@@ -272,17 +347,9 @@ def main():
         "config": {"workload": "uneven-batch skid-steer, 4096 worlds, dt 0.05, NR<=20 to ||h||<=1e-5, "
                                "PCR<=16 (Jacobi, the library default), 128 contact slots, Newton seeded by "
                                "the previous step's contacts", "worlds": W,
-                   "dt": DT, "variant": main_path, "termination": "per_world"},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "traffic_note": "HBM bytes per launch of fused_step_kernel<1,false> at 4096 worlds from "
-                                     "rocprofv3 FETCH_SIZE + WRITE_SIZE scaled by a 256 MiB copy "
-                                     "(profiles/fused_step_kernel_skid_steer_4096.md)",
-                     "kernel": "fused_step_kernel", "kernel_ms": r["kernel_ms"],
-                     "bytes_per_world_step": bws,
-                     "note": "compulsory HBM bytes; the fused kernel is latency/LDS bound",
-                     "operand_bytes_per_world_step": aws,
-                     "operand_GBs": W * aws / (r["kernel_ms"] / 1e3) / 1e9, "lds_peak_GBs": lds_peak},
+                   "dt": DT, "variant": main_path, "termination": "per_world",
+                   "launch": r.get("launch", re_.get("launch"))},
+        "roofline": roof,
         "solver": {"nr_mean": r["nr_mean"], "pcr_per_step_mean": r["pcr_per_step_mean"],
                    "converged_fraction": r["converged_fraction"], "contacts_mean": n_c},
         "paths": {p: results[p] for p in paths},

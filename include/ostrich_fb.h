@@ -326,6 +326,23 @@ int ofb_set_lambda_seed(ofb_solver* solver, const float* lambda_seed, void* stre
  * those launches; contact packing is excluded. Blocks until that work has finished. */
 int ofb_last_solve_ms(const ofb_solver* solver, float* ms);
 
+/* Duration in milliseconds of the last step's paired-class fused kernel (worlds of at
+ * most 32 rows, two per wavefront: the launch that holds most worlds of a typical batch),
+ * from HIP events recorded on the step's stream directly around that launch.  No
+ * reference counterpart (measurement hook).  Returns -2 when the last step launched no
+ * such kernel (plain path, pairing off) or was captured into a graph, in which case no
+ * step timing is recorded (ofb_last_solve_ms also returns -2). */
+int ofb_last_kernel_ms(const ofb_solver* solver, float* ms);
+
+/* Launch settings of the fused path: the largest number of wavefronts per workgroup of a
+ * row-class launch (1..16; the LDS limit may lower it per class) and whether worlds of at
+ * most 32 rows run two per wavefront.  No reference counterpart (Warp chooses its own
+ * launch shapes); the autotuner (OstrichEngine.autotune) sets them from measurements.
+ * The workgroup size leaves results bit-identical; pairing changes the width of the
+ * lane-group reductions, a rounding-level difference.  Fused path only (-2 otherwise). */
+int ofb_set_fused_launch(ofb_solver* solver, int32_t waves_per_block, int32_t pair_small_worlds);
+int ofb_get_fused_launch(const ofb_solver* solver, int32_t* waves_per_block, int32_t* pair_small_worlds);
+
 /* Last error message of the calling thread ("" if none). */
 const char* ofb_last_error(void);
 

@@ -23,6 +23,12 @@ class State:
     def clone(self):
         return State(self.body_q.clone(), self.body_qd.clone(), self.body_f.clone())
 
+    def copy_(self, other):
+        self.body_q.copy_(other.body_q)
+        self.body_qd.copy_(other.body_qd)
+        self.body_f.copy_(other.body_f)
+        return self
+
 
 @dataclass
 class Control:
@@ -58,6 +64,14 @@ class Contacts:
                         t(c["point1"], torch.float32).reshape(-1, 3), t(c["normal"], torch.float32).reshape(-1, 3),
                         t(c["shape0"], torch.int32).reshape(-1), t(c["shape1"], torch.int32).reshape(-1),
                         t(c["margin0"], torch.float32).reshape(-1), t(c["margin1"], torch.float32).reshape(-1))
+
+    def copy_(self, other):
+        """Copies another contact set of the same capacity into these tensors (in place,
+        so a step captured over them reads the new contacts on replay)."""
+        for f in ("rigid_contact_count", "rigid_contact_point0", "rigid_contact_point1", "rigid_contact_normal",
+                  "rigid_contact_shape0", "rigid_contact_shape1", "rigid_contact_margin0", "rigid_contact_margin1"):
+            getattr(self, f).copy_(getattr(other, f))
+        return self
 
     def to_dict(self):
         return dict(count=self.rigid_contact_count, max_contacts=self.rigid_contact_max,

@@ -171,6 +171,11 @@ __global__ void __launch_bounds__(256) pack_world_kernel(int W, int G, int S, in
   }
 }
 
+static bool stream_capturing(hipStream_t st) {
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  return hipStreamIsCapturing(st, &cap) == hipSuccess && cap != hipStreamCaptureStatusNone;
+}
+
 static int pack_contacts(ofb_solver* s, const ofb_contacts* c, hipStream_t stream) {
   const int W = s->m.W, n = c->max_contacts;
   if (n > s->rigid_contact_cap) {
@@ -251,7 +256,10 @@ int ofb_destroy(ofb_solver* s) {
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (s->plain) plain_destroy(s);
-  if (s->events_ready) { (void)hipEventDestroy(s->ev_solve0); (void)hipEventDestroy(s->ev_solve1); }
+  if (s->events_ready) {
+    (void)hipEventDestroy(s->ev_solve0); (void)hipEventDestroy(s->ev_solve1);
+    (void)hipEventDestroy(s->ev_kernel0); (void)hipEventDestroy(s->ev_kernel1);
+  }
   if (s->side_ready) {
     (void)hipEventDestroy(s->ev_fork);
     (void)hipEventDestroy(s->ev_join);
@@ -392,7 +400,8 @@ int ofb_create(const ofb_model* model, const ofb_config* config, int32_t path, o
     return -4;
   }
   s->fused_worlds_per_block = 4;
-  if (hipEventCreate(&s->ev_solve0) != hipSuccess || hipEventCreate(&s->ev_solve1) != hipSuccess) {
+  if (hipEventCreate(&s->ev_solve0) != hipSuccess || hipEventCreate(&s->ev_solve1) != hipSuccess ||
+      hipEventCreate(&s->ev_kernel0) != hipSuccess || hipEventCreate(&s->ev_kernel1) != hipSuccess) {
     set_error("hipEventCreate failed");
     ofb_destroy(s);
     return -4;
@@ -430,7 +439,11 @@ int ofb_step(ofb_solver* s, const ofb_state_in* in, const ofb_state_out* out, co
   if (rc) return rc;
   // the warm starter's seed reaches the solve only through seed_iterate (engine.py:52-59)
   const bool ws = s->cfg.warm_start && s->cfg.seed_iterate;
-  OFB_CHECK(hipEventRecord(s->ev_solve0, st));
+  // timing events are left out of a step captured into a graph
+  const bool timed = !stream_capturing(st);
+  s->solve_timed = timed ? 1 : 0;
+  s->kernel_timed = 0;
+  if (timed) OFB_CHECK(hipEventRecord(s->ev_solve0, st));
   if (ws && (rc = warm_start_apply(s, in, dt, st)) != 0) return rc;
   if (ws && s->ws_net && s->path == OFB_PATH_FUSED) {
     if ((rc = ofb_mlp_forward(s->ws_net, s->ws_feat, s->m.W * s->L.K, s->ws_y, st)) != 0) return rc;
@@ -439,7 +452,7 @@ int ofb_step(ofb_solver* s, const ofb_state_in* in, const ofb_state_out* out, co
   rc = s->path == OFB_PATH_FUSED ? fused_step(s, in, out, ctl, dt, st) : plain_step(s, in, out, ctl, dt, st);
   if (rc) return rc;
   if (ws && (rc = warm_start_snapshot(s, in, dt, st)) != 0) return rc;
-  OFB_CHECK(hipEventRecord(s->ev_solve1, st));
+  if (timed) OFB_CHECK(hipEventRecord(s->ev_solve1, st));
   return 0;
 }
 
@@ -484,6 +497,7 @@ int ofb_get_lambda_seed(const ofb_solver* s, float* seed_out, void* stream) {
 
 int ofb_last_solve_ms(const ofb_solver* s, float* ms) {
   if (!s || !ms) { set_error("null argument"); return -1; }
+  if (!s->solve_timed) { set_error("the last step was not timed (captured into a graph, or none yet)"); return -2; }
   OFB_CHECK(hipEventSynchronize(s->ev_solve1));
   OFB_CHECK(hipEventElapsedTime(ms, s->ev_solve0, s->ev_solve1));
   return 0;
@@ -546,10 +560,37 @@ int ofb_step_backward(ofb_solver* s, const ofb_state_in* in, const ofb_state_out
   hipStream_t st = (hipStream_t)stream;
   int rc = pack_contacts(s, contacts, st);
   if (rc) return rc;
-  OFB_CHECK(hipEventRecord(s->ev_solve0, st));
+  const bool timed = !stream_capturing(st);
+  s->solve_timed = timed ? 1 : 0;
+  s->kernel_timed = 0;
+  if (timed) OFB_CHECK(hipEventRecord(s->ev_solve0, st));
   rc = fused_adjoint(s, in, out, ctl, lambda, lambda_prev, gin, gout, dt, st);
   if (rc) return rc;
-  OFB_CHECK(hipEventRecord(s->ev_solve1, st));
+  if (timed) OFB_CHECK(hipEventRecord(s->ev_solve1, st));
+  return 0;
+}
+
+int ofb_last_kernel_ms(const ofb_solver* s, float* ms) {
+  if (!s || !ms) { set_error("null argument"); return -1; }
+  if (!s->kernel_timed) {
+    set_error("the last step launched no timed paired-class kernel (plain path, pairing off, or captured)");
+    return -2;
+  }
+  OFB_CHECK(hipEventSynchronize(s->ev_kernel1));
+  OFB_CHECK(hipEventElapsedTime(ms, s->ev_kernel0, s->ev_kernel1));
+  return 0;
+}
+
+int ofb_set_fused_launch(ofb_solver* s, int32_t waves_per_block, int32_t pair_small_worlds) {
+  if (!s) { set_error("null solver"); return -1; }
+  if (s->path != OFB_PATH_FUSED) { set_error("launch settings apply to the fused path only"); return -2; }
+  return fused_set_launch(s, waves_per_block, pair_small_worlds);
+}
+
+int ofb_get_fused_launch(const ofb_solver* s, int32_t* waves_per_block, int32_t* pair_small_worlds) {
+  if (!s) { set_error("null solver"); return -1; }
+  if (waves_per_block) *waves_per_block = s->fused_worlds_per_block;
+  if (pair_small_worlds) *pair_small_worlds = s->cfg.pair_small_worlds;
   return 0;
 }
 

@@ -167,7 +167,8 @@ static bool class_used(const ofb_solver* s, int c) {
   return c == 1 || kClassRHost[c - 1] < r_max;
 }
 
-int fused_classes_alloc(ofb_solver* s) {
+// workgroup shape of every class launch for the solver's current launch settings
+static int fused_configure(ofb_solver* s) {
   for (int c = 0; c < kNumClasses; ++c) {
     s->cls_wpb[c] = 1;
     if (!class_used(s, c)) continue;
@@ -175,6 +176,26 @@ int fused_classes_alloc(ofb_solver* s) {
     if (rc) return rc;
     s->cls_wpb[c] = choose_wpb(s, kClassRHost[c]);
   }
+  return 0;
+}
+
+// launch settings of the fused path: the largest number of waves per workgroup and whether
+// worlds of at most 32 rows run two per wavefront.  The workgroup size changes nothing
+// in a world's arithmetic; pairing changes the lane-group width of the reductions (a
+// rounding-level difference).
+int fused_set_launch(ofb_solver* s, int waves_per_block, int pair_small_worlds) {
+  if (waves_per_block < 1 || waves_per_block > 16) {
+    set_error("waves per workgroup must be in [1, 16] (got %d)", waves_per_block);
+    return -2;
+  }
+  s->fused_worlds_per_block = waves_per_block;
+  s->cfg.pair_small_worlds = pair_small_worlds ? 1 : 0;
+  return fused_configure(s);
+}
+
+int fused_classes_alloc(ofb_solver* s) {
+  int rc = fused_configure(s);
+  if (rc) return rc;
   if (hipStreamCreateWithFlags(&s->side_stream, hipStreamNonBlocking) != hipSuccess) {
     set_error("hipStreamCreate failed");
     return -4;
@@ -248,10 +269,24 @@ int fused_step(ofb_solver* s, const ofb_state_in* in, const ofb_state_out* out, 
     set_error("fused step: stream join failed");
     return -4;
   }
+  // the paired class, which holds most worlds of a typical batch, on the step's stream,
+  // bracketed by the kernel events (not recorded while the stream is being captured)
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(stream, &cap) != hipSuccess) cap = hipStreamCaptureStatusNone;
+  const bool timed = cap == hipStreamCaptureStatusNone && class_used(s, 0);
+  if (timed && hipEventRecord(s->ev_kernel0, stream) != hipSuccess) {
+    set_error("fused step: event record failed");
+    return -4;
+  }
   if (class_used(s, 0) && launch_fused(kClassRHost[0], P, s, 0, stream) != 0) {
     set_error("no fused kernel instance for class code %d", kClassRHost[0]);
     return -4;
   }
+  if (timed && hipEventRecord(s->ev_kernel1, stream) != hipSuccess) {
+    set_error("fused step: event record failed");
+    return -4;
+  }
+  s->kernel_timed = timed ? 1 : 0;
   if (hipStreamWaitEvent(stream, s->ev_join, 0) != hipSuccess) {
     set_error("fused step: stream join failed");
     return -4;

@@ -80,7 +80,8 @@ struct ofb_solver {
   int32_t *st_nr, *st_pcr, *st_best, *st_flags;
   float* st_merit;
   ofb::PlainBuffers* plain;
-  // launch shape chosen by the autotuner for the fused path
+  // largest number of waves per workgroup of a fused class launch (ofb_set_fused_launch;
+  // the Python autotuner picks it)
   int fused_worlds_per_block;
   // per-step bucketing of worlds by active-row class (fused path)
   int32_t* cls_counts;
@@ -100,15 +101,18 @@ struct ofb_solver {
   float* ls_steps;
   int32_t ls_count;
   float ls_host[512];
-  // events around the solve of the last step
-  hipEvent_t ev_solve0, ev_solve1;
+  // events around the solve of the last step, and around its paired-class kernel
+  hipEvent_t ev_solve0, ev_solve1, ev_kernel0, ev_kernel1;
   int events_ready;
+  int kernel_timed;   // the last step recorded the kernel events
+  int solve_timed;    // the last step recorded the solve events
 };
 
 namespace ofb {
 int fused_step(ofb_solver* s, const ofb_state_in* in, const ofb_state_out* out, const ofb_control* ctl, float dt,
                hipStream_t stream);
 int fused_classes_alloc(ofb_solver* s);
+int fused_set_launch(ofb_solver* s, int waves_per_block, int pair_small_worlds);
 int fused_adjoint(ofb_solver* s, const ofb_state_in* in, const ofb_state_out* out, const ofb_control* ctl,
                   const float* lambda, const float* lambda_prev, const ofb_adjoint_in* gin,
                   const ofb_adjoint_out* gout, float dt, hipStream_t stream);

@@ -176,6 +176,81 @@ class OstrichEngine:
         _lib.check(self.lib.ofb_last_solve_ms(self._h, C.byref(ms)), "ofb_last_solve_ms")
         return ms.value
 
+    def last_kernel_ms(self):
+        """Duration of the last step's paired-class fused kernel (the launch holding the
+        worlds of at most 32 rows), from HIP events on the step's stream around it."""
+        ms = C.c_float()
+        _lib.check(self.lib.ofb_last_kernel_ms(self._h, C.byref(ms)), "ofb_last_kernel_ms")
+        return ms.value
+
+    def fused_launch(self):
+        """(waves per workgroup, pair_small_worlds) of the fused path."""
+        w, p = C.c_int32(), C.c_int32()
+        _lib.check(self.lib.ofb_get_fused_launch(self._h, C.byref(w), C.byref(p)), "ofb_get_fused_launch")
+        return w.value, bool(p.value)
+
+    def set_fused_launch(self, waves_per_block, pair_small_worlds):
+        _lib.check(self.lib.ofb_set_fused_launch(self._h, int(waves_per_block), 1 if pair_small_worlds else 0),
+                   "ofb_set_fused_launch")
+
+    def autotune(self, state_in: State, control: Control, contacts: Contacts, dt: float, candidates=None,
+                 repeats=3, pairing=True):
+        """Chooses the fused launch settings for this batch from measurements.
+
+        Each candidate (waves per workgroup in {1, 2, 4, 8}, and with `pairing` small
+        worlds two per wavefront or one) runs one warm-up step and `repeats` timed steps of
+        the given inputs on a scratch solver built from the same model and config, timed
+        with HIP events on the current stream; the fastest is applied to this solver.
+        This solver's state (warm-start snapshot, last results) is left as it was.
+        Returns (best settings, {settings: best ms})."""
+        if self.config.execution.path != "fused":
+            raise ValueError("autotune applies to the fused path")
+        if candidates is None:
+            pairs = (True, False) if pairing else (self.fused_launch()[1],)
+            candidates = [(w, p) for p in pairs for w in (1, 2, 4, 8)]
+        scratch = OstrichEngine(self.model, config=self.config, device=self.device,
+                                rigid_contact_max=self.rigid_contact_max)
+        out = state_in.clone()
+        stream = torch.cuda.current_stream(self.device)
+        times = {}
+        for wpb, pair in candidates:
+            scratch.set_fused_launch(wpb, pair)
+            scratch.step(state_in, out, control, contacts, dt)
+            best = float("inf")
+            for _ in range(repeats):
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                scratch.step(state_in, out, control, contacts, dt)
+                e1.record(stream)
+                e1.synchronize()
+                best = min(best, e0.elapsed_time(e1))
+            times[(wpb, bool(pair))] = best
+        torch.cuda.synchronize(self.device)
+        del scratch
+        choice = min(times, key=times.get)
+        self.set_fused_launch(*choice)
+        return choice, times
+
+    def capture(self, state_in: State, state_out: State, control: Control, contacts: Contacts, dt: float):
+        """Records one full step (contact packing, warm start, the fused launches, the
+        warm-start snapshot) over these exact tensors into a HIP graph and returns it
+        (a torch.cuda.CUDAGraph): `graph.replay()` runs a step that reads the tensors'
+        current contents, so a caller refreshes them in place (State.copy_,
+        Contacts.copy_) between replays.  The step allocates nothing and launches only
+        kernels (no memset or host synchronisation), so it records as is; no timing events
+        are recorded inside the graph.  Fused path only: the plain path reads its stop flag
+        on the host every Newton iteration."""
+        if self.config.execution.path != "fused":
+            raise ValueError("graph capture applies to the fused path")
+        s = torch.cuda.Stream(device=self.device)
+        s.wait_stream(torch.cuda.current_stream(self.device))
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=s):
+            self.step(state_in, state_out, control, contacts, dt)
+        torch.cuda.current_stream(self.device).wait_stream(s)
+        self.timestep -= 1  # recording is not a step
+        return g
+
     def constraint_impulses_prev(self):
         """lambda_prev_iter [W, N_c] as the last step left it (the snapshot of its last
         Newton iteration, base_engine.py:441); with `constraint_impulses()` it is what the

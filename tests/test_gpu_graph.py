@@ -1,0 +1,71 @@
+"""A full fused step recorded into a HIP graph, and the launch-shape autotuner.
+
+The graph is replayed several times with the program's own eager work and
+synchronisations between replays; every replay must equal, bit for bit, the same step
+run eagerly on a second solver (same kernels, same inputs).  The autotuner may only
+change the launch shape: with pairing fixed, a tuned solver's results equal an untuned
+solver's bit for bit.
+"""
+import numpy as np
+import pytest
+import torch
+
+from ostrich_amd import collide, scenes
+from ostrich_amd.config import WarmStartConfig
+from ostrich_amd.containers import Contacts, Control, State
+from tests.helpers import config
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(worlds, device):
+    from ostrich_amd import OstrichEngine
+    model, state, control = scenes.skid_steer(worlds, seed=1)
+    cfg = config(K=128, warm_start=WarmStartConfig(enabled=True, seed_iterate=True))
+    col = collide.PrimitiveCollider(model, device=device, gap=0.02, max_contacts_per_world=128)
+    mk = lambda: OstrichEngine(model, config=cfg, device=device)
+    return model, State.from_numpy(state, device), Control.from_numpy(control, device), col, mk
+
+
+def test_captured_step_replays_match_eager(gpu_device):
+    model, s0, ctl, col, mk = _setup(128, gpu_device)
+    eager, graphed = mk(), mk()
+    a_in, a_out = s0.clone(), s0.clone()
+    b_in, b_out = s0.clone(), s0.clone()
+    b_con = Contacts.from_dict(col.collide(b_in.body_q), gpu_device)
+    g = graphed.capture(b_in, b_out, ctl, b_con, 0.05)
+    for k in range(4):
+        con = Contacts.from_dict(col.collide(a_in.body_q), gpu_device)
+        eager.step(a_in, a_out, ctl, con, 0.05)
+        b_con.copy_(con)
+        g.replay()
+        torch.cuda.synchronize()
+        print(f"[graph] replay {k}: nr mean {graphed.stats()['nr_iters'].float().mean().item():.2f}", flush=True)
+        assert torch.equal(a_out.body_q, b_out.body_q) and torch.equal(a_out.body_qd, b_out.body_qd), k
+        assert torch.equal(eager.constraint_impulses(), graphed.constraint_impulses()), k
+        for key in ("nr_iters", "pcr_iters", "flags"):
+            assert torch.equal(eager.stats()[key], graphed.stats()[key]), (k, key)
+        # the program's own eager work between replays
+        junk = torch.randn(1 << 16, device=gpu_device).sum()
+        torch.cuda.synchronize()
+        assert torch.isfinite(junk)
+        a_in.copy_(a_out)
+        b_in.copy_(b_out)
+    with pytest.raises(RuntimeError, match="not timed"):
+        graphed.last_solve_ms()
+
+
+def test_autotune_changes_only_the_launch_shape(gpu_device):
+    model, s0, ctl, col, mk = _setup(256, gpu_device)
+    tuned, plain = mk(), mk()
+    con = Contacts.from_dict(col.collide(s0.body_q), gpu_device)
+    choice, times = tuned.autotune(s0, ctl, con, 0.05, pairing=False, repeats=2)
+    print(f"[autotune] {choice} from {times}", flush=True)
+    assert choice in times and tuned.fused_launch() == choice
+    assert all(np.isfinite(v) and v > 0 for v in times.values())
+    out_t, out_p = s0.clone(), s0.clone()
+    tuned.step(s0, out_t, ctl, con, 0.05)
+    plain.step(s0, out_p, ctl, con, 0.05)
+    torch.cuda.synchronize()
+    assert torch.equal(out_t.body_q, out_p.body_q) and torch.equal(out_t.body_qd, out_p.body_qd)
+    assert tuned.last_kernel_ms() > 0.0
