@@ -282,6 +282,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-scenes", action="store_true", help="skip the tracked-robot and box-stack legs")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape (no autotuner run)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 256 MiB device copy (reference kernel for HBM byte counters)")
     args = ap.parse_args()
@@ -299,7 +300,7 @@ def main():
         print(f"[bench] path={p} worlds={W} warmup={args.warmup} steps={args.steps}", file=sys.stderr, flush=True)
         try:
             results[p] = run_path(p, model, state, control, args.steps, args.warmup, device, collider,
-                                  tune=p in (paths[0], GRAPH_PATH))
+                                  tune=not args.no_tune and p in (paths[0], GRAPH_PATH))
         except RuntimeError as e:
             results[p] = dict(error=str(e))
         print(f"[bench] {p}: {results[p]}", file=sys.stderr, flush=True)

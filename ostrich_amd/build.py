@@ -57,8 +57,14 @@ def _compile(unit, verbose):
     return obj, r
 
 
-def build(verbose=False, jobs=None):
-    os.makedirs(OUT_DIR, exist_ok=True)
+def build(verbose=False, jobs=None, out_dir=None):
+    """Builds the library into ostrich_amd/_lib (or `out_dir`, for a variant build such as
+    the phase-timing one: OFB_HIPCC_FLAGS=-DOFB_PHASE_TIMING)."""
+    global OBJ_DIR, LIB
+    if out_dir is not None:
+        out_dir = os.path.abspath(out_dir)
+        OBJ_DIR, LIB = os.path.join(out_dir, "obj"), os.path.join(out_dir, "libostrich_fb.so")
+    os.makedirs(os.path.dirname(LIB), exist_ok=True)
     os.makedirs(OBJ_DIR, exist_ok=True)
     jobs = jobs or min(len(UNITS), max(1, min(16, os.cpu_count() or 1)))
     with ThreadPoolExecutor(max_workers=jobs) as ex:
@@ -80,4 +86,5 @@ def build(verbose=False, jobs=None):
 
 
 if __name__ == "__main__":
-    build(verbose="-v" in sys.argv)
+    out = sys.argv[sys.argv.index("--out") + 1] if "--out" in sys.argv else None
+    build(verbose="-v" in sys.argv, out_dir=out)
