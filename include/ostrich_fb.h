@@ -50,6 +50,13 @@ enum { OFB_PRECOND_JACOBI = 0, OFB_PRECOND_PER_BODY_PAIR = 1 };
 enum { OFB_FRICTION_CONE = 0, OFB_FRICTION_BILATERAL = 1, OFB_FRICTION_BILATERAL_PATCH = 2 };
 enum { OFB_CONTACT_COMPLEMENTARITY = 0, OFB_CONTACT_EQUALITY = 1 };
 
+/* contacts.reduction.policy (collision/config.py:13-56): per (body0, body1) pair, keep
+ * the K deepest contacts (TOP_K, collision/top_k.py), a farthest-point sample seeded by
+ * the deepest (FPS, collision/fps.py), or the K deepest cluster representatives
+ * (CLUSTER, collision/cluster.py).  Applied to the packed contacts before the warm start,
+ * with the step-start poses; survivors keep their packed order. */
+enum { OFB_REDUCE_NONE = 0, OFB_REDUCE_TOP_K = 1, OFB_REDUCE_FPS = 2, OFB_REDUCE_CLUSTER = 3 };
+
 /* Termination of the Newton and PCR loops.
  * GLOBAL:    the reference's batch-global stop flag (base_engine.py:42-80,
  *            pcr_solver.py:84-117): every world iterates until all worlds converge.
@@ -147,6 +154,10 @@ typedef struct ofb_config {
   int32_t pair_small_worlds;     /* fused path: worlds of at most 32 rows run two per
                                     wavefront on the assembled (dense) Schur operator;
                                     0: one world per wavefront, matrix-free operator */
+  int32_t contact_reduction;     /* contacts.reduction.policy: OFB_REDUCE_* (collision/config.py:13) */
+  int32_t contact_max_per_pair;  /* contacts.reduction.max_per_pair: K per (body0, body1) pair */
+  float cluster_normal_dot;      /* contacts.reduction.cluster_normal_dot_thresh */
+  float cluster_pos;             /* contacts.reduction.cluster_pos_thresh [m] */
 } ofb_config;
 
 typedef struct ofb_state_in {

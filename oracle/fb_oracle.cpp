@@ -321,6 +321,146 @@ template <class T> void pack_contacts(Batch<T>& Bt, const ofb_contacts& c) {
 }
 
 // ----------------------------------------------------------------------------------------
+// Contact reduction: collision/top_k.py, fps.py, cluster.py.  Runs on the packed contacts
+// with the step-start poses (the reference calls it from load_data, where body_pose_prev
+// holds the step's input pose, base_engine.py:317-320).  Each reducer is restated as the
+// serial loop Warp's CPU device runs for its (world, contact) launch: leaders in contact
+// order, then a scatter that reserves slots in contact order, so survivors keep their
+// packed order.  A world holds min(count, Cmax) contacts (the reference loops to the raw
+// count, past the arrays, when a world overflows).  The reducer's geometry is evaluated in
+// double for both precisions of the oracle (and on the GPU): its decisions are
+// comparisons, and the near-ties of flat contact patches would otherwise fall by rounding.
+// ----------------------------------------------------------------------------------------
+template <class T> struct RedContact {
+  int b0, b1;
+  T depth;
+  V3<T> mid, n;
+};
+
+template <class T> RedContact<double> red_contact(const Batch<T>& Bt, const double* q, int w, int c) {
+  const size_t k = size_t(w) * Bt.Cmax + c;
+  RedContact<double> r;
+  auto body = [&](int s) { return s < 0 ? -1 : Bt.M.shape_body[w * Bt.SC + s]; };  // _resolve_body
+  r.b0 = body(Bt.cs0[k]);
+  r.b1 = body(Bt.cs1[k]);
+  auto pose = [&](int b) {
+    if (b < 0) return tident<double>();
+    const double* x = q + (size_t(w) * Bt.B + b) * 7;
+    return Tf<double>{v3<double>(x[0], x[1], x[2]), Qt<double>{x[3], x[4], x[5], x[6]}};
+  };
+  r.n = v3<double>(Bt.cn[k * 3], Bt.cn[k * 3 + 1], Bt.cn[k * 3 + 2]);
+  V3<double> p0 = v3<double>(Bt.cp0[k * 3], Bt.cp0[k * 3 + 1], Bt.cp0[k * 3 + 2]);
+  V3<double> p1 = v3<double>(Bt.cp1[k * 3], Bt.cp1[k * 3 + 1], Bt.cp1[k * 3 + 2]);
+  // _contact_depth / _depth_and_midpoint (top_k.py:55-87, fps.py:50-86)
+  V3<double> pa = sub(tpoint(pose(r.b0), p0), mul(r.n, double(Bt.ct0[k])));
+  V3<double> pb = add(tpoint(pose(r.b1), p1), mul(r.n, double(Bt.ct1[k])));
+  r.depth = dot(r.n, sub(pb, pa));
+  r.mid = mul(add(pa, pb), 0.5);
+  return r;
+}
+
+template <class T> inline bool deeper(const RedContact<T>& a, int ia, const RedContact<T>& b, int ib) {
+  return a.depth > b.depth || (a.depth == b.depth && ia < ib);
+}
+
+template <class T> void reduce_contacts(Batch<T>& Bt, const double* q) {
+  const int policy = Bt.cfg.contact_reduction;
+  if (policy == OFB_REDUCE_NONE) return;
+  const int K = Bt.cfg.contact_max_per_pair;
+  std::vector<RedContact<double>> C;
+  std::vector<int> keep;
+  for (int w = 0; w < Bt.W; ++w) {
+    const int n = std::min(Bt.ccount[w], Bt.Cmax);
+    C.resize(n);
+    for (int c = 0; c < n; ++c) C[c] = red_contact(Bt, q, w, c);
+    keep.assign(n, 0);
+    auto same = [&](int i, int j) { return C[i].b0 == C[j].b0 && C[i].b1 == C[j].b1; };
+    if (policy == OFB_REDUCE_TOP_K) {  // rank_within_pair_kernel (top_k.py:90-143)
+      for (int c = 0; c < n; ++c) {
+        int rank = 0;
+        for (int j = 0; j < n; ++j)
+          if (j != c && same(j, c) && deeper(C[j], j, C[c], c)) ++rank;
+        keep[c] = rank < K;
+      }
+    } else {
+      for (int c = 0; c < n; ++c) {
+        bool leader = true;  // smallest contact index of its pair
+        for (int j = 0; j < c; ++j)
+          if (same(j, c)) leader = false;
+        if (!leader) continue;
+        if (policy == OFB_REDUCE_FPS) {  // fps_per_pair_kernel (fps.py:89-225)
+          int seed = -1;
+          double best = -1.0e30;
+          for (int j = 0; j < n; ++j)
+            if (same(j, c) && C[j].depth > best) { best = C[j].depth; seed = j; }
+          if (seed < 0) continue;
+          keep[seed] = 1;
+          for (int r = 1; r < K; ++r) {
+            int pick = -1;
+            double pick_d = -1.0;
+            for (int j = 0; j < n; ++j) {
+              if (!same(j, c) || keep[j]) continue;
+              double md = 1.0e30;
+              for (int k = 0; k < n; ++k) {
+                if (!keep[k] || !same(k, c)) continue;
+                V3<double> d = sub(C[j].mid, C[k].mid);
+                double dd = dot(d, d);
+                if (dd < md) md = dd;
+              }
+              if (md > pick_d) { pick_d = md; pick = j; }
+            }
+            if (pick < 0) break;
+            keep[pick] = 1;
+          }
+        } else {  // cluster_per_pair_kernel (cluster.py:94-238)
+          const double pos_sq = double(Bt.cfg.cluster_pos) * double(Bt.cfg.cluster_pos);
+          const double ndot = double(Bt.cfg.cluster_normal_dot);
+          for (int j = 0; j < n; ++j) {  // pass 1: representatives of this pair
+            if (!same(j, c)) continue;
+            bool rep = true;
+            for (int k = 0; k < n; ++k) {
+              if (k == j || !same(k, c)) continue;
+              V3<double> d = sub(C[j].mid, C[k].mid);
+              if (dot(C[j].n, C[k].n) > ndot && dot(d, d) < pos_sq && deeper(C[k], k, C[j], j)) rep = false;
+            }
+            keep[j] = rep ? 1 : 0;
+          }
+          // pass 2, as the reference writes it: every contact of the world still marked
+          // (this pair's and earlier pairs' representatives) is ranked against this
+          // pair's marked representatives, in contact order, and dropped at rank >= K
+          for (int j = 0; j < n; ++j) {
+            if (!keep[j]) continue;
+            int rank = 0;
+            for (int k = 0; k < n; ++k)
+              if (k != j && keep[k] && same(k, c) && deeper(C[k], k, C[j], j)) ++rank;
+            if (rank >= K) keep[j] = 0;
+          }
+        }
+      }
+    }
+    // scatter in contact order (scatter_top_k_kernel / scatter_by_keep_kernel)
+    int m = 0;
+    for (int c = 0; c < n; ++c) {
+      if (!keep[c]) continue;
+      const size_t src = size_t(w) * Bt.Cmax + c, dst = size_t(w) * Bt.Cmax + m;
+      if (dst != src) {
+        for (int d = 0; d < 3; ++d) {
+          Bt.cp0[dst * 3 + d] = Bt.cp0[src * 3 + d];
+          Bt.cp1[dst * 3 + d] = Bt.cp1[src * 3 + d];
+          Bt.cn[dst * 3 + d] = Bt.cn[src * 3 + d];
+        }
+        Bt.ct0[dst] = Bt.ct0[src];
+        Bt.ct1[dst] = Bt.ct1[src];
+        Bt.cs0[dst] = Bt.cs0[src];
+        Bt.cs1[dst] = Bt.cs1[src];
+      }
+      ++m;
+    }
+    Bt.ccount[w] = m;
+  }
+}
+
+// ----------------------------------------------------------------------------------------
 // Joint kinematics: constraints/joint_kinematics.py:5-261
 // ----------------------------------------------------------------------------------------
 template <class T> struct JointFrames {
@@ -1177,6 +1317,7 @@ template <class T> int run_step(const ofb_model* m, const ofb_config* cfg, const
   Bt.dt = T(dt);
   const int W = Bt.W, B = Bt.B;
   pack_contacts(Bt, *contacts);
+  reduce_contacts(Bt, q_in);
   for (int w = 0; w < W; ++w) {  // load_data (base_engine.py:289-335), engine.py:48-59
     for (int b = 0; b < B; ++b) {
       size_t k = Bt.bi(w, b);
@@ -1334,6 +1475,7 @@ template <class T> int run_adjoint(const ofb_model* m, const ofb_config* cfg, co
   Bt.dt = T(dt);
   const int W = Bt.W, B = Bt.B, D = Bt.D, Nc = Bt.Nc, SC = Bt.SC;
   pack_contacts(Bt, *contacts);
+  reduce_contacts(Bt, q_prev_in);
   auto ld_tf = [](const double* x) {
     return Tf<T>{v3<T>(T(x[0]), T(x[1]), T(x[2])), Qt<T>{T(x[3]), T(x[4]), T(x[5]), T(x[6])}};
   };
@@ -1658,6 +1800,28 @@ void ws_snapshot(const Batch<double>& Bt, const double* lam, const WSPrev& P) {
 
 }  // namespace
 
+// Packed contacts after packing and the configured reduction (body poses q [W*B*7] of
+// the step start): counts [W] and the slot arrays [W*Cmax(*3)].
+template <class T> int run_pack(const ofb_model* m, const ofb_config* cfg, const double* q,
+                                const ofb_contacts* contacts, int32_t* count, int32_t* shape0, int32_t* shape1,
+                                double* point0, double* point1, double* normal, double* th0, double* th1) {
+  Batch<T> Bt(*m, *cfg);
+  pack_contacts(Bt, *contacts);
+  reduce_contacts(Bt, q);
+  const size_t n = size_t(Bt.W) * Bt.Cmax;
+  for (int w = 0; w < Bt.W; ++w) count[w] = Bt.ccount[w];
+  for (size_t k = 0; k < n; ++k) {
+    shape0[k] = Bt.cs0[k]; shape1[k] = Bt.cs1[k];
+    th0[k] = double(Bt.ct0[k]); th1[k] = double(Bt.ct1[k]);
+    for (int d = 0; d < 3; ++d) {
+      point0[k * 3 + d] = double(Bt.cp0[k * 3 + d]);
+      point1[k * 3 + d] = double(Bt.cp1[k * 3 + d]);
+      normal[k * 3 + d] = double(Bt.cn[k * 3 + d]);
+    }
+  }
+  return 0;
+}
+
 extern "C" {
 
 // Number of constraint rows of the model (core/engine_dims.py:56-61).
@@ -1665,6 +1829,14 @@ int oracle_dims(const ofb_model* m, const ofb_config* cfg, int32_t* n_c, int32_t
   Batch<double> probe(*m, *cfg);
   *n_c = probe.Nc; *n_j = probe.Nj; *n_ctrl = probe.Nctrl;
   return 0;
+}
+
+int oracle_pack(const ofb_model* m, const ofb_config* cfg, const double* q, const ofb_contacts* contacts,
+                int precision, int32_t* count, int32_t* shape0, int32_t* shape1, double* point0, double* point1,
+                double* normal, double* th0, double* th1) {
+  if (precision == 64)
+    return run_pack<double>(m, cfg, q, contacts, count, shape0, shape1, point0, point1, normal, th0, th1);
+  return run_pack<float>(m, cfg, q, contacts, count, shape0, shape1, point0, point1, normal, th0, th1);
 }
 
 // One OstrichEngine.step in FP32 (precision 32) or FP64 (precision 64).  State is passed
@@ -1708,6 +1880,7 @@ int oracle_linear_system(const ofb_model* m, const ofb_config* cfg, const double
   Batch<double> Bt(*m, *cfg);
   Bt.dt = dt;
   pack_contacts(Bt, *contacts);
+  reduce_contacts(Bt, q_in);
   const int W = Bt.W, B = Bt.B, Nc = Bt.Nc;
   for (int w = 0; w < W; ++w) {
     for (int b = 0; b < B; ++b) {
@@ -1774,6 +1947,7 @@ double oracle_weighted_residual(const ofb_model* m, const ofb_config* cfg, const
   Batch<double> Bt(*m, *cfg);
   Bt.dt = dt;
   pack_contacts(Bt, *contacts);
+  reduce_contacts(Bt, q_prev);
   const int W = Bt.W, B = Bt.B, D = Bt.D, Nc = Bt.Nc;
   std::vector<double> wl(size_t(W) * Nc);
   std::vector<Sp<double>> wu(size_t(W) * B);
@@ -1836,6 +2010,7 @@ int oracle_warm_start(const ofb_model* m, const ofb_config* cfg, const double* q
   Batch<double> Bt(*m, *cfg);
   Bt.dt = dt;
   pack_contacts(Bt, *contacts);
+  reduce_contacts(Bt, q_prev);
   for (int w = 0; w < Bt.W; ++w)
     for (int b = 0; b < Bt.B; ++b) {
       size_t k = Bt.bi(w, b);

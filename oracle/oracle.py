@@ -46,6 +46,8 @@ def lib():
         _lib.oracle_fb_diff.argtypes = [C.c_double] * 5 + [vp]
         _lib.oracle_integrate_pose.argtypes = [vp, vp, vp, C.c_double, vp]
         _lib.oracle_linear_system.argtypes = [vp] * 8 + [C.c_float] + [vp] * 8
+        _lib.oracle_pack.argtypes = [vp] * 4 + [C.c_int] + [vp] * 8
+        _lib.oracle_pack.restype = C.c_int
     return _lib
 
 
@@ -90,6 +92,27 @@ def dims(model, config):
     nc, nj, nctrl = C.c_int32(), C.c_int32(), C.c_int32()
     lib().oracle_dims(C.byref(hm.abi), C.byref(cfg), C.byref(nc), C.byref(nj), C.byref(nctrl))
     return nc.value, nj.value, nctrl.value
+
+
+def pack(model, config, state, contacts, precision=32):
+    """Packed contacts after core/contacts.py:25-99 and the configured per-pair reduction
+    (collision/top_k.py, fps.py, cluster.py), with state['body_q'] as the step-start
+    poses.  Returns count [W] and the slot arrays [W*K(*3)] (point0/1, normal,
+    thickness0/1, shape0/1), the layout of OstrichEngine.packed_contacts()."""
+    hm = HostModel(model)
+    hc = contacts if isinstance(contacts, HostContacts) else HostContacts(contacts)
+    cfg = config.to_abi()
+    W, K = model.num_worlds, config.contacts.max_per_world
+    q = np.ascontiguousarray(np.asarray(state["body_q"], np.float64).reshape(-1, 7))
+    out = dict(count=np.zeros(W, np.int32), shape0=np.zeros(W * K, np.int32), shape1=np.zeros(W * K, np.int32),
+               point0=np.zeros(W * K * 3), point1=np.zeros(W * K * 3), normal=np.zeros(W * K * 3),
+               thickness0=np.zeros(W * K), thickness1=np.zeros(W * K))
+    rc = lib().oracle_pack(C.byref(hm.abi), C.byref(cfg), _p(q), C.byref(hc.abi), precision,
+                           *(_p(out[k]) for k in ("count", "shape0", "shape1", "point0", "point1", "normal",
+                                                  "thickness0", "thickness1")))
+    if rc != 0:
+        raise RuntimeError("oracle_pack failed")
+    return out
 
 
 def step(model, config, state, control, contacts, dt, precision=32, lam_seed=None, host_model=None):

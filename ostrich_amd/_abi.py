@@ -95,6 +95,10 @@ class OfbConfig(C.Structure):
         ("linesearch_conservative_upper", C.c_double),
         ("linesearch_optimistic_window", C.c_double),
         ("pair_small_worlds", C.c_int32),
+        ("contact_reduction", C.c_int32),
+        ("contact_max_per_pair", C.c_int32),
+        ("cluster_normal_dot", C.c_float),
+        ("cluster_pos", C.c_float),
     ]
 
 

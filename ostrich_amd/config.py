@@ -189,11 +189,37 @@ class WarmStartConfig:
             )
 
 
+REDUCTION_POLICIES = {"none": 0, "top_k": 1, "fps": 2, "cluster": 3}
+
+
+@dataclass(frozen=True)
+class ContactReductionConfig:
+    """collision/config.py:15-56.  The reference's "hull" policy is declared there but
+    has no reducer (collision/__init__.py builds none for it); it raises here."""
+
+    policy: str = "none"
+    max_per_pair: int = 4
+    cluster_normal_dot_thresh: float = 0.996
+    cluster_pos_thresh: float = 5e-3
+
+    def __post_init__(self):
+        if self.policy not in REDUCTION_POLICIES:
+            raise NotImplementedError(f"contact reduction policy {self.policy!r} is not built "
+                                      f"(built: {', '.join(REDUCTION_POLICIES)})")
+        if self.max_per_pair < 1:
+            raise ValueError(f"max_per_pair must be >= 1, got {self.max_per_pair}")
+        if not (-1.0 <= self.cluster_normal_dot_thresh <= 1.0):
+            raise ValueError(f"cluster_normal_dot_thresh must be in [-1, 1], got {self.cluster_normal_dot_thresh}")
+        if self.cluster_pos_thresh < 0.0:
+            raise ValueError(f"cluster_pos_thresh must be >= 0, got {self.cluster_pos_thresh}")
+
+
 @dataclass(frozen=True)
 class ContactsConfig:
-    """engine_config.py:380-400 (contact reduction is not built; policy must be none)."""
+    """engine_config.py:380-400: contact slots per world and the per-pair reduction."""
 
     max_per_world: int = 128
+    reduction: ContactReductionConfig = field(default_factory=ContactReductionConfig)
 
     def __post_init__(self):
         if self.max_per_world < 1:
@@ -351,6 +377,11 @@ class OstrichEngineConfig:
         c.linesearch_optimistic_count = ls.optimistic_step_count
         c.linesearch_optimistic_window = ls.optimistic_window
         c.pair_small_worlds = 1 if self.execution.pair_small_worlds else 0
+        red = self.contacts.reduction
+        c.contact_reduction = REDUCTION_POLICIES[red.policy]
+        c.contact_max_per_pair = red.max_per_pair
+        c.cluster_normal_dot = red.cluster_normal_dot_thresh
+        c.cluster_pos = red.cluster_pos_thresh
         return c
 
     @property

@@ -285,6 +285,14 @@ int ofb_create(const ofb_model* model, const ofb_config* config, int32_t path, o
   }
   if (config->lin_max_iters < 1) { set_error("linear.max_iters must be >= 1"); return -2; }
   if (config->max_contacts_per_world < 1) { set_error("contacts.max_per_world must be >= 1"); return -2; }
+  if (config->contact_reduction < OFB_REDUCE_NONE || config->contact_reduction > OFB_REDUCE_CLUSTER) {
+    set_error("unknown contact reduction policy %d", config->contact_reduction);
+    return -2;
+  }
+  if (config->contact_reduction != OFB_REDUCE_NONE && config->contact_max_per_pair < 1) {
+    set_error("contacts.reduction.max_per_pair must be >= 1 (collision/config.py:49-52)");
+    return -2;
+  }
   if (config->preconditioner != OFB_PRECOND_JACOBI && config->preconditioner != OFB_PRECOND_PER_BODY_PAIR) {
     set_error("unknown preconditioner"); return -2;
   }
@@ -437,6 +445,7 @@ int ofb_step(ofb_solver* s, const ofb_state_in* in, const ofb_state_out* out, co
   hipStream_t st = (hipStream_t)stream;
   int rc = pack_contacts(s, contacts, st);
   if (rc) return rc;
+  if ((rc = reduce_contacts(s, in->body_q, st)) != 0) return rc;  // base_engine.py:317-320
   // the warm starter's seed reaches the solve only through seed_iterate (engine.py:52-59)
   const bool ws = s->cfg.warm_start && s->cfg.seed_iterate;
   // timing events are left out of a step captured into a graph
@@ -560,6 +569,7 @@ int ofb_step_backward(ofb_solver* s, const ofb_state_in* in, const ofb_state_out
   hipStream_t st = (hipStream_t)stream;
   int rc = pack_contacts(s, contacts, st);
   if (rc) return rc;
+  if ((rc = reduce_contacts(s, in->body_q, st)) != 0) return rc;
   const bool timed = !stream_capturing(st);
   s->solve_timed = timed ? 1 : 0;
   s->kernel_timed = 0;

@@ -120,6 +120,7 @@ int plain_create(ofb_solver* s);
 void plain_destroy(ofb_solver* s);
 int plain_step(ofb_solver* s, const ofb_state_in* in, const ofb_state_out* out, const ofb_control* ctl, float dt,
                hipStream_t stream);
+int reduce_contacts(ofb_solver* s, const float* body_q, hipStream_t stream);
 int warm_start_apply(ofb_solver* s, const ofb_state_in* in, float dt, hipStream_t stream);
 int warm_start_snapshot(ofb_solver* s, const ofb_state_in* in, float dt, hipStream_t stream);
 int warm_start_learned_seed(ofb_solver* s, const float* y, hipStream_t stream);
