@@ -354,6 +354,15 @@ int ofb_last_kernel_ms(const ofb_solver* solver, float* ms);
 int ofb_set_fused_launch(ofb_solver* solver, int32_t waves_per_block, int32_t pair_small_worlds);
 int ofb_get_fused_launch(const ofb_solver* solver, int32_t* waves_per_block, int32_t* pair_small_worlds);
 
+/* One Adam step on a control trajectory params[steps][worlds][dofs] (device, f32), moments
+ * m and v of the same shape (zero before the first step), for the dof slice
+ * [dof_offset, dof_offset + num_dofs) of every (step, world): optim/trajectory_adam.py:
+ * 6-53 and TrajectoryAdam.step (:105-131), with t the 0-based step counter.  Returns -2
+ * on a slice outside the array. */
+int ofb_trajectory_adam_step(float* params, const float* grad, float* m, float* v, int32_t steps,
+                             int32_t worlds, int32_t dofs, int32_t dof_offset, int32_t num_dofs, float lr,
+                             float beta1, float beta2, float eps, float clip_grad, int32_t t, void* stream);
+
 /* Last error message of the calling thread ("" if none). */
 const char* ofb_last_error(void);
 

@@ -15,7 +15,7 @@ EXPORTS = (
     "ofb_get_lambda_seed", "ofb_mlp_create", "ofb_mlp_destroy", "ofb_mlp_num_params", "ofb_mlp_set_params",
     "ofb_mlp_get_params", "ofb_mlp_forward", "ofb_mlp_gradient", "ofb_mlp_train_step",
     "ofb_set_warm_start_net", "ofb_get_contact_features", "ofb_get_linesearch_steps",
-    "ofb_last_kernel_ms", "ofb_set_fused_launch", "ofb_get_fused_launch",
+    "ofb_last_kernel_ms", "ofb_set_fused_launch", "ofb_get_fused_launch", "ofb_trajectory_adam_step",
 )
 
 _lib = None
@@ -41,6 +41,8 @@ def load():
     lib.ofb_last_kernel_ms.argtypes = [vp, C.POINTER(C.c_float)]
     lib.ofb_set_fused_launch.argtypes = [vp, i32, i32]
     lib.ofb_get_fused_launch.argtypes = [vp, vp, vp]
+    f32 = C.c_float
+    lib.ofb_trajectory_adam_step.argtypes = [vp, vp, vp, vp] + [i32] * 5 + [f32] * 5 + [i32, vp]
     lib.ofb_last_error.restype = C.c_char_p
     lib.ofb_get_lambda_prev.argtypes = [vp, vp, vp]
     lib.ofb_set_shape_mu.argtypes = [vp, vp, vp]

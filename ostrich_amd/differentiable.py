@@ -211,3 +211,46 @@ class OstrichRollout(torch.autograd.Function):
         N = len(ro.controls)
         return (None, None, None, None, g["body_q"], g["body_qd"], g["joint_target_pos"].reshape(N, -1),
                 g["joint_target_vel"].reshape(N, -1))
+
+
+class TrajectoryAdam:
+    """Adam over a control trajectory `params` [steps, worlds, dofs] (a contiguous float32
+    device tensor), restating optim/trajectory_adam.py:56-137: the dof slice
+    [dof_offset, dof_offset + num_dofs) is updated with per-element gradient clamping,
+    biased moments and bias correction with the 0-based step counter.  The update runs in
+    the gfx950 library (`ofb_trajectory_adam_step`)."""
+
+    def __init__(self, params: torch.Tensor, lr: float = 0.05, betas=(0.9, 0.999), eps: float = 1e-7,
+                 dof_offset: int = 0, num_dofs: Optional[int] = None, clip_grad: float = 100.0):
+        if params.dim() != 3 or params.dtype != torch.float32 or not params.is_contiguous():
+            raise ValueError("params must be a contiguous float32 tensor [steps, worlds, dofs]")
+        self.params = params
+        self.lr, (self.beta1, self.beta2), self.eps = lr, betas, eps
+        self.dof_offset = dof_offset
+        self.num_dofs = params.shape[2] - dof_offset if num_dofs is None else num_dofs
+        self.clip_grad = clip_grad
+        self.t = 0
+        self.m = torch.zeros_like(params)
+        self.v = torch.zeros_like(params)
+
+    def step(self, grad: torch.Tensor):
+        import ctypes as C
+
+        from . import _lib
+        if grad.shape != self.params.shape or grad.dtype != torch.float32:
+            raise ValueError("grad must match params")
+        grad = grad.contiguous()
+        lib = _lib.load()
+        S, W, D = self.params.shape
+        stream = C.c_void_p(torch.cuda.current_stream(self.params.device).cuda_stream)
+        _lib.check(lib.ofb_trajectory_adam_step(
+            C.c_void_p(self.params.data_ptr()), C.c_void_p(grad.data_ptr()), C.c_void_p(self.m.data_ptr()),
+            C.c_void_p(self.v.data_ptr()), S, W, D, self.dof_offset, self.num_dofs, C.c_float(self.lr),
+            C.c_float(self.beta1), C.c_float(self.beta2), C.c_float(self.eps), C.c_float(self.clip_grad), self.t,
+            stream), "ofb_trajectory_adam_step")
+        self.t += 1
+
+    def reset_internal_state(self):
+        self.m.zero_()
+        self.v.zero_()
+        self.t = 0
