@@ -69,18 +69,34 @@ __device__ __forceinline__ float wave_sum(float v) {
 // G = 32 (each half of the wave holds its own world).  The group reductions below stay
 // inside a group, so the two worlds of a wave never mix; every lane of an active group
 // must be active (the other group may have retired).
+// 32-lane group sums that every lane of the group holds bit for bit (the stop rules and
+// the PCR scalars of a world are evaluated by each of its lanes, so they must agree):
+// pair, quad, half-row and row exchanges, each adding a lane's partial sum to its
+// partner's, give every lane of a 16-lane row the same row sum (a + b and b + a round
+// alike); v_permlane16_swap_b32 with both operands v then returns (rows 0,0,2,2) and
+// (rows 1,1,3,3), whose sum is the half's total in every lane of the half -- no
+// read-back through scalar registers.  The adds are kept out of FMA contraction: a
+// product fused into the first exchange would round differently in the two lanes of a
+// pair.
+__device__ __forceinline__ float row_sum_sym(float v) {
+#pragma clang fp contract(off)
+  v = v + OFB_DPP(v, 0xB1, 0xF);   // quad_perm [1,0,3,2]
+  v = v + OFB_DPP(v, 0x4E, 0xF);   // quad_perm [2,3,0,1]
+  v = v + OFB_DPP(v, 0x141, 0xF);  // row_half_mirror: quads 0<->1, 2<->3
+  v = v + OFB_DPP(v, 0x140, 0xF);  // row_mirror: half-rows 0<->1
+  return v;
+}
+__device__ __forceinline__ float half_sum_from_rows(float v) {
+#pragma clang fp contract(off)
+  const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+  return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+
 template <int G>
 __device__ __forceinline__ float group_sum(float v) {
   static_assert(G == 32 || G == 64, "lane groups are 32 or 64 lanes");
   if (G == 64) return wave_sum(v);
-  v = v + OFB_DPP(v, 0xB1, 0xF);   // quad_perm [1,0,3,2]
-  v = v + OFB_DPP(v, 0x4E, 0xF);   // quad_perm [2,3,0,1]
-  v = v + OFB_DPP(v, 0x124, 0xF);  // row_ror:4
-  v = v + OFB_DPP(v, 0x128, 0xF);  // row_ror:8
-  v = v + OFB_DPP(v, 0x142, 0xA);  // row_bcast:15: lane 31 holds half 0's sum, lane 63 half 1's
-  const float lo = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 31));
-  const float hi = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
-  return (threadIdx.x & 32) ? hi : lo;
+  return half_sum_from_rows(row_sum_sym(v));
 }
 
 // two group sums whose DPP chains interleave (one chain's wait states are the other's
@@ -88,6 +104,20 @@ __device__ __forceinline__ float group_sum(float v) {
 template <int G>
 __device__ __forceinline__ void group_sum2(float& a, float& b) {
   static_assert(G == 32 || G == 64, "lane groups are 32 or 64 lanes");
+  if (G == 32) {
+#pragma clang fp contract(off)
+    a = a + OFB_DPP(a, 0xB1, 0xF);
+    b = b + OFB_DPP(b, 0xB1, 0xF);
+    a = a + OFB_DPP(a, 0x4E, 0xF);
+    b = b + OFB_DPP(b, 0x4E, 0xF);
+    a = a + OFB_DPP(a, 0x141, 0xF);
+    b = b + OFB_DPP(b, 0x141, 0xF);
+    a = a + OFB_DPP(a, 0x140, 0xF);
+    b = b + OFB_DPP(b, 0x140, 0xF);
+    a = half_sum_from_rows(a);
+    b = half_sum_from_rows(b);
+    return;
+  }
   a = a + OFB_DPP(a, 0xB1, 0xF);
   b = b + OFB_DPP(b, 0xB1, 0xF);
   a = a + OFB_DPP(a, 0x4E, 0xF);
@@ -98,20 +128,10 @@ __device__ __forceinline__ void group_sum2(float& a, float& b) {
   b = b + OFB_DPP(b, 0x128, 0xF);
   a = a + OFB_DPP(a, 0x142, 0xA);
   b = b + OFB_DPP(b, 0x142, 0xA);
-  if (G == 64) {
-    a = a + OFB_DPP(a, 0x143, 0xC);
-    b = b + OFB_DPP(b, 0x143, 0xC);
-    a = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(a), 63));
-    b = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(b), 63));
-    return;
-  }
-  const float alo = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(a), 31));
-  const float ahi = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(a), 63));
-  const float blo = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(b), 31));
-  const float bhi = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(b), 63));
-  const bool hi = (threadIdx.x & 32) != 0;
-  a = hi ? ahi : alo;
-  b = hi ? bhi : blo;
+  a = a + OFB_DPP(a, 0x143, 0xC);
+  b = b + OFB_DPP(b, 0x143, 0xC);
+  a = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(a), 63));
+  b = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(b), 63));
 }
 
 template <int G>
