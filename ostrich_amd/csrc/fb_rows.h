@@ -51,20 +51,6 @@ __device__ __forceinline__ void wsync() {
 #define OFB_DPP(v, ctrl, rmask)                                                                  \
   __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), (ctrl), (rmask), 0xF, false))
 
-// Sum over the 64 lanes with DPP: pairwise and quad swaps, row rotations by 4 and 8 (every
-// lane of a 16-lane row then holds the row sum), row broadcasts 15 and 31 (lane 63 holds
-// the wave sum), read back from lane 63 so every lane returns the same value.  All lanes
-// of the wave must be active.
-__device__ __forceinline__ float wave_sum(float v) {
-  v = v + OFB_DPP(v, 0xB1, 0xF);   // quad_perm [1,0,3,2]
-  v = v + OFB_DPP(v, 0x4E, 0xF);   // quad_perm [2,3,0,1]
-  v = v + OFB_DPP(v, 0x124, 0xF);  // row_ror:4
-  v = v + OFB_DPP(v, 0x128, 0xF);  // row_ror:8
-  v = v + OFB_DPP(v, 0x142, 0xA);  // row_bcast:15 into rows 1 and 3
-  v = v + OFB_DPP(v, 0x143, 0xC);  // row_bcast:31 into rows 2 and 3
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
-}
-
 // Lane groups: a fused kernel runs one world per group of G lanes, G = 64 (the wave) or
 // G = 32 (each half of the wave holds its own world).  The group reductions below stay
 // inside a group, so the two worlds of a wave never mix; every lane of an active group
@@ -92,6 +78,17 @@ __device__ __forceinline__ float half_sum_from_rows(float v) {
   return __uint_as_float(r[0]) + __uint_as_float(r[1]);
 }
 
+// 32-lane halves to the wave total: v_permlane32_swap_b32 with both operands v returns
+// (half 0, half 0) and (half 1, half 1), whose sum every lane holds
+__device__ __forceinline__ float wave_sum_from_halves(float v) {
+#pragma clang fp contract(off)
+  const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+  return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+
+// sum over the 64 lanes, the same bits in every lane; all lanes must be active
+__device__ __forceinline__ float wave_sum(float v) { return wave_sum_from_halves(half_sum_from_rows(row_sum_sym(v))); }
+
 template <int G>
 __device__ __forceinline__ float group_sum(float v) {
   static_assert(G == 32 || G == 64, "lane groups are 32 or 64 lanes");
@@ -103,35 +100,22 @@ __device__ __forceinline__ float group_sum(float v) {
 // issue slots)
 template <int G>
 __device__ __forceinline__ void group_sum2(float& a, float& b) {
-  static_assert(G == 32 || G == 64, "lane groups are 32 or 64 lanes");
-  if (G == 32) {
 #pragma clang fp contract(off)
-    a = a + OFB_DPP(a, 0xB1, 0xF);
-    b = b + OFB_DPP(b, 0xB1, 0xF);
-    a = a + OFB_DPP(a, 0x4E, 0xF);
-    b = b + OFB_DPP(b, 0x4E, 0xF);
-    a = a + OFB_DPP(a, 0x141, 0xF);
-    b = b + OFB_DPP(b, 0x141, 0xF);
-    a = a + OFB_DPP(a, 0x140, 0xF);
-    b = b + OFB_DPP(b, 0x140, 0xF);
-    a = half_sum_from_rows(a);
-    b = half_sum_from_rows(b);
-    return;
-  }
+  static_assert(G == 32 || G == 64, "lane groups are 32 or 64 lanes");
   a = a + OFB_DPP(a, 0xB1, 0xF);
   b = b + OFB_DPP(b, 0xB1, 0xF);
   a = a + OFB_DPP(a, 0x4E, 0xF);
   b = b + OFB_DPP(b, 0x4E, 0xF);
-  a = a + OFB_DPP(a, 0x124, 0xF);
-  b = b + OFB_DPP(b, 0x124, 0xF);
-  a = a + OFB_DPP(a, 0x128, 0xF);
-  b = b + OFB_DPP(b, 0x128, 0xF);
-  a = a + OFB_DPP(a, 0x142, 0xA);
-  b = b + OFB_DPP(b, 0x142, 0xA);
-  a = a + OFB_DPP(a, 0x143, 0xC);
-  b = b + OFB_DPP(b, 0x143, 0xC);
-  a = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(a), 63));
-  b = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(b), 63));
+  a = a + OFB_DPP(a, 0x141, 0xF);
+  b = b + OFB_DPP(b, 0x141, 0xF);
+  a = a + OFB_DPP(a, 0x140, 0xF);
+  b = b + OFB_DPP(b, 0x140, 0xF);
+  a = half_sum_from_rows(a);
+  b = half_sum_from_rows(b);
+  if (G == 64) {
+    a = wave_sum_from_halves(a);
+    b = wave_sum_from_halves(b);
+  }
 }
 
 template <int G>
