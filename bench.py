@@ -232,10 +232,24 @@ def adjoint_leg(device, worlds=1024, steps=50, checkpoint_every=10, seed=3):
     e2.record(stream)
     e2.synchronize()
     fwd, bwd = e0.elapsed_time(e1), e1.elapsed_time(e2)
+    # the adjoint solves alone: a second backward with each step_backward's own HIP-event
+    # time (packing + adjoint kernels on the step's stream) collected after the call
+    adj = []
+    inner = eng.step_backward
+
+    def timed_backward(*a, **kw):
+        out = inner(*a, **kw)
+        adj.append(eng.last_solve_ms())
+        return out
+    eng.step_backward = timed_backward
+    ro.backward(gq.reshape(-1, 7), torch.zeros_like(fin.body_qd))
+    eng.step_backward = inner
     return dict(worlds=worlds, steps=steps, checkpoint_every=checkpoint_every, forward_ms=fwd, backward_ms=bwd,
-                backward_ms_per_step=bwd / steps, loss=float(loss.item()),
-                grad_target_vel_norm=float(g["joint_target_vel"].norm().item()),
-                note="backward includes re-simulating each 10-step window from its checkpoint")
+                backward_ms_per_step=bwd / steps, adjoint_solve_ms_per_step=float(np.mean(adj)),
+                loss=float(loss.item()), grad_target_vel_norm=float(g["joint_target_vel"].norm().item()),
+                note="forward and backward include the collider and host work of every step; the backward "
+                     "re-simulates each 10-step window from its checkpoint; adjoint_solve_ms_per_step is the "
+                     "adjoint solve of one step (HIP events on its stream)")
 
 
 def cpu_baseline(model_fn, sample_worlds, sample_steps, precond="jacobi", seed="prev"):
