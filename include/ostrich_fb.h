@@ -363,6 +363,62 @@ int ofb_trajectory_adam_step(float* params, const float* grad, float* m, float* 
                              int32_t worlds, int32_t dofs, int32_t dof_offset, int32_t num_dofs, float lr,
                              float beta1, float beta2, float eps, float clip_grad, int32_t t, void* stream);
 
+/* ---- primitive collider on the GPU ---------------------------------------------------
+ * The contact-generation step before the solver (Newton's collide, called at
+ * simulation/base_simulator.py:118 before integrator.simulate).  Newton's narrow phase is
+ * a third-party dependency absent here; this collider generates contacts for the
+ * benchmark primitives (plane, box, cylinder, sphere, heightfield) from fixed per-model
+ * candidate points and writes them in the ofb_contacts format.  The description holds
+ * HOST pointers (copied at creation); geometry codes are 0 plane, 1 box, 2 cylinder,
+ * 3 sphere, 4 heightfield; per-shape arrays are [W*shape_cols] (xform 7 floats, size 3). */
+typedef struct ofb_collider_desc {
+  int32_t num_worlds;
+  int32_t bodies_per_world;
+  int32_t shape_cols;
+  int32_t num_pairs;
+  const int32_t* pairs;         /* [num_pairs*2] (column of shape0, column of plane/box) */
+  const int32_t* shape_geo;     /* [W*SC] */
+  const int32_t* shape_body;    /* [W*SC] world-local body, -1 static */
+  const float* shape_xform;     /* [W*SC*7] */
+  const float* shape_size;      /* [W*SC*3] */
+  const int32_t* flat_shape_id; /* [W*SC] id written to shape0/shape1 */
+  int32_t cylinder_samples;     /* k: 3k rim points of a cylinder against a box */
+  float gap;                    /* candidate is a contact when its distance < gap */
+  float lateral_slack;          /* against a box: allowed excess on the tangent axes */
+  int32_t max_contacts_per_world;
+  /* spheres on bodies against one heightfield column (num_hf_spheres = 0: none) */
+  int32_t num_hf_spheres;
+  const int32_t* hf_spheres;    /* [num_hf_spheres] shape columns */
+  int32_t hf_col;
+  int32_t hf_nx;
+  int32_t hf_ny;
+  float hf_origin[2];
+  float hf_spacing[2];
+  const float* hf_heights;      /* [hf_ny*hf_nx] row-major (y, x) */
+} ofb_collider_desc;
+
+/* Output of ofb_collide: the same fields as ofb_contacts, writable (device pointers). */
+typedef struct ofb_contact_buffers {
+  int32_t* count; /* [1] */
+  int32_t max_contacts;
+  float* point0;
+  float* point1;
+  float* normal;
+  int32_t* shape0;
+  int32_t* shape1;
+  float* margin0;
+  float* margin1;
+} ofb_contact_buffers;
+
+typedef struct ofb_collider ofb_collider;
+int ofb_collider_create(const ofb_collider_desc* desc, ofb_collider** out);
+int ofb_collider_destroy(ofb_collider* collider);
+/* Contacts of body poses body_q [W*B*7] (device), in (world, pair, point) order with the
+ * heightfield spheres after the pairs, capped at W*max_contacts_per_world; count[0] is
+ * written on the device (no host synchronisation, capturable).  Slots at and beyond
+ * count[0] are left as they were. */
+int ofb_collide(ofb_collider* collider, const float* body_q, const ofb_contact_buffers* out, void* stream);
+
 /* Last error message of the calling thread ("" if none). */
 const char* ofb_last_error(void);
 

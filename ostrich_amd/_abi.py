@@ -128,6 +128,32 @@ class OfbContacts(C.Structure):
     ]
 
 
+class OfbColliderDesc(C.Structure):
+    _fields_ = [
+        ("num_worlds", C.c_int32), ("bodies_per_world", C.c_int32), ("shape_cols", C.c_int32),
+        ("num_pairs", C.c_int32),
+        ("pairs", C.c_void_p),
+        ("shape_geo", C.c_void_p),
+        ("shape_body", C.c_void_p),
+        ("shape_xform", C.c_void_p),
+        ("shape_size", C.c_void_p),
+        ("flat_shape_id", C.c_void_p),
+        ("cylinder_samples", C.c_int32),
+        ("gap", C.c_float),
+        ("lateral_slack", C.c_float),
+        ("max_contacts_per_world", C.c_int32),
+        ("num_hf_spheres", C.c_int32),
+        ("hf_spheres", C.c_void_p),
+        ("hf_col", C.c_int32), ("hf_nx", C.c_int32), ("hf_ny", C.c_int32),
+        ("hf_origin", C.c_float * 2), ("hf_spacing", C.c_float * 2),
+        ("hf_heights", C.c_void_p),
+    ]
+
+
+class OfbContactBuffers(C.Structure):
+    _fields_ = OfbContacts._fields_
+
+
 class OfbAdjointIn(C.Structure):
     _fields_ = [("body_q_grad", C.c_void_p), ("body_qd_grad", C.c_void_p)]
 

@@ -16,6 +16,7 @@ EXPORTS = (
     "ofb_mlp_get_params", "ofb_mlp_forward", "ofb_mlp_gradient", "ofb_mlp_train_step",
     "ofb_set_warm_start_net", "ofb_get_contact_features", "ofb_get_linesearch_steps",
     "ofb_last_kernel_ms", "ofb_set_fused_launch", "ofb_get_fused_launch", "ofb_trajectory_adam_step",
+    "ofb_collider_create", "ofb_collider_destroy", "ofb_collide",
 )
 
 _lib = None
@@ -43,6 +44,9 @@ def load():
     lib.ofb_get_fused_launch.argtypes = [vp, vp, vp]
     f32 = C.c_float
     lib.ofb_trajectory_adam_step.argtypes = [vp, vp, vp, vp] + [i32] * 5 + [f32] * 5 + [i32, vp]
+    lib.ofb_collider_create.argtypes = [vp, C.POINTER(vp)]
+    lib.ofb_collider_destroy.argtypes = [vp]
+    lib.ofb_collide.argtypes = [vp, vp, vp, vp]
     lib.ofb_last_error.restype = C.c_char_p
     lib.ofb_get_lambda_prev.argtypes = [vp, vp, vp]
     lib.ofb_set_shape_mu.argtypes = [vp, vp, vp]

@@ -291,3 +291,91 @@ class PrimitiveCollider:
             margin0=torch.zeros(cap, dtype=torch.float32, device=self.device),
             margin1=torch.zeros(cap, dtype=torch.float32, device=self.device),
         )
+
+
+class HipCollider:
+    """PrimitiveCollider's contact generation as gfx950 kernels (`ofb_collide`).
+
+    Same candidate pairs, sample points, tests and (world, pair, point) order as
+    PrimitiveCollider; the contacts are written into persistent device tensors (one
+    `Contacts` object reused by every call), so a graph captured over `collide` and the
+    solver step replays both.  Requires the HIP library and a GPU device."""
+
+    def __init__(self, model, device="cuda:0", gap=0.02, cylinder_samples=16, lateral_slack=0.06,
+                 max_contacts_per_world=128):
+        import ctypes as C
+
+        from . import _abi, _lib
+        from .containers import Contacts
+        self._lib = _lib.load()
+        self.model = model
+        self.device = torch.device(device)
+        if self.device.type != "cuda":
+            raise ValueError("HipCollider runs on a GPU device")
+        plan = PrimitiveCollider(model, device="cpu", gap=gap, cylinder_samples=cylinder_samples,
+                                 lateral_slack=lateral_slack, max_contacts_per_world=max_contacts_per_world)
+        self.pairs = plan.pairs
+        self.hf_spheres = plan.hf_spheres
+        W, SC = model.shape_geo.shape
+        # host arrays the description points to (copied by ofb_collider_create)
+        keep = dict(
+            pairs=np.ascontiguousarray(np.array(self.pairs, np.int32).reshape(-1, 2)),
+            geo=np.ascontiguousarray(model.shape_geo, np.int32),
+            body=np.ascontiguousarray(model.shape_body_2d, np.int32),
+            xform=np.ascontiguousarray(model.shape_xform, np.float32),
+            size=np.ascontiguousarray(model.shape_size, np.float32),
+            flat=np.ascontiguousarray(model.flat_shape_id(np.arange(W)[:, None], np.arange(SC)[None, :]), np.int32),
+            hfs=np.ascontiguousarray(np.array(self.hf_spheres, np.int32)),
+        )
+        ptr = lambda a: a.ctypes.data if a.size else None
+        d = _abi.OfbColliderDesc()
+        d.num_worlds, d.bodies_per_world, d.shape_cols = W, model.bodies_per_world, SC
+        d.num_pairs = len(self.pairs)
+        d.pairs, d.shape_geo, d.shape_body = ptr(keep["pairs"]), ptr(keep["geo"]), ptr(keep["body"])
+        d.shape_xform, d.shape_size, d.flat_shape_id = ptr(keep["xform"]), ptr(keep["size"]), ptr(keep["flat"])
+        d.cylinder_samples, d.gap, d.lateral_slack = cylinder_samples, gap, lateral_slack
+        d.max_contacts_per_world = max_contacts_per_world
+        d.num_hf_spheres = len(self.hf_spheres)
+        d.hf_spheres = ptr(keep["hfs"])
+        d.hf_col = plan.hf_col
+        if self.hf_spheres:
+            h = np.ascontiguousarray(np.asarray(model.heightfield["heights"], np.float32))
+            keep["hf"] = h
+            d.hf_ny, d.hf_nx = h.shape
+            d.hf_origin[0], d.hf_origin[1] = plan.hf_origin[0], plan.hf_origin[1]
+            d.hf_spacing[0], d.hf_spacing[1] = plan.hf_spacing[0], plan.hf_spacing[1]
+            d.hf_heights = h.ctypes.data
+        self._handle = C.c_void_p()
+        with torch.cuda.device(self.device):
+            _lib.check(self._lib.ofb_collider_create(C.byref(d), C.byref(self._handle)), "ofb_collider_create")
+        cap = W * max_contacts_per_world
+        self.max_contacts = cap
+        z = lambda *s, dt=torch.float32: torch.zeros(*s, dtype=dt, device=self.device)
+        self.contacts = Contacts(z(1, dt=torch.int32), z(cap, 3), z(cap, 3), z(cap, 3), z(cap, dt=torch.int32),
+                                 z(cap, dt=torch.int32), z(cap), z(cap))
+        c = self.contacts
+        self._out = _abi.OfbContactBuffers(
+            c.rigid_contact_count.data_ptr(), cap, c.rigid_contact_point0.data_ptr(),
+            c.rigid_contact_point1.data_ptr(), c.rigid_contact_normal.data_ptr(), c.rigid_contact_shape0.data_ptr(),
+            c.rigid_contact_shape1.data_ptr(), c.rigid_contact_margin0.data_ptr(), c.rigid_contact_margin1.data_ptr())
+
+    def collide(self, body_q):
+        """Contacts of body_q [W*B, 7] (float32, on the collider's device), written into
+        `self.contacts` on the current stream; returns that Contacts object."""
+        import ctypes as C
+
+        from . import _lib
+        W, B = self.model.num_worlds, self.model.bodies_per_world
+        if (body_q.device != self.device or body_q.dtype != torch.float32 or not body_q.is_contiguous()
+                or body_q.numel() != W * B * 7):
+            raise ValueError(f"body_q must be a contiguous float32 [{W * B}, 7] tensor on {self.device}")
+        stream = C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+        _lib.check(self._lib.ofb_collide(self._handle, C.c_void_p(body_q.data_ptr()), C.byref(self._out), stream),
+                   "ofb_collide")
+        return self.contacts
+
+    def __del__(self):
+        h = getattr(self, "_handle", None)
+        if h is not None and h.value:
+            self._lib.ofb_collider_destroy(h)
+            self._handle = None

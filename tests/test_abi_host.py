@@ -39,7 +39,9 @@ def _struct_fields(name):
                                        ("ofb_contacts", _abi.OfbContacts), ("ofb_state_in", _abi.OfbStateIn),
                                        ("ofb_state_out", _abi.OfbStateOut), ("ofb_control", _abi.OfbControl),
                                        ("ofb_adjoint_in", _abi.OfbAdjointIn),
-                                       ("ofb_adjoint_out", _abi.OfbAdjointOut)])
+                                       ("ofb_adjoint_out", _abi.OfbAdjointOut),
+                                       ("ofb_collider_desc", _abi.OfbColliderDesc),
+                                       ("ofb_contact_buffers", _abi.OfbContactBuffers)])
 def test_ctypes_structs_match_header(cname, pyc):
     assert _struct_fields(cname) == [f[0] for f in pyc._fields_]
 
