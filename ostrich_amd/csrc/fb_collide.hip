@@ -11,7 +11,10 @@
 // lateral excess is within the slack on the non-normal axes).  Contacts are emitted in
 // (world, pair, point) order, capped at W * max_per_world, in two passes: one workgroup per
 // world counts its contacts, one exclusive scan over worlds, and a second pass writes each
-// world's contacts at its offset.  Fixed-size outputs and no host synchronisation: the
+// world's contacts at its offset.  Each pass first drops the pairs whose shapes cannot
+// touch (bounding sphere of shape0 against the plane or the box grown by the thresholds,
+// a conservative test, so no contact is lost) and compacts the candidate points of the
+// remaining pairs in their original order.  Fixed-size outputs and no host synchronisation: the
 // pass can be captured into a graph together with the solver step.
 
 #include <hip/hip_runtime.h>
@@ -29,7 +32,6 @@ struct ofb_collider {
   float *xform, *size, *hf_heights;
   int32_t* world_count;   // [W]
   int32_t* world_offset;  // [W + 1]
-  int cand;               // candidates per world
 };
 
 namespace {
@@ -67,8 +69,9 @@ struct ColCtx {
   __device__ v3 sz(int c) const { const float* s = size + (size_t)col(c) * 3; return mk(s[0], s[1], s[2]); }
 };
 
-// sample point s of shape column a in its shape frame (against a box)
-__device__ __forceinline__ v3 sample_point(const ColCtx& C, int a, int s) {
+// sample point s of shape column a in its shape frame (against a box); ang_cs / ang_sn
+// hold cos / sin of j * 2 pi / k for the cylinder rim
+__device__ __forceinline__ v3 sample_point(const ColCtx& C, int a, int s, const float* ang_cs, const float* ang_sn) {
   const int g = C.geo[C.col(a)];
   const v3 h = C.sz(a);
   if (g == CGEO_BOX) {  // corners in (x, y, z) sign order -,-,- ... +,+,+
@@ -77,13 +80,12 @@ __device__ __forceinline__ v3 sample_point(const ColCtx& C, int a, int s) {
   if (g == CGEO_CYLINDER) {  // 3 rings (y = -hh, 0, +hh) of k rim points, axis = local y
     const int k = C.d.cylinder_samples;
     const int ring = s / k, j = s % k;
-    const float ang = (float)j * (float)(6.283185307179586 / (double)k);
-    return mk(cosf(ang) * h.x, (float)(ring - 1) * h.y, sinf(ang) * h.x);
+    return mk(ang_cs[j] * h.x, (float)(ring - 1) * h.y, ang_sn[j] * h.x);
   }
   return mk(0.f, 0.f, 0.f);  // sphere: centre
 }
 
-__device__ Cand eval_pair_point(const ColCtx& C, int a, int b, int s) {
+__device__ Cand eval_pair_point(const ColCtx& C, int a, int b, int s, const float* ang_cs, const float* ang_sn) {
   Cand o;
   o.valid = false;
   o.s0 = C.flat_id[C.col(a)];
@@ -104,14 +106,14 @@ __device__ Cand eval_pair_point(const ColCtx& C, int a, int b, int s) {
     } else if (ga == CGEO_SPHERE) {
       pw = Xa.p - nrm * C.sz(a).x;
     } else {
-      pw = Xa.p + qrot(Xa.q, sample_point(C, a, s));
+      pw = Xa.p + qrot(Xa.q, sample_point(C, a, s, ang_cs, ang_sn));
     }
     dist = dot(pw - Xb.p, nrm);
     p1w = pw - nrm * dist;
     n_out = nrm;
   } else {  // box b
     const v3 hb = C.sz(b);
-    pw = Xa.p + qrot(Xa.q, sample_point(C, a, s));
+    pw = Xa.p + qrot(Xa.q, sample_point(C, a, s, ang_cs, ang_sn));
     const v3 q = qrot_inv(Xb.q, pw - Xb.p);
     const float ex[3] = {fabsf(q.x) - hb.x, fabsf(q.y) - hb.y, fabsf(q.z) - hb.z};
     int axis = 0;
@@ -186,21 +188,42 @@ __device__ Cand eval_heightfield(const ColCtx& C, const float* __restrict__ hgt,
   return o;
 }
 
-__device__ __forceinline__ Cand eval_candidate(const ColCtx& C, const int32_t* pairs, const int32_t* pair_off,
-                                               const int32_t* hf_spheres, const float* hgt, int c) {
-  const int P = C.d.num_pairs;
-  if (c >= pair_off[P]) return eval_heightfield(C, hgt, hf_spheres[c - pair_off[P]]);
-  int p = 0;
-  while (pair_off[p + 1] <= c) ++p;  // few pairs per model; uniform work per candidate
-  return eval_pair_point(C, pairs[2 * p], pairs[2 * p + 1], c - pair_off[p]);
+// Can any candidate of pair (a, b) be a contact?  Every sample point of shape a lies
+// within bound_radius of its centre (for a sphere the candidate is the centre itself, its
+// radius enters the thresholds), and a contact needs its signed distance below the gap
+// (plane) or every box-frame excess below max(gap (+ radius), slack) (box).  `eps` covers
+// the rounding of the sample points, so the test never drops a contact.
+__device__ bool pair_may_touch(const ColCtx& C, int a, int b) {
+  const int ga = C.geo[C.col(a)], gb = C.geo[C.col(b)];
+  const tf Xa = C.shape_pose(a), Xb = C.shape_pose(b);
+  const v3 h = C.sz(a);
+  float R = 0.f, r = 0.f;
+  if (ga == CGEO_BOX) R = sqrtf(dot(h, h));
+  else if (ga == CGEO_CYLINDER) R = sqrtf(h.x * h.x + h.y * h.y);
+  else r = h.x;  // sphere
+  const float eps = 1e-3f + 1e-5f * (sqrtf(dot(Xa.p, Xa.p)) + sqrtf(dot(Xb.p, Xb.p)));
+  const v3 d = Xa.p - Xb.p;
+  if (gb == CGEO_PLANE) {
+    const v3 nrm = qrot(Xb.q, mk(0.f, 0.f, 1.f));
+    return dot(d, nrm) - R - r < C.d.gap + eps;
+  }
+  const v3 q = qrot_inv(Xb.q, d), hb = C.sz(b);
+  const float m = fmaxf(C.d.gap, C.d.lateral_slack) + r;
+  const v3 e = mk(fmaxf(fabsf(q.x) - hb.x - m, 0.f), fmaxf(fabsf(q.y) - hb.y - m, 0.f),
+                  fmaxf(fabsf(q.z) - hb.z - m, 0.f));
+  return sqrtf(dot(e, e)) < R + eps;
 }
 
-// block-wide exclusive rank of `flag` in thread order and the block total
-__device__ __forceinline__ int block_rank(bool flag, int* wave_tot, int& total) {
+// block-wide exclusive prefix of v in thread order and the block total
+__device__ __forceinline__ int block_scan(int v, int* wave_tot, int& total) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const unsigned long long m = __ballot(flag);
-  const int in_wave = __popcll(m & ((1ull << lane) - 1ull));
-  if (lane == 0) wave_tot[wave] = __popcll(m);
+  int x = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int y = __shfl_up(x, off, 64);
+    if (lane >= off) x += y;
+  }
+  if (lane == 63) wave_tot[wave] = x;
   __syncthreads();
   int before = 0;
   total = 0;
@@ -209,24 +232,72 @@ __device__ __forceinline__ int block_rank(bool flag, int* wave_tot, int& total) 
     total += wave_tot[k];
   }
   __syncthreads();
-  return before + in_wave;
+  return before + x - v;
 }
 
-__global__ void __launch_bounds__(kColThreads) collide_count_kernel(ofb_collider_desc d, const int32_t* geo,
-                                                                    const int32_t* body, const int32_t* flat_id,
-                                                                    const float* xform, const float* size,
-                                                                    const int32_t* pairs, const int32_t* pair_off,
-                                                                    const int32_t* hf_spheres, const float* hgt,
-                                                                    int cand, const float* body_q,
+struct ColArgs {
+  ofb_collider_desc d;
+  const int32_t *geo, *body, *flat_id, *pairs, *pair_off, *hf_spheres;
+  const float *xform, *size, *hgt;
+};
+
+constexpr int kMaxCylSamples = 256;
+constexpr int kMaxColPairs = 8192;  // candidate-range table in LDS: (pairs + 1) ints
+
+// Per-world set-up shared by both passes: the rim angle table and the compacted candidate
+// ranges of the pairs that may touch (poff[p] .. poff[p+1]); returns their total.
+__device__ int world_setup(const ColArgs& A, const ColCtx& C, int* poff, float* ang_cs, float* ang_sn,
+                           int* wave_tot) {
+  const int P = A.d.num_pairs, k = A.d.cylinder_samples;
+  for (int j = threadIdx.x; j < k; j += kColThreads) {
+    const float ang = (float)j * (float)(6.283185307179586 / (double)k);
+    ang_cs[j] = cosf(ang);
+    ang_sn[j] = sinf(ang);
+  }
+  int run = 0;
+  for (int p0 = 0; p0 < P; p0 += kColThreads) {
+    const int p = p0 + threadIdx.x;
+    int n = 0;
+    if (p < P) {
+      n = A.pair_off[p + 1] - A.pair_off[p];
+      if (!pair_may_touch(C, A.pairs[2 * p], A.pairs[2 * p + 1])) n = 0;
+    }
+    int tot;
+    const int ex = block_scan(n, wave_tot, tot);
+    if (p < P) poff[p] = run + ex;
+    run += tot;
+  }
+  if (threadIdx.x == 0) poff[P] = run;
+  __syncthreads();
+  return run;
+}
+
+__device__ __forceinline__ Cand eval_candidate(const ColArgs& A, const ColCtx& C, const int* poff, int active,
+                                               const float* ang_cs, const float* ang_sn, int c) {
+  if (c >= active) return eval_heightfield(C, A.hgt, A.hf_spheres[c - active]);
+  int lo = 0, hi = A.d.num_pairs;  // largest p with poff[p] <= c (then c < poff[p + 1])
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (poff[mid] <= c) lo = mid;
+    else hi = mid;
+  }
+  return eval_pair_point(C, A.pairs[2 * lo], A.pairs[2 * lo + 1], c - poff[lo], ang_cs, ang_sn);
+}
+
+__global__ void __launch_bounds__(kColThreads) collide_count_kernel(ColArgs A, const float* body_q,
                                                                     int32_t* world_count) {
+  extern __shared__ int poff[];
   __shared__ int wave_tot[kColThreads / 64];
-  const ColCtx C{d, geo, body, flat_id, xform, size, body_q, (int)blockIdx.x};
+  __shared__ float ang_cs[kMaxCylSamples], ang_sn[kMaxCylSamples];
+  const ColCtx C{A.d, A.geo, A.body, A.flat_id, A.xform, A.size, body_q, (int)blockIdx.x};
+  const int active = world_setup(A, C, poff, ang_cs, ang_sn, wave_tot);
+  const int cand = active + A.d.num_hf_spheres;
   int count = 0;
   for (int c0 = 0; c0 < cand; c0 += kColThreads) {
     const int c = c0 + threadIdx.x;
-    const bool v = c < cand && eval_candidate(C, pairs, pair_off, hf_spheres, hgt, c).valid;
+    const bool v = c < cand && eval_candidate(A, C, poff, active, ang_cs, ang_sn, c).valid;
     int tot;
-    block_rank(v, wave_tot, tot);
+    block_scan(v ? 1 : 0, wave_tot, tot);
     count += tot;
   }
   if (threadIdx.x == 0) world_count[blockIdx.x] = count;
@@ -258,24 +329,23 @@ __global__ void __launch_bounds__(1024) collide_scan_kernel(int W, const int32_t
   }
 }
 
-__global__ void __launch_bounds__(kColThreads) collide_write_kernel(ofb_collider_desc d, const int32_t* geo,
-                                                                    const int32_t* body, const int32_t* flat_id,
-                                                                    const float* xform, const float* size,
-                                                                    const int32_t* pairs, const int32_t* pair_off,
-                                                                    const int32_t* hf_spheres, const float* hgt,
-                                                                    int cand, const float* body_q,
+__global__ void __launch_bounds__(kColThreads) collide_write_kernel(ColArgs A, const float* body_q,
                                                                     const int32_t* offset, int cap,
                                                                     ofb_contact_buffers out) {
+  extern __shared__ int poff[];
   __shared__ int wave_tot[kColThreads / 64];
-  const ColCtx C{d, geo, body, flat_id, xform, size, body_q, (int)blockIdx.x};
+  __shared__ float ang_cs[kMaxCylSamples], ang_sn[kMaxCylSamples];
+  const ColCtx C{A.d, A.geo, A.body, A.flat_id, A.xform, A.size, body_q, (int)blockIdx.x};
+  const int active = world_setup(A, C, poff, ang_cs, ang_sn, wave_tot);
+  const int cand = active + A.d.num_hf_spheres;
   int base = offset[blockIdx.x];
   for (int c0 = 0; c0 < cand; c0 += kColThreads) {
     const int c = c0 + threadIdx.x;
     Cand k;
     k.valid = false;
-    if (c < cand) k = eval_candidate(C, pairs, pair_off, hf_spheres, hgt, c);
+    if (c < cand) k = eval_candidate(A, C, poff, active, ang_cs, ang_sn, c);
     int tot;
-    const int r = block_rank(k.valid, wave_tot, tot);
+    const int r = block_scan(k.valid ? 1 : 0, wave_tot, tot);
     const int pos = base + r;
     if (k.valid && pos < cap) {
       out.point0[pos * 3 + 0] = k.p0.x; out.point0[pos * 3 + 1] = k.p0.y; out.point0[pos * 3 + 2] = k.p0.z;
@@ -323,6 +393,10 @@ int ofb_collider_create(const ofb_collider_desc* d, ofb_collider** out) {
     ofb::set_error("invalid collider description");
     return -2;
   }
+  if (d->cylinder_samples > kMaxCylSamples || P > kMaxColPairs) {
+    ofb::set_error("collider limits: cylinder_samples <= %d, num_pairs <= %d", kMaxCylSamples, kMaxColPairs);
+    return -2;
+  }
   if (d->num_hf_spheres > 0 && (d->hf_col < 0 || d->hf_col >= SC || d->hf_nx < 2 || d->hf_ny < 2 ||
                                 !d->hf_heights || !d->hf_spheres)) {
     ofb::set_error("invalid heightfield description");
@@ -346,7 +420,6 @@ int ofb_collider_create(const ofb_collider_desc* d, ofb_collider** out) {
   // the kernels read the device copies below; drop the caller's host pointers
   c->d.pairs = c->d.shape_geo = c->d.shape_body = c->d.flat_shape_id = c->d.hf_spheres = nullptr;
   c->d.shape_xform = c->d.shape_size = c->d.hf_heights = nullptr;
-  c->cand = off[P] + (d->num_hf_spheres > 0 ? d->num_hf_spheres : 0);
   const size_t WS = (size_t)W * SC;
   int rc = 0;
   rc |= up(&c->pairs, d->pairs, (size_t)2 * P);
@@ -384,14 +457,14 @@ int ofb_collide(ofb_collider* c, const float* body_q, const ofb_contact_buffers*
     return -2;
   }
   hipStream_t st = (hipStream_t)stream;
-  hipLaunchKernelGGL(collide_count_kernel, dim3(W), dim3(kColThreads), 0, st, c->d, c->geo, c->body, c->flat_id,
-                     c->xform, c->size, c->pairs, c->pair_off, c->hf_spheres, c->hf_heights, c->cand, body_q,
-                     c->world_count);
+  const ColArgs A{c->d, c->geo, c->body, c->flat_id, c->pairs, c->pair_off, c->hf_spheres, c->xform, c->size,
+                  c->hf_heights};
+  const size_t lds = (size_t)(c->d.num_pairs + 1) * sizeof(int);
+  hipLaunchKernelGGL(collide_count_kernel, dim3(W), dim3(kColThreads), lds, st, A, body_q, c->world_count);
   hipLaunchKernelGGL(collide_scan_kernel, dim3(1), dim3(1024), 0, st, W, c->world_count, c->world_offset, cap,
                      out->count);
-  hipLaunchKernelGGL(collide_write_kernel, dim3(W), dim3(kColThreads), 0, st, c->d, c->geo, c->body, c->flat_id,
-                     c->xform, c->size, c->pairs, c->pair_off, c->hf_spheres, c->hf_heights, c->cand, body_q,
-                     c->world_offset, cap, *out);
+  hipLaunchKernelGGL(collide_write_kernel, dim3(W), dim3(kColThreads), lds, st, A, body_q, c->world_offset, cap,
+                     *out);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     ofb::set_error("collider launch failed: %s", hipGetErrorString(e));
