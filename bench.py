@@ -73,6 +73,14 @@ NET_PATH = os.path.join(ROOT, "ostrich_amd", "data", "warm_start_net.npy")
 GRAPH_PATH = "fused:jacobi:prev:graph"
 
 
+def contacts_of(collider, body_q, device):
+    """Contacts of body_q: the HIP collider writes its persistent buffers in place; the
+    host-side PrimitiveCollider returns fresh tensors."""
+    if isinstance(collider, collide.HipCollider):
+        return collider.collide(body_q)
+    return Contacts.from_dict(collider.collide(body_q), device)
+
+
 def run_path(variant, model, state, control, steps, warmup, device, collider, K=K_CONTACTS, tune=False):
     """One solver variant "path:preconditioner[:seed[:option...]]" over warmup + steps
     closed-loop steps.  Options: "unpaired" (worlds of at most 32 rows one per wavefront),
@@ -103,18 +111,18 @@ def run_path(variant, model, state, control, steps, warmup, device, collider, K=
     base_rows = model_rows(model)
     launch = None
     if tune and path == "fused":
-        con0 = Contacts.from_dict(collider.collide(s_a.body_q), device)
+        con0 = contacts_of(collider, s_a.body_q, device)
         launch, times = eng.autotune(s_a, ctl, con0, DT, pairing=paired)
         launch = dict(waves_per_block=launch[0], pair_small_worlds=launch[1],
                       candidates_ms={f"{w}x{'paired' if p else 'single'}": round(t, 4) for (w, p), t in times.items()})
         print(f"[bench] autotune {variant}: {launch}", file=sys.stderr, flush=True)
     g, con_static = None, None
     if graph:
-        con_static = Contacts.from_dict(collider.collide(s_a.body_q), device)
+        con_static = contacts_of(collider, s_a.body_q, device).clone()
         g = eng.capture(s_a, s_b, ctl, con_static, DT)
     step_ms, solve_ms, kern_ms, nr, pcr, conv, ncon, lrn, pbytes, pops = ([] for _ in range(10))
     for k in range(warmup + steps):
-        con = Contacts.from_dict(collider.collide(s_a.body_q), device)
+        con = contacts_of(collider, s_a.body_q, device)
         if graph:
             con_static.copy_(con)
         e0 = torch.cuda.Event(enable_timing=True)
@@ -166,6 +174,43 @@ def run_path(variant, model, state, control, steps, warmup, device, collider, K=
     return out
 
 
+def full_step_leg(model, state, control, device, steps, warmup, launch=None):
+    """Whole simulation steps: the HIP collider, the solver step and the state advance
+    recorded in one graph and replayed `steps` times back to back (no host work between
+    steps).  Also times the collider alone (HIP events over 50 calls)."""
+    from ostrich_amd import OstrichEngine
+    cfg = solver_config("fused", seed="prev")
+    eng = OstrichEngine(model, config=cfg, device=device)
+    if launch is not None:
+        eng.set_fused_launch(launch["waves_per_block"], launch["pair_small_worlds"])
+    col = collide.HipCollider(model, device=device, gap=0.02, max_contacts_per_world=K_CONTACTS)
+    s_a = State.from_numpy(state, device)
+    s_b = s_a.clone()
+    ctl = Control.from_numpy(control, device)
+    stream = torch.cuda.current_stream(device)
+    g = eng.capture(s_a, s_b, ctl, None, DT, collider=col, advance=True)
+    for _ in range(warmup):
+        g.replay()
+    e0, e1, e2, e3 = (torch.cuda.Event(enable_timing=True) for _ in range(4))
+    e0.record(stream)
+    for _ in range(steps):
+        g.replay()
+    e1.record(stream)
+    e1.synchronize()
+    ms = e0.elapsed_time(e1)
+    st = eng.stats()
+    e2.record(stream)
+    for _ in range(50):
+        col.collide(s_a.body_q)
+    e3.record(stream)
+    e3.synchronize()
+    W = model.num_worlds
+    return dict(world_steps_per_s=W * steps / (ms / 1e3), ms_per_step=ms / steps, steps=steps, warmup=warmup,
+                collide_ms=e2.elapsed_time(e3) / 50, contacts=int(col.contacts.rigid_contact_count.item()),
+                nr_mean_last_step=float(st["nr_iters"].float().mean().item()),
+                note="collide (ofb_collide) + solver step + state advance per graph replay, skid-steer scene")
+
+
 def model_rows(model):
     """Joint and control rows of one world (every world shares the topology)."""
     from ostrich_amd.model import JOINT_BALL, JOINT_FIXED, JOINT_PRISMATIC, JOINT_REVOLUTE
@@ -183,7 +228,7 @@ def scene_leg(name, model_fn, worlds, K, steps, warmup, device, variants=("fused
     """Another workload through the same solver paths (a parity-test configuration, not
     the headline): world-steps/s, Newton counts and contact counts per variant."""
     model, state, control = model_fn(worlds)
-    collider = collide.PrimitiveCollider(model, device=device, gap=0.02, max_contacts_per_world=K)
+    collider = collide.HipCollider(model, device=device, gap=0.02, max_contacts_per_world=K)
     out = dict(worlds=worlds, contact_slots=K, steps=steps, warmup=warmup)
     for v in variants:
         print(f"[bench] {name} path={v} worlds={worlds}", file=sys.stderr, flush=True)
@@ -202,8 +247,8 @@ def adjoint_leg(device, worlds=1024, steps=50, checkpoint_every=10, seed=3):
     model, state, control = scenes.skid_steer(worlds, seed=seed, max_obstacles=0, mu_range=(0.8, 0.8))
     cfg = solver_config("fused")
     eng = OstrichEngine(model, config=cfg, device=device)
-    col = collide.PrimitiveCollider(model, device=device, gap=0.02, max_contacts_per_world=K_CONTACTS)
-    collide_fn = lambda q: Contacts.from_dict(col.collide(q), device)
+    col = collide.HipCollider(model, device=device, gap=0.02, max_contacts_per_world=K_CONTACTS)
+    collide_fn = lambda q: col.collide(q).clone()  # each step record keeps its own contacts
     rng = np.random.default_rng(seed)
     D, B = model.dofs_per_world, model.bodies_per_world
     controls = [Control(torch.as_tensor(control["joint_target_pos"], device=device),
@@ -214,21 +259,29 @@ def adjoint_leg(device, worlds=1024, steps=50, checkpoint_every=10, seed=3):
                                        rng.uniform(-np.pi / 4, np.pi / 4, worlds)], 1), dtype=torch.float32,
                              device=device)
     stream = torch.cuda.current_stream(device)
+    def loss_grad(fin):
+        """Loss on the final chassis x, y, yaw and its gradient w.r.t. body_q [W*B, 7]."""
+        q = fin.body_q.reshape(worlds, B, 7)[:, 0].detach().clone().requires_grad_(True)
+        yaw = torch.atan2(2 * (q[:, 6] * q[:, 5] + q[:, 3] * q[:, 4]), 1 - 2 * (q[:, 4] ** 2 + q[:, 5] ** 2))
+        loss = ((q[:, 0] - target[:, 0]) ** 2 + (q[:, 1] - target[:, 1]) ** 2 + (yaw - target[:, 2]) ** 2).sum()
+        loss.backward()
+        gq = torch.zeros_like(fin.body_q).reshape(worlds, B, 7)
+        gq[:, 0] = q.grad
+        return loss, gq.reshape(-1, 7)
+
     ro = CheckpointedRollout(eng, collide_fn, DT, checkpoint_every=checkpoint_every)
-    ro.forward(s0, controls[:2])  # warm-up of both directions
-    ro.backward(torch.zeros_like(s0.body_q), torch.zeros_like(s0.body_qd))
+    # warm-up over the whole rollout and the loss: the first launch of each kernel
+    # instance loads its code object and torch's first autograd call starts its engine,
+    # neither of which is rollout time
+    ro.backward(loss_grad(ro.forward(s0, controls))[1], torch.zeros_like(s0.body_qd))
+    torch.cuda.synchronize()
     e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
     ro = CheckpointedRollout(eng, collide_fn, DT, checkpoint_every=checkpoint_every)
     e0.record(stream)
     fin = ro.forward(s0, controls)
     e1.record(stream)
-    q = fin.body_q.reshape(worlds, B, 7)[:, 0].detach().clone().requires_grad_(True)
-    yaw = torch.atan2(2 * (q[:, 6] * q[:, 5] + q[:, 3] * q[:, 4]), 1 - 2 * (q[:, 4] ** 2 + q[:, 5] ** 2))
-    loss = ((q[:, 0] - target[:, 0]) ** 2 + (q[:, 1] - target[:, 1]) ** 2 + (yaw - target[:, 2]) ** 2).sum()
-    loss.backward()
-    gq = torch.zeros_like(fin.body_q).reshape(worlds, B, 7)
-    gq[:, 0] = q.grad
-    g = ro.backward(gq.reshape(-1, 7), torch.zeros_like(fin.body_qd))
+    loss, gq = loss_grad(fin)
+    g = ro.backward(gq, torch.zeros_like(fin.body_qd))
     e2.record(stream)
     e2.synchronize()
     fwd, bwd = e0.elapsed_time(e1), e1.elapsed_time(e2)
@@ -242,7 +295,7 @@ def adjoint_leg(device, worlds=1024, steps=50, checkpoint_every=10, seed=3):
         adj.append(eng.last_solve_ms())
         return out
     eng.step_backward = timed_backward
-    ro.backward(gq.reshape(-1, 7), torch.zeros_like(fin.body_qd))
+    ro.backward(gq, torch.zeros_like(fin.body_qd))
     eng.step_backward = inner
     return dict(worlds=worlds, steps=steps, checkpoint_every=checkpoint_every, forward_ms=fwd, backward_ms=bwd,
                 backward_ms_per_step=bwd / steps, adjoint_solve_ms_per_step=float(np.mean(adj)),
@@ -307,7 +360,7 @@ def main():
     device = "cuda:0"
     W = args.worlds
     model, state, control = scenes.skid_steer(W, seed=1)
-    collider = collide.PrimitiveCollider(model, device=device, gap=0.02, max_contacts_per_world=K_CONTACTS)
+    collider = collide.HipCollider(model, device=device, gap=0.02, max_contacts_per_world=K_CONTACTS)
     paths = [p for p in args.paths.split(",") if p]
     results = {}
     for p in paths:
@@ -358,7 +411,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded skid-steer scene, primitive collider)",
+        "data": "synthetic (seeded skid-steer scene, contacts from the HIP primitive collider)",
         "config": {"workload": "uneven-batch skid-steer, 4096 worlds, dt 0.05, NR<=20 to ||h||<=1e-5, "
                                "PCR<=16 (Jacobi, the library default), 128 contact slots, Newton seeded by "
                                "the previous step's contacts", "worlds": W,
@@ -375,6 +428,10 @@ def main():
     fw = results.get(main_path, {})
     if "world_steps_per_s" in fw and "world_steps_per_s" in pj:
         out["fused_warm_over_plain_cold"] = fw["world_steps_per_s"] / pj["world_steps_per_s"]
+    print("[bench] full simulation step (collide + solve, one graph)", file=sys.stderr, flush=True)
+    out["full_step"] = full_step_leg(model, state, control, device, args.steps, args.warmup,
+                                     launch=r.get("launch", re_.get("launch")))
+    print(f"[bench] full_step: {out['full_step']}", file=sys.stderr, flush=True)
     if not args.no_scenes:
         # tracked flipper robots on a Gaussian-bump heightfield (1024 worlds, anisotropic
         # track friction, position-controlled flippers) and 3-cube box stacks (16 worlds)

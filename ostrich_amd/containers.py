@@ -41,6 +41,10 @@ class Control:
                          for k in ("joint_target_pos", "joint_target_vel")))
 
 
+_CONTACT_FIELDS = ("rigid_contact_count", "rigid_contact_point0", "rigid_contact_point1", "rigid_contact_normal",
+                   "rigid_contact_shape0", "rigid_contact_shape1", "rigid_contact_margin0", "rigid_contact_margin1")
+
+
 @dataclass
 class Contacts:
     rigid_contact_count: torch.Tensor   # [1] int32
@@ -65,11 +69,13 @@ class Contacts:
                         t(c["shape0"], torch.int32).reshape(-1), t(c["shape1"], torch.int32).reshape(-1),
                         t(c["margin0"], torch.float32).reshape(-1), t(c["margin1"], torch.float32).reshape(-1))
 
+    def clone(self):
+        return Contacts(*(getattr(self, f).clone() for f in _CONTACT_FIELDS))
+
     def copy_(self, other):
         """Copies another contact set of the same capacity into these tensors (in place,
         so a step captured over them reads the new contacts on replay)."""
-        for f in ("rigid_contact_count", "rigid_contact_point0", "rigid_contact_point1", "rigid_contact_normal",
-                  "rigid_contact_shape0", "rigid_contact_shape1", "rigid_contact_margin0", "rigid_contact_margin1"):
+        for f in _CONTACT_FIELDS:
             getattr(self, f).copy_(getattr(other, f))
         return self
 

@@ -7,6 +7,7 @@ per-step diagnostics.  All arithmetic runs in the gfx950 library behind the C AB
 (include/ostrich_fb.h); this class only moves pointers.
 """
 import ctypes as C
+from typing import Optional
 
 import torch
 
@@ -231,7 +232,8 @@ class OstrichEngine:
         self.set_fused_launch(*choice)
         return choice, times
 
-    def capture(self, state_in: State, state_out: State, control: Control, contacts: Contacts, dt: float):
+    def capture(self, state_in: State, state_out: State, control: Control, contacts: Optional[Contacts], dt: float,
+                collider=None, advance: bool = False):
         """Records one full step (contact packing, warm start, the fused launches, the
         warm-start snapshot) over these exact tensors into a HIP graph and returns it
         (a torch.cuda.CUDAGraph): `graph.replay()` runs a step that reads the tensors'
@@ -239,14 +241,26 @@ class OstrichEngine:
         Contacts.copy_) between replays.  The step allocates nothing and launches only
         kernels (no memset or host synchronisation), so it records as is; no timing events
         are recorded inside the graph.  Fused path only: the plain path reads its stop flag
-        on the host every Newton iteration."""
+        on the host every Newton iteration.
+
+        With `collider` (a HipCollider) the graph first generates the contacts of
+        state_in.body_q into the collider's own buffers and steps with those (`contacts`
+        is then ignored); with `advance` it ends by copying state_out's poses and velocities into state_in, so
+        consecutive replays run consecutive simulation steps with no host work between."""
         if self.config.execution.path != "fused":
             raise ValueError("graph capture applies to the fused path")
+        if collider is None and contacts is None:
+            raise ValueError("capture needs contacts or a collider")
         s = torch.cuda.Stream(device=self.device)
         s.wait_stream(torch.cuda.current_stream(self.device))
         g = torch.cuda.CUDAGraph()
         with torch.cuda.graph(g, stream=s):
+            if collider is not None:
+                contacts = collider.collide(state_in.body_q)
             self.step(state_in, state_out, control, contacts, dt)
+            if advance:
+                state_in.body_q.copy_(state_out.body_q)
+                state_in.body_qd.copy_(state_out.body_qd)
         torch.cuda.current_stream(self.device).wait_stream(s)
         self.timestep -= 1  # recording is not a step
         return g

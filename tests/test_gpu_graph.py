@@ -69,3 +69,25 @@ def test_autotune_changes_only_the_launch_shape(gpu_device):
     torch.cuda.synchronize()
     assert torch.equal(out_t.body_q, out_p.body_q) and torch.equal(out_t.body_qd, out_p.body_qd)
     assert tuned.last_kernel_ms() > 0.0
+
+
+def test_captured_collide_step_advance_runs_consecutive_steps(gpu_device):
+    """collide + step + state advance recorded in one graph: N replays equal N eager
+    simulation steps (same kernels) bit for bit, with no host work between replays."""
+    model, s0, ctl, _, mk = _setup(128, gpu_device)
+    col_e = collide.HipCollider(model, device=gpu_device, max_contacts_per_world=128)
+    col_g = collide.HipCollider(model, device=gpu_device, max_contacts_per_world=128)
+    eager, graphed = mk(), mk()
+    a_in, a_out = s0.clone(), s0.clone()
+    b_in, b_out = s0.clone(), s0.clone()
+    g = graphed.capture(b_in, b_out, ctl, None, 0.05, collider=col_g, advance=True)
+    for k in range(6):
+        eager.step(a_in, a_out, ctl, col_e.collide(a_in.body_q), 0.05)
+        a_in.body_q.copy_(a_out.body_q)
+        a_in.body_qd.copy_(a_out.body_qd)
+        g.replay()
+    torch.cuda.synchronize()
+    print(f"[graph] 6 collide+step replays, {int(col_g.contacts.rigid_contact_count.item())} contacts", flush=True)
+    assert torch.equal(a_in.body_q, b_in.body_q) and torch.equal(a_in.body_qd, b_in.body_qd)
+    assert torch.equal(eager.constraint_impulses(), graphed.constraint_impulses())
+    assert torch.equal(col_e.contacts.rigid_contact_count, col_g.contacts.rigid_contact_count)
