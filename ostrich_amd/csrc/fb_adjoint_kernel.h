@@ -19,6 +19,13 @@
 //      each of the 7 pose coordinates of every body for q+ and q-, so every row type is
 //      covered; then the explicit carry dq+/dq- (body_pose_prev_grad_kernel,
 //      residual.py:260-339) of g_q + w^T dR/dq+.
+//
+// The central differences are most of the kernel's work (2 x 14 B residual evaluations
+// per world).  With P.split > 1 a workgroup holds one world in `split` wavefronts: wave 0
+// runs steps 1-4 and publishes lambda and w_lambda, every wave copies wave 0's body
+// block and lambda_prev into its own LDS region and evaluates every split-th pose
+// coordinate.  Each coordinate is computed by the same code on the same data, so the
+// result does not depend on the split.
 
 #pragma once
 
@@ -65,6 +72,10 @@ struct AdjParams {
   const int32_t* wlist;
   const int32_t* wcount;
   int rows_cap;
+  // waves per world (1..8): wave 0 of a world runs the solve; with more, every wave
+  // takes a share of the pose central differences on its own LDS copy of the world
+  int split;
+  int xfer_off;  // LDS offset (floats, wave 0's region) of lambda / w_lambda for the others
 };
 
 // LDS floats of the adjoint's own scratch per wave: pose pull-back [14 B], control
@@ -105,20 +116,28 @@ __global__ void __launch_bounds__(256) adjoint_step_kernel(AdjParams P) {
   extern __shared__ float lds[];
   const int lane = threadIdx.x & (WAVE - 1);
   const int wave = threadIdx.x / WAVE;
-  const int idx = blockIdx.x * (blockDim.x / WAVE) + wave;
-  if (idx >= *P.wcount) return;
+  const int split = P.split;  // > 1 only with one world per workgroup
+  const int sub = wave % split;
+  const int idx = blockIdx.x * (blockDim.x / WAVE / split) + wave / split;
+  if (idx >= *P.wcount) return;  // uniform over a world's waves
   const int w = P.wlist[idx];
   float* sm = lds + wave * P.lds_per_wave;
+  float* sm0 = lds + (wave - sub) * P.lds_per_wave;  // wave 0 of this world
   const int B = P.m.B, D = P.m.D, SC = P.m.SC;
   const RowLayout& L = P.L;
   const float dt = P.dt;
   float* lamprev = sm + B * BS;
-  float* gpose = lamprev + L.Nc + 1;  // [B][14]: w^T dR/dq+ (7), w^T dR/dq- (7)
-  float* gctl = gpose + 14 * B;       // [4][D]
-  float* gmu = gctl + 4 * D;          // [SC]
+  float* gpose = sm0 + B * BS + L.Nc + 1;  // [B][14]: w^T dR/dq+ (7), w^T dR/dq- (7)
+  float* gctl = gpose + 14 * B;            // [4][D]
+  float* gmu = gctl + 4 * D;               // [SC]
   const BJView V = bj_view(gmu + SC, P.rows_cap);
   BJRegs<R> G;
   int flags = 0;
+  const int n_c = min(P.ccount[w], L.K);
+  const int n_r = L.Nj + L.Nctrl + 3 * n_c;
+  float lam[R], x[R];
+  int pcr_it = 0;
+  if (sub == 0) {
 
   // ---- state: q+ / u+ in the iterate slots, q- / u- in the step-start slots ----
   for (int b = lane; b < B; b += WAVE) {
@@ -140,9 +159,7 @@ __global__ void __launch_bounds__(256) adjoint_step_kernel(AdjParams P) {
     for (int i = 0; i < 9; ++i) { bb[BIB + i] = P.m.body_inertia[k * 9 + i]; bb[BIIB + i] = P.m.body_inv_inertia[k * 9 + i]; }
   }
   for (int e = lane; e < 14 * B + 4 * D + SC; e += WAVE) gpose[e] = 0.f;
-  const int n_c = min(P.ccount[w], L.K);
-  const int n_r = L.Nj + L.Nctrl + 3 * n_c;
-  float lam[R], x[R], Cr[R], Pi[R], rhs[R];
+  float Cr[R], Pi[R], rhs[R];
   sv J0[R], J1[R];
   int b0[R], b1[R];
   bool act[R];
@@ -227,8 +244,8 @@ __global__ void __launch_bounds__(256) adjoint_step_kernel(AdjParams P) {
   }
   const float tol_sq = P.cfg.lin_tol >= 0.f ? P.cfg.lin_tol * P.cfg.lin_tol : -1.f;
   const float latol_sq = P.cfg.lin_atol >= 0.f ? P.cfg.lin_atol * P.cfg.lin_atol : -1.f;
-  const int pcr_it = pcr_solve<R, BJ>(sm, lane, B, V, G, act, b0, b1, J0, J1, Cr, Pi, rhs, bsq, tol_sq, latol_sq,
-                                      P.cfg.lin_max_iters, x);
+  pcr_it = pcr_solve<R, BJ>(sm, lane, B, V, G, act, b0, b1, J0, J1, Cr, Pi, rhs, bsq, tol_sq, latol_sq,
+                            P.cfg.lin_max_iters, x);
 
   // ---- 4. w_u = w_u0 - M^-1 J^T w_lambda ----
   for (int e = lane; e < 6 * B; e += WAVE) sm[(e / 6) * BS + BACC + (e % 6)] = 0.f;
@@ -305,10 +322,34 @@ __global__ void __launch_bounds__(256) adjoint_step_kernel(AdjParams P) {
   }
   wsync();
 
+  if (split > 1) {  // publish lambda and w_lambda for the other waves of this world
+    float* xl = sm + P.xfer_off;
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      int i = lane + WAVE * k;
+      if (i < n_r) { xl[i] = lam[k]; xl[P.rows_cap + i] = x[k]; }
+    }
+  }
+  }  // sub == 0
+  if (split > 1) {
+    __syncthreads();
+    if (sub > 0) {
+      for (int e = lane; e < B * BS + L.Nc + 1; e += WAVE) sm[e] = sm0[e];
+      const float* xl = sm0 + P.xfer_off;
+#pragma unroll
+      for (int k = 0; k < R; ++k) {
+        int i = lane + WAVE * k;
+        lam[k] = i < n_r ? xl[i] : 0.f;
+        x[k] = i < n_r ? xl[P.rows_cap + i] : 0.f;
+      }
+      wsync();
+    }
+  }
+
   // ---- 5b. w^T dR/dq+ and w^T dR/dq- by central differences ----
   if (P.cfg.adjoint_pose_vjp) {
     const float h = P.cfg.adjoint_pose_eps;
-    for (int d = 0; d < 14 * B; ++d) {
+    for (int d = sub; d < 14 * B; d += split) {
       const int b = d / 14, kk = d % 14;
       float* ptr = sm + b * BS + (kk < 7 ? BQ + kk : BQP + kk - 7);
       const float orig = *ptr;
@@ -322,6 +363,11 @@ __global__ void __launch_bounds__(256) adjoint_step_kernel(AdjParams P) {
       if (lane == 0) { *ptr = orig; gpose[d] = (sp - sn) / (2.f * h); }
       wsync();
     }
+  }
+
+  if (split > 1) {
+    __syncthreads();
+    if (sub > 0) return;
   }
 
   // ---- 5c. carry dq+/dq- of g_q + w^T dR/dq+ (residual.py:260-339), outputs ----

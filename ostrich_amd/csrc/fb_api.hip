@@ -408,6 +408,13 @@ int ofb_create(const ofb_model* model, const ofb_config* config, int32_t path, o
     return -4;
   }
   s->fused_worlds_per_block = 4;
+  {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+      cus = 256;
+    s->num_simds = 4 * cus;
+  }
   if (hipEventCreate(&s->ev_solve0) != hipSuccess || hipEventCreate(&s->ev_solve1) != hipSuccess ||
       hipEventCreate(&s->ev_kernel0) != hipSuccess || hipEventCreate(&s->ev_kernel1) != hipSuccess) {
     set_error("hipEventCreate failed");

@@ -296,6 +296,11 @@ int fused_step(ofb_solver* s, const ofb_state_in* in, const ofb_state_out* out, 
   return 0;
 }
 
+// waves per world of the adjoint's pose central differences (at most 4: 256 threads)
+#ifndef OFB_ADJ_MAX_SPLIT
+#define OFB_ADJ_MAX_SPLIT 4
+#endif
+
 int fused_adjoint(ofb_solver* s, const ofb_state_in* in, const ofb_state_out* out, const ofb_control* ctl,
                   const float* lambda, const float* lambda_prev, const ofb_adjoint_in* gin,
                   const ofb_adjoint_out* gout, float dt, hipStream_t stream) {
@@ -334,10 +339,27 @@ int fused_adjoint(ofb_solver* s, const ofb_state_in* in, const ofb_state_out* ou
     P.wcount = s->cls_counts + c;
     P.rows_cap = WAVE * R;
     P.lds_per_wave = adj_lds_words(s, R);
+    P.xfer_off = P.lds_per_wave;
+    P.split = 1;
     int wpb = s->fused_worlds_per_block;
+    // Fewer worlds than resident wave slots (the kernel's registers allow two waves per
+    // SIMD): give each world `split` waves for the pose central differences (one world
+    // per workgroup, each wave its own LDS copy).  1024 worlds on 1024 SIMDs: split 2.
+    if (s->cfg.adjoint_pose_vjp) {
+      int split = 1;
+      while (split < OFB_ADJ_MAX_SPLIT && (size_t)W * split * 2 <= (size_t)2 * s->num_simds) split *= 2;
+      const int words = P.lds_per_wave + 2 * P.rows_cap;
+      while (split > 1 && (size_t)split * words * sizeof(float) > 65536) split /= 2;
+      if (split > 1) {
+        P.split = split;
+        P.lds_per_wave = words;
+        wpb = 1;
+      }
+    }
     while (wpb > 1 && (size_t)wpb * P.lds_per_wave * sizeof(float) > 65536) --wpb;
-    size_t lds_bytes = (size_t)wpb * P.lds_per_wave * sizeof(float);
-    if (adj_class_op(R, 0, P, (W + wpb - 1) / wpb, wpb * WAVE, lds_bytes, stream, bj) != 0) {
+    const int waves = wpb * P.split;
+    size_t lds_bytes = (size_t)waves * P.lds_per_wave * sizeof(float);
+    if (adj_class_op(R, 0, P, (W + wpb - 1) / wpb, waves * WAVE, lds_bytes, stream, bj) != 0) {
       set_error("no adjoint kernel instance for %d row slots per lane", R);
       return -4;
     }

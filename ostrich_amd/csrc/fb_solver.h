@@ -83,6 +83,7 @@ struct ofb_solver {
   // largest number of waves per workgroup of a fused class launch (ofb_set_fused_launch;
   // the Python autotuner picks it)
   int fused_worlds_per_block;
+  int num_simds;  // SIMDs of the device (4 per CU)
   // per-step bucketing of worlds by active-row class (fused path)
   int32_t* cls_counts;
   int32_t* cls_lists;
