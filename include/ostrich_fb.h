@@ -363,6 +363,26 @@ int ofb_trajectory_adam_step(float* params, const float* grad, float* m, float* 
                              int32_t worlds, int32_t dofs, int32_t dof_offset, int32_t num_dofs, float lr,
                              float beta1, float beta2, float eps, float clip_grad, int32_t t, void* stream);
 
+/* Attaches (merit != NULL) or detaches caller device buffers [capacity][W] that each
+ * fused step fills with its per-Newton-iteration history: ||h||^2 after iteration k
+ * (the reference's candidates_res_norm_sq, core/history_group.py buffers logged by
+ * logging/sim_logger.py) and the PCR iterations of iteration k (pcr_history_iter_count);
+ * iterations a world did not run hold NaN and -1.  Fused path only. */
+int ofb_set_iteration_history(ofb_solver* solver, float* merit, int32_t* pcr_iters, int32_t capacity);
+
+/* ---- device-resident step logs (core/history_group.py:12-80, logging/sim_logger.py,
+ * logging/dataset_logger.py) ----------------------------------------------------------
+ * A log copies num_arrays device arrays of words_per_step[a] 4-byte elements into slot
+ * *step_index of their [capacity, words] destination buffers (all device pointers, kept
+ * by the caller); a step index outside [0, capacity) copies nothing.  The index is read
+ * on the device and, with advance, incremented after the copy, so a capture records into
+ * a graph. */
+typedef struct ofb_log ofb_log;
+int ofb_log_create(int32_t num_arrays, const void* const* src, void* const* dst, const int64_t* words_per_step,
+                   int32_t capacity, ofb_log** out);
+int ofb_log_capture(ofb_log* log, int32_t* step_index, int32_t advance, void* stream);
+int ofb_log_destroy(ofb_log* log);
+
 /* ---- primitive collider on the GPU ---------------------------------------------------
  * The contact-generation step before the solver (Newton's collide, called at
  * simulation/base_simulator.py:118 before integrator.simulate).  Newton's narrow phase is

@@ -16,7 +16,8 @@ EXPORTS = (
     "ofb_mlp_get_params", "ofb_mlp_forward", "ofb_mlp_gradient", "ofb_mlp_train_step",
     "ofb_set_warm_start_net", "ofb_get_contact_features", "ofb_get_linesearch_steps",
     "ofb_last_kernel_ms", "ofb_set_fused_launch", "ofb_get_fused_launch", "ofb_trajectory_adam_step",
-    "ofb_collider_create", "ofb_collider_destroy", "ofb_collide",
+    "ofb_collider_create", "ofb_collider_destroy", "ofb_collide", "ofb_set_iteration_history",
+    "ofb_log_create", "ofb_log_capture", "ofb_log_destroy",
 )
 
 _lib = None
@@ -47,6 +48,10 @@ def load():
     lib.ofb_collider_create.argtypes = [vp, C.POINTER(vp)]
     lib.ofb_collider_destroy.argtypes = [vp]
     lib.ofb_collide.argtypes = [vp, vp, vp, vp]
+    lib.ofb_set_iteration_history.argtypes = [vp, vp, vp, i32]
+    lib.ofb_log_create.argtypes = [i32, vp, vp, vp, i32, C.POINTER(vp)]
+    lib.ofb_log_capture.argtypes = [vp, vp, i32, vp]
+    lib.ofb_log_destroy.argtypes = [vp]
     lib.ofb_last_error.restype = C.c_char_p
     lib.ofb_get_lambda_prev.argtypes = [vp, vp, vp]
     lib.ofb_set_shape_mu.argtypes = [vp, vp, vp]

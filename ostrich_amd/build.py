@@ -27,6 +27,7 @@ UNITS = [
     ("fb_reduce", "fb_reduce.hip", []),
     ("fb_trajopt", "fb_trajopt.hip", []),
     ("fb_collide", "fb_collide.hip", []),
+    ("fb_log", "fb_log.hip", []),
     ("fb_fused_j0", "fb_fused_classes.hip", ["-DOFB_BJ=0", "-DOFB_RSET=0"]),
     ("fb_fused_j1", "fb_fused_classes.hip", ["-DOFB_BJ=0", "-DOFB_RSET=1"]),
     ("fb_fused_b0", "fb_fused_classes.hip", ["-DOFB_BJ=1", "-DOFB_RSET=0"]),

@@ -604,6 +604,17 @@ int ofb_set_fused_launch(ofb_solver* s, int32_t waves_per_block, int32_t pair_sm
   return fused_set_launch(s, waves_per_block, pair_small_worlds);
 }
 
+int ofb_set_iteration_history(ofb_solver* s, float* merit, int32_t* pcr_iters, int32_t capacity) {
+  if (!s) { set_error("null solver"); return -1; }
+  if (s->path != OFB_PATH_FUSED) { set_error("the iteration history is written by the fused path"); return -2; }
+  if ((merit == nullptr) != (pcr_iters == nullptr)) { set_error("give both history buffers or neither"); return -1; }
+  if (merit && capacity < 1) { set_error("history capacity must be >= 1"); return -2; }
+  s->hist_merit = merit;
+  s->hist_pcr = pcr_iters;
+  s->hist_cap = merit ? capacity : 0;
+  return 0;
+}
+
 int ofb_get_fused_launch(const ofb_solver* s, int32_t* waves_per_block, int32_t* pair_small_worlds) {
   if (!s) { set_error("null solver"); return -1; }
   if (waves_per_block) *waves_per_block = s->fused_worlds_per_block;

@@ -239,6 +239,7 @@ int fused_step(ofb_solver* s, const ofb_state_in* in, const ofb_state_out* out, 
   P.ls_steps = s->ls_steps;
   P.ls_count = s->cfg.linesearch ? s->ls_count : 0;
   P.st_nr = s->st_nr; P.st_pcr = s->st_pcr; P.st_best = s->st_best; P.st_flags = s->st_flags; P.st_merit = s->st_merit;
+  P.hist_merit = s->hist_merit; P.hist_pcr = s->hist_pcr; P.hist_cap = s->hist_cap;
   const int W = s->m.W;
   const int r_max = (s->L.Nc + WAVE - 1) / WAVE;
   if (r_max > kClassRHost[kNumClasses - 1]) {

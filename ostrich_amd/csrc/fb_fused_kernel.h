@@ -80,6 +80,11 @@ struct FusedParams {
   float* lambda_prev_out;  // lambda_prev_iter as the step leaves it (input of the adjoint)
   int32_t *st_nr, *st_pcr, *st_best, *st_flags;
   float* st_merit;
+  // per-Newton-iteration history [hist_cap][W] (null: off): ||h||^2 after the iteration
+  // (candidates_res_norm_sq) and its PCR iterations (pcr_history_iter_count)
+  float* hist_merit;
+  int32_t* hist_pcr;
+  int hist_cap;
   FusedLayout lay;        // LDS regions of one world (floats)
   const int32_t* wlist;   // worlds of this launch's row class
   const int32_t* wcount;  // number of them
@@ -922,8 +927,9 @@ fused_step_kernel(FusedParams P) {
     OFB_PHASE(2)
     if constexpr (DENSE) dense_build(D, Adense, lane, B, act[0], b0[0], b1[0], J0[0], J1[0], W0[0], W1[0], Cr[0]);
     OFB_PHASE(7)
-    pcr_total += pcr_solve<R, BJ, GW, DENSE>(sm, lane, B, V, G, act, b0, b1, J0, J1, Cr, Pi, rhs, bsq, tol_sq,
-                                             latol_sq, P.cfg.lin_max_iters, x, D, Adense);
+    const int pcr_it = pcr_solve<R, BJ, GW, DENSE>(sm, lane, B, V, G, act, b0, b1, J0, J1, Cr, Pi, rhs, bsq,
+                                                   tol_sq, latol_sq, P.cfg.lin_max_iters, x, D, Adense);
+    pcr_total += pcr_it;
     OFB_PHASE(3)
 
     // ---- velocity recovery + Newton update (linear_system.py:344-377, newton_step.py:22-54) ----
@@ -1045,6 +1051,10 @@ fused_step_kernel(FusedParams P) {
       msum += v * v;
     }
     merit = group_sum<GW>(msum);
+    if (P.hist_merit && lane == 0 && n_it < P.hist_cap) {
+      P.hist_merit[(size_t)n_it * P.m.W + w] = merit;
+      P.hist_pcr[(size_t)n_it * P.m.W + w] = pcr_it;
+    }
     OFB_PHASE(5)
 
     // ---- candidate bookkeeping (history_group.py + backtracking.py:12-95) ----
@@ -1118,6 +1128,11 @@ fused_step_kernel(FusedParams P) {
     P.lambda_prev_out[o + L.off_f + 2 * c] = 0.f;
     P.lambda_prev_out[o + L.off_f + 2 * c + 1] = 0.f;
   }
+  if (P.hist_merit)  // iterations the world did not run
+    for (int k = n_it + lane; k < P.hist_cap; k += GW) {
+      P.hist_merit[(size_t)k * P.m.W + w] = __builtin_nanf("");
+      P.hist_pcr[(size_t)k * P.m.W + w] = -1;
+    }
   int bj_any = 0;
   if (group_any<GW>(bj_flags & FLAG_FACTOR_FAILURE)) bj_any |= FLAG_FACTOR_FAILURE;
   if (group_any<GW>(bj_flags & FLAG_PAIR_OVERFLOW)) bj_any |= FLAG_PAIR_OVERFLOW;

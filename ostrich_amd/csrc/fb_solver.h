@@ -84,6 +84,10 @@ struct ofb_solver {
   // the Python autotuner picks it)
   int fused_worlds_per_block;
   int num_simds;  // SIMDs of the device (4 per CU)
+  // caller buffers of the per-Newton-iteration history (ofb_set_iteration_history)
+  float* hist_merit;
+  int32_t* hist_pcr;
+  int hist_cap;
   // per-step bucketing of worlds by active-row class (fused path)
   int32_t* cls_counts;
   int32_t* cls_lists;

@@ -97,22 +97,42 @@ class OstrichEngine:
         """core/base_engine.py:278-279."""
         self.timestep = 0
 
-    def stats(self):
-        """Per-world diagnostics of the last step (tensors on the device)."""
+    def stats(self, out=None):
+        """Per-world diagnostics of the last step (tensors on the device).  `out` may be a
+        dict of preallocated tensors (nr_iters, pcr_iters, best_iter, flags int32 [W];
+        merit float32 [W]) to fill in place, e.g. inside a captured graph."""
         W = self.num_worlds
-        out = {k: torch.zeros(W, dtype=torch.int32, device=self.device)
-               for k in ("nr_iters", "pcr_iters", "best_iter", "flags")}
-        out["merit"] = torch.zeros(W, dtype=torch.float32, device=self.device)
+        if out is None:
+            out = {k: torch.zeros(W, dtype=torch.int32, device=self.device)
+                   for k in ("nr_iters", "pcr_iters", "best_iter", "flags")}
+            out["merit"] = torch.zeros(W, dtype=torch.float32, device=self.device)
         _lib.check(self.lib.ofb_get_stats(self._h, _ptr(out["nr_iters"]), _ptr(out["pcr_iters"]),
                                           _ptr(out["merit"]), _ptr(out["best_iter"]), _ptr(out["flags"]),
                                           self._stream()), "ofb_get_stats")
         return out
 
-    def constraint_impulses(self):
-        """lambda [W, N_c] of the picked iterate (reference row layout)."""
-        lam = torch.zeros(self.num_worlds, self.num_constraints, dtype=torch.float32, device=self.device)
+    def constraint_impulses(self, out=None):
+        """lambda [W, N_c] of the picked iterate (reference row layout), into `out` if given."""
+        lam = out if out is not None else torch.zeros(self.num_worlds, self.num_constraints, dtype=torch.float32,
+                                                      device=self.device)
         _lib.check(self.lib.ofb_get_lambda(self._h, _ptr(lam), self._stream()), "ofb_get_lambda")
         return lam
+
+    def set_iteration_history(self, merit=None, pcr_iters=None):
+        """Per-Newton-iteration history of every following step into merit (float32) and
+        pcr_iters (int32) device tensors [capacity, W]: ||h||^2 after each iteration and
+        its PCR iterations, NaN / -1 past a world's last iteration.  None detaches."""
+        if merit is None:
+            _lib.check(self.lib.ofb_set_iteration_history(self._h, None, None, 0), "ofb_set_iteration_history")
+            self._hist_keepalive = None
+            return
+        if (merit.dtype != torch.float32 or pcr_iters.dtype != torch.int32 or merit.shape != pcr_iters.shape
+                or merit.dim() != 2 or merit.shape[1] != self.num_worlds or not merit.is_contiguous()
+                or not pcr_iters.is_contiguous()):
+            raise ValueError("history buffers must be contiguous [capacity, W] float32 / int32 tensors")
+        _lib.check(self.lib.ofb_set_iteration_history(self._h, _ptr(merit), _ptr(pcr_iters), merit.shape[0]),
+                   "ofb_set_iteration_history")
+        self._hist_keepalive = (merit, pcr_iters)
 
     def set_warm_start_net(self, net):
         """Attaches a learned warm start (`ostrich_amd.learned.WarmStartNet`) or detaches it
