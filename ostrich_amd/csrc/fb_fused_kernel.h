@@ -52,8 +52,12 @@ __device__ unsigned long long g_phase_cycles[8];
 // per joint, one per contact slot of the class (cs_cap), the block-Jacobi region, the
 // dense Schur operator.  `words` is a multiple of 4, so every world's base stays aligned.
 struct FusedLayout {
-  int lamprev, js, cs, bj, dense, words;
+  int lamprev, js, cs, bj, dense, hist, words;
 };
+// Per-Newton-iteration history of a world (||h||^2 and the running PCR count of the first
+// kHistIters iterations) kept in LDS and written to the caller's buffers once per step.
+constexpr int kHistIters = 32;
+constexpr int kHistWords = 2 * kHistIters;
 __host__ __device__ inline int up4(int x) { return (x + 3) & ~3; }
 
 struct FusedParams {
@@ -713,7 +717,8 @@ __host__ __device__ inline FusedLayout fused_layout(int B, int J, int rows_cap, 
   l.cs = l.js + JS_WORDS * J;
   l.bj = l.cs + CS_WORDS * cs_cap;
   l.dense = up4(l.bj + (bj ? bj_words(rows_cap) : 0));
-  l.words = up4(l.dense + (dense ? dense_words(rows_cap, B) : 0));
+  l.hist = up4(l.dense + (dense ? dense_words(rows_cap, B) : 0));
+  l.words = l.hist + kHistWords;
   return l;
 }
 
@@ -733,9 +738,16 @@ __host__ __device__ inline FusedLayout fused_layout(int B, int J, int rows_cap, 
 #ifndef OFB_DENSE_WAVES_PER_EU
 #define OFB_DENSE_WAVES_PER_EU 2
 #endif
+// The paired instance sits at the register edge of three waves per SIMD (168 VGPRs); at
+// two, its launch no longer fits the chip in one round at 4096 worlds (0.52 -> 0.72 ms).
+#ifndef OFB_PAIR_WAVES_PER_EU
+#define OFB_PAIR_WAVES_PER_EU 3
+#endif
 template <int R, bool BJ, int GW, bool DENSE>
 __global__ void __launch_bounds__(256)
-__attribute__((amdgpu_waves_per_eu(R == 1 ? (DENSE ? OFB_DENSE_WAVES_PER_EU : OFB_R1_WAVES_PER_EU) : 1)))
+__attribute__((amdgpu_waves_per_eu(R == 1 ? (DENSE ? (GW == 32 ? OFB_PAIR_WAVES_PER_EU : OFB_DENSE_WAVES_PER_EU)
+                                                : OFB_R1_WAVES_PER_EU)
+                                         : 1)))
 fused_step_kernel(FusedParams P) {
   extern __shared__ __align__(16) float lds[];
   static_assert(GW == 64 || (GW == 32 && R == 1), "two worlds per wave hold one row slot per lane");
@@ -927,9 +939,8 @@ fused_step_kernel(FusedParams P) {
     OFB_PHASE(2)
     if constexpr (DENSE) dense_build(D, Adense, lane, B, act[0], b0[0], b1[0], J0[0], J1[0], W0[0], W1[0], Cr[0]);
     OFB_PHASE(7)
-    const int pcr_it = pcr_solve<R, BJ, GW, DENSE>(sm, lane, B, V, G, act, b0, b1, J0, J1, Cr, Pi, rhs, bsq,
-                                                   tol_sq, latol_sq, P.cfg.lin_max_iters, x, D, Adense);
-    pcr_total += pcr_it;
+    pcr_total += pcr_solve<R, BJ, GW, DENSE>(sm, lane, B, V, G, act, b0, b1, J0, J1, Cr, Pi, rhs, bsq, tol_sq,
+                                             latol_sq, P.cfg.lin_max_iters, x, D, Adense);
     OFB_PHASE(3)
 
     // ---- velocity recovery + Newton update (linear_system.py:344-377, newton_step.py:22-54) ----
@@ -1051,9 +1062,9 @@ fused_step_kernel(FusedParams P) {
       msum += v * v;
     }
     merit = group_sum<GW>(msum);
-    if (P.hist_merit && lane == 0 && n_it < P.hist_cap) {
-      P.hist_merit[(size_t)n_it * P.m.W + w] = merit;
-      P.hist_pcr[(size_t)n_it * P.m.W + w] = pcr_it;
+    if (lane == 0 && n_it < kHistIters) {  // ||h||^2 and the PCR iterations so far
+      sm[P.lay.hist + n_it] = merit;
+      sm[P.lay.hist + kHistIters + n_it] = __int_as_float(pcr_total);
     }
     OFB_PHASE(5)
 
@@ -1128,11 +1139,16 @@ fused_step_kernel(FusedParams P) {
     P.lambda_prev_out[o + L.off_f + 2 * c] = 0.f;
     P.lambda_prev_out[o + L.off_f + 2 * c + 1] = 0.f;
   }
-  if (P.hist_merit)  // iterations the world did not run
-    for (int k = n_it + lane; k < P.hist_cap; k += GW) {
-      P.hist_merit[(size_t)k * P.m.W + w] = __builtin_nanf("");
-      P.hist_pcr[(size_t)k * P.m.W + w] = -1;
+  if (P.hist_merit) {  // NaN / -1: iterations the world did not run (or past kHistIters)
+    wsync();
+    const float* hs = sm + P.lay.hist;
+    for (int k = lane; k < P.hist_cap; k += GW) {
+      const bool ran = k < n_it && k < kHistIters;
+      const int before = k > 0 ? __float_as_int(hs[kHistIters + k - 1]) : 0;
+      P.hist_merit[(size_t)k * P.m.W + w] = ran ? hs[k] : __builtin_nanf("");
+      P.hist_pcr[(size_t)k * P.m.W + w] = ran ? __float_as_int(hs[kHistIters + k]) - before : -1;
     }
+  }
   int bj_any = 0;
   if (group_any<GW>(bj_flags & FLAG_FACTOR_FAILURE)) bj_any |= FLAG_FACTOR_FAILURE;
   if (group_any<GW>(bj_flags & FLAG_PAIR_OVERFLOW)) bj_any |= FLAG_PAIR_OVERFLOW;
