@@ -738,14 +738,15 @@ __host__ __device__ inline FusedLayout fused_layout(int B, int J, int rows_cap, 
 #ifndef OFB_DENSE_WAVES_PER_EU
 #define OFB_DENSE_WAVES_PER_EU 2
 #endif
-// The paired instance sits at the register edge of three waves per SIMD (168 VGPRs); at
-// two, its launch no longer fits the chip in one round at 4096 worlds (0.52 -> 0.72 ms).
+// The paired Jacobi instance sits at the register edge of three waves per SIMD (168
+// VGPRs); at two, its launch no longer fits the chip in one round at 4096 worlds (0.52 ->
+// 0.72 ms).  The block-Jacobi one would spill heavily at three and keeps two.
 #ifndef OFB_PAIR_WAVES_PER_EU
 #define OFB_PAIR_WAVES_PER_EU 3
 #endif
 template <int R, bool BJ, int GW, bool DENSE>
 __global__ void __launch_bounds__(256)
-__attribute__((amdgpu_waves_per_eu(R == 1 ? (DENSE ? (GW == 32 ? OFB_PAIR_WAVES_PER_EU : OFB_DENSE_WAVES_PER_EU)
+__attribute__((amdgpu_waves_per_eu(R == 1 ? (DENSE ? (GW == 32 && !BJ ? OFB_PAIR_WAVES_PER_EU : OFB_DENSE_WAVES_PER_EU)
                                                 : OFB_R1_WAVES_PER_EU)
                                          : 1)))
 fused_step_kernel(FusedParams P) {
