@@ -76,7 +76,13 @@ struct AdjParams {
   // takes a share of the pose central differences on its own LDS copy of the world
   int split;
   int xfer_off;  // LDS offset (floats, wave 0's region) of lambda / w_lambda for the others
+  int fd_off;    // LDS offset of the row-local difference table (< 0: full-residual differences)
 };
+
+// Words of the row-local difference table: per row slot and side (body 0 / body 1) the 14
+// pose-coordinate differences of the row's term, the two body indices, and per body the
+// 7 q+ differences of its dynamics term.
+__host__ __device__ inline int adj_fd_words(int rows_cap, int B) { return 30 * rows_cap + 7 * B; }
 
 // LDS floats of the adjoint's own scratch per wave: pose pull-back [14 B], control
 // gradients [4 D] (target_pos, target_vel, ke, kd), mu gradients [S + G]
@@ -85,6 +91,27 @@ __host__ __device__ inline int adj_words(int B, int D, int SC) { return 14 * B +
 // The adjoint keeps w_u of each body in the slot of the forward's best-candidate pose
 // (BBQ, unused here): BVV is the Schur operator's scratch.
 constexpr int BWU = BBQ;
+
+// The weighted residual splits into per-row terms, each a function of the row's two bodies
+// only, plus per-body dynamics terms:
+//   s = sum_rows [w_lambda h_c + (-lambda dt)(w_u(b0) . J0 + w_u(b1) . J1)]
+//       + sum_bodies w_u . (M(q+)(u - u-) - dt (f + m g + gyro)),
+// the J^T lambda part of h_d being the rows' scatter.  term of one row at poses read by pr:
+template <class PR>
+__device__ __forceinline__ float adjoint_row_term(const AdjParams& P, float* sm, const float* lamprev, int w, int i,
+                                                  int n_c, float lam, float wl, const PR& pr, int* b0, int* b1) {
+  RowOut o = assemble_row(P, sm, lamprev, w, i, n_c, lam, pr, false);
+  *b0 = o.b0;
+  *b1 = o.b1;
+  float t = wl * o.hc;
+  if (o.active) {
+    float d = 0.f;
+    if (o.b0 >= 0) d += sdot(o.J0, ldsv(sm + o.b0 * BS + BWU));
+    if (o.b1 >= 0) d += sdot(o.J1, ldsv(sm + o.b1 * BS + BWU));
+    t += d * (-lam * P.dt);
+  }
+  return t;
+}
 
 // weighted residual s = sum_rows w_lambda h_c + sum_bodies w_u . h_d at the pose in LDS
 template <int R>
@@ -347,7 +374,58 @@ __global__ void __launch_bounds__(256) adjoint_step_kernel(AdjParams P) {
   }
 
   // ---- 5b. w^T dR/dq+ and w^T dR/dq- by central differences ----
-  if (P.cfg.adjoint_pose_vjp) {
+  if (P.cfg.adjoint_pose_vjp && P.fd_off >= 0) {
+    // row-local: each lane differences its own rows' terms in the 14 coordinates of each
+    // of the row's bodies (the shifted pose is read privately, the LDS blocks stay
+    // unperturbed); wave 0 then sums them per body in row order
+    const float h = P.cfg.adjoint_pose_eps;
+    float* T = sm0 + P.fd_off;
+    int* tb = (int*)(T + 28 * P.rows_cap);
+    float* Td = T + 30 * P.rows_cap;
+    if constexpr (R == 1) {
+#pragma unroll
+      for (int k = 0; k < R; ++k) {
+        const int i = lane + WAVE * k;
+        if (i >= n_r) continue;
+        // item 0 evaluates the row unshifted to learn its bodies; item 1 + 2 (14 side + c)
+        // + sign evaluates it with coordinate c of body `side` shifted by +h / -h.  One
+        // call site, so the row code is inlined once.
+        int rb0 = -1, rb1 = -1;
+        float tp = 0.f;
+#pragma unroll 1
+        for (int it = 0; it < 1 + 56; ++it) {
+          const int side = it > 0 ? (it - 1) / 28 : 0, c = it > 0 ? ((it - 1) >> 1) % 14 : 0;
+          const int sgn = it > 0 ? (it - 1) & 1 : 0;
+          if (it > 0 && c % split != sub) continue;
+          const int b = it == 0 ? -1 : (side ? rb1 : rb0);
+          if (it > 0 && b < 0) {
+            if (sgn) T[(2 * i + side) * 14 + c] = 0.f;
+            continue;
+          }
+          const float t = adjoint_row_term(P, sm, lamprev, w, i, n_c, lam[k], x[k],
+                                           ShiftedPose{b, c, sgn ? -h : h}, &rb0, &rb1);
+          if (it == 0) {
+            if (sub == 0) { tb[2 * i] = rb0; tb[2 * i + 1] = rb1; }
+          } else if (sgn == 0) {
+            tp = t;
+          } else {
+            T[(2 * i + side) * 14 + c] = (tp - t) / (2.f * h);
+          }
+        }
+      }
+    }
+    for (int e = lane; e < 7 * B; e += WAVE) {  // dynamics terms: q+ only
+      const int b = e / 7, c = e % 7;
+      if (c % split != sub) continue;
+      const float* bb = sm + b * BS;
+      const tf q0 = ldtf(bb + BQ);
+      const ShiftedPose sp{b, c, h}, sn{b, c, -h};
+      const sv wu = ldsv(bb + BWU);
+      const float dp = sdot(wu, dyn_residual(P, bb, sp.shift(q0, b, c).q));
+      const float dn = sdot(wu, dyn_residual(P, bb, sn.shift(q0, b, c).q));
+      Td[e] = (dp - dn) / (2.f * h);
+    }
+  } else if (P.cfg.adjoint_pose_vjp) {
     const float h = P.cfg.adjoint_pose_eps;
     for (int d = sub; d < 14 * B; d += split) {
       const int b = d / 14, kk = d % 14;
@@ -368,6 +446,23 @@ __global__ void __launch_bounds__(256) adjoint_step_kernel(AdjParams P) {
   if (split > 1) {
     __syncthreads();
     if (sub > 0) return;
+  }
+  if (P.cfg.adjoint_pose_vjp && P.fd_off >= 0) {  // sum the row-local differences per body
+    wsync();
+    const float* T = sm0 + P.fd_off;
+    const int* tb = (const int*)(T + 28 * P.rows_cap);
+    const float* Td = T + 30 * P.rows_cap;
+    for (int e = lane; e < 14 * B; e += WAVE) {
+      const int b = e / 14, c = e % 14;
+      float acc = 0.f;
+      for (int i = 0; i < n_r; ++i) {
+        if (tb[2 * i] == b) acc += T[(2 * i) * 14 + c];
+        if (tb[2 * i + 1] == b) acc += T[(2 * i + 1) * 14 + c];
+      }
+      if (c < 7) acc += Td[b * 7 + c];
+      gpose[e] = acc;
+    }
+    wsync();
   }
 
   // ---- 5c. carry dq+/dq- of g_q + w^T dR/dq+ (residual.py:260-339), outputs ----

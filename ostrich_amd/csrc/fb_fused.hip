@@ -342,15 +342,25 @@ int fused_adjoint(ofb_solver* s, const ofb_state_in* in, const ofb_state_out* ou
     P.lds_per_wave = adj_lds_words(s, R);
     P.xfer_off = P.lds_per_wave;
     P.split = 1;
+    P.fd_off = -1;
     int wpb = s->fused_worlds_per_block;
-    // Fewer worlds than resident wave slots (the kernel's registers allow two waves per
-    // SIMD): give each world `split` waves for the pose central differences (one world
-    // per workgroup, each wave its own LDS copy).  1024 worlds on 1024 SIMDs: split 2.
     if (s->cfg.adjoint_pose_vjp) {
+      // Row-local central differences (worlds of at most 64 rows) need a table of
+      // 30 words per row slot after the transfer area.
+      const int base = P.lds_per_wave;
+      const int fd_words = adj_fd_words(P.rows_cap, s->m.B);
+      const bool row_fd = R == 1 && (size_t)(base + 2 * P.rows_cap + fd_words) * sizeof(float) <= 65536;
+      const int words = base + 2 * P.rows_cap + (row_fd ? fd_words : 0);
+      // Fewer worlds than resident wave slots (the kernel's registers allow two waves per
+      // SIMD): give each world `split` waves for the pose central differences (one world
+      // per workgroup, each wave its own LDS copy).  1024 worlds on 1024 SIMDs: split 2.
       int split = 1;
       while (split < OFB_ADJ_MAX_SPLIT && (size_t)W * split * 2 <= (size_t)2 * s->num_simds) split *= 2;
-      const int words = P.lds_per_wave + 2 * P.rows_cap;
       while (split > 1 && (size_t)split * words * sizeof(float) > 65536) split /= 2;
+      if (row_fd) {
+        P.fd_off = base + 2 * P.rows_cap;
+        P.lds_per_wave = words;
+      }
       if (split > 1) {
         P.split = split;
         P.lds_per_wave = words;

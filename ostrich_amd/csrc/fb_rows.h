@@ -247,25 +247,55 @@ __device__ __forceinline__ JointStatic load_joint_static(const float* d) {
   return s;
 }
 
+// How the row functions read body poses: straight from the LDS body blocks (every
+// solver pass), or with one coordinate of one body shifted (the adjoint's central
+// differences, which evaluate a row at a perturbed pose without touching the shared
+// block).  coord 0..6: q+ (p, quaternion xyzw), 7..13: the step-start pose q-.
+struct LdsPoses {
+  __device__ __forceinline__ tf q(const float* sm, int b) const { return ldtf(sm + b * BS + BQ); }
+  __device__ __forceinline__ tf qp(const float* sm, int b) const { return ldtf(sm + b * BS + BQP); }
+};
+struct ShiftedPose {
+  int body, coord;
+  float delta;
+  __device__ __forceinline__ tf shift(tf x, int b, int k) const {
+    if (b != body) return x;
+    x.p.x += k == 0 ? delta : 0.f;
+    x.p.y += k == 1 ? delta : 0.f;
+    x.p.z += k == 2 ? delta : 0.f;
+    x.q.x += k == 3 ? delta : 0.f;
+    x.q.y += k == 4 ? delta : 0.f;
+    x.q.z += k == 5 ? delta : 0.f;
+    x.q.w += k == 6 ? delta : 0.f;
+    return x;
+  }
+  __device__ __forceinline__ tf q(const float* sm, int b) const { return shift(ldtf(sm + b * BS + BQ), b, coord); }
+  __device__ __forceinline__ tf qp(const float* sm, int b) const {
+    return shift(ldtf(sm + b * BS + BQP), b, coord - 7);
+  }
+};
+
 // world frames of a joint at the current poses (joint_kinematics.py:5-40)
+template <class PR = LdsPoses>
 __device__ __forceinline__ void joint_frames(const JointStatic& s, const float* sm, tf& Xwp, tf& Xwc, v3& r_p,
-                                             v3& r_c, v3& pos_p, v3& pos_c, v3& com_p) {
+                                             v3& r_c, v3& pos_p, v3& pos_c, v3& com_p, const PR& pr = PR()) {
   const float* bc = sm + s.child * BS;
-  tf bq = ldtf(bc + BQ);
+  tf bq = pr.q(sm, s.child);
   Xwc = tmul(bq, s.Xc);
   pos_c = Xwc.p;
   r_c = pos_c - tpoint(bq, ld3(bc + BCOM));
   tf bp = tident();
   com_p = mk(0.f, 0.f, 0.f);
-  if (s.parent >= 0) { bp = ldtf(sm + s.parent * BS + BQ); com_p = ld3(sm + s.parent * BS + BCOM); }
+  if (s.parent >= 0) { bp = pr.q(sm, s.parent); com_p = ld3(sm + s.parent * BS + BCOM); }
   Xwp = tmul(bp, s.Xp);
   pos_p = Xwp.p;
   r_p = pos_p - tpoint(bp, com_p);
 }
 
 // row k of a joint (constraints/joint_constraint.py:62-162, joint_kinematics.py:5-261)
-template <class PT>
-__device__ RowOut joint_row_s(const PT& P, float* sm, const JointStatic& s, int k, float lam, bool scatter = true) {
+template <class PT, class PR = LdsPoses>
+__device__ RowOut joint_row_s(const PT& P, float* sm, const JointStatic& s, int k, float lam, bool scatter = true,
+                              const PR& pr = PR()) {
   RowOut o;
   o.J0 = svzero(); o.J1 = svzero(); o.hc = 0.f; o.C = 0.f; o.active = false;
   o.b0 = s.parent;
@@ -275,7 +305,7 @@ __device__ RowOut joint_row_s(const PT& P, float* sm, const JointStatic& s, int 
   if (type < 0 || type > 3) return o;
   const float comp_ = s.comp;
   tf Xwp, Xwc; v3 r_p, r_c, pos_p, pos_c, com_p;
-  joint_frames(s, sm, Xwp, Xwc, r_p, r_c, pos_p, pos_c, com_p);
+  joint_frames(s, sm, Xwp, Xwc, r_p, r_c, pos_p, pos_c, com_p, pr);
   const v3 ax = s.ax;
   sv Jp = svzero(), Jc = svzero();
   float err = 0.f;
@@ -332,14 +362,16 @@ __device__ RowOut joint_row_s(const PT& P, float* sm, const JointStatic& s, int 
   return o;
 }
 
-template <class PT>
-__device__ RowOut joint_row(const PT& P, float* sm, int w, int row, float lam, bool /*full*/) {
-  return joint_row_s(P, sm, joint_static(P, w, P.topo[row]), P.topo[P.L.Nj + row], lam);
+template <class PT, class PR = LdsPoses>
+__device__ RowOut joint_row(const PT& P, float* sm, int w, int row, float lam, bool /*full*/, const PR& pr = PR(),
+                            bool scatter = true) {
+  return joint_row_s(P, sm, joint_static(P, w, P.topo[row]), P.topo[P.L.Nj + row], lam, scatter, pr);
 }
 
 // control row of a joint (constraints/control_constraint.py:117-338)
-template <class PT>
-__device__ RowOut control_row_s(const PT& P, float* sm, const JointStatic& s, float lam, bool scatter = true) {
+template <class PT, class PR = LdsPoses>
+__device__ RowOut control_row_s(const PT& P, float* sm, const JointStatic& s, float lam, bool scatter = true,
+                                const PR& pr = PR()) {
   RowOut o;
   o.J0 = svzero(); o.J1 = svzero(); o.hc = 0.f; o.C = 0.f; o.active = false;
   o.b0 = s.parent;
@@ -350,7 +382,7 @@ __device__ RowOut control_row_s(const PT& P, float* sm, const JointStatic& s, fl
   const int mode = s.dof_mode;
   if (mode == 0 || o.b1 < 0) return o;
   tf Xwp, Xwc; v3 r_p, r_c, pos_p, pos_c, com_p;
-  joint_frames(s, sm, Xwp, Xwc, r_p, r_c, pos_p, pos_c, com_p);
+  joint_frames(s, sm, Xwp, Xwc, r_p, r_c, pos_p, pos_c, com_p, pr);
   const v3 ax = s.ax;
   sv up = svzero();
   if (o.b0 >= 0) up = ldsv(sm + o.b0 * BS + BU);
@@ -410,9 +442,10 @@ __device__ RowOut control_row_s(const PT& P, float* sm, const JointStatic& s, fl
   return o;
 }
 
-template <class PT>
-__device__ RowOut control_row(const PT& P, float* sm, int w, int crow, float lam) {
-  return control_row_s(P, sm, joint_static(P, w, P.topo[2 * P.L.Nj + crow]), lam);
+template <class PT, class PR = LdsPoses>
+__device__ RowOut control_row(const PT& P, float* sm, int w, int crow, float lam, const PR& pr = PR(),
+                              bool scatter = true) {
+  return control_row_s(P, sm, joint_static(P, w, P.topo[2 * P.L.Nj + crow]), lam, scatter, pr);
 }
 
 struct ContactGeo {
@@ -442,16 +475,18 @@ __device__ __forceinline__ ContactGeo load_contact(const PT& P, int w, int c) {
 
 // effective mass J M^-1 J^T with the inverse inertia rotated by the step-start pose
 // (constraints/utils.py:6-33)
-__device__ __forceinline__ float eff_mass_prev(const float* sm, int b0, int b1, const sv& J0, const sv& J1) {
+template <class PR = LdsPoses>
+__device__ __forceinline__ float eff_mass_prev(const float* sm, int b0, int b1, const sv& J0, const sv& J1,
+                                               const PR& pr = PR()) {
   float val = 0.f;
   if (b0 >= 0) {
     const float* bb = sm + b0 * BS;
-    m3 Iw = rot_inertia(qt{bb[BQP + 3], bb[BQP + 4], bb[BQP + 5], bb[BQP + 6]}, ldm3(bb + BIIB));
+    m3 Iw = rot_inertia(pr.qp(sm, b0).q, ldm3(bb + BIIB));
     val += dot(J0.a, J0.a * bb[BMINV]) + dot(J0.b, mv(Iw, J0.b));
   }
   if (b1 >= 0) {
     const float* bb = sm + b1 * BS;
-    m3 Iw = rot_inertia(qt{bb[BQP + 3], bb[BQP + 4], bb[BQP + 5], bb[BQP + 6]}, ldm3(bb + BIIB));
+    m3 Iw = rot_inertia(pr.qp(sm, b1).q, ldm3(bb + BIIB));
     val += dot(J1.a, J1.a * bb[BMINV]) + dot(J1.b, mv(Iw, J1.b));
   }
   return val;
@@ -466,8 +501,9 @@ struct NormalStatic {
   sv Ja, Jb;
 };
 
-template <class PT>
-__device__ __forceinline__ NormalStatic normal_static(const PT& P, const float* sm, int w, int c) {
+template <class PT, class PR = LdsPoses>
+__device__ __forceinline__ NormalStatic normal_static(const PT& P, const float* sm, int w, int c,
+                                                      const PR& pr = PR()) {
   ContactGeo g = load_contact(P, w, c);
   NormalStatic s;
   s.skip = g.s0 == g.s1;
@@ -475,28 +511,29 @@ __device__ __forceinline__ NormalStatic normal_static(const PT& P, const float* 
   s.p0 = g.p0; s.p1 = g.p1; s.n = g.n; s.th0 = g.th0; s.th1 = g.th1;
   tf q0p = tident(), q1p = tident();
   v3 com0 = mk(0.f, 0.f, 0.f), com1 = com0;
-  if (g.b0 >= 0) { const float* bb = sm + g.b0 * BS; q0p = ldtf(bb + BQP); com0 = ld3(bb + BCOM); }
-  if (g.b1 >= 0) { const float* bb = sm + g.b1 * BS; q1p = ldtf(bb + BQP); com1 = ld3(bb + BCOM); }
+  if (g.b0 >= 0) { q0p = pr.qp(sm, g.b0); com0 = ld3(sm + g.b0 * BS + BCOM); }
+  if (g.b1 >= 0) { q1p = pr.qp(sm, g.b1); com1 = ld3(sm + g.b1 * BS + BCOM); }
   v3 p0_adj_prev = tpoint(q0p, g.p0) - g.n * g.th0;
   v3 p1_adj_prev = tpoint(q1p, g.p1) + g.n * g.th1;
   s.Ja = svzero(); s.Jb = svzero();
   if (g.b0 >= 0) s.Ja = sv{g.n, cross(p0_adj_prev - tpoint(q0p, com0), g.n)};
   if (g.b1 >= 0) s.Jb = sv{-g.n, cross(p1_adj_prev - tpoint(q1p, com1), -g.n)};
   const float dt = P.dt;
-  s.precond = dt * dt * eff_mass_prev(sm, g.b0, g.b1, s.Ja, s.Jb);
+  s.precond = dt * dt * eff_mass_prev(sm, g.b0, g.b1, s.Ja, s.Jb, pr);
   return s;
 }
 
 // contact normal row at the current poses (constraints/contact_constraint.py:48-374)
-template <class PT>
-__device__ RowOut normal_row_s(const PT& P, float* sm, const NormalStatic& s, float lam, bool scatter = true) {
+template <class PT, class PR = LdsPoses>
+__device__ RowOut normal_row_s(const PT& P, float* sm, const NormalStatic& s, float lam, bool scatter = true,
+                               const PR& pr = PR()) {
   RowOut o;
   o.J0 = svzero(); o.J1 = svzero(); o.hc = 0.f; o.C = 0.f; o.active = false; o.b0 = -1; o.b1 = -1;
   if (s.skip) return o;
   o.b0 = s.b0; o.b1 = s.b1;
   tf q0 = tident(), q1 = tident();
-  if (s.b0 >= 0) q0 = ldtf(sm + s.b0 * BS + BQ);
-  if (s.b1 >= 0) q1 = ldtf(sm + s.b1 * BS + BQ);
+  if (s.b0 >= 0) q0 = pr.q(sm, s.b0);
+  if (s.b1 >= 0) q1 = pr.q(sm, s.b1);
   v3 p0_adj = tpoint(q0, s.p0) - s.n * s.th0;
   v3 p1_adj = tpoint(q1, s.p1) + s.n * s.th1;
   float gap = dot(s.n, p0_adj - p1_adj);
@@ -524,9 +561,10 @@ __device__ RowOut normal_row_s(const PT& P, float* sm, const NormalStatic& s, fl
   return o;
 }
 
-template <class PT>
-__device__ RowOut normal_row(const PT& P, float* sm, int w, int c, float lam) {
-  return normal_row_s(P, sm, normal_static(P, sm, w, c), lam);
+template <class PT, class PR = LdsPoses>
+__device__ RowOut normal_row(const PT& P, float* sm, int w, int c, float lam, const PR& pr = PR(),
+                             bool scatter = true) {
+  return normal_row_s(P, sm, normal_static(P, sm, w, c, pr), lam, scatter, pr);
 }
 
 // friction rows of a contact: the step-constant part - skip rule, shape parameters, the
@@ -539,8 +577,9 @@ struct FrictionStatic {
   float rr, mux, muy, gate_mu, mss, vst, lat;
 };
 
-template <class PT>
-__device__ __forceinline__ FrictionStatic friction_static(const PT& P, const float* sm, int w, int c) {
+template <class PT, class PR = LdsPoses>
+__device__ __forceinline__ FrictionStatic friction_static(const PT& P, const float* sm, int w, int c,
+                                                          const PR& pr = PR()) {
   ContactGeo g = load_contact(P, w, c);
   FrictionStatic s;
   s.b0 = g.b0; s.b1 = g.b1;
@@ -561,8 +600,8 @@ __device__ __forceinline__ FrictionStatic friction_static(const PT& P, const flo
   s.gate_mu = fmaxf(mu0, mu1);
   tf q0p = tident(), q1p = tident();
   v3 com0 = mk(0.f, 0.f, 0.f), com1 = com0;
-  if (g.b0 >= 0) { const float* bb = sm + g.b0 * BS; q0p = ldtf(bb + BQP); com0 = ld3(bb + BCOM); }
-  if (g.b1 >= 0) { const float* bb = sm + g.b1 * BS; q1p = ldtf(bb + BQP); com1 = ld3(bb + BCOM); }
+  if (g.b0 >= 0) { q0p = pr.qp(sm, g.b0); com0 = ld3(sm + g.b0 * BS + BCOM); }
+  if (g.b1 >= 0) { q1p = pr.qp(sm, g.b1); com1 = ld3(sm + g.b1 * BS + BCOM); }
   // resolve_friction_frame :79-156
   v3 ax0 = ld3(P.m.shape_friction_axis + (size_t)(w * SC + g.s0) * 3);
   v3 ax1 = ld3(P.m.shape_friction_axis + (size_t)(w * SC + g.s1) * 3);
@@ -613,8 +652,8 @@ __device__ __forceinline__ FrictionStatic friction_static(const PT& P, const flo
     s.Jt1b = sv{-t1, -cross(r_, t1)};
     s.Jt2b = sv{-t2, -cross(r_, t2)};
   }
-  float wt1 = eff_mass_prev(sm, g.b0, g.b1, s.Jt1a, s.Jt1b);
-  float wt2 = eff_mass_prev(sm, g.b0, g.b1, s.Jt2a, s.Jt2b);
+  float wt1 = eff_mass_prev(sm, g.b0, g.b1, s.Jt1a, s.Jt1b, pr);
+  float wt2 = eff_mass_prev(sm, g.b0, g.b1, s.Jt2a, s.Jt2b, pr);
   s.rr = (wt1 + wt2) * 0.5f;
   return s;
 }
@@ -760,23 +799,23 @@ __device__ RowOut friction_row_s(const PT& P, float* sm, const float* lamprev, c
   return o;
 }
 
-template <class PT>
+template <class PT, class PR = LdsPoses>
 __device__ RowOut friction_row(const PT& P, float* sm, const float* lamprev, int w, int c, int t, int n_c,
-                               float lam, float* dwdmu = nullptr) {
-  return friction_row_s(P, sm, lamprev, friction_static(P, sm, w, c), c, t, n_c, lam, dwdmu);
+                               float lam, float* dwdmu = nullptr, const PR& pr = PR(), bool scatter = true) {
+  return friction_row_s(P, sm, lamprev, friction_static(P, sm, w, c, pr), c, t, n_c, lam, dwdmu, scatter);
 }
 
 // dispatch on the compacted row index
-template <class PT>
+template <class PT, class PR = LdsPoses>
 __device__ __forceinline__ RowOut assemble_row(const PT& P, float* sm, const float* lamprev, int w, int i,
-                                               int n_c, float lam) {
+                                               int n_c, float lam, const PR& pr = PR(), bool scatter = true) {
   const int Nj = P.L.Nj, Nct = P.L.Nctrl;
-  if (i < Nj) return joint_row(P, sm, w, i, lam, true);
-  if (i < Nj + Nct) return control_row(P, sm, w, i - Nj, lam);
+  if (i < Nj) return joint_row(P, sm, w, i, lam, true, pr, scatter);
+  if (i < Nj + Nct) return control_row(P, sm, w, i - Nj, lam, pr, scatter);
   int c = i - Nj - Nct;
-  if (c < n_c) return normal_row(P, sm, w, c, lam);
+  if (c < n_c) return normal_row(P, sm, w, c, lam, pr, scatter);
   int f = c - n_c;
-  return friction_row(P, sm, lamprev, w, f >> 1, f & 1, n_c, lam);
+  return friction_row(P, sm, lamprev, w, f >> 1, f & 1, n_c, lam, nullptr, pr, scatter);
 }
 
 // dispatch on the compacted row index with the step's static records in LDS (js: one
@@ -804,11 +843,25 @@ __device__ __forceinline__ int original_row(const RowLayout& L, int i, int n_c) 
   return L.off_f + f;
 }
 
-// dynamics residual of every body (constraints/dynamics_constraint.py:6-50); lanes over bodies
+// dynamics residual of one body at rotation q (constraints/dynamics_constraint.py:6-50):
+// M(q)(u - u-) - dt (f + m g + gyro)
+template <class PT>
+__device__ __forceinline__ sv dyn_residual(const PT& P, const float* bb, qt q) {
+  const v3 g = mk(P.m.gravity[0], P.m.gravity[1], P.m.gravity[2]);
+  m3 Iw = rot_inertia(q, ldm3(bb + BIB));
+  sv u = ldsv(bb + BU), up = ldsv(bb + BUP), f = ldsv(bb + BF);
+  float m = bb[BM];
+  v3 wv = u.b;
+  v3 gyro = cross(wv, mv(Iw, wv)) * P.cfg.gyro_scale;
+  v3 lin = (u.a - up.a) * m - (f.a + g * m) * P.dt;
+  v3 ang = mv(Iw, u.b - up.b) - (f.b + gyro) * P.dt;
+  return sv{lin, ang};
+}
+
+// dynamics residual of every body; lanes over bodies
 template <class PT>
 __device__ __forceinline__ void dynamics_bodies(const PT& P, float* sm, int lane, bool with_inv, int stride = WAVE) {
   const int B = P.m.B;
-  const v3 g = mk(P.m.gravity[0], P.m.gravity[1], P.m.gravity[2]);
   for (int b = lane; b < B; b += stride) {
     float* bb = sm + b * BS;
     qt q = qt{bb[BQ + 3], bb[BQ + 4], bb[BQ + 5], bb[BQ + 6]};
@@ -817,15 +870,7 @@ __device__ __forceinline__ void dynamics_bodies(const PT& P, float* sm, int lane
 #pragma unroll
       for (int k = 0; k < 9; ++k) bb[BIW + k] = Iwi.m[k];
     }
-    m3 Iw = rot_inertia(q, ldm3(bb + BIB));
-    sv u = ldsv(bb + BU), up = ldsv(bb + BUP), f = ldsv(bb + BF);
-    float m = bb[BM];
-    v3 wv = u.b;
-    v3 gyro = cross(wv, mv(Iw, wv)) * P.cfg.gyro_scale;
-    v3 lin = (u.a - up.a) * m - (f.a + g * m) * P.dt;
-    v3 ang = mv(Iw, u.b - up.b) - (f.b + gyro) * P.dt;
-    bb[BHD + 0] = lin.x; bb[BHD + 1] = lin.y; bb[BHD + 2] = lin.z;
-    bb[BHD + 3] = ang.x; bb[BHD + 4] = ang.y; bb[BHD + 5] = ang.z;
+    stsv(bb + BHD, dyn_residual(P, bb, q));
   }
 }
 
