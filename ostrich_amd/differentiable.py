@@ -120,14 +120,17 @@ class CheckpointedRollout:
         return _record_step(self.engine, state, self.controls[t], contacts, self.dt)
 
     def forward(self, state0: State, controls: List[Control]) -> State:
-        """Runs len(controls) steps from state0; keeps the states at steps 0, k, 2k, ..."""
+        """Runs len(controls) steps from state0; keeps the states at steps 0, k, 2k, ...
+        (no per-step records: the backward re-simulates each window from its checkpoint
+        with the same kernels, which reproduces these steps bit for bit)."""
         self.controls = list(controls)
         self.checkpoints = []
-        s = state0.clone()
+        s, nxt = state0.clone(), state0.clone()
         for t in range(len(self.controls)):
             if t % self.k == 0:
                 self.checkpoints.append(s.clone())
-            s = self._step(s, t).state_out
+            self.engine.step(s, nxt, self.controls[t], self.collide(s.body_q), self.dt)
+            s, nxt = nxt, s
         self.final = s
         return s
 
