@@ -39,7 +39,7 @@ def main():
     import bench
     from . import collide, scenes
     model, state, control = scenes.skid_steer(args.worlds, seed=1)
-    col = collide.PrimitiveCollider(model, device="cuda:0", gap=0.02, max_contacts_per_world=128)
+    col = collide.HipCollider(model, device="cuda:0", gap=0.02, max_contacts_per_world=128)
     f = lib.ofb_debug_phase_cycles
     f.argtypes = [C.c_void_p, C.c_int]
     buf = (C.c_ulonglong * 8)()

@@ -23,7 +23,7 @@ import torch
 from . import collide, scenes
 from .config import (ContactsConfig, ExecutionConfig, LinearSolverConfig, NewtonRaphsonConfig, OstrichEngineConfig,
                      WarmStartConfig)
-from .containers import Contacts, Control, State
+from .containers import Control, State
 from .engine import OstrichEngine
 from .learned import WarmStartNet
 
@@ -54,14 +54,14 @@ def rollout(kind, worlds, seed, steps, device, n_cubes=3, net=None, seeded=True,
     eng = OstrichEngine(model, config=cfg, device=device)
     if net is not None:
         eng.set_warm_start_net(net)
-    col = collide.PrimitiveCollider(model, device=device, gap=0.02, max_contacts_per_world=K)
+    col = collide.HipCollider(model, device=device, gap=0.02, max_contacts_per_world=K)
     s_a = State.from_numpy(state, device)
     s_b = s_a.clone()
     ctl = Control.from_numpy(control, device)
     nr, xs, ts, learned = [], [], [], []
     atol_sq = cfg.nr.atol ** 2
     for _ in range(steps):
-        con = Contacts.from_dict(col.collide(s_a.body_q), device)
+        con = col.collide(s_a.body_q)
         eng.step(s_a, s_b, ctl, con, DT)
         st = eng.stats()
         nr.append(st["nr_iters"].float().mean().item())
