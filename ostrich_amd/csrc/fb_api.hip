@@ -252,7 +252,8 @@ int ofb_destroy(ofb_solver* s) {
                   m.shape_v_stribeck, m.shape_stribeck_lateral_only, m.shape_world, s->ccount, s->cf, s->ci,
                   s->pk_keys, s->pk_members, s->topo,
                   s->lambda, s->lambda_prev, s->lambda_seed, s->ws_count, s->ws_b0, s->ws_b1, s->ws_p, s->ws_n,
-                  s->ws_ln, s->ws_lt, s->ws_feat, s->ws_fscale, s->ws_y, s->lambda_seed2, s->ls_steps, s->st_nr, s->st_pcr, s->st_best, s->st_flags, s->st_merit};
+                  s->ws_ln, s->ws_lt, s->ws_feat, s->ws_fscale, s->ws_y, s->ws_off, s->lambda_seed2, s->ls_steps, s->st_nr,
+                  s->st_pcr, s->st_best, s->st_flags, s->st_merit};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (s->plain) plain_destroy(s);
@@ -394,6 +395,7 @@ int ofb_create(const ofb_model* model, const ofb_config* config, int32_t path, o
     ALLOC(ws_count, W); ALLOC(ws_b0, (size_t)W * K); ALLOC(ws_b1, (size_t)W * K); ALLOC(ws_p, (size_t)W * K * 3);
     ALLOC(ws_n, (size_t)W * K * 3); ALLOC(ws_ln, (size_t)W * K); ALLOC(ws_lt, (size_t)W * K * 2);
     ALLOC(ws_feat, (size_t)W * K * 12); ALLOC(ws_fscale, (size_t)W * K); ALLOC(ws_y, (size_t)W * K * 3);
+    ALLOC(ws_off, (size_t)W + 1);
     ALLOC(lambda_seed2, (size_t)W * L.Nc);
     if (hipMemset(s->ws_count, 0, (size_t)W * sizeof(int32_t)) != hipSuccess) {
       set_error("hipMemset failed (warm start)");
@@ -462,7 +464,8 @@ int ofb_step(ofb_solver* s, const ofb_state_in* in, const ofb_state_out* out, co
   if (timed) OFB_CHECK(hipEventRecord(s->ev_solve0, st));
   if (ws && (rc = warm_start_apply(s, in, dt, st)) != 0) return rc;
   if (ws && s->ws_net && s->path == OFB_PATH_FUSED) {
-    if ((rc = ofb_mlp_forward(s->ws_net, s->ws_feat, s->m.W * s->L.K, s->ws_y, st)) != 0) return rc;
+    if ((rc = mlp_forward_packed(s->ws_net, s->ws_feat, s->ccount, s->m.W, s->L.K, s->ws_off, s->ws_y, st)) != 0)
+      return rc;
     if ((rc = warm_start_learned_seed(s, s->ws_y, st)) != 0) return rc;
   }
   rc = s->path == OFB_PATH_FUSED ? fused_step(s, in, out, ctl, dt, st) : plain_step(s, in, out, ctl, dt, st);

@@ -130,6 +130,71 @@ __global__ void __launch_bounds__(64) mlp_forward_kernel(const float* __restrict
   }
 }
 
+// Forward over the valid contact slots only: dense row g of the packed list is slot c of
+// world w, where off[w] <= g < off[w + 1] (off = exclusive scan of the worlds' contact
+// counts, off[W] = total); x and y stay in the slot layout [W * K] rows.  Every row's
+// result depends on that row alone, so it equals the forward over all W * K slots.
+__global__ void __launch_bounds__(64) mlp_forward_packed_kernel(const float* __restrict__ w,
+                                                                const uint16_t* __restrict__ wb,
+                                                                const float* __restrict__ x,
+                                                                const int32_t* __restrict__ off, int W, int K,
+                                                                float* __restrict__ y) {
+  __shared__ uint16_t sX[ROWS * LX], sH1[ROWS * LH], sH2[ROWS * LH];
+  __shared__ float sY[ROWS * LY];
+  __shared__ int sRow[ROWS];
+  const int lane = threadIdx.x, row0 = blockIdx.x * ROWS;
+  const int n = off[W];
+  if (row0 >= n) return;  // uniform over the block
+  {
+    const int g = row0 + lane;  // ROWS == 64 == blockDim.x: one packed row per lane
+    int r = -1;
+    if (g < n) {
+      int lo = 0, hi = W;  // largest world with off[world] <= g
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (off[mid] <= g) lo = mid;
+        else hi = mid;
+      }
+      r = lo * K + (g - off[lo]);
+    }
+    sRow[lane] = r;
+  }
+  __syncthreads();
+  for (int e = lane; e < ROWS * INP; e += 64) {
+    const int i = e / INP, k = e % INP, r = sRow[i];
+    sX[i * LX + k] = (r >= 0 && k < IN) ? to_bf16(x[(size_t)r * IN + k]) : (uint16_t)0;
+  }
+  __syncthreads();
+  block_forward(wb, w, sX, sH1, sH2, sY, lane);
+  for (int e = lane; e < ROWS * OUT; e += 64) {
+    const int i = e / OUT, c = e % OUT, r = sRow[i];
+    if (r >= 0) y[(size_t)r * OUT + c] = sY[i * LY + c];
+  }
+}
+
+// exclusive scan of min(count[w], K) over the worlds (one workgroup); off[W] = total
+__global__ void __launch_bounds__(1024) mlp_offsets_kernel(int W, int K, const int32_t* __restrict__ count,
+                                                           int32_t* __restrict__ off) {
+  __shared__ int part[1024];
+  const int t = threadIdx.x, per = (W + 1023) / 1024;
+  int s = 0;
+  for (int i = t * per; i < min(W, (t + 1) * per); ++i) s += min(count[i], K);
+  part[t] = s;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    const int v = t >= o ? part[t - o] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int run = t > 0 ? part[t - 1] : 0;
+  for (int i = t * per; i < min(W, (t + 1) * per); ++i) {
+    off[i] = run;
+    run += min(count[i], K);
+  }
+  if (t == 1023) off[W] = part[1023];
+}
+
 // forward + backward of one block; writes the block's gradient slot and loss
 __global__ void __launch_bounds__(64) mlp_grad_kernel(const float* __restrict__ w, const uint16_t* __restrict__ wb,
                                                       const float* __restrict__ x, const float* __restrict__ tgt,
@@ -284,6 +349,19 @@ static int mlp_grad_launch(ofb_mlp* net, const float* x, const float* t, int n, 
                      net->n.grad, net->n.lpart, loss);
   return hipGetLastError() == hipSuccess ? 0 : -4;
 }
+
+namespace ofb {
+int mlp_forward_packed(const ofb_mlp* net, const float* x, const int32_t* count, int W, int K, int32_t* off,
+                       float* y, hipStream_t st) {
+  static_assert(ROWS == 64, "one packed row per lane of the 64-lane block");
+  hipLaunchKernelGGL(mlp_offsets_kernel, dim3(1), dim3(1024), 0, st, W, K, count, off);
+  const int blocks = (int)(((size_t)W * K + ROWS - 1) / ROWS);  // upper bound; blocks past the total exit
+  hipLaunchKernelGGL(mlp_forward_packed_kernel, dim3(blocks), dim3(64), 0, st, net->n.w, net->n.wb, x, off, W, K,
+                     y);
+  if (hipGetLastError() != hipSuccess) { set_error("mlp forward launch failed"); return -4; }
+  return 0;
+}
+}  // namespace ofb
 
 extern "C" {
 

@@ -100,6 +100,7 @@ struct ofb_solver {
   int32_t *ws_count, *ws_b0, *ws_b1;
   float *ws_p, *ws_n, *ws_ln, *ws_lt;
   float *ws_feat, *ws_fscale, *ws_y;  // learned warm start: features [W*K*12], scale [W*K], output [W*K*3]
+  int32_t* ws_off;                     // [W+1] offsets of the worlds' contacts in the packed MLP rows
   float* lambda_seed2;                // [W*Nc] learned seed
   const struct ofb_mlp* ws_net;       // attached network (not owned)
   // merit line search candidates (core/engine_data.py:15-39), device and host copies
@@ -129,5 +130,9 @@ int reduce_contacts(ofb_solver* s, const float* body_q, hipStream_t stream);
 int warm_start_apply(ofb_solver* s, const ofb_state_in* in, float dt, hipStream_t stream);
 int warm_start_snapshot(ofb_solver* s, const ofb_state_in* in, float dt, hipStream_t stream);
 int warm_start_learned_seed(ofb_solver* s, const float* y, hipStream_t stream);
+// network forward over the W * K contact slots, valid ones only (c < count[w]), through
+// a packed row list (off: [W + 1] scratch)
+int mlp_forward_packed(const ofb_mlp* net, const float* x, const int32_t* count, int W, int K, int32_t* off,
+                       float* y, hipStream_t st);
 void set_error(const char* fmt, ...);
 }  // namespace ofb
