@@ -147,13 +147,14 @@ class CheckpointedRollout:
         return buf
 
     def backward(self, grad_q: torch.Tensor, grad_qd: torch.Tensor,
-                 step_loss_grads: Optional[Callable[[int], Optional[tuple]]] = None):
+                 step_loss_grads: Optional[Callable[[int], Optional[tuple]]] = None, adjoint_log=None):
         """Adjoint of the whole rollout from dL/dq_N, dL/du_N.
 
         `step_loss_grads(t)` may return (dL/dq_t, dL/du_t) of a loss on intermediate state
         t (1 <= t < N), added to the carried gradient.  Returns dict: body_q, body_qd (of
         the initial state), body_f [N, ...], joint_target_pos / joint_target_vel [N, W, D],
-        joint_target_ke, joint_target_kd [W, D] and shape_mu [W, S+G] summed over steps."""
+        joint_target_ke, joint_target_kd [W, D] and shape_mu [W, S+G] summed over steps.
+        `adjoint_log` (sim_log.AdjointLog) logs every step's adjoint in slot t."""
         N = len(self.controls)
         gq = grad_q.reshape(self.final.body_q.shape).clone()
         gu = grad_qd.reshape(self.final.body_qd.shape).clone()
@@ -166,7 +167,9 @@ class CheckpointedRollout:
                 t = t0 + i
                 r = buf[i]
                 g = self.engine.step_backward(r.state_in, r.state_out, r.control, r.contacts, self.dt, r.lam,
-                                              r.lam_prev, gq, gu)
+                                              r.lam_prev, gq, gu, adjoint_multipliers=adjoint_log is not None)
+                if adjoint_log is not None:
+                    adjoint_log.capture_step(t, g, gq, gu)
                 out["body_f"][t] = g["body_f"]
                 out["joint_target_pos"][t] = g["joint_target_pos"]
                 out["joint_target_vel"][t] = g["joint_target_vel"]

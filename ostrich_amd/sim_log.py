@@ -198,3 +198,51 @@ class DatasetLog:
 
     def save(self, path):
         save_log(path, self.num_steps, self.dt, self.log.numpy(), _model_arrays(self.engine.model), _dims(self.engine))
+
+
+class AdjointLog:
+    """AdjointHDF5Logger (logging/adjoint_logger.py:15-132): per differentiated step t, in
+    slot t, the adjoint multipliers (_w = w_lambda [W, N_c], w_u [W*B, 6]), the incoming
+    gradients (body_pose_grad, body_vel_grad), the gradients of the step's inputs
+    (body_pose_prev_grad, body_vel_prev_grad, ext_force_grad, joint_target_pos_grad,
+    joint_target_vel_grad, shape_mu_grad) and the adjoint PCR iterations per world
+    (pcr_iter_count).  `capture_step(t, grads, grad_body_q, grad_body_qd)` takes the dict
+    `step_backward(..., adjoint_multipliers=True)` returned; `CheckpointedRollout.backward`
+    calls it with `adjoint_log=`."""
+
+    def __init__(self, num_steps, engine, dt):
+        self.engine, self.dt, self.num_steps = engine, dt, num_steps
+        W, dev = engine.num_worlds, engine.device
+        m = engine.model
+        B, D, SC = m.bodies_per_world, m.dofs_per_world, m.shapes_per_world + m.num_global_shapes
+        z = lambda *s, dt=torch.float32: torch.zeros(*s, dtype=dt, device=dev)
+        self.stage = dict(_w=z(W, engine.num_constraints), w_u=z(W * B, 6), body_pose_grad=z(W * B, 7),
+                          body_vel_grad=z(W * B, 6), body_pose_prev_grad=z(W * B, 7),
+                          body_vel_prev_grad=z(W * B, 6), ext_force_grad=z(W * B, 6), shape_mu_grad=z(W, SC),
+                          pcr_iter_count=z(W, dt=torch.int32))
+        if D:
+            self.stage.update(joint_target_pos_grad=z(W, D), joint_target_vel_grad=z(W, D))
+        self._stats = {k: z(W, dt=torch.int32) for k in ("nr_iters", "pcr_iters", "best_iter", "flags")}
+        self._stats["merit"] = z(W)
+        self.log = DeviceLog(num_steps, self.stage)
+
+    _FROM = dict(_w="w_lambda", w_u="w_u", body_pose_prev_grad="body_q", body_vel_prev_grad="body_qd",
+                 ext_force_grad="body_f", shape_mu_grad="shape_mu", joint_target_pos_grad="joint_target_pos",
+                 joint_target_vel_grad="joint_target_vel")
+
+    def capture_step(self, t, grads, grad_body_q, grad_body_qd):
+        """Logs the backward of step t (device copies; slot t, nothing past num_steps)."""
+        for k, src in self._FROM.items():
+            if k in self.stage:
+                if src not in grads:
+                    raise KeyError(f"{src} missing: call step_backward with adjoint_multipliers=True")
+                self.stage[k].copy_(grads[src].reshape(self.stage[k].shape))
+        self.stage["body_pose_grad"].copy_(grad_body_q.reshape(self.stage["body_pose_grad"].shape))
+        self.stage["body_vel_grad"].copy_(grad_body_qd.reshape(self.stage["body_vel_grad"].shape))
+        self.engine.stats(out=self._stats)
+        self.stage["pcr_iter_count"].copy_(self._stats["pcr_iters"])
+        self.log.step_index.fill_(int(t))
+        self.log.capture(advance=False)
+
+    def save(self, path):
+        save_log(path, self.num_steps, self.dt, self.log.numpy(), _model_arrays(self.engine.model), _dims(self.engine))
