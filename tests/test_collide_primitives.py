@@ -3,10 +3,10 @@
 The host collider is the reference the GPU collider is checked against; these cases fix
 its geometry where the answer is known exactly: a box resting on a plane touches it with
 its four lower corners at distance 0, a box tilted about one axis with the two corners of
-its lowest edge, a sphere on a
-plane or on a box face with the point below its centre, a cylinder lying on a plane with
-the lowest point of its middle circle, and the Newton conventions (normal from shape0 to
-shape1, point0 in the body frame, point1 in the frame of shape1's body or the world).
+its lowest edge, a sphere on a plane or on a box face with the point below its centre, a
+cylinder lying on a plane with the lowest point of its middle circle, and the Newton
+conventions (normal from shape0 to shape1, point0 in the body frame, point1 in the frame
+of shape1's body or the world).
 """
 import math
 
