@@ -367,7 +367,8 @@ int ofb_trajectory_adam_step(float* params, const float* grad, float* m, float* 
  * fused step fills with its per-Newton-iteration history: ||h||^2 after iteration k
  * (the reference's candidates_res_norm_sq, core/history_group.py buffers logged by
  * logging/sim_logger.py) and the PCR iterations of iteration k (pcr_history_iter_count);
- * iterations a world did not run hold NaN and -1.  Fused path only. */
+ * the first 32 iterations are recorded, later ones and iterations a world did not run
+ * hold NaN and -1.  Fused path only. */
 int ofb_set_iteration_history(ofb_solver* solver, float* merit, int32_t* pcr_iters, int32_t capacity);
 
 /* ---- device-resident step logs (core/history_group.py:12-80, logging/sim_logger.py,
