@@ -444,6 +444,8 @@ def main():
     if not args.no_adjoint:
         print("[bench] adjoint rollout", file=sys.stderr, flush=True)
         out["adjoint"] = adjoint_leg(device)
+        # the metric's second figure: the adjoint solve of one step (1024 worlds)
+        out["adjoint_ms"] = out["adjoint"]["adjoint_solve_ms_per_step"]
         print(f"[bench] adjoint: {out['adjoint']}", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
