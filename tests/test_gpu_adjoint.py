@@ -84,6 +84,16 @@ def test_adjoint_box_stack(oracle_mod, gpu_device):
     _compare(oracle_mod, model, cfg, st, control, col, gpu_device)
 
 
+def test_adjoint_tall_stack_full_residual_differences(oracle_mod, gpu_device):
+    # six-cube stacks: 66-72 rows per world (two row slots per lane), so the pose
+    # pull-back takes the full-residual central differences instead of the row-local ones
+    model, state, control = scenes.box_stack(8, n_cubes=6, seed=0)
+    cfg = config(K=48)
+    col = collide.PrimitiveCollider(model, gap=0.02, max_contacts_per_world=48)
+    st = _advance(oracle_mod, model, cfg, state, control, col, 2)
+    _compare(oracle_mod, model, cfg, st, control, col, gpu_device, seed=3)
+
+
 def test_adjoint_box_stack_block_jacobi(oracle_mod, gpu_device):
     model, state, control = scenes.box_stack(16, seed=0)
     cfg = config(K=16, precond="per_body_pair")
