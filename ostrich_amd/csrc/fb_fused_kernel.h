@@ -1063,7 +1063,7 @@ fused_step_kernel(FusedParams P) {
       msum += v * v;
     }
     merit = group_sum<GW>(msum);
-    if (lane == 0 && n_it < kHistIters) {  // ||h||^2 and the PCR iterations so far
+    if (P.hist_merit && lane == 0 && n_it < kHistIters) {  // ||h||^2 and the PCR iterations so far
       sm[P.lay.hist + n_it] = merit;
       sm[P.lay.hist + kHistIters + n_it] = __int_as_float(pcr_total);
     }
