@@ -2,17 +2,22 @@
 """World-steps/s of the Fisher-Burmeister Newton contact solver on one MI355X.
 
 Workload: the uneven-batch skid-steer scene - 4096 worlds, each a 10 kg chassis with three
-velocity-driven wheels, 0-24 static boxes, mu U(0.4,1.0), dt 0.05 s, Newton <= 20
-iterations to ||h|| <= 1e-5, PCR <= 16 iterations (Jacobi).  The headline variant seeds
-Newton with the cross-step contact warm starter ("fused + warm start"); the other
-variants (plain per-kernel path, cold start, block-Jacobi, learned warm start) are
-reported beside it.  A step is one full solver
-step (contact packing + Newton loop + backtracking) over all worlds; contacts are
-generated between steps by the primitive collider on the GPU and are not timed, as the
-reference times collision separately (profiling/engine_profiler.py:46-53).
+velocity-driven wheels among 0-24 static boxes (robots start among and against them),
+mu U(0.4,1.0), dt 0.05 s, Newton <= 20 iterations to ||h|| <= 1e-5, PCR <= 16 iterations
+(Jacobi), contacts within 0.1 m.  The timed window is the scene's 200-step run (after a
+few warm-up steps).  The headline variant seeds Newton with the cross-step contact warm
+starter ("fused + warm start") and replays each step from a HIP graph; the other variants
+(plain per-kernel path, cold start, block-Jacobi, learned warm start) are reported beside
+it.  A step is one full solver step (contact packing + Newton loop + backtracking) over
+all worlds; contacts are generated between steps by the primitive collider on the GPU and
+are not timed, as the reference times collision separately
+(profiling/engine_profiler.py:46-53).
 
 Prints ONE JSON line.  `value` is whole-job world-steps/s of the default path over the
-timed steps, computed from HIP events around each step on the solver's stream.
+timed steps (every world-step, converged or stopped at the Newton cap, as the metric's
+run-to-tolerance-or-cap definition counts them), computed from HIP events around each
+step on the solver's stream; `converged_world_steps_per_s` counts only the world-steps
+that reached ||h|| <= 1e-5.
 """
 import argparse
 import json
@@ -34,6 +39,10 @@ from ostrich_amd.containers import Contacts, Control, State  # noqa: E402
 DT = 0.05
 K_CONTACTS = 128
 HBM_PEAK_GBS = 8000.0
+# contact gap of the skid-steer scene: the reference's Helhest scenes generate contacts
+# within rigid_gap = 0.1-1.0 m (examples/helhest/gradient/*.py 0.1, obstacle_benchmark.py
+# 1.0), so worlds near obstacles carry many speculative contact rows
+C1_GAP = 0.1
 
 
 def solver_config(path, termination="per_world", precond="jacobi", seed="cold", paired=True, K=K_CONTACTS):
@@ -81,14 +90,53 @@ def contacts_of(collider, body_q, device):
     return Contacts.from_dict(collider.collide(body_q), device)
 
 
-def run_path(variant, model, state, control, steps, warmup, device, collider, K=K_CONTACTS, tune=False):
+def row_class(rows, pair_rows=32):
+    """Class code of worlds with `rows` active rows (tensor): 0 = paired (<= 32 rows), else
+    R = the smallest class of row slots per lane holding them (1 = dense, <= 64 rows)."""
+    R_LIST = (1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 16)
+    r = (rows + 63) // 64
+    code = torch.zeros_like(rows)
+    for R in reversed(R_LIST):
+        code = torch.where(r <= R, torch.full_like(rows, R), code)
+    return torch.where(rows <= pair_rows, torch.zeros_like(rows), code)
+
+
+def summarise(per_step, W, steps, step_ms, atol, nr_cap):
+    """Distribution figures of a variant's timed steps from per-step per-world arrays."""
+    nr = np.concatenate(per_step["nr"])
+    pcr = np.concatenate(per_step["pcr"])
+    merit = np.concatenate(per_step["merit"])
+    cnt = np.concatenate(per_step["cnt"])
+    conv = merit <= atol * atol
+    capped = (nr >= nr_cap) & ~conv
+    t = np.sum(step_ms) / 1e3
+    h_cap = np.sqrt(merit[capped]) if capped.any() else np.zeros(1)
+    hist_edges = [0, 3, 4, 5, 8, 16, 32, 64, 100, 128, 10 ** 9]
+    hist = np.histogram(cnt, bins=hist_edges)[0]
+    return dict(
+        world_steps_per_s=W * steps / t, converged_world_steps_per_s=float(conv.sum()) / t,
+        ms_per_step=float(np.mean(step_ms)),
+        nr_mean=float(nr.mean()), nr_p95=float(np.percentile(nr, 95)),
+        pcr_per_step_mean=float(pcr.mean()), pcr_per_step_p95=float(np.percentile(pcr, 95)),
+        pcr_per_nr=float(pcr.sum() / max(nr.sum(), 1)),
+        converged_fraction=float(conv.mean()), capped_fraction=float(capped.mean()),
+        capped_h={"median": float(np.median(h_cap)), "p90": float(np.percentile(h_cap, 90)),
+                  "max": float(h_cap.max())},
+        contacts_mean=float(cnt.mean()), contacts_p50=float(np.median(cnt)),
+        contacts_p95=float(np.percentile(cnt, 95)), contacts_max=int(cnt.max()), contacts_min=int(cnt.min()),
+        contacts_hist={f"{a}-{b - 1}" if b < 10 ** 9 else f">={a}": int(h) for a, b, h in
+                       zip(hist_edges[:-1], hist_edges[1:], hist)})
+
+
+def run_path(variant, model, state, control, steps, warmup, device, collider, K=K_CONTACTS, tune=False,
+             class_timing=False):
     """One solver variant "path:preconditioner[:seed[:option...]]" over warmup + steps
     closed-loop steps.  Options: "unpaired" (worlds of at most 32 rows one per wavefront),
     "graph" (each step is a replay of a HIP graph recorded over static tensors; the
     collider's contacts are copied into them before each replay, untimed).  tune: run the
     launch-shape autotuner on the first step's inputs before timing.  Step times are HIP
-    events on the stream around each step (or replay); kernel times are the paired-class
-    kernel's own events (eager steps only)."""
+    events on the stream around each step (or replay); with class_timing (eager fused
+    steps) every row-class launch is also timed by its own events."""
     from ostrich_amd import OstrichEngine
     parts = variant.split(":")
     path, precond = parts[0], parts[1]
@@ -116,11 +164,16 @@ def run_path(variant, model, state, control, steps, warmup, device, collider, K=
         launch = dict(waves_per_block=launch[0], pair_small_worlds=launch[1],
                       candidates_ms={f"{w}x{'paired' if p else 'single'}": round(t, 4) for (w, p), t in times.items()})
         print(f"[bench] autotune {variant}: {launch}", file=sys.stderr, flush=True)
+    class_timing = class_timing and path == "fused" and not graph
+    if class_timing:
+        eng.set_class_timing(True)
     g, con_static = None, None
     if graph:
         con_static = contacts_of(collider, s_a.body_q, device).clone()
         g = eng.capture(s_a, s_b, ctl, con_static, DT)
-    step_ms, solve_ms, kern_ms, nr, pcr, conv, ncon, lrn, pbytes, pops = ([] for _ in range(10))
+    step_ms = []
+    per = dict(nr=[], pcr=[], merit=[], cnt=[])
+    cls_ms, cls_worlds, cls_bytes = [], [], []
     for k in range(warmup + steps):
         con = contacts_of(collider, s_a.body_q, device)
         if graph:
@@ -136,39 +189,42 @@ def run_path(variant, model, state, control, steps, warmup, device, collider, K=
         e1.synchronize()
         if k >= warmup:
             step_ms.append(e0.elapsed_time(e1))
-            if not graph:
-                solve_ms.append(eng.last_solve_ms())
             st = eng.stats()
             cnt = eng.packed_contacts()["count"].clamp(max=K)
-            nr.append(st["nr_iters"].float().mean().item())
-            pcr.append(st["pcr_iters"].float().mean().item())
-            conv.append((st["merit"] <= cfg.nr.atol ** 2).float().mean().item())
-            ncon.append(cnt.float().mean().item())
-            lrn.append(((st["flags"] & 32) != 0).float().mean().item())
-            if path == "fused" and not graph and eng.fused_launch()[1]:
-                kern_ms.append(eng.last_kernel_ms())
+            per["nr"].append(st["nr_iters"].cpu().numpy())
+            per["pcr"].append(st["pcr_iters"].cpu().numpy())
+            per["merit"].append(st["merit"].cpu().numpy())
+            per["cnt"].append(cnt.cpu().numpy())
+            if class_timing:
+                ct = eng.class_times()
                 rows = base_rows + 3 * cnt
-                sel = rows <= 32  # the paired class
-                nc = cnt[sel].double()
-                pbytes.append(float(compulsory_bytes_per_world(model, nc, eng.num_constraints).sum()))
-                pops.append(float(operand_bytes_per_world(rows[sel].double(), st["nr_iters"][sel].double(),
-                                                          st["pcr_iters"][sel].double()).sum()))
+                code = row_class(rows, 32 if paired else -1)
+                by = compulsory_bytes_per_world(model, cnt.double(), eng.num_constraints)
+                cls_ms.append(ct["ms"])
+                cls_worlds.append(ct["worlds"])
+                cls_bytes.append([float(by[code == R].sum()) for R in (0, 1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 16)])
         if graph:
             s_a.copy_(s_b)  # the graph reads s_a and writes s_b
         else:
             s_a, s_b = s_b, s_a
     torch.cuda.synchronize()
-    out = dict(step_ms=float(np.sum(step_ms)), steps=steps,
-               world_steps_per_s=W * steps / (np.sum(step_ms) / 1e3),
-               ms_per_step=float(np.mean(step_ms)),
-               nr_mean=float(np.mean(nr)), pcr_per_step_mean=float(np.mean(pcr)),
-               converged_fraction=float(np.mean(conv)), contacts_mean=float(np.mean(ncon)),
-               learned_seed_fraction=float(np.mean(lrn)))
-    if solve_ms:
-        out.update(solve_ms=float(np.sum(solve_ms)), kernel_ms=float(np.mean(solve_ms)))
-    if kern_ms:
-        out.update(paired_kernel_ms=float(np.mean(kern_ms)), paired_bytes=float(np.mean(pbytes)),
-                   paired_operand_bytes=float(np.mean(pops)))
+    out = dict(steps=steps, warmup=warmup, variant=variant)
+    out.update(summarise(per, W, steps, step_ms, cfg.nr.atol, cfg.nr.max_iters))
+    out["step_ms_first20"] = float(np.mean(step_ms[:20]))
+    if class_timing and cls_ms:
+        ms = np.array(cls_ms)
+        wk = np.array(cls_worlds)
+        by = np.array(cls_bytes)
+        classes = []
+        caps = eng.class_times()["rows_cap"]
+        for c in range(ms.shape[1]):
+            timed = ms[:, c] >= 0
+            if not timed.any() and wk[:, c].max() == 0:
+                continue
+            classes.append(dict(rows_cap=int(caps[c]), worlds_mean=float(wk[:, c].mean()),
+                                ms_mean=float(ms[timed, c].mean()) if timed.any() else None,
+                                bytes_mean=float(by[:, c].mean())))
+        out["classes"] = classes
     if launch is not None:
         out["launch"] = launch
     return out
@@ -183,7 +239,7 @@ def full_step_leg(model, state, control, device, steps, warmup, launch=None):
     eng = OstrichEngine(model, config=cfg, device=device)
     if launch is not None:
         eng.set_fused_launch(launch["waves_per_block"], launch["pair_small_worlds"])
-    col = collide.HipCollider(model, device=device, gap=0.02, max_contacts_per_world=K_CONTACTS)
+    col = collide.HipCollider(model, device=device, gap=C1_GAP, max_contacts_per_world=K_CONTACTS)
     s_a = State.from_numpy(state, device)
     s_b = s_a.clone()
     ctl = Control.from_numpy(control, device)
@@ -305,14 +361,16 @@ def adjoint_leg(device, worlds=1024, steps=50, checkpoint_every=10, seed=3):
                      "adjoint solve of one step (HIP events on its stream)")
 
 
-def cpu_baseline(model_fn, sample_worlds, sample_steps, precond="jacobi", seed="prev"):
-    """The oracle (FP32 CPU restatement of the solve, one thread; FP64 warm starter) on the
-    first `sample_worlds` worlds of the same seeded scene, with the same Newton seeding as
-    the headline; warm start + solver calls are timed, collision is not."""
+def cpu_baseline(model_fn, sample_worlds, sample_steps, precond="jacobi", seed="prev", threads=1, gap=C1_GAP):
+    """The oracle (FP32 CPU restatement of the solve; FP64 warm starter) on the first
+    `sample_worlds` worlds of the same seeded scene, with the same Newton seeding as the
+    headline, on `threads` host threads (OpenMP over worlds); warm start + solver calls
+    are timed, collision is not."""
     from oracle import oracle
+    used = oracle.set_threads(threads)
     model, state, control = model_fn(sample_worlds)
     cfg = solver_config("fused", precond=precond, seed=seed)
-    col = collide.PrimitiveCollider(model, device="cpu", gap=0.02, max_contacts_per_world=K_CONTACTS)
+    col = collide.PrimitiveCollider(model, device="cpu", gap=gap, max_contacts_per_world=K_CONTACTS)
     hm = oracle.HostModel(model)
     ws = oracle.WarmStarter(model, cfg) if seed != "cold" else None
     st = {k: np.asarray(v, np.float64) for k, v in state.items()}
@@ -327,25 +385,121 @@ def cpu_baseline(model_fn, sample_worlds, sample_steps, precond="jacobi", seed="
             ws.snapshot(st, hc, DT, o["lam"])
         t += time.perf_counter() - t0
         st["body_q"], st["body_qd"] = o["body_q"], o["body_qd"]
-    return dict(value=sample_worlds * sample_steps / t, unit="world-steps/s", cores=1, kind="port",
-                sample=f"oracle FP32 (C++ CPU restatement, 1 thread), first {sample_worlds} worlds of the "
-                       f"same seeded scene, {sample_steps} closed-loop steps, Newton seed '{seed}', solver and "
-                       f"warm-start calls only ({t:.1f} s)")
+    return dict(value=sample_worlds * sample_steps / t, unit="world-steps/s", cores=used, kind="port",
+                sample=f"oracle FP32 (C++ CPU restatement, {used} thread{'s' if used > 1 else ''}), first "
+                       f"{sample_worlds} worlds of the same seeded scene, {sample_steps} closed-loop steps, Newton "
+                       f"seed '{seed}', solver and warm-start calls only ({t:.1f} s)")
+
+
+def host_cpu():
+    """Host CPU model and logical core count (lscpu / nproc where available)."""
+    info = dict(nproc=os.cpu_count())
+    try:
+        import subprocess
+        info["nproc"] = int(subprocess.run(["nproc"], capture_output=True, text=True).stdout.strip())
+        for line in subprocess.run(["lscpu"], capture_output=True, text=True).stdout.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core", "CPU(s)"):
+                info[k.strip()] = v.strip()
+    except (OSError, ValueError):
+        pass
+    return info
+
+
+def fp64_error_leg(model_fn, worlds, lead_steps, device, variant="fused:jacobi:prev"):
+    """Error of one GPU step against the FP64 oracle on the same inputs.  The first
+    `worlds` worlds of the scene are stepped on the GPU for `lead_steps` closed-loop steps;
+    the next step's inputs (state, contacts, the warm starter's Newton seed) are then
+    solved by the GPU, the FP32 oracle and the FP64 oracle.  Reports per-world maximum
+    pose / velocity errors of the GPU and of the FP32 oracle against FP64 (the FP32
+    rounding spread of this system), over all worlds and over the worlds that all three
+    solves converge, and the Newton-count agreement."""
+    from oracle import oracle
+    from ostrich_amd import OstrichEngine
+    model, state, control = model_fn(worlds)
+    parts = variant.split(":")
+    cfg = solver_config(parts[0], precond=parts[1], seed=parts[2])
+    eng = OstrichEngine(model, config=cfg, device=device)
+    col = collide.HipCollider(model, device=device, gap=C1_GAP, max_contacts_per_world=K_CONTACTS)
+    s_a = State.from_numpy(state, device)
+    s_b = s_a.clone()
+    ctl = Control.from_numpy(control, device)
+    for _ in range(lead_steps):
+        eng.step(s_a, s_b, ctl, col.collide(s_a.body_q), DT)
+        s_a, s_b = s_b, s_a
+    con = col.collide(s_a.body_q).clone()
+    st_in = {k: getattr(s_a, k).cpu().numpy() for k in ("body_q", "body_qd", "body_f")}
+    eng.step(s_a, s_b, ctl, con, DT)
+    seed = eng.lambda_seed().cpu().numpy()
+    stats = {k: v.cpu().numpy() for k, v in eng.stats().items()}
+    q_gpu, u_gpu = s_b.body_q.cpu().numpy(), s_b.body_qd.cpu().numpy()
+    c = con.to_dict() if hasattr(con, "to_dict") else {
+        "count": con.rigid_contact_count, "point0": con.rigid_contact_point0, "point1": con.rigid_contact_point1,
+        "normal": con.rigid_contact_normal, "shape0": con.rigid_contact_shape0, "shape1": con.rigid_contact_shape1,
+        "margin0": con.rigid_contact_margin0, "margin1": con.rigid_contact_margin1,
+        "max_contacts": con.rigid_contact_point0.shape[0]}
+    hc = oracle.HostContacts(c)
+    o32 = oracle.step(model, cfg, st_in, control, hc, DT, precision=32, lam_seed=seed)
+    o64 = oracle.step(model, cfg, st_in, control, hc, DT, precision=64, lam_seed=seed)
+    B = model.bodies_per_world
+    def per_world(a, b, cols):
+        return np.abs(a.reshape(worlds, B, -1)[..., cols] - b.reshape(worlds, B, -1)[..., cols]).max(axis=(1, 2))
+    gq, gu = per_world(q_gpu, o64["body_q"], slice(0, 3)), per_world(u_gpu, o64["body_qd"], slice(0, 6))
+    sq, su = per_world(o32["body_q"], o64["body_q"], slice(0, 3)), per_world(o32["body_qd"], o64["body_qd"], slice(0, 6))
+    atol_sq = cfg.nr.atol ** 2
+    conv = (stats["merit"] <= atol_sq) & (o32["merit"] <= atol_sq) & (o64["merit"] <= atol_sq)
+    def dist(x, m=None):
+        x = x if m is None else x[m]
+        if x.size == 0:
+            return None
+        return {"median": float(np.median(x)), "p95": float(np.percentile(x, 95)), "max": float(x.max())}
+    return dict(worlds=worlds, step=lead_steps, variant=variant, converged_all_three=int(conv.sum()),
+                gpu_vs_fp64_pos_m=dist(gq), gpu_vs_fp64_vel_mps=dist(gu),
+                fp32_oracle_vs_fp64_pos_m=dist(sq), fp32_oracle_vs_fp64_vel_mps=dist(su),
+                gpu_vs_fp64_pos_m_converged=dist(gq, conv), gpu_vs_fp64_vel_mps_converged=dist(gu, conv),
+                fp32_oracle_vs_fp64_vel_mps_converged=dist(su, conv),
+                nr_within_1_of_fp64=float(np.mean(np.abs(stats["nr_iters"] - o64["nr_iters"]) <= 1)),
+                nr_within_1_of_fp64_converged=float(np.mean(np.abs(stats["nr_iters"] - o64["nr_iters"])[conv] <= 1))
+                if conv.any() else None)
+
+
+METRIC = "world-steps/s at FB residual\u22641e-5 (plain vs fused+warm-start); adjoint ms"
+
+
+def dominant_class_roofline(res):
+    """Roofline of the eager variant's dominant row-class launch (largest mean duration):
+    the compulsory HBM bytes of the worlds it solved per launch over its HIP-event time."""
+    classes = [c for c in res.get("classes", []) if c.get("ms_mean")]
+    if not classes:
+        return None
+    c = max(classes, key=lambda x: x["ms_mean"] * (x["worlds_mean"] > 0))
+    achieved = c["bytes_mean"] / (c["ms_mean"] / 1e3) / 1e9
+    kind = ("fused_step_kernel<1, *, 32, true> (worlds of at most 32 rows, two per wavefront)" if c["rows_cap"] == 32
+            else f"fused_step_kernel<1, *, 64, true> (worlds of at most 64 rows, dense Schur matrix)"
+            if c["rows_cap"] == 64 else
+            f"fused_step_kernel<{c['rows_cap'] // 64}, *, 64, false> (worlds of at most {c['rows_cap']} rows, "
+            "matrix-free operator)")
+    return dict(achieved=achieved, frac=achieved / HBM_PEAK_GBS, kernel=kind, kernel_ms=c["ms_mean"],
+                worlds_per_launch=c["worlds_mean"], bytes_per_launch=c["bytes_mean"])
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=200, help="timed steps (the scene's run is 200 steps)")
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--worlds", type=int, default=4096)
     ap.add_argument("--paths",
                     default=f"fused:jacobi:prev,{GRAPH_PATH},fused:jacobi:prev:unpaired,fused:jacobi:cold,"
-                            "plain:jacobi:cold,fused:per_body_pair:cold,fused:jacobi:learned",
+                            "plain:jacobi:cold,fused:per_body_pair:prev,fused:jacobi:learned",
                     help="comma-separated path:preconditioner[:seed[:option]] variants; the headline is "
                          f"{GRAPH_PATH} when listed, else the first")
+    ap.add_argument("--plain-steps", type=int, default=20,
+                    help="timed steps of the plain per-kernel path (the first of the same run)")
     ap.add_argument("--cpu-sample-worlds", type=int, default=128)
-    ap.add_argument("--cpu-sample-steps", type=int, default=120)
+    ap.add_argument("--cpu-sample-steps", type=int, default=40)
+    ap.add_argument("--cpu-all-core-worlds", type=int, default=1024)
+    ap.add_argument("--fp64-worlds", type=int, default=128)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-scenes", action="store_true", help="skip the tracked-robot and box-stack legs")
@@ -360,47 +514,47 @@ def main():
     device = "cuda:0"
     W = args.worlds
     model, state, control = scenes.skid_steer(W, seed=1)
-    collider = collide.HipCollider(model, device=device, gap=0.02, max_contacts_per_world=K_CONTACTS)
+    collider = collide.HipCollider(model, device=device, gap=C1_GAP, max_contacts_per_world=K_CONTACTS)
     paths = [p for p in args.paths.split(",") if p]
     results = {}
     for p in paths:
-        print(f"[bench] path={p} worlds={W} warmup={args.warmup} steps={args.steps}", file=sys.stderr, flush=True)
+        steps = min(args.steps, args.plain_steps) if p.startswith("plain") else args.steps
+        print(f"[bench] path={p} worlds={W} warmup={args.warmup} steps={steps}", file=sys.stderr, flush=True)
         try:
-            results[p] = run_path(p, model, state, control, args.steps, args.warmup, device, collider,
-                                  tune=not args.no_tune and p in (paths[0], GRAPH_PATH))
+            results[p] = run_path(p, model, state, control, steps, args.warmup, device, collider,
+                                  tune=not args.no_tune and p in (paths[0], GRAPH_PATH),
+                                  class_timing=p == paths[0])
         except RuntimeError as e:
             results[p] = dict(error=str(e))
         print(f"[bench] {p}: {results[p]}", file=sys.stderr, flush=True)
     # headline: the autotuned fused step replayed from a HIP graph when that variant ran,
-    # else the first variant; the roofline comes from the eager variant's paired-class
-    # kernel events (a graph records no events)
+    # else the first variant; the roofline comes from the eager variant's class events
+    # (a graph records no events)
     eager_path = paths[0]
     main_path = GRAPH_PATH if "world_steps_per_s" in results.get(GRAPH_PATH, {}) else eager_path
     r = results[main_path]
     re_ = results[eager_path]
-    n_c = r["contacts_mean"]
-    lds_peak = 256 * 128 * 2.4e9 / 1e9  # 256 CUs x 128 B/clk x 2.4 GHz
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "fused_step_kernel_traffic.json")
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            t = json.load(f)
-        traffic = t["fetch_bytes"] + t["write_bytes"]
-    roof = {"bound": "hbm", "peak": HBM_PEAK_GBS, "unit": "GB/s", "traffic": traffic}
-    if "paired_kernel_ms" in re_:
-        kms = re_["paired_kernel_ms"]
-        achieved = re_["paired_bytes"] / (kms / 1e3) / 1e9
-        roof.update(achieved=achieved, frac=achieved / HBM_PEAK_GBS,
-                    kernel="fused_step_kernel<1, false, 32, true> (worlds of at most 32 rows, two per wavefront)",
-                    kernel_ms=kms, bytes_per_launch=re_["paired_bytes"],
-                    traffic_note="HBM bytes per launch of that kernel from rocprofv3 FETCH_SIZE + WRITE_SIZE "
-                                 "scaled by a 256 MiB copy (profiles/fused_step_kernel_skid_steer_4096.md)",
-                    note="compulsory HBM bytes of the class's worlds per launch; the kernel is bound by "
-                         "instruction issue and LDS latency of one wave per two worlds, not by HBM",
-                    operand_bytes_per_launch=re_["paired_operand_bytes"],
-                    operand_GBs=re_["paired_operand_bytes"] / (kms / 1e3) / 1e9, lds_peak_GBs=lds_peak)
+    roof = {"bound": "hbm", "peak": HBM_PEAK_GBS, "unit": "GB/s", "traffic": None}
+    dom = dominant_class_roofline(re_)
+    if dom:
+        roof.update(dom)
+        tpath = os.path.join(ROOT, "profiles", "fused_step_traffic_skid_steer_4096.json")
+        if os.path.exists(tpath):
+            with open(tpath) as f:
+                t = json.load(f)
+            if t.get("rows_cap") == dom_rows(dom):
+                roof["traffic"] = t["fetch_bytes"] + t["write_bytes"]
+                roof["traffic_source"] = ("not measured in this run: HBM bytes per launch of that kernel from "
+                                          "rocprofv3 FETCH_SIZE + WRITE_SIZE scaled by a 256 MiB copy, "
+                                          f"{os.path.relpath(tpath, ROOT)}")
+        roof["note"] = ("achieved = compulsory HBM bytes of the class's worlds per launch over the launch's "
+                        "HIP-event duration; the kernel is bound by instruction issue and LDS latency of one "
+                        "wavefront per world (two per wavefront in the paired class), not by HBM")
+    short = {k: r[k] for k in ("nr_mean", "nr_p95", "pcr_per_step_mean", "pcr_per_step_p95", "pcr_per_nr",
+                               "converged_fraction", "capped_fraction", "capped_h", "contacts_mean", "contacts_p50",
+                               "contacts_p95", "contacts_max", "contacts_min", "contacts_hist") if k in r}
     out = {
-        "metric": "world-steps/s at FB residual<=1e-5 (fused persistent solver + cross-step warm start)",
+        "metric": METRIC,
         "value": r["world_steps_per_s"],
         "unit": "world-steps/s",
         "n_gpus": 1,
@@ -412,33 +566,40 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded skid-steer scene, contacts from the HIP primitive collider)",
-        "config": {"workload": "uneven-batch skid-steer, 4096 worlds, dt 0.05, NR<=20 to ||h||<=1e-5, "
-                               "PCR<=16 (Jacobi, the library default), 128 contact slots, Newton seeded by "
-                               "the previous step's contacts", "worlds": W,
+        "config": {"workload": f"uneven-batch skid-steer, {W} worlds among 0-24 static boxes (robots start among "
+                               "and against them), contacts within 0.1 m, dt 0.05, 200-step run, NR<=20 to "
+                               "||h||<=1e-5 or the cap, PCR<=16 (Jacobi, the library default), 128 contact slots, "
+                               "Newton seeded by the previous step's contacts", "worlds": W,
                    "dt": DT, "variant": main_path, "termination": "per_world",
                    "launch": r.get("launch", re_.get("launch"))},
+        "converged_world_steps_per_s": r["converged_world_steps_per_s"],
         "roofline": roof,
-        "solver": {"nr_mean": r["nr_mean"], "pcr_per_step_mean": r["pcr_per_step_mean"],
-                   "converged_fraction": r["converged_fraction"], "contacts_mean": n_c},
+        "solver": short,
+        "classes": re_.get("classes"),
         "paths": {p: results[p] for p in paths},
     }
     fj, pj = results.get("fused:jacobi:cold", {}), results.get("plain:jacobi:cold", {})
-    if "world_steps_per_s" in fj and "world_steps_per_s" in pj:
-        out["fused_over_plain_cold"] = fj["world_steps_per_s"] / pj["world_steps_per_s"]
+    if "step_ms_first20" in fj and "ms_per_step" in pj:
+        # same window: the plain path's timed steps are the first of the run
+        out["fused_over_plain_cold"] = pj["ms_per_step"] / fj["step_ms_first20"]
     fw = results.get(main_path, {})
-    if "world_steps_per_s" in fw and "world_steps_per_s" in pj:
-        out["fused_warm_over_plain_cold"] = fw["world_steps_per_s"] / pj["world_steps_per_s"]
+    if "step_ms_first20" in fw and "ms_per_step" in pj:
+        out["fused_warm_over_plain_cold"] = pj["ms_per_step"] / fw["step_ms_first20"]
     print("[bench] full simulation step (collide + solve, one graph)", file=sys.stderr, flush=True)
     out["full_step"] = full_step_leg(model, state, control, device, args.steps, args.warmup,
                                      launch=r.get("launch", re_.get("launch")))
     print(f"[bench] full_step: {out['full_step']}", file=sys.stderr, flush=True)
+    print("[bench] fp64 error", file=sys.stderr, flush=True)
+    out["fp64_error"] = fp64_error_leg(lambda n: scenes.skid_steer(n, seed=1), args.fp64_worlds, 40, device)
+    print(f"[bench] fp64_error: {out['fp64_error']}", file=sys.stderr, flush=True)
     if not args.no_scenes:
         # tracked flipper robots on a Gaussian-bump heightfield (1024 worlds, anisotropic
         # track friction, position-controlled flippers) and 3-cube box stacks (16 worlds)
+        sc_steps = min(args.steps, 40)
         out["tracked_flipper"] = scene_leg("tracked_flipper", lambda n: scenes.tracked_flipper(n, seed=2), 1024, 64,
-                                           args.steps, max(args.warmup, 6), device,
+                                           sc_steps, max(args.warmup, 6), device,
                                            ("fused:jacobi:prev", "fused:jacobi:cold"))
-        out["box_stack"] = scene_leg("box_stack", lambda n: scenes.box_stack(n, 3, seed=0), 16, 16, args.steps,
+        out["box_stack"] = scene_leg("box_stack", lambda n: scenes.box_stack(n, 3, seed=0), 16, 16, sc_steps,
                                      args.warmup, device, ("fused:jacobi:prev", "fused:jacobi:cold",
                                                            "plain:jacobi:cold"))
     if not args.no_adjoint:
@@ -450,15 +611,32 @@ def main():
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         mp = main_path.split(":")
-        out["cpu_baseline"] = cpu_baseline(lambda n: scenes.skid_steer(n, seed=1), args.cpu_sample_worlds,
-                                           args.cpu_sample_steps, precond=mp[1],
-                                           seed=mp[2] if len(mp) > 2 and mp[2] != "learned" else "cold")
+        seed = mp[2] if len(mp) > 2 and mp[2] != "learned" else "cold"
+        fn = lambda n: scenes.skid_steer(n, seed=1)
+        out["cpu_baseline"] = cpu_baseline(fn, args.cpu_sample_worlds, args.cpu_sample_steps, precond=mp[1],
+                                           seed=seed, threads=1)
+        print(f"[bench] cpu_baseline: {out['cpu_baseline']}", file=sys.stderr, flush=True)
+        host = host_cpu()
+        out["cpu_baseline_all_cores"] = cpu_baseline(fn, args.cpu_all_core_worlds, args.cpu_sample_steps,
+                                                     precond=mp[1], seed=seed, threads=min(16, host["nproc"] or 1))
+        out["cpu_baseline_all_cores"]["host"] = host
+        print(f"[bench] cpu_baseline_all_cores: {out['cpu_baseline_all_cores']}", file=sys.stderr, flush=True)
     if args.copy_calibration:
         src = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device=device)
         dst = torch.empty_like(src)
         dst.copy_(src)
         torch.cuda.synchronize()
     print(json.dumps(out))
+
+
+def dom_rows(dom):
+    """Row capacity of the kernel a roofline entry names."""
+    k = dom["kernel"]
+    if "two per wavefront" in k:
+        return 32
+    if "dense Schur" in k:
+        return 64
+    return int(k.split("at most ")[1].split(" rows")[0])
 
 
 if __name__ == "__main__":

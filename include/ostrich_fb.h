@@ -345,6 +345,18 @@ int ofb_last_solve_ms(const ofb_solver* solver, float* ms);
  * step timing is recorded (ofb_last_solve_ms also returns -2). */
 int ofb_last_kernel_ms(const ofb_solver* solver, float* ms);
 
+/* Per-class timing of the fused path (no reference counterpart; measurement hook).  With
+ * timing on, every eager step records HIP events around each row-class launch on the
+ * stream it runs on (the paired class on the step's stream, the others on the solver's
+ * side stream, where they run one after another beside the paired class).
+ * ofb_get_class_times fills, for each of the `count` classes (capacity >= 12), its row
+ * capacity, the number of worlds the last step put in it and its duration in ms (-1 when
+ * the class was not launched or the step was not timed); it blocks until the step has
+ * finished.  -2 on the plain path. */
+int ofb_set_class_timing(ofb_solver* solver, int32_t enabled);
+int ofb_get_class_times(const ofb_solver* solver, int32_t capacity, int32_t* rows_cap, int32_t* worlds, float* ms,
+                        int32_t* count);
+
 /* Launch settings of the fused path: the largest number of wavefronts per workgroup of a
  * row-class launch (1..16; the LDS limit may lower it per class) and whether worlds of at
  * most 32 rows run two per wavefront.  No reference counterpart (Warp chooses its own

@@ -23,6 +23,8 @@
 #include <limits>
 #include <vector>
 
+#include <omp.h>
+
 namespace {
 
 // ----------------------------------------------------------------------------------------
@@ -1340,13 +1342,9 @@ template <class T> int run_step(const ofb_model* m, const ofb_config* cfg, const
   const T atol_sq = T(cfg->nr_atol) * T(cfg->nr_atol);
   const std::vector<float> ls_steps = cfg->linesearch ? linesearch_step_list(*cfg) : std::vector<float>();
   std::vector<char> running(W, 1);
-  int iter = 0;
-  std::vector<int> all(W);
-  for (int w = 0; w < W; ++w) all[w] = w;
-  while (true) {  // _solve (base_engine.py:416-526)
-    std::vector<int> act;
-    for (int w = 0; w < W; ++w) if (global || running[w]) act.push_back(w);
-    if (act.empty()) break;
+  // one Newton iteration (_solve, base_engine.py:416-526) of the worlds in `act`;
+  // returns whether any of them is above the tolerance
+  auto iterate = [&](const std::vector<int>& act) {
     for (int w : act) {
       for (int i = 0; i < Bt.Nc; ++i) Bt.lam_prev[Bt.ri(w, i)] = Bt.lam[Bt.ri(w, i)];
       linear_system(Bt, w);
@@ -1370,8 +1368,8 @@ template <class T> int run_step(const ofb_model* m, const ofb_config* cfg, const
       save_candidate(Bt, w, Bt.nr_iters[w]);
       Bt.nr_iters[w] += 1;
       bool unconv = Bt.merit[w] > atol_sq;
-      if (global) any |= unconv;
-      else {
+      any |= unconv;
+      if (!global) {
         int it = Bt.nr_iters[w];
         bool keep = unconv;
         if (it < cfg->nr_backtrack_min_iter) keep = true;
@@ -1379,12 +1377,27 @@ template <class T> int run_step(const ofb_model* m, const ofb_config* cfg, const
         running[w] = keep ? 1 : 0;
       }
     }
-    if (global) {
+    return any;
+  };
+  if (global) {  // every world iterates until none is above the tolerance
+    std::vector<int> all(W);
+    for (int w = 0; w < W; ++w) all[w] = w;
+    int iter = 0;
+    while (true) {
+      bool keep = iterate(all);
       ++iter;
-      bool keep = any;
       if (iter < cfg->nr_backtrack_min_iter) keep = true;
       if (iter >= cfg->nr_max_iters) keep = false;
       if (!keep) break;
+    }
+  } else {
+    // per-world termination: each world's loop touches only its own rows and bodies, so
+    // worlds run independently (OpenMP over worlds; the arithmetic of a world does not
+    // depend on the thread count)
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int w = 0; w < W; ++w) {
+      const std::vector<int> one(1, w);
+      while (running[w]) iterate(one);
     }
   }
   for (int w = 0; w < W; ++w) {
@@ -1823,6 +1836,12 @@ template <class T> int run_pack(const ofb_model* m, const ofb_config* cfg, const
 }
 
 extern "C" {
+
+// threads of the per-world loops (OpenMP); returns the number in effect
+int oracle_set_threads(int n) {
+  if (n > 0) omp_set_num_threads(n);
+  return omp_get_max_threads();
+}
 
 // Number of constraint rows of the model (core/engine_dims.py:56-61).
 int oracle_dims(const ofb_model* m, const ofb_config* cfg, int32_t* n_c, int32_t* n_j, int32_t* n_ctrl) {

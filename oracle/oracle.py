@@ -43,12 +43,20 @@ def lib():
         _lib.oracle_linesearch_steps.restype = C.c_int
         _lib.oracle_step.restype = C.c_int
         _lib.oracle_dims.argtypes = [vp, vp, vp, vp, vp]
+        _lib.oracle_set_threads.argtypes = [C.c_int]
+        _lib.oracle_set_threads.restype = C.c_int
         _lib.oracle_fb_diff.argtypes = [C.c_double] * 5 + [vp]
         _lib.oracle_integrate_pose.argtypes = [vp, vp, vp, C.c_double, vp]
         _lib.oracle_linear_system.argtypes = [vp] * 8 + [C.c_float] + [vp] * 8
         _lib.oracle_pack.argtypes = [vp] * 4 + [C.c_int] + [vp] * 8
         _lib.oracle_pack.restype = C.c_int
     return _lib
+
+
+def set_threads(n):
+    """Threads of the oracle's per-world loops (OpenMP); n <= 0 keeps the current
+    setting.  Returns the number in effect."""
+    return lib().oracle_set_threads(int(n))
 
 
 def _p(a):

@@ -17,7 +17,7 @@ EXPORTS = (
     "ofb_set_warm_start_net", "ofb_get_contact_features", "ofb_get_linesearch_steps",
     "ofb_last_kernel_ms", "ofb_set_fused_launch", "ofb_get_fused_launch", "ofb_trajectory_adam_step",
     "ofb_collider_create", "ofb_collider_destroy", "ofb_collide", "ofb_set_iteration_history",
-    "ofb_log_create", "ofb_log_capture", "ofb_log_destroy",
+    "ofb_log_create", "ofb_log_capture", "ofb_log_destroy", "ofb_set_class_timing", "ofb_get_class_times",
 )
 
 _lib = None
@@ -42,6 +42,8 @@ def load():
     lib.ofb_last_solve_ms.argtypes = [vp, C.POINTER(C.c_float)]
     lib.ofb_last_kernel_ms.argtypes = [vp, C.POINTER(C.c_float)]
     lib.ofb_set_fused_launch.argtypes = [vp, i32, i32]
+    lib.ofb_set_class_timing.argtypes = [vp, i32]
+    lib.ofb_get_class_times.argtypes = [vp, i32, vp, vp, vp, vp]
     lib.ofb_get_fused_launch.argtypes = [vp, vp, vp]
     f32 = C.c_float
     lib.ofb_trajectory_adam_step.argtypes = [vp, vp, vp, vp] + [i32] * 5 + [f32] * 5 + [i32, vp]

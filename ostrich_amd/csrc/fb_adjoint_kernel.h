@@ -271,8 +271,10 @@ __global__ void __launch_bounds__(256) adjoint_step_kernel(AdjParams P) {
   }
   const float tol_sq = P.cfg.lin_tol >= 0.f ? P.cfg.lin_tol * P.cfg.lin_tol : -1.f;
   const float latol_sq = P.cfg.lin_atol >= 0.f ? P.cfg.lin_atol * P.cfg.lin_atol : -1.f;
-  pcr_it = pcr_solve<R, BJ>(sm, lane, B, V, G, act, b0, b1, J0, J1, Cr, Pi, rhs, bsq, tol_sq, latol_sq,
-                            P.cfg.lin_max_iters, x);
+  // the operator's body sums use direct LDS atomics here (BodySum without private copies)
+  const BodySum<WAVE, false> S{sm, nullptr, lane, B, false};
+  pcr_it = pcr_solve<R, BJ, WAVE, false>(sm, lane, B, n_r, S, V, G, act, b0, b1, J0, J1, Cr, Pi, rhs, bsq, tol_sq,
+                                         latol_sq, P.cfg.lin_max_iters, x);
 
   // ---- 4. w_u = w_u0 - M^-1 J^T w_lambda ----
   for (int e = lane; e < 6 * B; e += WAVE) sm[(e / 6) * BS + BACC + (e % 6)] = 0.f;

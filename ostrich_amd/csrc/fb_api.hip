@@ -261,13 +261,7 @@ int ofb_destroy(ofb_solver* s) {
     (void)hipEventDestroy(s->ev_solve0); (void)hipEventDestroy(s->ev_solve1);
     (void)hipEventDestroy(s->ev_kernel0); (void)hipEventDestroy(s->ev_kernel1);
   }
-  if (s->side_ready) {
-    (void)hipEventDestroy(s->ev_fork);
-    (void)hipEventDestroy(s->ev_join);
-    (void)hipStreamDestroy(s->side_stream);
-  }
-  if (s->cls_counts) (void)hipFree(s->cls_counts);
-  if (s->cls_lists) (void)hipFree(s->cls_lists);
+  fused_classes_free(s);
   delete s;
   return 0;
 }
@@ -599,6 +593,21 @@ int ofb_last_kernel_ms(const ofb_solver* s, float* ms) {
   OFB_CHECK(hipEventSynchronize(s->ev_kernel1));
   OFB_CHECK(hipEventElapsedTime(ms, s->ev_kernel0, s->ev_kernel1));
   return 0;
+}
+
+int ofb_set_class_timing(ofb_solver* s, int32_t enabled) {
+  if (!s) { set_error("null solver"); return -1; }
+  if (s->path != OFB_PATH_FUSED) { set_error("class timing applies to the fused path only"); return -2; }
+  s->cls_timing = enabled ? 1 : 0;
+  s->cls_timed = 0;
+  return 0;
+}
+
+int ofb_get_class_times(const ofb_solver* s, int32_t capacity, int32_t* rows_cap, int32_t* worlds, float* ms,
+                        int32_t* count) {
+  if (!s || !count) { set_error("null argument"); return -1; }
+  if (s->path != OFB_PATH_FUSED) { set_error("class timing applies to the fused path only"); return -2; }
+  return fused_class_times(s, capacity, rows_cap, worlds, ms, count);
 }
 
 int ofb_set_fused_launch(ofb_solver* s, int32_t waves_per_block, int32_t pair_small_worlds) {

@@ -14,6 +14,9 @@ namespace ofb {
 // 0 and run two per wavefront.  Worlds are bucketed on the device every step.
 constexpr int kNumClasses = 12;
 constexpr int kPairRows = 32;
+constexpr int kSideStreams = 3;
+// side stream of a one-world-per-wave class: the dense class, the R = 2 class, the rest
+static int side_of(int c) { return c <= 1 ? 0 : (c == 2 ? 1 : 2); }
 __constant__ int kClassR[kNumClasses] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 16};
 static const int kClassRHost[kNumClasses] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 16};
 
@@ -26,7 +29,7 @@ __global__ void reset_classes_kernel(int32_t* counts) {
 // block's range of each list (one atomic per world on a single counter serialises).
 // The order of a class list is not significant: every world is solved on its own.
 __global__ void __launch_bounds__(256) classify_worlds_kernel(int W, const int32_t* __restrict__ ccount, int K,
-                                                              int base_rows, int pair_rows,
+                                                              int base_rows, int pair_rows, int merge_from,
                                                               int32_t* __restrict__ counts,
                                                               int32_t* __restrict__ lists) {
   __shared__ int wcnt[4][kNumClasses];  // worlds per (wave, class)
@@ -41,6 +44,7 @@ __global__ void __launch_bounds__(256) classify_worlds_kernel(int W, const int32
       int r = max(1, (n_r + WAVE - 1) / WAVE);
       c = 1;
       while (c < kNumClasses - 1 && kClassR[c] < r) ++c;
+      if (c > merge_from) c = merge_from;
     }
   }
   const unsigned long long below = (1ull << lane) - 1ull;
@@ -63,6 +67,24 @@ __global__ void __launch_bounds__(256) classify_worlds_kernel(int W, const int32
     for (int v = 0; v < wave; ++v) off += wcnt[v][c];
     lists[(size_t)c * W + off + rank] = w;
   }
+}
+
+// The forward step merges the classes from kMergeFrom on into one launch of the instance
+// for the largest world the solver can hold: those classes hold few worlds each, and one
+// after another their launches would add up their slowest worlds' Newton solves.
+constexpr int kMergeFrom = 3;
+
+// largest class a world of this solver can fall in
+static int top_class(const ofb_solver* s) {
+  const int r_max = (s->L.Nc + WAVE - 1) / WAVE;
+  int c = 1;
+  while (c < kNumClasses - 1 && kClassRHost[c] < r_max) ++c;
+  return c;
+}
+
+// kernel instance (class code) of the forward launch of class list c
+static int launch_code(const ofb_solver* s, int c) {
+  return c == kMergeFrom ? kClassRHost[std::max(c, top_class(s))] : kClassRHost[c];
 }
 
 // lanes per world and row capacity of a class code
@@ -166,6 +188,8 @@ static bool class_used(const ofb_solver* s, int c) {
   if (c == 0) return s->cfg.pair_small_worlds != 0;
   return c == 1 || kClassRHost[c - 1] < r_max;
 }
+// forward launches: classes past kMergeFrom are merged into it
+static bool class_launched(const ofb_solver* s, int c) { return c <= kMergeFrom && class_used(s, c); }
 
 // workgroup shape of every class launch for the solver's current launch settings
 static int fused_configure(ofb_solver* s) {
@@ -174,7 +198,7 @@ static int fused_configure(ofb_solver* s) {
     if (!class_used(s, c)) continue;
     int rc = allow_lds(kClassRHost[c], s);
     if (rc) return rc;
-    s->cls_wpb[c] = choose_wpb(s, kClassRHost[c]);
+    s->cls_wpb[c] = choose_wpb(s, launch_code(s, c));
   }
   return 0;
 }
@@ -196,26 +220,81 @@ int fused_set_launch(ofb_solver* s, int waves_per_block, int pair_small_worlds) 
 int fused_classes_alloc(ofb_solver* s) {
   int rc = fused_configure(s);
   if (rc) return rc;
-  if (hipStreamCreateWithFlags(&s->side_stream, hipStreamNonBlocking) != hipSuccess) {
-    set_error("hipStreamCreate failed");
-    return -4;
-  }
   if (hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming) != hipSuccess) {
-    (void)hipStreamDestroy(s->side_stream);
-    set_error("hipEventCreate failed");
-    return -4;
-  }
-  if (hipEventCreateWithFlags(&s->ev_join, hipEventDisableTiming) != hipSuccess) {
-    (void)hipEventDestroy(s->ev_fork);
-    (void)hipStreamDestroy(s->side_stream);
     set_error("hipEventCreate failed");
     return -4;
   }
   s->side_ready = 1;
+  for (int i = 0; i < kSideStreams; ++i) {
+    if (hipStreamCreateWithFlags(&s->side_stream[i], hipStreamNonBlocking) != hipSuccess) {
+      set_error("hipStreamCreate failed");
+      return -4;
+    }
+    s->side_ready = 2 + 2 * i;
+    if (hipEventCreateWithFlags(&s->ev_join[i], hipEventDisableTiming) != hipSuccess) {
+      set_error("hipEventCreate failed");
+      return -4;
+    }
+    s->side_ready = 3 + 2 * i;
+  }
+  for (int c = 0; c < kNumClasses; ++c) {
+    if (hipEventCreate(&s->ev_cls0[c]) != hipSuccess || hipEventCreate(&s->ev_cls1[c]) != hipSuccess) {
+      set_error("hipEventCreate failed (class timing)");
+      return -4;
+    }
+    s->cls_events_ready = c + 1;
+  }
   if (hipMalloc((void**)&s->cls_counts, kNumClasses * sizeof(int32_t)) != hipSuccess ||
       hipMalloc((void**)&s->cls_lists, (size_t)kNumClasses * s->m.W * sizeof(int32_t)) != hipSuccess) {
     set_error("hipMalloc failed (world classes)");
     return -4;
+  }
+  return 0;
+}
+
+void fused_classes_free(ofb_solver* s) {
+  // side_ready counts what fused_classes_alloc created: the fork event, then a stream and
+  // its join event per side stream
+  if (s->side_ready >= 1) (void)hipEventDestroy(s->ev_fork);
+  for (int i = 0; i < kSideStreams; ++i) {
+    if (s->side_ready >= 2 + 2 * i) (void)hipStreamDestroy(s->side_stream[i]);
+    if (s->side_ready >= 3 + 2 * i) (void)hipEventDestroy(s->ev_join[i]);
+  }
+  s->side_ready = 0;
+  for (int c = 0; c < s->cls_events_ready; ++c) {
+    (void)hipEventDestroy(s->ev_cls0[c]);
+    (void)hipEventDestroy(s->ev_cls1[c]);
+  }
+  s->cls_events_ready = 0;
+  if (s->cls_counts) (void)hipFree(s->cls_counts);
+  if (s->cls_lists) (void)hipFree(s->cls_lists);
+  s->cls_counts = nullptr;
+  s->cls_lists = nullptr;
+}
+
+// Row capacity, world count and duration of every class of the last step (eager steps
+// with class timing on; ms = -1 for a class that was not launched or not timed).
+int fused_class_times(const ofb_solver* s, int capacity, int32_t* rows_cap, int32_t* worlds, float* ms,
+                      int32_t* count) {
+  *count = kNumClasses;
+  if (capacity < kNumClasses) { set_error("class times need room for %d classes", kNumClasses); return -2; }
+  int32_t n[kNumClasses];
+  if (hipMemcpy(n, s->cls_counts, sizeof(n), hipMemcpyDeviceToHost) != hipSuccess) {
+    set_error("class count read-back failed");
+    return -4;
+  }
+  for (int c = 0; c < kNumClasses; ++c) {
+    if (rows_cap) rows_cap[c] = class_rows(launch_code(s, c));
+    if (worlds) worlds[c] = n[c];
+    if (!ms) continue;
+    ms[c] = -1.f;
+    if (s->cls_timed & (1 << c)) {
+      hipEvent_t e0 = c == 0 ? s->ev_kernel0 : s->ev_cls0[c], e1 = c == 0 ? s->ev_kernel1 : s->ev_cls1[c];
+      if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(ms + c, e0, e1) != hipSuccess) {
+        set_error("class event read failed");
+        return -4;
+      }
+    }
   }
   return 0;
 }
@@ -249,31 +328,53 @@ int fused_step(ofb_solver* s, const ofb_state_in* in, const ofb_state_out* out, 
   }
   hipLaunchKernelGGL(reset_classes_kernel, dim3(1), dim3(64), 0, stream, s->cls_counts);
   hipLaunchKernelGGL(classify_worlds_kernel, dim3((W + 255) / 256), dim3(256), 0, stream, W, s->ccount, s->L.K,
-                     s->L.Nj + s->L.Nctrl, s->cfg.pair_small_worlds ? kPairRows : -1, s->cls_counts,
+                     s->L.Nj + s->L.Nctrl, s->cfg.pair_small_worlds ? kPairRows : -1, kMergeFrom, s->cls_counts,
                      s->cls_lists);
-  // The one-world-per-wave classes hold few worlds on a typical batch, but each of their
-  // waves takes a whole Newton solve: they run on a side stream, launched first so that
-  // their waves are resident beside the paired class's, instead of after it.
-  if (hipEventRecord(s->ev_fork, stream) != hipSuccess ||
-      hipStreamWaitEvent(s->side_stream, s->ev_fork, 0) != hipSuccess) {
+  // The one-world-per-wave classes run on side streams, launched first so that their
+  // waves are resident beside the paired class's: the dense class, the R = 2 class and
+  // the larger classes (few worlds each, one after another) each on a stream of its own,
+  // so that no class waits for another to drain.
+  if (hipEventRecord(s->ev_fork, stream) != hipSuccess) {
     set_error("fused step: stream fork failed");
     return -4;
   }
-  for (int c = 1; c < kNumClasses; ++c) {
-    if (!class_used(s, c)) continue;  // no world can fall in this class
-    if (launch_fused(kClassRHost[c], P, s, c, s->side_stream) != 0) {
-      set_error("no fused kernel instance for class code %d", kClassRHost[c]);
+  for (int i = 0; i < kSideStreams; ++i) {
+    if (hipStreamWaitEvent(s->side_stream[i], s->ev_fork, 0) != hipSuccess) {
+      set_error("fused step: stream fork failed");
       return -4;
     }
   }
-  if (hipEventRecord(s->ev_join, s->side_stream) != hipSuccess) {
-    set_error("fused step: stream join failed");
-    return -4;
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(stream, &cap) != hipSuccess) cap = hipStreamCaptureStatusNone;
+  const bool cls_timed = s->cls_timing && cap == hipStreamCaptureStatusNone;
+  s->cls_timed = 0;
+  for (int c = kMergeFrom; c >= 1; --c) {  // the slowest worlds first
+    if (!class_launched(s, c)) continue;  // no world can fall in this class
+    hipStream_t side = s->side_stream[side_of(c)];
+    if (cls_timed && hipEventRecord(s->ev_cls0[c], side) != hipSuccess) {
+      set_error("fused step: event record failed");
+      return -4;
+    }
+    if (launch_fused(launch_code(s, c), P, s, c, side) != 0) {
+      set_error("no fused kernel instance for class code %d", launch_code(s, c));
+      return -4;
+    }
+    if (cls_timed) {
+      if (hipEventRecord(s->ev_cls1[c], side) != hipSuccess) {
+        set_error("fused step: event record failed");
+        return -4;
+      }
+      s->cls_timed |= 1 << c;
+    }
+  }
+  for (int i = 0; i < kSideStreams; ++i) {
+    if (hipEventRecord(s->ev_join[i], s->side_stream[i]) != hipSuccess) {
+      set_error("fused step: stream join failed");
+      return -4;
+    }
   }
   // the paired class, which holds most worlds of a typical batch, on the step's stream,
   // bracketed by the kernel events (not recorded while the stream is being captured)
-  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
-  if (hipStreamIsCapturing(stream, &cap) != hipSuccess) cap = hipStreamCaptureStatusNone;
   const bool timed = cap == hipStreamCaptureStatusNone && class_used(s, 0);
   if (timed && hipEventRecord(s->ev_kernel0, stream) != hipSuccess) {
     set_error("fused step: event record failed");
@@ -288,9 +389,12 @@ int fused_step(ofb_solver* s, const ofb_state_in* in, const ofb_state_out* out, 
     return -4;
   }
   s->kernel_timed = timed ? 1 : 0;
-  if (hipStreamWaitEvent(stream, s->ev_join, 0) != hipSuccess) {
-    set_error("fused step: stream join failed");
-    return -4;
+  if (timed && cls_timed) s->cls_timed |= 1;  // the paired class's events are the kernel events
+  for (int i = 0; i < kSideStreams; ++i) {
+    if (hipStreamWaitEvent(stream, s->ev_join[i], 0) != hipSuccess) {
+      set_error("fused step: stream join failed");
+      return -4;
+    }
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) { set_error("fused kernel launch failed: %s", hipGetErrorString(e)); return -4; }
@@ -332,7 +436,7 @@ int fused_adjoint(ofb_solver* s, const ofb_state_in* in, const ofb_state_out* ou
   // the adjoint runs one world per wavefront: no world is put in the paired class
   hipLaunchKernelGGL(reset_classes_kernel, dim3(1), dim3(64), 0, stream, s->cls_counts);
   hipLaunchKernelGGL(classify_worlds_kernel, dim3((W + 255) / 256), dim3(256), 0, stream, W, s->ccount, s->L.K,
-                     s->L.Nj + s->L.Nctrl, -1, s->cls_counts, s->cls_lists);
+                     s->L.Nj + s->L.Nctrl, -1, kNumClasses, s->cls_counts, s->cls_lists);
   for (int c = 1; c < kNumClasses; ++c) {
     if (!class_used(s, c)) continue;
     const int R = kClassRHost[c];

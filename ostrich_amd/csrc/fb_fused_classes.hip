@@ -32,13 +32,14 @@ int OFB_CAT(fused_class_op_bj, OFB_BJ, _set, OFB_RSET)(int R, int op, const Fuse
   return -1;
 }
 
-// Profiling build only: the phase counters of the Jacobi instances R = 1..5 (each
-// translation unit holds its own copy of the counters).
-#if defined(OFB_PHASE_TIMING) && OFB_BJ == 0 && OFB_RSET == 0
-extern "C" int ofb_debug_phase_cycles(unsigned long long* out, int reset) {
-  if (hipMemcpyFromSymbol(out, HIP_SYMBOL(g_phase_cycles), sizeof(unsigned long long) * 8) != hipSuccess) return -4;
+// Profiling build only: the phase counters [17][10] of the Jacobi instances of this unit
+// (each translation unit holds its own copy: the paired and dense classes and R = 2..5
+// here with OFB_RSET = 0, R = 6..16 with OFB_RSET = 1).
+#if defined(OFB_PHASE_TIMING) && OFB_BJ == 0
+extern "C" int OFB_CAT(ofb_debug_phase_cycles, _set, OFB_RSET, )(unsigned long long* out, int reset) {
+  if (hipMemcpyFromSymbol(out, HIP_SYMBOL(g_phase_cycles), sizeof(unsigned long long) * 170) != hipSuccess) return -4;
   if (reset) {
-    unsigned long long z[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    static const unsigned long long z[170] = {};
     if (hipMemcpyToSymbol(HIP_SYMBOL(g_phase_cycles), z, sizeof(z)) != hipSuccess) return -4;
   }
   return 0;

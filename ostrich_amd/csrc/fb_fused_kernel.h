@@ -36,11 +36,19 @@ namespace ofb {
 using namespace rows;
 
 #ifdef OFB_PHASE_TIMING
-// Profiling build only: per-phase cycle counts of the Newton loop, summed over worlds.
-__device__ unsigned long long g_phase_cycles[8];
+// Profiling build only: per-phase cycle counts of the Newton loop, summed over worlds,
+// per instance (row 0: paired, 1: dense, R: matrix-free with R slots per lane); columns
+// 8 and 9 count Newton and PCR iterations.
+__device__ unsigned long long g_phase_cycles[17][10];
 #define OFB_PHASE_INIT unsigned long long ph_[8] = {0, 0, 0, 0, 0, 0, 0, 0}; unsigned long long ph_t = clock64();
 #define OFB_PHASE(k) { unsigned long long t_ = clock64(); ph_[k] += t_ - ph_t; ph_t = t_; }
-#define OFB_PHASE_FLUSH if (lane == 0) for (int k_ = 0; k_ < 8; ++k_) atomicAdd(&g_phase_cycles[k_], ph_[k_]);
+#define OFB_PHASE_FLUSH                                                                       \
+  if (lane == 0) {                                                                            \
+    const int ci_ = DENSE ? (GW == 32 ? 0 : 1) : R;                                           \
+    for (int k_ = 0; k_ < 8; ++k_) atomicAdd(&g_phase_cycles[ci_][k_], ph_[k_]);              \
+    atomicAdd(&g_phase_cycles[ci_][8], (unsigned long long)n_it);                             \
+    atomicAdd(&g_phase_cycles[ci_][9], (unsigned long long)pcr_total);                        \
+  }
 #else
 #define OFB_PHASE_INIT
 #define OFB_PHASE(k)
@@ -52,7 +60,7 @@ __device__ unsigned long long g_phase_cycles[8];
 // per joint, one per contact slot of the class (cs_cap), the block-Jacobi region, the
 // dense Schur operator.  `words` is a multiple of 4, so every world's base stays aligned.
 struct FusedLayout {
-  int lamprev, js, cs, bj, dense, hist, words;
+  int lamprev, js, cs, bj, dense, rs, hist, words;
 };
 // Per-Newton-iteration history of a world (||h||^2 and the running PCR count of the first
 // kHistIters iterations) kept in LDS and written to the caller's buffers once per step.
@@ -358,20 +366,16 @@ __device__ void bj_apply(const BJView& V, const bool (&act)[R], const float (&Pi
 }
 
 // out = (J M^-1 J^T + C + reg) v over the rows of one world (optim/system_operator.py:36-124):
-// scatter J^T v into the LDS body accumulator, apply M^-1 per body, gather.
-template <int R, int GW = WAVE>
-__device__ __forceinline__ void schur_matvec(float* sm, int lane, int B, const bool (&act)[R], const int (&b0)[R],
-                                             const int (&b1)[R], const sv (&J0)[R], const sv (&J1)[R],
-                                             const float (&Cr)[R], const float (&v)[R], float (&out)[R]) {
-  for (int e = lane; e < 6 * B; e += GW) sm[(e / 6) * BS + BACC + (e % 6)] = 0.f;
-  wsync();
+// sum J^T v per body into the LDS accumulator, apply M^-1 per body, gather.
+template <int R, int GW, class BS_>
+__device__ __forceinline__ void schur_matvec(float* sm, int lane, int B, int n_r, const BS_& S, const bool (&act)[R],
+                                             const int (&b0)[R], const int (&b1)[R], const sv (&J0)[R],
+                                             const sv (&J1)[R], const float (&Cr)[R], const float (&v)[R],
+                                             float (&out)[R]) {
+  S.begin(BACC, true);
 #pragma unroll
-  for (int k = 0; k < R; ++k) {
-    if (act[k]) {
-      if (b0[k] >= 0) atomic_add_sv(sm + b0[k] * BS + BACC, J0[k] * v[k]);
-      if (b1[k] >= 0) atomic_add_sv(sm + b1[k] * BS + BACC, J1[k] * v[k]);
-    }
-  }
+  for (int k = 0; k < R; ++k) S.add(BACC, lane + GW * k, act[k], b0[k], b1[k], J0[k] * v[k], J1[k] * v[k]);
+  S.end(BACC, true, n_r);
   wsync();
   for (int b = lane; b < B; b += GW) {
     float* bb = sm + b * BS;
@@ -616,12 +620,120 @@ __device__ __forceinline__ void gather_contrib(float* sm, const float* __restric
   }
 }
 
+// ---------------------------------------------------------------------------------
+// Row -> body sums through lane-private copies (worlds of at most kRsMaxBodies bodies).
+// Each lane owns a copy of the 6 B body accumulators in LDS (odd stride, so the copies of
+// consecutive lanes start on consecutive banks) and adds its rows' contributions to it
+// with plain reads and writes of its own words.  LDS float atomics (ds_add_f32) are not
+// used: on gfx950 one of them costs about as much as twenty plain read-modify-writes
+// (measured: 24 private ds_add_f32 per lane 4.6 k cycles, the same adds as reads and
+// writes 0.6 k), whether or not lanes share addresses.  The copies are then summed per
+// (body, component) by lanes that walk the copies in lane order (four interleaved
+// chains): a fixed order, the same for every launch shape.  Larger worlds keep the
+// staged records (dense instances) or direct LDS atomics.
+// ---------------------------------------------------------------------------------
+constexpr int kRsMaxBodies = 10;
+__host__ __device__ inline bool rs_enabled(int B) { return B <= kRsMaxBodies; }
+__host__ __device__ inline int rs_stride(int B) { return 6 * B + 1; }
+__host__ __device__ inline int rs_words(int lanes, int B) { return rs_enabled(B) ? lanes * rs_stride(B) : 0; }
+
+template <int GW, bool DENSE>
+struct BodySum {
+  float* sm;    // the world's body blocks
+  float* area;  // lane-private copies, or the staged records of the dense fallback
+  int lane, B;
+  bool rs;
+
+  // begin a pass that sums into body field `field`: with `zero` the pass's sums replace
+  // the field, else they are added to it
+  __device__ __forceinline__ void begin(int field, bool zero) const {
+    if (rs) {
+      float* p = area + lane * rs_stride(B);
+      for (int b = 0; b < B; ++b, p += 6) {
+        p[0] = 0.f; p[1] = 0.f; p[2] = 0.f; p[3] = 0.f; p[4] = 0.f; p[5] = 0.f;
+      }
+    } else if (!DENSE && zero) {
+      for (int e = lane; e < 6 * B; e += GW) sm[(e / 6) * BS + field + (e % 6)] = 0.f;
+      wsync();
+    }
+  }
+  // contributions c0 to body b0 and c1 to body b1 of row i (none when !on; a body < 0 is
+  // a static side)
+  __device__ __forceinline__ void add(int field, int i, bool on, int b0, int b1, const sv& c0, const sv& c1) const {
+    if (rs) {
+      if (!on) return;
+      float* p = area + lane * rs_stride(B);
+      if (b0 >= 0) add_sv(p + 6 * b0, c0);
+      if (b1 >= 0) add_sv(p + 6 * b1, c1);
+    } else if (DENSE) {
+      stage_contrib(area, i, on, b0, b1, c0, c1);
+    } else if (on) {
+      if (b0 >= 0) atomic_add_sv(sm + b0 * BS + field, c0);
+      if (b1 >= 0) atomic_add_sv(sm + b1 * BS + field, c1);
+    }
+  }
+  // p[0..5] += x on this lane's own words
+  __device__ __forceinline__ static void add_sv(float* p, const sv& x) {
+    const float a0 = p[0], a1 = p[1], a2 = p[2], a3 = p[3], a4 = p[4], a5 = p[5];
+    p[0] = a0 + x.a.x; p[1] = a1 + x.a.y; p[2] = a2 + x.a.z;
+    p[3] = a3 + x.b.x; p[4] = a4 + x.b.y; p[5] = a5 + x.b.z;
+  }
+  // completes the pass (rows 0..n_r-1 staged for the record fallback); the caller
+  // synchronises before reading the field
+  __device__ __forceinline__ void end(int field, bool zero, int n_r) const {
+    wsync();
+    if (rs) sums(field, zero);
+    else if (DENSE) gather_contrib<GW>(sm, area, lane, n_r, B, field, zero);
+  }
+
+  // value j of lane t's copy is area[t * stride + j]; nt is 32 or 64: the loads of each
+  // group of 16 copies are issued together, then summed in four interleaved chains
+  __device__ __forceinline__ float copies_sum(int j, int t0, int nt) const {
+    const int st = rs_stride(B);
+    const float* p = area + t0 * st + j;
+    float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
+    for (int t = 0; t < nt; t += 16, p += 16 * st) {
+      float x[16];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) x[u] = p[u * st];
+#pragma unroll
+      for (int u = 0; u < 16; u += 4) {
+        a0 += x[u];
+        a1 += x[u + 1];
+        a2 += x[u + 2];
+        a3 += x[u + 3];
+      }
+    }
+    return (a0 + a1) + (a2 + a3);
+  }
+  __device__ __forceinline__ void sums(int field, bool zero) const {
+    const int nv = 6 * B;
+    if (GW == 64 && nv <= 32) {
+      // lanes j and j + 32 sum the two halves of the copies of value j; the swap of
+      // the halves gives both the same total
+      const int j = lane & 31;
+      float s = j < nv ? copies_sum(j, lane & 32, 32) : 0.f;
+      s = wave_sum_from_halves(s);
+      if (lane < nv) {
+        float* f = sm + (lane / 6) * BS + field + (lane % 6);
+        *f = zero ? s : *f + s;
+      }
+    } else {
+      for (int j = lane; j < nv; j += GW) {
+        const float s = copies_sum(j, 0, GW);
+        float* f = sm + (j / 6) * BS + field + (j % 6);
+        *f = zero ? s : *f + s;
+      }
+    }
+  }
+};
+
 // ||[h_d | h_c]||^2 at the step-start state with lambda = l (and lambda_prev = l, as the
 // first Newton iteration snapshots it): the gate between two Newton seeds.
-template <int R, int GW, bool GATHER>
+template <int R, int GW, bool DENSE>
 __device__ __forceinline__ float initial_merit(const FusedParams& P, float* sm, float* lamprev, const float* js,
                                                const float* cs, const int (&rj)[R], const int (&rk)[R], int lane,
-                                               int n_r, int n_c, const float (&l)[R], float* rec) {
+                                               int n_r, int n_c, const float (&l)[R], const BodySum<GW, DENSE>& S) {
   const int B = P.m.B;
 #pragma unroll
   for (int k = 0; k < R; ++k) {
@@ -630,21 +742,19 @@ __device__ __forceinline__ float initial_merit(const FusedParams& P, float* sm, 
   }
   dynamics_bodies(P, sm, lane, false, GW);
   wsync();
+  S.begin(BHD, false);
   float s = 0.f;
 #pragma unroll
   for (int k = 0; k < R; ++k) {
     int i = lane + GW * k;
     if (i < n_r) {
-      RowOut o = assemble_row_cached(P, sm, lamprev, js, cs, i, n_c, rj[k], rk[k], l[k], !GATHER);
+      RowOut o = assemble_row_cached(P, sm, lamprev, js, cs, i, n_c, rj[k], rk[k], l[k], false);
       s += o.hc * o.hc;
-      if (GATHER) stage_contrib(rec, i, o.active, o.b0, o.b1, o.J0 * (-l[k] * P.dt), o.J1 * (-l[k] * P.dt));
+      S.add(BHD, i, o.active, o.b0, o.b1, o.J0 * (-l[k] * P.dt), o.J1 * (-l[k] * P.dt));
     }
   }
+  S.end(BHD, false, n_r);
   wsync();
-  if (GATHER) {
-    gather_contrib<GW>(sm, rec, lane, n_r, B, BHD, false);
-    wsync();
-  }
   for (int e = lane; e < 6 * B; e += GW) {
     float v = sm[(e / 6) * BS + BHD + (e % 6)];
     s += v * v;
@@ -653,21 +763,21 @@ __device__ __forceinline__ float initial_merit(const FusedParams& P, float* sm, 
 }
 
 // PCR on A x = rhs from x0 = 0 (optim/pcr_solver.py:170-295): the seven vectors live in
-// VGPRs, dots are DPP group reductions, A is the LDS matrix-free operator (or, DENSE, the
-// assembled matrix of the dense view).  Returns the iteration count.
-template <int R, bool BJ, int GW = WAVE, bool DENSE = false>
-__device__ __forceinline__ int pcr_solve(float* sm, int lane, int B, const BJView& V, const BJRegs<R>& G,
-                                         const bool (&act)[R], const int (&b0)[R], const int (&b1)[R],
-                                         const sv (&J0)[R], const sv (&J1)[R], const float (&Cr)[R],
-                                         const float (&Pi)[R], const float (&rhs)[R], float bsq, float tol_sq,
-                                         float latol_sq, int max_iters, float (&x)[R],
+// VGPRs, dots are DPP group reductions, A is the LDS matrix-free operator with its body
+// sums S (or, DENSE, the assembled matrix of the dense view).  Returns the iteration count.
+template <int R, bool BJ, int GW, bool DENSE, class BS_>
+__device__ __forceinline__ int pcr_solve(float* sm, int lane, int B, int n_r, const BS_& S, const BJView& V,
+                                         const BJRegs<R>& G, const bool (&act)[R], const int (&b0)[R],
+                                         const int (&b1)[R], const sv (&J0)[R], const sv (&J1)[R],
+                                         const float (&Cr)[R], const float (&Pi)[R], const float (&rhs)[R],
+                                         float bsq, float tol_sq, float latol_sq, int max_iters, float (&x)[R],
                                          const DenseView& D = DenseView{},
                                          const DenseRow<GW>& Ad = DenseRow<GW>{}) {
   static_assert(!DENSE || R == 1, "the dense operator holds one row per lane");
   float r[R], z[R], p[R], Ap[R], y[R], Az[R];
   auto matvec = [&](const float (&v)[R], float (&out)[R]) {
     if constexpr (DENSE) out[0] = dense_matvec(D, Ad, lane, act[0], v[0]);
-    else schur_matvec<R, GW>(sm, lane, B, act, b0, b1, J0, J1, Cr, v, out);
+    else schur_matvec<R, GW>(sm, lane, B, n_r, S, act, b0, b1, J0, J1, Cr, v, out);
   };
 #pragma unroll
   for (int k = 0; k < R; ++k) { x[k] = 0.f; r[k] = rhs[k]; z[k] = Pi[k] * r[k]; }
@@ -712,12 +822,23 @@ __device__ __forceinline__ int pcr_solve(float* sm, int lane, int B, const BJVie
 
 __host__ __device__ inline FusedLayout fused_layout(int B, int J, int rows_cap, int cs_cap, bool bj, bool dense) {
   FusedLayout l;
+  const int lanes = rows_cap < WAVE ? rows_cap : WAVE;  // lanes per world
   l.lamprev = B * BS;
   l.js = up4(l.lamprev + rows_cap + 1);
   l.cs = l.js + JS_WORDS * J;
   l.bj = l.cs + CS_WORDS * cs_cap;
   l.dense = up4(l.bj + (bj ? bj_words(rows_cap) : 0));
-  l.hist = up4(l.dense + (dense ? dense_words(rows_cap, B) : 0));
+  // the dense instances keep the lane-private body sums in their dense region (free
+  // outside the Schur build); the matrix-free ones get a region of their own
+  const int rs = rs_words(lanes, B);
+  if (dense) {
+    const int dw = dense_words(rows_cap, B);
+    l.rs = l.dense;
+    l.hist = up4(l.dense + (dw > rs ? dw : rs));
+  } else {
+    l.rs = l.dense;
+    l.hist = up4(l.dense + rs);
+  }
   l.words = l.hist + kHistWords;
   return l;
 }
@@ -757,6 +878,10 @@ fused_step_kernel(FusedParams P) {
   const int idx = blockIdx.x * (blockDim.x / GW) + grp;
   if (idx >= *P.wcount) return;
   const int w = P.wlist[idx];
+  // issue priority of the wave on its SIMD: the classes of larger worlds, whose Newton
+  // solves are the longest and set the step's duration, ahead of the small ones that
+  // fill the gaps
+  __builtin_amdgcn_s_setprio(DENSE ? (GW == 32 ? 0 : 1) : (R >= 3 ? 3 : 2));
   float* sm = lds + grp * P.lay.words;
   const int B = P.m.B;
   const RowLayout& L = P.L;
@@ -791,8 +916,9 @@ fused_step_kernel(FusedParams P) {
   const int n_r = L.Nj + L.Nctrl + 3 * n_c;
   const DenseView D = dense_view<GW>(sm + P.lay.dense, n_r, B);
   DenseRow<GW> Adense;  // row of the assembled Schur matrix (DENSE)
-  // row -> body sums by staged records instead of LDS atomics (DENSE: at most 64 rows)
-  constexpr bool kGather = DENSE;
+  // row -> body sums: lane-private copies (worlds of at most kRsMaxBodies bodies), else
+  // staged records (DENSE) or LDS atomics
+  const BodySum<GW, DENSE> S{sm, sm + P.lay.rs, lane, B, rs_enabled(B)};
   wsync();
   // ---- step-constant row data (model, control, contacts, Jacobians at the step-start
   // poses) into LDS, once per step ----
@@ -828,8 +954,8 @@ fused_step_kernel(FusedParams P) {
       int i = lane + GW * k;
       alt[k] = i < n_r ? P.lambda_seed2[(size_t)w * L.Nc + original_row(L, i, n_c)] : 0.f;
     }
-    const float ma = initial_merit<R, GW, kGather>(P, sm, lamprev, js, cs, rj, rk, lane, n_r, n_c, lam, D.rowd);
-    const float mb = initial_merit<R, GW, kGather>(P, sm, lamprev, js, cs, rj, rk, lane, n_r, n_c, alt, D.rowd);
+    const float ma = initial_merit<R, GW, DENSE>(P, sm, lamprev, js, cs, rj, rk, lane, n_r, n_c, lam, S);
+    const float mb = initial_merit<R, GW, DENSE>(P, sm, lamprev, js, cs, rj, rk, lane, n_r, n_c, alt, S);
     if (mb < ma) {
       seed_flag = FLAG_LEARNED_SEED;
 #pragma unroll
@@ -862,6 +988,7 @@ fused_step_kernel(FusedParams P) {
     wsync();
     OFB_PHASE(0)
     // ---- assembly (linear_system.py:140-322) ----
+    S.begin(BHD, false);
     float hc[R];
 #pragma unroll
     for (int k = 0; k < R; ++k) {
@@ -871,18 +998,13 @@ fused_step_kernel(FusedParams P) {
         RowOut o;
         if (kRowCache && rvalid && i < L.Nj + L.Nctrl + n_c) {
           // joint, control and normal rows do not read lambda_prev: the residual pass of
-          // the previous iteration evaluated them at this same (q, u, lambda); replay
-          // their -J^T lambda dt into h_d
+          // the previous iteration evaluated them at this same (q, u, lambda); its
+          // -J^T lambda dt goes into h_d again
           o = rcache[k];
-          if (o.active && !kGather) {
-            const float sc = -lam[k] * dt;
-            if (o.b0 >= 0) atomic_add_sv(sm + o.b0 * BS + BHD, o.J0 * sc);
-            if (o.b1 >= 0) atomic_add_sv(sm + o.b1 * BS + BHD, o.J1 * sc);
-          }
         } else {
-          o = assemble_row_cached(P, sm, lamprev, js, cs, i, n_c, rj[k], rk[k], lam[k], !kGather);
+          o = assemble_row_cached(P, sm, lamprev, js, cs, i, n_c, rj[k], rk[k], lam[k], false);
         }
-        if (kGather) stage_contrib(D.rowd, i, o.active, o.b0, o.b1, o.J0 * (-lam[k] * dt), o.J1 * (-lam[k] * dt));
+        S.add(BHD, i, o.active, o.b0, o.b1, o.J0 * (-lam[k] * dt), o.J1 * (-lam[k] * dt));
         J0[k] = o.J0; J1[k] = o.J1; b0[k] = o.b0; b1[k] = o.b1; act[k] = o.active;
         hc[k] = o.hc;
         Cr[k] = o.C + reg;
@@ -893,11 +1015,8 @@ fused_step_kernel(FusedParams P) {
         act[k] = false;
       }
     }
+    S.end(BHD, false, n_r);
     wsync();
-    if (kGather) {
-      gather_contrib<GW>(sm, D.rowd, lane, n_r, B, BHD, false);
-      wsync();
-    }
     OFB_PHASE(1)
     // M^-1 h_d per body
     for (int b = lane; b < B; b += GW) {
@@ -940,28 +1059,18 @@ fused_step_kernel(FusedParams P) {
     OFB_PHASE(2)
     if constexpr (DENSE) dense_build(D, Adense, lane, B, act[0], b0[0], b1[0], J0[0], J1[0], W0[0], W1[0], Cr[0]);
     OFB_PHASE(7)
-    pcr_total += pcr_solve<R, BJ, GW, DENSE>(sm, lane, B, V, G, act, b0, b1, J0, J1, Cr, Pi, rhs, bsq, tol_sq,
-                                             latol_sq, P.cfg.lin_max_iters, x, D, Adense);
+    pcr_total += pcr_solve<R, BJ, GW, DENSE>(sm, lane, B, n_r, S, V, G, act, b0, b1, J0, J1, Cr, Pi, rhs, bsq,
+                                             tol_sq, latol_sq, P.cfg.lin_max_iters, x, D, Adense);
     OFB_PHASE(3)
 
     // ---- velocity recovery + Newton update (linear_system.py:344-377, newton_step.py:22-54) ----
-    if (kGather) {
-      stage_contrib(D.rowd, lane, act[0] && lane < n_r, b0[0], b1[0], J0[0] * x[0], J1[0] * x[0]);
-      if (P.ls_count == 0) lam[0] = lam[0] + x[0];
-      wsync();
-      gather_contrib<GW>(sm, D.rowd, lane, n_r, B, BACC, true);
-    } else {
-      for (int e = lane; e < 6 * B; e += GW) sm[(e / 6) * BS + BACC + (e % 6)] = 0.f;
-      wsync();
+    S.begin(BACC, true);
 #pragma unroll
-      for (int k = 0; k < R; ++k) {
-        if (act[k]) {
-          if (b0[k] >= 0) atomic_add_sv(sm + b0[k] * BS + BACC, J0[k] * x[k]);
-          if (b1[k] >= 0) atomic_add_sv(sm + b1[k] * BS + BACC, J1[k] * x[k]);
-        }
-        if (P.ls_count == 0) lam[k] = lam[k] + x[k];
-      }
+    for (int k = 0; k < R; ++k) {
+      S.add(BACC, lane + GW * k, act[k], b0[k], b1[k], J0[k] * x[k], J1[k] * x[k]);
+      if (P.ls_count == 0) lam[k] = lam[k] + x[k];
     }
+    S.end(BACC, true, n_r);
     wsync();
     if (P.ls_count == 0) {
       for (int b = lane; b < B; b += GW) {
@@ -1002,21 +1111,19 @@ fused_step_kernel(FusedParams P) {
         for (int k = 0; k < R; ++k) la[k] = lam[k] + a * x[k];
         dynamics_bodies(P, sm, lane, false, GW);
         wsync();
+        S.begin(BHD, false);
         float ms = 0.f;
 #pragma unroll
         for (int k = 0; k < R; ++k) {
           int i = lane + GW * k;
           if (i < n_r) {
-            RowOut o = assemble_row_cached(P, sm, lamprev, js, cs, i, n_c, rj[k], rk[k], la[k], !kGather);
+            RowOut o = assemble_row_cached(P, sm, lamprev, js, cs, i, n_c, rj[k], rk[k], la[k], false);
             ms += o.hc * o.hc;
-            if (kGather) stage_contrib(D.rowd, i, o.active, o.b0, o.b1, o.J0 * (-la[k] * dt), o.J1 * (-la[k] * dt));
+            S.add(BHD, i, o.active, o.b0, o.b1, o.J0 * (-la[k] * dt), o.J1 * (-la[k] * dt));
           }
         }
+        S.end(BHD, false, n_r);
         wsync();
-        if (kGather) {
-          gather_contrib<GW>(sm, D.rowd, lane, n_r, B, BHD, false);
-          wsync();
-        }
         for (int e = lane; e < 6 * B; e += GW) {
           float v = sm[(e / 6) * BS + BHD + (e % 6)];
           ms += v * v;
@@ -1041,23 +1148,21 @@ fused_step_kernel(FusedParams P) {
     // ---- residual at the new iterate (core/residual.py:18-173) ----
     dynamics_bodies(P, sm, lane, false, GW);
     wsync();
+    S.begin(BHD, false);
     float msum = 0.f;
 #pragma unroll
     for (int k = 0; k < R; ++k) {
       int i = lane + GW * k;
       if (i < n_r) {
-        RowOut o = assemble_row_cached(P, sm, lamprev, js, cs, i, n_c, rj[k], rk[k], lam[k], !kGather);
+        RowOut o = assemble_row_cached(P, sm, lamprev, js, cs, i, n_c, rj[k], rk[k], lam[k], false);
         msum += o.hc * o.hc;
         if (kRowCache) rcache[k] = o;
-        if (kGather) stage_contrib(D.rowd, i, o.active, o.b0, o.b1, o.J0 * (-lam[k] * dt), o.J1 * (-lam[k] * dt));
+        S.add(BHD, i, o.active, o.b0, o.b1, o.J0 * (-lam[k] * dt), o.J1 * (-lam[k] * dt));
       }
     }
     rvalid = true;
+    S.end(BHD, false, n_r);
     wsync();
-    if (kGather) {
-      gather_contrib<GW>(sm, D.rowd, lane, n_r, B, BHD, false);
-      wsync();
-    }
     for (int e = lane; e < 6 * B; e += GW) {
       float v = sm[(e / 6) * BS + BHD + (e % 6)];
       msum += v * v;

@@ -92,10 +92,11 @@ struct ofb_solver {
   int32_t* cls_counts;
   int32_t* cls_lists;
   int cls_wpb[16];   // worlds per workgroup of each class launch (LDS-limited)
-  // the one-world-per-wave classes run on a side stream beside the paired class
-  hipStream_t side_stream;
-  hipEvent_t ev_fork, ev_join;
-  int side_ready;
+  // the one-world-per-wave classes run on side streams beside the paired class (the
+  // dense class, the R = 2 class and the larger classes one after another)
+  hipStream_t side_stream[3];
+  hipEvent_t ev_fork, ev_join[3];
+  int side_ready;  // side streams and join events created
   // cross-step contact warm start: snapshot of the previous step (fb_warmstart.hip)
   int32_t *ws_count, *ws_b0, *ws_b1;
   float *ws_p, *ws_n, *ws_ln, *ws_lt;
@@ -111,6 +112,12 @@ struct ofb_solver {
   hipEvent_t ev_solve0, ev_solve1, ev_kernel0, ev_kernel1;
   int events_ready;
   int kernel_timed;   // the last step recorded the kernel events
+  // optional per-class timing of the fused path (ofb_set_class_timing): events around
+  // each class launch on the stream it runs on; bit c of cls_timed: class c was timed
+  int cls_timing;
+  int cls_timed;
+  hipEvent_t ev_cls0[16], ev_cls1[16];
+  int cls_events_ready;
   int solve_timed;    // the last step recorded the solve events
 };
 
@@ -119,6 +126,9 @@ int fused_step(ofb_solver* s, const ofb_state_in* in, const ofb_state_out* out, 
                hipStream_t stream);
 int fused_classes_alloc(ofb_solver* s);
 int fused_set_launch(ofb_solver* s, int waves_per_block, int pair_small_worlds);
+int fused_class_times(const ofb_solver* s, int capacity, int32_t* rows_cap, int32_t* worlds, float* ms,
+                      int32_t* count);
+void fused_classes_free(ofb_solver* s);
 int fused_adjoint(ofb_solver* s, const ofb_state_in* in, const ofb_state_out* out, const ofb_control* ctl,
                   const float* lambda, const float* lambda_prev, const ofb_adjoint_in* gin,
                   const ofb_adjoint_out* gout, float dt, hipStream_t stream);

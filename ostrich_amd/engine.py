@@ -204,6 +204,21 @@ class OstrichEngine:
         _lib.check(self.lib.ofb_last_kernel_ms(self._h, C.byref(ms)), "ofb_last_kernel_ms")
         return ms.value
 
+    def set_class_timing(self, enabled=True):
+        """Record HIP events around every row-class launch of each eager fused step."""
+        _lib.check(self.lib.ofb_set_class_timing(self._h, 1 if enabled else 0), "ofb_set_class_timing")
+
+    def class_times(self):
+        """Per row class of the last step: dict(rows_cap, worlds, ms) lists (ms -1: the
+        class was not launched or not timed).  Blocks until the step has finished."""
+        import numpy as np
+        rows, worlds, ms = (np.zeros(16, np.int32), np.zeros(16, np.int32), np.zeros(16, np.float32))
+        n = C.c_int32()
+        _lib.check(self.lib.ofb_get_class_times(self._h, 16, rows.ctypes.data, worlds.ctypes.data, ms.ctypes.data,
+                                                C.byref(n)), "ofb_get_class_times")
+        k = n.value
+        return dict(rows_cap=rows[:k].tolist(), worlds=worlds[:k].tolist(), ms=[float(x) for x in ms[:k]])
+
     def fused_launch(self):
         """(waves per workgroup, pair_small_worlds) of the fused path."""
         w, p = C.c_int32(), C.c_int32()

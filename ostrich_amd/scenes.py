@@ -69,12 +69,61 @@ SKID_WHEEL_POS = ((0.2, 0.26, -0.05), (0.2, -0.26, -0.05), (-0.45, 0.0, -0.05))
 SKID_MAX_OBSTACLES = 24
 
 
+def _skid_parts():
+    """Axis-aligned bounds (lo, hi) of the robot's parts at its spawn pose (origin, yaw 0):
+    the chassis box and the bounding box of each wheel cylinder (axis y)."""
+    z0 = SKID_WHEEL_R - SKID_WHEEL_POS[0][2] + 0.002
+    hx, hy, hz = SKID_CHASSIS_HALF
+    parts = [((-hx, -hy, z0 - hz), (hx, hy, z0 + hz))]
+    for x, y, z in SKID_WHEEL_POS:
+        c = (x, y, z0 + z)
+        parts.append(((c[0] - SKID_WHEEL_R, c[1] - SKID_WHEEL_HH, c[2] - SKID_WHEEL_R),
+                      (c[0] + SKID_WHEEL_R, c[1] + SKID_WHEEL_HH, c[2] + SKID_WHEEL_R)))
+    return z0, parts
+
+
+def _place_against(x, y, half, parts, clearance=1e-3):
+    """Moves a ground box of half extent `half` centred at (x, y) out of every robot part it
+    overlaps, along the horizontal axis of least travel, so it ends `clearance` from that
+    part's face; returns the new centre.  A box that clears every part stays where it is."""
+    for _ in range(8):
+        moved = False
+        for lo, hi in parts:
+            if 2.0 * half <= lo[2]:  # below the part: no overlap in z
+                continue
+            ox = min(x + half, hi[0]) - max(x - half, lo[0])
+            oy = min(y + half, hi[1]) - max(y - half, lo[1])
+            if ox <= -clearance or oy <= -clearance:
+                continue
+            cx, cy = 0.5 * (lo[0] + hi[0]), 0.5 * (lo[1] + hi[1])
+            # candidate moves to each side of the part; the shortest wins
+            moves = [(hi[0] + half + clearance - x, 0.0), (lo[0] - half - clearance - x, 0.0),
+                     (0.0, hi[1] + half + clearance - y), (0.0, lo[1] - half - clearance - y)]
+            # prefer the side the box centre is already on when travels tie
+            moves.sort(key=lambda m: (abs(m[0]) + abs(m[1]),
+                                      -((x - cx) * m[0] + (y - cy) * m[1])))
+            x, y = x + moves[0][0], y + moves[0][1]
+            moved = True
+        if not moved:
+            break
+    return x, y
+
+
 def skid_steer(num_worlds=4096, seed=1, max_obstacles=SKID_MAX_OBSTACLES, ke=150.0, kd=0.0,
-               target_range=5.0, mu_range=(0.4, 1.0), area=4.0, clear_radius=0.8):
-    """Uneven-batch skid-steer worlds (4 bodies, 15 joint rows, 3 velocity motor rows)."""
+               target_range=5.0, mu_range=(0.4, 1.0), area=4.0, layout="cluttered", clear_radius=0.8):
+    """Uneven-batch skid-steer worlds (4 bodies, 15 joint rows, 3 velocity motor rows).
+
+    The robot starts at the origin; 0-24 static boxes are placed uniformly in the
+    `area` x `area` square around it.  layout "cluttered": a box that would overlap the
+    robot is moved out along its shortest horizontal path to rest 1 mm from the part it
+    hit, so robots start among and against obstacles (as the reference's obstacle
+    benchmark drives its robot into boxes, examples/helhest/obstacle_benchmark.py:3-8).
+    layout "clear": boxes keep out of a disc of `clear_radius` around the robot."""
+    if layout not in ("cluttered", "clear"):
+        raise ValueError(f"layout must be 'cluttered' or 'clear', got {layout!r}")
     rng = np.random.default_rng(seed)
     worlds, qs, targets = [], [], []
-    z0 = SKID_WHEEL_R - SKID_WHEEL_POS[0][2] + 0.002
+    z0, parts = _skid_parts()
     for _ in range(num_worlds):
         mu = float(rng.uniform(*mu_range))
         wb = WorldBuilder()
@@ -94,10 +143,14 @@ def skid_steer(num_worlds=4096, seed=1, max_obstacles=SKID_MAX_OBSTACLES, ke=150
         for k in range(max_obstacles):
             if k < n_obs:
                 e = float(rng.uniform(0.05, 0.3))
-                while True:
+                if layout == "clear":
+                    while True:
+                        x, y = rng.uniform(-0.5 * area, 0.5 * area, size=2)
+                        if x * x + y * y > clear_radius * clear_radius:
+                            break
+                else:
                     x, y = rng.uniform(-0.5 * area, 0.5 * area, size=2)
-                    if x * x + y * y > clear_radius * clear_radius:
-                        break
+                    x, y = _place_against(float(x), float(y), 0.5 * e, parts)
                 wb.add_shape(Shape(GEO_BOX, -1, (float(x), float(y), 0.5 * e, 0.0, 0.0, 0.0, 1.0),
                                    (0.5 * e, 0.5 * e, 0.5 * e), mu=mu))
             else:  # unused slot: far below the ground, never in contact
@@ -116,7 +169,6 @@ def skid_steer(num_worlds=4096, seed=1, max_obstacles=SKID_MAX_OBSTACLES, ke=150
     control = dict(joint_target_pos=np.zeros(num_worlds * model.dofs_per_world, np.float32),
                    joint_target_vel=np.concatenate(targets).astype(np.float32))
     return model, state, control
-
 
 
 # tracked flipper robot on rough terrain

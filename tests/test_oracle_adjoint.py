@@ -89,8 +89,8 @@ def test_frictionless_contact_without_compliance_matches_fd(oracle_mod):
         np.testing.assert_allclose(g[key].reshape(-1)[:6], fd, atol=2e-3 * scale)
 
 
-def _skid_state(oracle_mod, worlds=2, steps=4, seed=3, K=16):
-    model, state, control = scenes.skid_steer(worlds, seed=seed)
+def _skid_state(oracle_mod, worlds=2, steps=4, seed=3, K=16, layout="clear"):
+    model, state, control = scenes.skid_steer(worlds, seed=seed, layout=layout)
     col = collide.PrimitiveCollider(model, gap=0.02, max_contacts_per_world=K)
     st = {k: np.asarray(v, np.float64) for k, v in state.items()}
     cfg = config(K=K)
