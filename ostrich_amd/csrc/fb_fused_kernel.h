@@ -47,7 +47,7 @@ __device__ unsigned long long g_phase_cycles[17][10];
     const int ci_ = DENSE ? (GW == 32 ? 0 : 1) : R;                                           \
     for (int k_ = 0; k_ < 8; ++k_) atomicAdd(&g_phase_cycles[ci_][k_], ph_[k_]);              \
     atomicAdd(&g_phase_cycles[ci_][8], (unsigned long long)n_it);                             \
-    atomicAdd(&g_phase_cycles[ci_][9], (unsigned long long)pcr_total);                        \
+    atomicAdd(&g_phase_cycles[ci_][9], (unsigned long long)*pcr_count);                       \
   }
 #else
 #define OFB_PHASE_INIT
@@ -839,7 +839,7 @@ __host__ __device__ inline FusedLayout fused_layout(int B, int J, int rows_cap, 
     l.rs = l.dense;
     l.hist = up4(l.dense + rs);
   }
-  l.words = l.hist + kHistWords;
+  l.words = l.hist + kHistWords + 4;  // + the world's running PCR count
   return l;
 }
 
@@ -939,6 +939,7 @@ fused_step_kernel(FusedParams P) {
     if (P.cfg.seed_iterate && i < n_r) l0 = P.lambda_seed[(size_t)w * L.Nc + original_row(L, i, n_c)];
     lam[k] = l0; lamb[k] = l0;
     act[k] = false; b0[k] = -1; b1[k] = -1;
+    J0[k] = svzero(); J1[k] = svzero(); Cr[k] = 0.f;
     rj[k] = 0; rk[k] = 0;
     if (i < L.Nj) { rj[k] = P.topo[i]; rk[k] = P.topo[L.Nj + i]; }
     else if (i < L.Nj + L.Nctrl) rj[k] = P.topo[2 * L.Nj + (i - L.Nj)];
@@ -968,7 +969,11 @@ fused_step_kernel(FusedParams P) {
   const float tol_sq = P.cfg.lin_tol >= 0.f ? P.cfg.lin_tol * P.cfg.lin_tol : -1.f;
   const float latol_sq = P.cfg.lin_atol >= 0.f ? P.cfg.lin_atol * P.cfg.lin_atol : -1.f;
   const float reg = P.cfg.regularization;
-  int n_it = 0, pcr_total = 0, best_idx = -1;
+  int n_it = 0, best_idx = -1;
+  // running PCR count of the step, kept by lane 0 in the world's LDS (a register copy came
+  // out wrong in the two largest instances, which run at the register file's edge)
+  int* const pcr_count = reinterpret_cast<int*>(sm + P.lay.hist + kHistWords);
+  if (lane == 0) *pcr_count = 0;
   float best_merit = 0.f, merit = 0.f;
 
   // rows of the last residual pass (the dense instances have the registers for them)
@@ -1011,8 +1016,9 @@ fused_step_kernel(FusedParams P) {
         // inactive contact / friction rows force lambda := 0 (contact_constraint.py:289-296,
         // friction_constraint.py:713-787)
         if (!o.active && i >= L.Nj + L.Nctrl) lam[k] = 0.f;
-      } else {
+      } else {  // a slot past the world's rows: no row, zero Jacobians (read by the sums)
         act[k] = false;
+        J0[k] = svzero(); J1[k] = svzero(); b0[k] = -1; b1[k] = -1; Cr[k] = 0.f;
       }
     }
     S.end(BHD, false, n_r);
@@ -1059,8 +1065,9 @@ fused_step_kernel(FusedParams P) {
     OFB_PHASE(2)
     if constexpr (DENSE) dense_build(D, Adense, lane, B, act[0], b0[0], b1[0], J0[0], J1[0], W0[0], W1[0], Cr[0]);
     OFB_PHASE(7)
-    pcr_total += pcr_solve<R, BJ, GW, DENSE>(sm, lane, B, n_r, S, V, G, act, b0, b1, J0, J1, Cr, Pi, rhs, bsq,
-                                             tol_sq, latol_sq, P.cfg.lin_max_iters, x, D, Adense);
+    const int pcr_it = pcr_solve<R, BJ, GW, DENSE>(sm, lane, B, n_r, S, V, G, act, b0, b1, J0, J1, Cr, Pi, rhs,
+                                                   bsq, tol_sq, latol_sq, P.cfg.lin_max_iters, x, D, Adense);
+    if (lane == 0) *pcr_count += pcr_it;
     OFB_PHASE(3)
 
     // ---- velocity recovery + Newton update (linear_system.py:344-377, newton_step.py:22-54) ----
@@ -1170,7 +1177,7 @@ fused_step_kernel(FusedParams P) {
     merit = group_sum<GW>(msum);
     if (P.hist_merit && lane == 0 && n_it < kHistIters) {  // ||h||^2 and the PCR iterations so far
       sm[P.lay.hist + n_it] = merit;
-      sm[P.lay.hist + kHistIters + n_it] = __int_as_float(pcr_total);
+      sm[P.lay.hist + kHistIters + n_it] = __int_as_float(*pcr_count);
     }
     OFB_PHASE(5)
 
@@ -1260,7 +1267,7 @@ fused_step_kernel(FusedParams P) {
   if (group_any<GW>(bj_flags & FLAG_PAIR_OVERFLOW)) bj_any |= FLAG_PAIR_OVERFLOW;
   if (lane == 0) {
     P.st_nr[w] = n_it;
-    P.st_pcr[w] = pcr_total;
+    P.st_pcr[w] = *pcr_count;
     P.st_merit[w] = picked_merit;
     P.st_best[w] = picked;
     int fl = (P.st_flags[w] & FLAG_CONTACT_OVERFLOW) | bj_any | seed_flag;

@@ -40,7 +40,7 @@ def test_plain_box_stack_matches_oracle(oracle_mod, gpu_device, termination):
 def test_fixed_iterations_plain_and_fused_agree(oracle_mod, gpu_device, path):
     """Two Newton iterations on skid-steer worlds: both GPU paths match the FP32 oracle
     within 1e-4 of scale."""
-    model, state, control = scenes.skid_steer(64, seed=3)
+    model, state, control = scenes.skid_steer(64, seed=3, layout="clear")
     base = config(K=128, path=path)
     cfg = type(base)(nr=NewtonRaphsonConfig(max_iters=2, backtrack_min_iter=0, atol=1e-5), linear=base.linear,
                      contacts=base.contacts, execution=base.execution)
