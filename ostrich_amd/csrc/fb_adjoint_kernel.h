@@ -77,6 +77,7 @@ struct AdjParams {
   int split;
   int xfer_off;  // LDS offset (floats, wave 0's region) of lambda / w_lambda for the others
   int fd_off;    // LDS offset of the row-local difference table (< 0: full-residual differences)
+  int rs_off;    // LDS offset (floats, each wave's region) of the lane-private body sums (< 0: none)
 };
 
 // Words of the row-local difference table: per row slot and side (body 0 / body 1) the 14
@@ -117,19 +118,22 @@ __device__ __forceinline__ float adjoint_row_term(const AdjParams& P, float* sm,
 template <int R>
 __device__ __forceinline__ float adjoint_weighted_residual(const AdjParams& P, float* sm, const float* lamprev,
                                                            int lane, int w, int n_r, int n_c, const float (&lam)[R],
-                                                           const float (&wl)[R]) {
+                                                           const float (&wl)[R], const BodySum<WAVE, false>& S) {
   const int B = P.m.B;
   dynamics_bodies(P, sm, lane, false);
   wsync();
+  S.begin(BHD, false);
   float s = 0.f;
 #pragma unroll
   for (int k = 0; k < R; ++k) {
     int i = lane + WAVE * k;
     if (i < n_r) {
-      RowOut o = assemble_row(P, sm, lamprev, w, i, n_c, lam[k]);
+      RowOut o = assemble_row(P, sm, lamprev, w, i, n_c, lam[k], LdsPoses(), false);
       s += wl[k] * o.hc;
+      S.add(BHD, i, o.active, o.b0, o.b1, o.J0 * (-lam[k] * P.dt), o.J1 * (-lam[k] * P.dt));
     }
   }
+  S.end(BHD, false, n_r);
   wsync();
   for (int e = lane; e < 6 * B; e += WAVE) {
     const float* bb = sm + (e / 6) * BS;
@@ -162,6 +166,9 @@ __global__ void __launch_bounds__(256) adjoint_step_kernel(AdjParams P) {
   int flags = 0;
   const int n_c = min(P.ccount[w], L.K);
   const int n_r = L.Nj + L.Nctrl + 3 * n_c;
+  // row -> body sums: lane-private copies when the launch gave each wave a region for
+  // them, else LDS atomics
+  const BodySum<WAVE, false> S{sm, sm + (P.rs_off >= 0 ? P.rs_off : 0), lane, B, P.rs_off >= 0 && rs_enabled(B)};
   float lam[R], x[R];
   int pcr_it = 0;
   if (sub == 0) {
@@ -206,12 +213,15 @@ __global__ void __launch_bounds__(256) adjoint_step_kernel(AdjParams P) {
   // ---- 1. J, C at the converged state ----
   dynamics_bodies(P, sm, lane, true);
   wsync();
+  S.begin(BHD, false);
 #pragma unroll
   for (int k = 0; k < R; ++k) {
     int i = lane + WAVE * k;
     Cr[k] = 0.f;
+    J0[k] = svzero(); J1[k] = svzero();
     if (i < n_r) {
-      RowOut o = assemble_row(P, sm, lamprev, w, i, n_c, lam[k]);
+      RowOut o = assemble_row(P, sm, lamprev, w, i, n_c, lam[k], LdsPoses(), false);
+      S.add(BHD, i, o.active, o.b0, o.b1, o.J0 * (-lam[k] * dt), o.J1 * (-lam[k] * dt));
       J0[k] = o.J0; J1[k] = o.J1; b0[k] = o.b0; b1[k] = o.b1; act[k] = o.active;
       Cr[k] = o.C + P.cfg.regularization + (o.active ? P.cfg.adjoint_regularization : 0.f);
       if (!o.active && i >= L.Nj + L.Nctrl) lam[k] = 0.f;
@@ -271,21 +281,14 @@ __global__ void __launch_bounds__(256) adjoint_step_kernel(AdjParams P) {
   }
   const float tol_sq = P.cfg.lin_tol >= 0.f ? P.cfg.lin_tol * P.cfg.lin_tol : -1.f;
   const float latol_sq = P.cfg.lin_atol >= 0.f ? P.cfg.lin_atol * P.cfg.lin_atol : -1.f;
-  // the operator's body sums use direct LDS atomics here (BodySum without private copies)
-  const BodySum<WAVE, false> S{sm, nullptr, lane, B, false};
   pcr_it = pcr_solve<R, BJ, WAVE, false>(sm, lane, B, n_r, S, V, G, act, b0, b1, J0, J1, Cr, Pi, rhs, bsq, tol_sq,
                                          latol_sq, P.cfg.lin_max_iters, x);
 
   // ---- 4. w_u = w_u0 - M^-1 J^T w_lambda ----
-  for (int e = lane; e < 6 * B; e += WAVE) sm[(e / 6) * BS + BACC + (e % 6)] = 0.f;
-  wsync();
+  S.begin(BACC, true);
 #pragma unroll
-  for (int k = 0; k < R; ++k) {
-    if (act[k]) {
-      if (b0[k] >= 0) atomic_add_sv(sm + b0[k] * BS + BACC, J0[k] * x[k]);
-      if (b1[k] >= 0) atomic_add_sv(sm + b1[k] * BS + BACC, J1[k] * x[k]);
-    }
-  }
+  for (int k = 0; k < R; ++k) S.add(BACC, lane + WAVE * k, act[k], b0[k], b1[k], J0[k] * x[k], J1[k] * x[k]);
+  S.end(BACC, true, n_r);
   wsync();
   for (int b = lane; b < B; b += WAVE) {
     float* bb = sm + b * BS;
@@ -436,10 +439,10 @@ __global__ void __launch_bounds__(256) adjoint_step_kernel(AdjParams P) {
       wsync();
       if (lane == 0) *ptr = orig + h;
       wsync();
-      float sp = adjoint_weighted_residual<R>(P, sm, lamprev, lane, w, n_r, n_c, lam, x);
+      float sp = adjoint_weighted_residual<R>(P, sm, lamprev, lane, w, n_r, n_c, lam, x, S);
       if (lane == 0) *ptr = orig - h;
       wsync();
-      float sn = adjoint_weighted_residual<R>(P, sm, lamprev, lane, w, n_r, n_c, lam, x);
+      float sn = adjoint_weighted_residual<R>(P, sm, lamprev, lane, w, n_r, n_c, lam, x, S);
       if (lane == 0) { *ptr = orig; gpose[d] = (sp - sn) / (2.f * h); }
       wsync();
     }

@@ -171,7 +171,7 @@ static int allow_lds(int code, const ofb_solver* s) {
     }
   }
   const int R = code == 0 ? 1 : code;
-  bytes = (size_t)adj_lds_words(s, R) * sizeof(float);
+  bytes = (size_t)(adj_lds_words(s, R) + rs_words(WAVE, s->m.B)) * sizeof(float);
   if (bytes > 65536) {
     AdjParams A{};
     if (adj_class_op(R, 1, A, 0, 0, bytes, nullptr, bj) != 0) {
@@ -443,7 +443,10 @@ int fused_adjoint(ofb_solver* s, const ofb_state_in* in, const ofb_state_out* ou
     P.wlist = s->cls_lists + (size_t)c * W;
     P.wcount = s->cls_counts + c;
     P.rows_cap = WAVE * R;
-    P.lds_per_wave = adj_lds_words(s, R);
+    // [forward's bodies, lambda_prev, adjoint scratch, block-Jacobi | lane-private body
+    // sums | transfer area | row-local difference table]
+    P.rs_off = rs_enabled(s->m.B) ? adj_lds_words(s, R) : -1;
+    P.lds_per_wave = adj_lds_words(s, R) + rs_words(WAVE, s->m.B);
     P.xfer_off = P.lds_per_wave;
     P.split = 1;
     P.fd_off = -1;
