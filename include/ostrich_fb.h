@@ -283,6 +283,16 @@ int ofb_step_backward(ofb_solver* solver, const ofb_state_in* state_in, const of
  * 620-640), e.g. at an episode reset: the next step cold-starts every contact. */
 int ofb_warm_start_reset(ofb_solver* solver, void* stream);
 
+/* The warm starter's snapshot as one opaque device block (no reference counterpart: the
+ * reference keeps it in engine buffers that a re-simulation overwrites).  A checkpointed
+ * rollout saves it with each checkpoint and restores it before re-simulating a window, so
+ * the re-simulated steps start Newton from the seeds the forward pass used.  _size gives
+ * the block's bytes (0 when the warm start is off); _get / _set copy it device to device
+ * on `stream` (caller-owned buffer of that size). */
+int ofb_warm_start_state_size(const ofb_solver* solver, int64_t* bytes);
+int ofb_warm_start_state_get(const ofb_solver* solver, void* dst, void* stream);
+int ofb_warm_start_state_set(ofb_solver* solver, const void* src, void* stream);
+
 /* Copies the Newton seed [W*N_c] the last step started from (warm starter output, or the
  * seed set with ofb_set_lambda_seed) - _constr_force after engine.py:55-59. */
 int ofb_get_lambda_seed(const ofb_solver* solver, float* seed_out, void* stream);

@@ -476,6 +476,54 @@ int ofb_warm_start_reset(ofb_solver* s, void* stream) {
   return 0;
 }
 
+// the snapshot buffers in the order of the opaque block: count [W] i32, body pairs
+// [W*K] i32 twice, midpoints and normals [W*K*3], normal impulses [W*K], tangential
+// impulses [W*K*2] (f32)
+static int ws_parts(const ofb_solver* s, void* (&p)[7], size_t (&b)[7]) {
+  const size_t W = s->m.W, WK = W * s->L.K;
+  p[0] = s->ws_count; p[1] = s->ws_b0; p[2] = s->ws_b1; p[3] = s->ws_p; p[4] = s->ws_n; p[5] = s->ws_ln;
+  p[6] = s->ws_lt;
+  b[0] = W * 4; b[1] = WK * 4; b[2] = WK * 4; b[3] = WK * 12; b[4] = WK * 12; b[5] = WK * 4; b[6] = WK * 8;
+  return s->ws_count ? 7 : 0;
+}
+
+int ofb_warm_start_state_size(const ofb_solver* s, int64_t* bytes) {
+  if (!s || !bytes) { set_error("null argument"); return -1; }
+  void* p[7];
+  size_t b[7];
+  const int n = ws_parts(s, p, b);
+  int64_t t = 0;
+  for (int i = 0; i < n; ++i) t += (int64_t)b[i];
+  *bytes = t;
+  return 0;
+}
+
+int ofb_warm_start_state_get(const ofb_solver* s, void* dst, void* stream) {
+  if (!s || !dst) { set_error("null argument"); return -1; }
+  void* p[7];
+  size_t b[7];
+  const int n = ws_parts(s, p, b);
+  char* d = static_cast<char*>(dst);
+  for (int i = 0; i < n; ++i) {
+    OFB_CHECK(hipMemcpyAsync(d, p[i], b[i], hipMemcpyDeviceToDevice, (hipStream_t)stream));
+    d += b[i];
+  }
+  return 0;
+}
+
+int ofb_warm_start_state_set(ofb_solver* s, const void* src, void* stream) {
+  if (!s || !src) { set_error("null argument"); return -1; }
+  void* p[7];
+  size_t b[7];
+  const int n = ws_parts(s, p, b);
+  const char* d = static_cast<const char*>(src);
+  for (int i = 0; i < n; ++i) {
+    OFB_CHECK(hipMemcpyAsync(p[i], d, b[i], hipMemcpyDeviceToDevice, (hipStream_t)stream));
+    d += b[i];
+  }
+  return 0;
+}
+
 int ofb_get_linesearch_steps(const ofb_solver* s, float* steps, int32_t* count) {
   if (!s || !count) { set_error("null argument"); return -1; }
   *count = s->ls_count;

@@ -1257,7 +1257,7 @@ fused_step_kernel(FusedParams P) {
     const float* hs = sm + P.lay.hist;
     for (int k = lane; k < P.hist_cap; k += GW) {
       const bool ran = k < n_it && k < kHistIters;
-      const int before = k > 0 ? __float_as_int(hs[kHistIters + k - 1]) : 0;
+      const int before = (ran && k > 0) ? __float_as_int(hs[kHistIters + k - 1]) : 0;
       P.hist_merit[(size_t)k * P.m.W + w] = ran ? hs[k] : __builtin_nanf("");
       P.hist_pcr[(size_t)k * P.m.W + w] = ran ? __float_as_int(hs[kHistIters + k]) - before : -1;
     }

@@ -7,13 +7,15 @@
 * `TrajectoryBuffer` - per-step records the adjoint needs (input/output state, control,
   contacts, lambda, lambda_prev_iter), as simulation/trajectory_buffer.py:106-351 keeps
   them, held on the device.
-* `CheckpointedRollout` - forward over N steps keeping only the states at every k-th step;
-  the backward re-simulates each window of k steps from its checkpoint into a
-  TrajectoryBuffer and runs the adjoint backwards through it
-  (simulation/differentiable_simulator.py:234-271 with the windowed recompute of
-  experiments/4_scalability_box/ostrich_sim_ckpt.py).  Memory is O(N/k + k) step records
-  instead of O(N).  The solver is deterministic (fixed accumulation order), so a
-  re-simulated window reproduces the forward bit for bit.
+* `CheckpointedRollout` - forward over N steps keeping only the states at every k-th step
+  (and, when the warm starter seeds Newton, its snapshot there); the backward
+  re-simulates each window of k steps from its checkpoint into a TrajectoryBuffer and
+  runs the adjoint backwards through it (simulation/differentiable_simulator.py:234-271
+  with the windowed recompute of experiments/4_scalability_box/ostrich_sim_ckpt.py).
+  Memory is O(N/k + k) step records instead of O(N).  The solver is deterministic (fixed
+  accumulation order) and a rollout starts from a reset warm starter, so it is a function
+  of (state0, controls) alone and a re-simulated window reproduces the forward bit for
+  bit.
 * `OstrichRollout` - a `torch.autograd.Function` over a whole checkpointed rollout.
 
 Gradient flow between steps follows the reference: dL/dq- of step t is added to the
@@ -41,13 +43,58 @@ class StepRecord:
 
 
 class TrajectoryBuffer:
-    """Step records of one window (simulation/trajectory_buffer.py:106-351)."""
+    """Step records (simulation/trajectory_buffer.py:106-351).
+
+    Two ways in: `save_step(record)` appends a StepRecord (the checkpointed rollout), and
+    the reference's `save_step(step_idx, data)` records the step an engine just took from
+    its EngineData (`engine.data`).  `load_step(step_idx, data)` puts record step_idx back
+    into an engine's data with dL/dq+, dL/du+ of the step's output state (state
+    step_idx + 1) so that `engine.step_backward()` differentiates it; `save_gradients`
+    and `save_pose_gradients` carry the step's input gradients to state step_idx
+    (trajectory_buffer.py:266-297)."""
 
     def __init__(self):
         self.steps: List[StepRecord] = []
+        self.body_q_grad = {}   # state index -> dL/dq
+        self.body_qd_grad = {}  # state index -> dL/du
+        self.input_grads = {}   # step index -> gradients of the step's inputs
 
-    def save_step(self, rec: StepRecord):
-        self.steps.append(rec)
+    def save_step(self, rec, data=None, contacts=None):
+        if isinstance(rec, StepRecord):
+            self.steps.append(rec)
+            return
+        step_idx = int(rec)
+        if step_idx != len(self.steps):
+            raise ValueError(f"steps are recorded in order: expected step {len(self.steps)}, got {step_idx}")
+        lam = data.lam if data.lam is not None else data.engine_lam()
+        lam_prev = data.lam_prev if data.lam_prev is not None else data.engine_lam_prev()
+        self.steps.append(StepRecord(data.state_in.clone(), data.state_out.clone(), data.control,
+                                     contacts if contacts is not None else data.contacts, lam.clone(),
+                                     lam_prev.clone()))
+
+    def set_output_grad(self, state_idx, grad_q, grad_qd):
+        """dL/dq, dL/du of state state_idx from the loss (the tape gradients the reference's
+        buffer holds in body_pose.grad / body_vel.grad)."""
+        self.body_q_grad[state_idx] = grad_q
+        self.body_qd_grad[state_idx] = grad_qd
+
+    def load_step(self, step_idx, data, contacts=None):
+        r = self.steps[step_idx]
+        data.state_in, data.state_out, data.control, data.contacts = r.state_in, r.state_out, r.control, r.contacts
+        data.lam, data.lam_prev = r.lam, r.lam_prev
+        data.body_q_grad = self.body_q_grad.get(step_idx + 1)
+        data.body_qd_grad = self.body_qd_grad.get(step_idx + 1)
+
+    def save_gradients(self, step_idx, data):
+        """dL/du of state step_idx and the step's input gradients from data.grads."""
+        self.body_qd_grad[step_idx] = data.grads["body_qd"]
+        self.input_grads[step_idx] = {k: v for k, v in data.grads.items() if k not in ("body_q", "body_qd")}
+
+    def save_pose_gradients(self, step_idx, data):
+        """Adds dL/dq- of the step to the pose gradient of state step_idx."""
+        g = data.grads["body_q"]
+        prev = self.body_q_grad.get(step_idx)
+        self.body_q_grad[step_idx] = g if prev is None else prev + g
 
     def __len__(self):
         return len(self.steps)
@@ -57,6 +104,9 @@ class TrajectoryBuffer:
 
     def clear(self):
         self.steps.clear()
+        self.body_q_grad.clear()
+        self.body_qd_grad.clear()
+        self.input_grads.clear()
 
 
 def _record_step(engine, state_in: State, control: Control, contacts: Contacts, dt: float) -> StepRecord:
@@ -74,13 +124,15 @@ class OstrichStep(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, engine, contacts, dt, body_q, body_qd, body_f, target_pos, target_vel, shape_mu=None):
-        if shape_mu is not None:
-            engine.set_shape_mu(shape_mu.detach())
+        # friction coefficients of this step: the given ones, else the model's; the engine
+        # holds one set, so the step's set is kept and put back for its backward
+        mu = shape_mu.detach().clone() if shape_mu is not None else engine.model_shape_mu()
+        engine.set_shape_mu(mu)
         s_in = State(body_q.detach().clone().contiguous(), body_qd.detach().clone().contiguous(),
                      body_f.detach().clone().contiguous())
         ctl = Control(target_pos.detach().clone().contiguous(), target_vel.detach().clone().contiguous())
         rec = _record_step(engine, s_in, ctl, contacts, dt)
-        ctx.engine, ctx.dt, ctx.rec = engine, dt, rec
+        ctx.engine, ctx.dt, ctx.rec, ctx.mu = engine, dt, rec, mu
         ctx.has_mu = shape_mu is not None
         return rec.state_out.body_q.clone(), rec.state_out.body_qd.clone()
 
@@ -91,8 +143,9 @@ class OstrichStep(torch.autograd.Function):
             g_q = torch.zeros_like(r.state_out.body_q)
         if g_qd is None:
             g_qd = torch.zeros_like(r.state_out.body_qd)
-        g = ctx.engine.step_backward(r.state_in, r.state_out, r.control, r.contacts, ctx.dt, r.lam, r.lam_prev,
-                                     g_q, g_qd)
+        ctx.engine.set_shape_mu(ctx.mu)  # the friction of this step, whatever later steps set
+        g = ctx.engine.step_backward_explicit(r.state_in, r.state_out, r.control, r.contacts, ctx.dt, r.lam,
+                                              r.lam_prev, g_q, g_qd)
         gmu = g["shape_mu"].reshape(-1) if ctx.has_mu else None
         return (None, None, None, g["body_q"], g["body_qd"], g["body_f"],
                 g["joint_target_pos"].reshape(r.control.joint_target_pos.shape),
@@ -111,6 +164,7 @@ class CheckpointedRollout:
         gn = engine.config.adjoint.gradient_normalization if gradient_normalization is None else gradient_normalization
         self.gradient_normalization = gn
         self.checkpoints: List[State] = []
+        self.ws_checkpoints: List[Optional[torch.Tensor]] = []
         self.controls: List[Control] = []
         self.final: Optional[State] = None
         self.recomputed_steps = 0
@@ -119,16 +173,27 @@ class CheckpointedRollout:
         contacts = self.collide(state.body_q)
         return _record_step(self.engine, state, self.controls[t], contacts, self.dt)
 
+    def _seeds_from_warm_start(self):
+        ws = self.engine.config.warm_start
+        return ws.enabled and ws.seed_iterate
+
     def forward(self, state0: State, controls: List[Control]) -> State:
         """Runs len(controls) steps from state0; keeps the states at steps 0, k, 2k, ...
-        (no per-step records: the backward re-simulates each window from its checkpoint
-        with the same kernels, which reproduces these steps bit for bit)."""
+        and, when the warm starter seeds Newton, its snapshot there (no per-step records:
+        the backward re-simulates each window from its checkpoint with the same kernels,
+        which reproduces these steps bit for bit).  The rollout starts from a reset warm
+        starter, so it depends on (state0, controls) only."""
         self.controls = list(controls)
         self.checkpoints = []
+        self.ws_checkpoints = []
+        seeds = self._seeds_from_warm_start()
+        if seeds:
+            self.engine.reset_warm_start()
         s, nxt = state0.clone(), state0.clone()
         for t in range(len(self.controls)):
             if t % self.k == 0:
                 self.checkpoints.append(s.clone())
+                self.ws_checkpoints.append(self.engine.warm_start_state() if seeds else None)
             self.engine.step(s, nxt, self.controls[t], self.collide(s.body_q), self.dt)
             s, nxt = nxt, s
         self.final = s
@@ -139,6 +204,8 @@ class CheckpointedRollout:
         t0 = j * self.k
         t1 = min(t0 + self.k, len(self.controls))
         s = self.checkpoints[j].clone()
+        if self.ws_checkpoints and self.ws_checkpoints[j] is not None:
+            self.engine.set_warm_start_state(self.ws_checkpoints[j])
         for t in range(t0, t1):
             rec = self._step(s, t)
             buf.save_step(rec)
@@ -166,8 +233,9 @@ class CheckpointedRollout:
             for i in reversed(range(len(buf))):
                 t = t0 + i
                 r = buf[i]
-                g = self.engine.step_backward(r.state_in, r.state_out, r.control, r.contacts, self.dt, r.lam,
-                                              r.lam_prev, gq, gu, adjoint_multipliers=adjoint_log is not None)
+                g = self.engine.step_backward_explicit(r.state_in, r.state_out, r.control, r.contacts, self.dt,
+                                                       r.lam, r.lam_prev, gq, gu,
+                                                       adjoint_multipliers=adjoint_log is not None)
                 if adjoint_log is not None:
                     adjoint_log.capture_step(t, g, gq, gu)
                 out["body_f"][t] = g["body_f"]

@@ -20,6 +20,39 @@ def _ptr(t):
     return C.c_void_p(t.data_ptr()) if t is not None else None
 
 
+class EngineData:
+    """The step data `step_backward()` reads and the gradients it writes, the part of the
+    reference's EngineData (core/engine_data.py:42-300) that its adjoint uses: the step's
+    input state (q-, u-, f), output state (q+, u+), control, contacts, dt, lambda and
+    lambda_prev_iter, the incoming dL/dq+ and dL/du+, and after the call the gradients
+    of the step's inputs (`grads`: body_q, body_qd, body_f, joint_target_pos,
+    joint_target_vel, joint_target_ke, joint_target_kd, shape_mu).
+
+    `OstrichEngine.step` fills the step fields with the tensors it was called with (lambda
+    and lambda_prev_iter are read from the solver when `step_backward()` runs, unless set);
+    `TrajectoryBuffer.load_step` (ostrich_amd.differentiable) overwrites them with a
+    recorded step, as simulation/trajectory_buffer.py:235-264 does."""
+
+    def __init__(self, engine=None):
+        self._engine = engine
+        self.state_in = self.state_out = self.control = self.contacts = None
+        self.dt = None
+        self.lam = self.lam_prev = None
+        self.body_q_grad = self.body_qd_grad = None
+        self.grads = {}
+
+    def engine_lam(self):
+        """lambda of the engine's last step (its solver buffer)."""
+        return self._engine.constraint_impulses()
+
+    def engine_lam_prev(self):
+        return self._engine.constraint_impulses_prev()
+
+    def zero_gradients(self):
+        """EngineData.zero_gradients: clears the outputs of the last step_backward()."""
+        self.grads = {}
+
+
 class OstrichEngine:
     def __init__(self, model, sim_steps=None, config: OstrichEngineConfig = OstrichEngineConfig(),
                  logging_config=None, differentiable_simulation=False, device="cuda:0",
@@ -61,6 +94,7 @@ class OstrichEngine:
         self._out = _abi.OfbStateOut()
         self._ctl = _abi.OfbControl()
         self._con = _abi.OfbContacts()
+        self.data = EngineData(self)
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -92,6 +126,9 @@ class OstrichEngine:
         _lib.check(self.lib.ofb_step(self._h, C.byref(self._in), C.byref(self._out), C.byref(self._ctl),
                                      C.byref(c), C.c_float(dt), self._stream()), "ofb_step")
         self.timestep += 1
+        d = self.data
+        d.state_in, d.state_out, d.control, d.contacts, d.dt = state_in, state_out, control, contacts, dt
+        d.lam = d.lam_prev = None  # the solver's own buffers hold this step's
 
     def reset_timestep_counter(self):
         """core/base_engine.py:278-279."""
@@ -162,6 +199,26 @@ class OstrichEngine:
     def reset_warm_start(self):
         """Forgets the warm starter's snapshot (next step cold-starts every contact)."""
         _lib.check(self.lib.ofb_warm_start_reset(self._h, self._stream()), "ofb_warm_start_reset")
+
+    def warm_start_state(self):
+        """The warm starter's snapshot as an opaque uint8 device tensor (empty when the warm
+        start is off); `set_warm_start_state` puts it back."""
+        n = C.c_int64()
+        _lib.check(self.lib.ofb_warm_start_state_size(self._h, C.byref(n)), "ofb_warm_start_state_size")
+        buf = torch.empty(n.value, dtype=torch.uint8, device=self.device)
+        if n.value:
+            _lib.check(self.lib.ofb_warm_start_state_get(self._h, _ptr(buf), self._stream()),
+                       "ofb_warm_start_state_get")
+        return buf
+
+    def set_warm_start_state(self, buf):
+        n = C.c_int64()
+        _lib.check(self.lib.ofb_warm_start_state_size(self._h, C.byref(n)), "ofb_warm_start_state_size")
+        if buf.numel() != n.value or buf.dtype != torch.uint8 or buf.device != self.device:
+            raise ValueError(f"warm-start state must be {n.value} uint8 bytes on {self.device}")
+        if n.value:
+            _lib.check(self.lib.ofb_warm_start_state_set(self._h, _ptr(buf.contiguous()), self._stream()),
+                       "ofb_warm_start_state_set")
 
     def lambda_seed(self):
         """The Newton seed [W, N_c] the last step started from."""
@@ -308,6 +365,10 @@ class OstrichEngine:
         _lib.check(self.lib.ofb_get_lambda_prev(self._h, _ptr(lam), self._stream()), "ofb_get_lambda_prev")
         return lam
 
+    def model_shape_mu(self):
+        """The model's per-shape friction coefficients [W*(S+G)] on the device."""
+        return torch.as_tensor(self.model.arrays["shape_mu"], dtype=torch.float32, device=self.device).contiguous()
+
     def set_shape_mu(self, shape_mu):
         """Replaces the per-shape friction coefficients [W*(S+G)] copied at construction."""
         mu = shape_mu.to(self.device, torch.float32).contiguous().reshape(-1)
@@ -329,19 +390,38 @@ class OstrichEngine:
         c.margin1 = contacts.rigid_contact_margin1.data_ptr()
         return c
 
-    def step_backward(self, state_in: State, state_out: State, control: Control, contacts: Contacts, dt: float,
-                      lam, lam_prev, grad_body_q, grad_body_qd, adjoint_multipliers=False):
+    def step_backward(self, *args, **kwargs):
         """Adjoint of one step (core/base_engine.py:528-743).
 
-        The reference reads the step's data from engine buffers that its TrajectoryBuffer
-        restores (simulation/trajectory_buffer.py:229-264); here the same data are
-        arguments: the step's input state (q-, u-, f), its output state (q+, u+), control,
-        contacts, lambda and lambda_prev_iter (`constraint_impulses()` /
-        `constraint_impulses_prev()` after that step) and dL/dq+, dL/du+.  Returns a dict of
-        dL/d(input): body_q, body_qd, body_f, joint_target_pos, joint_target_vel,
-        joint_target_ke, joint_target_kd, shape_mu (per world copy), plus w_lambda and w_u
-        when `adjoint_multipliers`.  Per-world adjoint PCR counts and flags are then in
-        `stats()`."""
+        `step_backward()` with no arguments is the reference's form: it reads the step
+        from `self.data` (filled by the last `step` or by `TrajectoryBuffer.load_step`),
+        with dL/dq+ and dL/du+ from data.body_q_grad / data.body_qd_grad (zero when unset),
+        and leaves the gradients in data.grads (also returned).  The explicit form takes
+        the same data as arguments: `step_backward(state_in, state_out, control, contacts,
+        dt, lam, lam_prev, grad_body_q, grad_body_qd, adjoint_multipliers=False)`.  See
+        `step_backward_explicit`."""
+        if args or any(k != "adjoint_multipliers" for k in kwargs):
+            return self.step_backward_explicit(*args, **kwargs)
+        d = self.data
+        if d.state_in is None:
+            raise RuntimeError("step_backward() needs a step: call step() or TrajectoryBuffer.load_step first")
+        lam = d.lam if d.lam is not None else self.constraint_impulses()
+        lam_prev = d.lam_prev if d.lam_prev is not None else self.constraint_impulses_prev()
+        gq = d.body_q_grad if d.body_q_grad is not None else torch.zeros_like(d.state_out.body_q)
+        gu = d.body_qd_grad if d.body_qd_grad is not None else torch.zeros_like(d.state_out.body_qd)
+        d.grads = self.step_backward_explicit(d.state_in, d.state_out, d.control, d.contacts, d.dt, lam, lam_prev,
+                                              gq, gu, **kwargs)
+        return d.grads
+
+    def step_backward_explicit(self, state_in: State, state_out: State, control: Control, contacts: Contacts,
+                               dt: float, lam, lam_prev, grad_body_q, grad_body_qd, adjoint_multipliers=False):
+        """Adjoint of one step with its data as arguments: the step's input state (q-, u-,
+        f), its output state (q+, u+), control, contacts, lambda and lambda_prev_iter
+        (`constraint_impulses()` / `constraint_impulses_prev()` after that step) and dL/dq+,
+        dL/du+.  Returns a dict of dL/d(input): body_q, body_qd, body_f, joint_target_pos,
+        joint_target_vel, joint_target_ke, joint_target_kd, shape_mu (per world copy), plus
+        w_lambda and w_u when `adjoint_multipliers`.  Per-world adjoint PCR counts and flags
+        are then in `stats()`."""
         if self.config.execution.path != "fused":
             raise NotImplementedError("step_backward runs on the fused path")
         d, W = self.device, self.num_worlds

@@ -117,7 +117,9 @@ class SimulationLog:
     * constraint data: _constr_force (lambda [W, N_c], reference row layout);
     * solver history: iter_count, res_norm_sq, candidates_best_idx, flags, pcr_iter_count
       [W], and per Newton iteration candidates_res_norm_sq and pcr_history_iter_count
-      [newton_max_iters, W] (NaN / -1 past a world's last iteration).
+      [newton_max_iters, W] (NaN / -1 past a world's last iteration and past the 32nd: the
+      fused kernel keeps the first 32 iterations of a world on chip, so with
+      nr.max_iters > 32 rows 32 and later read NaN / -1 even for iterations that ran).
 
     The engine's iteration history is attached for the log's lifetime."""
 

@@ -119,3 +119,82 @@ def test_autograd_functions_match_direct_calls(gpu_device):
     ro.forward(s0, controls)
     gr = ro.backward(gq, gu)
     assert torch.equal(tvs.grad, gr["joint_target_vel"].reshape(N, -1))
+
+
+def _setup_seeded(gpu_device, worlds=16):
+    """As _setup, with the cross-step warm starter seeding Newton (warm_start.seed_iterate)."""
+    from ostrich_amd import OstrichEngine
+    from ostrich_amd.config import WarmStartConfig
+    model, cfg, _, collide_fn, controls, s0 = _setup(gpu_device, worlds)
+    base = config(K=32)
+    cfg = type(base)(nr=base.nr, linear=base.linear, contacts=base.contacts, execution=base.execution,
+                     warm_start=WarmStartConfig(enabled=True, seed_iterate=True))
+    return model, cfg, OstrichEngine(model, config=cfg, device=gpu_device), collide_fn, controls, s0
+
+
+def test_seeded_rollout_is_a_function_of_its_inputs(gpu_device):
+    """With the warm starter seeding Newton, each checkpoint carries the warm starter's
+    snapshot and a rollout starts from a reset one: checkpoint spacing 1, 3 and N give
+    bit-identical final states and gradients, and running the rollout again (after other
+    steps have changed the warm starter) gives the same bits."""
+    model, cfg, eng, collide_fn, controls, s0 = _setup_seeded(gpu_device)
+    gq, gu = _seed(model, gpu_device, 3)
+    res = []
+    for k in (1, 3, N, 1):
+        ro = CheckpointedRollout(eng, collide_fn, DT, checkpoint_every=k)
+        fin = ro.forward(s0, controls)
+        g = ro.backward(gq, gu)
+        res.append((fin.body_q.cpu(), {n: v.cpu() for n, v in g.items()}))
+        # unrelated steps in between move the engine's warm-start snapshot
+        junk = s0.clone()
+        eng.step(s0, junk, controls[0], collide_fn(s0.body_q), DT)
+    for fin, g in res[1:]:
+        assert torch.equal(fin, res[0][0])
+        for n in g:
+            assert torch.equal(g[n], res[0][1][n]), n
+    assert res[0][1]["joint_target_vel"].abs().max() > 0
+
+
+def test_step_backward_without_arguments(gpu_device):
+    """The reference's form (core/base_engine.py:528, simulation/differentiable_simulator.py:
+    234-271): the forward loop records each step with TrajectoryBuffer.save_step(i,
+    engine.data), the backward loop restores it with load_step(i, engine.data), calls
+    engine.step_backward() with no arguments and carries the gradients with
+    save_gradients / save_pose_gradients.  It gives the same bits as the explicit form
+    (CheckpointedRollout.backward), and right after a step the no-argument call equals the
+    explicit call on that step's data."""
+    from ostrich_amd.differentiable import TrajectoryBuffer
+    model, cfg, eng, collide_fn, controls, s0 = _setup(gpu_device, worlds=8)
+    gq, gu = _seed(model, gpu_device, 4)
+    # one step: no-argument call right after it
+    s1 = s0.clone()
+    con = collide_fn(s0.body_q)
+    eng.step(s0, s1, controls[0], con, DT)
+    eng.data.body_q_grad, eng.data.body_qd_grad = gq, gu
+    g0 = {k: v.clone() for k, v in eng.step_backward().items()}
+    g1 = eng.step_backward(s0, s1, controls[0], con, DT, eng.constraint_impulses(), eng.constraint_impulses_prev(),
+                           gq, gu)
+    for k in g1:
+        assert torch.equal(g0[k], g1[k]), k
+    # the reference's loop over a rollout
+    traj = TrajectoryBuffer()
+    states = [s0.clone()]
+    for t in range(N):
+        nxt = states[-1].clone()
+        eng.step(states[-1], nxt, controls[t], collide_fn(states[-1].body_q), DT)
+        traj.save_step(t, eng.data)
+        states.append(nxt)
+    traj.set_output_grad(N, gq, gu)
+    for t in reversed(range(N)):
+        traj.load_step(t, eng.data)
+        eng.data.zero_gradients()
+        eng.step_backward()
+        traj.save_gradients(t, eng.data)
+        traj.save_pose_gradients(t, eng.data)
+    ro = CheckpointedRollout(eng, collide_fn, DT, checkpoint_every=N)
+    ro.forward(s0, controls)
+    gr = ro.backward(gq, gu)
+    assert torch.equal(traj.body_q_grad[0], gr["body_q"])
+    assert torch.equal(traj.body_qd_grad[0], gr["body_qd"])
+    for t in range(N):
+        assert torch.equal(traj.input_grads[t]["joint_target_vel"], gr["joint_target_vel"][t])
