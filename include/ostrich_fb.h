@@ -310,6 +310,17 @@ int ofb_mlp_forward(const ofb_mlp* net, const float* x, int n, float* y, void* s
 /* gradient of mean_rows sum_c (y_c - target_c)^2 w.r.t. the parameters; loss may be NULL */
 int ofb_mlp_gradient(ofb_mlp* net, const float* x, const float* target, int n, float* grad, float* loss,
                      void* stream);
+/* ofb_mlp_gradient that allocates nothing.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight
+ * training loops, harnesses that forbid hipMalloc).  ofb_mlp_gradient grows its per-block
+ * gradient slots with hipMalloc when n grows; this one keeps them in a file-scope device
+ * array of the library (at most 8192 rows: 3.2 MB that each GPU holds until the process
+ * ends) and calls no hipMalloc / hipFree.  Same kernels, so the same bits as
+ * ofb_mlp_gradient.  grad (NULL: the network's own buffer) and loss (may be NULL) are
+ * device pointers.  Returns -2 for n > 8192.  Calls that overlap on one GPU share the
+ * array and race: serialise them (one stream, or events). */
+int ofb_mlp_gradient_noalloc(ofb_mlp* net, const float* x, const float* target, int n, float* grad, float* loss,
+                             void* stream);
 /* one Adam step (beta 0.9 / 0.999, eps 1e-8) on that loss */
 int ofb_mlp_train_step(ofb_mlp* net, const float* x, const float* target, int n, float lr, float* loss,
                        void* stream);

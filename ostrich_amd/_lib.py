@@ -19,6 +19,7 @@ EXPORTS = (
     "ofb_collider_create", "ofb_collider_destroy", "ofb_collide", "ofb_set_iteration_history",
     "ofb_log_create", "ofb_log_capture", "ofb_log_destroy", "ofb_set_class_timing", "ofb_get_class_times",
     "ofb_warm_start_state_size", "ofb_warm_start_state_get", "ofb_warm_start_state_set",
+    "ofb_mlp_gradient_noalloc",
 )
 
 _lib = None
@@ -71,6 +72,7 @@ def load():
     lib.ofb_mlp_get_params.argtypes = [vp, vp, vp]
     lib.ofb_mlp_forward.argtypes = [vp, vp, C.c_int, vp, vp]
     lib.ofb_mlp_gradient.argtypes = [vp, vp, vp, C.c_int, vp, vp, vp]
+    lib.ofb_mlp_gradient_noalloc.argtypes = [vp, vp, vp, C.c_int, vp, vp, vp]
     lib.ofb_mlp_train_step.argtypes = [vp, vp, vp, C.c_int, C.c_float, vp, vp]
     lib.ofb_set_warm_start_net.argtypes = [vp, vp]
     lib.ofb_get_contact_features.argtypes = [vp, vp, vp, vp]

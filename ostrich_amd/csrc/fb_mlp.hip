@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "fb_solver.h"
@@ -350,6 +351,42 @@ static int mlp_grad_launch(ofb_mlp* net, const float* x, const float* t, int n, 
   return hipGetLastError() == hipSuccess ? 0 : -4;
 }
 
+// ---- gradient without allocation (ofb_mlp_gradient_noalloc) -------------------------
+// The per-block gradient slots and losses of at most kNoAllocMaxRows rows live in a
+// file-scope device array that every GPU the library runs on holds until the process
+// ends (3.2 MB).  On the host its name is not a device address: host code uses only the
+// address hipGetSymbolAddress returns for the current device, looked up once per device
+// on the entry point's first call there and kept in g_scratch_ptr.  Every block of the
+// gradient kernel writes its whole slot (all NPARAM entries and its loss), so nothing is
+// cleared between calls.  Calls that overlap on one GPU share the array and race.
+constexpr int kNoAllocMaxRows = 8192;
+constexpr int kNoAllocBlocks = kNoAllocMaxRows / ROWS;
+static __device__ __align__(256) float scratch[(size_t)kNoAllocBlocks * NPARAM + kNoAllocBlocks];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static float* g_scratch_ptr[kMaxDevices];
+
+// this device's copy of `scratch` (null with the error set on failure)
+static float* scratch_on_current_device() {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+    set_error("no current HIP device (or device index >= %d)", kMaxDevices);
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_error("hipGetSymbolAddress of the gradient scratch failed on device %d: %s", dev, hipGetErrorString(e));
+      return nullptr;
+    }
+    g_scratch_ptr[dev] = static_cast<float*>(p);
+  }
+  return g_scratch_ptr[dev];
+}
+
 namespace ofb {
 int mlp_forward_packed(const ofb_mlp* net, const float* x, const int32_t* count, int W, int K, int32_t* off,
                        float* y, hipStream_t st) {
@@ -436,6 +473,27 @@ int ofb_mlp_gradient(ofb_mlp* net, const float* x, const float* target, int n, f
     set_error("copy of mlp gradient failed");
     return -4;
   }
+  return 0;
+}
+
+int ofb_mlp_gradient_noalloc(ofb_mlp* net, const float* x, const float* target, int n, float* grad, float* loss,
+                             void* stream) {
+  if (!net || !x || !target || n < 1) { set_error("invalid argument"); return -1; }
+  if (n > kNoAllocMaxRows) {
+    set_error("ofb_mlp_gradient_noalloc takes at most %d rows (got %d); use ofb_mlp_gradient", kNoAllocMaxRows, n);
+    return -2;
+  }
+  float* sp = scratch_on_current_device();
+  if (!sp) return -4;
+  hipStream_t st = (hipStream_t)stream;
+  const int blocks = (n + ROWS - 1) / ROWS;
+  float* gpart = sp;
+  float* lpart = sp + (size_t)kNoAllocBlocks * NPARAM;
+  hipLaunchKernelGGL(mlp_grad_kernel, dim3(blocks), dim3(64), 0, st, net->n.w, net->n.wb, x, target, n,
+                     1.f / (float)n, gpart, lpart);
+  hipLaunchKernelGGL(mlp_reduce_kernel, dim3((NPARAM + 255) / 256), dim3(256), 0, st, gpart, blocks,
+                     grad ? grad : net->n.grad, lpart, loss);
+  if (hipGetLastError() != hipSuccess) { set_error("mlp gradient launch failed"); return -4; }
   return 0;
 }
 
