@@ -462,3 +462,86 @@ def test_tracked_flipper_rollout_parity(oracle_mod, gpu_device):
               flush=True)
         assert (d[determined] <= 1).all(), d[determined]
         st = {k: v.astype(np.float32) for k, v in dict(st, body_q=o32["body_q"], body_qd=o32["body_qd"]).items()}
+
+
+def _stribeck_worlds(speeds):
+    """100 kg cubes of 1 m with Stribeck friction (mu 0.5, stiction scale 2, v_s 0.2: the
+    reference's Stribeck scene, tests/test_stribeck_friction.py:113-146), one per kick
+    speed, resting on the ground."""
+    from ostrich_amd.model import GEO_BOX as _BOX, GEO_PLANE as _PLANE, IDENTITY7 as _ID, Model as _Model
+    from ostrich_amd.model import Shape as _Shape, WorldBuilder as _WB, box_inertia as _bi
+    worlds = []
+    for _ in speeds:
+        wb = _WB()
+        b = wb.add_body(100.0, _bi(100.0, 0.5, 0.5, 0.5))
+        wb.add_shape(_Shape(_BOX, b, _ID, (0.5, 0.5, 0.5), mu=0.5, mu_stiction_scale=2.0, v_stribeck=0.2))
+        wb.add_joint_free(b)
+        worlds.append(wb)
+    model = _Model.from_worlds(worlds, [_Shape(_PLANE, -1, _ID, (0, 0, 0), mu=0.5)])
+    W = len(speeds)
+    state = dict(body_q=np.tile([0, 0, 0.5, 0, 0, 0, 1.0], (W, 1)).astype(np.float32),
+                 body_qd=np.zeros((W, 6), np.float32), body_f=np.zeros((W, 6), np.float32))
+    state["body_qd"][:, 0] = speeds
+    return model, state
+
+
+STRIBECK_SPEEDS = np.array([0.02, 0.05, 0.1, 0.2, 0.4, 0.8, 1.4, 2.0])
+
+
+@pytest.mark.parametrize("nr_iters", [2, 3])
+def test_stribeck_fixed_newton_iterations(oracle_mod, gpu_device, nr_iters):
+    """Stribeck friction (friction_constraint.py:269-281: mu scaled by 1 + (s - 1)
+    exp(-|v_t| / v_s) from the current slip) on the GPU after 2-3 Newton iterations from
+    identical inputs, each world kicked to another speed between 0.02 and 2 m/s: every
+    world within four times its FP32-vs-FP64 spread plus 1e-4 of scale, identical Newton
+    counts."""
+    from ostrich_amd.config import NewtonRaphsonConfig
+    model, st = _stribeck_worlds(STRIBECK_SPEEDS)
+    base = config(K=16, pcr_iters=64)
+    cfg = type(base)(nr=NewtonRaphsonConfig(max_iters=nr_iters, backtrack_min_iter=0, atol=1e-5),
+                     linear=base.linear, contacts=base.contacts, execution=base.execution)
+    col = collide.PrimitiveCollider(model, gap=0.05, max_contacts_per_world=16)
+    c = col.collide(torch.as_tensor(st["body_q"]))
+    c = {k: (v.numpy() if hasattr(v, "numpy") else v) for k, v in c.items()}
+    ctl = zero_control(model)
+    o = oracle_mod.step(model, cfg, st, ctl, c, 0.005, precision=32)
+    o64 = oracle_mod.step(model, cfg, st, ctl, c, 0.005, precision=64)
+    g = _gpu_step(_engine(model, cfg, gpu_device), st, ctl, c, 0.005, gpu_device)
+    W = model.num_worlds
+    for key, scale in (("body_q", 1.0), ("body_qd", G * 0.005)):
+        eg = np.abs(g[key] - o[key]).reshape(W, -1).max(1)
+        ep = np.abs(o[key] - o64[key]).reshape(W, -1).max(1)
+        print(f"[stribeck] nr={nr_iters} {key}: gpu-vs-f32 max {eg.max():.2e}, f32-vs-f64 max {ep.max():.2e}",
+              flush=True)
+        assert not (eg > 4.0 * ep + 1e-4 * scale).any(), (key, eg, ep)
+    assert np.array_equal(g["nr_iters"], o["nr_iters"])
+
+
+def test_stribeck_rollout_parity(oracle_mod, gpu_device):
+    """Five closed-loop steps of the kicked Stribeck cubes at dt 0.005 (full Newton loops:
+    the stiction-to-sliding transition keeps the lagged mu(v) iteration from converging,
+    so every world stops at the cap): the GPU's per-world deviation from the FP32 oracle
+    is distributed like the FP32 oracle's deviation from FP64 (median within 3x, maximum
+    within 4x the largest spread, plus 1e-4 of scale), as for the other capped loops."""
+    model, st = _stribeck_worlds(STRIBECK_SPEEDS)
+    W = model.num_worlds
+    cfg = config(K=16, pcr_iters=64)
+    eng = _engine(model, cfg, gpu_device)
+    col = collide.PrimitiveCollider(model, gap=0.05, max_contacts_per_world=16)
+    ctl = zero_control(model)
+    st = {k: v.astype(np.float64) for k, v in st.items()}
+    for _ in range(5):
+        c = col.collide(torch.as_tensor(st["body_q"], dtype=torch.float32))
+        c = {k: (v.numpy() if hasattr(v, "numpy") else v) for k, v in c.items()}
+        s32 = {k: v.astype(np.float32) for k, v in st.items()}
+        g = _gpu_step(eng, s32, ctl, c, 0.005, gpu_device)
+        o32 = oracle_mod.step(model, cfg, s32, ctl, c, 0.005, precision=32)
+        o64 = oracle_mod.step(model, cfg, s32, ctl, c, 0.005, precision=64)
+        for key, scale in (("body_q", 1.0), ("body_qd", G * 0.005)):
+            eg = np.abs(g[key] - o32[key]).reshape(W, -1).max(1)
+            ep = np.abs(o32[key] - o64[key]).reshape(W, -1).max(1)
+            print(f"[stribeck rollout] {key}: median {np.median(eg):.2e} vs {np.median(ep):.2e}, "
+                  f"max {eg.max():.2e} vs {ep.max():.2e}", flush=True)
+            assert np.median(eg) <= 3.0 * np.median(ep) + 1e-4 * scale, (key, np.median(eg), np.median(ep))
+            assert eg.max() <= 4.0 * ep.max() + 1e-4 * scale, (key, eg.max(), ep.max())
+        st = dict(st, body_q=o32["body_q"], body_qd=o32["body_qd"])
