@@ -302,7 +302,7 @@ def adjoint_leg(device, worlds=1024, steps=50, checkpoint_every=10, seed=3):
     from ostrich_amd.differentiable import CheckpointedRollout
     model, state, control = scenes.skid_steer(worlds, seed=seed, max_obstacles=0, mu_range=(0.8, 0.8))
     cfg = solver_config("fused")
-    eng = OstrichEngine(model, config=cfg, device=device)
+    eng = OstrichEngine(model, config=cfg, device=device, differentiable_simulation=True)
     col = collide.HipCollider(model, device=device, gap=0.02, max_contacts_per_world=K_CONTACTS)
     collide_fn = lambda q: col.collide(q).clone()  # each step record keeps its own contacts
     rng = np.random.default_rng(seed)

@@ -233,8 +233,16 @@ int ofb_get_lambda(const ofb_solver* solver, float* lambda_out, void* stream);
 /* Copies lambda_prev_iter [W*N_c] as the last step left it: the snapshot taken at the
  * start of its last Newton iteration (core/base_engine.py:441), which backtracking does
  * not restore.  With lambda it is what TrajectoryBuffer.save_step keeps per step
- * (simulation/trajectory_buffer.py:222-223) for the adjoint.  Fused path only. */
+ * (simulation/trajectory_buffer.py:222-223) for the adjoint.  Fused path only; -2 when
+ * the solver does not keep it (ofb_set_keep_lambda_prev). */
 int ofb_get_lambda_prev(const ofb_solver* solver, float* lambda_prev_out, void* stream);
+
+/* Whether the following steps write lambda_prev_iter for ofb_get_lambda_prev (default 1).
+ * The reference keeps it in its engine data and allocates the adjoint's arrays only for
+ * differentiable_simulation (core/base_engine.py:118-146); a caller that never runs the
+ * adjoint sets 0 and the fused kernel skips that W*N_c*4-byte write per step.  Plays no
+ * part in the forward solve. */
+int ofb_set_keep_lambda_prev(ofb_solver* solver, int32_t keep);
 
 /* Replaces the per-shape friction coefficients shape_mu [W*(S+G)] (OstrichModel
  * shape_material_mu, core/model.py) copied at ofb_create, e.g. between steps of a

@@ -309,6 +309,7 @@ int ofb_create(const ofb_model* model, const ofb_config* config, int32_t path, o
   std::memset(s, 0, sizeof(*s));
   s->cfg = *config;
   s->path = path;
+  s->keep_lambda_prev = 1;
   DeviceModel& m = s->m;
   m.W = W; m.B = B; m.J = J; m.D = D; m.S = S; m.G = G; m.SC = S + G;
   for (int i = 0; i < 3; ++i) m.gravity[i] = model->gravity[i];
@@ -587,6 +588,10 @@ int ofb_get_lambda(const ofb_solver* s, float* lambda_out, void* stream) {
 int ofb_get_lambda_prev(const ofb_solver* s, float* lambda_prev_out, void* stream) {
   if (!s || !lambda_prev_out) { set_error("null argument"); return -1; }
   if (s->path != OFB_PATH_FUSED) { set_error("lambda_prev_iter is recorded by the fused path only"); return -2; }
+  if (!s->keep_lambda_prev) {
+    set_error("lambda_prev_iter is not kept by this solver (ofb_set_keep_lambda_prev(s, 1) before the step)");
+    return -2;
+  }
   OFB_CHECK(hipMemcpyAsync(lambda_prev_out, s->lambda_prev, (size_t)s->m.W * s->L.Nc * 4, hipMemcpyDeviceToDevice,
                            (hipStream_t)stream));
   return 0;
@@ -640,6 +645,12 @@ int ofb_last_kernel_ms(const ofb_solver* s, float* ms) {
   }
   OFB_CHECK(hipEventSynchronize(s->ev_kernel1));
   OFB_CHECK(hipEventElapsedTime(ms, s->ev_kernel0, s->ev_kernel1));
+  return 0;
+}
+
+int ofb_set_keep_lambda_prev(ofb_solver* s, int32_t keep) {
+  if (!s) { set_error("null solver"); return -1; }
+  s->keep_lambda_prev = keep ? 1 : 0;
   return 0;
 }
 

@@ -313,7 +313,7 @@ int fused_step(ofb_solver* s, const ofb_state_in* in, const ofb_state_out* out, 
   P.topo = s->topo;
   P.lambda_seed = s->lambda_seed;
   P.lambda_out = s->lambda;
-  P.lambda_prev_out = s->lambda_prev;
+  P.lambda_prev_out = s->keep_lambda_prev ? s->lambda_prev : nullptr;
   P.lambda_seed2 = (s->cfg.warm_start && s->ws_net) ? s->lambda_seed2 : nullptr;
   P.ls_steps = s->ls_steps;
   P.ls_count = s->cfg.linesearch ? s->ls_count : 0;

@@ -89,7 +89,7 @@ struct FusedParams {
   const float* ls_steps;      // merit line-search candidates (ls_count 0: full Newton step)
   int ls_count;
   float* lambda_out;
-  float* lambda_prev_out;  // lambda_prev_iter as the step leaves it (input of the adjoint)
+  float* lambda_prev_out;  // lambda_prev_iter as the step leaves it (input of the adjoint); null: not kept
   int32_t *st_nr, *st_pcr, *st_best, *st_flags;
   float* st_merit;
   // per-Newton-iteration history [hist_cap][W] (null: off): ||h||^2 after the iteration
@@ -1239,7 +1239,7 @@ fused_step_kernel(FusedParams P) {
     if (i < n_r) {
       size_t o = (size_t)w * L.Nc + original_row(L, i, n_c);
       P.lambda_out[o] = lam[k];
-      P.lambda_prev_out[o] = lamprev[i];
+      if (P.lambda_prev_out) P.lambda_prev_out[o] = lamprev[i];
     }
   }
   // contact slots beyond the world's contact count hold lambda = 0
@@ -1248,9 +1248,11 @@ fused_step_kernel(FusedParams P) {
     P.lambda_out[o + L.off_n + c] = 0.f;
     P.lambda_out[o + L.off_f + 2 * c] = 0.f;
     P.lambda_out[o + L.off_f + 2 * c + 1] = 0.f;
-    P.lambda_prev_out[o + L.off_n + c] = 0.f;
-    P.lambda_prev_out[o + L.off_f + 2 * c] = 0.f;
-    P.lambda_prev_out[o + L.off_f + 2 * c + 1] = 0.f;
+    if (P.lambda_prev_out) {
+      P.lambda_prev_out[o + L.off_n + c] = 0.f;
+      P.lambda_prev_out[o + L.off_f + 2 * c] = 0.f;
+      P.lambda_prev_out[o + L.off_f + 2 * c + 1] = 0.f;
+    }
   }
   if (P.hist_merit) {  // NaN / -1: iterations the world did not run (or past kHistIters)
     wsync();

@@ -115,6 +115,8 @@ struct ofb_solver {
   // optional per-class timing of the fused path (ofb_set_class_timing): events around
   // each class launch on the stream it runs on; bit c of cls_timed: class c was timed
   int cls_timing;
+  // the fused path writes lambda_prev_iter (the adjoint's input) only when set
+  int keep_lambda_prev;
   int cls_timed;
   hipEvent_t ev_cls0[16], ev_cls1[16];
   int cls_events_ready;

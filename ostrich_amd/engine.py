@@ -83,6 +83,10 @@ class OstrichEngine:
         torch.cuda.synchronize(self.device)
         _lib.check(self.lib.ofb_create(C.byref(abi_model), C.byref(cfg), config.path_id, C.byref(h)), "ofb_create")
         self._h = h
+        # lambda_prev_iter, the adjoint's input, is written out only for a differentiable
+        # engine (the reference allocates its adjoint arrays only then, base_engine.py:146)
+        _lib.check(self.lib.ofb_set_keep_lambda_prev(h, 1 if differentiable_simulation else 0),
+                   "ofb_set_keep_lambda_prev")
         del dev
         nc, oc, on, of = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
         _lib.check(self.lib.ofb_dims(h, C.byref(nc), C.byref(oc), C.byref(on), C.byref(of)), "ofb_dims")
@@ -360,7 +364,11 @@ class OstrichEngine:
     def constraint_impulses_prev(self):
         """lambda_prev_iter [W, N_c] as the last step left it (the snapshot of its last
         Newton iteration, base_engine.py:441); with `constraint_impulses()` it is what the
-        reference's TrajectoryBuffer keeps per step for the adjoint."""
+        reference's TrajectoryBuffer keeps per step for the adjoint.  Kept only by an
+        engine built with differentiable_simulation=True."""
+        if not self.differentiable_simulation:
+            raise RuntimeError("lambda_prev_iter is kept only by an engine built with differentiable_simulation=True "
+                               "(the adjoint's input; a forward-only engine skips writing it)")
         lam = torch.zeros(self.num_worlds, self.num_constraints, dtype=torch.float32, device=self.device)
         _lib.check(self.lib.ofb_get_lambda_prev(self._h, _ptr(lam), self._stream()), "ofb_get_lambda_prev")
         return lam

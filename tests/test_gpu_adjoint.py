@@ -26,7 +26,7 @@ KEYS = ("body_q", "body_qd", "body_f", "joint_target_pos", "joint_target_vel", "
 
 def _engine(model, cfg, device):
     from ostrich_amd import OstrichEngine
-    return OstrichEngine(model, config=cfg, device=device)
+    return OstrichEngine(model, config=cfg, device=device, differentiable_simulation=True)
 
 
 def _advance(oracle_mod, model, cfg, state, control, col, steps):
@@ -151,7 +151,7 @@ def test_adjoint_does_not_depend_on_the_wave_split(gpu_device):
     big_m, big_s, big_c = scenes.skid_steer(2048, seed=1)
     small_m, _, small_c = scenes.skid_steer(64, seed=1)
     B = big_m.bodies_per_world
-    big = OstrichEngine(big_m, config=cfg, device=gpu_device)
+    big = OstrichEngine(big_m, config=cfg, device=gpu_device, differentiable_simulation=True)
     s_in = State.from_numpy(big_s, gpu_device)
     s_out = s_in.clone()
     ctl = Control.from_numpy(big_c, gpu_device)
@@ -164,7 +164,7 @@ def test_adjoint_does_not_depend_on_the_wave_split(gpu_device):
     g_big = big.step_backward(s_in, s_out, ctl, con, DT, lam, lam_prev, gq, gu, adjoint_multipliers=True)
 
     n = 64 * B
-    small = OstrichEngine(small_m, config=cfg, device=gpu_device)
+    small = OstrichEngine(small_m, config=cfg, device=gpu_device, differentiable_simulation=True)
     si = State(s_in.body_q[:n].clone(), s_in.body_qd[:n].clone(), s_in.body_f[:n].clone())
     so = State(s_out.body_q[:n].clone(), s_out.body_qd[:n].clone(), s_out.body_f[:n].clone())
     D = small_m.dofs_per_world
@@ -177,3 +177,28 @@ def test_adjoint_does_not_depend_on_the_wave_split(gpu_device):
     for k, v in g_small.items():
         a = g_big[k].reshape(-1)[:v.numel()].reshape(v.shape)
         assert torch.equal(a, v), k
+
+
+def test_forward_only_engine_skips_lambda_prev(gpu_device):
+    """An engine built without differentiable_simulation does not write lambda_prev_iter
+    (and says so when asked for it); its forward results are bit-identical to a
+    differentiable engine's."""
+    from ostrich_amd import OstrichEngine
+    model, state, control = scenes.skid_steer(64, seed=1)
+    cfg = config(K=128)
+    s_in = State.from_numpy(state, gpu_device)
+    ctl = Control.from_numpy(control, gpu_device)
+    con = collide.HipCollider(model, device=gpu_device, max_contacts_per_world=128).collide(s_in.body_q)
+    outs = []
+    for diff in (False, True):
+        eng = OstrichEngine(model, config=cfg, device=gpu_device, differentiable_simulation=diff)
+        s_out = s_in.clone()
+        eng.step(s_in, s_out, ctl, con, DT)
+        outs.append((s_out.body_q.cpu(), s_out.body_qd.cpu(), eng.constraint_impulses().cpu()))
+        if not diff:
+            with pytest.raises(RuntimeError, match="differentiable_simulation"):
+                eng.constraint_impulses_prev()
+        else:
+            assert torch.isfinite(eng.constraint_impulses_prev()).all()
+    for a, b in zip(*outs):
+        assert torch.equal(a, b)

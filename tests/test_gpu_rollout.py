@@ -26,7 +26,7 @@ def _setup(gpu_device, worlds=16):
     from ostrich_amd import OstrichEngine
     model, state, control = scenes.skid_steer(worlds, seed=5, max_obstacles=4, mu_range=(0.8, 0.8))
     cfg = config(K=32)
-    eng = OstrichEngine(model, config=cfg, device=gpu_device)
+    eng = OstrichEngine(model, config=cfg, device=gpu_device, differentiable_simulation=True)
     col = collide.PrimitiveCollider(model, device=gpu_device, gap=0.02, max_contacts_per_world=32)
     collide_fn = lambda q: Contacts.from_dict(col.collide(q), gpu_device)
     rng = np.random.default_rng(11)
@@ -129,7 +129,7 @@ def _setup_seeded(gpu_device, worlds=16):
     base = config(K=32)
     cfg = type(base)(nr=base.nr, linear=base.linear, contacts=base.contacts, execution=base.execution,
                      warm_start=WarmStartConfig(enabled=True, seed_iterate=True))
-    return model, cfg, OstrichEngine(model, config=cfg, device=gpu_device), collide_fn, controls, s0
+    return model, cfg, OstrichEngine(model, config=cfg, device=gpu_device, differentiable_simulation=True), collide_fn, controls, s0
 
 
 def test_seeded_rollout_is_a_function_of_its_inputs(gpu_device):

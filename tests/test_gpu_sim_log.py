@@ -24,7 +24,7 @@ DT = 0.05
 def _setup(device, W=64):
     from ostrich_amd import OstrichEngine
     model, state, control = scenes.skid_steer(W, seed=1)
-    eng = OstrichEngine(model, config=config(K=128), device=device)
+    eng = OstrichEngine(model, config=config(K=128), device=device, differentiable_simulation=True)
     col = collide.HipCollider(model, device=device, max_contacts_per_world=128)
     return model, eng, col, State.from_numpy(state, device), Control.from_numpy(control, device)
 

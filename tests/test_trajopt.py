@@ -62,7 +62,7 @@ def test_control_optimisation_lowers_the_loss(gpu_device):
     from tests.helpers import config
     W, N, dt = 16, 10, 0.05
     model, state, control = scenes.skid_steer(W, seed=3, max_obstacles=0, mu_range=(0.8, 0.8))
-    eng = OstrichEngine(model, config=config(K=32), device=gpu_device)
+    eng = OstrichEngine(model, config=config(K=32), device=gpu_device, differentiable_simulation=True)
     col = collide.PrimitiveCollider(model, device=gpu_device, gap=0.02, max_contacts_per_world=32)
     D, B = model.dofs_per_world, model.bodies_per_world
     tv = torch.zeros(N, W, D, device=gpu_device)
