@@ -295,7 +295,9 @@ def scene_leg(name, model_fn, worlds, K, steps, warmup, device, variants=("fused
 
 def adjoint_leg(device, worlds=1024, steps=50, checkpoint_every=10, seed=3):
     """Differentiable 50-step rollout of skid-steer robots on a flat plane (mu 0.8, wheel
-    velocity targets U(-3,3) per step), loss on the final chassis x, y and yaw.  Times the
+    velocity targets U(-3,3) per step), loss on the final chassis x, y and yaw, gradients
+    w.r.t. every step's targets and the friction coefficient (grad_mu_norm: the norm over
+    worlds of dL/dmu, all of a world's shapes moved together).  Times the
     forward rollout and the checkpointed backward (re-simulation of every window plus
     one adjoint per step) with HIP events on the solver stream."""
     from ostrich_amd import OstrichEngine
@@ -356,6 +358,7 @@ def adjoint_leg(device, worlds=1024, steps=50, checkpoint_every=10, seed=3):
     return dict(worlds=worlds, steps=steps, checkpoint_every=checkpoint_every, forward_ms=fwd, backward_ms=bwd,
                 backward_ms_per_step=bwd / steps, adjoint_solve_ms_per_step=float(np.mean(adj)),
                 loss=float(loss.item()), grad_target_vel_norm=float(g["joint_target_vel"].norm().item()),
+                grad_mu_norm=float(g["shape_mu"].reshape(worlds, -1).sum(1).norm().item()),
                 note="forward and backward include the collider and host work of every step; the backward "
                      "re-simulates each 10-step window from its checkpoint; adjoint_solve_ms_per_step is the "
                      "adjoint solve of one step (HIP events on its stream)")
@@ -461,6 +464,108 @@ def fp64_error_leg(model_fn, worlds, lead_steps, device, variant="fused:jacobi:p
                 nr_within_1_of_fp64=float(np.mean(np.abs(stats["nr_iters"] - o64["nr_iters"]) <= 1)),
                 nr_within_1_of_fp64_converged=float(np.mean(np.abs(stats["nr_iters"] - o64["nr_iters"])[conv] <= 1))
                 if conv.any() else None)
+
+
+def _yaw_loss(q, target, W, B):
+    """Loss of the differentiable rollout on the final chassis x, y and yaw, per world, and
+    its gradient w.r.t. body_q [W*B, 7] (numpy, float64)."""
+    qc = q.reshape(W, B, 7)[:, 0]
+    x, y, qx, qy, qz, qw = qc[:, 0], qc[:, 1], qc[:, 3], qc[:, 4], qc[:, 5], qc[:, 6]
+    a, b = 2 * (qw * qz + qx * qy), 1 - 2 * (qy ** 2 + qz ** 2)
+    yaw = np.arctan2(a, b)
+    L = (x - target[:, 0]) ** 2 + (y - target[:, 1]) ** 2 + (yaw - target[:, 2]) ** 2
+    g = np.zeros((W, B, 7))
+    g[:, 0, 0], g[:, 0, 1] = 2 * (x - target[:, 0]), 2 * (y - target[:, 1])
+    dyaw, den = 2 * (yaw - target[:, 2]), a * a + b * b
+    da, db = b / den * dyaw, -a / den * dyaw
+    g[:, 0, 3], g[:, 0, 4] = da * 2 * qy, da * 2 * qx - db * 4 * qy
+    g[:, 0, 5], g[:, 0, 6] = da * 2 * qw - db * 4 * qz, da * 2 * qz
+    return L, g.reshape(W * B, 7)
+
+
+def adjoint_fd_check(device, worlds=8, horizons=(1, 50), h=1e-4, seed=3):
+    """The differentiable rollout's gradients on `worlds` skid-steer worlds (flat plane,
+    mu 0.8, wheel targets U(-3,3) per step, loss on the final chassis x, y, yaw) against
+    central differences (step h) of the FP64 oracle rollout: dL/d(targets) along two
+    random directions and dL/dmu (every shape's mu moved together), per world.  Reported
+    for the GPU adjoint (FP32) and for the FP64 oracle adjoint along the FP64 trajectory,
+    whose deviation from the differences is the method's own (its Newton matrix keeps
+    contact Jacobians at the step-start poses and contact compliance on its diagonal, and
+    capped Newton loops make the rollout non-smooth)."""
+    import copy
+    from oracle import oracle
+    from ostrich_amd import OstrichEngine
+    from ostrich_amd.differentiable import CheckpointedRollout
+    model, state, control = scenes.skid_steer(worlds, seed=seed, max_obstacles=0, mu_range=(0.8, 0.8))
+    cfg = solver_config("fused")
+    W, D, B = worlds, model.dofs_per_world, model.bodies_per_world
+    N = max(horizons)
+    rng = np.random.default_rng(seed)
+    TV = np.stack([rng.uniform(-3, 3, W * D) for _ in range(N)]).astype(np.float32)
+    target = np.stack([rng.uniform(-1, 1, W), rng.uniform(-1, 1, W), rng.uniform(-np.pi / 4, np.pi / 4, W)], 1)
+    hcol = collide.PrimitiveCollider(model, gap=0.02, max_contacts_per_world=32)
+    cfg_o = solver_config("fused", K=32)
+    tp = control["joint_target_pos"]
+    def contacts(q):
+        c = hcol.collide(torch.as_tensor(np.asarray(q), dtype=torch.float32))
+        return {k: (v.numpy() if hasattr(v, "numpy") else v) for k, v in c.items()}
+    def rollout64(tv, n, m=model, rec=False):
+        st, recs = {k: np.asarray(v, np.float64) for k, v in state.items()}, []
+        for t in range(n):
+            ctl = dict(joint_target_pos=tp, joint_target_vel=tv[t])
+            c = contacts(st["body_q"])
+            o = oracle.step(m, cfg_o, st, ctl, c, DT, precision=64)
+            if rec:
+                recs.append((st, o, c, ctl))
+            st = dict(st, body_q=o["body_q"], body_qd=o["body_qd"])
+        return st, recs
+    eng = OstrichEngine(model, config=solver_config("fused", K=32), device=device, differentiable_simulation=True)
+    gcol = collide.HipCollider(model, device=device, gap=0.02, max_contacts_per_world=32)
+    dirs = [np.random.default_rng(100 + k).normal(size=TV.shape) for k in range(2)]
+    out = dict(worlds=W, h=h, directions="two random directions in the wheel targets, and mu")
+    def rel(a, f):
+        r = np.abs(a - f) / np.maximum(np.abs(f), 1e-12)
+        return {"median": float(np.median(r)), "max": float(r.max())}
+    for n in horizons:
+        # GPU gradient (FP32 adjoint through the checkpointed rollout)
+        controls = [Control(torch.as_tensor(tp, device=device), torch.as_tensor(TV[t], device=device))
+                    for t in range(n)]
+        ro = CheckpointedRollout(eng, lambda q: gcol.collide(q).clone(), DT, checkpoint_every=10)
+        fin = ro.forward(State.from_numpy(state, device), controls)
+        _, gq = _yaw_loss(fin.body_q.cpu().numpy().astype(np.float64), target, W, B)
+        g = ro.backward(torch.as_tensor(gq, dtype=torch.float32, device=device), torch.zeros_like(fin.body_qd))
+        g_tv = g["joint_target_vel"].cpu().numpy().reshape(n, W * D).astype(np.float64)
+        g_mu = g["shape_mu"].cpu().numpy().reshape(W, -1).sum(1).astype(np.float64)
+        # FP64 oracle adjoint along the FP64 trajectory
+        st, recs = rollout64(TV, n, rec=True)
+        L0, gq64 = _yaw_loss(st["body_q"], target, W, B)
+        gu64, o_tv, o_mu = np.zeros((W * B, 6)), np.zeros((n, W * D)), np.zeros(W)
+        for t in reversed(range(n)):
+            s_in, o, c, ctl = recs[t]
+            a = oracle.adjoint(model, cfg_o, s_in, dict(body_q=o["body_q"], body_qd=o["body_qd"]), ctl, c, DT,
+                               o["lam"], o["lam_prev"], gq64, gu64, precision=64, fd_eps=1e-6)
+            o_tv[t] = a["joint_target_vel"].reshape(-1)
+            o_mu += a["shape_mu"].sum(1)
+            gq64, gu64 = a["body_q"], a["body_qd"]
+        res = dict(steps=n, converged_fraction_fp64=float(np.mean([r[1]["merit"] <= cfg_o.nr.atol ** 2
+                                                                     for r in recs])))
+        for k, v in enumerate(dirs):
+            vt = v[:n]
+            tp_, tm_ = (TV[:n] + h * vt).astype(np.float32), (TV[:n] - h * vt).astype(np.float32)
+            fd = (_yaw_loss(rollout64(tp_, n)[0]["body_q"], target, W, B)[0]
+                  - _yaw_loss(rollout64(tm_, n)[0]["body_q"], target, W, B)[0])
+            step = (tp_.astype(np.float64) - tm_.astype(np.float64)).reshape(n, W, D)
+            res[f"targets_dir{k}"] = dict(gpu_vs_fd=rel((g_tv.reshape(n, W, D) * step).sum((0, 2)), fd),
+                                          fp64_adjoint_vs_fd=rel((o_tv.reshape(n, W, D) * step).sum((0, 2)), fd))
+        mp, mm = copy.deepcopy(model), copy.deepcopy(model)
+        mp.arrays["shape_mu"] = (model.arrays["shape_mu"] + h).astype(np.float32)
+        mm.arrays["shape_mu"] = (model.arrays["shape_mu"] - h).astype(np.float32)
+        dmu = float(mp.arrays["shape_mu"].astype(np.float64)[0] - mm.arrays["shape_mu"].astype(np.float64)[0])
+        fd_mu = (_yaw_loss(rollout64(TV, n, m=mp)[0]["body_q"], target, W, B)[0]
+                 - _yaw_loss(rollout64(TV, n, m=mm)[0]["body_q"], target, W, B)[0]) / dmu
+        res["mu"] = dict(gpu=g_mu.tolist(), fp64_adjoint=o_mu.tolist(), fd=fd_mu.tolist())
+        out[f"horizon_{n}"] = res
+    return out
 
 
 METRIC = "world-steps/s at FB residual\u22641e-5 (plain vs fused+warm-start); adjoint ms"
@@ -589,9 +694,6 @@ def main():
     out["full_step"] = full_step_leg(model, state, control, device, args.steps, args.warmup,
                                      launch=r.get("launch", re_.get("launch")))
     print(f"[bench] full_step: {out['full_step']}", file=sys.stderr, flush=True)
-    print("[bench] fp64 error", file=sys.stderr, flush=True)
-    out["fp64_error"] = fp64_error_leg(lambda n: scenes.skid_steer(n, seed=1), args.fp64_worlds, 40, device)
-    print(f"[bench] fp64_error: {out['fp64_error']}", file=sys.stderr, flush=True)
     if not args.no_scenes:
         # tracked flipper robots on a Gaussian-bump heightfield (1024 worlds, anisotropic
         # track friction, position-controlled flippers) and 3-cube box stacks (16 worlds)
@@ -621,6 +723,15 @@ def main():
                                                      precond=mp[1], seed=seed, threads=min(16, host["nproc"] or 1))
         out["cpu_baseline_all_cores"]["host"] = host
         print(f"[bench] cpu_baseline_all_cores: {out['cpu_baseline_all_cores']}", file=sys.stderr, flush=True)
+        # the oracle as the checker of the GPU results: FP64 error of one step, and the
+        # differentiable rollout's gradients against FP64 central differences
+        print("[bench] fp64 error", file=sys.stderr, flush=True)
+        out["fp64_error"] = fp64_error_leg(lambda n: scenes.skid_steer(n, seed=1), args.fp64_worlds, 40, device)
+        print(f"[bench] fp64_error: {out['fp64_error']}", file=sys.stderr, flush=True)
+        if not args.no_adjoint:
+            print("[bench] adjoint fd check", file=sys.stderr, flush=True)
+            out.setdefault("adjoint", {})["fd_check"] = adjoint_fd_check(device)
+            print(f"[bench] adjoint fd_check: {out['adjoint']['fd_check']}", file=sys.stderr, flush=True)
     if args.copy_calibration:
         src = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device=device)
         dst = torch.empty_like(src)

@@ -260,18 +260,24 @@ class CheckpointedRollout:
 
 
 class OstrichRollout(torch.autograd.Function):
-    """(body_q0, body_qd0, target_pos [N, W*D], target_vel [N, W*D]) -> (body_q_N, body_qd_N)
-    through a CheckpointedRollout (body_f is held at zero)."""
+    """(body_q0, body_qd0, target_pos [N, W*D], target_vel [N, W*D], shape_mu [W*(S+G)] or
+    None) -> (body_q_N, body_qd_N) through a CheckpointedRollout (body_f is held at zero).
+    `shape_mu` sets the friction of every step (None keeps the model's) and receives
+    dL/dmu summed over the steps (the rollout of simulation/differentiable_simulator.py:
+    234-271 with the friction coefficients as a parameter)."""
 
     @staticmethod
-    def forward(ctx, engine, collide, dt, checkpoint_every, body_q0, body_qd0, target_pos, target_vel):
+    def forward(ctx, engine, collide, dt, checkpoint_every, body_q0, body_qd0, target_pos, target_vel,
+                shape_mu=None):
         N = target_vel.shape[0]
         s0 = State(body_q0.detach().contiguous(), body_qd0.detach().contiguous(), torch.zeros_like(body_qd0))
         controls = [Control(target_pos[t].detach().contiguous(), target_vel[t].detach().contiguous())
                     for t in range(N)]
+        mu = shape_mu.detach().clone() if shape_mu is not None else engine.model_shape_mu()
+        engine.set_shape_mu(mu)
         ro = CheckpointedRollout(engine, collide, dt, checkpoint_every)
         fin = ro.forward(s0, controls)
-        ctx.ro = ro
+        ctx.ro, ctx.mu, ctx.has_mu = ro, mu, shape_mu is not None
         return fin.body_q.clone(), fin.body_qd.clone()
 
     @staticmethod
@@ -281,10 +287,12 @@ class OstrichRollout(torch.autograd.Function):
             g_q = torch.zeros_like(ro.final.body_q)
         if g_qd is None:
             g_qd = torch.zeros_like(ro.final.body_qd)
+        ro.engine.set_shape_mu(ctx.mu)  # the friction of this rollout, whatever ran since
         g = ro.backward(g_q, g_qd)
         N = len(ro.controls)
+        gmu = g["shape_mu"].reshape(-1) if ctx.has_mu else None
         return (None, None, None, None, g["body_q"], g["body_qd"], g["joint_target_pos"].reshape(N, -1),
-                g["joint_target_vel"].reshape(N, -1))
+                g["joint_target_vel"].reshape(N, -1), gmu)
 
 
 class TrajectoryAdam:

@@ -9,6 +9,10 @@ skid_steer  - a 10 kg chassis box 0.6x0.4x0.2 m with three 1 kg cylinder wheels
               free base + three revolute wheels, axis y), wheel targets U(-5,5) rad/s,
               0-24 static boxes with edge U(0.05,0.3) m placed uniformly in a 4x4 m area,
               mu U(0.4,1.0).
+taros4      - the reference's four-wheeled Taros-4 gradient-test robot
+              (examples/taros_4/common.py:15-50): 1200 kg chassis box 2.74x1.5x1.0 m with
+              its centre of mass 0.2 m up, four 50 kg cylinder wheels (r 0.45 m, width
+              0.3 m) at x +-1.0, y +-0.9, z -0.3 on revolute joints about y.
 tracked_flipper - a 40 kg body box 0.9x0.5x0.3 m with four 3 kg flippers on
               position-controlled revolute joints (targets U(-0.8,0.8) rad), six tracks of ten
               r = 0.05 m sphere contact points with anisotropic friction (mu 1.0 along the
@@ -168,6 +172,55 @@ def skid_steer(num_worlds=4096, seed=1, max_obstacles=SKID_MAX_OBSTACLES, ke=150
                  body_f=np.zeros((num_worlds * B, 6), np.float32))
     control = dict(joint_target_pos=np.zeros(num_worlds * model.dofs_per_world, np.float32),
                    joint_target_vel=np.concatenate(targets).astype(np.float32))
+    return model, state, control
+
+
+# the reference's Taros-4 robot (examples/taros_4/common.py:15-50, 183-290)
+TAROS_CHASSIS_HALF = (1.37, 0.75, 0.5)
+TAROS_CHASSIS_MASS = 1200.0
+TAROS_CHASSIS_I = (600.0, 1000.0, 1000.0)
+TAROS_CHASSIS_COM = (0.0, 0.0, 0.2)
+TAROS_WHEEL_R = 0.45
+TAROS_WHEEL_HH = 0.15
+TAROS_WHEEL_MASS = 50.0
+TAROS_WHEEL_I = (3.0, 3.0, 5.0)
+TAROS_WHEEL_POS = ((1.0, 0.9, -0.3), (1.0, -0.9, -0.3), (-1.0, 0.9, -0.3), (-1.0, -0.9, -0.3))
+
+
+def taros4(num_worlds=1, friction=0.8, control_mode="velocity", k_p=1000.0, k_d=0.0, z=0.75):
+    """Taros-4 worlds: free chassis plus four wheels (front-left, front-right, rear-left,
+    rear-right; dofs 6-9) on revolute joints about y, velocity or position motors with
+    gains k_p / k_d, wheel and ground friction `friction`.  The chassis origin starts at
+    height `z`: 0.75 sets the wheels on the ground (the reference's tests start it at 0.8,
+    5 cm up, inside its 5 cm contact gap, tests/differentiable_simulator/
+    test_wheeled_robot.py:29-41)."""
+    if control_mode not in ("velocity", "position"):
+        raise ValueError(f"control_mode must be 'velocity' or 'position', got {control_mode!r}")
+    mode = DOF_TARGET_VELOCITY if control_mode == "velocity" else DOF_TARGET_POSITION
+    worlds, qs = [], []
+    for _ in range(num_worlds):
+        wb = WorldBuilder()
+        chassis = wb.add_body(TAROS_CHASSIS_MASS, np.diag(TAROS_CHASSIS_I), com=TAROS_CHASSIS_COM)
+        wb.add_shape(Shape(GEO_BOX, chassis, TAROS_CHASSIS_COM + (0.0, 0.0, 0.0, 1.0), TAROS_CHASSIS_HALF,
+                           mu=friction))
+        wb.add_joint_free(chassis)
+        poses = [(0.0, 0.0, z, 0.0, 0.0, 0.0, 1.0)]
+        for pos in TAROS_WHEEL_POS:
+            wheel = wb.add_body(TAROS_WHEEL_MASS, np.diag(TAROS_WHEEL_I))
+            wb.add_shape(Shape(GEO_CYLINDER, wheel, IDENTITY7, (TAROS_WHEEL_R, TAROS_WHEEL_HH, 0.0), mu=friction))
+            wb.add_joint_revolute(chassis, wheel, X_p=tuple(pos) + (0.0, 0.0, 0.0, 1.0), axis=(0.0, 1.0, 0.0),
+                                  mode=mode, ke=k_p, kd=k_d)
+            poses.append((pos[0], pos[1], z + pos[2], 0.0, 0.0, 0.0, 1.0))
+        wb.add_shape(Shape(GEO_PLANE, -1, IDENTITY7, (0.0, 0.0, 0.0), mu=friction))
+        worlds.append(wb)
+        qs.append(poses)
+    model = Model.from_worlds(worlds, [])
+    B = model.bodies_per_world
+    body_q = np.array(qs, np.float32).reshape(num_worlds * B, 7)
+    state = dict(body_q=body_q, body_qd=np.zeros((num_worlds * B, 6), np.float32),
+                 body_f=np.zeros((num_worlds * B, 6), np.float32))
+    control = dict(joint_target_pos=np.zeros(num_worlds * model.dofs_per_world, np.float32),
+                   joint_target_vel=np.zeros(num_worlds * model.dofs_per_world, np.float32))
     return model, state, control
 
 

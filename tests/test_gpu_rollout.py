@@ -119,6 +119,22 @@ def test_autograd_functions_match_direct_calls(gpu_device):
     ro.forward(s0, controls)
     gr = ro.backward(gq, gu)
     assert torch.equal(tvs.grad, gr["joint_target_vel"].reshape(N, -1))
+    # the rollout with the friction coefficients as a parameter (0.1 instead of the
+    # model's 0.8, so that wheels slip and dL/dmu is not zero): the same gradients as a
+    # CheckpointedRollout under that friction, and dL/dmu as its per-step sum
+    mu_r = torch.full((model.num_worlds * SC,), 0.1, device=gpu_device).requires_grad_(True)
+    tvs2 = tvs.detach().clone().requires_grad_(True)
+    qN2, qdN2 = OstrichRollout.apply(eng, collide_fn, DT, 2, s0.body_q, s0.body_qd, tp, tvs2, mu_r)
+    ((qN2 * gq).sum() + (qdN2 * gu).sum()).backward()
+    eng.set_shape_mu(mu_r.detach())
+    ro2 = CheckpointedRollout(eng, collide_fn, DT, checkpoint_every=2)
+    fin2 = ro2.forward(s0, controls)
+    gr2 = ro2.backward(gq, gu)
+    eng.set_shape_mu(eng.model_shape_mu())
+    assert torch.equal(qN2, fin2.body_q) and not torch.equal(qN2, qN)
+    assert torch.equal(tvs2.grad, gr2["joint_target_vel"].reshape(N, -1))
+    assert torch.equal(mu_r.grad, gr2["shape_mu"].reshape(-1))
+    assert mu_r.grad.abs().max() > 0
 
 
 def _setup_seeded(gpu_device, worlds=16):
