@@ -658,9 +658,11 @@ __device__ __forceinline__ FrictionStatic friction_static(const PT& P, const flo
   return s;
 }
 
-// LDS record of a contact: 64 floats (16 quads), the normal row's 28 then the friction
-// rows' 36
-constexpr int CS_WORDS = 64;
+// LDS record of a contact: 16 quads, the normal row's 28 floats then the friction rows'
+// 36, and one quad of padding: lanes read the records of different contacts with 128-bit
+// loads, and a 68-float stride puts eight consecutive lanes' quads on distinct banks (a
+// 64-float stride put them all on the same four, an eight-way conflict on every load)
+constexpr int CS_WORDS = 68;
 constexpr int CS_FRICTION = 28;
 __device__ __forceinline__ void st_sv2(float4* q, const sv& a, const sv& b) {
   q[0] = make_float4(a.a.x, a.a.y, a.a.z, a.b.x);
