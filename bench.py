@@ -280,12 +280,14 @@ def model_rows(model):
     return nj + nctrl
 
 
-def scene_leg(name, model_fn, worlds, K, steps, warmup, device, variants=("fused:jacobi:prev",)):
+def scene_leg(name, model_fn, worlds, K, steps, warmup, device, variants=("fused:jacobi:prev",), gap=0.02,
+              hf_contacts="point"):
     """Another workload through the same solver paths (a parity-test configuration, not
     the headline): world-steps/s, Newton counts and contact counts per variant."""
     model, state, control = model_fn(worlds)
-    collider = collide.HipCollider(model, device=device, gap=0.02, max_contacts_per_world=K)
-    out = dict(worlds=worlds, contact_slots=K, steps=steps, warmup=warmup)
+    collider = collide.HipCollider(model, device=device, gap=gap, max_contacts_per_world=K, hf_contacts=hf_contacts)
+    out = dict(worlds=worlds, contact_slots=K, gap=gap, heightfield_contacts=hf_contacts, steps=steps,
+               warmup=warmup)
     for v in variants:
         print(f"[bench] {name} path={v} worlds={worlds}", file=sys.stderr, flush=True)
         out[v] = run_path(v, model, state, control, steps, warmup, device, collider, K=K)
@@ -696,11 +698,13 @@ def main():
     print(f"[bench] full_step: {out['full_step']}", file=sys.stderr, flush=True)
     if not args.no_scenes:
         # tracked flipper robots on a Gaussian-bump heightfield (1024 worlds, anisotropic
-        # track friction, position-controlled flippers) and 3-cube box stacks (16 worlds)
+        # track friction, position-controlled flippers, four grid-cell contacts per track
+        # sphere within 0.05 m) and 3-cube box stacks (16 worlds)
         sc_steps = min(args.steps, 40)
-        out["tracked_flipper"] = scene_leg("tracked_flipper", lambda n: scenes.tracked_flipper(n, seed=2), 1024, 64,
+        out["tracked_flipper"] = scene_leg("tracked_flipper", lambda n: scenes.tracked_flipper(n, seed=2), 1024, 256,
                                            sc_steps, max(args.warmup, 6), device,
-                                           ("fused:jacobi:prev", "fused:jacobi:cold"))
+                                           ("fused:jacobi:prev", "fused:jacobi:cold"), gap=0.05,
+                                           hf_contacts="cells")
         out["box_stack"] = scene_leg("box_stack", lambda n: scenes.box_stack(n, 3, seed=0), 16, 16, sc_steps,
                                      args.warmup, device, ("fused:jacobi:prev", "fused:jacobi:cold",
                                                            "plain:jacobi:cold"))

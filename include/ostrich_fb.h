@@ -378,7 +378,7 @@ int ofb_last_kernel_ms(const ofb_solver* solver, float* ms);
  * timing on, every eager step records HIP events around each row-class launch on the
  * stream it runs on (the paired class on the step's stream, the others on the solver's
  * side stream, where they run one after another beside the paired class).
- * ofb_get_class_times fills, for each of the `count` classes (capacity >= 12), its row
+ * ofb_get_class_times fills, for each of the `count` classes (capacity >= 10), its row
  * capacity, the number of worlds the last step put in it and its duration in ms (-1 when
  * the class was not launched or the step was not timed); it blocks until the step has
  * finished.  -2 on the plain path. */
@@ -457,6 +457,10 @@ typedef struct ofb_collider_desc {
   float hf_origin[2];
   float hf_spacing[2];
   const float* hf_heights;      /* [hf_ny*hf_nx] row-major (y, x) */
+  /* contacts per sphere: 1 = the bilinear surface below its centre; 4 = the four grid
+   * cells around its nearest grid vertex, each at the closest point of the cell's nearer
+   * triangle (cells outside the grid give none) */
+  int32_t hf_points;
 } ofb_collider_desc;
 
 /* Output of ofb_collide: the same fields as ofb_contacts, writable (device pointers). */

@@ -147,6 +147,7 @@ class OfbColliderDesc(C.Structure):
         ("hf_col", C.c_int32), ("hf_nx", C.c_int32), ("hf_ny", C.c_int32),
         ("hf_origin", C.c_float * 2), ("hf_spacing", C.c_float * 2),
         ("hf_heights", C.c_void_p),
+        ("hf_points", C.c_int32),
     ]
 
 

@@ -59,9 +59,53 @@ def _cylinder_samples(r, hh, k):
     return torch.cat(pts, dim=-2)  # [..., 3k, 3], axis = local y
 
 
+HF_CONTACT_MODES = {"point": 1, "cells": 4}
+
+
+def _closest_on_triangle(p, a, b, c):
+    """Closest point of triangle (a, b, c) to p, broadcast over leading dimensions, and
+    whether it lies in the face's interior region (Ericson, Real-Time Collision Detection,
+    5.1.5: the Voronoi region of p among the triangle's vertices, edges and face, tested
+    in that order)."""
+    dot = lambda x, y: (x * y).sum(-1, keepdim=True)
+    ab, ac, ap = b - a, c - a, p - a
+    d1, d2 = dot(ab, ap), dot(ac, ap)
+    bp = p - b
+    d3, d4 = dot(ab, bp), dot(ac, bp)
+    cp = p - c
+    d5, d6 = dot(ab, cp), dot(ac, cp)
+    vc = d1 * d4 - d3 * d2
+    vb = d5 * d2 - d1 * d6
+    va = d3 * d6 - d5 * d4
+    r_a = (d1 <= 0) & (d2 <= 0)
+    r_b = (d3 >= 0) & (d4 <= d3)
+    r_ab = (vc <= 0) & (d1 >= 0) & (d3 <= 0)
+    r_c = (d6 >= 0) & (d5 <= d6)
+    r_ac = (vb <= 0) & (d2 >= 0) & (d6 <= 0)
+    r_bc = (va <= 0) & (d4 - d3 >= 0) & (d5 - d6 >= 0)
+    denom = 1.0 / (va + vb + vc)
+    res = a + ab * (vb * denom) + ac * (vc * denom)                           # face
+    res = torch.where(r_bc, b + (c - b) * ((d4 - d3) / ((d4 - d3) + (d5 - d6))), res)
+    res = torch.where(r_ac, a + ac * (d2 / (d2 - d6)), res)
+    res = torch.where(r_c, c, res)
+    res = torch.where(r_ab, a + ab * (d1 / (d1 - d3)), res)
+    res = torch.where(r_b, b, res)
+    res = torch.where(r_a, a, res)
+    face = ~(r_a | r_b | r_ab | r_c | r_ac | r_bc)
+    return res, face[..., 0]
+
+
 class PrimitiveCollider:
+    """hf_contacts: "point" (one contact per body sphere: the bilinear surface below its
+    centre) or "cells" (up to four per sphere: the four grid cells around the grid vertex
+    nearest its centre, each cell two triangles, the contact at the closest point of the
+    nearer triangle, as a triangle-mesh narrow phase generates them)."""
+
     def __init__(self, model, device="cpu", gap=0.02, cylinder_samples=16, lateral_slack=0.06,
-                 max_contacts_per_world=128):
+                 max_contacts_per_world=128, hf_contacts="point"):
+        if hf_contacts not in HF_CONTACT_MODES:
+            raise ValueError(f"hf_contacts must be one of {tuple(HF_CONTACT_MODES)}, got {hf_contacts!r}")
+        self.hf_contacts = hf_contacts
         self.model = model
         self.device = torch.device(device)
         self.gap = gap
@@ -161,6 +205,60 @@ class PrimitiveCollider:
         dzdy = ((h01 - h00) * (1 - fu) + (h11 - h10) * fu) / self.hf_spacing[1]
         return z, dzdx, dzdy
 
+    def _heightfield_cell_contacts(self, c, r):
+        """Contacts of sphere centres c [W,Ns,3] (radii r [W,Ns]) with the four grid cells
+        around each centre's nearest grid vertex: [W,Ns,4] point on the terrain, unit normal
+        from the terrain to the sphere, signed distance, cell-in-grid mask."""
+        h = self.hf_h
+        ny, nx = h.shape
+        sx, sy = self.hf_spacing
+        ox, oy = self.hf_origin
+        u = (c[..., 0] - ox) / sx
+        v = (c[..., 1] - oy) / sy
+        I = torch.floor(u + 0.5).long()
+        J = torch.floor(v + 0.5).long()
+        di = torch.tensor([0, 1, 0, 1], device=self.device)
+        dj = torch.tensor([0, 0, 1, 1], device=self.device)
+        ci = I[..., None] - 1 + di                                                 # [W,Ns,4]
+        cj = J[..., None] - 1 + dj
+        inside = (ci >= 0) & (ci <= nx - 2) & (cj >= 0) & (cj <= ny - 2)
+        i0, j0 = ci.clamp(0, nx - 2), cj.clamp(0, ny - 2)
+        def vert(i, j):
+            return torch.stack([ox + i.to(torch.float32) * sx, oy + j.to(torch.float32) * sy, h[j, i]], dim=-1)
+        P00, P10, P01, P11 = vert(i0, j0), vert(i0 + 1, j0), vert(i0, j0 + 1), vert(i0 + 1, j0 + 1)
+        cc = c[..., None, :].expand_as(P00)
+        rr = r[..., None]
+
+        def tri(A, B_, C_):
+            """contact with one triangle: foot, normal, distance, usable"""
+            nt = torch.cross(B_ - A, C_ - A, dim=-1)
+            nt = nt / nt.norm(dim=-1, keepdim=True)
+            q, face = _closest_on_triangle(cc, A, B_, C_)
+            e = cc - q
+            side = (e * nt).sum(-1)
+            d = e.norm(dim=-1)
+            # inside the face region the contact is the face's (signed: the centre may be
+            # below it); beside it, the nearest edge or vertex, for a centre in front of
+            # the face only (behind it, a neighbouring triangle holds the contact), along
+            # the face normal when the centre is within r/4 of that edge or vertex (the
+            # direction to it is then dominated by rounding)
+            edge_ok = (side > 0) & (d > 1e-6)
+            flat = face | (d <= 0.25 * rr)
+            nrm = torch.where(flat[..., None], nt, e / d.clamp_min(1e-6)[..., None])
+            dist = torch.where(flat, side, d) - rr
+            return q, nrm, dist, face | edge_ok, face
+
+        qa, na, da, ua, fa = tri(P00, P10, P11)
+        qb, nb, db, ub, fb = tri(P00, P11, P01)
+        # a face contact wins over an edge or vertex one; between two of a kind the nearer,
+        # near-ties going to the first triangle so that rounding does not pick between two
+        # distinct feet
+        use_a = ua & (~ub | (fa & ~fb) | ((fa == fb) & (da <= db + 1e-6)))
+        foot = torch.where(use_a[..., None], qa, qb)
+        nrm = torch.where(use_a[..., None], na, nb)
+        dist = torch.where(use_a, da, db)
+        return foot, nrm, dist, inside & (ua | ub)
+
     def _heightfield_contacts(self, body_q):
         W, B = self.model.num_worlds, self.model.bodies_per_world
         cols = torch.as_tensor(self.hf_spheres, device=self.device)
@@ -168,6 +266,19 @@ class PrimitiveCollider:
         bq = body_q.view(W, B, 7)[torch.arange(W, device=self.device)[:, None], bidx]  # [W,Ns,7]
         c = bq[..., :3] + _quat_rotate(bq[..., 3:], self.xform[:, cols, :3])
         r = self.size[:, cols, 0]                                                   # [W,Ns]
+        fa = torch.as_tensor(self.model.flat_shape_id(np.arange(W)[:, None], np.array(self.hf_spheres)[None, :]),
+                             device=self.device)
+        fb = torch.as_tensor(self.model.flat_shape_id(np.arange(W), np.full(W, self.hf_col)), device=self.device)
+        if self.hf_contacts == "cells":
+            p1w, nrm, dist, inside = self._heightfield_cell_contacts(c, r)        # [W,Ns,4]
+            cq = c[..., None, :].expand_as(p1w)
+            pw = cq - nrm * r[..., None, None]
+            p0 = _quat_rotate_inv(bq[..., None, 3:].expand(*p1w.shape[:-1], 4), pw - bq[..., None, :3])
+            Ns = len(self.hf_spheres)
+            flat = lambda t: t.reshape(W, Ns * 4, *t.shape[3:])
+            s0 = fa[..., None].expand(W, Ns, 4)
+            return (flat(p0), flat(p1w), flat(-nrm), flat(s0), fb[:, None].expand(W, Ns * 4),
+                    flat(inside & (dist < self.gap)))
         z, gx, gy = self.heightfield_surface(c[..., 0], c[..., 1])
         nrm = torch.stack([-gx, -gy, torch.ones_like(z)], dim=-1)
         nrm = nrm / nrm.norm(dim=-1, keepdim=True)
@@ -176,9 +287,6 @@ class PrimitiveCollider:
         pw = c - nrm * r[..., None]
         p1w = pw - nrm * dist[..., None]
         p0 = _quat_rotate_inv(bq[..., 3:], pw - bq[..., :3])
-        fa = torch.as_tensor(self.model.flat_shape_id(np.arange(W)[:, None], np.array(self.hf_spheres)[None, :]),
-                             device=self.device)
-        fb = torch.as_tensor(self.model.flat_shape_id(np.arange(W), np.full(W, self.hf_col)), device=self.device)
         return p0, p1w, -nrm, fa, fb[:, None].expand_as(fa), dist < self.gap
 
     def collide(self, body_q):
@@ -302,7 +410,7 @@ class HipCollider:
     solver step replays both.  Requires the HIP library and a GPU device."""
 
     def __init__(self, model, device="cuda:0", gap=0.02, cylinder_samples=16, lateral_slack=0.06,
-                 max_contacts_per_world=128):
+                 max_contacts_per_world=128, hf_contacts="point"):
         import ctypes as C
 
         from . import _abi, _lib
@@ -313,7 +421,8 @@ class HipCollider:
         if self.device.type != "cuda":
             raise ValueError("HipCollider runs on a GPU device")
         plan = PrimitiveCollider(model, device="cpu", gap=gap, cylinder_samples=cylinder_samples,
-                                 lateral_slack=lateral_slack, max_contacts_per_world=max_contacts_per_world)
+                                 lateral_slack=lateral_slack, max_contacts_per_world=max_contacts_per_world,
+                                 hf_contacts=hf_contacts)
         self.pairs = plan.pairs
         self.hf_spheres = plan.hf_spheres
         W, SC = model.shape_geo.shape
@@ -345,6 +454,7 @@ class HipCollider:
             d.hf_origin[0], d.hf_origin[1] = plan.hf_origin[0], plan.hf_origin[1]
             d.hf_spacing[0], d.hf_spacing[1] = plan.hf_spacing[0], plan.hf_spacing[1]
             d.hf_heights = h.ctypes.data
+            d.hf_points = HF_CONTACT_MODES[hf_contacts]
         self._handle = C.c_void_p()
         with torch.cuda.device(self.device):
             _lib.check(self._lib.ofb_collider_create(C.byref(d), C.byref(self._handle)), "ofb_collider_create")

@@ -21,7 +21,7 @@ int OFB_CAT(adjoint_class_op_bj, OFB_BJ, _set, OFB_RSET)(int R, int op, const Ad
 #if OFB_RSET == 0
     CASE(1) CASE(2) CASE(3) CASE(4) CASE(5)
 #else
-    CASE(6) CASE(7) CASE(8) CASE(10) CASE(12) CASE(16)
+    CASE(6) CASE(7) CASE(12) CASE(16)
 #endif
 #undef CASE
   }

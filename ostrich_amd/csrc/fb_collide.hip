@@ -6,7 +6,8 @@
 // The candidate set of a world is fixed by the model: for each shape pair (a, b) with a
 // on a body and b a plane or a box, the sample points of a (8 box corners; for a cylinder
 // the lowest point of its middle circle against a plane, 3 x k rim points against a box;
-// a sphere's centre), then one point per body sphere against a heightfield.  A candidate
+// a sphere's centre), then one point (or four grid cells) per body sphere against a
+// heightfield.  A candidate
 // is a contact when its signed distance is below the gap (and, against a box, its
 // lateral excess is within the slack on the non-normal axes).  Contacts are emitted in
 // (world, pair, point) order, capped at W * max_per_world, in two passes: one workgroup per
@@ -158,6 +159,94 @@ __device__ Cand eval_pair_point(const ColCtx& C, int a, int b, int s, const floa
   return o;
 }
 
+// closest point of triangle (a, b, c) to p and whether it lies in the face's interior
+// region (Ericson, Real-Time Collision Detection, 5.1.5: the Voronoi region of p among the
+// vertices, edges and face, tested in that order)
+__device__ v3 closest_on_triangle(v3 p, v3 a, v3 b, v3 c, bool& face) {
+  face = false;
+  const v3 ab = b - a, ac = c - a, ap = p - a;
+  const float d1 = dot(ab, ap), d2 = dot(ac, ap);
+  if (d1 <= 0.f && d2 <= 0.f) return a;
+  const v3 bp = p - b;
+  const float d3 = dot(ab, bp), d4 = dot(ac, bp);
+  if (d3 >= 0.f && d4 <= d3) return b;
+  const float vc = d1 * d4 - d3 * d2;
+  if (vc <= 0.f && d1 >= 0.f && d3 <= 0.f) return a + ab * (d1 / (d1 - d3));
+  const v3 cp = p - c;
+  const float d5 = dot(ab, cp), d6 = dot(ac, cp);
+  if (d6 >= 0.f && d5 <= d6) return c;
+  const float vb = d5 * d2 - d1 * d6;
+  if (vb <= 0.f && d2 >= 0.f && d6 <= 0.f) return a + ac * (d2 / (d2 - d6));
+  const float va = d3 * d6 - d5 * d4;
+  if (va <= 0.f && d4 - d3 >= 0.f && d5 - d6 >= 0.f) return b + (c - b) * ((d4 - d3) / ((d4 - d3) + (d5 - d6)));
+  face = true;
+  const float denom = 1.f / (va + vb + vc);
+  return a + ab * (vb * denom) + ac * (vc * denom);
+}
+
+// contact of a sphere centre c (radius r) with triangle (A, B, C) of the terrain: inside
+// the face region the face's (signed distance along its upward normal: the centre may be
+// below it); beside it, the nearest edge or vertex, for a centre in front of the face
+// only (behind it, a neighbouring triangle holds the contact), along the face normal
+// when the centre is within r/4 of that edge or vertex
+struct TriContact {
+  v3 foot, nrm;
+  float dist;
+  bool usable, face;
+};
+__device__ TriContact tri_contact(v3 c, float r, v3 A, v3 B, v3 C) {
+  TriContact t;
+  v3 nt = cross(B - A, C - A);
+  nt = nt * (1.f / sqrtf(dot(nt, nt)));
+  bool face;
+  t.foot = closest_on_triangle(c, A, B, C, face);
+  const v3 e = c - t.foot;
+  const float side = dot(e, nt), d = sqrtf(dot(e, e));
+  const bool edge_ok = side > 0.f && d > 1e-6f;
+  // within r/4 of an edge or vertex the direction to it is dominated by rounding: the
+  // contact then takes the face normal
+  const bool flat = face || d <= 0.25f * r;
+  t.nrm = flat ? nt : e * (1.f / fmaxf(d, 1e-6f));
+  t.dist = (flat ? side : d) - r;
+  t.usable = face || edge_ok;
+  t.face = face;
+  return t;
+}
+
+// sphere column a against grid cell q (0..3) of the 2 x 2 block around the grid vertex
+// nearest its centre, the cell split into two triangles along its (i, j)-(i+1, j+1)
+// diagonal: one of the two triangles' usable contacts
+__device__ Cand eval_heightfield_cell(const ColCtx& C, const float* __restrict__ hgt, int a, int q) {
+  Cand o;
+  const ofb_collider_desc& d = C.d;
+  o.s0 = C.flat_id[C.col(a)];
+  o.s1 = C.flat_id[C.col(d.hf_col)];
+  const tf Ba = C.body_pose(a);
+  const v3 c = C.shape_pose(a).p;
+  const float r = C.sz(a).x;
+  const int nx = d.hf_nx, ny = d.hf_ny;
+  const float sx = d.hf_spacing[0], sy = d.hf_spacing[1], ox = d.hf_origin[0], oy = d.hf_origin[1];
+  const int I = (int)floorf((c.x - ox) / sx + 0.5f), J = (int)floorf((c.y - oy) / sy + 0.5f);
+  const int ci = I - 1 + (q & 1), cj = J - 1 + (q >> 1);
+  const bool inside = ci >= 0 && ci <= nx - 2 && cj >= 0 && cj <= ny - 2;
+  const int i0 = min(max(ci, 0), nx - 2), j0 = min(max(cj, 0), ny - 2);
+  auto vert = [&](int i, int j) { return mk(ox + (float)i * sx, oy + (float)j * sy, hgt[j * nx + i]); };
+  const v3 P00 = vert(i0, j0), P10 = vert(i0 + 1, j0), P01 = vert(i0, j0 + 1), P11 = vert(i0 + 1, j0 + 1);
+  const TriContact ta = tri_contact(c, r, P00, P10, P11), tb = tri_contact(c, r, P00, P11, P01);
+  // a face contact wins over an edge or vertex one; between two of a kind the nearer,
+  // near-ties going to the first triangle so that rounding does not pick between two
+  // distinct feet
+  const bool use_a =
+      ta.usable && (!tb.usable || (ta.face && !tb.face) || (ta.face == tb.face && ta.dist <= tb.dist + 1e-6f));
+  const TriContact& t = use_a ? ta : tb;
+  const v3 pw = c - t.nrm * r;
+  o.valid = inside && (ta.usable || tb.usable) && t.dist < d.gap;
+  o.p0 = qrot_inv(Ba.q, pw - Ba.p);
+  o.p1 = t.foot;
+  o.n = t.nrm * -1.f;
+  return o;
+}
+
 // sphere column a against the heightfield (bilinear surface below the centre)
 __device__ Cand eval_heightfield(const ColCtx& C, const float* __restrict__ hgt, int a) {
   Cand o;
@@ -274,7 +363,10 @@ __device__ int world_setup(const ColArgs& A, const ColCtx& C, int* poff, float* 
 
 __device__ __forceinline__ Cand eval_candidate(const ColArgs& A, const ColCtx& C, const int* poff, int active,
                                                const float* ang_cs, const float* ang_sn, int c) {
-  if (c >= active) return eval_heightfield(C, A.hgt, A.hf_spheres[c - active]);
+  if (c >= active) {
+    if (A.d.hf_points == 4) return eval_heightfield_cell(C, A.hgt, A.hf_spheres[(c - active) >> 2], (c - active) & 3);
+    return eval_heightfield(C, A.hgt, A.hf_spheres[c - active]);
+  }
   int lo = 0, hi = A.d.num_pairs;  // largest p with poff[p] <= c (then c < poff[p + 1])
   while (hi - lo > 1) {
     const int mid = (lo + hi) >> 1;
@@ -291,7 +383,7 @@ __global__ void __launch_bounds__(kColThreads) collide_count_kernel(ColArgs A, c
   __shared__ float ang_cs[kMaxCylSamples], ang_sn[kMaxCylSamples];
   const ColCtx C{A.d, A.geo, A.body, A.flat_id, A.xform, A.size, body_q, (int)blockIdx.x};
   const int active = world_setup(A, C, poff, ang_cs, ang_sn, wave_tot);
-  const int cand = active + A.d.num_hf_spheres;
+  const int cand = active + A.d.num_hf_spheres * A.d.hf_points;
   int count = 0;
   for (int c0 = 0; c0 < cand; c0 += kColThreads) {
     const int c = c0 + threadIdx.x;
@@ -337,7 +429,7 @@ __global__ void __launch_bounds__(kColThreads) collide_write_kernel(ColArgs A, c
   __shared__ float ang_cs[kMaxCylSamples], ang_sn[kMaxCylSamples];
   const ColCtx C{A.d, A.geo, A.body, A.flat_id, A.xform, A.size, body_q, (int)blockIdx.x};
   const int active = world_setup(A, C, poff, ang_cs, ang_sn, wave_tot);
-  const int cand = active + A.d.num_hf_spheres;
+  const int cand = active + A.d.num_hf_spheres * A.d.hf_points;
   int base = offset[blockIdx.x];
   for (int c0 = 0; c0 < cand; c0 += kColThreads) {
     const int c = c0 + threadIdx.x;
@@ -395,6 +487,10 @@ int ofb_collider_create(const ofb_collider_desc* d, ofb_collider** out) {
   }
   if (d->cylinder_samples > kMaxCylSamples || P > kMaxColPairs) {
     ofb::set_error("collider limits: cylinder_samples <= %d, num_pairs <= %d", kMaxCylSamples, kMaxColPairs);
+    return -2;
+  }
+  if (d->num_hf_spheres > 0 && d->hf_points != 1 && d->hf_points != 4) {
+    ofb::set_error("hf_points must be 1 or 4");
     return -2;
   }
   if (d->num_hf_spheres > 0 && (d->hf_col < 0 || d->hf_col >= SC || d->hf_nx < 2 || d->hf_ny < 2 ||

@@ -12,13 +12,17 @@ namespace ofb {
 // row count's registers.  Worlds of at most 64 rows (R = 1) solve PCR on the assembled
 // (dense) Schur matrix; with pair_small_worlds, those of at most 32 rows take class code
 // 0 and run two per wavefront.  Worlds are bucketed on the device every step.
-constexpr int kNumClasses = 12;
+// Row classes: rows per lane of each instance (code 0: two worlds of at most 32 rows per
+// wave).  There are no 8- and 10-row instances: compiled, they returned wrong iterates for
+// worlds of 129-640 rows (against the oracle and against the 7-, 12- and 16-row
+// instances on the same worlds), so worlds above 448 rows run in the 12- or 16-row one.
+constexpr int kNumClasses = 10;
 constexpr int kPairRows = 32;
 constexpr int kSideStreams = 3;
 // side stream of a one-world-per-wave class: the dense class, the R = 2 class, the rest
 static int side_of(int c) { return c <= 1 ? 0 : (c == 2 ? 1 : 2); }
-__constant__ int kClassR[kNumClasses] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 16};
-static const int kClassRHost[kNumClasses] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 16};
+__constant__ int kClassR[kNumClasses] = {0, 1, 2, 3, 4, 5, 6, 7, 12, 16};
+static const int kClassRHost[kNumClasses] = {0, 1, 2, 3, 4, 5, 6, 7, 12, 16};
 
 __global__ void reset_classes_kernel(int32_t* counts) {
   if (threadIdx.x < kNumClasses) counts[threadIdx.x] = 0;

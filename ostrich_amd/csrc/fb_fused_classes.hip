@@ -2,7 +2,7 @@
 // set of row classes.  Compiled once per (OFB_BJ, OFB_RSET) pair so that the instances
 // build in parallel; OFB_BJ = 1 selects per-body-pair block-Jacobi, OFB_RSET = 0 the
 // dense classes (code 0: two worlds per wave, code 1: one) and R = 2..5, OFB_RSET = 1 the
-// classes R = 6..16.
+// classes R = 6, 7, 12 and 16.
 
 #include "fb_fused_kernel.h"
 
@@ -25,7 +25,7 @@ int OFB_CAT(fused_class_op_bj, OFB_BJ, _set, OFB_RSET)(int R, int op, const Fuse
     case 1: return fused_class_op<1, OFB_BJ != 0, 64, true>(op, P, blocks, threads, lds, st);
     CASE(2) CASE(3) CASE(4) CASE(5)
 #else
-    CASE(6) CASE(7) CASE(8) CASE(10) CASE(12) CASE(16)
+    CASE(6) CASE(7) CASE(12) CASE(16)
 #endif
 #undef CASE
   }

@@ -94,3 +94,98 @@ def test_tracked_flipper_scene_shapes():
     col = collide.PrimitiveCollider(model, gap=0.02, max_contacts_per_world=64)
     out = col.collide(torch.as_tensor(state["body_q"]))
     assert int(out["count"][0]) >= 4  # every robot starts touching the terrain
+
+
+def _brute_cell(hf, c, ci, cj, n=300):
+    """Closest point of grid cell (ci, cj)'s two triangles to c by dense barycentric
+    sampling, and its distance."""
+    h = hf["heights"]
+    (ox, oy), (sx, sy) = hf["origin"], hf["spacing"]
+    P = lambda i, j: np.array([ox + i * sx, oy + j * sy, h[j, i]], np.float64)
+    P00, P10, P01, P11 = P(ci, cj), P(ci + 1, cj), P(ci, cj + 1), P(ci + 1, cj + 1)
+    s, t = np.meshgrid(np.linspace(0, 1, n), np.linspace(0, 1, n))
+    m = s + t <= 1.0
+    s, t = s[m], t[m]
+    best, bd = None, np.inf
+    for a, b, cc in ((P00, P10, P11), (P00, P11, P01)):
+        pts = a + s[:, None] * (b - a) + t[:, None] * (cc - a)
+        d = np.linalg.norm(pts - c, axis=1)
+        k = int(d.argmin())
+        if d[k] < bd:
+            bd, best = d[k], pts[k]
+    return best, bd
+
+
+def test_cell_contacts_are_closest_points_of_the_cells():
+    """Mode "cells": a sphere's contact with each of the four grid cells around its
+    nearest grid vertex lies at the closest point of the cell (checked against dense
+    sampling of the cell's two triangles), with the normal from that point to the centre
+    and distance |c - p| - r; a cell gives none only when the centre is behind the planes
+    of both its triangles (a neighbouring triangle holds the contact)."""
+    hf = scenes.gaussian_bump_heightfield(7, n=32, spacing=0.05)
+    (ox, oy), (sx, sy) = hf["origin"], hf["spacing"]
+    rng = np.random.default_rng(3)
+    r = 0.05
+    xy = rng.uniform(ox + 3 * sx, ox + 27 * sx, size=(12, 2))
+    z = scenes.heightfield_height(hf, xy[:, 0], xy[:, 1]) + r + rng.uniform(-0.01, 0.015, 12)
+    centres = np.column_stack([xy, z])
+    model, q = _sphere_world(centres, r, hf)
+    col = collide.PrimitiveCollider(model, gap=0.5, max_contacts_per_world=4, hf_contacts="cells")
+    c = torch.as_tensor(centres, dtype=torch.float32)[:, None, :]
+    foot, nrm, dist, ok = (t.numpy() for t in col._heightfield_cell_contacts(c, torch.full((12, 1), r)))
+    h = hf["heights"]
+    P = lambda i, j: np.array([ox + i * sx, oy + j * sy, h[j, i]], np.float64)
+    used = 0
+    for w in range(12):
+        cw = centres[w]
+        I = int(np.floor((cw[0] - ox) / sx + 0.5))
+        J = int(np.floor((cw[1] - oy) / sy + 0.5))
+        for qd in range(4):
+            ci, cj = I - 1 + (qd & 1), J - 1 + (qd >> 1)
+            if not ok[w, 0, qd]:
+                P00, P10, P01, P11 = P(ci, cj), P(ci + 1, cj), P(ci, cj + 1), P(ci + 1, cj + 1)
+                for A, B_, C_ in ((P00, P10, P11), (P00, P11, P01)):
+                    n = np.cross(B_ - A, C_ - A)
+                    assert (cw - A) @ n <= 1e-6 * np.linalg.norm(n)
+                continue
+            used += 1
+            pb, db = _brute_cell(hf, cw, ci, cj)
+            assert np.linalg.norm(foot[w, 0, qd] - pb) < 4e-3 * sx
+            d = np.linalg.norm(cw - foot[w, 0, qd])
+            assert abs(d - db) < 1e-4 * sx + 1e-6
+            assert abs(dist[w, 0, qd] - (d - r)) < 1e-5
+            np.testing.assert_allclose(nrm[w, 0, qd], (cw - foot[w, 0, qd]) / d, atol=2e-4)
+    assert used >= 36
+    # the contacts the collider emits are those cells' (normal from the sphere to the terrain)
+    out = col.collide(torch.as_tensor(q))
+    n = int(out["count"][0])
+    assert n == used
+    np.testing.assert_allclose(out["normal"][:n].numpy(), -nrm[ok], atol=1e-6)
+    np.testing.assert_allclose(out["point1"][:n].numpy(), foot[ok], atol=1e-6)
+
+
+def test_cell_contacts_on_a_flat_field():
+    """Flat terrain at height 0.3, sphere centre at (x, y, 0.3 + r + g): the cell under the
+    centre gives distance g with a vertical normal; a cell beside it, at lateral offset l
+    from the centre, gives sqrt((r + g)^2 + l^2) - r; cells off the grid give none."""
+    r, g = 0.05, 0.004
+    hf = _plane_field(0.0, 0.0, 0.3, n=16, spacing=0.05)
+    (ox, oy), (sx, sy) = hf["origin"], hf["spacing"]
+    x, y = ox + 7.3 * sx, oy + 7.6 * sy    # nearest vertex (7, 8): cells i 6..7, j 7..8
+    model, q = _sphere_world(np.array([[x, y, 0.3 + r + g]]), r, hf)
+    out = collide.PrimitiveCollider(model, gap=0.5, max_contacts_per_world=4, hf_contacts="cells").collide(
+        torch.as_tensor(q))
+    assert int(out["count"][0]) == 4
+    p1 = out["point1"][:4].numpy()
+    cells = [(6, 7), (7, 7), (6, 8), (7, 8)]
+    for k, (ci, cj) in enumerate(cells):
+        px = np.clip(x, ox + ci * sx, ox + (ci + 1) * sx)
+        py = np.clip(y, oy + cj * sy, oy + (cj + 1) * sy)
+        np.testing.assert_allclose(p1[k], [px, py, 0.3], atol=1e-5)
+    d = np.linalg.norm(np.array([x, y, 0.3 + r + g]) - p1, axis=1) - r
+    assert abs(d[1] - g) < 1e-5                                            # the cell under the centre
+    # at the corner of the grid only the cell inside it remains
+    model, q = _sphere_world(np.array([[ox + 0.2 * sx, oy + 0.3 * sy, 0.3 + r]]), r, hf)
+    out = collide.PrimitiveCollider(model, gap=0.5, max_contacts_per_world=4, hf_contacts="cells").collide(
+        torch.as_tensor(q))
+    assert int(out["count"][0]) == 1

@@ -65,9 +65,9 @@ def _perturb(q, rng, pos=0.03, rot=0.15):
     return q.astype(np.float32)
 
 
-def _host(model, q, K, gap=GAP, slack=SLACK):
-    return collide.PrimitiveCollider(model, gap=gap, lateral_slack=slack, max_contacts_per_world=K).collide(
-        torch.as_tensor(q))
+def _host(model, q, K, gap=GAP, slack=SLACK, hf="point"):
+    return collide.PrimitiveCollider(model, gap=gap, lateral_slack=slack, max_contacts_per_world=K,
+                                     hf_contacts=hf).collide(torch.as_tensor(q))
 
 
 def _world_of(model, shape0):
@@ -80,13 +80,13 @@ def _by_world(model, c):
     return n, w, torch.bincount(w, minlength=model.num_worlds)
 
 
-def _compare(model, hip, q):
+def _compare(model, hip, q, gap=GAP, hf="point"):
     """Per-world comparison of the uncapped HIP contacts with the host's; returns the
     number of worlds compared exactly."""
     big = 4096
-    ref = _host(model, q, big)
-    lo = _by_world(model, _host(model, q, big, GAP - DELTA, SLACK - DELTA))[2]
-    hi = _by_world(model, _host(model, q, big, GAP + DELTA, SLACK + DELTA))[2]
+    ref = _host(model, q, big, gap, hf=hf)
+    lo = _by_world(model, _host(model, q, big, gap - DELTA, SLACK - DELTA, hf))[2]
+    hi = _by_world(model, _host(model, q, big, gap + DELTA, SLACK + DELTA, hf))[2]
     out = hip.collide(torch.as_tensor(q, device="cuda:0").contiguous())
     torch.cuda.synchronize()
     n = int(out.rigid_contact_count.item())
@@ -121,7 +121,7 @@ def _scene(name):
         model, state, _ = scenes.skid_steer(128, seed=1)
     elif name == "box_stack":
         model, state, _ = scenes.box_stack(64, n_cubes=3, seed=0)
-    elif name == "tracked_flipper":
+    elif name in ("tracked_flipper", "tracked_flipper_cells"):
         model, state, _ = scenes.tracked_flipper(64, seed=2)
     else:
         model, q = _mixed_scene()
@@ -129,14 +129,18 @@ def _scene(name):
     return model, np.asarray(state["body_q"], np.float64)
 
 
-@pytest.mark.parametrize("name", ["skid_steer", "box_stack", "tracked_flipper", "mixed"])
+@pytest.mark.parametrize("name", ["skid_steer", "box_stack", "tracked_flipper", "tracked_flipper_cells", "mixed"])
 def test_hip_collider_matches_host_collider(name, gpu_device):
+    """tracked_flipper_cells: four contacts per track sphere (the grid cells around it) at
+    a 0.05 m gap, the tracked-robot workload's contact density (tens to ~240 a world)."""
     model, q0 = _scene(name)
-    hip = collide.HipCollider(model, device=gpu_device, gap=GAP, lateral_slack=SLACK, max_contacts_per_world=4096)
+    hf, gap = ("cells", 0.05) if name.endswith("_cells") else ("point", GAP)
+    hip = collide.HipCollider(model, device=gpu_device, gap=gap, lateral_slack=SLACK, max_contacts_per_world=4096,
+                              hf_contacts=hf)
     rng = np.random.default_rng(11)
     for k in range(4):
         q = _perturb(q0, rng) if k else np.asarray(q0, np.float32)
-        worlds, n = _compare(model, hip, q)
+        worlds, n = _compare(model, hip, q, gap, hf)
         print(f"[collide] {name} pose set {k}: {n} contacts, {worlds}/{model.num_worlds} worlds exact", flush=True)
 
 
