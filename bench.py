@@ -348,15 +348,17 @@ def adjoint_leg(device, worlds=1024, steps=50, checkpoint_every=10, seed=3):
     # the adjoint solves alone: a second backward with each step_backward's own HIP-event
     # time (packing + adjoint kernels on the step's stream) collected after the call
     adj = []
-    inner = eng.step_backward
+    inner = eng.step_backward_explicit
 
     def timed_backward(*a, **kw):
         out = inner(*a, **kw)
         adj.append(eng.last_solve_ms())
         return out
-    eng.step_backward = timed_backward
+    eng.step_backward_explicit = timed_backward
     ro.backward(gq, torch.zeros_like(fin.body_qd))
-    eng.step_backward = inner
+    del eng.step_backward_explicit  # back to the class's method
+    if len(adj) != steps:
+        raise RuntimeError(f"adjoint timing saw {len(adj)} adjoint solves for {steps} steps")
     return dict(worlds=worlds, steps=steps, checkpoint_every=checkpoint_every, forward_ms=fwd, backward_ms=bwd,
                 backward_ms_per_step=bwd / steps, adjoint_solve_ms_per_step=float(np.mean(adj)),
                 loss=float(loss.item()), grad_target_vel_norm=float(g["joint_target_vel"].norm().item()),
