@@ -90,15 +90,15 @@ def contacts_of(collider, body_q, device):
     return Contacts.from_dict(collider.collide(body_q), device)
 
 
-def row_class(rows, pair_rows=32):
-    """Class code of worlds with `rows` active rows (tensor): 0 = paired (<= 32 rows), else
-    R = the smallest class of row slots per lane holding them (1 = dense, <= 64 rows)."""
-    R_LIST = (1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 16)
-    r = (rows + 63) // 64
-    code = torch.zeros_like(rows)
-    for R in reversed(R_LIST):
-        code = torch.where(r <= R, torch.full_like(rows, R), code)
-    return torch.where(rows <= pair_rows, torch.zeros_like(rows), code)
+def row_class(rows, max_rows):
+    """Class index of worlds with `rows` active rows (tensor): the first class whose largest
+    row count (OstrichEngine.class_shapes()["max_rows"], 0 for a class not launched) holds
+    them, as the solver buckets them."""
+    idx = torch.full_like(rows, -1)
+    for c in reversed(range(len(max_rows))):
+        if max_rows[c] > 0:
+            idx = torch.where(rows <= max_rows[c], torch.full_like(rows, c), idx)
+    return idx
 
 
 def summarise(per_step, W, steps, step_ms, atol, nr_cap):
@@ -197,12 +197,13 @@ def run_path(variant, model, state, control, steps, warmup, device, collider, K=
             per["cnt"].append(cnt.cpu().numpy())
             if class_timing:
                 ct = eng.class_times()
+                shapes = eng.class_shapes()
                 rows = base_rows + 3 * cnt
-                code = row_class(rows, 32 if paired else -1)
+                idx = row_class(rows, shapes["max_rows"])
                 by = compulsory_bytes_per_world(model, cnt.double(), eng.num_constraints)
                 cls_ms.append(ct["ms"])
                 cls_worlds.append(ct["worlds"])
-                cls_bytes.append([float(by[code == R].sum()) for R in (0, 1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 16)])
+                cls_bytes.append([float(by[idx == c].sum()) for c in range(len(ct["ms"]))])
         if graph:
             s_a.copy_(s_b)  # the graph reads s_a and writes s_b
         else:
@@ -216,12 +217,14 @@ def run_path(variant, model, state, control, steps, warmup, device, collider, K=
         wk = np.array(cls_worlds)
         by = np.array(cls_bytes)
         classes = []
-        caps = eng.class_times()["rows_cap"]
+        shapes = eng.class_shapes()
         for c in range(ms.shape[1]):
             timed = ms[:, c] >= 0
             if not timed.any() and wk[:, c].max() == 0:
                 continue
-            classes.append(dict(rows_cap=int(caps[c]), worlds_mean=float(wk[:, c].mean()),
+            classes.append(dict(rows_cap=int(shapes["slots"][c] * shapes["lanes"][c]),
+                                max_rows=int(shapes["max_rows"][c]), slots_per_lane=int(shapes["slots"][c]),
+                                lanes_per_world=int(shapes["lanes"][c]), worlds_mean=float(wk[:, c].mean()),
                                 ms_mean=float(ms[timed, c].mean()) if timed.any() else None,
                                 bytes_mean=float(by[:, c].mean())))
         out["classes"] = classes
@@ -575,19 +578,32 @@ def adjoint_fd_check(device, worlds=8, horizons=(1, 50), h=1e-4, seed=3):
 METRIC = "world-steps/s at FB residual\u22641e-5 (plain vs fused+warm-start); adjoint ms"
 
 
+def kernel_name(c):
+    """Kernel instance of a class entry of run_path's "classes"."""
+    R, G = c["slots_per_lane"], c["lanes_per_world"]
+    if G == 32:
+        return "fused_step_kernel<1, *, 32, true> (worlds of at most 32 rows, two per wavefront)"
+    if G == 64 and R == 1:
+        return "fused_step_kernel<1, *, 64, true> (worlds of at most 64 rows, dense Schur matrix)"
+    if G == 64:
+        return f"fused_step_kernel<{R}, *, 64, false> (worlds of at most {R * G} rows, matrix-free operator)"
+    return (f"fused_step_kernel<{R}, false, {G}, false> (worlds of {c['max_rows'] // 2 + 1}-{c['max_rows']} rows "
+            f"over {G // 64} wavefronts, matrix-free operator)")
+
+
 def dominant_class_roofline(res):
-    """Roofline of the eager variant's dominant row-class launch (largest mean duration):
-    the compulsory HBM bytes of the worlds it solved per launch over its HIP-event time."""
+    """Roofline of the eager variant's dominant row-class launch, the one that keeps the most
+    wavefronts busy for the longest (wavefronts of its worlds x its mean duration): the
+    compulsory HBM bytes of the worlds it solved per launch over its HIP-event time."""
     classes = [c for c in res.get("classes", []) if c.get("ms_mean")]
     if not classes:
         return None
-    c = max(classes, key=lambda x: x["ms_mean"] * (x["worlds_mean"] > 0))
+    def wave_ms(x):
+        waves = x["worlds_mean"] * x["lanes_per_world"] / 64.0
+        return x["ms_mean"] * waves
+    c = max(classes, key=wave_ms)
     achieved = c["bytes_mean"] / (c["ms_mean"] / 1e3) / 1e9
-    kind = ("fused_step_kernel<1, *, 32, true> (worlds of at most 32 rows, two per wavefront)" if c["rows_cap"] == 32
-            else f"fused_step_kernel<1, *, 64, true> (worlds of at most 64 rows, dense Schur matrix)"
-            if c["rows_cap"] == 64 else
-            f"fused_step_kernel<{c['rows_cap'] // 64}, *, 64, false> (worlds of at most {c['rows_cap']} rows, "
-            "matrix-free operator)")
+    kind = kernel_name(c)
     return dict(achieved=achieved, frac=achieved / HBM_PEAK_GBS, kernel=kind, kernel_ms=c["ms_mean"],
                 worlds_per_launch=c["worlds_mean"], bytes_per_launch=c["bytes_mean"])
 
@@ -651,7 +667,7 @@ def main():
         if os.path.exists(tpath):
             with open(tpath) as f:
                 t = json.load(f)
-            if t.get("rows_cap") == dom_rows(dom):
+            if dom["kernel"].replace("*", "false").startswith(t.get("kernel", "?")):
                 roof["traffic"] = t["fetch_bytes"] + t["write_bytes"]
                 roof["traffic_source"] = ("not measured in this run: HBM bytes per launch of that kernel from "
                                           "rocprofv3 FETCH_SIZE + WRITE_SIZE scaled by a 256 MiB copy, "
@@ -744,16 +760,6 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
     print(json.dumps(out))
-
-
-def dom_rows(dom):
-    """Row capacity of the kernel a roofline entry names."""
-    k = dom["kernel"]
-    if "two per wavefront" in k:
-        return 32
-    if "dense Schur" in k:
-        return 64
-    return int(k.split("at most ")[1].split(" rows")[0])
 
 
 if __name__ == "__main__":

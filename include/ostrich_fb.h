@@ -376,15 +376,22 @@ int ofb_last_kernel_ms(const ofb_solver* solver, float* ms);
 
 /* Per-class timing of the fused path (no reference counterpart; measurement hook).  With
  * timing on, every eager step records HIP events around each row-class launch on the
- * stream it runs on (the paired class on the step's stream, the others on the solver's
- * side stream, where they run one after another beside the paired class).
+ * stream it runs on (the step's stream or one of the solver's side streams, which run
+ * beside it).
  * ofb_get_class_times fills, for each of the `count` classes (capacity >= 10), its row
  * capacity, the number of worlds the last step put in it and its duration in ms (-1 when
  * the class was not launched or the step was not timed); it blocks until the step has
- * finished.  -2 on the plain path. */
+ * finished.  ofb_get_class_shapes fills, per class, the row slots per lane and the lanes
+ * per world of the kernel instance that class runs (lanes 32: two worlds per wavefront;
+ * 64: one world per wavefront; 128, 256, 512: one world over 2, 4, 8 wavefronts of its
+ * workgroup) and the largest row count of a world in that class (0: the class is not
+ * launched); a world of n rows is in the first class whose largest row count is >= n.
+ * -2 on the plain path. */
 int ofb_set_class_timing(ofb_solver* solver, int32_t enabled);
 int ofb_get_class_times(const ofb_solver* solver, int32_t capacity, int32_t* rows_cap, int32_t* worlds, float* ms,
                         int32_t* count);
+int ofb_get_class_shapes(const ofb_solver* solver, int32_t capacity, int32_t* slots_per_lane,
+                         int32_t* lanes_per_world, int32_t* max_rows, int32_t* count);
 
 /* Launch settings of the fused path: the largest number of wavefronts per workgroup of a
  * row-class launch (1..16; the LDS limit may lower it per class) and whether worlds of at

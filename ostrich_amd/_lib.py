@@ -17,7 +17,8 @@ EXPORTS = (
     "ofb_set_warm_start_net", "ofb_get_contact_features", "ofb_get_linesearch_steps",
     "ofb_last_kernel_ms", "ofb_set_fused_launch", "ofb_get_fused_launch", "ofb_trajectory_adam_step",
     "ofb_collider_create", "ofb_collider_destroy", "ofb_collide", "ofb_set_iteration_history",
-    "ofb_log_create", "ofb_log_capture", "ofb_log_destroy", "ofb_set_class_timing", "ofb_get_class_times", "ofb_set_keep_lambda_prev",
+    "ofb_log_create", "ofb_log_capture", "ofb_log_destroy", "ofb_set_class_timing", "ofb_get_class_times",
+    "ofb_get_class_shapes", "ofb_set_keep_lambda_prev",
     "ofb_warm_start_state_size", "ofb_warm_start_state_get", "ofb_warm_start_state_set",
     "ofb_mlp_gradient_noalloc",
 )
@@ -50,6 +51,7 @@ def load():
     lib.ofb_warm_start_state_get.argtypes = [vp, vp, vp]
     lib.ofb_warm_start_state_set.argtypes = [vp, vp, vp]
     lib.ofb_get_class_times.argtypes = [vp, i32, vp, vp, vp, vp]
+    lib.ofb_get_class_shapes.argtypes = [vp, i32, vp, vp, vp, vp]
     lib.ofb_get_fused_launch.argtypes = [vp, vp, vp]
     f32 = C.c_float
     lib.ofb_trajectory_adam_step.argtypes = [vp, vp, vp, vp] + [i32] * 5 + [f32] * 5 + [i32, vp]

@@ -168,7 +168,8 @@ __global__ void __launch_bounds__(256) adjoint_step_kernel(AdjParams P) {
   const int n_r = L.Nj + L.Nctrl + 3 * n_c;
   // row -> body sums: lane-private copies when the launch gave each wave a region for
   // them, else LDS atomics
-  const BodySum<WAVE, false> S{sm, sm + (P.rs_off >= 0 ? P.rs_off : 0), lane, B, P.rs_off >= 0 && rs_enabled(B)};
+  const BodySum<WAVE, false> S{sm, sm + (P.rs_off >= 0 ? P.rs_off : 0), lane, B, P.rs_off >= 0 && rs_enabled(B),
+                                 nullptr};
   float lam[R], x[R];
   int pcr_it = 0;
   if (sub == 0) {
@@ -281,8 +282,9 @@ __global__ void __launch_bounds__(256) adjoint_step_kernel(AdjParams P) {
   }
   const float tol_sq = P.cfg.lin_tol >= 0.f ? P.cfg.lin_tol * P.cfg.lin_tol : -1.f;
   const float latol_sq = P.cfg.lin_atol >= 0.f ? P.cfg.lin_atol * P.cfg.lin_atol : -1.f;
+  WorldSum<WAVE> WS{nullptr, 0};  // one world per wave: reductions stay inside the wave
   pcr_it = pcr_solve<R, BJ, WAVE, false>(sm, lane, B, n_r, S, V, G, act, b0, b1, J0, J1, Cr, Pi, rhs, bsq, tol_sq,
-                                         latol_sq, P.cfg.lin_max_iters, x);
+                                         latol_sq, P.cfg.lin_max_iters, x, WS);
 
   // ---- 4. w_u = w_u0 - M^-1 J^T w_lambda ----
   S.begin(BACC, true);

@@ -669,6 +669,13 @@ int ofb_get_class_times(const ofb_solver* s, int32_t capacity, int32_t* rows_cap
   return fused_class_times(s, capacity, rows_cap, worlds, ms, count);
 }
 
+int ofb_get_class_shapes(const ofb_solver* s, int32_t capacity, int32_t* slots, int32_t* lanes, int32_t* max_rows,
+                         int32_t* count) {
+  if (!s || !count) { set_error("null argument"); return -1; }
+  if (s->path != OFB_PATH_FUSED) { set_error("row classes apply to the fused path only"); return -2; }
+  return fused_class_shapes(s, capacity, slots, lanes, max_rows, count);
+}
+
 int ofb_set_fused_launch(ofb_solver* s, int32_t waves_per_block, int32_t pair_small_worlds) {
   if (!s) { set_error("null solver"); return -1; }
   if (s->path != OFB_PATH_FUSED) { set_error("launch settings apply to the fused path only"); return -2; }

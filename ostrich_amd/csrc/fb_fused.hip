@@ -7,25 +7,78 @@
 
 namespace ofb {
 
-// Row classes: a world with n_r active rows runs the instance with R = the smallest
-// class >= ceil(n_r / 64) slots per lane, so small worlds do not pay for the padded
-// row count's registers.  Worlds of at most 64 rows (R = 1) solve PCR on the assembled
-// (dense) Schur matrix; with pair_small_worlds, those of at most 32 rows take class code
-// 0 and run two per wavefront.  Worlds are bucketed on the device every step.
-// Row classes: rows per lane of each instance (code 0: two worlds of at most 32 rows per
-// wave).  There are no 8- and 10-row instances: compiled, they returned wrong iterates for
-// worlds of 129-640 rows (against the oracle and against the 7-, 12- and 16-row
-// instances on the same worlds), so worlds above 448 rows run in the 12- or 16-row one.
-constexpr int kNumClasses = 10;
+// Row classes.  Worlds are bucketed on the device every step by active rows n_r; a class
+// is a kernel instance with a row capacity:
+//  * code 0: worlds of at most 32 rows, two per wavefront, assembled (dense) Schur matrix
+//    in VGPRs (only with pair_small_worlds);
+//  * code 1: at most 64 rows, one world per wavefront, dense;
+//  * code R = 2..16: at most 64 R rows, one world per wavefront, R row slots per lane,
+//    matrix-free operator;
+//  * code kMW + NW: at most 128 NW rows, one world per workgroup of NW wavefronts, two
+//    row slots per lane, matrix-free (Jacobi only).
+// A world runs in the first class whose capacity holds it.  Tables:
+//  * Jacobi forward: the paired and dense classes, R = 2, then worlds of 2, 4 and 8
+//    waves.  Large worlds set a step's duration (a wave's time is its slowest world's),
+//    and on several waves each lane walks two rows instead of up to sixteen, without the
+//    register spills of the many-slot instances.
+//  * block-Jacobi forward: one wavefront per world up to R = 16 (the per-pair Cholesky
+//    works inside one wave); the classes from R = 3 on hold few worlds each and are
+//    launched as one, in the instance of the largest world the solver can hold.  There
+//    are no 8- and 10-row instances: compiled, they returned wrong iterates for worlds of
+//    129-640 rows (against the oracle and against the 7-, 12- and 16-row instances on the
+//    same worlds), so worlds above 448 rows run in the 12- or 16-row one.
+//  * adjoint: one wavefront per world, R = 1..16 (no pairing, no merging).
+constexpr int kMaxClasses = 10;
 constexpr int kPairRows = 32;
-constexpr int kSideStreams = 3;
-// side stream of a one-world-per-wave class: the dense class, the R = 2 class, the rest
-static int side_of(int c) { return c <= 1 ? 0 : (c == 2 ? 1 : 2); }
-__constant__ int kClassR[kNumClasses] = {0, 1, 2, 3, 4, 5, 6, 7, 12, 16};
-static const int kClassRHost[kNumClasses] = {0, 1, 2, 3, 4, 5, 6, 7, 12, 16};
+#ifndef OFB_SIDE_STREAMS
+#define OFB_SIDE_STREAMS 3
+#endif
+constexpr int kSideStreams = OFB_SIDE_STREAMS;
+static_assert(kSideStreams >= 1 && kSideStreams <= 8, "the solver holds at most 8 side streams");
+constexpr int kMW = kMultiWave;
+
+struct ClassTable {
+  int n;                   // classes
+  int rows[kMaxClasses];   // row capacity of each class
+  int code[kMaxClasses];   // kernel instance of each class
+  int side[kMaxClasses];   // side stream of each class (-1: the step's stream; class 0 runs there, last)
+  int merge_from;          // classes from this index on are launched as one (n: none)
+};
+// Streams of the Jacobi classes (measured, 4096 uneven skid-steer worlds): the dense class
+// on the step's stream ahead of the paired class, the R = 2 class and the worlds of two
+// waves on side streams of their own, the rare worlds of four and eight waves on the
+// third.  One class per stream (five streams) replays about as fast from a graph but
+// steps slower eagerly; the large classes sharing one stream added their durations.
+#ifndef OFB_JACOBI_SIDES
+#define OFB_JACOBI_SIDES -1, -1, 0, 1, 2, 2
+#endif
+static const ClassTable kTabJacobi = {6, {32, 64, 128, 256, 512, 1024}, {0, 1, 2, kMW + 2, kMW + 4, kMW + 8},
+                                      {OFB_JACOBI_SIDES}, 6};
+static const ClassTable kTabBlockJacobi = {10, {32, 64, 128, 192, 256, 320, 384, 448, 768, 1024},
+                                           {0, 1, 2, 3, 4, 5, 6, 7, 12, 16}, {-1, 0, 1, 2, 2, 2, 2, 2, 2, 2}, 3};
+static const ClassTable kTabAdjoint = {10, {32, 64, 128, 192, 256, 320, 384, 448, 768, 1024},
+                                       {0, 1, 2, 3, 4, 5, 6, 7, 12, 16}, {-1, 0, 0, 0, 0, 0, 0, 0, 0, 0}, 10};
+
+static const ClassTable& fwd_table(const ofb_solver* s) {
+  return s->cfg.preconditioner == OFB_PRECOND_PER_BODY_PAIR ? kTabBlockJacobi : kTabJacobi;
+}
+
+// the capacities passed to the classifier by value
+struct ClassCaps {
+  int n, first, merge_from;  // classes, first class a world may take (1: no pairing), merge index
+  int rows[kMaxClasses];
+};
+static ClassCaps class_caps(const ClassTable& t, bool pairing) {
+  ClassCaps c;
+  c.n = t.n;
+  c.first = pairing ? 0 : 1;
+  c.merge_from = t.merge_from;
+  for (int i = 0; i < kMaxClasses; ++i) c.rows[i] = i < t.n ? t.rows[i] : 0;
+  return c;
+}
 
 __global__ void reset_classes_kernel(int32_t* counts) {
-  if (threadIdx.x < kNumClasses) counts[threadIdx.x] = 0;
+  if (threadIdx.x < kMaxClasses) counts[threadIdx.x] = 0;
 }
 
 // One pass over the worlds: each wave ranks its worlds per class with ballots, the
@@ -33,33 +86,29 @@ __global__ void reset_classes_kernel(int32_t* counts) {
 // block's range of each list (one atomic per world on a single counter serialises).
 // The order of a class list is not significant: every world is solved on its own.
 __global__ void __launch_bounds__(256) classify_worlds_kernel(int W, const int32_t* __restrict__ ccount, int K,
-                                                              int base_rows, int pair_rows, int merge_from,
+                                                              int base_rows, ClassCaps caps,
                                                               int32_t* __restrict__ counts,
                                                               int32_t* __restrict__ lists) {
-  __shared__ int wcnt[4][kNumClasses];  // worlds per (wave, class)
-  __shared__ int bbase[kNumClasses];    // block's base in each list
+  __shared__ int wcnt[4][kMaxClasses];  // worlds per (wave, class)
+  __shared__ int bbase[kMaxClasses];    // block's base in each list
   const int w = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
   int c = -1;
   if (w < W) {
-    int n_r = base_rows + 3 * min(ccount[w], K);
-    c = 0;
-    if (n_r > pair_rows) {
-      int r = max(1, (n_r + WAVE - 1) / WAVE);
-      c = 1;
-      while (c < kNumClasses - 1 && kClassR[c] < r) ++c;
-      if (c > merge_from) c = merge_from;
-    }
+    const int n_r = base_rows + 3 * min(ccount[w], K);
+    c = caps.first;
+    while (c < caps.n - 1 && caps.rows[c] < n_r) ++c;
+    if (c > caps.merge_from) c = caps.merge_from;
   }
   const unsigned long long below = (1ull << lane) - 1ull;
   int rank = 0;
-  for (int k = 0; k < kNumClasses; ++k) {
+  for (int k = 0; k < kMaxClasses; ++k) {
     const unsigned long long m = __ballot(c == k);
     if (c == k) rank = __popcll(m & below);
     if (lane == 0) wcnt[wave][k] = __popcll(m);
   }
   __syncthreads();
-  if (threadIdx.x < kNumClasses) {
+  if (threadIdx.x < kMaxClasses) {
     const int k = threadIdx.x;
     int tot = 0;
     for (int v = 0; v < (int)(blockDim.x / WAVE); ++v) tot += wcnt[v][k];
@@ -73,27 +122,35 @@ __global__ void __launch_bounds__(256) classify_worlds_kernel(int W, const int32
   }
 }
 
-// The forward step merges the classes from kMergeFrom on into one launch of the instance
-// for the largest world the solver can hold: those classes hold few worlds each, and one
-// after another their launches would add up their slowest worlds' Newton solves.
-constexpr int kMergeFrom = 3;
+// classes a world of this solver can fall in (capacities are increasing)
+static bool class_used(const ofb_solver* s, const ClassTable& t, int c, bool pairing) {
+  if (c >= t.n) return false;
+  if (c == 0) return pairing;
+  return c == 1 || t.rows[c - 1] < s->L.Nc;
+}
 
 // largest class a world of this solver can fall in
-static int top_class(const ofb_solver* s) {
-  const int r_max = (s->L.Nc + WAVE - 1) / WAVE;
+static int top_class(const ofb_solver* s, const ClassTable& t) {
   int c = 1;
-  while (c < kNumClasses - 1 && kClassRHost[c] < r_max) ++c;
+  while (c < t.n - 1 && t.rows[c] < s->L.Nc) ++c;
   return c;
 }
 
-// kernel instance (class code) of the forward launch of class list c
+// kernel instance of the forward launch of class list c
 static int launch_code(const ofb_solver* s, int c) {
-  return c == kMergeFrom ? kClassRHost[std::max(c, top_class(s))] : kClassRHost[c];
+  const ClassTable& t = fwd_table(s);
+  return c == t.merge_from ? t.code[std::max(c, top_class(s, t))] : t.code[c];
+}
+// forward launches: classes past the merge index are merged into it
+static bool class_launched(const ofb_solver* s, int c) {
+  const ClassTable& t = fwd_table(s);
+  return c <= t.merge_from && class_used(s, t, c, s->cfg.pair_small_worlds != 0);
 }
 
-// lanes per world and row capacity of a class code
-static int class_lanes(int code) { return code == 0 ? kPairRows : WAVE; }
-static int class_rows(int code) { return code == 0 ? kPairRows : WAVE * code; }
+// lanes per world, row slots per lane and row capacity of a kernel instance
+static int class_lanes(int code) { return code == 0 ? kPairRows : code > kMW ? WAVE * (code - kMW) : WAVE; }
+static int class_slots(int code) { return code == 0 ? 1 : code > kMW ? 2 : code; }
+static int class_rows(int code) { return class_lanes(code) * class_slots(code); }
 
 // LDS layout of one world of a class: a world of the class has at most
 // (rows_cap - joint and control rows) / 3 contacts
@@ -102,14 +159,16 @@ static FusedLayout class_layout(const ofb_solver* s, int code) {
   const int cap = class_rows(code);
   const int base = s->L.Nj + s->L.Nctrl;
   const int cs_cap = std::min(s->L.K, std::max(0, (cap - base) / 3));
-  return fused_layout(s->m.B, s->m.J, cap, cs_cap, bj, code <= 1);
+  return fused_layout(s->m.B, s->m.J, cap, cs_cap, bj, code <= 1, class_lanes(code));
 }
 static int lds_words(const ofb_solver* s, int code) { return class_layout(s, code).words; }
 
-// worlds per workgroup of a class: at most the tuned number of waves' worth, a whole
-// number of waves, within 64 KB of LDS per workgroup, and among those the choice that
-// keeps the most worlds resident in a CU's 160 KB (larger workgroups on ties)
+// worlds per workgroup of a class: a world of several waves has its workgroup to itself;
+// otherwise at most the tuned number of waves' worth, a whole number of waves, within
+// 64 KB of LDS per workgroup, and among those the choice that keeps the most worlds
+// resident in a CU's 160 KB (larger workgroups on ties)
 static int choose_wpb(const ofb_solver* s, int code) {
+  if (class_lanes(code) > WAVE) return 1;
   const int per_wave = WAVE / class_lanes(code);
   const size_t world_bytes = (size_t)lds_words(s, code) * sizeof(float);
   int best = per_wave, best_res = -1;
@@ -140,13 +199,13 @@ static int adj_class_op(int R, int op, const AdjParams& P, int blocks, int threa
   return rc;
 }
 
-static int class_op(int R, int op, const FusedParams& P, int blocks, int threads, size_t lds, hipStream_t st,
+static int class_op(int code, int op, const FusedParams& P, int blocks, int threads, size_t lds, hipStream_t st,
                     bool bj) {
-  int rc = bj ? fused_class_op_bj1_set0(R, op, P, blocks, threads, lds, st)
-              : fused_class_op_bj0_set0(R, op, P, blocks, threads, lds, st);
+  int rc = bj ? fused_class_op_bj1_set0(code, op, P, blocks, threads, lds, st)
+              : fused_class_op_bj0_set0(code, op, P, blocks, threads, lds, st);
   if (rc == -1)
-    rc = bj ? fused_class_op_bj1_set1(R, op, P, blocks, threads, lds, st)
-            : fused_class_op_bj0_set1(R, op, P, blocks, threads, lds, st);
+    rc = bj ? fused_class_op_bj1_set1(code, op, P, blocks, threads, lds, st)
+            : fused_class_op_bj0_set1(code, op, P, blocks, threads, lds, st);
   return rc;
 }
 
@@ -156,26 +215,37 @@ static int launch_fused(int code, FusedParams P, const ofb_solver* s, int cls, h
   P.rows_cap = class_rows(code);
   P.lay = class_layout(s, code);
   const int wpb = s->cls_wpb[cls];
+  const int per_block = class_lanes(code) > WAVE ? class_lanes(code) : wpb * class_lanes(code);
   size_t lds_bytes = (size_t)wpb * P.lay.words * sizeof(float);
   int blocks = (P.m.W + wpb - 1) / wpb;
-  return class_op(code, 0, P, blocks, wpb * class_lanes(code), lds_bytes, stream,
+  return class_op(code, 0, P, blocks, per_block, lds_bytes, stream,
                   s->cfg.preconditioner == OFB_PRECOND_PER_BODY_PAIR);
 }
 
-// lets a class kernel (forward and adjoint) use more than 64 KB of LDS when one world
-// needs it
-static int allow_lds(int code, const ofb_solver* s) {
+// lets a forward class instance use more than 64 KB of LDS when its smallest workgroup
+// (one wave, or one world of several waves) needs it
+static int allow_lds_fwd(int code, const ofb_solver* s) {
   const bool bj = s->cfg.preconditioner == OFB_PRECOND_PER_BODY_PAIR;
-  size_t bytes = (size_t)lds_words(s, code) * sizeof(float) * (WAVE / class_lanes(code));
+  const int per_wave = class_lanes(code) > WAVE ? 1 : WAVE / class_lanes(code);
+  const size_t bytes = (size_t)lds_words(s, code) * sizeof(float) * per_wave;
+  if (bytes > 163840) {
+    set_error("fused kernel needs %zu bytes of LDS per workgroup (at most 160 KB)", bytes);
+    return -4;
+  }
   if (bytes > 65536) {
     FusedParams P{};
     if (class_op(code, 1, P, 0, 0, bytes, nullptr, bj) != 0) {
-      set_error("fused kernel needs %zu bytes of LDS per wavefront", bytes);
+      set_error("fused kernel needs %zu bytes of LDS per workgroup", bytes);
       return -4;
     }
   }
-  const int R = code == 0 ? 1 : code;
-  bytes = (size_t)(adj_lds_words(s, R) + rs_words(WAVE, s->m.B)) * sizeof(float);
+  return 0;
+}
+
+// the same for the adjoint instance with R row slots per lane
+static int allow_lds_adj(int R, const ofb_solver* s) {
+  const bool bj = s->cfg.preconditioner == OFB_PRECOND_PER_BODY_PAIR;
+  const size_t bytes = (size_t)(adj_lds_words(s, R) + rs_words(WAVE, s->m.B)) * sizeof(float);
   if (bytes > 65536) {
     AdjParams A{};
     if (adj_class_op(R, 1, A, 0, 0, bytes, nullptr, bj) != 0) {
@@ -186,23 +256,19 @@ static int allow_lds(int code, const ofb_solver* s) {
   return 0;
 }
 
-// classes a world of this solver can fall in (class codes are increasing)
-static bool class_used(const ofb_solver* s, int c) {
-  const int r_max = (s->L.Nc + WAVE - 1) / WAVE;
-  if (c == 0) return s->cfg.pair_small_worlds != 0;
-  return c == 1 || kClassRHost[c - 1] < r_max;
-}
-// forward launches: classes past kMergeFrom are merged into it
-static bool class_launched(const ofb_solver* s, int c) { return c <= kMergeFrom && class_used(s, c); }
-
 // workgroup shape of every class launch for the solver's current launch settings
 static int fused_configure(ofb_solver* s) {
-  for (int c = 0; c < kNumClasses; ++c) {
+  for (int c = 0; c < kMaxClasses; ++c) {
     s->cls_wpb[c] = 1;
-    if (!class_used(s, c)) continue;
-    int rc = allow_lds(kClassRHost[c], s);
+    if (!class_launched(s, c)) continue;
+    int rc = allow_lds_fwd(launch_code(s, c), s);
     if (rc) return rc;
     s->cls_wpb[c] = choose_wpb(s, launch_code(s, c));
+  }
+  for (int c = 1; c < kTabAdjoint.n; ++c) {
+    if (!class_used(s, kTabAdjoint, c, false)) continue;
+    int rc = allow_lds_adj(kTabAdjoint.code[c], s);
+    if (rc) return rc;
   }
   return 0;
 }
@@ -241,15 +307,15 @@ int fused_classes_alloc(ofb_solver* s) {
     }
     s->side_ready = 3 + 2 * i;
   }
-  for (int c = 0; c < kNumClasses; ++c) {
+  for (int c = 0; c < kMaxClasses; ++c) {
     if (hipEventCreate(&s->ev_cls0[c]) != hipSuccess || hipEventCreate(&s->ev_cls1[c]) != hipSuccess) {
       set_error("hipEventCreate failed (class timing)");
       return -4;
     }
     s->cls_events_ready = c + 1;
   }
-  if (hipMalloc((void**)&s->cls_counts, kNumClasses * sizeof(int32_t)) != hipSuccess ||
-      hipMalloc((void**)&s->cls_lists, (size_t)kNumClasses * s->m.W * sizeof(int32_t)) != hipSuccess) {
+  if (hipMalloc((void**)&s->cls_counts, kMaxClasses * sizeof(int32_t)) != hipSuccess ||
+      hipMalloc((void**)&s->cls_lists, (size_t)kMaxClasses * s->m.W * sizeof(int32_t)) != hipSuccess) {
     set_error("hipMalloc failed (world classes)");
     return -4;
   }
@@ -276,18 +342,19 @@ void fused_classes_free(ofb_solver* s) {
   s->cls_lists = nullptr;
 }
 
-// Row capacity, world count and duration of every class of the last step (eager steps
-// with class timing on; ms = -1 for a class that was not launched or not timed).
+// Row capacity, world count and duration of every forward class of the last step (eager
+// steps with class timing on; ms = -1 for a class that was not launched or not timed).
 int fused_class_times(const ofb_solver* s, int capacity, int32_t* rows_cap, int32_t* worlds, float* ms,
                       int32_t* count) {
-  *count = kNumClasses;
-  if (capacity < kNumClasses) { set_error("class times need room for %d classes", kNumClasses); return -2; }
-  int32_t n[kNumClasses];
+  const ClassTable& t = fwd_table(s);
+  *count = t.n;
+  if (capacity < kMaxClasses) { set_error("class times need room for %d classes", kMaxClasses); return -2; }
+  int32_t n[kMaxClasses];
   if (hipMemcpy(n, s->cls_counts, sizeof(n), hipMemcpyDeviceToHost) != hipSuccess) {
     set_error("class count read-back failed");
     return -4;
   }
-  for (int c = 0; c < kNumClasses; ++c) {
+  for (int c = 0; c < t.n; ++c) {
     if (rows_cap) rows_cap[c] = class_rows(launch_code(s, c));
     if (worlds) worlds[c] = n[c];
     if (!ms) continue;
@@ -299,6 +366,24 @@ int fused_class_times(const ofb_solver* s, int capacity, int32_t* rows_cap, int3
         return -4;
       }
     }
+  }
+  return 0;
+}
+
+// Kernel shape of every forward class: row slots per lane and lanes per world of the
+// instance that class's launch runs, and the largest row count of a world the class list
+// holds (0: no world can fall in it, or it is merged into an earlier launch).  A world
+// of n_r rows is in the first class whose largest row count is >= n_r.
+int fused_class_shapes(const ofb_solver* s, int capacity, int32_t* slots, int32_t* lanes, int32_t* max_rows,
+                       int32_t* count) {
+  const ClassTable& t = fwd_table(s);
+  *count = t.n;
+  if (capacity < kMaxClasses) { set_error("class shapes need room for %d classes", kMaxClasses); return -2; }
+  for (int c = 0; c < t.n; ++c) {
+    const int code = launch_code(s, c);
+    if (slots) slots[c] = class_slots(code);
+    if (lanes) lanes[c] = class_lanes(code);
+    if (max_rows) max_rows[c] = !class_launched(s, c) ? 0 : c == t.merge_from ? s->L.Nc : t.rows[c];
   }
   return 0;
 }
@@ -324,20 +409,19 @@ int fused_step(ofb_solver* s, const ofb_state_in* in, const ofb_state_out* out, 
   P.st_nr = s->st_nr; P.st_pcr = s->st_pcr; P.st_best = s->st_best; P.st_flags = s->st_flags; P.st_merit = s->st_merit;
   P.hist_merit = s->hist_merit; P.hist_pcr = s->hist_pcr; P.hist_cap = s->hist_cap;
   const int W = s->m.W;
-  const int r_max = (s->L.Nc + WAVE - 1) / WAVE;
-  if (r_max > kClassRHost[kNumClasses - 1]) {
-    set_error("fused path supports at most %d constraint rows per world (got %d)",
-              WAVE * kClassRHost[kNumClasses - 1], s->L.Nc);
+  const ClassTable& t = fwd_table(s);
+  if (s->L.Nc > t.rows[t.n - 1]) {
+    set_error("fused path supports at most %d constraint rows per world (got %d)", t.rows[t.n - 1], s->L.Nc);
     return -3;
   }
   hipLaunchKernelGGL(reset_classes_kernel, dim3(1), dim3(64), 0, stream, s->cls_counts);
   hipLaunchKernelGGL(classify_worlds_kernel, dim3((W + 255) / 256), dim3(256), 0, stream, W, s->ccount, s->L.K,
-                     s->L.Nj + s->L.Nctrl, s->cfg.pair_small_worlds ? kPairRows : -1, kMergeFrom, s->cls_counts,
+                     s->L.Nj + s->L.Nctrl, class_caps(t, s->cfg.pair_small_worlds != 0), s->cls_counts,
                      s->cls_lists);
-  // The one-world-per-wave classes run on side streams, launched first so that their
-  // waves are resident beside the paired class's: the dense class, the R = 2 class and
-  // the larger classes (few worlds each, one after another) each on a stream of its own,
-  // so that no class waits for another to drain.
+  // The one-world-per-wave classes (and the worlds of several waves) run on side
+  // streams, launched first so that their waves are resident beside the paired class's,
+  // the largest worlds first; the table spreads the classes over the side streams so
+  // that the large classes do not wait for one another to drain.
   if (hipEventRecord(s->ev_fork, stream) != hipSuccess) {
     set_error("fused step: stream fork failed");
     return -4;
@@ -352,9 +436,9 @@ int fused_step(ofb_solver* s, const ofb_state_in* in, const ofb_state_out* out, 
   if (hipStreamIsCapturing(stream, &cap) != hipSuccess) cap = hipStreamCaptureStatusNone;
   const bool cls_timed = s->cls_timing && cap == hipStreamCaptureStatusNone;
   s->cls_timed = 0;
-  for (int c = kMergeFrom; c >= 1; --c) {  // the slowest worlds first
+  for (int c = std::min(t.merge_from, t.n - 1); c >= 1; --c) {  // the slowest worlds first
     if (!class_launched(s, c)) continue;  // no world can fall in this class
-    hipStream_t side = s->side_stream[side_of(c)];
+    hipStream_t side = t.side[c] < 0 ? stream : s->side_stream[t.side[c]];
     if (cls_timed && hipEventRecord(s->ev_cls0[c], side) != hipSuccess) {
       set_error("fused step: event record failed");
       return -4;
@@ -379,13 +463,13 @@ int fused_step(ofb_solver* s, const ofb_state_in* in, const ofb_state_out* out, 
   }
   // the paired class, which holds most worlds of a typical batch, on the step's stream,
   // bracketed by the kernel events (not recorded while the stream is being captured)
-  const bool timed = cap == hipStreamCaptureStatusNone && class_used(s, 0);
+  const bool timed = cap == hipStreamCaptureStatusNone && class_launched(s, 0);
   if (timed && hipEventRecord(s->ev_kernel0, stream) != hipSuccess) {
     set_error("fused step: event record failed");
     return -4;
   }
-  if (class_used(s, 0) && launch_fused(kClassRHost[0], P, s, 0, stream) != 0) {
-    set_error("no fused kernel instance for class code %d", kClassRHost[0]);
+  if (class_launched(s, 0) && launch_fused(t.code[0], P, s, 0, stream) != 0) {
+    set_error("no fused kernel instance for class code %d", t.code[0]);
     return -4;
   }
   if (timed && hipEventRecord(s->ev_kernel1, stream) != hipSuccess) {
@@ -430,20 +514,19 @@ int fused_adjoint(ofb_solver* s, const ofb_state_in* in, const ofb_state_out* ou
   P.g_mu = gout->shape_mu_grad; P.w_lam = gout->adjoint_lambda; P.w_u = gout->adjoint_body;
   P.st_pcr = s->st_pcr; P.st_flags = s->st_flags;
   const int W = s->m.W;
-  const int r_max = (s->L.Nc + WAVE - 1) / WAVE;
-  if (r_max > kClassRHost[kNumClasses - 1]) {
-    set_error("fused path supports at most %d constraint rows per world (got %d)",
-              WAVE * kClassRHost[kNumClasses - 1], s->L.Nc);
+  const ClassTable& t = kTabAdjoint;
+  if (s->L.Nc > t.rows[t.n - 1]) {
+    set_error("fused path supports at most %d constraint rows per world (got %d)", t.rows[t.n - 1], s->L.Nc);
     return -3;
   }
   const bool bj = s->cfg.preconditioner == OFB_PRECOND_PER_BODY_PAIR;
   // the adjoint runs one world per wavefront: no world is put in the paired class
   hipLaunchKernelGGL(reset_classes_kernel, dim3(1), dim3(64), 0, stream, s->cls_counts);
   hipLaunchKernelGGL(classify_worlds_kernel, dim3((W + 255) / 256), dim3(256), 0, stream, W, s->ccount, s->L.K,
-                     s->L.Nj + s->L.Nctrl, -1, kNumClasses, s->cls_counts, s->cls_lists);
-  for (int c = 1; c < kNumClasses; ++c) {
-    if (!class_used(s, c)) continue;
-    const int R = kClassRHost[c];
+                     s->L.Nj + s->L.Nctrl, class_caps(t, false), s->cls_counts, s->cls_lists);
+  for (int c = 1; c < t.n; ++c) {
+    if (!class_used(s, t, c, false)) continue;
+    const int R = t.code[c];
     P.wlist = s->cls_lists + (size_t)c * W;
     P.wcount = s->cls_counts + c;
     P.rows_cap = WAVE * R;

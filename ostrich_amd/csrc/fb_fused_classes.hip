@@ -1,8 +1,10 @@
 // fb_fused_classes.hip -- instances of the fused kernel for one preconditioner and one
 // set of row classes.  Compiled once per (OFB_BJ, OFB_RSET) pair so that the instances
 // build in parallel; OFB_BJ = 1 selects per-body-pair block-Jacobi, OFB_RSET = 0 the
-// dense classes (code 0: two worlds per wave, code 1: one) and R = 2..5, OFB_RSET = 1 the
-// classes R = 6, 7, 12 and 16.
+// dense classes (code 0: two worlds per wave, code 1: one), R = 2 and, for block-Jacobi,
+// R = 3..5; OFB_RSET = 1 the block-Jacobi classes R = 6, 7, 12 and 16, or the Jacobi
+// worlds of 2, 4 and 8 waves (Jacobi solvers run worlds of more than 128 rows on several
+// waves, block-Jacobi ones on one wave with more row slots per lane).
 
 #include "fb_fused_kernel.h"
 
@@ -15,17 +17,24 @@
 
 namespace ofb {
 
-int OFB_CAT(fused_class_op_bj, OFB_BJ, _set, OFB_RSET)(int R, int op, const FusedParams& P, int blocks, int threads,
+int OFB_CAT(fused_class_op_bj, OFB_BJ, _set, OFB_RSET)(int code, int op, const FusedParams& P, int blocks, int threads,
                                                      size_t lds, hipStream_t st) {
-  switch (R) {
+  switch (code) {
 #define CASE(RR) \
   case RR: return fused_class_op<RR, OFB_BJ != 0, 64, false>(op, P, blocks, threads, lds, st);
 #if OFB_RSET == 0
     case 0: return fused_class_op<1, OFB_BJ != 0, 32, true>(op, P, blocks, threads, lds, st);
     case 1: return fused_class_op<1, OFB_BJ != 0, 64, true>(op, P, blocks, threads, lds, st);
-    CASE(2) CASE(3) CASE(4) CASE(5)
-#else
+    CASE(2)
+#if OFB_BJ != 0
+    CASE(3) CASE(4) CASE(5)
+#endif
+#elif OFB_BJ != 0
     CASE(6) CASE(7) CASE(12) CASE(16)
+#else
+    case kMultiWave + 2: return fused_class_op<2, false, 128, false>(op, P, blocks, threads, lds, st);
+    case kMultiWave + 4: return fused_class_op<2, false, 256, false>(op, P, blocks, threads, lds, st);
+    case kMultiWave + 8: return fused_class_op<2, false, 512, false>(op, P, blocks, threads, lds, st);
 #endif
 #undef CASE
   }
@@ -33,8 +42,8 @@ int OFB_CAT(fused_class_op_bj, OFB_BJ, _set, OFB_RSET)(int R, int op, const Fuse
 }
 
 // Profiling build only: the phase counters [17][10] of the Jacobi instances of this unit
-// (each translation unit holds its own copy: the paired and dense classes and R = 2..5
-// here with OFB_RSET = 0, R = 6..16 with OFB_RSET = 1).
+// (each translation unit holds its own copy: the paired, dense and R = 2 classes with
+// OFB_RSET = 0, the worlds of several waves with OFB_RSET = 1).
 #if defined(OFB_PHASE_TIMING) && OFB_BJ == 0
 extern "C" int OFB_CAT(ofb_debug_phase_cycles, _set, OFB_RSET, )(unsigned long long* out, int reset) {
   if (hipMemcpyFromSymbol(out, HIP_SYMBOL(g_phase_cycles), sizeof(unsigned long long) * 170) != hipSuccess) return -4;

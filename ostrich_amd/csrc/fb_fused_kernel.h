@@ -37,14 +37,14 @@ using namespace rows;
 
 #ifdef OFB_PHASE_TIMING
 // Profiling build only: per-phase cycle counts of the Newton loop, summed over worlds,
-// per instance (row 0: paired, 1: dense, R: matrix-free with R slots per lane); columns
-// 8 and 9 count Newton and PCR iterations.
+// per instance (row 0: paired, 1: dense, R: matrix-free with R slots per lane, 13-15:
+// worlds of 2, 4, 8 waves); columns 8 and 9 count Newton and PCR iterations.
 __device__ unsigned long long g_phase_cycles[17][10];
 #define OFB_PHASE_INIT unsigned long long ph_[8] = {0, 0, 0, 0, 0, 0, 0, 0}; unsigned long long ph_t = clock64();
 #define OFB_PHASE(k) { unsigned long long t_ = clock64(); ph_[k] += t_ - ph_t; ph_t = t_; }
 #define OFB_PHASE_FLUSH                                                                       \
   if (lane == 0) {                                                                            \
-    const int ci_ = DENSE ? (GW == 32 ? 0 : 1) : R;                                           \
+    const int ci_ = DENSE ? (GW == 32 ? 0 : 1) : GW > WAVE ? (GW == 128 ? 13 : GW == 256 ? 14 : 15) : R; \
     for (int k_ = 0; k_ < 8; ++k_) atomicAdd(&g_phase_cycles[ci_][k_], ph_[k_]);              \
     atomicAdd(&g_phase_cycles[ci_][8], (unsigned long long)n_it);                             \
     atomicAdd(&g_phase_cycles[ci_][9], (unsigned long long)*pcr_count);                       \
@@ -58,9 +58,11 @@ __device__ unsigned long long g_phase_cycles[17][10];
 // LDS layout of one world (floats; regions that are read as quads start on 16-byte
 // boundaries): bodies, lambda_prev (rows_cap + 1, compacted row index), one static record
 // per joint, one per contact slot of the class (cs_cap), the block-Jacobi region, the
-// dense Schur operator.  `words` is a multiple of 4, so every world's base stays aligned.
+// dense Schur operator or the lane-private body sums, and for a world of several waves
+// the per-wave body-sum partials and the cross-wave reduction slots.  `words` is a
+// multiple of 4, so every world's base stays aligned.
 struct FusedLayout {
-  int lamprev, js, cs, bj, dense, rs, hist, words;
+  int lamprev, js, cs, bj, dense, rs, part, red, hist, words;
 };
 // Per-Newton-iteration history of a world (||h||^2 and the running PCR count of the first
 // kHistIters iterations) kept in LDS and written to the caller's buffers once per step.
@@ -101,6 +103,62 @@ struct FusedParams {
   const int32_t* wlist;   // worlds of this launch's row class
   const int32_t* wcount;  // number of them
   int rows_cap;           // G R: row capacity of this launch's class
+};
+
+// ---------------------------------------------------------------------------------
+// Sums over the lanes of one world, which every lane of the world holds bit for bit (the
+// stop rules and the PCR scalars are evaluated by each lane).  A world of one wave or half
+// of one reduces inside the wave (group_sum / group_sum2).  A world of NW waves reduces
+// each wave, lane 0 of each wave leaves its total in LDS, and after the workgroup barrier
+// every lane adds the NW totals in wave order.  The totals alternate between two LDS
+// slots: slot s is read right after the barrier of reduction n, and written again only by
+// reduction n + 2, which no wave reaches before every wave has passed the barrier of
+// reduction n + 1, i.e. finished its reads of reduction n.  All lanes of the world must
+// call every reduction (world-uniform control flow).
+// ---------------------------------------------------------------------------------
+template <int GW>
+struct WorldSum {
+  static constexpr int NW = GW > WAVE ? GW / WAVE : 1;
+  float* red;  // 2 slots x NW waves x 2 values (worlds of several waves)
+  int slot;
+
+  __device__ __forceinline__ float sum(float v) {
+    if constexpr (GW <= WAVE) {
+      return group_sum<GW>(v);
+    } else {
+      const float s = wave_sum(v);
+      float* r = red + slot * 2 * NW;
+      if ((threadIdx.x & (WAVE - 1)) == 0) r[threadIdx.x / WAVE] = s;
+      __syncthreads();
+      float t = r[0];
+#pragma unroll
+      for (int w = 1; w < NW; ++w) t = t + r[w];
+      slot ^= 1;
+      return t;
+    }
+  }
+  __device__ __forceinline__ void sum2(float& a, float& b) {
+    if constexpr (GW <= WAVE) {
+      group_sum2<GW>(a, b);
+    } else {
+      group_sum2<WAVE>(a, b);
+      float* r = red + slot * 2 * NW;
+      if ((threadIdx.x & (WAVE - 1)) == 0) {
+        r[threadIdx.x / WAVE] = a;
+        r[NW + threadIdx.x / WAVE] = b;
+      }
+      __syncthreads();
+      float ta = r[0], tb = r[NW];
+#pragma unroll
+      for (int w = 1; w < NW; ++w) {
+        ta = ta + r[w];
+        tb = tb + r[NW + w];
+      }
+      a = ta;
+      b = tb;
+      slot ^= 1;
+    }
+  }
 };
 
 // ---------------------------------------------------------------------------------
@@ -168,8 +226,8 @@ struct BJRegs {
 
 // pair ids (_compute_pair_ids_kernel :62-109), member ranks and sizes (:112-144)
 template <int R, int GW = WAVE>
-__device__ void bj_structure(const BJView& V, int lane, int n_r, int B, const bool (&act)[R], const int (&b0)[R],
-                             const int (&b1)[R], BJRegs<R>& G, int& flags) {
+__device__ __forceinline__ void bj_structure(const BJView& V, int lane, int n_r, int B, const bool (&act)[R],
+                                             const int (&b0)[R], const int (&b1)[R], BJRegs<R>& G, int& flags) {
   int mypid[R], leader[R];
 #pragma unroll
   for (int k = 0; k < R; ++k) {
@@ -223,8 +281,9 @@ __device__ void bj_structure(const BJView& V, int lane, int n_r, int B, const bo
 
 // dense blocks (_extract_pair_blocks_kernel :187-264) and their inverses
 template <int R>
-__device__ void bj_factor(const BJView& V, const float* sm, const int (&b0)[R], const int (&b1)[R],
-                          const sv (&J0)[R], const sv (&J1)[R], const float (&Cr)[R], BJRegs<R>& G, int& flags) {
+__device__ __forceinline__ void bj_factor(const BJView& V, const float* sm, const int (&b0)[R], const int (&b1)[R],
+                                          const sv (&J0)[R], const sv (&J1)[R], const float (&Cr)[R], BJRegs<R>& G,
+                                          int& flags) {
 #pragma unroll
   for (int k = 0; k < R; ++k) {
     G.fail[k] = 0;
@@ -244,16 +303,28 @@ __device__ void bj_factor(const BJView& V, const float* sm, const int (&b0)[R], 
     if (b1[k] >= 0) m1 = apply_minv(sm + b1[k] * BS, J1[k]);
     const int rank = G.rank[k], base = G.slot[k] - rank;
     float* Xr = V.X + G.poff[k] + rank * (rank + 1) / 2;
-    for (int c = 0; c <= rank; ++c) {
-      const float* dc = V.rowd + 16 * (base + c);
-      sv Jc0{mk(dc[0], dc[1], dc[2]), mk(dc[3], dc[4], dc[5])};
-      sv Jc1{mk(dc[6], dc[7], dc[8]), mk(dc[9], dc[10], dc[11])};
-      int c0 = __float_as_int(dc[12]), c1 = __float_as_int(dc[13]);
-      float val = 0.f;
-      if (b0[k] >= 0) { if (c0 == b0[k]) val += sdot(Jc0, m0); else if (c1 == b0[k]) val += sdot(Jc1, m0); }
-      if (b1[k] >= 0) { if (c0 == b1[k]) val += sdot(Jc0, m1); else if (c1 == b1[k]) val += sdot(Jc1, m1); }
-      if (c == rank) val += Cr[k];
-      Xr[c] = val;
+    // columns in batches of four: the batch's member records are read before any entry
+    // is stored, so the LDS reads of a batch overlap
+    for (int c0 = 0; c0 <= rank; c0 += 4) {
+      float4 q[4][4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const float4* dc = reinterpret_cast<const float4*>(V.rowd + 16 * (base + min(c0 + u, rank)));
+        q[u][0] = dc[0]; q[u][1] = dc[1]; q[u][2] = dc[2]; q[u][3] = dc[3];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int c = c0 + u;
+        if (c > rank) break;
+        sv Jc0{mk(q[u][0].x, q[u][0].y, q[u][0].z), mk(q[u][0].w, q[u][1].x, q[u][1].y)};
+        sv Jc1{mk(q[u][1].z, q[u][1].w, q[u][2].x), mk(q[u][2].y, q[u][2].z, q[u][2].w)};
+        int cb0 = __float_as_int(q[u][3].x), cb1 = __float_as_int(q[u][3].y);
+        float val = 0.f;
+        if (b0[k] >= 0) { if (cb0 == b0[k]) val += sdot(Jc0, m0); else if (cb1 == b0[k]) val += sdot(Jc1, m0); }
+        if (b1[k] >= 0) { if (cb0 == b1[k]) val += sdot(Jc0, m1); else if (cb1 == b1[k]) val += sdot(Jc1, m1); }
+        if (c == rank) val += Cr[k];
+        Xr[c] = val;
+      }
     }
   }
   wsync();
@@ -291,7 +362,19 @@ __device__ void bj_factor(const BJView& V, const float* sm, const int (&b0)[R], 
       } else {
         const float lak = colv[k] * inv;
         Xr[piv] = lak;
-        for (int c = piv + 1; c <= rank; ++c) Xr[c] -= lak * (V.colbuf[base + c] * inv);
+        // trailing entries in batches of eight: loads first, then the stores
+        for (int c0 = piv + 1; c0 <= rank; c0 += 8) {
+          float xc[8], cb[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            const int c = min(c0 + u, rank);
+            xc[u] = Xr[c];
+            cb[u] = V.colbuf[base + c];
+          }
+#pragma unroll
+          for (int u = 0; u < 8; ++u)
+            if (c0 + u <= rank) Xr[c0 + u] = xc[u] - lak * (cb[u] * inv);
+        }
       }
     }
     wsync();
@@ -314,8 +397,20 @@ __device__ void bj_factor(const BJView& V, const float* sm, const int (&b0)[R], 
       if (c == a) {
         z = 1.f / laa;
       } else {
+        // the sum in q order; its operands are read eight at a time
         float acc = 0.f;
-        for (int q = c; q < a; ++q) acc += V.colbuf[base + q] * Xp[q * (q + 1) / 2 + c];
+        for (int q0 = c; q0 < a; q0 += 8) {
+          float lb[8], xz[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            const int q = min(q0 + u, a - 1);
+            lb[u] = V.colbuf[base + q];
+            xz[u] = Xp[q * (q + 1) / 2 + c];
+          }
+#pragma unroll
+          for (int u = 0; u < 8; ++u)
+            if (q0 + u < a) acc += lb[u] * xz[u];
+        }
         z = -acc / laa;
       }
       V.X[G.poff[k] + a * (a + 1) / 2 + c] = z;
@@ -328,8 +423,8 @@ __device__ void bj_factor(const BJView& V, const float* sm, const int (&b0)[R], 
 // 324-410, whose two triangular solves become two dense triangular matvecs with the
 // stored L^-1); failed pairs and rows outside any block use the Jacobi diagonal
 template <int R>
-__device__ void bj_apply(const BJView& V, const bool (&act)[R], const float (&Pi)[R], const BJRegs<R>& G,
-                         const float (&v)[R], float (&out)[R]) {
+__device__ __forceinline__ void bj_apply(const BJView& V, const bool (&act)[R], const float (&Pi)[R],
+                                         const BJRegs<R>& G, const float (&v)[R], float (&out)[R]) {
 #pragma unroll
   for (int k = 0; k < R; ++k)
     if (G.slot[k] >= 0 && !G.fail[k]) V.vbuf[G.slot[k]] = v[k];
@@ -341,7 +436,18 @@ __device__ void bj_apply(const BJView& V, const bool (&act)[R], const float (&Pi
     const int rank = G.rank[k], base = G.slot[k] - rank;
     const float* Zr = V.X + G.poff[k] + rank * (rank + 1) / 2;
     float y = 0.f;
-    for (int c = 0; c <= rank; ++c) y += Zr[c] * V.vbuf[base + c];
+    for (int c0 = 0; c0 <= rank; c0 += 8) {  // in c order, operands read eight at a time
+      float zr[8], vb[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int c = min(c0 + u, rank);
+        zr[u] = Zr[c];
+        vb[u] = V.vbuf[base + c];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (c0 + u <= rank) y += zr[u] * vb[u];
+    }
     V.colbuf[G.slot[k]] = y;
   }
   wsync();
@@ -354,7 +460,18 @@ __device__ void bj_apply(const BJView& V, const bool (&act)[R], const float (&Pi
         const int rank = G.rank[k], base = G.slot[k] - rank;
         const float* Xp = V.X + G.poff[k];
         float z = 0.f;
-        for (int c = rank; c < G.m[k]; ++c) z += Xp[c * (c + 1) / 2 + rank] * V.colbuf[base + c];
+        for (int c0 = rank; c0 < G.m[k]; c0 += 8) {  // in c order, operands read eight at a time
+          float xp[8], cb[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            const int c = min(c0 + u, G.m[k] - 1);
+            xp[u] = Xp[c * (c + 1) / 2 + rank];
+            cb[u] = V.colbuf[base + c];
+          }
+#pragma unroll
+          for (int u = 0; u < 8; ++u)
+            if (c0 + u < G.m[k]) z += xp[u] * cb[u];
+        }
         o = z;
       } else {
         o = Pi[k] * v[k];
@@ -376,12 +493,12 @@ __device__ __forceinline__ void schur_matvec(float* sm, int lane, int B, int n_r
 #pragma unroll
   for (int k = 0; k < R; ++k) S.add(BACC, lane + GW * k, act[k], b0[k], b1[k], J0[k] * v[k], J1[k] * v[k]);
   S.end(BACC, true, n_r);
-  wsync();
+  gsync<GW>();
   for (int b = lane; b < B; b += GW) {
     float* bb = sm + b * BS;
     stsv(bb + BVV, apply_minv(bb, ldsv(bb + BACC)));
   }
-  wsync();
+  gsync<GW>();
 #pragma unroll
   for (int k = 0; k < R; ++k) {
     float o = 0.f;
@@ -392,7 +509,7 @@ __device__ __forceinline__ void schur_matvec(float* sm, int lane, int B, int n_r
     }
     out[k] = o;
   }
-  wsync();
+  gsync<GW>();
 }
 
 // ---------------------------------------------------------------------------------
@@ -636,6 +753,8 @@ constexpr int kRsMaxBodies = 10;
 __host__ __device__ inline bool rs_enabled(int B) { return B <= kRsMaxBodies; }
 __host__ __device__ inline int rs_stride(int B) { return 6 * B + 1; }
 __host__ __device__ inline int rs_words(int lanes, int B) { return rs_enabled(B) ? lanes * rs_stride(B) : 0; }
+// floats per wave of the per-wave body sums of a world of several waves
+__host__ __device__ inline int part_stride(int B) { return 6 * B > 64 ? (6 * B + 3) & ~3 : 64; }
 
 template <int GW, bool DENSE>
 struct BodySum {
@@ -643,6 +762,7 @@ struct BodySum {
   float* area;  // lane-private copies, or the staged records of the dense fallback
   int lane, B;
   bool rs;
+  float* part;  // a world of several waves: each wave's sums [NW][part_stride(B)]
 
   // begin a pass that sums into body field `field`: with `zero` the pass's sums replace
   // the field, else they are added to it
@@ -652,9 +772,17 @@ struct BodySum {
       for (int b = 0; b < B; ++b, p += 6) {
         p[0] = 0.f; p[1] = 0.f; p[2] = 0.f; p[3] = 0.f; p[4] = 0.f; p[5] = 0.f;
       }
+    } else if constexpr (!DENSE && GW > WAVE) {
+      // worlds of several waves and more than kRsMaxBodies bodies: each wave adds its
+      // rows into its own accumulators with LDS atomics (a fixed order inside one wave),
+      // and the waves' accumulators are summed in wave order at the end, so the sums do
+      // not depend on the order in which the waves run
+      float* p = part + (lane / WAVE) * part_stride(B);
+      for (int e = lane & (WAVE - 1); e < 6 * B; e += WAVE) p[e] = 0.f;
+      wsync();
     } else if (!DENSE && zero) {
       for (int e = lane; e < 6 * B; e += GW) sm[(e / 6) * BS + field + (e % 6)] = 0.f;
-      wsync();
+      gsync<GW>();
     }
   }
   // contributions c0 to body b0 and c1 to body b1 of row i (none when !on; a body < 0 is
@@ -668,8 +796,15 @@ struct BodySum {
     } else if (DENSE) {
       stage_contrib(area, i, on, b0, b1, c0, c1);
     } else if (on) {
-      if (b0 >= 0) atomic_add_sv(sm + b0 * BS + field, c0);
-      if (b1 >= 0) atomic_add_sv(sm + b1 * BS + field, c1);
+      float* base = sm;
+      int st = BS, off = field;
+      if constexpr (GW > WAVE) {
+        base = part + (lane / WAVE) * part_stride(B);
+        st = 6;
+        off = 0;
+      }
+      if (b0 >= 0) atomic_add_sv(base + b0 * st + off, c0);
+      if (b1 >= 0) atomic_add_sv(base + b1 * st + off, c1);
     }
   }
   // p[0..5] += x on this lane's own words
@@ -682,8 +817,20 @@ struct BodySum {
   // synchronises before reading the field
   __device__ __forceinline__ void end(int field, bool zero, int n_r) const {
     wsync();
-    if (rs) sums(field, zero);
-    else if (DENSE) gather_contrib<GW>(sm, area, lane, n_r, B, field, zero);
+    if (rs) {
+      sums(field, zero);
+    } else if constexpr (DENSE) {
+      gather_contrib<GW>(sm, area, lane, n_r, B, field, zero);
+    } else if constexpr (GW > WAVE) {
+      __syncthreads();
+      for (int e = lane; e < 6 * B; e += GW) {
+        float t = part[e];
+#pragma unroll
+        for (int v = 1; v < GW / WAVE; ++v) t = t + part[v * part_stride(B) + e];
+        float* f = sm + (e / 6) * BS + field + (e % 6);
+        *f = zero ? t : *f + t;
+      }
+    }
   }
 
   // value j of lane t's copy is area[t * stride + j]; nt is 32 or 64: the loads of each
@@ -708,6 +855,29 @@ struct BodySum {
   }
   __device__ __forceinline__ void sums(int field, bool zero) const {
     const int nv = 6 * B;
+    if constexpr (GW > WAVE) {
+      // each wave sums the copies of its own 64 lanes (as a one-wave world does), then
+      // lanes j < 6B add the waves' sums of value j in wave order
+      constexpr int NW = GW / WAVE;
+      const int wl = lane & (WAVE - 1), wv = lane / WAVE;
+      if (nv <= 32) {
+        const int j = wl & 31;
+        float s = j < nv ? copies_sum(j, wv * WAVE + (wl & 32), 32) : 0.f;
+        s = wave_sum_from_halves(s);
+        if (wl < nv) part[wv * part_stride(B) + wl] = s;
+      } else {
+        for (int j = wl; j < nv; j += WAVE) part[wv * part_stride(B) + j] = copies_sum(j, wv * WAVE, WAVE);
+      }
+      __syncthreads();
+      if (lane < nv) {
+        float t = part[lane];
+#pragma unroll
+        for (int v = 1; v < NW; ++v) t = t + part[v * part_stride(B) + lane];
+        float* f = sm + (lane / 6) * BS + field + (lane % 6);
+        *f = zero ? t : *f + t;
+      }
+      return;
+    }
     if (GW == 64 && nv <= 32) {
       // lanes j and j + 32 sum the two halves of the copies of value j; the swap of
       // the halves gives both the same total
@@ -733,7 +903,8 @@ struct BodySum {
 template <int R, int GW, bool DENSE>
 __device__ __forceinline__ float initial_merit(const FusedParams& P, float* sm, float* lamprev, const float* js,
                                                const float* cs, const int (&rj)[R], const int (&rk)[R], int lane,
-                                               int n_r, int n_c, const float (&l)[R], const BodySum<GW, DENSE>& S) {
+                                               int n_r, int n_c, const float (&l)[R], const BodySum<GW, DENSE>& S,
+                                               WorldSum<GW>& WS) {
   const int B = P.m.B;
 #pragma unroll
   for (int k = 0; k < R; ++k) {
@@ -741,7 +912,7 @@ __device__ __forceinline__ float initial_merit(const FusedParams& P, float* sm, 
     if (i < n_r) lamprev[i] = l[k];
   }
   dynamics_bodies(P, sm, lane, false, GW);
-  wsync();
+  gsync<GW>();
   S.begin(BHD, false);
   float s = 0.f;
 #pragma unroll
@@ -754,12 +925,12 @@ __device__ __forceinline__ float initial_merit(const FusedParams& P, float* sm, 
     }
   }
   S.end(BHD, false, n_r);
-  wsync();
+  gsync<GW>();
   for (int e = lane; e < 6 * B; e += GW) {
     float v = sm[(e / 6) * BS + BHD + (e % 6)];
     s += v * v;
   }
-  return group_sum<GW>(s);
+  return WS.sum(s);
 }
 
 // PCR on A x = rhs from x0 = 0 (optim/pcr_solver.py:170-295): the seven vectors live in
@@ -771,8 +942,8 @@ __device__ __forceinline__ int pcr_solve(float* sm, int lane, int B, int n_r, co
                                          const int (&b1)[R], const sv (&J0)[R], const sv (&J1)[R],
                                          const float (&Cr)[R], const float (&Pi)[R], const float (&rhs)[R],
                                          float bsq, float tol_sq, float latol_sq, int max_iters, float (&x)[R],
-                                         const DenseView& D = DenseView{},
-                                         const DenseRow<GW>& Ad = DenseRow<GW>{}) {
+                                         WorldSum<GW>& WS, const DenseView& D = DenseView{},
+                                         const DenseRow<DENSE ? GW : 1>& Ad = DenseRow<DENSE ? GW : 1>{}) {
   static_assert(!DENSE || R == 1, "the dense operator holds one row per lane");
   float r[R], z[R], p[R], Ap[R], y[R], Az[R];
   auto matvec = [&](const float (&v)[R], float (&out)[R]) {
@@ -786,7 +957,7 @@ __device__ __forceinline__ int pcr_solve(float* sm, int lane, int B, int n_r, co
   float zAz = 0.f, rsq = 0.f;
 #pragma unroll
   for (int k = 0; k < R; ++k) { p[k] = z[k]; Ap[k] = Az[k]; zAz += z[k] * Az[k]; rsq += r[k] * r[k]; }
-  group_sum2<GW>(zAz, rsq);
+  WS.sum2(zAz, rsq);
   float target = latol_sq;
   if (bsq * tol_sq > target) target = bsq * tol_sq;
   int it = 0;
@@ -798,7 +969,7 @@ __device__ __forceinline__ int pcr_solve(float* sm, int lane, int B, int n_r, co
     if (BJ) bj_apply<R>(V, act, Pi, G, Ap, y);
 #pragma unroll
     for (int k = 0; k < R; ++k) yAp += y[k] * Ap[k];
-    yAp = group_sum<GW>(yAp);
+    yAp = WS.sum(yAp);
     float a = yAp > 0.f ? zAz_old / yAp : 0.f;
     rsq = 0.f;
 #pragma unroll
@@ -810,7 +981,7 @@ __device__ __forceinline__ int pcr_solve(float* sm, int lane, int B, int n_r, co
     zAz = 0.f;
 #pragma unroll
     for (int k = 0; k < R; ++k) zAz += z[k] * Az[k];
-    group_sum2<GW>(zAz, rsq);  // ||r||^2 of the updated residual rides along
+    WS.sum2(zAz, rsq);  // ||r||^2 of the updated residual rides along
     float bt = zAz_old > 0.f ? zAz / zAz_old : 0.f;
 #pragma unroll
     for (int k = 0; k < R; ++k) { p[k] = z[k] + bt * p[k]; Ap[k] = Az[k] + bt * Ap[k]; }
@@ -820,9 +991,10 @@ __device__ __forceinline__ int pcr_solve(float* sm, int lane, int B, int n_r, co
   return it;
 }
 
-__host__ __device__ inline FusedLayout fused_layout(int B, int J, int rows_cap, int cs_cap, bool bj, bool dense) {
+// lanes: lanes per world (32, 64, or a multiple of 64 for a world of several waves)
+__host__ __device__ inline FusedLayout fused_layout(int B, int J, int rows_cap, int cs_cap, bool bj, bool dense,
+                                                    int lanes) {
   FusedLayout l;
-  const int lanes = rows_cap < WAVE ? rows_cap : WAVE;  // lanes per world
   l.lamprev = B * BS;
   l.js = up4(l.lamprev + rows_cap + 1);
   l.cs = l.js + JS_WORDS * J;
@@ -835,9 +1007,13 @@ __host__ __device__ inline FusedLayout fused_layout(int B, int J, int rows_cap, 
     const int dw = dense_words(rows_cap, B);
     l.rs = l.dense;
     l.hist = up4(l.dense + (dw > rs ? dw : rs));
+    l.part = l.red = l.hist;
   } else {
     l.rs = l.dense;
-    l.hist = up4(l.dense + rs);
+    const int nw = lanes > WAVE ? lanes / WAVE : 0;  // waves of a multi-wave world
+    l.part = up4(l.dense + rs);
+    l.red = l.part + nw * part_stride(B);
+    l.hist = up4(l.red + 4 * nw);
   }
   l.words = l.hist + kHistWords + 4;  // + the world's running PCR count
   return l;
@@ -851,7 +1027,11 @@ __host__ __device__ inline FusedLayout fused_layout(int B, int J, int rows_cap, 
 // operator is the assembled Schur matrix (R = 1 only, a matrix row of GW floats per
 // lane).  The dense instances hold the matrix row in VGPRs and are allowed 256 of them
 // (two waves per SIMD, no spills); a matrix-free R = 1 instance would be held to 128 so
-// that four waves share a SIMD.
+// that four waves share a SIMD.  GW = 128, 256, 512: one world spread over GW / 64 waves
+// of its own workgroup (matrix-free, Jacobi), whose lanes synchronise with the
+// workgroup barrier and reduce across waves through LDS: the large worlds' Newton solves
+// take fewer dependent steps per lane than on one wave with many row slots, and are the
+// ones that set a step's duration.
 // ------------------------------------------------------------------------------------
 #ifndef OFB_R1_WAVES_PER_EU
 #define OFB_R1_WAVES_PER_EU 4
@@ -865,14 +1045,19 @@ __host__ __device__ inline FusedLayout fused_layout(int B, int J, int rows_cap, 
 #ifndef OFB_PAIR_WAVES_PER_EU
 #define OFB_PAIR_WAVES_PER_EU 3
 #endif
+#ifndef OFB_MW_WAVES_PER_EU
+#define OFB_MW_WAVES_PER_EU 2
+#endif
 template <int R, bool BJ, int GW, bool DENSE>
-__global__ void __launch_bounds__(256)
-__attribute__((amdgpu_waves_per_eu(R == 1 ? (DENSE ? (GW == 32 && !BJ ? OFB_PAIR_WAVES_PER_EU : OFB_DENSE_WAVES_PER_EU)
-                                                : OFB_R1_WAVES_PER_EU)
-                                         : 1)))
+__global__ void __launch_bounds__(GW > 256 ? GW : 256)
+__attribute__((amdgpu_waves_per_eu(GW > WAVE ? OFB_MW_WAVES_PER_EU
+                                   : R == 1 ? (DENSE ? (GW == 32 && !BJ ? OFB_PAIR_WAVES_PER_EU : OFB_DENSE_WAVES_PER_EU)
+                                                     : OFB_R1_WAVES_PER_EU)
+                                            : 1)))
 fused_step_kernel(FusedParams P) {
   extern __shared__ __align__(16) float lds[];
-  static_assert(GW == 64 || (GW == 32 && R == 1), "two worlds per wave hold one row slot per lane");
+  static_assert(GW == 64 || (GW == 32 && R == 1) || ((GW == 128 || GW == 256 || GW == 512) && !DENSE && !BJ),
+                "two worlds per wave hold one row slot per lane; worlds of several waves are matrix-free, Jacobi");
   const int lane = threadIdx.x & (GW - 1);
   const int grp = threadIdx.x / GW;
   const int idx = blockIdx.x * (blockDim.x / GW) + grp;
@@ -881,7 +1066,7 @@ fused_step_kernel(FusedParams P) {
   // issue priority of the wave on its SIMD: the classes of larger worlds, whose Newton
   // solves are the longest and set the step's duration, ahead of the small ones that
   // fill the gaps
-  __builtin_amdgcn_s_setprio(DENSE ? (GW == 32 ? 0 : 1) : (R >= 3 ? 3 : 2));
+  __builtin_amdgcn_s_setprio(DENSE ? (GW == 32 ? 0 : 1) : (R >= 3 || GW > WAVE ? 3 : 2));
   float* sm = lds + grp * P.lay.words;
   const int B = P.m.B;
   const RowLayout& L = P.L;
@@ -915,11 +1100,12 @@ fused_step_kernel(FusedParams P) {
   const int n_c = min(P.ccount[w], L.K);
   const int n_r = L.Nj + L.Nctrl + 3 * n_c;
   const DenseView D = dense_view<GW>(sm + P.lay.dense, n_r, B);
-  DenseRow<GW> Adense;  // row of the assembled Schur matrix (DENSE)
+  DenseRow<DENSE ? GW : 1> Adense;  // row of the assembled Schur matrix (DENSE)
   // row -> body sums: lane-private copies (worlds of at most kRsMaxBodies bodies), else
   // staged records (DENSE) or LDS atomics
-  const BodySum<GW, DENSE> S{sm, sm + P.lay.rs, lane, B, rs_enabled(B)};
-  wsync();
+  const BodySum<GW, DENSE> S{sm, sm + P.lay.rs, lane, B, rs_enabled(B), sm + P.lay.part};
+  WorldSum<GW> WS{sm + P.lay.red, 0};
+  gsync<GW>();
   // ---- step-constant row data (model, control, contacts, Jacobians at the step-start
   // poses) into LDS, once per step ----
   for (int j = lane; j < P.m.J; j += GW) store_joint_static(js + JS_WORDS * j, joint_static(P, w, j));
@@ -944,7 +1130,7 @@ fused_step_kernel(FusedParams P) {
     if (i < L.Nj) { rj[k] = P.topo[i]; rk[k] = P.topo[L.Nj + i]; }
     else if (i < L.Nj + L.Nctrl) rj[k] = P.topo[2 * L.Nj + (i - L.Nj)];
   }
-  wsync();
+  gsync<GW>();
   // learned warm start: keep whichever seed starts from the lower residual, so the
   // network never replaces a better previous-step seed
   int seed_flag = 0;
@@ -955,14 +1141,14 @@ fused_step_kernel(FusedParams P) {
       int i = lane + GW * k;
       alt[k] = i < n_r ? P.lambda_seed2[(size_t)w * L.Nc + original_row(L, i, n_c)] : 0.f;
     }
-    const float ma = initial_merit<R, GW, DENSE>(P, sm, lamprev, js, cs, rj, rk, lane, n_r, n_c, lam, S);
-    const float mb = initial_merit<R, GW, DENSE>(P, sm, lamprev, js, cs, rj, rk, lane, n_r, n_c, alt, S);
+    const float ma = initial_merit<R, GW, DENSE>(P, sm, lamprev, js, cs, rj, rk, lane, n_r, n_c, lam, S, WS);
+    const float mb = initial_merit<R, GW, DENSE>(P, sm, lamprev, js, cs, rj, rk, lane, n_r, n_c, alt, S, WS);
     if (mb < ma) {
       seed_flag = FLAG_LEARNED_SEED;
 #pragma unroll
       for (int k = 0; k < R; ++k) { lam[k] = alt[k]; lamb[k] = alt[k]; }
     }
-    wsync();
+    gsync<GW>();
   }
 
   const float atol_sq = P.cfg.nr_atol * P.cfg.nr_atol;
@@ -990,7 +1176,7 @@ fused_step_kernel(FusedParams P) {
       if (i < n_r) lamprev[i] = lam[k];
     }
     dynamics_bodies(P, sm, lane, true, GW);
-    wsync();
+    gsync<GW>();
     OFB_PHASE(0)
     // ---- assembly (linear_system.py:140-322) ----
     S.begin(BHD, false);
@@ -1022,14 +1208,14 @@ fused_step_kernel(FusedParams P) {
       }
     }
     S.end(BHD, false, n_r);
-    wsync();
+    gsync<GW>();
     OFB_PHASE(1)
     // M^-1 h_d per body
     for (int b = lane; b < B; b += GW) {
       float* bb = sm + b * BS;
       stsv(bb + BVV, apply_minv(bb, ldsv(bb + BHD)));
     }
-    wsync();
+    gsync<GW>();
     // Schur RHS (linear_system.py:17-65) and Jacobi diagonal (preconditioner.py:9-50)
     float bsq = 0.f;
     sv W0[R], W1[R];  // M^-1 J^T of each row's two sides
@@ -1056,7 +1242,7 @@ fused_step_kernel(FusedParams P) {
       }
       bsq += rhs[k] * rhs[k];
     }
-    bsq = group_sum<GW>(bsq);
+    bsq = WS.sum(bsq);
     if (BJ) {
       if (n_it == 0) bj_structure<R, GW>(V, lane, n_r, B, act, b0, b1, G, bj_flags);
       bj_factor<R>(V, sm, b0, b1, J0, J1, Cr, G, bj_flags);
@@ -1066,7 +1252,7 @@ fused_step_kernel(FusedParams P) {
     if constexpr (DENSE) dense_build(D, Adense, lane, B, act[0], b0[0], b1[0], J0[0], J1[0], W0[0], W1[0], Cr[0]);
     OFB_PHASE(7)
     const int pcr_it = pcr_solve<R, BJ, GW, DENSE>(sm, lane, B, n_r, S, V, G, act, b0, b1, J0, J1, Cr, Pi, rhs,
-                                                   bsq, tol_sq, latol_sq, P.cfg.lin_max_iters, x, D, Adense);
+                                                   bsq, tol_sq, latol_sq, P.cfg.lin_max_iters, x, WS, D, Adense);
     if (lane == 0) *pcr_count += pcr_it;
     OFB_PHASE(3)
 
@@ -1078,7 +1264,7 @@ fused_step_kernel(FusedParams P) {
       if (P.ls_count == 0) lam[k] = lam[k] + x[k];
     }
     S.end(BACC, true, n_r);
-    wsync();
+    gsync<GW>();
     if (P.ls_count == 0) {
       for (int b = lane; b < B; b += GW) {
         float* bb = sm + b * BS;
@@ -1089,7 +1275,7 @@ fused_step_kernel(FusedParams P) {
         stsv(bb + BU, u);
         sttf(bb + BQ, integrate(u, ldtf(bb + BQP), ld3(bb + BCOM), dt));
       }
-      wsync();
+      gsync<GW>();
     } else {
       // merit line search (core/linesearch.py:330-339): every candidate alpha gives
       // u = u0 + alpha du, q = integrate(u, q_prev), lambda = lambda0 + alpha dlambda, with
@@ -1101,7 +1287,7 @@ fused_step_kernel(FusedParams P) {
         stsv(bb + BVV, apply_minv(bb, sv{acc.a * dt - hd.a, acc.b * dt - hd.b}));
         stsv(bb + BU0, ldsv(bb + BU));
       }
-      wsync();
+      gsync<GW>();
       float best_a = 0.f, best_m = 0.f;
       for (int si = 0; si < P.ls_count; ++si) {
         const float a = P.ls_steps[si];
@@ -1112,12 +1298,12 @@ fused_step_kernel(FusedParams P) {
           stsv(bb + BU, u);
           sttf(bb + BQ, integrate(u, ldtf(bb + BQP), ld3(bb + BCOM), dt));
         }
-        wsync();
+        gsync<GW>();
         float la[R];
 #pragma unroll
         for (int k = 0; k < R; ++k) la[k] = lam[k] + a * x[k];
         dynamics_bodies(P, sm, lane, false, GW);
-        wsync();
+        gsync<GW>();
         S.begin(BHD, false);
         float ms = 0.f;
 #pragma unroll
@@ -1130,14 +1316,14 @@ fused_step_kernel(FusedParams P) {
           }
         }
         S.end(BHD, false, n_r);
-        wsync();
+        gsync<GW>();
         for (int e = lane; e < 6 * B; e += GW) {
           float v = sm[(e / 6) * BS + BHD + (e % 6)];
           ms += v * v;
         }
-        ms = group_sum<GW>(ms);
+        ms = WS.sum(ms);
         if (si == 0 || ms < best_m) { best_m = ms; best_a = a; }
-        wsync();
+        gsync<GW>();
       }
 #pragma unroll
       for (int k = 0; k < R; ++k) lam[k] += best_a * x[k];
@@ -1148,13 +1334,13 @@ fused_step_kernel(FusedParams P) {
         stsv(bb + BU, u);
         sttf(bb + BQ, integrate(u, ldtf(bb + BQP), ld3(bb + BCOM), dt));
       }
-      wsync();
+      gsync<GW>();
     }
 
     OFB_PHASE(4)
     // ---- residual at the new iterate (core/residual.py:18-173) ----
     dynamics_bodies(P, sm, lane, false, GW);
-    wsync();
+    gsync<GW>();
     S.begin(BHD, false);
     float msum = 0.f;
 #pragma unroll
@@ -1169,12 +1355,12 @@ fused_step_kernel(FusedParams P) {
     }
     rvalid = true;
     S.end(BHD, false, n_r);
-    wsync();
+    gsync<GW>();
     for (int e = lane; e < 6 * B; e += GW) {
       float v = sm[(e / 6) * BS + BHD + (e % 6)];
       msum += v * v;
     }
-    merit = group_sum<GW>(msum);
+    merit = WS.sum(msum);
     if (P.hist_merit && lane == 0 && n_it < kHistIters) {  // ||h||^2 and the PCR iterations so far
       sm[P.lay.hist + n_it] = merit;
       sm[P.lay.hist + kHistIters + n_it] = __int_as_float(*pcr_count);
@@ -1203,9 +1389,9 @@ fused_step_kernel(FusedParams P) {
     if (n_it >= P.cfg.nr_max_iters) keep = false;
     OFB_PHASE(6)
     if (!keep) break;
-    wsync();
+    gsync<GW>();
   }
-  wsync();
+  gsync<GW>();
   OFB_PHASE_FLUSH
   // ---- pick the candidate ----
   int picked = n_it - 1;
@@ -1222,7 +1408,7 @@ fused_step_kernel(FusedParams P) {
 #pragma unroll
     for (int k = 0; k < R; ++k) lam[k] = lamb[k];
   }
-  wsync();
+  gsync<GW>();
   // ---- outputs (engine.py:68-69) ----
   for (int e = lane; e < 13 * B; e += GW) {
     int b = e / 13, c = e % 13;
@@ -1255,7 +1441,7 @@ fused_step_kernel(FusedParams P) {
     }
   }
   if (P.hist_merit) {  // NaN / -1: iterations the world did not run (or past kHistIters)
-    wsync();
+    gsync<GW>();
     const float* hs = sm + P.lay.hist;
     for (int k = lane; k < P.hist_cap; k += GW) {
       const bool ran = k < n_it && k < kHistIters;
@@ -1265,8 +1451,10 @@ fused_step_kernel(FusedParams P) {
     }
   }
   int bj_any = 0;
-  if (group_any<GW>(bj_flags & FLAG_FACTOR_FAILURE)) bj_any |= FLAG_FACTOR_FAILURE;
-  if (group_any<GW>(bj_flags & FLAG_PAIR_OVERFLOW)) bj_any |= FLAG_PAIR_OVERFLOW;
+  if constexpr (BJ) {
+    if (group_any<GW>(bj_flags & FLAG_FACTOR_FAILURE)) bj_any |= FLAG_FACTOR_FAILURE;
+    if (group_any<GW>(bj_flags & FLAG_PAIR_OVERFLOW)) bj_any |= FLAG_PAIR_OVERFLOW;
+  }
   if (lane == 0) {
     P.st_nr[w] = n_it;
     P.st_pcr[w] = *pcr_count;
@@ -1277,6 +1465,10 @@ fused_step_kernel(FusedParams P) {
     P.st_flags[w] = fl;
   }
 }
+
+// Class code of the instance for worlds spread over NW waves, two row slots per lane:
+// kMultiWave + NW (NW = 2, 4, 8).
+constexpr int kMultiWave = 100;
 
 // Host side of one class instance: op 0 launches it, op 1 raises its dynamic-LDS limit
 // to `lds` bytes.  Instances are compiled in several translation units
@@ -1294,7 +1486,8 @@ int fused_class_op(int op, const FusedParams& P, int blocks, int threads, size_t
 // return -1 when the class code is not in that unit's set.  Class code 0 is the
 // two-worlds-per-wave dense instance (at most 32 rows), code 1 the one-world-per-wave
 // dense instance (at most 64 rows), code R >= 2 the one-world-per-wave matrix-free
-// instance with R row slots per lane.
+// instance with R row slots per lane, code kMultiWave + NW the instance whose world spans
+// NW waves (Jacobi only).
 int fused_class_op_bj0_set0(int R, int op, const FusedParams& P, int blocks, int threads, size_t lds, hipStream_t st);
 int fused_class_op_bj0_set1(int R, int op, const FusedParams& P, int blocks, int threads, size_t lds, hipStream_t st);
 int fused_class_op_bj1_set0(int R, int op, const FusedParams& P, int blocks, int threads, size_t lds, hipStream_t st);

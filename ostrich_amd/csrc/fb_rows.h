@@ -48,6 +48,15 @@ __device__ __forceinline__ void wsync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Synchronises the lanes of one world: the wave barrier for a world of one wave (or half
+// of one), the workgroup barrier for a world that spans several waves (such a world has
+// its workgroup to itself).
+template <int GW>
+__device__ __forceinline__ void gsync() {
+  if constexpr (GW <= WAVE) wsync();
+  else __syncthreads();
+}
+
 #define OFB_DPP(v, ctrl, rmask)                                                                  \
   __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), (ctrl), (rmask), 0xF, false))
 

@@ -94,8 +94,8 @@ struct ofb_solver {
   int cls_wpb[16];   // worlds per workgroup of each class launch (LDS-limited)
   // the one-world-per-wave classes run on side streams beside the paired class (the
   // dense class, the R = 2 class and the larger classes one after another)
-  hipStream_t side_stream[3];
-  hipEvent_t ev_fork, ev_join[3];
+  hipStream_t side_stream[8];
+  hipEvent_t ev_fork, ev_join[8];
   int side_ready;  // side streams and join events created
   // cross-step contact warm start: snapshot of the previous step (fb_warmstart.hip)
   int32_t *ws_count, *ws_b0, *ws_b1;
@@ -130,6 +130,8 @@ int fused_classes_alloc(ofb_solver* s);
 int fused_set_launch(ofb_solver* s, int waves_per_block, int pair_small_worlds);
 int fused_class_times(const ofb_solver* s, int capacity, int32_t* rows_cap, int32_t* worlds, float* ms,
                       int32_t* count);
+int fused_class_shapes(const ofb_solver* s, int capacity, int32_t* slots, int32_t* lanes, int32_t* max_rows,
+                       int32_t* count);
 void fused_classes_free(ofb_solver* s);
 int fused_adjoint(ofb_solver* s, const ofb_state_in* in, const ofb_state_out* out, const ofb_control* ctl,
                   const float* lambda, const float* lambda_prev, const ofb_adjoint_in* gin,

@@ -280,6 +280,20 @@ class OstrichEngine:
         k = n.value
         return dict(rows_cap=rows[:k].tolist(), worlds=worlds[:k].tolist(), ms=[float(x) for x in ms[:k]])
 
+    def class_shapes(self):
+        """Per row class: dict(slots, lanes, max_rows) lists, the row slots per lane and the
+        lanes per world of the kernel instance the class runs (32: two worlds per
+        wavefront, 64: one, 128 / 256 / 512: one world over 2 / 4 / 8 wavefronts), and the
+        largest row count of a world in the class (0: not launched).  A world of n rows is
+        in the first class whose max_rows is >= n."""
+        import numpy as np
+        slots, lanes, max_rows = (np.zeros(16, np.int32) for _ in range(3))
+        n = C.c_int32()
+        _lib.check(self.lib.ofb_get_class_shapes(self._h, 16, slots.ctypes.data, lanes.ctypes.data,
+                                                 max_rows.ctypes.data, C.byref(n)), "ofb_get_class_shapes")
+        k = n.value
+        return dict(slots=slots[:k].tolist(), lanes=lanes[:k].tolist(), max_rows=max_rows[:k].tolist())
+
     def fused_launch(self):
         """(waves per workgroup, pair_small_worlds) of the fused path."""
         w, p = C.c_int32(), C.c_int32()

@@ -590,11 +590,11 @@ def test_stribeck_rollout_parity(oracle_mod, gpu_device):
 
 @pytest.mark.parametrize("K", [64, 80, 110, 128, 240, 256])
 def test_every_large_class_instance_matches_oracle(oracle_mod, gpu_device, K):
-    """Worlds of more than 128 rows run in the instance of the solver's largest row class,
-    which the contact slots K set: 4, 5, 6, 7, 12 or 16 rows per lane for K = 64, 80, 110,
-    128, 240, 256.  The uneven skid-steer worlds (up to ~75 contacts) after two Newton
-    iterations from identical inputs: every world within four times its FP32-vs-FP64
-    spread plus 1e-4 of scale, identical Newton counts."""
+    """Worlds of more than 128 rows run on several wavefronts (129-256 rows on two, 257-512
+    on four), whatever the contact slots K (64 ... 256) make the solver's largest world.
+    The uneven skid-steer worlds (up to ~75 contacts) after two Newton iterations from
+    identical inputs: every world within four times its FP32-vs-FP64 spread plus 1e-4 of
+    scale, identical Newton counts."""
     from ostrich_amd import OstrichEngine
     from ostrich_amd.config import NewtonRaphsonConfig
     W = 64
