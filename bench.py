@@ -298,6 +298,49 @@ def scene_leg(name, model_fn, worlds, K, steps, warmup, device, variants=("fused
     return out
 
 
+def learned_warm_start_leg(device, skid_result=None, skid_prev=None, stack_worlds=64, max_cubes=32, steps=10,
+                           warmup=3):
+    """The learned warm start on its evaluation batch: 2048 box-stack worlds of 1-32 cubes
+    (64 worlds per cube count, seeds 3000 + n; the network was trained on skid-steer and
+    tracked-flipper rollouts only)
+    and the 4096 skid-steer worlds of the headline (the fused:jacobi:learned and
+    fused:jacobi:prev variants above).  Per cube count the same closed-loop steps run with
+    the previous-step seed and with the network gated per world; reported: Newton
+    iterations per world-step and world-steps/s both ways, over the stacks and over the
+    whole batch."""
+    rows = []
+    for n in range(1, max_cubes + 1):
+        model, state, control = scenes.box_stack(stack_worlds, n_cubes=n, seed=3000 + n)
+        K = max(8, 4 * n + 4)
+        col = collide.HipCollider(model, device=device, gap=0.02, max_contacts_per_world=K)
+        row = dict(cubes=n)
+        for v in ("fused:jacobi:prev", "fused:jacobi:learned"):
+            r = run_path(v, model, state, control, steps, warmup, device, col, K=K)
+            row[v.split(":")[2]] = dict(nr_mean=r["nr_mean"], ms_per_step=r["ms_per_step"],
+                                        converged_fraction=r["converged_fraction"])
+        rows.append(row)
+
+    def total(rs, key, extra=None):
+        ws = stack_worlds * steps * len(rs)
+        nr = sum(stack_worlds * steps * r[key]["nr_mean"] for r in rs)
+        ms = sum(steps * r[key]["ms_per_step"] for r in rs)
+        if extra:
+            ws += extra["worlds"] * extra["steps"]
+            nr += extra["worlds"] * extra["steps"] * extra["nr_mean"]
+            ms += extra["steps"] * extra["ms_per_step"]
+        return dict(world_steps=ws, nr_mean=nr / ws, world_steps_per_s=ws / (ms / 1e3))
+    out = dict(stack_worlds=stack_worlds, cube_counts=f"1-{max_cubes}", steps=steps, warmup=warmup,
+               per_cube_count=rows, stacks=dict(prev=total(rows, "prev"), learned=total(rows, "learned")))
+    if skid_result and skid_prev and "nr_mean" in skid_result and "nr_mean" in skid_prev:
+        sk = lambda r: dict(worlds=4096, steps=r["steps"], nr_mean=r["nr_mean"], ms_per_step=r["ms_per_step"])
+        out["whole_batch"] = dict(prev=total(rows, "prev", sk(skid_prev)),
+                                  learned=total(rows, "learned", sk(skid_result)))
+        out["whole_batch"]["nr_change"] = out["whole_batch"]["learned"]["nr_mean"] / \
+            out["whole_batch"]["prev"]["nr_mean"] - 1.0
+    out["stacks"]["nr_change"] = out["stacks"]["learned"]["nr_mean"] / out["stacks"]["prev"]["nr_mean"] - 1.0
+    return out
+
+
 def adjoint_leg(device, worlds=1024, steps=50, checkpoint_every=10, seed=3):
     """Differentiable 50-step rollout of skid-steer robots on a flat plane (mu 0.8, wheel
     velocity targets U(-3,3) per step), loss on the final chassis x, y and yaw, gradients
@@ -726,6 +769,12 @@ def main():
         out["box_stack"] = scene_leg("box_stack", lambda n: scenes.box_stack(n, 3, seed=0), 16, 16, sc_steps,
                                      args.warmup, device, ("fused:jacobi:prev", "fused:jacobi:cold",
                                                            "plain:jacobi:cold"))
+    if not args.no_scenes and "fused:jacobi:learned" in results:
+        print("[bench] learned warm start on its evaluation batch", file=sys.stderr, flush=True)
+        out["learned_warm_start"] = learned_warm_start_leg(device, results.get("fused:jacobi:learned"),
+                                                           results.get("fused:jacobi:prev"))
+        print(f"[bench] learned_warm_start: stacks {out['learned_warm_start']['stacks']}, whole batch "
+              f"{out['learned_warm_start'].get('whole_batch')}", file=sys.stderr, flush=True)
     if not args.no_adjoint:
         print("[bench] adjoint rollout", file=sys.stderr, flush=True)
         out["adjoint"] = adjoint_leg(device)

@@ -86,7 +86,7 @@ def test_chain_matches_oracle_per_step(gpu_device, oracle_mod):
             scale = max(np.abs(o64[k]).max(), 1e-6)
             err = np.abs(gk - o32[k]).max()
             spread = np.abs(o32[k] - o64[k]).max()
-            assert err <= 4 * spread + 2e-3 * scale, f"step {t} {k}: {err:.3e} (spread {spread:.3e}, scale {scale:.3e})"
+            assert err <= 4 * spread + 1e-3 * scale, f"step {t} {k}: {err:.3e} (spread {spread:.3e}, scale {scale:.3e})"
         gq, gu = g["body_q"], g["body_qd"]
 
 

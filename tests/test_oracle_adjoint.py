@@ -10,8 +10,9 @@ step_backward, before it is trusted.
   adjoint's own (w_lambda, w_u).
 * The explicit pose carry equals the transposed Jacobian of the unnormalised quaternion
   update q + dt/2 Q(q) w, formed here by finite differences.
-* A zero seed gives zero gradients; the FP32 adjoint (pose differences with step 2e-3)
-  agrees with the FP64 one.
+* A zero seed gives zero gradients; the FP32 adjoint (pose differences of the default
+  step) agrees with the FP64 one, and that step's truncation is bounded against FP64
+  differences of step 1e-6.
 """
 import copy
 
@@ -219,12 +220,46 @@ def test_zero_seed_gives_zero_gradients(oracle_mod):
 
 
 def test_fp32_adjoint_close_to_fp64(oracle_mod):
-    """The FP32 restatement with pose differences of step 2e-3 (the GPU's arithmetic)
-    stays within 1e-4 of the FP64 gradients' scale."""
+    """The FP32 restatement with pose differences of the default step (the GPU's
+    arithmetic) stays within 1e-4 of the FP64 gradients' scale."""
     model, st, control, c, cfg = _skid_state(oracle_mod, worlds=3)
     o, out, a_q, a_u, g64 = _forward_and_adjoint(oracle_mod, model, cfg, st, control, c)
     g32 = oracle_mod.adjoint(model, cfg, st, out, control, c, DT, o["lam"], o["lam_prev"], a_q, a_u, precision=32,
-                             fd_eps=2e-3)
+                             fd_eps=cfg.adjoint.pose_fd_eps)
     for k in ("body_q", "body_qd", "body_f", "joint_target_vel"):
         scale = np.abs(g64[k]).max()
         assert np.abs(g32[k] - g64[k]).max() <= 1e-4 * scale, k
+
+
+@pytest.mark.parametrize("scene,bound", [("skid_steer_clear", 1e-5), ("skid_steer_obstacles", 1e-4),
+                                         ("tracked_flipper", 5e-4)])
+def test_pose_difference_step_truncation(oracle_mod, scene, bound):
+    """The pose pull-back w^T dR/dq by central differences of the default step (5e-4, the
+    GPU's), against the same differences at 1e-6, both in FP64: the truncation alone on
+    skid-steer robots on open ground and among obstacles, and on tracked flipper robots
+    on a heightfield (rotating bodies, position-controlled joints, anisotropic track
+    friction, contacts near the kink of the FB function, where the truncation constant
+    is large).  Measured: 3e-7, 1e-5 and 2.5e-4 of the gradient's scale (at 2e-3 it was
+    5e-6, 1.4e-4 and 4.6e-3)."""
+    if scene.startswith("skid_steer"):
+        model, st, control, c, cfg = _skid_state(oracle_mod, worlds=3,
+                                                 layout="clear" if scene.endswith("clear") else "cluttered")
+    else:
+        model, state, control = scenes.tracked_flipper(2, seed=5)
+        K = 64
+        col = collide.PrimitiveCollider(model, gap=0.05, max_contacts_per_world=K, hf_contacts="cells")
+        st = {k: np.asarray(v, np.float64) for k, v in state.items()}
+        cfg = config(K=K)
+        for _ in range(3):
+            c = col.collide(torch.as_tensor(st["body_q"], dtype=torch.float32))
+            o = oracle_mod.step(model, cfg, st, control, c, DT, precision=64)
+            st = dict(st, body_q=o["body_q"], body_qd=o["body_qd"])
+        c = col.collide(torch.as_tensor(st["body_q"], dtype=torch.float32))
+    o, out, a_q, a_u, g_exact = _forward_and_adjoint(oracle_mod, model, cfg, st, control, c, eps=1e-6)
+    g_step = oracle_mod.adjoint(model, cfg, st, out, control, c, DT, o["lam"], o["lam_prev"], a_q, a_u,
+                                precision=64, fd_eps=cfg.adjoint.pose_fd_eps)
+    for k in ("body_q", "body_qd", "body_f", "joint_target_pos", "joint_target_vel"):
+        scale = np.abs(g_exact[k]).max()
+        err = np.abs(g_step[k] - g_exact[k]).max()
+        print(f"[truncation {scene}] {k}: {err:.2e} of scale {scale:.2e}")
+        assert err <= bound * scale, (k, err, scale)
