@@ -14,7 +14,8 @@ solve's truncation from the Newton iteration itself:
                PCR <= 26 at 1e-3, compliances 6e-10 / 1e-10 / 1e-8)
 
 Prints one markdown table row per (scene, setting): Newton and PCR iterations, converged
-fraction (||h|| <= 1e-5 at the picked iterate) and ||h|| of the capped world-steps.
+fraction (||h|| <= 1e-5 at the picked iterate), the fraction whose picked ||h|| is not
+finite (the Newton iteration diverged) and ||h|| of the other capped world-steps.
 
     python -m oracle.newton_cap_study [--threads 8]
 """
@@ -82,9 +83,12 @@ def run(scene, worlds, steps, name, skip=3):
             merit.append(o["merit"])
     nr, pcr, merit = (np.concatenate(a) for a in (nr, pcr, merit))
     conv = merit <= 1e-10
-    h = np.sqrt(merit[~conv]) if (~conv).any() else np.array([np.nan])
+    nonfinite = ~np.isfinite(merit)
+    capped = ~conv & ~nonfinite
+    h = np.sqrt(merit[capped]) if capped.any() else np.array([np.nan])
     return dict(scene=scene, setting=name, nr=nr.mean(), nr_p95=np.percentile(nr, 95), pcr_per_nr=pcr.sum() / nr.sum(),
-                converged=conv.mean(), h_med=np.median(h), h_p90=np.percentile(h, 90), secs=time.time() - t0)
+                converged=conv.mean(), nonfinite=nonfinite.mean(), h_med=np.median(h), h_p90=np.percentile(h, 90),
+                secs=time.time() - t0)
 
 
 def main():
@@ -93,14 +97,16 @@ def main():
     args = ap.parse_args()
     from oracle import oracle
     oracle.set_threads(args.threads)
-    print("| scene | setting | NR/step | NR p95 | PCR/NR | converged | capped ||h|| median | capped ||h|| p90 |")
-    print("|---|---|---|---|---|---|---|---|")
+    print("| scene | setting | NR/step | NR p95 | PCR/NR | converged | non-finite ||h|| | capped ||h|| median | "
+          "capped ||h|| p90 |")
+    print("|---|---|---|---|---|---|---|---|---|")
     plan = [("skid_steer", 128, 40, n) for n in settings()] + \
            [("tracked_flipper", 32, 30, n) for n in ("bench", "pcr200", "pcr64", "refyaml")]
     for scene, W, steps, name in plan:
         r = run(scene, W, steps, name)
         print(f"| {scene} ({W} worlds, {steps} steps) | {name} | {r['nr']:.2f} | {r['nr_p95']:.0f} | "
-              f"{r['pcr_per_nr']:.1f} | {r['converged']:.3f} | {r['h_med']:.1e} | {r['h_p90']:.1e} |", flush=True)
+              f"{r['pcr_per_nr']:.1f} | {r['converged']:.3f} | {r['nonfinite']:.3f} | {r['h_med']:.1e} | "
+              f"{r['h_p90']:.1e} |", flush=True)
 
 
 if __name__ == "__main__":

@@ -369,6 +369,9 @@ class OstrichEngineConfig:
         c.adjoint_pose_eps = self.adjoint.pose_fd_eps
         if self.warm_start.method != "position_match" and self.warm_start.enabled and self.warm_start.seed_iterate:
             raise NotImplementedError("warm_start.method='pair_aggregate' is not built; use 'position_match'")
+        if self.adjoint.friction_linearization != "true":
+            raise NotImplementedError("adjoint.friction_linearization='frozen' (the legacy stick/slip surrogate) "
+                                      "is not built; use 'true'")
         c.warm_start = 1 if self.warm_start.enabled else 0
         c.warm_start_cold_gravity = 1 if self.warm_start.cold_gravity else 0
         c.warm_start_cold_impact = 1 if self.warm_start.cold_impact else 0

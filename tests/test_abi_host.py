@@ -60,6 +60,11 @@ def test_config_validation_mirrors_reference():
     with pytest.raises(NotImplementedError):
         OstrichEngineConfig(relaxation=RelaxationConfig(friction="bilateral")).create_engine(
             model=None, differentiable_simulation=True)
+    # the reference's legacy modes that are not built raise rather than run as another mode
+    with pytest.raises(NotImplementedError):
+        OstrichEngineConfig(adjoint={"friction_linearization": "frozen"}).to_abi()
+    with pytest.raises(NotImplementedError):
+        OstrichEngineConfig(warm_start={"method": "pair_aggregate", "seed_iterate": True}).to_abi()
 
 
 def test_engine_requires_gpu_on_host():
