@@ -52,8 +52,10 @@ struct ClassTable {
 #ifndef OFB_JACOBI_SIDES
 #define OFB_JACOBI_SIDES -1, -1, 0, 1, 2, 2
 #endif
-static const ClassTable kTabJacobi = {6, {32, 64, 128, 256, 512, 1024}, {0, 1, 2, kMW + 2, kMW + 4, kMW + 8},
-                                      {OFB_JACOBI_SIDES}, 6};
+#ifndef OFB_JACOBI_CODES
+#define OFB_JACOBI_CODES 0, 1, 2, kMW + 2, kMW + 4, kMW + 8
+#endif
+static const ClassTable kTabJacobi = {6, {32, 64, 128, 256, 512, 1024}, {OFB_JACOBI_CODES}, {OFB_JACOBI_SIDES}, 6};
 static const ClassTable kTabBlockJacobi = {10, {32, 64, 128, 192, 256, 320, 384, 448, 768, 1024},
                                            {0, 1, 2, 3, 4, 5, 6, 7, 12, 16}, {-1, 0, 1, 2, 2, 2, 2, 2, 2, 2}, 3};
 static const ClassTable kTabAdjoint = {10, {32, 64, 128, 192, 256, 320, 384, 448, 768, 1024},
