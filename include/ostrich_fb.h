@@ -57,6 +57,12 @@ enum { OFB_CONTACT_COMPLEMENTARITY = 0, OFB_CONTACT_EQUALITY = 1 };
  * with the step-start poses; survivors keep their packed order. */
 enum { OFB_REDUCE_NONE = 0, OFB_REDUCE_TOP_K = 1, OFB_REDUCE_FPS = 2, OFB_REDUCE_CLUSTER = 3 };
 
+/* warm_start.method (collision/warm_start.py): match each contact to the previous step's
+ * nearest contact of its body pair (position_match, :84-519), or spread each body pair's
+ * previous total normal impulse evenly over the pair's current contacts (pair_aggregate,
+ * :896-1010). */
+enum { OFB_WS_POSITION_MATCH = 0, OFB_WS_PAIR_AGGREGATE = 1 };
+
 /* Termination of the Newton and PCR loops.
  * GLOBAL:    the reference's batch-global stop flag (base_engine.py:42-80,
  *            pcr_solver.py:84-117): every world iterates until all worlds converge.
@@ -138,9 +144,9 @@ typedef struct ofb_config {
                                     pose gradients */
   float adjoint_pose_eps;        /* central-difference step of that pull-back, in pose
                                     coordinates (m and quaternion components) */
-  int32_t warm_start;            /* warm_start.enabled, method position_match: seed each
-                                    step from the previous step's contacts (acts only
-                                    with seed_iterate, as in the reference) */
+  int32_t warm_start;            /* warm_start.enabled: seed each step from the previous
+                                    step's contacts (acts only with seed_iterate, as in
+                                    the reference) */
   int32_t warm_start_cold_gravity; /* warm_start.cold_gravity */
   int32_t warm_start_cold_impact;  /* warm_start.cold_impact */
   float warm_start_cold_friction_v;/* warm_start.cold_friction_v_threshold (<= 0: off) */
@@ -158,6 +164,7 @@ typedef struct ofb_config {
   int32_t contact_max_per_pair;  /* contacts.reduction.max_per_pair: K per (body0, body1) pair */
   float cluster_normal_dot;      /* contacts.reduction.cluster_normal_dot_thresh */
   float cluster_pos;             /* contacts.reduction.cluster_pos_thresh [m] */
+  int32_t warm_start_method;     /* warm_start.method: OFB_WS_* */
 } ofb_config;
 
 typedef struct ofb_state_in {

@@ -1675,9 +1675,10 @@ inline V3<double> ws_midpoint(const Batch<double>& Bt, int w, int c, const std::
   return mul(add(a, b), 0.5);
 }
 
-// apply(): _match_kernel (:84-297) then _cold_start_kernel (:300-462) -> seed [W*Nc]
-void ws_apply(Batch<double>& Bt, const WSPrev& P, int cold_gravity, int cold_impact, double v_t_threshold,
-              double* seed) {
+// apply(): _match_kernel (:84-297), or with pair_aggregate _pair_count_current_kernel and
+// _pair_apply_kernel (:943-1010), then _cold_start_kernel (:300-462) -> seed [W*Nc]
+void ws_apply(Batch<double>& Bt, const WSPrev& P, int method, int cold_gravity, int cold_impact,
+              double v_t_threshold, double* seed) {
   const int W = Bt.W, K = Bt.Cmax, B = Bt.B;
   const double dt = Bt.dt;
   V3<double> g = v3<double>(Bt.M.gravity[0], Bt.M.gravity[1], Bt.M.gravity[2]);
@@ -1695,6 +1696,29 @@ void ws_apply(Batch<double>& Bt, const WSPrev& P, int cold_gravity, int cold_imp
       int s0 = Bt.cs0[k], s1 = Bt.cs1[k];
       if (s0 == s1) continue;
       int b0 = s0 >= 0 ? Bt.shape_body(w, s0) : -1, b1 = s1 >= 0 ? Bt.shape_body(w, s1) : -1;
+      if (method == OFB_WS_PAIR_AGGREGATE) {
+        // the pair's previous total normal impulse (_pair_snapshot_kernel, :910-940, summed
+        // here in contact order) spread evenly over the pair's current contacts; lambda_t
+        // stays 0 and the gravity share of the cold start sees no unmatched contacts
+        double total = 0;
+        int n_pp = 0, n_cur = 0;
+        for (int j = 0; j < n_prev; ++j) {
+          size_t kj = size_t(w) * K + j;
+          if (P.b0[kj] == b0 && P.b1[kj] == b1) { total += P.ln[kj]; ++n_pp; }
+        }
+        for (int j = 0; j < n_c; ++j) {
+          size_t kj = size_t(w) * K + j;
+          if (Bt.cs0[kj] == Bt.cs1[kj]) continue;
+          int c0 = Bt.cs0[kj] >= 0 ? Bt.shape_body(w, Bt.cs0[kj]) : -1;
+          int c1 = Bt.cs1[kj] >= 0 ? Bt.shape_body(w, Bt.cs1[kj]) : -1;
+          if (c0 == b0 && c1 == b1) ++n_cur;
+        }
+        if (n_pp > 0 && n_cur > 0) {
+          seed[Bt.ri(w, on + c)] = total / double(n_cur);
+          matched[c] = 1;
+        }
+        continue;
+      }
       V3<double> pc = ws_midpoint(Bt, w, c, Bt.q_prev, b0, b1);
       V3<double> nc = Bt.cvec(Bt.cn, w, c);
       V3<double> vt = v3<double>(0, 0, 0);
@@ -2038,7 +2062,7 @@ int oracle_warm_start(const ofb_model* m, const ofb_config* cfg, const double* q
       for (int i = 0; i < 6; ++i) Bt.u_prev[k].v[i] = qd_prev[k * 6 + i];
     }
   WSPrev P{prev_count, prev_b0, prev_b1, prev_p, prev_n, prev_ln, prev_lt};
-  if (mode == 0) ws_apply(Bt, P, cfg->warm_start_cold_gravity, cfg->warm_start_cold_impact,
+  if (mode == 0) ws_apply(Bt, P, cfg->warm_start_method, cfg->warm_start_cold_gravity, cfg->warm_start_cold_impact,
                           cfg->warm_start_cold_friction_v, seed);
   else ws_snapshot(Bt, lam, P);
   return 0;

@@ -14,6 +14,8 @@ OFB_CONTACT_COMPLEMENTARITY = 0
 OFB_CONTACT_EQUALITY = 1
 OFB_TERM_GLOBAL = 0
 OFB_TERM_PER_WORLD = 1
+OFB_WS_POSITION_MATCH = 0
+OFB_WS_PAIR_AGGREGATE = 1
 OFB_PATH_FUSED = 0
 OFB_PATH_PLAIN = 1
 
@@ -99,6 +101,7 @@ class OfbConfig(C.Structure):
         ("contact_max_per_pair", C.c_int32),
         ("cluster_normal_dot", C.c_float),
         ("cluster_pos", C.c_float),
+        ("warm_start_method", C.c_int32),
     ]
 
 

@@ -367,8 +367,8 @@ class OstrichEngineConfig:
         c.adjoint_regularization = self.adjoint.regularization
         c.adjoint_pose_vjp = 1 if self.adjoint.pose_vjp else 0
         c.adjoint_pose_eps = self.adjoint.pose_fd_eps
-        if self.warm_start.method != "position_match" and self.warm_start.enabled and self.warm_start.seed_iterate:
-            raise NotImplementedError("warm_start.method='pair_aggregate' is not built; use 'position_match'")
+        c.warm_start_method = (_abi.OFB_WS_PAIR_AGGREGATE if self.warm_start.method == "pair_aggregate"
+                               else _abi.OFB_WS_POSITION_MATCH)
         if self.adjoint.friction_linearization != "true":
             raise NotImplementedError("adjoint.friction_linearization='frozen' (the legacy stick/slip surrogate) "
                                       "is not built; use 'true'")

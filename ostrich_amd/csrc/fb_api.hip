@@ -284,6 +284,10 @@ int ofb_create(const ofb_model* model, const ofb_config* config, int32_t path, o
     set_error("unknown contact reduction policy %d", config->contact_reduction);
     return -2;
   }
+  if (config->warm_start_method != OFB_WS_POSITION_MATCH && config->warm_start_method != OFB_WS_PAIR_AGGREGATE) {
+    set_error("unknown warm-start method %d", config->warm_start_method);
+    return -2;
+  }
   if (config->contact_reduction != OFB_REDUCE_NONE && config->contact_max_per_pair < 1) {
     set_error("contacts.reduction.max_per_pair must be >= 1 (collision/config.py:49-52)");
     return -2;

@@ -1,5 +1,5 @@
-// fb_warmstart.hip -- cross-step contact warm start (collision/warm_start.py:84-519,
-// method position_match) for both solver paths.
+// fb_warmstart.hip -- cross-step contact warm start (collision/warm_start.py, methods
+// position_match :84-519 and pair_aggregate :896-1010) for both solver paths.
 //
 // apply (before the solve, one wavefront per world): every current contact looks for the
 // previous step's contact of the same body pair whose start-of-step midpoint, moved by
@@ -31,6 +31,7 @@ struct WsParams {
   DeviceModel m;
   RowLayout L;
   float dt;
+  int method;  // OFB_WS_*
   int cold_gravity, cold_impact;
   float v_t_threshold;
   const float* q_prev;   // [W*B*7] start of step
@@ -147,6 +148,37 @@ __global__ void __launch_bounds__(64) ws_apply_kernel(WsParams P) {
     const int b0 = s0 >= 0 ? P.m.shape_body[w * SC + s0] : -1;
     const int b1 = s1 >= 0 ? P.m.shape_body[w * SC + s1] : -1;
     const v3 nc = cvec(P, w, c, CF_NX);
+    if (P.method == OFB_WS_PAIR_AGGREGATE) {
+      // pair_aggregate (_pair_count_current_kernel, _pair_apply_kernel, :943-1010): the
+      // pair's previous total normal impulse, summed in contact order, spread evenly over
+      // the pair's current contacts; lambda_t stays 0, and the cold start's gravity share
+      // sees no unmatched contacts
+      float total = 0.f;
+      int n_pp = 0, n_cur = 0;
+      for (int j = 0; j < n_prev; ++j) {
+        const size_t kj = (size_t)w * K + j;
+        if (P.prev_b0[kj] == b0 && P.prev_b1[kj] == b1) { total += P.prev_ln[kj]; ++n_pp; }
+      }
+      for (int j = 0; j < n_c; ++j) {
+        const int t0 = P.ci[(size_t)w * 2 * K + j], t1 = P.ci[(size_t)w * 2 * K + K + j];
+        if (t0 == t1) continue;
+        const int c0 = t0 >= 0 ? P.m.shape_body[w * SC + t0] : -1;
+        const int c1 = t1 >= 0 ? P.m.shape_body[w * SC + t1] : -1;
+        if (c0 == b0 && c1 == b1) ++n_cur;
+      }
+      float lnm = 0.f;
+      if (n_pp > 0 && n_cur > 0) {
+        lnm = total / (float)n_cur;
+        seed[on + c] = lnm;
+        matched[c] = 1;
+      }
+      if (P.feat) {
+        v3 ct1, ct2;
+        orthogonal_basis(nc, ct1, ct2);
+        write_features(P, w, c, b0, b1, s0, s1, nc, ct1, ct2, gmag, lnm, 0.f, 0.f);
+      }
+      continue;
+    }
     const v3 pc = midpoint(P, w, c, b0, b1, nc);
     v3 vt = mk(0.f, 0.f, 0.f);
     if (b0 >= 0) vt = vt + lin_vel(P, w, b0);
@@ -290,6 +322,7 @@ __global__ void ws_snapshot_kernel(WsParams P) {
 WsParams ws_params(ofb_solver* s, const ofb_state_in* in, float dt) {
   WsParams P;
   P.m = s->m; P.L = s->L; P.dt = dt;
+  P.method = s->cfg.warm_start_method;
   P.cold_gravity = s->cfg.warm_start_cold_gravity; P.cold_impact = s->cfg.warm_start_cold_impact;
   P.v_t_threshold = s->cfg.warm_start_cold_friction_v;
   P.q_prev = in->body_q; P.qd_prev = in->body_qd;

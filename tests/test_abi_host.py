@@ -63,8 +63,8 @@ def test_config_validation_mirrors_reference():
     # the reference's legacy modes that are not built raise rather than run as another mode
     with pytest.raises(NotImplementedError):
         OstrichEngineConfig(adjoint={"friction_linearization": "frozen"}).to_abi()
-    with pytest.raises(NotImplementedError):
-        OstrichEngineConfig(warm_start={"method": "pair_aggregate", "seed_iterate": True}).to_abi()
+    assert OstrichEngineConfig(warm_start={"method": "pair_aggregate"}).to_abi().warm_start_method == \
+        _abi.OFB_WS_PAIR_AGGREGATE
 
 
 def test_engine_requires_gpu_on_host():

@@ -40,11 +40,13 @@ def _run(gpu_device, model, state, control, cfg, steps, oracle_ws=None, K=16):
     return nr, seeds
 
 
-def test_warm_start_seed_matches_oracle(gpu_device, oracle_mod):
+@pytest.mark.parametrize("method", ["position_match", "pair_aggregate"])
+def test_warm_start_seed_matches_oracle(gpu_device, oracle_mod, method):
+    """Both warm-start methods (collision/warm_start.py: position_match, pair_aggregate)."""
     model, state, control = scenes.box_stack(16, seed=2)
-    cfg = config(K=16, warm_start=WarmStartConfig(seed_iterate=True))
+    cfg = config(K=16, warm_start=WarmStartConfig(seed_iterate=True, method=method))
     ws = oracle_mod.WarmStarter(model, cfg)
-    print("[warm start] parity, 16 box stacks", flush=True)
+    print(f"[warm start] parity, 16 box stacks, {method}", flush=True)
     _, seeds = _run(gpu_device, model, state, control, cfg, 8, oracle_ws=ws)
     for t, (g, o) in enumerate(seeds):
         scale = max(np.abs(o).max(), 1.0)

@@ -148,3 +148,52 @@ def test_seeded_iterate_cuts_newton_iterations_on_resting_stacks(oracle_mod):
             st = dict(st, body_q=o["body_q"], body_qd=o["body_qd"])
         nr[seeded] = np.mean(its[3:])
     assert nr[True] < 0.7 * nr[False], nr
+
+
+def _drop_contact(c, i):
+    """The flat contact list without entry i (later entries move up one slot)."""
+    d = {k: (v.clone() if torch.is_tensor(v) else np.array(v, copy=True)) for k, v in c.items()}
+    n = int(d["count"][0])
+    for k, v in d.items():
+        if k in ("count", "max_contacts"):
+            continue
+        v[i:n - 1] = v[i + 1:n].clone() if torch.is_tensor(v) else v[i + 1:n].copy()
+    d["count"][0] = n - 1
+    return d
+
+
+def test_pair_aggregate_spreads_each_pairs_total(oracle_mod):
+    """warm_start.method = pair_aggregate (collision/warm_start.py:896-1010): each body
+    pair's previous total normal impulse is spread evenly over the pair's current
+    contacts, friction seeds stay zero.  Two stacked cubes: four cube-cube contacts and
+    three cube-ground contacts with impulses 1..7 in the snapshot; the same contacts give
+    10/4 and 18/3 per contact, and with one cube-cube contact gone the first pair's 10 is
+    spread over the three that remain.  The position-match method on the same inputs
+    copies the contacts' own impulses instead."""
+    model, state, control = scenes.box_stack(1, n_cubes=2, seed=0)
+    K = 16
+    col = collide.PrimitiveCollider(model, gap=0.02, max_contacts_per_world=K)
+    st = {k: np.asarray(v, np.float64) for k, v in state.items()}
+    c = col.collide(torch.as_tensor(np.asarray(st["body_q"], np.float32)))
+    s0, s1 = np.asarray(c["shape0"][:7]), np.asarray(c["shape1"][:7])
+    assert int(c["count"][0]) == 7 and (s0[:4] == s0[0]).all() and (s1[:4] == s1[0]).all()
+    assert (s0[4:] == s0[4]).all() and (s1[4:] == s1[4]).all() and (s0[0], s1[0]) != (s0[4], s1[4])
+    lam_n = np.arange(1.0, 8.0)
+    res = {}
+    for method in ("pair_aggregate", "position_match"):
+        cfg = config(K=K, warm_start=WarmStartConfig(seed_iterate=True, method=method))
+        ws = oracle_mod.WarmStarter(model, cfg)
+        nc, nj, nctrl = oracle_mod.dims(model, cfg)
+        on, of = nj + nctrl, nj + nctrl + K
+        lam = np.zeros((1, nc))
+        lam[0, on:on + 7] = lam_n
+        lam[0, of:of + 14] = 0.3  # friction impulses: not carried by pair_aggregate
+        ws.snapshot(st, c, DT, lam)
+        res[method] = (ws.apply(st, c, DT), ws.apply(st, _drop_contact(c, 3), DT), on, of)
+    seed, seed_drop, on, of = res["pair_aggregate"]
+    np.testing.assert_allclose(seed[0, on:on + 7], [2.5] * 4 + [6.0] * 3, rtol=1e-12)
+    np.testing.assert_allclose(seed[0, of:of + 14], 0.0)
+    np.testing.assert_allclose(seed_drop[0, on:on + 6], [10.0 / 3] * 3 + [6.0] * 3, rtol=1e-12)
+    np.testing.assert_allclose(seed_drop[0, on + 6:on + K], 0.0)
+    pm, _, on, of = res["position_match"]
+    np.testing.assert_allclose(pm[0, on:on + 7], lam_n, rtol=1e-12)
