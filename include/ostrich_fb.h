@@ -165,6 +165,15 @@ typedef struct ofb_config {
   float cluster_normal_dot;      /* contacts.reduction.cluster_normal_dot_thresh */
   float cluster_pos;             /* contacts.reduction.cluster_pos_thresh [m] */
   int32_t warm_start_method;     /* warm_start.method: OFB_WS_* */
+  int32_t adjoint_friction_frozen; /* adjoint.friction_linearization == "frozen": the adjoint
+                                      replaces each friction row's compliance by the stick /
+                                      slip surrogate of adjoint/adjoint_friction.py:20-158
+                                      (rows of contacts without normal impulse dropped) and
+                                      leaves the friction rows out of w^T dR/dxi
+                                      (base_engine.py:536-590, 704-711) */
+  int32_t adjoint_soft_blending;   /* adjoint.soft_blending: sigmoid between the two
+                                      compliances instead of the 0.9 threshold */
+  float adjoint_soft_temperature;  /* adjoint.soft_blending_temperature */
 } ofb_config;
 
 typedef struct ofb_state_in {

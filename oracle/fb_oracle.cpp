@@ -1476,6 +1476,40 @@ template <class T> T& pose_coord(Tf<T>& X, int k) {
   }
 }
 
+// adjoint.friction_linearization = "frozen" (freeze_contact_mode_kernel and its soft
+// variant, adjoint/adjoint_friction.py:20-158, launched in base_engine.py:547-590): each
+// friction row of a contact with normal impulse >= 1e-6 gets the compliance of its contact
+// mode -- sticking (|lambda_t| < 0.9 mu lambda_n): compliance.joint; sliding: 100 -- or,
+// with soft blending, a sigmoid of (ratio - 0.9) / temperature between the two; rows of
+// contacts without normal impulse leave the system.
+template <class T> void freeze_friction(Batch<T>& Bt, int w, const ofb_config& cfg) {
+  const int K = Bt.Cmax, on = Bt.Nj + Bt.Nctrl, of = on + K;
+  const T stick = T(cfg.compliance_joint), slide = T(100);
+  for (int c = 0; c < std::min(Bt.ccount[w], K); ++c) {
+    size_t kc = size_t(w) * K + c;
+    size_t r0 = Bt.ri(w, of + 2 * c), r1 = r0 + 1;
+    T lt1 = Bt.lam[r0], lt2 = Bt.lam[r1], ln = Bt.lam[Bt.ri(w, on + c)];
+    T mu = T(0.5) * (Bt.smu(w, Bt.cs0[kc]) + Bt.smu(w, Bt.cs1[kc]));
+    if (ln < T(1e-6)) {
+      Bt.active[r0] = T(0);
+      Bt.active[r1] = T(0);
+      continue;
+    }
+    T ratio = std::sqrt(lt1 * lt1 + lt2 * lt2 + T(1e-10)) / (mu * ln + T(1e-10));
+    T ca;
+    if (cfg.adjoint_soft_blending) {
+      T e = -(ratio - T(0.9)) / std::max(T(cfg.adjoint_soft_temperature), T(1e-8));
+      e = std::min(std::max(e, T(-20)), T(20));
+      T sg = T(1) / (T(1) + std::exp(e));
+      ca = (T(1) - sg) * stick + sg * slide;
+    } else {
+      ca = ratio > T(0.9) ? slide : stick;
+    }
+    Bt.C[r0] = ca;
+    Bt.C[r1] = ca;
+  }
+}
+
 template <class T> int run_adjoint(const ofb_model* m, const ofb_config* cfg, const double* q_prev_in,
                                    const double* qd_prev_in, const float* f_in, const double* q_in,
                                    const double* qd_in, const float* tpos, const float* tvel,
@@ -1511,6 +1545,7 @@ template <class T> int run_adjoint(const ofb_model* m, const ofb_config* cfg, co
     for (int i = 0; i < Nc; ++i) { Bt.lam[Bt.ri(w, i)] = T(lam_in[Bt.ri(w, i)]); Bt.lam_prev[Bt.ri(w, i)] = T(lam_prev_in[Bt.ri(w, i)]); }
     // compute_linear_system at the converged state (base_engine.py:533-535)
     linear_system(Bt, w);
+    if (cfg->adjoint_friction_frozen) freeze_friction(Bt, w, *cfg);
     // adjoint_regularize_compliance_kernel: C += gamma on active rows
     if (cfg->adjoint_regularization > 0)
       for (int i = 0; i < Nc; ++i) if (Bt.active[Bt.ri(w, i)] > T(0)) Bt.C[Bt.ri(w, i)] += T(cfg->adjoint_regularization);
@@ -1596,6 +1631,11 @@ template <class T> int run_adjoint(const ofb_model* m, const ofb_config* cfg, co
         }
       }
     }
+    // frozen friction: the friction rows are not seeded in w^T dR/dxi (base_engine.py:
+    // 704-711), so neither the mu gradient nor the pose terms see them
+    std::vector<T> wl_seed(wl);
+    if (cfg->adjoint_friction_frozen)
+      for (int i = Bt.Nj + Bt.Nctrl + Bt.Cmax; i < Nc; ++i) wl_seed[Bt.ri(w, i)] = T(0);
     // friction coefficient: h_f = v_t + w(mu) lambda on both tangent rows
     residual(Bt, w);  // refreshes dw_dmu at the converged state
     if (g_mu) for (int s_ = 0; s_ < SC; ++s_) g_mu[size_t(w) * SC + s_] = 0;
@@ -1603,7 +1643,7 @@ template <class T> int run_adjoint(const ofb_model* m, const ofb_config* cfg, co
       size_t kc = size_t(w) * Bt.Cmax + c;
       size_t r0 = Bt.ri(w, Bt.Nj + Bt.Nctrl + Bt.Cmax + 2 * c), r1 = r0 + 1;
       T dw = Bt.dw_dmu[kc];
-      T contrib = wl[r0] * Bt.lam[r0] + wl[r1] * Bt.lam[r1];
+      T contrib = wl_seed[r0] * Bt.lam[r0] + wl_seed[r1] * Bt.lam[r1];
       if (std::isnan(dw)) { if (contrib != T(0)) flags |= 16; continue; }
       if (dw == T(0) || !g_mu) continue;
       int s0 = Bt.cs0[kc], s1 = Bt.cs1[kc];
@@ -1621,9 +1661,9 @@ template <class T> int run_adjoint(const ofb_model* m, const ofb_config* cfg, co
             T& v = pose_coord(X, kk);
             T orig = v;
             v = orig + h;
-            T sp_ = adjoint_weighted_residual(Bt, w, wl, wu);
+            T sp_ = adjoint_weighted_residual(Bt, w, wl_seed, wu);
             v = orig - h;
-            T sm_ = adjoint_weighted_residual(Bt, w, wl, wu);
+            T sm_ = adjoint_weighted_residual(Bt, w, wl_seed, wu);
             v = orig;
             (side == 0 ? gplus : gminus)[b * 7 + kk] = (sp_ - sm_) / (T(2) * h);
           }

@@ -102,6 +102,9 @@ class OfbConfig(C.Structure):
         ("cluster_normal_dot", C.c_float),
         ("cluster_pos", C.c_float),
         ("warm_start_method", C.c_int32),
+        ("adjoint_friction_frozen", C.c_int32),
+        ("adjoint_soft_blending", C.c_int32),
+        ("adjoint_soft_temperature", C.c_float),
     ]
 
 

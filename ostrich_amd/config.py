@@ -369,9 +369,9 @@ class OstrichEngineConfig:
         c.adjoint_pose_eps = self.adjoint.pose_fd_eps
         c.warm_start_method = (_abi.OFB_WS_PAIR_AGGREGATE if self.warm_start.method == "pair_aggregate"
                                else _abi.OFB_WS_POSITION_MATCH)
-        if self.adjoint.friction_linearization != "true":
-            raise NotImplementedError("adjoint.friction_linearization='frozen' (the legacy stick/slip surrogate) "
-                                      "is not built; use 'true'")
+        c.adjoint_friction_frozen = 1 if self.adjoint.friction_linearization == "frozen" else 0
+        c.adjoint_soft_blending = 1 if self.adjoint.soft_blending else 0
+        c.adjoint_soft_temperature = self.adjoint.soft_blending_temperature
         c.warm_start = 1 if self.warm_start.enabled else 0
         c.warm_start_cold_gravity = 1 if self.warm_start.cold_gravity else 0
         c.warm_start_cold_impact = 1 if self.warm_start.cold_impact else 0

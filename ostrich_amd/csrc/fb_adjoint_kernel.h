@@ -142,6 +142,36 @@ __device__ __forceinline__ float adjoint_weighted_residual(const AdjParams& P, f
   return wave_sum(s);
 }
 
+// adjoint.friction_linearization = "frozen" (freeze_contact_mode_kernel and its soft
+// variant, adjoint/adjoint_friction.py:20-158, launched in base_engine.py:547-590): the
+// friction rows of contact c take the compliance of the contact's mode at the converged
+// impulses -- sticking (|lambda_t| <= 0.9 mu lambda_n): compliance.joint; sliding: 100 --
+// or, with soft blending, a sigmoid of (ratio - 0.9) / temperature between the two; a
+// contact without normal impulse (< 1e-6) drops its friction rows from the system.
+template <class PT>
+__device__ __forceinline__ RowOut frozen_friction(const PT& P, int w, int c, RowOut o) {
+  const RowLayout& L = P.L;
+  const float* lw = P.lam + (size_t)w * L.Nc;
+  const float ln = lw[L.off_n + c], t1 = lw[L.off_f + 2 * c], t2 = lw[L.off_f + 2 * c + 1];
+  if (ln < 1e-6f) {
+    o.active = false;
+    return o;
+  }
+  const ContactGeo g = load_contact(P, w, c);
+  const int SC = P.m.SC;
+  const float mu = 0.5f * (P.m.shape_mu[w * SC + g.s0] + P.m.shape_mu[w * SC + g.s1]);
+  const float ratio = sqrtf(t1 * t1 + t2 * t2 + 1e-10f) / (mu * ln + 1e-10f);
+  const float stick = P.cfg.compliance_joint, slide = 100.f;
+  if (P.cfg.adjoint_soft_blending) {
+    const float e = fminf(fmaxf(-(ratio - 0.9f) / fmaxf(P.cfg.adjoint_soft_temperature, 1e-8f), -20.f), 20.f);
+    const float sg = 1.f / (1.f + expf(e));
+    o.C = (1.f - sg) * stick + sg * slide;
+  } else {
+    o.C = ratio > 0.9f ? slide : stick;
+  }
+  return o;
+}
+
 template <int R, bool BJ>
 __global__ void __launch_bounds__(256) adjoint_step_kernel(AdjParams P) {
   extern __shared__ float lds[];
@@ -223,9 +253,12 @@ __global__ void __launch_bounds__(256) adjoint_step_kernel(AdjParams P) {
     if (i < n_r) {
       RowOut o = assemble_row(P, sm, lamprev, w, i, n_c, lam[k], LdsPoses(), false);
       S.add(BHD, i, o.active, o.b0, o.b1, o.J0 * (-lam[k] * dt), o.J1 * (-lam[k] * dt));
-      J0[k] = o.J0; J1[k] = o.J1; b0[k] = o.b0; b1[k] = o.b1; act[k] = o.active;
-      Cr[k] = o.C + P.cfg.regularization + (o.active ? P.cfg.adjoint_regularization : 0.f);
+      J0[k] = o.J0; J1[k] = o.J1; b0[k] = o.b0; b1[k] = o.b1;
       if (!o.active && i >= L.Nj + L.Nctrl) lam[k] = 0.f;
+      if (P.cfg.adjoint_friction_frozen && i >= L.Nj + L.Nctrl + n_c && o.active)
+        o = frozen_friction(P, w, (i - L.Nj - L.Nctrl - n_c) >> 1, o);
+      act[k] = o.active;
+      Cr[k] = o.C + P.cfg.regularization + (o.active ? P.cfg.adjoint_regularization : 0.f);
     }
   }
   wsync();
@@ -340,7 +373,7 @@ __global__ void __launch_bounds__(256) adjoint_step_kernel(AdjParams P) {
         float den = dt * ke;
         if (den > 1e-6f) atomicAdd(&gctl[2 * D + qs], x[k] * (-dt / (den * den)) * lam[k] * dt);
       }
-    } else if (i >= L.Nj + L.Nctrl + n_c) {  // friction: h = v_t + w(mu) lambda
+    } else if (i >= L.Nj + L.Nctrl + n_c && !P.cfg.adjoint_friction_frozen) {  // friction: h = v_t + w(mu) lambda
       int f = i - L.Nj - L.Nctrl - n_c;
       float dw = 0.f;
       (void)friction_row(P, sm, lamprev, w, f >> 1, f & 1, n_c, lam[k], &dw);
@@ -381,6 +414,14 @@ __global__ void __launch_bounds__(256) adjoint_step_kernel(AdjParams P) {
   }
 
   // ---- 5b. w^T dR/dq+ and w^T dR/dq- by central differences ----
+  // with frozen friction the friction rows are not seeded (base_engine.py:704-711): their
+  // own residual term drops out, their share of h_d stays
+  float ws_[R];
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    const int i = lane + WAVE * k;
+    ws_[k] = (P.cfg.adjoint_friction_frozen && i >= L.Nj + L.Nctrl + n_c) ? 0.f : x[k];
+  }
   if (P.cfg.adjoint_pose_vjp && P.fd_off >= 0) {
     // row-local: each lane differences its own rows' terms in the 14 coordinates of each
     // of the row's bodies (the shifted pose is read privately, the LDS blocks stay
@@ -409,7 +450,7 @@ __global__ void __launch_bounds__(256) adjoint_step_kernel(AdjParams P) {
             if (sgn) T[(2 * i + side) * 14 + c] = 0.f;
             continue;
           }
-          const float t = adjoint_row_term(P, sm, lamprev, w, i, n_c, lam[k], x[k],
+          const float t = adjoint_row_term(P, sm, lamprev, w, i, n_c, lam[k], ws_[k],
                                            ShiftedPose{b, c, sgn ? -h : h}, &rb0, &rb1);
           if (it == 0) {
             if (sub == 0) { tb[2 * i] = rb0; tb[2 * i + 1] = rb1; }
@@ -441,10 +482,10 @@ __global__ void __launch_bounds__(256) adjoint_step_kernel(AdjParams P) {
       wsync();
       if (lane == 0) *ptr = orig + h;
       wsync();
-      float sp = adjoint_weighted_residual<R>(P, sm, lamprev, lane, w, n_r, n_c, lam, x, S);
+      float sp = adjoint_weighted_residual<R>(P, sm, lamprev, lane, w, n_r, n_c, lam, ws_, S);
       if (lane == 0) *ptr = orig - h;
       wsync();
-      float sn = adjoint_weighted_residual<R>(P, sm, lamprev, lane, w, n_r, n_c, lam, x, S);
+      float sn = adjoint_weighted_residual<R>(P, sm, lamprev, lane, w, n_r, n_c, lam, ws_, S);
       if (lane == 0) { *ptr = orig; gpose[d] = (sp - sn) / (2.f * h); }
       wsync();
     }

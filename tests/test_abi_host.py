@@ -60,9 +60,9 @@ def test_config_validation_mirrors_reference():
     with pytest.raises(NotImplementedError):
         OstrichEngineConfig(relaxation=RelaxationConfig(friction="bilateral")).create_engine(
             model=None, differentiable_simulation=True)
-    # the reference's legacy modes that are not built raise rather than run as another mode
-    with pytest.raises(NotImplementedError):
-        OstrichEngineConfig(adjoint={"friction_linearization": "frozen"}).to_abi()
+    # the reference's legacy adjoint and warm-start modes reach the ABI
+    fz = OstrichEngineConfig(adjoint={"friction_linearization": "frozen", "soft_blending": False}).to_abi()
+    assert fz.adjoint_friction_frozen == 1 and fz.adjoint_soft_blending == 0
     assert OstrichEngineConfig(warm_start={"method": "pair_aggregate"}).to_abi().warm_start_method == \
         _abi.OFB_WS_PAIR_AGGREGATE
 

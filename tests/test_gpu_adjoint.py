@@ -111,6 +111,27 @@ def test_adjoint_skid_steer(oracle_mod, gpu_device):
     assert np.abs(o64["joint_target_vel"]).max() > 0
 
 
+@pytest.mark.parametrize("soft", [False, True])
+@pytest.mark.parametrize("scene", ["skid_steer", "box_stack"])
+def test_adjoint_frozen_friction(oracle_mod, gpu_device, scene, soft):
+    """adjoint.friction_linearization = "frozen" (the stick / slip compliance surrogate of
+    adjoint/adjoint_friction.py, with the 0.9 threshold or the sigmoid blend; friction
+    rows left out of w^T dR/dxi) against the oracle's restatement."""
+    from ostrich_amd.config import AdjointConfig
+    adj = AdjointConfig(friction_linearization="frozen", soft_blending=soft)
+    if scene == "skid_steer":
+        model, state, control = scenes.skid_steer(32, seed=3)
+        K, steps = 32, 4
+    else:
+        model, state, control = scenes.box_stack(16, seed=0)
+        K, steps = 16, 3
+    cfg = config(K=K, adjoint=adj)
+    col = collide.PrimitiveCollider(model, gap=0.02, max_contacts_per_world=K)
+    st = _advance(oracle_mod, model, cfg, state, control, col, steps)
+    g, o64 = _compare(oracle_mod, model, cfg, st, control, col, gpu_device, seed=4)
+    assert np.abs(g["shape_mu"]).max() == 0.0
+
+
 def test_adjoint_free_body_closed_form(gpu_device):
     """No contacts: w_u = M^-1 f_drive, dL/du- = -M w_u = g_u + dt g_p (translation)."""
     model, state = single_box(mass=2.0, pose=(0.3, -0.2, 3.0, 0, 0, 0, 1))
