@@ -311,7 +311,7 @@ __global__ void __launch_bounds__(256) adjoint_step_kernel(AdjParams P) {
   bsq = wave_sum(bsq);
   if (BJ) {
     bj_structure<R>(V, lane, n_r, B, act, b0, b1, G, flags);
-    bj_factor<R>(V, sm, b0, b1, J0, J1, Cr, G, flags);
+    bj_factor<R>(V, sm, lane, b0, b1, J0, J1, Cr, G, flags);
   }
   const float tol_sq = P.cfg.lin_tol >= 0.f ? P.cfg.lin_tol * P.cfg.lin_tol : -1.f;
   const float latol_sq = P.cfg.lin_atol >= 0.f ? P.cfg.lin_atol * P.cfg.lin_atol : -1.f;

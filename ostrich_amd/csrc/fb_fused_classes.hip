@@ -41,11 +41,10 @@ int OFB_CAT(fused_class_op_bj, OFB_BJ, _set, OFB_RSET)(int code, int op, const F
   return -1;
 }
 
-// Profiling build only: the phase counters [17][10] of the Jacobi instances of this unit
-// (each translation unit holds its own copy: the paired, dense and R = 2 classes with
-// OFB_RSET = 0, the worlds of several waves with OFB_RSET = 1).
-#if defined(OFB_PHASE_TIMING) && OFB_BJ == 0
-extern "C" int OFB_CAT(ofb_debug_phase_cycles, _set, OFB_RSET, )(unsigned long long* out, int reset) {
+// Profiling build only: the phase counters [17][10] of the instances of this unit (each
+// translation unit holds its own copy).
+#if defined(OFB_PHASE_TIMING)
+extern "C" int OFB_CAT(ofb_debug_phase_cycles_bj, OFB_BJ, _set, OFB_RSET)(unsigned long long* out, int reset) {
   if (hipMemcpyFromSymbol(out, HIP_SYMBOL(g_phase_cycles), sizeof(unsigned long long) * 170) != hipSuccess) return -4;
   if (reset) {
     static const unsigned long long z[170] = {};

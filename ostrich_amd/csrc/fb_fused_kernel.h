@@ -167,22 +167,28 @@ struct WorldSum {
 // row's rank is its position among the pair's rows in row order, and rows ranked 64 or
 // more fall back to the Jacobi diagonal (the reference's fixed 64-member blocks, :59).
 //
-// Work is spread over the pair's member lanes instead of one thread per pair:
+// Work is spread over the lanes by member slot (pair offset + rank; a pair's members
+// hold consecutive slots), so a 64-member block of a large world occupies one slot of
+// every lane instead of a few lanes in each of its row slots:
 //  * structure (once per step: row activity and bodies do not change inside a step):
-//    pair ids, ranks, sizes, the pair's slot range and packed-block offset;
-//  * factor (once per Newton iteration): lane a builds row a of the packed lower
-//    triangle of A_pair = sum_b J_a,b M_b^-1 J_c,b^T (+ C + reg on the diagonal); the
-//    wave factors it in place with a right-looking Cholesky, pivot by pivot (the
-//    reference's factor, same failure test pivot <= 0, :296-321; a failed pair uses the
-//    Jacobi diagonal, :374-380), then inverts L in place, one column per lane;
+//    pair ids, ranks, sizes, the pair's slot range and packed-block offset, kept per
+//    member slot in LDS;
+//  * factor (once per Newton iteration): member s builds row s of its pair's packed lower
+//    triangle of A_pair = sum_b J_a,b M_b^-1 J_c,b^T (+ C + reg on the diagonal) from
+//    the staged member records; the wave factors it in place with a right-looking
+//    Cholesky, pivot by pivot (the reference's factor, same failure test pivot <= 0,
+//    :296-321; a failed pair uses the Jacobi diagonal, :374-380), then inverts L in
+//    place, one column per member;
 //  * apply (twice per PCR iteration): y = L^-1 v and z = L^-T y as two dense triangular
-//    matvecs of at most 64 LDS reads per lane with no serial dependence, where the
+//    matvecs of at most 64 LDS reads per member with no serial dependence, where the
 //    reference runs two triangular solves (:384-410).  An explicit A_pair^-1 would be
 //    cheaper still, but in FP32 it loses an order of magnitude more accuracy on the
 //    near-singular blocks of coplanar contacts (friction rows with C ~ 1e-6).
 // LDS per wave after the bodies and lambda_prev (cap = 64 R rows):
-//   pid[cap] i32, lsz[cap] i32, colbuf[cap], vbuf[cap], rowd[16 cap] (J0 6, J1 6,
-//   b0 bits, b1 bits, C + reg, pad) indexed by member slot, X[R * 2080] packed blocks.
+//   pid[cap] i32, lsz[cap] i32 (structure), mrank/mm/mpoff/mfail[cap] i32 (per member
+//   slot: rank, block size, packed offset, failed pivot), colbuf[cap], vbuf[cap],
+//   rowd[16 cap] (J0 6, J1 6, b0 bits, b1 bits, C + reg, pad) per member slot,
+//   X[R * 2080] packed blocks.
 // ---------------------------------------------------------------------------------
 constexpr int kPairCap = 64;
 constexpr int kPairL = kPairCap * (kPairCap + 1) / 2;
@@ -190,12 +196,16 @@ constexpr int kPairL = kPairCap * (kPairCap + 1) / 2;
 // packed blocks: at most cap / 64 full 64-row pairs (cap a multiple of 64), or one pair
 // holding every row (cap < 64)
 __host__ __device__ inline int bj_words(int cap) {
-  return 20 * cap + (cap >= kPairCap ? (cap / kPairCap) * kPairL : cap * (cap + 1) / 2);
+  return 24 * cap + (cap >= kPairCap ? (cap / kPairCap) * kPairL : cap * (cap + 1) / 2);
 }
 
 struct BJView {
   int* pid;
   int* lsz;
+  int* mrank;
+  int* mm;
+  int* mpoff;
+  int* mfail;
   float* colbuf;
   float* vbuf;
   float* rowd;
@@ -206,7 +216,11 @@ __device__ __forceinline__ BJView bj_view(float* base, int cap) {
   BJView v;
   v.pid = reinterpret_cast<int*>(base);
   v.lsz = v.pid + cap;
-  v.colbuf = reinterpret_cast<float*>(v.lsz + cap);
+  v.mrank = v.lsz + cap;
+  v.mm = v.mrank + cap;
+  v.mpoff = v.mm + cap;
+  v.mfail = v.mpoff + cap;
+  v.colbuf = reinterpret_cast<float*>(v.mfail + cap);
   v.vbuf = v.colbuf + cap;
   v.rowd = v.vbuf + cap;
   v.X = v.rowd + 16 * cap;
@@ -222,6 +236,7 @@ struct BJRegs {
   int poff[R];  // offset of the pair's packed block in X
   int fail[R];  // the pair's block has a pivot <= 0 this Newton iteration
   int mmax;     // largest block of the wave
+  int nmem;     // member slots in use (every block's rows)
 };
 
 // pair ids (_compute_pair_ids_kernel :62-109), member ranks and sizes (:112-144)
@@ -262,7 +277,7 @@ __device__ __forceinline__ void bj_structure(const BJView& V, int lane, int n_r,
     if (lead == i && size > kPairCap) flags |= FLAG_PAIR_OVERFLOW;
   }
   wsync();
-  int mm = 0;
+  int mmx = 0, nm = 0;
 #pragma unroll
   for (int k = 0; k < R; ++k) {
     int mo = 0, po = 0;
@@ -273,20 +288,27 @@ __device__ __forceinline__ void bj_structure(const BJView& V, int lane, int n_r,
     }
     G.slot[k] = (leader[k] >= 0 && G.rank[k] < kPairCap) ? mo + G.rank[k] : -1;
     G.poff[k] = po;
-    if (G.slot[k] >= 0) mm = max(mm, G.m[k]);
+    if (G.slot[k] >= 0) {
+      mmx = max(mmx, G.m[k]);
+      nm = max(nm, G.slot[k] + 1);
+      V.mrank[G.slot[k]] = G.rank[k];
+      V.mm[G.slot[k]] = G.m[k];
+      V.mpoff[G.slot[k]] = po;
+    }
   }
-  G.mmax = group_max_int<GW>(mm);
+  G.mmax = group_max_int<GW>(mmx);
+  G.nmem = group_max_int<GW>(nm);
   wsync();
 }
 
-// dense blocks (_extract_pair_blocks_kernel :187-264) and their inverses
+// dense blocks (_extract_pair_blocks_kernel :187-264) and their inverses; the member
+// loops run over member slots s = lane + 64 j (j < R: a world has at most 64 R members)
 template <int R>
-__device__ __forceinline__ void bj_factor(const BJView& V, const float* sm, const int (&b0)[R], const int (&b1)[R],
-                                          const sv (&J0)[R], const sv (&J1)[R], const float (&Cr)[R], BJRegs<R>& G,
-                                          int& flags) {
+__device__ __forceinline__ void bj_factor(const BJView& V, const float* sm, int lane, const int (&b0)[R],
+                                          const int (&b1)[R], const sv (&J0)[R], const sv (&J1)[R],
+                                          const float (&Cr)[R], BJRegs<R>& G, int& flags) {
 #pragma unroll
   for (int k = 0; k < R; ++k) {
-    G.fail[k] = 0;
     if (G.slot[k] >= 0) {
       float* d = V.rowd + 16 * G.slot[k];
       d[0] = J0[k].a.x; d[1] = J0[k].a.y; d[2] = J0[k].a.z; d[3] = J0[k].b.x; d[4] = J0[k].b.y; d[5] = J0[k].b.z;
@@ -295,14 +317,30 @@ __device__ __forceinline__ void bj_factor(const BJView& V, const float* sm, cons
     }
   }
   wsync();
+  // member registers: rank, block size, packed offset, failed pivot of member slot
+  // lane + 64 j (m = 0: no member)
+  int mr[R], msz[R], mpo[R], mfl[R];
 #pragma unroll
-  for (int k = 0; k < R; ++k) {
-    if (G.slot[k] < 0) continue;
+  for (int j = 0; j < R; ++j) {
+    const int s = lane + WAVE * j;
+    mr[j] = 0; msz[j] = 0; mpo[j] = 0; mfl[j] = 0;
+    if (s < G.nmem) { mr[j] = V.mrank[s]; msz[j] = V.mm[s]; mpo[j] = V.mpoff[s]; }
+  }
+  // block row of member s: entries c <= rank from the members' staged records
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    if (msz[j] == 0) continue;
+    const int s = lane + WAVE * j, rank = mr[j], base = s - rank;
+    const float4* ds = reinterpret_cast<const float4*>(V.rowd + 16 * s);
+    const float4 e0 = ds[0], e1 = ds[1], e2 = ds[2], e3 = ds[3];
+    const sv Ja{mk(e0.x, e0.y, e0.z), mk(e0.w, e1.x, e1.y)};
+    const sv Jb{mk(e1.z, e1.w, e2.x), mk(e2.y, e2.z, e2.w)};
+    const int ba = __float_as_int(e3.x), bb = __float_as_int(e3.y);
+    const float crr = e3.z;
     sv m0 = svzero(), m1 = svzero();
-    if (b0[k] >= 0) m0 = apply_minv(sm + b0[k] * BS, J0[k]);
-    if (b1[k] >= 0) m1 = apply_minv(sm + b1[k] * BS, J1[k]);
-    const int rank = G.rank[k], base = G.slot[k] - rank;
-    float* Xr = V.X + G.poff[k] + rank * (rank + 1) / 2;
+    if (ba >= 0) m0 = apply_minv(sm + ba * BS, Ja);
+    if (bb >= 0) m1 = apply_minv(sm + bb * BS, Jb);
+    float* Xr = V.X + mpo[j] + rank * (rank + 1) / 2;
     // columns in batches of four: the batch's member records are read before any entry
     // is stored, so the LDS reads of a batch overlap
     for (int c0 = 0; c0 <= rank; c0 += 4) {
@@ -320,9 +358,9 @@ __device__ __forceinline__ void bj_factor(const BJView& V, const float* sm, cons
         sv Jc1{mk(q[u][1].z, q[u][1].w, q[u][2].x), mk(q[u][2].y, q[u][2].z, q[u][2].w)};
         int cb0 = __float_as_int(q[u][3].x), cb1 = __float_as_int(q[u][3].y);
         float val = 0.f;
-        if (b0[k] >= 0) { if (cb0 == b0[k]) val += sdot(Jc0, m0); else if (cb1 == b0[k]) val += sdot(Jc1, m0); }
-        if (b1[k] >= 0) { if (cb0 == b1[k]) val += sdot(Jc0, m1); else if (cb1 == b1[k]) val += sdot(Jc1, m1); }
-        if (c == rank) val += Cr[k];
+        if (ba >= 0) { if (cb0 == ba) val += sdot(Jc0, m0); else if (cb1 == ba) val += sdot(Jc1, m0); }
+        if (bb >= 0) { if (cb0 == bb) val += sdot(Jc0, m1); else if (cb1 == bb) val += sdot(Jc1, m1); }
+        if (c == rank) val += crr;
         Xr[c] = val;
       }
     }
@@ -330,37 +368,37 @@ __device__ __forceinline__ void bj_factor(const BJView& V, const float* sm, cons
   wsync();
   // right-looking Cholesky in place, pivot by pivot (the reference's column Cholesky,
   // :296-321): column k is snapshotted, L_kk = sqrt(X_kk), L_ak = X_ak / L_kk and the
-  // trailing rows update X_ac -= L_ak L_ck.  Every member lane reads the same pivot, so
-  // the pair agrees on a failure.
+  // trailing rows update X_ac -= L_ak L_ck.  Every member reads the same pivot, so the
+  // pair agrees on a failure.
   for (int piv = 0; piv < G.mmax; ++piv) {
     float colv[R];
 #pragma unroll
-    for (int k = 0; k < R; ++k) {
-      colv[k] = 0.f;
-      if (G.slot[k] >= 0 && piv < G.m[k] && !G.fail[k] && G.rank[k] >= piv) {
-        const int rank = G.rank[k];
-        colv[k] = V.X[G.poff[k] + rank * (rank + 1) / 2 + piv];
-        V.colbuf[G.slot[k]] = colv[k];
+    for (int j = 0; j < R; ++j) {
+      colv[j] = 0.f;
+      if (piv < msz[j] && !mfl[j] && mr[j] >= piv) {
+        const int s = lane + WAVE * j, rank = mr[j];
+        colv[j] = V.X[mpo[j] + rank * (rank + 1) / 2 + piv];
+        V.colbuf[s] = colv[j];
       }
     }
     wsync();
 #pragma unroll
-    for (int k = 0; k < R; ++k) {
-      if (G.slot[k] < 0 || piv >= G.m[k] || G.fail[k]) continue;
-      const int rank = G.rank[k], base = G.slot[k] - rank;
+    for (int j = 0; j < R; ++j) {
+      if (piv >= msz[j] || mfl[j]) continue;
+      const int s = lane + WAVE * j, rank = mr[j], base = s - rank;
       const float dpiv = V.colbuf[base + piv];
       if (dpiv <= 0.f) {
-        G.fail[k] = 1;
+        mfl[j] = 1;
         flags |= FLAG_FACTOR_FAILURE;
         continue;
       }
       if (rank < piv) continue;
       const float lkk = sqrtf(dpiv), inv = 1.f / lkk;
-      float* Xr = V.X + G.poff[k] + rank * (rank + 1) / 2;
+      float* Xr = V.X + mpo[j] + rank * (rank + 1) / 2;
       if (rank == piv) {
         Xr[piv] = lkk;
       } else {
-        const float lak = colv[k] * inv;
+        const float lak = colv[j] * inv;
         Xr[piv] = lak;
         // trailing entries in batches of eight: loads first, then the stores
         for (int c0 = piv + 1; c0 <= rank; c0 += 8) {
@@ -379,19 +417,19 @@ __device__ __forceinline__ void bj_factor(const BJView& V, const float* sm, cons
     }
     wsync();
   }
-  // X <- L^-1 in place, one column per lane (lane of rank c owns column c): at step a
-  // row a of L is copied out, then Z_ac = -(sum_{q=c}^{a-1} L_aq Z_qc) / L_aa
+  // X <- L^-1 in place, one column per member (member of rank c owns column c): at step
+  // a row a of L is copied out, then Z_ac = -(sum_{q=c}^{a-1} L_aq Z_qc) / L_aa
   for (int a = 0; a < G.mmax; ++a) {
 #pragma unroll
-    for (int k = 0; k < R; ++k)
-      if (G.slot[k] >= 0 && a < G.m[k] && !G.fail[k] && G.rank[k] <= a)
-        V.colbuf[G.slot[k]] = V.X[G.poff[k] + a * (a + 1) / 2 + G.rank[k]];
+    for (int j = 0; j < R; ++j)
+      if (a < msz[j] && !mfl[j] && mr[j] <= a)
+        V.colbuf[lane + WAVE * j] = V.X[mpo[j] + a * (a + 1) / 2 + mr[j]];
     wsync();
 #pragma unroll
-    for (int k = 0; k < R; ++k) {
-      if (G.slot[k] < 0 || a >= G.m[k] || G.fail[k] || G.rank[k] > a) continue;
-      const int c = G.rank[k], base = G.slot[k] - c;
-      const float* Xp = V.X + G.poff[k];
+    for (int j = 0; j < R; ++j) {
+      if (a >= msz[j] || mfl[j] || mr[j] > a) continue;
+      const int s = lane + WAVE * j, c = mr[j], base = s - c;
+      const float* Xp = V.X + mpo[j];
       const float laa = V.colbuf[base + a];
       float z;
       if (c == a) {
@@ -413,28 +451,39 @@ __device__ __forceinline__ void bj_factor(const BJView& V, const float* sm, cons
         }
         z = -acc / laa;
       }
-      V.X[G.poff[k] + a * (a + 1) / 2 + c] = z;
+      V.X[mpo[j] + a * (a + 1) / 2 + c] = z;
     }
     wsync();
   }
+  // failed pairs for the rows (apply's Jacobi fallback)
+#pragma unroll
+  for (int j = 0; j < R; ++j)
+    if (msz[j] > 0) V.mfail[lane + WAVE * j] = mfl[j];
+  wsync();
+#pragma unroll
+  for (int k = 0; k < R; ++k) G.fail[k] = G.slot[k] >= 0 ? V.mfail[G.slot[k]] : 0;
 }
 
 // out = P v = L^-T (L^-1 v) over each pair (apply of per_body_pair_preconditioner.py:
 // 324-410, whose two triangular solves become two dense triangular matvecs with the
-// stored L^-1); failed pairs and rows outside any block use the Jacobi diagonal
+// stored L^-1, computed per member slot); failed pairs and rows outside any block use
+// the Jacobi diagonal
 template <int R>
-__device__ __forceinline__ void bj_apply(const BJView& V, const bool (&act)[R], const float (&Pi)[R],
+__device__ __forceinline__ void bj_apply(const BJView& V, int lane, const bool (&act)[R], const float (&Pi)[R],
                                          const BJRegs<R>& G, const float (&v)[R], float (&out)[R]) {
 #pragma unroll
   for (int k = 0; k < R; ++k)
     if (G.slot[k] >= 0 && !G.fail[k]) V.vbuf[G.slot[k]] = v[k];
   wsync();
-  // y = L^-1 v: row a of the packed triangle
+  // y = L^-1 v: row `rank` of each member's packed triangle
+  float yv[R];
 #pragma unroll
-  for (int k = 0; k < R; ++k) {
-    if (G.slot[k] < 0 || G.fail[k]) continue;
-    const int rank = G.rank[k], base = G.slot[k] - rank;
-    const float* Zr = V.X + G.poff[k] + rank * (rank + 1) / 2;
+  for (int j = 0; j < R; ++j) {
+    const int s = lane + WAVE * j;
+    yv[j] = 0.f;
+    if (s >= G.nmem || V.mfail[s]) continue;
+    const int rank = V.mrank[s], base = s - rank;
+    const float* Zr = V.X + V.mpoff[s] + rank * (rank + 1) / 2;
     float y = 0.f;
     for (int c0 = 0; c0 <= rank; c0 += 8) {  // in c order, operands read eight at a time
       float zr[8], vb[8];
@@ -448,35 +497,47 @@ __device__ __forceinline__ void bj_apply(const BJView& V, const bool (&act)[R], 
       for (int u = 0; u < 8; ++u)
         if (c0 + u <= rank) y += zr[u] * vb[u];
     }
-    V.colbuf[G.slot[k]] = y;
+    yv[j] = y;
+  }
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    const int s = lane + WAVE * j;
+    if (s < G.nmem) V.colbuf[s] = yv[j];
   }
   wsync();
-  // z = L^-T y: column a of the packed triangle
+  // z = L^-T y: column `rank` of each member's packed triangle, into vbuf
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    const int s = lane + WAVE * j;
+    if (s >= G.nmem || V.mfail[s]) continue;
+    const int rank = V.mrank[s], base = s - rank, m = V.mm[s];
+    const float* Xp = V.X + V.mpoff[s];
+    float z = 0.f;
+    for (int c0 = rank; c0 < m; c0 += 8) {  // in c order, operands read eight at a time
+      float xp[8], cb[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int c = min(c0 + u, m - 1);
+        xp[u] = Xp[c * (c + 1) / 2 + rank];
+        cb[u] = V.colbuf[base + c];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (c0 + u < m) z += xp[u] * cb[u];
+    }
+    yv[j] = z;
+  }
+  wsync();
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    const int s = lane + WAVE * j;
+    if (s < G.nmem) V.vbuf[s] = yv[j];
+  }
+  wsync();
 #pragma unroll
   for (int k = 0; k < R; ++k) {
     float o = 0.f;
-    if (act[k]) {
-      if (G.slot[k] >= 0 && !G.fail[k]) {
-        const int rank = G.rank[k], base = G.slot[k] - rank;
-        const float* Xp = V.X + G.poff[k];
-        float z = 0.f;
-        for (int c0 = rank; c0 < G.m[k]; c0 += 8) {  // in c order, operands read eight at a time
-          float xp[8], cb[8];
-#pragma unroll
-          for (int u = 0; u < 8; ++u) {
-            const int c = min(c0 + u, G.m[k] - 1);
-            xp[u] = Xp[c * (c + 1) / 2 + rank];
-            cb[u] = V.colbuf[base + c];
-          }
-#pragma unroll
-          for (int u = 0; u < 8; ++u)
-            if (c0 + u < G.m[k]) z += xp[u] * cb[u];
-        }
-        o = z;
-      } else {
-        o = Pi[k] * v[k];
-      }
-    }
+    if (act[k]) o = (G.slot[k] >= 0 && !G.fail[k]) ? V.vbuf[G.slot[k]] : Pi[k] * v[k];
     out[k] = o;
   }
   wsync();
@@ -952,7 +1013,7 @@ __device__ __forceinline__ int pcr_solve(float* sm, int lane, int B, int n_r, co
   };
 #pragma unroll
   for (int k = 0; k < R; ++k) { x[k] = 0.f; r[k] = rhs[k]; z[k] = Pi[k] * r[k]; }
-  if (BJ) bj_apply<R>(V, act, Pi, G, r, z);
+  if (BJ) bj_apply<R>(V, lane, act, Pi, G, r, z);
   matvec(z, Az);
   float zAz = 0.f, rsq = 0.f;
 #pragma unroll
@@ -966,7 +1027,7 @@ __device__ __forceinline__ int pcr_solve(float* sm, int lane, int B, int n_r, co
     float yAp = 0.f;
 #pragma unroll
     for (int k = 0; k < R; ++k) y[k] = Pi[k] * Ap[k];
-    if (BJ) bj_apply<R>(V, act, Pi, G, Ap, y);
+    if (BJ) bj_apply<R>(V, lane, act, Pi, G, Ap, y);
 #pragma unroll
     for (int k = 0; k < R; ++k) yAp += y[k] * Ap[k];
     yAp = WS.sum(yAp);
@@ -1245,7 +1306,7 @@ fused_step_kernel(FusedParams P) {
     bsq = WS.sum(bsq);
     if (BJ) {
       if (n_it == 0) bj_structure<R, GW>(V, lane, n_r, B, act, b0, b1, G, bj_flags);
-      bj_factor<R>(V, sm, b0, b1, J0, J1, Cr, G, bj_flags);
+      bj_factor<R>(V, sm, lane, b0, b1, J0, J1, Cr, G, bj_flags);
     }
 
     OFB_PHASE(2)
