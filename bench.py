@@ -672,6 +672,9 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-scenes", action="store_true", help="skip the tracked-robot and box-stack legs")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape (no autotuner run)")
+    ap.add_argument("--scene-only", choices=("tracked_flipper", "box_stack"), default=None,
+                    help="run only that workload's leg (previous-step warm start, class timing) and print its JSON "
+                         "(for profiling runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 256 MiB device copy (reference kernel for HBM byte counters)")
     args = ap.parse_args()
@@ -680,6 +683,19 @@ def main():
               "with no exchange; multi-GPU runs are not part of this build", file=sys.stderr)
         sys.exit(2)
     device = "cuda:0"
+    if args.scene_only:
+        if args.scene_only == "tracked_flipper":
+            model, state, control = scenes.tracked_flipper(1024, seed=2)
+            col = collide.HipCollider(model, device=device, gap=0.05, max_contacts_per_world=256, hf_contacts="cells")
+            K = 256
+        else:
+            model, state, control = scenes.box_stack(16, 3, seed=0)
+            col = collide.HipCollider(model, device=device, gap=0.02, max_contacts_per_world=16)
+            K = 16
+        res = run_path("fused:jacobi:prev", model, state, control, args.steps, args.warmup, device, col, K=K,
+                       class_timing=True)
+        print(json.dumps(dict(workload=args.scene_only, worlds=model.num_worlds, **res)))
+        return
     W = args.worlds
     model, state, control = scenes.skid_steer(W, seed=1)
     collider = collide.HipCollider(model, device=device, gap=C1_GAP, max_contacts_per_world=K_CONTACTS)
