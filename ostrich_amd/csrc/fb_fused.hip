@@ -44,13 +44,19 @@ struct ClassTable {
   int side[kMaxClasses];   // side stream of each class (-1: the step's stream; class 0 runs there, last)
   int merge_from;          // classes from this index on are launched as one (n: none)
 };
-// Streams of the Jacobi classes (measured, 4096 uneven skid-steer worlds): the dense class
-// on the step's stream ahead of the paired class, the R = 2 class and the worlds of two
-// waves on side streams of their own, the rare worlds of four and eight waves on the
-// third.  One class per stream (five streams) replays about as fast from a graph but
-// steps slower eagerly; the large classes sharing one stream added their durations.
+// Streams of the Jacobi classes (measured on 4096 uneven skid-steer worlds and 1024
+// tracked-flipper worlds; four streams, as many as a process's hardware queues): the
+// dense class, the R = 2 class and the worlds of two waves on side streams of their own;
+// the worlds of eight and of four waves, then the paired class, on the step's stream.
+// The skid-steer batch has a few worlds of four waves and none of eight, so the paired
+// class (the shortest launch) follows the four-wave launch; in the tracked-flipper batch
+// the eight-wave worlds (LDS of a whole CU each) start first and take the CUs before the
+// other classes fill them.  The dense class ahead of the paired one on the step's stream
+// stepped 0.04 ms slower from a graph and 0.15 ms slower eagerly (skid-steer), and the
+// eight- and four-wave launches on a side stream 0.5 ms slower (tracked flipper; on
+// separate streams, 0.2 ms faster there but 0.05 ms slower from a graph on skid-steer).
 #ifndef OFB_JACOBI_SIDES
-#define OFB_JACOBI_SIDES -1, -1, 0, 1, 2, 2
+#define OFB_JACOBI_SIDES -1, 2, 0, 1, -1, -1
 #endif
 #ifndef OFB_JACOBI_CODES
 #define OFB_JACOBI_CODES 0, 1, 2, kMW + 2, kMW + 4, kMW + 8

@@ -554,7 +554,11 @@ __device__ __forceinline__ void schur_matvec(float* sm, int lane, int B, int n_r
 #pragma unroll
   for (int k = 0; k < R; ++k) S.add(BACC, lane + GW * k, act[k], b0[k], b1[k], J0[k] * v[k], J1[k] * v[k]);
   S.end(BACC, true, n_r);
-  gsync<GW>();
+  // A world of several waves with lane-private sums: the body totals were written by the
+  // first wave's lanes (6B <= 60 values) after the workgroup barrier inside S.end, and
+  // the same wave's lanes apply M^-1 below, so a wave barrier suffices here.
+  if (GW > WAVE && S.rs) wsync();
+  else gsync<GW>();
   for (int b = lane; b < B; b += GW) {
     float* bb = sm + b * BS;
     stsv(bb + BVV, apply_minv(bb, ldsv(bb + BACC)));
@@ -570,7 +574,11 @@ __device__ __forceinline__ void schur_matvec(float* sm, int lane, int B, int n_r
     }
     out[k] = o;
   }
-  gsync<GW>();
+  // No trailing barrier for a world of several waves: the next writes of BACC and BVV
+  // come after the workgroup barrier inside the next S.end, which no wave passes before
+  // every wave has finished the gather above (and after the workgroup barriers of the
+  // PCR dot products in between); the lane-private copies are only read by their own wave.
+  if constexpr (GW <= WAVE) gsync<GW>();
 }
 
 // ---------------------------------------------------------------------------------
