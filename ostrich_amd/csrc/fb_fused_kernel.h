@@ -1117,12 +1117,15 @@ __host__ __device__ inline FusedLayout fused_layout(int B, int J, int rows_cap, 
 #ifndef OFB_MW_WAVES_PER_EU
 #define OFB_MW_WAVES_PER_EU 2
 #endif
+#ifndef OFB_RN_WAVES_PER_EU
+#define OFB_RN_WAVES_PER_EU 1
+#endif
 template <int R, bool BJ, int GW, bool DENSE>
 __global__ void __launch_bounds__(GW > 256 ? GW : 256)
 __attribute__((amdgpu_waves_per_eu(GW > WAVE ? OFB_MW_WAVES_PER_EU
                                    : R == 1 ? (DENSE ? (GW == 32 && !BJ ? OFB_PAIR_WAVES_PER_EU : OFB_DENSE_WAVES_PER_EU)
                                                      : OFB_R1_WAVES_PER_EU)
-                                            : 1)))
+                                            : OFB_RN_WAVES_PER_EU)))
 fused_step_kernel(FusedParams P) {
   extern __shared__ __align__(16) float lds[];
   static_assert(GW == 64 || (GW == 32 && R == 1) || ((GW == 128 || GW == 256 || GW == 512) && !DENSE && !BJ),

@@ -508,6 +508,7 @@ struct NormalStatic {
   v3 p0, p1, n;
   float th0, th1, precond;
   sv Ja, Jb;
+  v3 r0, r1;  // lever arms (contact point less the centre of mass) at the step-start poses
 };
 
 template <class PT, class PR = LdsPoses>
@@ -525,8 +526,9 @@ __device__ __forceinline__ NormalStatic normal_static(const PT& P, const float* 
   v3 p0_adj_prev = tpoint(q0p, g.p0) - g.n * g.th0;
   v3 p1_adj_prev = tpoint(q1p, g.p1) + g.n * g.th1;
   s.Ja = svzero(); s.Jb = svzero();
-  if (g.b0 >= 0) s.Ja = sv{g.n, cross(p0_adj_prev - tpoint(q0p, com0), g.n)};
-  if (g.b1 >= 0) s.Jb = sv{-g.n, cross(p1_adj_prev - tpoint(q1p, com1), -g.n)};
+  s.r0 = mk(0.f, 0.f, 0.f); s.r1 = s.r0;
+  if (g.b0 >= 0) { s.r0 = p0_adj_prev - tpoint(q0p, com0); s.Ja = sv{g.n, cross(s.r0, g.n)}; }
+  if (g.b1 >= 0) { s.r1 = p1_adj_prev - tpoint(q1p, com1); s.Jb = sv{-g.n, cross(s.r1, -g.n)}; }
   const float dt = P.dt;
   s.precond = dt * dt * eff_mass_prev(sm, g.b0, g.b1, s.Ja, s.Jb, pr);
   return s;
@@ -584,6 +586,7 @@ struct FrictionStatic {
   int closed;  // isotropic frame without anisotropy axes (d w / d mu has a closed form)
   sv Jt1a, Jt2a, Jt1b, Jt2b;
   float rr, mux, muy, gate_mu, mss, vst, lat;
+  v3 t1, t2;  // the friction frame
 };
 
 template <class PT, class PR = LdsPoses>
@@ -595,6 +598,7 @@ __device__ __forceinline__ FrictionStatic friction_static(const PT& P, const flo
   s.closed = 1;
   s.Jt1a = svzero(); s.Jt2a = svzero(); s.Jt1b = svzero(); s.Jt2b = svzero();
   s.rr = 0.f; s.mux = 0.f; s.muy = 0.f; s.gate_mu = 0.f; s.mss = -1.f; s.vst = -1.f; s.lat = -1.f;
+  s.t1 = mk(0.f, 0.f, 0.f); s.t2 = s.t1;
   bool skip = g.s0 == g.s1;
   if (P.cfg.friction_mode == OFB_FRICTION_BILATERAL_PATCH && !skip) {  // bilateral_row_is_skipped :39-76
     for (int jj = 0; jj < c; ++jj) {
@@ -641,6 +645,7 @@ __device__ __forceinline__ FrictionStatic friction_static(const PT& P, const flo
   }
   s.closed = (!an0 && !an1) ? 1 : 0;
   s.mux = mux; s.muy = muy;
+  s.t1 = t1; s.t2 = t2;
   // resolve_stribeck_params :159-176 / resolve_stribeck_lateral_only :179-194
   float ss0 = P.m.shape_mu_stiction_scale[w * SC + g.s0], vs0 = P.m.shape_v_stribeck[w * SC + g.s0];
   float ss1 = P.m.shape_mu_stiction_scale[w * SC + g.s1], vs1 = P.m.shape_v_stribeck[w * SC + g.s1];
@@ -667,56 +672,65 @@ __device__ __forceinline__ FrictionStatic friction_static(const PT& P, const flo
   return s;
 }
 
-// LDS record of a contact: 16 quads, the normal row's 28 floats then the friction rows'
-// 36, and one quad of padding: lanes read the records of different contacts with 128-bit
-// loads, and a 68-float stride puts eight consecutive lanes' quads on distinct banks (a
-// 64-float stride put them all on the same four, an eight-way conflict on every load)
-constexpr int CS_WORDS = 68;
-constexpr int CS_FRICTION = 28;
-__device__ __forceinline__ void st_sv2(float4* q, const sv& a, const sv& b) {
-  q[0] = make_float4(a.a.x, a.a.y, a.a.z, a.b.x);
-  q[1] = make_float4(a.b.y, a.b.z, b.a.x, b.a.y);
-  q[2] = make_float4(b.a.z, b.b.x, b.b.y, b.b.z);
-}
-__device__ __forceinline__ void ld_sv2(const float4* q, sv& a, sv& b) {
-  const float4 x = q[0], y = q[1], z = q[2];
-  a = sv{mk(x.x, x.y, x.z), mk(x.w, y.x, y.y)};
-  b = sv{mk(y.z, y.w, z.x), mk(z.y, z.z, z.w)};
-}
+// LDS record of a contact, 9 quads: bodies and skip flags; the geometry (body-frame
+// points, normal, thicknesses) and the normal row's preconditioner; the lever arms r0, r1
+// at the step-start poses; the friction frame t1, t2 and the friction row parameters.
+// The four Jacobians are rebuilt from them on load (normal (n, r x n), tangents (t, r x t)
+// and their negations for the second body, as the builders form them), which holds a
+// contact in 36 floats instead of 68.  Lanes read the records of different contacts with
+// 128-bit loads; an odd number of quads per record puts consecutive lanes' quads on
+// distinct banks.
+constexpr int CS_WORDS = 36;
 __device__ __forceinline__ void store_contact_static(float* d, const NormalStatic& n, const FrictionStatic& f) {
   float4* q = reinterpret_cast<float4*>(d);
-  q[0] = make_float4(__int_as_float(n.b0), __int_as_float(n.b1), __int_as_float(n.skip), 0.f);
+  q[0] = make_float4(__int_as_float(n.b0), __int_as_float(n.b1), __int_as_float(n.skip), __int_as_float(f.skip));
   q[1] = make_float4(n.p0.x, n.p0.y, n.p0.z, n.p1.x);
   q[2] = make_float4(n.p1.y, n.p1.z, n.n.x, n.n.y);
   q[3] = make_float4(n.n.z, n.th0, n.th1, n.precond);
-  st_sv2(q + 4, n.Ja, n.Jb);
-  q[7] = make_float4(__int_as_float(f.b0), __int_as_float(f.b1), __int_as_float(f.skip), __int_as_float(f.closed));
-  st_sv2(q + 8, f.Jt1a, f.Jt2a);
-  st_sv2(q + 11, f.Jt1b, f.Jt2b);
-  q[14] = make_float4(f.rr, f.mux, f.muy, f.gate_mu);
-  q[15] = make_float4(f.mss, f.vst, f.lat, 0.f);
+  q[4] = make_float4(n.r0.x, n.r0.y, n.r0.z, __int_as_float(f.closed));
+  q[5] = make_float4(n.r1.x, n.r1.y, n.r1.z, f.rr);
+  q[6] = make_float4(f.t1.x, f.t1.y, f.t1.z, f.mux);
+  q[7] = make_float4(f.t2.x, f.t2.y, f.t2.z, f.muy);
+  q[8] = make_float4(f.gate_mu, f.mss, f.vst, f.lat);
 }
 __device__ __forceinline__ NormalStatic load_normal_static(const float* d) {
   const float4* q = reinterpret_cast<const float4*>(d);
-  const float4 a = q[0], b = q[1], c = q[2], e = q[3];
+  const float4 a = q[0], b = q[1], c = q[2], e = q[3], r0 = q[4], r1 = q[5];
   NormalStatic s;
   s.b0 = __float_as_int(a.x); s.b1 = __float_as_int(a.y); s.skip = __float_as_int(a.z);
   s.p0 = mk(b.x, b.y, b.z);
   s.p1 = mk(b.w, c.x, c.y);
   s.n = mk(c.z, c.w, e.x);
   s.th0 = e.y; s.th1 = e.z; s.precond = e.w;
-  ld_sv2(q + 4, s.Ja, s.Jb);
+  s.r0 = mk(r0.x, r0.y, r0.z);
+  s.r1 = mk(r1.x, r1.y, r1.z);
+  s.Ja = svzero(); s.Jb = svzero();
+  if (s.b0 >= 0) s.Ja = sv{s.n, cross(s.r0, s.n)};
+  if (s.b1 >= 0) s.Jb = sv{-s.n, cross(s.r1, -s.n)};
   return s;
 }
 __device__ __forceinline__ FrictionStatic load_friction_static(const float* d) {
   const float4* q = reinterpret_cast<const float4*>(d);
-  const float4 a = q[7], b = q[14], c = q[15];
+  const float4 a = q[0], r0 = q[4], r1 = q[5], t1 = q[6], t2 = q[7], e = q[8];
   FrictionStatic s;
-  s.b0 = __float_as_int(a.x); s.b1 = __float_as_int(a.y); s.skip = __float_as_int(a.z); s.closed = __float_as_int(a.w);
-  ld_sv2(q + 8, s.Jt1a, s.Jt2a);
-  ld_sv2(q + 11, s.Jt1b, s.Jt2b);
-  s.rr = b.x; s.mux = b.y; s.muy = b.z; s.gate_mu = b.w;
-  s.mss = c.x; s.vst = c.y; s.lat = c.z;
+  s.b0 = __float_as_int(a.x); s.b1 = __float_as_int(a.y); s.skip = __float_as_int(a.w);
+  s.closed = __float_as_int(r0.w);
+  s.rr = r1.w; s.mux = t1.w; s.muy = t2.w;
+  s.gate_mu = e.x; s.mss = e.y; s.vst = e.z; s.lat = e.w;
+  s.t1 = mk(t1.x, t1.y, t1.z);
+  s.t2 = mk(t2.x, t2.y, t2.z);
+  s.Jt1a = svzero(); s.Jt2a = svzero(); s.Jt1b = svzero(); s.Jt2b = svzero();
+  if (!s.skip) {
+    const v3 ra = mk(r0.x, r0.y, r0.z), rb = mk(r1.x, r1.y, r1.z);
+    if (s.b0 >= 0) {
+      s.Jt1a = sv{s.t1, cross(ra, s.t1)};
+      s.Jt2a = sv{s.t2, cross(ra, s.t2)};
+    }
+    if (s.b1 >= 0) {
+      s.Jt1b = sv{-s.t1, -cross(rb, s.t1)};
+      s.Jt2b = sv{-s.t2, -cross(rb, s.t2)};
+    }
+  }
   return s;
 }
 
