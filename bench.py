@@ -161,8 +161,9 @@ def run_path(variant, model, state, control, steps, warmup, device, collider, K=
     if tune and path == "fused":
         con0 = contacts_of(collider, s_a.body_q, device)
         launch, times = eng.autotune(s_a, ctl, con0, DT, pairing=paired)
-        launch = dict(waves_per_block=launch[0], pair_small_worlds=launch[1],
-                      candidates_ms={f"{w}x{'paired' if p else 'single'}": round(t, 4) for (w, p), t in times.items()})
+        launch = dict(waves_per_block=launch[0], pair_small_worlds=launch[1], large_world_slots=launch[2],
+                      candidates_ms={f"{w}x{'paired' if p else 'single'}/{l}": round(t, 4)
+                                     for (w, p, l), t in times.items()})
         print(f"[bench] autotune {variant}: {launch}", file=sys.stderr, flush=True)
     class_timing = class_timing and path == "fused" and not graph
     if class_timing:
@@ -242,6 +243,7 @@ def full_step_leg(model, state, control, device, steps, warmup, launch=None):
     eng = OstrichEngine(model, config=cfg, device=device)
     if launch is not None:
         eng.set_fused_launch(launch["waves_per_block"], launch["pair_small_worlds"])
+        eng.set_fused_layout(launch["large_world_slots"])
     col = collide.HipCollider(model, device=device, gap=C1_GAP, max_contacts_per_world=K_CONTACTS)
     s_a = State.from_numpy(state, device)
     s_b = s_a.clone()
@@ -284,16 +286,17 @@ def model_rows(model):
 
 
 def scene_leg(name, model_fn, worlds, K, steps, warmup, device, variants=("fused:jacobi:prev",), gap=0.02,
-              hf_contacts="point"):
+              hf_contacts="point", tune=True):
     """Another workload through the same solver paths (a parity-test configuration, not
-    the headline): world-steps/s, Newton counts and contact counts per variant."""
+    the headline): world-steps/s, Newton counts and contact counts per variant, each
+    fused variant with the launch settings the autotuner picks for this workload."""
     model, state, control = model_fn(worlds)
     collider = collide.HipCollider(model, device=device, gap=gap, max_contacts_per_world=K, hf_contacts=hf_contacts)
     out = dict(worlds=worlds, contact_slots=K, gap=gap, heightfield_contacts=hf_contacts, steps=steps,
                warmup=warmup)
     for v in variants:
         print(f"[bench] {name} path={v} worlds={worlds}", file=sys.stderr, flush=True)
-        out[v] = run_path(v, model, state, control, steps, warmup, device, collider, K=K)
+        out[v] = run_path(v, model, state, control, steps, warmup, device, collider, K=K, tune=tune)
         print(f"[bench] {name} {v}: {out[v]}", file=sys.stderr, flush=True)
     return out
 
@@ -693,7 +696,7 @@ def main():
             col = collide.HipCollider(model, device=device, gap=0.02, max_contacts_per_world=16)
             K = 16
         res = run_path("fused:jacobi:prev", model, state, control, args.steps, args.warmup, device, col, K=K,
-                       class_timing=True)
+                       class_timing=True, tune=not args.no_tune)
         print(json.dumps(dict(workload=args.scene_only, worlds=model.num_worlds, **res)))
         return
     W = args.worlds

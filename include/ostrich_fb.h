@@ -174,6 +174,12 @@ typedef struct ofb_config {
   int32_t adjoint_soft_blending;   /* adjoint.soft_blending: sigmoid between the two
                                       compliances instead of the 0.9 threshold */
   float adjoint_soft_temperature;  /* adjoint.soft_blending_temperature */
+  int32_t large_world_slots;       /* fused path, Jacobi: row slots per lane of the worlds of
+                                      more than 256 rows; 2 (or 0): 2, 4 or 8 wavefronts per
+                                      world, the shortest solve of one world; 4: half the
+                                      wavefronts, about twice as many large worlds resident
+                                      at once.  No reference counterpart (a launch layout);
+                                      like pair_small_worlds a rounding-level difference */
 } ofb_config;
 
 typedef struct ofb_state_in {
@@ -410,13 +416,20 @@ int ofb_get_class_shapes(const ofb_solver* solver, int32_t capacity, int32_t* sl
                          int32_t* lanes_per_world, int32_t* max_rows, int32_t* count);
 
 /* Launch settings of the fused path: the largest number of wavefronts per workgroup of a
- * row-class launch (1..16; the LDS limit may lower it per class) and whether worlds of at
+ * row-class launch (1..16; the kernels' 256-thread bound caps it at 4, and the LDS limit
+ * may lower it per class) and whether worlds of at
  * most 32 rows run two per wavefront.  No reference counterpart (Warp chooses its own
  * launch shapes); the autotuner (OstrichEngine.autotune) sets them from measurements.
  * The workgroup size leaves results bit-identical; pairing changes the width of the
  * lane-group reductions, a rounding-level difference.  Fused path only (-2 otherwise). */
 int ofb_set_fused_launch(ofb_solver* solver, int32_t waves_per_block, int32_t pair_small_worlds);
 int ofb_get_fused_launch(const ofb_solver* solver, int32_t* waves_per_block, int32_t* pair_small_worlds);
+
+/* Layout of the large worlds of the fused Jacobi path: ofb_config.large_world_slots (2 or
+ * 4; -2 otherwise), applied to the next step.  No reference counterpart; the autotuner
+ * (OstrichEngine.autotune) chooses it with the launch settings.  Fused path only. */
+int ofb_set_fused_layout(ofb_solver* solver, int32_t large_world_slots);
+int ofb_get_fused_layout(const ofb_solver* solver, int32_t* large_world_slots);
 
 /* One Adam step on a control trajectory params[steps][worlds][dofs] (device, f32), moments
  * m and v of the same shape (zero before the first step), for the dof slice

@@ -105,6 +105,7 @@ class OfbConfig(C.Structure):
         ("adjoint_friction_frozen", C.c_int32),
         ("adjoint_soft_blending", C.c_int32),
         ("adjoint_soft_temperature", C.c_float),
+        ("large_world_slots", C.c_int32),
     ]
 
 

@@ -20,7 +20,7 @@ EXPORTS = (
     "ofb_log_create", "ofb_log_capture", "ofb_log_destroy", "ofb_set_class_timing", "ofb_get_class_times",
     "ofb_get_class_shapes", "ofb_set_keep_lambda_prev",
     "ofb_warm_start_state_size", "ofb_warm_start_state_get", "ofb_warm_start_state_set",
-    "ofb_mlp_gradient_noalloc",
+    "ofb_mlp_gradient_noalloc", "ofb_set_fused_layout", "ofb_get_fused_layout",
 )
 
 _lib = None
@@ -53,6 +53,8 @@ def load():
     lib.ofb_get_class_times.argtypes = [vp, i32, vp, vp, vp, vp]
     lib.ofb_get_class_shapes.argtypes = [vp, i32, vp, vp, vp, vp]
     lib.ofb_get_fused_launch.argtypes = [vp, vp, vp]
+    lib.ofb_set_fused_layout.argtypes = [vp, i32]
+    lib.ofb_get_fused_layout.argtypes = [vp, vp]
     f32 = C.c_float
     lib.ofb_trajectory_adam_step.argtypes = [vp, vp, vp, vp] + [i32] * 5 + [f32] * 5 + [i32, vp]
     lib.ofb_collider_create.argtypes = [vp, C.POINTER(vp)]

@@ -1,9 +1,9 @@
 """Builds the gfx950 solver library ostrich_amd/_lib/libostrich_fb.so with hipcc.
 
 Each translation unit compiles to an object under ostrich_amd/_build/ in parallel; the
-fused kernel's instances are split over four units (fb_fused_classes.hip compiled with
-OFB_BJ x OFB_RSET), then hipcc links the shared library.  Packed-FP32 instructions are
-disabled for every device pass (-Xclang -target-feature -Xclang -packed-fp32-ops).
+fused kernel's instances are split over five units (fb_fused_classes.hip compiled with
+OFB_BJ x OFB_RSET, and the Jacobi wide-layout set), then hipcc links the shared library.
+Packed-FP32 instructions are disabled for every device pass (-Xclang -target-feature -Xclang -packed-fp32-ops).
 Extra hipcc flags may be given in OFB_HIPCC_FLAGS (e.g. -DOFB_PHASE_TIMING).
 """
 import os
@@ -30,6 +30,7 @@ UNITS = [
     ("fb_log", "fb_log.hip", []),
     ("fb_fused_j0", "fb_fused_classes.hip", ["-DOFB_BJ=0", "-DOFB_RSET=0"]),
     ("fb_fused_j1", "fb_fused_classes.hip", ["-DOFB_BJ=0", "-DOFB_RSET=1"]),
+    ("fb_fused_j2", "fb_fused_classes.hip", ["-DOFB_BJ=0", "-DOFB_RSET=2"]),
     ("fb_fused_b0", "fb_fused_classes.hip", ["-DOFB_BJ=1", "-DOFB_RSET=0"]),
     ("fb_fused_b1", "fb_fused_classes.hip", ["-DOFB_BJ=1", "-DOFB_RSET=1"]),
     ("fb_adj_j0", "fb_adjoint_classes.hip", ["-DOFB_BJ=0", "-DOFB_RSET=0"]),

@@ -273,11 +273,16 @@ class ExecutionConfig:
     pair_small_worlds: fused path - worlds of at most 32 constraint rows run two per
                  wavefront (32 lanes each) and solve PCR on the assembled Schur matrix;
                  False keeps one world per wavefront and the matrix-free operator.
+    large_world_slots: fused path, Jacobi - row slots per lane of the worlds of more than
+                 256 constraint rows: 2 spreads such a world over 2, 4 or 8 wavefronts
+                 (the shortest solve of one world), 4 over half as many (more of them
+                 resident at once: the faster layout for batches of many large worlds).
     """
 
     path: str = "fused"
     termination: str = "per_world"
     pair_small_worlds: bool = True
+    large_world_slots: int = 2
 
     def __post_init__(self):
         if self.path not in ("fused", "plain"):
@@ -286,6 +291,8 @@ class ExecutionConfig:
             raise ValueError(
                 f"execution.termination must be 'per_world' or 'global', got {self.termination!r}"
             )
+        if self.large_world_slots not in (2, 4):
+            raise ValueError(f"execution.large_world_slots must be 2 or 4, got {self.large_world_slots!r}")
 
 
 _SUBS = (
@@ -384,6 +391,7 @@ class OstrichEngineConfig:
         c.linesearch_optimistic_count = ls.optimistic_step_count
         c.linesearch_optimistic_window = ls.optimistic_window
         c.pair_small_worlds = 1 if self.execution.pair_small_worlds else 0
+        c.large_world_slots = int(self.execution.large_world_slots)
         red = self.contacts.reduction
         c.contact_reduction = REDUCTION_POLICIES[red.policy]
         c.contact_max_per_pair = red.max_per_pair

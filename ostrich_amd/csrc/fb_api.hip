@@ -284,6 +284,10 @@ int ofb_create(const ofb_model* model, const ofb_config* config, int32_t path, o
     set_error("unknown contact reduction policy %d", config->contact_reduction);
     return -2;
   }
+  if (config->large_world_slots != 0 && config->large_world_slots != 2 && config->large_world_slots != 4) {
+    set_error("large_world_slots must be 2 or 4 (got %d)", config->large_world_slots);
+    return -2;
+  }
   if (config->warm_start_method != OFB_WS_POSITION_MATCH && config->warm_start_method != OFB_WS_PAIR_AGGREGATE) {
     set_error("unknown warm-start method %d", config->warm_start_method);
     return -2;
@@ -312,6 +316,7 @@ int ofb_create(const ofb_model* model, const ofb_config* config, int32_t path, o
   ofb_solver* s = new ofb_solver;
   std::memset(s, 0, sizeof(*s));
   s->cfg = *config;
+  if (s->cfg.large_world_slots == 0) s->cfg.large_world_slots = 2;
   s->path = path;
   s->keep_lambda_prev = 1;
   DeviceModel& m = s->m;
@@ -684,6 +689,18 @@ int ofb_set_fused_launch(ofb_solver* s, int32_t waves_per_block, int32_t pair_sm
   if (!s) { set_error("null solver"); return -1; }
   if (s->path != OFB_PATH_FUSED) { set_error("launch settings apply to the fused path only"); return -2; }
   return fused_set_launch(s, waves_per_block, pair_small_worlds);
+}
+
+int ofb_set_fused_layout(ofb_solver* s, int32_t large_world_slots) {
+  if (!s) { set_error("null solver"); return -1; }
+  if (s->path != OFB_PATH_FUSED) { set_error("the large-world layout applies to the fused path only"); return -2; }
+  return fused_set_layout(s, large_world_slots);
+}
+
+int ofb_get_fused_layout(const ofb_solver* s, int32_t* large_world_slots) {
+  if (!s) { set_error("null solver"); return -1; }
+  if (large_world_slots) *large_world_slots = s->cfg.large_world_slots;
+  return 0;
 }
 
 int ofb_set_iteration_history(ofb_solver* s, float* merit, int32_t* pcr_iters, int32_t capacity) {

@@ -14,8 +14,8 @@ namespace ofb {
 //  * code 1: at most 64 rows, one world per wavefront, dense;
 //  * code R = 2..16: at most 64 R rows, one world per wavefront, R row slots per lane,
 //    matrix-free operator;
-//  * code kMW + NW: at most 128 NW rows, one world per workgroup of NW wavefronts, two
-//    row slots per lane, matrix-free (Jacobi only).
+//  * code multiwave_code(NW, R): at most 64 NW R rows, one world per workgroup of NW
+//    wavefronts, R = 2 or 4 row slots per lane, matrix-free (Jacobi only).
 // A world runs in the first class whose capacity holds it.  Tables:
 //  * Jacobi forward: the paired and dense classes, R = 2, then worlds of 2, 4 and 8
 //    waves.  Large worlds set a step's duration (a wave's time is its slowest world's),
@@ -62,13 +62,29 @@ struct ClassTable {
 #define OFB_JACOBI_CODES 0, 1, 2, kMW + 2, kMW + 4, kMW + 8
 #endif
 static const ClassTable kTabJacobi = {6, {32, 64, 128, 256, 512, 1024}, {OFB_JACOBI_CODES}, {OFB_JACOBI_SIDES}, 6};
+// The wide layout of large worlds (large_world_slots = 4): worlds of 257-1024 rows on half
+// the wavefronts, four row slots per lane.  A world's solve takes longer, but a batch
+// of many large worlds keeps about twice as many of them resident, so it is the faster
+// one when such worlds outnumber what the chip holds at once (contact-rich tracked robots)
+// and the slower one when a few of them set the step (the uneven skid-steer batch).  The
+// eight-wave class of the default layout becomes a four-wave one; it starts first on the
+// step's stream (its worlds need the most LDS), the 257-512-row class runs beside it.
+#ifndef OFB_JACOBI_WIDE_SIDES
+#define OFB_JACOBI_WIDE_SIDES -1, 0, 0, 1, 2, -1
+#endif
+#ifndef OFB_JACOBI_WIDE_CODES
+#define OFB_JACOBI_WIDE_CODES 0, 1, 2, multiwave_code(2, 2), multiwave_code(2, 4), multiwave_code(4, 4)
+#endif
+static const ClassTable kTabJacobiWide = {6, {32, 64, 128, 256, 512, 1024}, {OFB_JACOBI_WIDE_CODES},
+                                          {OFB_JACOBI_WIDE_SIDES}, 6};
 static const ClassTable kTabBlockJacobi = {10, {32, 64, 128, 192, 256, 320, 384, 448, 768, 1024},
                                            {0, 1, 2, 3, 4, 5, 6, 7, 12, 16}, {-1, 0, 1, 2, 2, 2, 2, 2, 2, 2}, 3};
 static const ClassTable kTabAdjoint = {10, {32, 64, 128, 192, 256, 320, 384, 448, 768, 1024},
                                        {0, 1, 2, 3, 4, 5, 6, 7, 12, 16}, {-1, 0, 0, 0, 0, 0, 0, 0, 0, 0}, 10};
 
 static const ClassTable& fwd_table(const ofb_solver* s) {
-  return s->cfg.preconditioner == OFB_PRECOND_PER_BODY_PAIR ? kTabBlockJacobi : kTabJacobi;
+  if (s->cfg.preconditioner == OFB_PRECOND_PER_BODY_PAIR) return kTabBlockJacobi;
+  return s->cfg.large_world_slots == 4 ? kTabJacobiWide : kTabJacobi;
 }
 
 // the capacities passed to the classifier by value
@@ -156,8 +172,10 @@ static bool class_launched(const ofb_solver* s, int c) {
 }
 
 // lanes per world, row slots per lane and row capacity of a kernel instance
-static int class_lanes(int code) { return code == 0 ? kPairRows : code > kMW ? WAVE * (code - kMW) : WAVE; }
-static int class_slots(int code) { return code == 0 ? 1 : code > kMW ? 2 : code; }
+static int class_lanes(int code) {
+  return code == 0 ? kPairRows : code > kMW ? WAVE * ((code - kMW) % kMultiWaveR) : WAVE;
+}
+static int class_slots(int code) { return code == 0 ? 1 : code > kMW ? 2 + (code - kMW) / kMultiWaveR : code; }
 static int class_rows(int code) { return class_lanes(code) * class_slots(code); }
 
 // LDS layout of one world of a class: a world of the class has at most
@@ -172,15 +190,17 @@ static FusedLayout class_layout(const ofb_solver* s, int code) {
 static int lds_words(const ofb_solver* s, int code) { return class_layout(s, code).words; }
 
 // worlds per workgroup of a class: a world of several waves has its workgroup to itself;
-// otherwise at most the tuned number of waves' worth, a whole number of waves, within
-// 64 KB of LDS per workgroup, and among those the choice that keeps the most worlds
-// resident in a CU's 160 KB (larger workgroups on ties)
+// otherwise at most the tuned number of waves' worth and at most kGroupThreads threads
+// (the instances' launch bound), a whole number of waves, within 64 KB of LDS per
+// workgroup, and among those the choice that keeps the most worlds resident in a CU's
+// 160 KB (larger workgroups on ties)
 static int choose_wpb(const ofb_solver* s, int code) {
   if (class_lanes(code) > WAVE) return 1;
   const int per_wave = WAVE / class_lanes(code);
   const size_t world_bytes = (size_t)lds_words(s, code) * sizeof(float);
   int best = per_wave, best_res = -1;
-  for (int n = 1; n <= s->fused_worlds_per_block; ++n) {
+  const int max_waves = std::min(s->fused_worlds_per_block, kGroupThreads / WAVE);
+  for (int n = 1; n <= max_waves; ++n) {
     const int wpb = n * per_wave;
     const size_t bytes = wpb * world_bytes;
     if (bytes > 65536 && n > 1) break;
@@ -214,6 +234,7 @@ static int class_op(int code, int op, const FusedParams& P, int blocks, int thre
   if (rc == -1)
     rc = bj ? fused_class_op_bj1_set1(code, op, P, blocks, threads, lds, st)
             : fused_class_op_bj0_set1(code, op, P, blocks, threads, lds, st);
+  if (rc == -1 && !bj) rc = fused_class_op_bj0_set2(code, op, P, blocks, threads, lds, st);
   return rc;
 }
 
@@ -224,6 +245,10 @@ static int launch_fused(int code, FusedParams P, const ofb_solver* s, int cls, h
   P.lay = class_layout(s, code);
   const int wpb = s->cls_wpb[cls];
   const int per_block = class_lanes(code) > WAVE ? class_lanes(code) : wpb * class_lanes(code);
+  if (per_block > (class_lanes(code) > WAVE ? class_lanes(code) : kGroupThreads)) {
+    set_error("fused launch of %d threads per workgroup exceeds the instance's bound", per_block);
+    return -4;
+  }
   size_t lds_bytes = (size_t)wpb * P.lay.words * sizeof(float);
   int blocks = (P.m.W + wpb - 1) / wpb;
   return class_op(code, 0, P, blocks, per_block, lds_bytes, stream,
@@ -292,6 +317,17 @@ int fused_set_launch(ofb_solver* s, int waves_per_block, int pair_small_worlds) 
   }
   s->fused_worlds_per_block = waves_per_block;
   s->cfg.pair_small_worlds = pair_small_worlds ? 1 : 0;
+  return fused_configure(s);
+}
+
+// Row slots per lane of the Jacobi worlds of more than 256 rows (2: the default layout,
+// 4: the wide one); like pairing, a rounding-level difference (other reduction widths).
+int fused_set_layout(ofb_solver* s, int large_world_slots) {
+  if (large_world_slots != 2 && large_world_slots != 4) {
+    set_error("large_world_slots must be 2 or 4 (got %d)", large_world_slots);
+    return -2;
+  }
+  s->cfg.large_world_slots = large_world_slots;
   return fused_configure(s);
 }
 
@@ -451,8 +487,8 @@ int fused_step(ofb_solver* s, const ofb_state_in* in, const ofb_state_out* out, 
       set_error("fused step: event record failed");
       return -4;
     }
-    if (launch_fused(launch_code(s, c), P, s, c, side) != 0) {
-      set_error("no fused kernel instance for class code %d", launch_code(s, c));
+    if (const int rc = launch_fused(launch_code(s, c), P, s, c, side)) {
+      if (rc == -1) set_error("no fused kernel instance for class code %d", launch_code(s, c));
       return -4;
     }
     if (cls_timed) {
@@ -476,9 +512,11 @@ int fused_step(ofb_solver* s, const ofb_state_in* in, const ofb_state_out* out, 
     set_error("fused step: event record failed");
     return -4;
   }
-  if (class_launched(s, 0) && launch_fused(t.code[0], P, s, 0, stream) != 0) {
-    set_error("no fused kernel instance for class code %d", t.code[0]);
-    return -4;
+  if (class_launched(s, 0)) {
+    if (const int rc = launch_fused(t.code[0], P, s, 0, stream)) {
+      if (rc == -1) set_error("no fused kernel instance for class code %d", t.code[0]);
+      return -4;
+    }
   }
   if (timed && hipEventRecord(s->ev_kernel1, stream) != hipSuccess) {
     set_error("fused step: event record failed");

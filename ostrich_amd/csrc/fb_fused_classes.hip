@@ -4,12 +4,13 @@
 // dense classes (code 0: two worlds per wave, code 1: one), R = 2 and, for block-Jacobi,
 // R = 3..5; OFB_RSET = 1 the block-Jacobi classes R = 6, 7, 12 and 16, or the Jacobi
 // worlds of 2, 4 and 8 waves (Jacobi solvers run worlds of more than 128 rows on several
-// waves, block-Jacobi ones on one wave with more row slots per lane).
+// waves, block-Jacobi ones on one wave with more row slots per lane); OFB_RSET = 2 (Jacobi
+// only) the worlds of 2 and 4 waves with four row slots per lane (the wide layout).
 
 #include "fb_fused_kernel.h"
 
 #if !defined(OFB_BJ) || !defined(OFB_RSET)
-#error "compile with -DOFB_BJ=0|1 -DOFB_RSET=0|1"
+#error "compile with -DOFB_BJ=0|1 -DOFB_RSET=0|1|2"
 #endif
 
 #define OFB_CAT_(a, b, c, d) a##b##c##d
@@ -29,12 +30,15 @@ int OFB_CAT(fused_class_op_bj, OFB_BJ, _set, OFB_RSET)(int code, int op, const F
 #if OFB_BJ != 0
     CASE(3) CASE(4) CASE(5)
 #endif
-#elif OFB_BJ != 0
+#elif OFB_RSET == 1 && OFB_BJ != 0
     CASE(6) CASE(7) CASE(12) CASE(16)
-#else
-    case kMultiWave + 2: return fused_class_op<2, false, 128, false>(op, P, blocks, threads, lds, st);
-    case kMultiWave + 4: return fused_class_op<2, false, 256, false>(op, P, blocks, threads, lds, st);
-    case kMultiWave + 8: return fused_class_op<2, false, 512, false>(op, P, blocks, threads, lds, st);
+#elif OFB_RSET == 1
+    case multiwave_code(2, 2): return fused_class_op<2, false, 128, false>(op, P, blocks, threads, lds, st);
+    case multiwave_code(4, 2): return fused_class_op<2, false, 256, false>(op, P, blocks, threads, lds, st);
+    case multiwave_code(8, 2): return fused_class_op<2, false, 512, false>(op, P, blocks, threads, lds, st);
+#elif OFB_BJ == 0
+    case multiwave_code(2, 4): return fused_class_op<4, false, 128, false>(op, P, blocks, threads, lds, st);
+    case multiwave_code(4, 4): return fused_class_op<4, false, 256, false>(op, P, blocks, threads, lds, st);
 #endif
 #undef CASE
   }

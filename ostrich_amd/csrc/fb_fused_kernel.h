@@ -38,13 +38,14 @@ using namespace rows;
 #ifdef OFB_PHASE_TIMING
 // Profiling build only: per-phase cycle counts of the Newton loop, summed over worlds,
 // per instance (row 0: paired, 1: dense, R: matrix-free with R slots per lane, 13-15:
-// worlds of 2, 4, 8 waves); columns 8 and 9 count Newton and PCR iterations.
+// worlds of 2, 4, 8 waves with two slots per lane, 9-11: with four); columns 8 and 9
+// count Newton and PCR iterations.
 __device__ unsigned long long g_phase_cycles[17][10];
 #define OFB_PHASE_INIT unsigned long long ph_[8] = {0, 0, 0, 0, 0, 0, 0, 0}; unsigned long long ph_t = clock64();
 #define OFB_PHASE(k) { unsigned long long t_ = clock64(); ph_[k] += t_ - ph_t; ph_t = t_; }
 #define OFB_PHASE_FLUSH                                                                       \
   if (lane == 0) {                                                                            \
-    const int ci_ = DENSE ? (GW == 32 ? 0 : 1) : GW > WAVE ? (GW == 128 ? 13 : GW == 256 ? 14 : 15) : R; \
+    const int ci_ = DENSE ? (GW == 32 ? 0 : 1) : GW > WAVE ? (GW == 128 ? 13 : GW == 256 ? 14 : 15) - (R - 2) * 2 : R; \
     for (int k_ = 0; k_ < 8; ++k_) atomicAdd(&g_phase_cycles[ci_][k_], ph_[k_]);              \
     atomicAdd(&g_phase_cycles[ci_][8], (unsigned long long)n_it);                             \
     atomicAdd(&g_phase_cycles[ci_][9], (unsigned long long)*pcr_count);                       \
@@ -1120,16 +1121,21 @@ __host__ __device__ inline FusedLayout fused_layout(int B, int J, int rows_cap, 
 #ifndef OFB_RN_WAVES_PER_EU
 #define OFB_RN_WAVES_PER_EU 1
 #endif
+// threads of a workgroup of one-wave (or two-worlds-per-wave) worlds at most: the
+// instances are compiled for it, and the launches keep to it
+constexpr int kGroupThreads = 256;
 template <int R, bool BJ, int GW, bool DENSE>
-__global__ void __launch_bounds__(GW > 256 ? GW : 256)
+__global__ void __launch_bounds__(GW > kGroupThreads ? GW : kGroupThreads)
 __attribute__((amdgpu_waves_per_eu(GW > WAVE ? OFB_MW_WAVES_PER_EU
                                    : R == 1 ? (DENSE ? (GW == 32 && !BJ ? OFB_PAIR_WAVES_PER_EU : OFB_DENSE_WAVES_PER_EU)
                                                      : OFB_R1_WAVES_PER_EU)
                                             : OFB_RN_WAVES_PER_EU)))
 fused_step_kernel(FusedParams P) {
   extern __shared__ __align__(16) float lds[];
-  static_assert(GW == 64 || (GW == 32 && R == 1) || ((GW == 128 || GW == 256 || GW == 512) && !DENSE && !BJ),
-                "two worlds per wave hold one row slot per lane; worlds of several waves are matrix-free, Jacobi");
+  static_assert(GW == 64 || (GW == 32 && R == 1) ||
+                    ((GW == 128 || GW == 256 || GW == 512) && (R == 2 || R == 4) && !DENSE && !BJ),
+                "two worlds per wave hold one row slot per lane; worlds of several waves are matrix-free, Jacobi, "
+                "with two or four row slots per lane");
   const int lane = threadIdx.x & (GW - 1);
   const int grp = threadIdx.x / GW;
   const int idx = blockIdx.x * (blockDim.x / GW) + grp;
@@ -1538,11 +1544,14 @@ fused_step_kernel(FusedParams P) {
   }
 }
 
-// Class code of the instance for worlds spread over NW waves, two row slots per lane:
-// kMultiWave + NW (NW = 2, 4, 8).  (One slot per lane on twice the waves was slower:
-// 1.70 against 1.57 ms per replayed step on the uneven skid-steer batch, 5.2 against 4.5
-// ms on the tracked robots; the extra barriers outweigh the shorter lanes.)
+// Class code of the instance for worlds spread over NW waves with R row slots per lane:
+// kMultiWave + NW + kMultiWaveR (R - 2) (NW = 2, 4, 8; R = 2, or 4 for the wide layout
+// of large worlds).  (One slot per lane on twice the waves was slower: 1.70 against 1.57
+// ms per replayed step on the uneven skid-steer batch, 5.2 against 4.5 ms on the tracked
+// robots; the extra barriers outweigh the shorter lanes.)
 constexpr int kMultiWave = 100;
+constexpr int kMultiWaveR = 32;
+__host__ __device__ constexpr int multiwave_code(int nw, int r) { return kMultiWave + nw + kMultiWaveR * (r - 2); }
 
 // Host side of one class instance: op 0 launches it, op 1 raises its dynamic-LDS limit
 // to `lds` bytes.  Instances are compiled in several translation units
@@ -1560,10 +1569,11 @@ int fused_class_op(int op, const FusedParams& P, int blocks, int threads, size_t
 // return -1 when the class code is not in that unit's set.  Class code 0 is the
 // two-worlds-per-wave dense instance (at most 32 rows), code 1 the one-world-per-wave
 // dense instance (at most 64 rows), code R >= 2 the one-world-per-wave matrix-free
-// instance with R row slots per lane, code kMultiWave + NW the instance whose world spans
-// NW waves (Jacobi only).
+// instance with R row slots per lane, code multiwave_code(NW, R) the instance whose world
+// spans NW waves (Jacobi only).
 int fused_class_op_bj0_set0(int R, int op, const FusedParams& P, int blocks, int threads, size_t lds, hipStream_t st);
 int fused_class_op_bj0_set1(int R, int op, const FusedParams& P, int blocks, int threads, size_t lds, hipStream_t st);
+int fused_class_op_bj0_set2(int R, int op, const FusedParams& P, int blocks, int threads, size_t lds, hipStream_t st);
 int fused_class_op_bj1_set0(int R, int op, const FusedParams& P, int blocks, int threads, size_t lds, hipStream_t st);
 int fused_class_op_bj1_set1(int R, int op, const FusedParams& P, int blocks, int threads, size_t lds, hipStream_t st);
 

@@ -128,6 +128,7 @@ int fused_step(ofb_solver* s, const ofb_state_in* in, const ofb_state_out* out, 
                hipStream_t stream);
 int fused_classes_alloc(ofb_solver* s);
 int fused_set_launch(ofb_solver* s, int waves_per_block, int pair_small_worlds);
+int fused_set_layout(ofb_solver* s, int large_world_slots);
 int fused_class_times(const ofb_solver* s, int capacity, int32_t* rows_cap, int32_t* worlds, float* ms,
                       int32_t* count);
 int fused_class_shapes(const ofb_solver* s, int capacity, int32_t* slots, int32_t* lanes, int32_t* max_rows,

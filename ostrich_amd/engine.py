@@ -304,42 +304,73 @@ class OstrichEngine:
         _lib.check(self.lib.ofb_set_fused_launch(self._h, int(waves_per_block), 1 if pair_small_worlds else 0),
                    "ofb_set_fused_launch")
 
+    def fused_layout(self) -> int:
+        """Row slots per lane of the fused path's worlds of more than 256 rows (2 or 4)."""
+        v = C.c_int32()
+        _lib.check(self.lib.ofb_get_fused_layout(self._h, C.byref(v)), "ofb_get_fused_layout")
+        return v.value
+
+    def set_fused_layout(self, large_world_slots: int):
+        _lib.check(self.lib.ofb_set_fused_layout(self._h, int(large_world_slots)), "ofb_set_fused_layout")
+
     def autotune(self, state_in: State, control: Control, contacts: Contacts, dt: float, candidates=None,
-                 repeats=3, pairing=True):
+                 repeats=3, pairing=True, layouts=None):
         """Chooses the fused launch settings for this batch from measurements.
 
-        Each candidate (waves per workgroup in {1, 2, 4, 8}, and with `pairing` small
-        worlds two per wavefront or one) runs one warm-up step and `repeats` timed steps of
-        the given inputs on a scratch solver built from the same model and config, timed
-        with HIP events on the current stream; the fastest is applied to this solver.
-        This solver's state (warm-start snapshot, last results) is left as it was.
-        Returns (best settings, {settings: best ms})."""
+        Each candidate (waves per workgroup in {1, 2, 4}; with `pairing` small worlds
+        two per wavefront or one; with `layouts` the row slots per lane of the worlds of
+        more than 256 rows, 2 or 4) runs one warm-up step and `repeats` timed steps of the
+        given inputs on a scratch solver built from the same model and config, timed with
+        HIP events on the current stream; the fastest is applied to this solver.  By
+        default the wide layout (4) is a candidate on the Jacobi path when the batch holds
+        at least 256 worlds of more than 256 rows (about one per CU); with fewer, they are
+        all resident at once in either layout, and the default one solves each faster.  Every step starts from the same warm-start
+        snapshot (this solver's), so the candidates solve the same systems from the same
+        seeds.  This solver's state (warm-start snapshot, last results) is left as it was.
+        Returns (best (waves_per_block, pair_small_worlds, large_world_slots),
+        {candidate: best ms})."""
         if self.config.execution.path != "fused":
             raise ValueError("autotune applies to the fused path")
-        if candidates is None:
-            pairs = (True, False) if pairing else (self.fused_launch()[1],)
-            candidates = [(w, p) for p in pairs for w in (1, 2, 4, 8)]
         scratch = OstrichEngine(self.model, config=self.config, device=self.device,
                                 rigid_contact_max=self.rigid_contact_max)
         out = state_in.clone()
         stream = torch.cuda.current_stream(self.device)
+        snapshot = self.warm_start_state()
+        if layouts is None:
+            layouts = (self.fused_layout(),)
+            if self.config.linear.preconditioner_type == "jacobi":
+                scratch.set_warm_start_state(snapshot)
+                scratch.step(state_in, out, control, contacts, dt)  # packs this batch's contacts
+                K = self.config.contacts.max_per_world
+                rows = (self.num_constraints - 3 * K) + 3 * scratch.packed_contacts()["count"]
+                if int((rows > 256).sum().item()) >= 256:
+                    layouts = (2, 4)
+        if candidates is None:
+            pairs = (True, False) if pairing else (self.fused_launch()[1],)
+            candidates = [(w, p, l) for l in layouts for p in pairs for w in (1, 2, 4)]
         times = {}
-        for wpb, pair in candidates:
+        for wpb, pair, slots in candidates:
             scratch.set_fused_launch(wpb, pair)
+            scratch.set_fused_layout(slots)
+            if snapshot is not None:
+                scratch.set_warm_start_state(snapshot)
             scratch.step(state_in, out, control, contacts, dt)
             best = float("inf")
             for _ in range(repeats):
+                if snapshot is not None:
+                    scratch.set_warm_start_state(snapshot)
                 e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
                 e0.record(stream)
                 scratch.step(state_in, out, control, contacts, dt)
                 e1.record(stream)
                 e1.synchronize()
                 best = min(best, e0.elapsed_time(e1))
-            times[(wpb, bool(pair))] = best
+            times[(wpb, bool(pair), int(slots))] = best
         torch.cuda.synchronize(self.device)
         del scratch
         choice = min(times, key=times.get)
-        self.set_fused_launch(*choice)
+        self.set_fused_launch(choice[0], choice[1])
+        self.set_fused_layout(choice[2])
         return choice, times
 
     def capture(self, state_in: State, state_out: State, control: Control, contacts: Optional[Contacts], dt: float,

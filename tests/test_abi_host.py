@@ -65,6 +65,11 @@ def test_config_validation_mirrors_reference():
     assert fz.adjoint_friction_frozen == 1 and fz.adjoint_soft_blending == 0
     assert OstrichEngineConfig(warm_start={"method": "pair_aggregate"}).to_abi().warm_start_method == \
         _abi.OFB_WS_PAIR_AGGREGATE
+    # the large-world layout (no reference counterpart) reaches the ABI; 2 or 4 only
+    assert OstrichEngineConfig(execution={"large_world_slots": 4}).to_abi().large_world_slots == 4
+    assert OstrichEngineConfig().to_abi().large_world_slots == 2
+    with pytest.raises(ValueError):
+        ExecutionConfig(large_world_slots=3)
 
 
 def test_engine_requires_gpu_on_host():

@@ -464,6 +464,53 @@ def test_tracked_flipper_fixed_newton_iterations(oracle_mod, gpu_device, nr_iter
             assert eg.max() <= 4.0 * ep.max() + 1e-4 * sc, (key, eg.max(), ep.max())
 
 
+@pytest.mark.parametrize("slots", [2, 4])
+def test_large_world_layouts_match_oracle(oracle_mod, gpu_device, slots):
+    """Both layouts of the worlds of more than 256 rows (execution.large_world_slots: 2
+    row slots per lane on 4 or 8 wavefronts, or 4 slots on 2 or 4 wavefronts).  Settled
+    tracked robots with grid-cell contacts within 0.1 m: worlds of 129-256, 257-512 and
+    513-1024 rows, so every large-world instance of the layout runs.  After two Newton
+    iterations from identical inputs the deviations from the FP32 oracle are compared as
+    distributions over the worlds, as for the grid-cell contacts above: median within 3x,
+    maximum within 4x the largest FP32-vs-FP64 spread."""
+    import dataclasses
+    from ostrich_amd.config import NewtonRaphsonConfig
+    model, st, control, _ = _settled_tracked(oracle_mod, 24, density="cells")
+    col = collide.PrimitiveCollider(model, gap=0.1, max_contacts_per_world=256, hf_contacts="cells")
+    base = config(K=256)
+    cfg = type(base)(nr=NewtonRaphsonConfig(max_iters=2, backtrack_min_iter=1, atol=1e-5),
+                     linear=base.linear, contacts=base.contacts,
+                     execution=dataclasses.replace(base.execution, large_world_slots=slots))
+    c = col.collide(torch.as_tensor(st["body_q"]))
+    c = {k: (v.numpy() if hasattr(v, "numpy") else v) for k, v in c.items()}
+    o = oracle_mod.step(model, cfg, st, control, c, 0.05, precision=32)
+    o64 = oracle_mod.step(model, cfg, st, control, c, 0.05, precision=64)
+    rows = 24 + 3 * o["contact_count"]
+    assert (rows > 512).any() and ((rows > 256) & (rows <= 512)).any() and ((rows > 128) & (rows <= 256)).any()
+    eng = _engine(model, cfg, gpu_device)
+    assert eng.fused_layout() == slots
+    eng.set_class_timing(True)
+    g = _gpu_step(eng, st, control, c, 0.05, gpu_device)
+    shapes, times = eng.class_shapes(), eng.class_times()
+    used = {(sl, ln) for sl, ln, n in zip(shapes["slots"], shapes["lanes"], times["worlds"]) if n > 0 and ln > 64}
+    print(f"[layout {slots}] rows {rows.min()}..{rows.max()}, multi-wave instances (slots, lanes) {sorted(used)}",
+          flush=True)
+    assert used == ({(2, 128), (2, 256), (2, 512)} if slots == 2 else {(2, 128), (4, 128), (4, 256)})
+    W = model.num_worlds
+    nc, nj, nctrl = oracle_mod.dims(model, cfg)
+    for d in (g, o, o64):
+        d["lam_jc"] = d["lam"][:, :nj + nctrl]
+        d["lam_n"] = d["lam"][:, nj + nctrl: nj + nctrl + 256].sum(1)
+    lscale = np.median(np.maximum(np.abs(o["lam"]).max(1), 1.0))
+    for key, scale in (("body_q", 1.0), ("body_qd", G * 0.05), ("lam_jc", lscale), ("lam_n", lscale)):
+        eg = np.abs(g[key] - o[key]).reshape(W, -1).max(1)
+        ep = np.abs(o[key] - o64[key]).reshape(W, -1).max(1)
+        print(f"[layout {slots}] {key}: gpu-vs-f32 median {np.median(eg):.2e} max {eg.max():.2e}, "
+              f"f32-vs-f64 median {np.median(ep):.2e} max {ep.max():.2e}", flush=True)
+        assert np.median(eg) <= 3.0 * np.median(ep) + 1e-4 * scale, (key, np.median(eg), np.median(ep))
+        assert eg.max() <= 4.0 * ep.max() + 1e-4 * scale, (key, eg.max(), ep.max())
+
+
 @pytest.mark.parametrize("density", ["point", "cells"])
 def test_tracked_flipper_rollout_parity(oracle_mod, gpu_device, density):
     """Full Newton loops (cap 20) on the heightfield, three closed-loop steps.  Every world

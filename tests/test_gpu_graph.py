@@ -59,9 +59,9 @@ def test_autotune_changes_only_the_launch_shape(gpu_device):
     model, s0, ctl, col, mk = _setup(256, gpu_device)
     tuned, plain = mk(), mk()
     con = Contacts.from_dict(col.collide(s0.body_q), gpu_device)
-    choice, times = tuned.autotune(s0, ctl, con, 0.05, pairing=False, repeats=2)
+    choice, times = tuned.autotune(s0, ctl, con, 0.05, pairing=False, layouts=(2,), repeats=2)
     print(f"[autotune] {choice} from {times}", flush=True)
-    assert choice in times and tuned.fused_launch() == choice
+    assert choice in times and tuned.fused_launch() + (tuned.fused_layout(),) == choice
     assert all(np.isfinite(v) and v > 0 for v in times.values())
     out_t, out_p = s0.clone(), s0.clone()
     tuned.step(s0, out_t, ctl, con, 0.05)
@@ -69,6 +69,30 @@ def test_autotune_changes_only_the_launch_shape(gpu_device):
     torch.cuda.synchronize()
     assert torch.equal(out_t.body_q, out_p.body_q) and torch.equal(out_t.body_qd, out_p.body_qd)
     assert tuned.last_kernel_ms() > 0.0
+
+
+@pytest.mark.parametrize("waves", [8, 16])
+def test_wide_workgroup_requests_stay_within_the_kernel_bound(gpu_device, waves):
+    """Worlds small enough that 8 or 16 wavefronts' worth would fit the 64 KB of LDS per
+    workgroup (3-cube box stacks): the launches still keep to the kernels' 256-thread
+    workgroup bound, and the step equals the default 4-wavefront setting bit for bit."""
+    from ostrich_amd import OstrichEngine
+    model, state, control = scenes.box_stack(16, 3, seed=0)
+    cfg = config(K=16)
+    col = collide.PrimitiveCollider(model, device=gpu_device, gap=0.02, max_contacts_per_world=16)
+    s0, ctl = State.from_numpy(state, gpu_device), Control.from_numpy(control, gpu_device)
+    con = Contacts.from_dict(col.collide(s0.body_q), gpu_device)
+    outs = []
+    for w in (4, waves):
+        eng = OstrichEngine(model, config=cfg, device=gpu_device)
+        eng.set_fused_launch(w, True)
+        out = s0.clone()
+        eng.step(s0, out, ctl, con, 0.05)
+        torch.cuda.synchronize()
+        outs.append((out, eng.constraint_impulses()))
+    print(f"[launch] {waves} waves requested on box stacks", flush=True)
+    assert torch.equal(outs[0][0].body_q, outs[1][0].body_q) and torch.equal(outs[0][0].body_qd, outs[1][0].body_qd)
+    assert torch.equal(outs[0][1], outs[1][1])
 
 
 def test_captured_collide_step_advance_runs_consecutive_steps(gpu_device):
