@@ -1,6 +1,7 @@
 // fb_fused.hip -- host side of the fused persistent solver: per-step bucketing of worlds
 // into row classes, LDS sizing, and the class launches (kernel in fb_fused_kernel.h).
 
+#include <mutex>
 #include <algorithm>
 
 #include "fb_adjoint_kernel.h"
@@ -331,6 +332,40 @@ int fused_set_layout(ofb_solver* s, int large_world_slots) {
   return fused_configure(s);
 }
 
+// The side streams are shared by every solver of the process on a device: created with
+// the first solver there and kept until the process ends.  A process spreads its streams
+// over a few hardware queues (GPU_MAX_HW_QUEUES, 4 by default); with streams of their
+// own, the solvers created later in a process (a scratch solver of the autotuner, one
+// solver per workload) found their side streams on the queues of other streams, and the
+// classes of a step then ran one after another.  Solvers that step at the same time from
+// different host threads queue their classes behind one another on the shared streams;
+// each step orders its own launches with its own events, so results are unaffected.
+constexpr int kMaxDevices = 64;
+static std::mutex g_side_mu;
+static hipStream_t g_side[kMaxDevices][kSideStreams];
+static bool g_side_made[kMaxDevices];
+
+static int shared_side_streams(hipStream_t (&out)[8]) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+    set_error("no current HIP device for the side streams");
+    return -4;
+  }
+  std::lock_guard<std::mutex> lock(g_side_mu);
+  if (!g_side_made[dev]) {
+    for (int i = 0; i < kSideStreams; ++i) {
+      if (hipStreamCreateWithFlags(&g_side[dev][i], hipStreamNonBlocking) != hipSuccess) {
+        for (int j = 0; j < i; ++j) (void)hipStreamDestroy(g_side[dev][j]);
+        set_error("hipStreamCreate failed");
+        return -4;
+      }
+    }
+    g_side_made[dev] = true;
+  }
+  for (int i = 0; i < kSideStreams; ++i) out[i] = g_side[dev][i];
+  return 0;
+}
+
 int fused_classes_alloc(ofb_solver* s) {
   int rc = fused_configure(s);
   if (rc) return rc;
@@ -339,11 +374,8 @@ int fused_classes_alloc(ofb_solver* s) {
     return -4;
   }
   s->side_ready = 1;
+  if ((rc = shared_side_streams(s->side_stream)) != 0) return rc;
   for (int i = 0; i < kSideStreams; ++i) {
-    if (hipStreamCreateWithFlags(&s->side_stream[i], hipStreamNonBlocking) != hipSuccess) {
-      set_error("hipStreamCreate failed");
-      return -4;
-    }
     s->side_ready = 2 + 2 * i;
     if (hipEventCreateWithFlags(&s->ev_join[i], hipEventDisableTiming) != hipSuccess) {
       set_error("hipEventCreate failed");
@@ -367,11 +399,10 @@ int fused_classes_alloc(ofb_solver* s) {
 }
 
 void fused_classes_free(ofb_solver* s) {
-  // side_ready counts what fused_classes_alloc created: the fork event, then a stream and
-  // its join event per side stream
+  // side_ready counts what fused_classes_alloc set up: the fork event, then a (shared,
+  // kept) stream and its join event per side stream
   if (s->side_ready >= 1) (void)hipEventDestroy(s->ev_fork);
   for (int i = 0; i < kSideStreams; ++i) {
-    if (s->side_ready >= 2 + 2 * i) (void)hipStreamDestroy(s->side_stream[i]);
     if (s->side_ready >= 3 + 2 * i) (void)hipEventDestroy(s->ev_join[i]);
   }
   s->side_ready = 0;
