@@ -129,14 +129,15 @@ def summarise(per_step, W, steps, step_ms, atol, nr_cap):
 
 
 def run_path(variant, model, state, control, steps, warmup, device, collider, K=K_CONTACTS, tune=False,
-             class_timing=False):
+             class_timing=False, large_world_slots=None):
     """One solver variant "path:preconditioner[:seed[:option...]]" over warmup + steps
     closed-loop steps.  Options: "unpaired" (worlds of at most 32 rows one per wavefront),
     "graph" (each step is a replay of a HIP graph recorded over static tensors; the
     collider's contacts are copied into them before each replay, untimed).  tune: run the
     launch-shape autotuner on the first step's inputs before timing.  Step times are HIP
     events on the stream around each step (or replay); with class_timing (eager fused
-    steps) every row-class launch is also timed by its own events."""
+    steps) every row-class launch is also timed by its own events.  large_world_slots
+    (without tune): the layout of the worlds of more than 256 rows, else the default."""
     from ostrich_amd import OstrichEngine
     parts = variant.split(":")
     path, precond = parts[0], parts[1]
@@ -146,6 +147,8 @@ def run_path(variant, model, state, control, steps, warmup, device, collider, K=
     graph = "graph" in opts
     cfg = solver_config(path, precond=precond, seed=seed, paired=paired, K=K)
     eng = OstrichEngine(model, config=cfg, device=device)
+    if large_world_slots is not None and path == "fused":
+        eng.set_fused_layout(large_world_slots)
     if seed == "learned":
         from ostrich_amd.learned import WarmStartNet
         if not os.path.exists(NET_PATH):
@@ -678,6 +681,9 @@ def main():
     ap.add_argument("--scene-only", choices=("tracked_flipper", "box_stack"), default=None,
                     help="run only that workload's leg (previous-step warm start, class timing) and print its JSON "
                          "(for profiling runs)")
+    ap.add_argument("--large-world-slots", type=int, choices=(2, 4), default=None,
+                    help="with --no-tune and --scene-only: row slots per lane of the worlds of more than 256 rows "
+                         "(the layout the autotuner picks for that workload, for profiling runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 256 MiB device copy (reference kernel for HBM byte counters)")
     args = ap.parse_args()
@@ -696,7 +702,7 @@ def main():
             col = collide.HipCollider(model, device=device, gap=0.02, max_contacts_per_world=16)
             K = 16
         res = run_path("fused:jacobi:prev", model, state, control, args.steps, args.warmup, device, col, K=K,
-                       class_timing=True, tune=not args.no_tune)
+                       class_timing=True, tune=not args.no_tune, large_world_slots=args.large_world_slots)
         print(json.dumps(dict(workload=args.scene_only, worlds=model.num_worlds, **res)))
         return
     W = args.worlds
@@ -729,8 +735,10 @@ def main():
         if os.path.exists(tpath):
             with open(tpath) as f:
                 t = json.load(f)
-            if dom["kernel"].replace("*", "false").startswith(t.get("kernel", "?")):
-                roof["traffic"] = t["fetch_bytes"] + t["write_bytes"]
+            name = dom["kernel"].replace("*", "false").split(" (")[0]
+            k = t.get("kernels", {}).get(name)
+            if k is not None:
+                roof["traffic"] = k["fetch_bytes"] + k["write_bytes"]
                 roof["traffic_source"] = ("not measured in this run: HBM bytes per launch of that kernel from "
                                           "rocprofv3 FETCH_SIZE + WRITE_SIZE scaled by a 256 MiB copy, "
                                           f"{os.path.relpath(tpath, ROOT)}")
