@@ -95,6 +95,41 @@ def test_wide_workgroup_requests_stay_within_the_kernel_bound(gpu_device, waves)
     assert torch.equal(outs[0][1], outs[1][1])
 
 
+def test_captured_step_in_the_wide_layout_matches_eager(gpu_device):
+    """The wide layout of large worlds (four row slots per lane on 2 or 4 wavefronts)
+    recorded into a graph: tracked robots with grid-cell contacts within 0.1 m (worlds of
+    up to ~700 rows), replays equal eager steps bit for bit."""
+    import dataclasses
+    from ostrich_amd import OstrichEngine
+    model, state, control = scenes.tracked_flipper(32, seed=2)
+    base = config(K=256, warm_start=WarmStartConfig(enabled=True, seed_iterate=True))
+    cfg = dataclasses.replace(base, execution=dataclasses.replace(base.execution, large_world_slots=4))
+    col = collide.HipCollider(model, device=gpu_device, gap=0.1, max_contacts_per_world=256, hf_contacts="cells")
+    eager, graphed = OstrichEngine(model, config=cfg, device=gpu_device), OstrichEngine(model, config=cfg,
+                                                                                           device=gpu_device)
+    assert eager.fused_layout() == 4 and graphed.fused_layout() == 4
+    s0, ctl = State.from_numpy(state, gpu_device), Control.from_numpy(control, gpu_device)
+    a_in, a_out = s0.clone(), s0.clone()
+    b_in, b_out = s0.clone(), s0.clone()
+    b_con = col.collide(b_in.body_q).clone()
+    g = graphed.capture(b_in, b_out, ctl, b_con, 0.05)
+    for k in range(12):
+        con = col.collide(a_in.body_q)
+        eager.step(a_in, a_out, ctl, con, 0.05)
+        b_con.copy_(con)
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(a_out.body_q, b_out.body_q) and torch.equal(a_out.body_qd, b_out.body_qd), k
+        assert torch.equal(eager.constraint_impulses(), graphed.constraint_impulses()), k
+        a_in.copy_(a_out)
+        b_in.copy_(b_out)
+    cnt = graphed.packed_contacts()["count"]
+    rows = (graphed.num_constraints - 3 * 256) + 3 * cnt
+    print(f"[graph wide] 12 replays, rows {int(rows.min())}..{int(rows.max())}, "
+          f"above 256: {int((rows > 256).sum())}", flush=True)
+    assert int((rows > 256).sum()) > 0
+
+
 def test_captured_collide_step_advance_runs_consecutive_steps(gpu_device):
     """collide + step + state advance recorded in one graph: N replays equal N eager
     simulation steps (same kernels) bit for bit, with no host work between replays."""
