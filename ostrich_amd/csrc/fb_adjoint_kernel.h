@@ -174,7 +174,7 @@ __device__ __forceinline__ RowOut frozen_friction(const PT& P, int w, int c, Row
 
 template <int R, bool BJ>
 __global__ void __launch_bounds__(256) adjoint_step_kernel(AdjParams P) {
-  extern __shared__ float lds[];
+  extern __shared__ __align__(16) float lds[];
   const int lane = threadIdx.x & (WAVE - 1);
   const int wave = threadIdx.x / WAVE;
   const int split = P.split;  // > 1 only with one world per workgroup

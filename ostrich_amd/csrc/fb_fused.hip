@@ -279,7 +279,7 @@ static int allow_lds_fwd(int code, const ofb_solver* s) {
 // the same for the adjoint instance with R row slots per lane
 static int allow_lds_adj(int R, const ofb_solver* s) {
   const bool bj = s->cfg.preconditioner == OFB_PRECOND_PER_BODY_PAIR;
-  const size_t bytes = (size_t)(adj_lds_words(s, R) + rs_words(WAVE, s->m.B)) * sizeof(float);
+  const size_t bytes = (size_t)(up4(adj_lds_words(s, R)) + rs_words(WAVE, s->m.B)) * sizeof(float);
   if (bytes > 65536) {
     AdjParams A{};
     if (adj_class_op(R, 1, A, 0, 0, bytes, nullptr, bj) != 0) {
@@ -609,8 +609,10 @@ int fused_adjoint(ofb_solver* s, const ofb_state_in* in, const ofb_state_out* ou
     P.rows_cap = WAVE * R;
     // [forward's bodies, lambda_prev, adjoint scratch, block-Jacobi | lane-private body
     // sums | transfer area | row-local difference table]
-    P.rs_off = rs_enabled(s->m.B) ? adj_lds_words(s, R) : -1;
-    P.lds_per_wave = adj_lds_words(s, R) + rs_words(WAVE, s->m.B);
+    // (each wave's region and its body sums start on 16-byte boundaries: the sums are
+    // read with 128-bit loads)
+    P.rs_off = rs_enabled(s->m.B) ? up4(adj_lds_words(s, R)) : -1;
+    P.lds_per_wave = up4(adj_lds_words(s, R)) + rs_words(WAVE, s->m.B);
     P.xfer_off = P.lds_per_wave;
     P.split = 1;
     P.fd_off = -1;
@@ -620,8 +622,9 @@ int fused_adjoint(ofb_solver* s, const ofb_state_in* in, const ofb_state_out* ou
       // 30 words per row slot after the transfer area.
       const int base = P.lds_per_wave;
       const int fd_words = adj_fd_words(P.rows_cap, s->m.B);
-      const bool row_fd = R == 1 && (size_t)(base + 2 * P.rows_cap + fd_words) * sizeof(float) <= 65536;
-      const int words = base + 2 * P.rows_cap + (row_fd ? fd_words : 0);
+      const int words_fd = up4(base + 2 * P.rows_cap + fd_words);
+      const bool row_fd = R == 1 && (size_t)words_fd * sizeof(float) <= 65536;
+      const int words = row_fd ? words_fd : up4(base + 2 * P.rows_cap);
       // Fewer worlds than resident wave slots (the kernel's registers allow two waves per
       // SIMD): give each world `split` waves for the pose central differences (one world
       // per workgroup, each wave its own LDS copy).  1024 worlds on 1024 SIMDs: split 2.

@@ -809,9 +809,13 @@ __device__ __forceinline__ void gather_contrib(float* sm, const float* __restric
 
 // ---------------------------------------------------------------------------------
 // Row -> body sums through lane-private copies (worlds of at most kRsMaxBodies bodies).
-// Each lane owns a copy of the 6 B body accumulators in LDS (odd stride, so the copies of
-// consecutive lanes start on consecutive banks) and adds its rows' contributions to it
-// with plain reads and writes of its own words.  LDS float atomics (ds_add_f32) are not
+// Each lane owns a copy of the 6 B body accumulators in LDS and adds its rows'
+// contributions to it with plain reads and writes of its own words.  The copies are
+// stored value-major: value j of lane t at area[j * (G + 4) + t] (G lanes per world), so
+// a lane's six words of one body are six loads at immediate offsets from one address,
+// the lanes of a wave touch consecutive words, and the lane that sums value j reads
+// consecutive copies four at a time (128-bit loads, 16-byte aligned rows; the 4-float
+// pad spreads the rows of different values over the banks).  LDS float atomics (ds_add_f32) are not
 // used: on gfx950 one of them costs about as much as twenty plain read-modify-writes
 // (measured: 24 private ds_add_f32 per lane 4.6 k cycles, the same adds as reads and
 // writes 0.6 k), whether or not lanes share addresses.  The copies are then summed per
@@ -821,8 +825,9 @@ __device__ __forceinline__ void gather_contrib(float* sm, const float* __restric
 // ---------------------------------------------------------------------------------
 constexpr int kRsMaxBodies = 10;
 __host__ __device__ inline bool rs_enabled(int B) { return B <= kRsMaxBodies; }
-__host__ __device__ inline int rs_stride(int B) { return 6 * B + 1; }
-__host__ __device__ inline int rs_words(int lanes, int B) { return rs_enabled(B) ? lanes * rs_stride(B) : 0; }
+// floats between the rows of two values of the copies of a world of `lanes` lanes
+__host__ __device__ constexpr int rs_row(int lanes) { return lanes + 4; }
+__host__ __device__ inline int rs_words(int lanes, int B) { return rs_enabled(B) ? 6 * B * rs_row(lanes) : 0; }
 // floats per wave of the per-wave body sums of a world of several waves
 __host__ __device__ inline int part_stride(int B) { return 6 * B > 64 ? (6 * B + 3) & ~3 : 64; }
 
@@ -836,11 +841,17 @@ struct BodySum {
 
   // begin a pass that sums into body field `field`: with `zero` the pass's sums replace
   // the field, else they are added to it
+  static constexpr int VS = rs_row(GW);  // floats between the rows of two values
+
   __device__ __forceinline__ void begin(int field, bool zero) const {
     if (rs) {
-      float* p = area + lane * rs_stride(B);
-      for (int b = 0; b < B; ++b, p += 6) {
-        p[0] = 0.f; p[1] = 0.f; p[2] = 0.f; p[3] = 0.f; p[4] = 0.f; p[5] = 0.f;
+      float* p = area + lane;
+#pragma unroll
+      for (int b = 0; b < kRsMaxBodies; ++b) {
+        if (b < B) {
+#pragma unroll
+          for (int c = 0; c < 6; ++c) p[(6 * b + c) * VS] = 0.f;
+        }
       }
     } else if constexpr (!DENSE && GW > WAVE) {
       // worlds of several waves and more than kRsMaxBodies bodies: each wave adds its
@@ -860,9 +871,8 @@ struct BodySum {
   __device__ __forceinline__ void add(int field, int i, bool on, int b0, int b1, const sv& c0, const sv& c1) const {
     if (rs) {
       if (!on) return;
-      float* p = area + lane * rs_stride(B);
-      if (b0 >= 0) add_sv(p + 6 * b0, c0);
-      if (b1 >= 0) add_sv(p + 6 * b1, c1);
+      if (b0 >= 0) add_sv(area + 6 * b0 * VS + lane, c0);
+      if (b1 >= 0) add_sv(area + 6 * b1 * VS + lane, c1);
     } else if (DENSE) {
       stage_contrib(area, i, on, b0, b1, c0, c1);
     } else if (on) {
@@ -877,11 +887,11 @@ struct BodySum {
       if (b1 >= 0) atomic_add_sv(base + b1 * st + off, c1);
     }
   }
-  // p[0..5] += x on this lane's own words
+  // p[c VS] += x_c (c = 0..5) on this lane's own words of one body
   __device__ __forceinline__ static void add_sv(float* p, const sv& x) {
-    const float a0 = p[0], a1 = p[1], a2 = p[2], a3 = p[3], a4 = p[4], a5 = p[5];
-    p[0] = a0 + x.a.x; p[1] = a1 + x.a.y; p[2] = a2 + x.a.z;
-    p[3] = a3 + x.b.x; p[4] = a4 + x.b.y; p[5] = a5 + x.b.z;
+    const float a0 = p[0], a1 = p[VS], a2 = p[2 * VS], a3 = p[3 * VS], a4 = p[4 * VS], a5 = p[5 * VS];
+    p[0] = a0 + x.a.x; p[VS] = a1 + x.a.y; p[2 * VS] = a2 + x.a.z;
+    p[3 * VS] = a3 + x.b.x; p[4 * VS] = a4 + x.b.y; p[5 * VS] = a5 + x.b.z;
   }
   // completes the pass (rows 0..n_r-1 staged for the record fallback); the caller
   // synchronises before reading the field
@@ -903,22 +913,22 @@ struct BodySum {
     }
   }
 
-  // value j of lane t's copy is area[t * stride + j]; nt is 32 or 64: the loads of each
-  // group of 16 copies are issued together, then summed in four interleaved chains
+  // the sum of value j over the copies of lanes t0 .. t0 + nt - 1 (nt is 32 or 64): four
+  // copies per 128-bit load, summed in four interleaved chains (copy t0 + u into chain
+  // u % 4)
   __device__ __forceinline__ float copies_sum(int j, int t0, int nt) const {
-    const int st = rs_stride(B);
-    const float* p = area + t0 * st + j;
+    const float4* q = reinterpret_cast<const float4*>(area + j * VS + t0);
     float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
-    for (int t = 0; t < nt; t += 16, p += 16 * st) {
-      float x[16];
+    for (int t = 0; t < nt; t += 16, q += 4) {
+      float4 x[4];
 #pragma unroll
-      for (int u = 0; u < 16; ++u) x[u] = p[u * st];
+      for (int u = 0; u < 4; ++u) x[u] = q[u];
 #pragma unroll
-      for (int u = 0; u < 16; u += 4) {
-        a0 += x[u];
-        a1 += x[u + 1];
-        a2 += x[u + 2];
-        a3 += x[u + 3];
+      for (int u = 0; u < 4; ++u) {
+        a0 += x[u].x;
+        a1 += x[u].y;
+        a2 += x[u].z;
+        a3 += x[u].w;
       }
     }
     return (a0 + a1) + (a2 + a3);
