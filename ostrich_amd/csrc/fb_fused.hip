@@ -42,7 +42,7 @@ struct ClassTable {
   int n;                   // classes
   int rows[kMaxClasses];   // row capacity of each class
   int code[kMaxClasses];   // kernel instance of each class
-  int side[kMaxClasses];   // side stream of each class (-1: the step's stream; class 0 runs there, last)
+  int side[kMaxClasses];   // side stream of each class (-1: the step's stream); class 0 runs last on its stream
   int merge_from;          // classes from this index on are launched as one (n: none)
 };
 // Streams of the Jacobi classes (measured on 4096 uneven skid-steer worlds and 1024
@@ -530,31 +530,32 @@ int fused_step(ofb_solver* s, const ofb_state_in* in, const ofb_state_out* out, 
       s->cls_timed |= 1 << c;
     }
   }
+  // the paired class, which holds most worlds of a typical batch, last on its stream,
+  // bracketed by the kernel events (not recorded while the stream is being captured)
+  hipStream_t st0 = t.side[0] < 0 ? stream : s->side_stream[t.side[0]];
+  const bool timed = cap == hipStreamCaptureStatusNone && class_launched(s, 0);
+  if (timed && hipEventRecord(s->ev_kernel0, st0) != hipSuccess) {
+    set_error("fused step: event record failed");
+    return -4;
+  }
+  if (class_launched(s, 0)) {
+    if (const int rc = launch_fused(t.code[0], P, s, 0, st0)) {
+      if (rc == -1) set_error("no fused kernel instance for class code %d", t.code[0]);
+      return -4;
+    }
+  }
+  if (timed && hipEventRecord(s->ev_kernel1, st0) != hipSuccess) {
+    set_error("fused step: event record failed");
+    return -4;
+  }
+  s->kernel_timed = timed ? 1 : 0;
+  if (timed && cls_timed) s->cls_timed |= 1;  // the paired class's events are the kernel events
   for (int i = 0; i < kSideStreams; ++i) {
     if (hipEventRecord(s->ev_join[i], s->side_stream[i]) != hipSuccess) {
       set_error("fused step: stream join failed");
       return -4;
     }
   }
-  // the paired class, which holds most worlds of a typical batch, on the step's stream,
-  // bracketed by the kernel events (not recorded while the stream is being captured)
-  const bool timed = cap == hipStreamCaptureStatusNone && class_launched(s, 0);
-  if (timed && hipEventRecord(s->ev_kernel0, stream) != hipSuccess) {
-    set_error("fused step: event record failed");
-    return -4;
-  }
-  if (class_launched(s, 0)) {
-    if (const int rc = launch_fused(t.code[0], P, s, 0, stream)) {
-      if (rc == -1) set_error("no fused kernel instance for class code %d", t.code[0]);
-      return -4;
-    }
-  }
-  if (timed && hipEventRecord(s->ev_kernel1, stream) != hipSuccess) {
-    set_error("fused step: event record failed");
-    return -4;
-  }
-  s->kernel_timed = timed ? 1 : 0;
-  if (timed && cls_timed) s->cls_timed |= 1;  // the paired class's events are the kernel events
   for (int i = 0; i < kSideStreams; ++i) {
     if (hipStreamWaitEvent(stream, s->ev_join[i], 0) != hipSuccess) {
       set_error("fused step: stream join failed");
