@@ -174,12 +174,15 @@ typedef struct ofb_config {
   int32_t adjoint_soft_blending;   /* adjoint.soft_blending: sigmoid between the two
                                       compliances instead of the 0.9 threshold */
   float adjoint_soft_temperature;  /* adjoint.soft_blending_temperature */
-  int32_t large_world_slots;       /* fused path, Jacobi: row slots per lane of the worlds of
-                                      more than 256 rows; 2 (or 0): 2, 4 or 8 wavefronts per
-                                      world, the shortest solve of one world; 4: half the
-                                      wavefronts, about twice as many large worlds resident
-                                      at once.  No reference counterpart (a launch layout);
-                                      like pair_small_worlds a rounding-level difference */
+  int32_t large_world_slots;       /* fused path, Jacobi: layout of the large worlds; 2 (or
+                                      0): worlds of more than 128 rows on 2, 4 or 8
+                                      wavefronts with two row slots per lane, the shortest
+                                      solve of one world; 4 (the wide layout): 65-512 rows
+                                      on two wavefronts with two to four slots per lane,
+                                      513-1024 on four with four: about twice as many large
+                                      worlds resident at once.  No reference counterpart (a
+                                      launch layout); like pair_small_worlds a
+                                      rounding-level difference */
 } ofb_config;
 
 typedef struct ofb_state_in {

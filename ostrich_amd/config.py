@@ -273,10 +273,12 @@ class ExecutionConfig:
     pair_small_worlds: fused path - worlds of at most 32 constraint rows run two per
                  wavefront (32 lanes each) and solve PCR on the assembled Schur matrix;
                  False keeps one world per wavefront and the matrix-free operator.
-    large_world_slots: fused path, Jacobi - row slots per lane of the worlds of more than
-                 256 constraint rows: 2 spreads such a world over 2, 4 or 8 wavefronts
-                 (the shortest solve of one world), 4 over half as many (more of them
-                 resident at once: the faster layout for batches of many large worlds).
+    large_world_slots: fused path, Jacobi - layout of the large worlds: 2 spreads a world
+                 of more than 128 constraint rows over 2, 4 or 8 wavefronts with two row
+                 slots per lane (the shortest solve of one world); 4, the wide layout,
+                 runs 65-512 rows on two wavefronts with two to four slots per lane and
+                 513-1024 on four (more of them resident at once: the faster layout for
+                 batches of many large worlds).
     """
 
     path: str = "fused"

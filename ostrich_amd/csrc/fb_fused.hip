@@ -63,21 +63,34 @@ struct ClassTable {
 #define OFB_JACOBI_CODES 0, 1, 2, kMW + 2, kMW + 4, kMW + 8
 #endif
 static const ClassTable kTabJacobi = {6, {32, 64, 128, 256, 512, 1024}, {OFB_JACOBI_CODES}, {OFB_JACOBI_SIDES}, 6};
-// The wide layout of large worlds (large_world_slots = 4): worlds of 257-1024 rows on half
-// the wavefronts, four row slots per lane.  A world's solve takes longer, but a batch
-// of many large worlds keeps about twice as many of them resident, so it is the faster
-// one when such worlds outnumber what the chip holds at once (contact-rich tracked robots)
-// and the slower one when a few of them set the step (the uneven skid-steer batch).  The
-// eight-wave class of the default layout becomes a four-wave one; it starts first on the
-// step's stream (its worlds need the most LDS), the 257-512-row class runs beside it.
+// The wide layout of large worlds (large_world_slots = 4): worlds of 129-1024 rows on
+// two wavefronts with two, three or four row slots per lane (129-256, 257-384, 385-512
+// rows) and 513-1024 on four with four.  A world's solve takes longer than on the
+// default layout's 4 or 8 wavefronts, but a batch of many large worlds keeps about twice
+// as many of them resident, so it is the faster layout when such worlds outnumber what
+// the chip holds at once (contact-rich tracked robots) and the slower one when a few of
+// them set the step (the uneven skid-steer batch).  Worlds of 65-128 rows join the
+// 129-256-row class (a batch of many large worlds has few of them), which keeps the
+// four big classes on four streams: the four-wavefront class first on the step's stream
+// (its worlds need the most LDS), the dense class after the 129-256-row one.  Measured on
+// 1024 tracked robots: 2.19-2.23 ms per step against 2.33-2.35 ms for six classes with
+// the 65-128-row one and 257-512 rows in one four-slot class.
 #ifndef OFB_JACOBI_WIDE_SIDES
-#define OFB_JACOBI_WIDE_SIDES -1, 0, 0, 1, 2, -1
+#define OFB_JACOBI_WIDE_SIDES -1, 0, 0, 2, 1, -1
 #endif
 #ifndef OFB_JACOBI_WIDE_CODES
-#define OFB_JACOBI_WIDE_CODES 0, 1, 2, multiwave_code(2, 2), multiwave_code(2, 4), multiwave_code(4, 4)
+#define OFB_JACOBI_WIDE_CODES \
+  0, 1, multiwave_code(2, 2), multiwave_code(2, 3), multiwave_code(2, 4), multiwave_code(4, 4)
 #endif
-static const ClassTable kTabJacobiWide = {6, {32, 64, 128, 256, 512, 1024}, {OFB_JACOBI_WIDE_CODES},
-                                          {OFB_JACOBI_WIDE_SIDES}, 6};
+#ifndef OFB_JACOBI_WIDE_ROWS
+#define OFB_JACOBI_WIDE_ROWS 32, 64, 256, 384, 512, 1024
+#endif
+#ifndef OFB_JACOBI_WIDE_N
+#define OFB_JACOBI_WIDE_N 6
+#endif
+static const ClassTable kTabJacobiWide = {OFB_JACOBI_WIDE_N, {OFB_JACOBI_WIDE_ROWS}, {OFB_JACOBI_WIDE_CODES},
+                                          {OFB_JACOBI_WIDE_SIDES}, OFB_JACOBI_WIDE_N};
+
 static const ClassTable kTabBlockJacobi = {10, {32, 64, 128, 192, 256, 320, 384, 448, 768, 1024},
                                            {0, 1, 2, 3, 4, 5, 6, 7, 12, 16}, {-1, 0, 1, 2, 2, 2, 2, 2, 2, 2}, 3};
 static const ClassTable kTabAdjoint = {10, {32, 64, 128, 192, 256, 320, 384, 448, 768, 1024},

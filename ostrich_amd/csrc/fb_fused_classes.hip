@@ -5,7 +5,8 @@
 // R = 3..5; OFB_RSET = 1 the block-Jacobi classes R = 6, 7, 12 and 16, or the Jacobi
 // worlds of 2, 4 and 8 waves (Jacobi solvers run worlds of more than 128 rows on several
 // waves, block-Jacobi ones on one wave with more row slots per lane); OFB_RSET = 2 (Jacobi
-// only) the worlds of 2 and 4 waves with four row slots per lane (the wide layout).
+// only) the worlds of 2 waves with three or four row slots per lane and of 4 waves with
+// four (the wide layout).
 
 #include "fb_fused_kernel.h"
 
@@ -37,6 +38,7 @@ int OFB_CAT(fused_class_op_bj, OFB_BJ, _set, OFB_RSET)(int code, int op, const F
     case multiwave_code(4, 2): return fused_class_op<2, false, 256, false>(op, P, blocks, threads, lds, st);
     case multiwave_code(8, 2): return fused_class_op<2, false, 512, false>(op, P, blocks, threads, lds, st);
 #elif OFB_BJ == 0
+    case multiwave_code(2, 3): return fused_class_op<3, false, 128, false>(op, P, blocks, threads, lds, st);
     case multiwave_code(2, 4): return fused_class_op<4, false, 128, false>(op, P, blocks, threads, lds, st);
     case multiwave_code(4, 4): return fused_class_op<4, false, 256, false>(op, P, blocks, threads, lds, st);
 #endif

@@ -38,14 +38,14 @@ using namespace rows;
 #ifdef OFB_PHASE_TIMING
 // Profiling build only: per-phase cycle counts of the Newton loop, summed over worlds,
 // per instance (row 0: paired, 1: dense, R: matrix-free with R slots per lane, 13-15:
-// worlds of 2, 4, 8 waves with two slots per lane, 9-11: with four); columns 8 and 9
-// count Newton and PCR iterations.
+// worlds of 2, 4, 8 waves with two slots per lane, 9-11: with four, 8: with three);
+// columns 8 and 9 count Newton and PCR iterations.
 __device__ unsigned long long g_phase_cycles[17][10];
 #define OFB_PHASE_INIT unsigned long long ph_[8] = {0, 0, 0, 0, 0, 0, 0, 0}; unsigned long long ph_t = clock64();
 #define OFB_PHASE(k) { unsigned long long t_ = clock64(); ph_[k] += t_ - ph_t; ph_t = t_; }
 #define OFB_PHASE_FLUSH                                                                       \
   if (lane == 0) {                                                                            \
-    const int ci_ = DENSE ? (GW == 32 ? 0 : 1) : GW > WAVE ? (GW == 128 ? 13 : GW == 256 ? 14 : 15) - (R - 2) * 2 : R; \
+    const int ci_ = DENSE ? (GW == 32 ? 0 : 1) : GW > WAVE ? (R == 3 ? 8 : (GW == 128 ? 13 : GW == 256 ? 14 : 15) - (R - 2) * 2) : R; \
     for (int k_ = 0; k_ < 8; ++k_) atomicAdd(&g_phase_cycles[ci_][k_], ph_[k_]);              \
     atomicAdd(&g_phase_cycles[ci_][8], (unsigned long long)n_it);                             \
     atomicAdd(&g_phase_cycles[ci_][9], (unsigned long long)*pcr_count);                       \
@@ -1143,9 +1143,9 @@ __attribute__((amdgpu_waves_per_eu(GW > WAVE ? OFB_MW_WAVES_PER_EU
 fused_step_kernel(FusedParams P) {
   extern __shared__ __align__(16) float lds[];
   static_assert(GW == 64 || (GW == 32 && R == 1) ||
-                    ((GW == 128 || GW == 256 || GW == 512) && (R == 2 || R == 4) && !DENSE && !BJ),
+                    ((GW == 128 || GW == 256 || GW == 512) && R >= 2 && R <= 4 && !DENSE && !BJ),
                 "two worlds per wave hold one row slot per lane; worlds of several waves are matrix-free, Jacobi, "
-                "with two or four row slots per lane");
+                "with two to four row slots per lane");
   const int lane = threadIdx.x & (GW - 1);
   const int grp = threadIdx.x / GW;
   const int idx = blockIdx.x * (blockDim.x / GW) + grp;

@@ -322,11 +322,16 @@ class OstrichEngine:
         more than 256 rows, 2 or 4) runs one warm-up step and `repeats` timed steps of the
         given inputs on a scratch solver built from the same model and config, timed with
         HIP events on the current stream; the fastest is applied to this solver.  By
-        default the wide layout (4) is a candidate on the Jacobi path when the batch holds
-        at least 256 worlds of more than 256 rows (about one per CU); with fewer, they are
-        all resident at once in either layout, and the default one solves each faster.  Every step starts from the same warm-start
-        snapshot (this solver's), so the candidates solve the same systems from the same
-        seeds.  This solver's state (warm-start snapshot, last results) is left as it was.
+        default the layout is not timed but follows the batch: the wide one (4) on the
+        Jacobi path when the batch holds at least 256 worlds of more than 256 rows (about
+        one per CU), else the default (2).  With fewer such worlds they are all resident at
+        once in either layout, and the default one solves each faster; with many, the wide
+        one keeps more of them resident (on 1024 tracked robots 2.2 against 2.9 ms per
+        step), while a step timed on a batch's first inputs, where the robots have not yet
+        settled into their contacts, can rank the two the other way.  Every step starts
+        from the same warm-start snapshot (this solver's), so the candidates solve the same
+        systems from the same seeds.  This solver's state (warm-start snapshot, last
+        results) is left as it was.
         Returns (best (waves_per_block, pair_small_worlds, large_world_slots),
         {candidate: best ms})."""
         if self.config.execution.path != "fused":
@@ -343,8 +348,7 @@ class OstrichEngine:
                 scratch.step(state_in, out, control, contacts, dt)  # packs this batch's contacts
                 K = self.config.contacts.max_per_world
                 rows = (self.num_constraints - 3 * K) + 3 * scratch.packed_contacts()["count"]
-                if int((rows > 256).sum().item()) >= 256:
-                    layouts = (2, 4)
+                layouts = (4,) if int((rows > 256).sum().item()) >= 256 else (2,)
         if candidates is None:
             pairs = (True, False) if pairing else (self.fused_launch()[1],)
             candidates = [(w, p, l) for l in layouts for p in pairs for w in (1, 2, 4)]
