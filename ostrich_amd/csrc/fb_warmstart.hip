@@ -125,7 +125,25 @@ __global__ void __launch_bounds__(64) ws_apply_kernel(WsParams P) {
   float* support = ws_lds;                                   // [B]
   int* unmatched = reinterpret_cast<int*>(support + B);      // [B]
   int* matched = unmatched + B;                              // [K]
+  // the previous step's contacts of this world, staged once for the scans below (every
+  // current contact walks all of them): body pair, midpoint and normal impulse
+  int* pb0 = matched + K;                                    // [K]
+  int* pb1 = pb0 + K;                                        // [K]
+  float* ppos = reinterpret_cast<float*>(pb1 + K);           // [3K]
+  float* pln = ppos + 3 * K;                                 // [K]
   for (int b = lane; b < B; b += 64) { support[b] = 0.f; unmatched[b] = 0; }
+  {
+    const int n_prev0 = min(P.prev_count[w], K);
+    for (int j = lane; j < n_prev0; j += 64) {
+      const size_t kj = (size_t)w * K + j;
+      pb0[j] = P.prev_b0[kj];
+      pb1[j] = P.prev_b1[kj];
+      ppos[3 * j] = P.prev_p[kj * 3];
+      ppos[3 * j + 1] = P.prev_p[kj * 3 + 1];
+      ppos[3 * j + 2] = P.prev_p[kj * 3 + 2];
+      pln[j] = P.prev_ln[kj];
+    }
+  }
   for (int i = lane; i < Nc; i += 64) P.seed[(size_t)w * Nc + i] = 0.f;
   if (P.feat)
     for (int e = lane; e < K * 12; e += 64) P.feat[(size_t)w * K * 12 + e] = 0.f;
@@ -156,8 +174,7 @@ __global__ void __launch_bounds__(64) ws_apply_kernel(WsParams P) {
       float total = 0.f;
       int n_pp = 0, n_cur = 0;
       for (int j = 0; j < n_prev; ++j) {
-        const size_t kj = (size_t)w * K + j;
-        if (P.prev_b0[kj] == b0 && P.prev_b1[kj] == b1) { total += P.prev_ln[kj]; ++n_pp; }
+        if (pb0[j] == b0 && pb1[j] == b1) { total += pln[j]; ++n_pp; }
       }
       for (int j = 0; j < n_c; ++j) {
         const int t0 = P.ci[(size_t)w * 2 * K + j], t1 = P.ci[(size_t)w * 2 * K + K + j];
@@ -188,9 +205,8 @@ __global__ void __launch_bounds__(64) ws_apply_kernel(WsParams P) {
     float best_d = thr * thr;
     int best = -1;
     for (int j = 0; j < n_prev; ++j) {
-      size_t kj = (size_t)w * K + j;
-      if (P.prev_b0[kj] != b0 || P.prev_b1[kj] != b1) continue;
-      v3 pp = mk(P.prev_p[kj * 3], P.prev_p[kj * 3 + 1], P.prev_p[kj * 3 + 2]) + vt * dt;
+      if (pb0[j] != b0 || pb1[j] != b1) continue;
+      v3 pp = mk(ppos[3 * j], ppos[3 * j + 1], ppos[3 * j + 2]) + vt * dt;
       v3 d = pc - pp;
       float dsq = dot(d, d);
       if (dsq < best_d) { best_d = dsq; best = j; }
@@ -337,7 +353,7 @@ WsParams ws_params(ofb_solver* s, const ofb_state_in* in, float dt) {
 
 int warm_start_apply(ofb_solver* s, const ofb_state_in* in, float dt, hipStream_t stream) {
   WsParams P = ws_params(s, in, dt);
-  size_t lds = (size_t)(2 * s->m.B + s->L.K) * 4;
+  size_t lds = (size_t)(2 * s->m.B + 2 * s->L.K + 5 * s->L.K) * 4;
   hipLaunchKernelGGL(ws_apply_kernel, dim3(s->m.W), dim3(64), lds, stream, P);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) { set_error("warm start launch failed: %s", hipGetErrorString(e)); return -4; }
