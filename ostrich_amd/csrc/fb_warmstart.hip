@@ -118,29 +118,29 @@ __device__ void write_features(const WsParams& P, int w, int c, int b0, int b1, 
   P.fscale[(size_t)w * K + c] = sc;
 }
 
+// one int per body pair (bodies -1 .. 65534): equal keys, equal pairs
+__device__ __forceinline__ int pair_key(int b0, int b1) { return ((b0 + 1) << 16) | (b1 + 1); }
+
 __global__ void __launch_bounds__(64) ws_apply_kernel(WsParams P) {
-  extern __shared__ float ws_lds[];
+  extern __shared__ __align__(16) float ws_lds[];
   const int w = blockIdx.x, lane = threadIdx.x;
   const int B = P.m.B, K = P.L.K, Nc = P.L.Nc, SC = P.m.SC;
-  float* support = ws_lds;                                   // [B]
+  // the previous step's contacts of this world, staged once for the scans below (every
+  // current contact walks all of them): midpoint (one 128-bit load), body pair as one
+  // key (one load to reject another pair's contact), normal impulse
+  float4* ppos = reinterpret_cast<float4*>(ws_lds);          // [K]
+  int* pkey = reinterpret_cast<int*>(ppos + K);              // [K]
+  float* pln = reinterpret_cast<float*>(pkey + K);           // [K]
+  float* support = pln + K;                                  // [B]
   int* unmatched = reinterpret_cast<int*>(support + B);      // [B]
   int* matched = unmatched + B;                              // [K]
-  // the previous step's contacts of this world, staged once for the scans below (every
-  // current contact walks all of them): body pair, midpoint and normal impulse
-  int* pb0 = matched + K;                                    // [K]
-  int* pb1 = pb0 + K;                                        // [K]
-  float* ppos = reinterpret_cast<float*>(pb1 + K);           // [3K]
-  float* pln = ppos + 3 * K;                                 // [K]
   for (int b = lane; b < B; b += 64) { support[b] = 0.f; unmatched[b] = 0; }
   {
     const int n_prev0 = min(P.prev_count[w], K);
     for (int j = lane; j < n_prev0; j += 64) {
       const size_t kj = (size_t)w * K + j;
-      pb0[j] = P.prev_b0[kj];
-      pb1[j] = P.prev_b1[kj];
-      ppos[3 * j] = P.prev_p[kj * 3];
-      ppos[3 * j + 1] = P.prev_p[kj * 3 + 1];
-      ppos[3 * j + 2] = P.prev_p[kj * 3 + 2];
+      ppos[j] = make_float4(P.prev_p[kj * 3], P.prev_p[kj * 3 + 1], P.prev_p[kj * 3 + 2], 0.f);
+      pkey[j] = pair_key(P.prev_b0[kj], P.prev_b1[kj]);
       pln[j] = P.prev_ln[kj];
     }
   }
@@ -173,8 +173,9 @@ __global__ void __launch_bounds__(64) ws_apply_kernel(WsParams P) {
       // sees no unmatched contacts
       float total = 0.f;
       int n_pp = 0, n_cur = 0;
+      const int key = pair_key(b0, b1);
       for (int j = 0; j < n_prev; ++j) {
-        if (pb0[j] == b0 && pb1[j] == b1) { total += pln[j]; ++n_pp; }
+        if (pkey[j] == key) { total += pln[j]; ++n_pp; }
       }
       for (int j = 0; j < n_c; ++j) {
         const int t0 = P.ci[(size_t)w * 2 * K + j], t1 = P.ci[(size_t)w * 2 * K + K + j];
@@ -204,9 +205,11 @@ __global__ void __launch_bounds__(64) ws_apply_kernel(WsParams P) {
     const float thr = fmaxf(kWsMinThreshold, kWsAlpha * sqrtf(dot(vt, vt)) * dt);
     float best_d = thr * thr;
     int best = -1;
+    const int key = pair_key(b0, b1);
     for (int j = 0; j < n_prev; ++j) {
-      if (pb0[j] != b0 || pb1[j] != b1) continue;
-      v3 pp = mk(ppos[3 * j], ppos[3 * j + 1], ppos[3 * j + 2]) + vt * dt;
+      if (pkey[j] != key) continue;
+      const float4 q = ppos[j];
+      v3 pp = mk(q.x, q.y, q.z) + vt * dt;
       v3 d = pc - pp;
       float dsq = dot(d, d);
       if (dsq < best_d) { best_d = dsq; best = j; }
@@ -352,8 +355,12 @@ WsParams ws_params(ofb_solver* s, const ofb_state_in* in, float dt) {
 }  // namespace
 
 int warm_start_apply(ofb_solver* s, const ofb_state_in* in, float dt, hipStream_t stream) {
+  if (s->m.B > 65534) {  // the match keys a body pair in one int
+    set_error("the warm start supports at most 65534 bodies per world");
+    return -2;
+  }
   WsParams P = ws_params(s, in, dt);
-  size_t lds = (size_t)(2 * s->m.B + 2 * s->L.K + 5 * s->L.K) * 4;
+  size_t lds = (size_t)(7 * s->L.K + 2 * s->m.B) * 4;
   hipLaunchKernelGGL(ws_apply_kernel, dim3(s->m.W), dim3(64), lds, stream, P);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) { set_error("warm start launch failed: %s", hipGetErrorString(e)); return -4; }
