@@ -95,13 +95,35 @@ __global__ void pack_members_kernel(int n_max, const int32_t* __restrict__ count
                                     const int32_t* __restrict__ s1, const int32_t* __restrict__ shape_world, int W,
                                     int K, int32_t* __restrict__ ccount, int32_t* __restrict__ members,
                                     int32_t* __restrict__ keys) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n_max || i >= count[0]) return;
-  int w = contact_world(s0[i], s1[i], shape_world);
-  keys[i] = w;
-  if (w < 0 || w >= W) return;
-  int slot = atomicAdd(ccount + w, 1);
-  if (slot < K) members[(size_t)w * K + slot] = i;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const bool live = i < n_max && i < count[0];
+  int w = -1;
+  if (live) {
+    w = contact_world(s0[i], s1[i], shape_world);
+    keys[i] = w;
+  }
+  const bool take = live && w >= 0 && w < W;
+  // Contacts usually arrive grouped by world, so the lanes of a wave form runs of one
+  // world: the first lane of each run reserves the run's slots with one atomic add (one
+  // per lane contended on the same counter), and the run's lanes take consecutive slots.
+  // A lane that takes no slot is a run of its own.  pack_world_kernel orders each world's
+  // members by input index afterwards, so the slot order is immaterial.
+  const int key = take ? w : -1 - lane;
+  const int prev = __shfl_up(key, 1);
+  const bool head = lane == 0 || key != prev;
+  const unsigned long long heads = __ballot(head);
+  const unsigned long long upto = lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1);
+  const int first = 63 - __clzll(heads & upto);  // this lane's run starts here
+  const unsigned long long after = lane == 63 ? 0ull : (heads >> (lane + 1)) << (lane + 1);
+  const int end = after ? __ffsll(after) - 1 : 64;  // the next run starts here
+  int base = 0;
+  if (take && lane == first) base = atomicAdd(ccount + w, end - lane);
+  base = __shfl(base, first);
+  if (take) {
+    const int slot = base + (lane - first);
+    if (slot < K) members[(size_t)w * K + slot] = i;
+  }
 }
 
 __device__ __forceinline__ void pack_write(int w, int slot, int i, int G, int S, int K, const float* __restrict__ p0,
